@@ -1,0 +1,7 @@
+"""MI355X-native FLIP step with the interface of Ariel's fluidCore::FlipSim.
+
+The compute path is libariel_amd.so (hand-written HIP for gfx950) behind the C ABI
+declared in include/ariel.h; this package is its Python mirror.
+"""
+from . import _abi, scenes  # noqa: F401
+from .flipsim import ArielError, FlipSim  # noqa: F401

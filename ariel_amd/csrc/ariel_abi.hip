@@ -1,0 +1,975 @@
+// ariel_abi.hip — host side of the C ABI (include/ariel.h): device state of one
+// simulation handle and the orchestration of FlipSim::Init / FlipSim::Step
+// (src/sim/flip.cpp:42-128) as a sequence of HIP kernel launches on the
+// handle's stream. Host work per step is launch-only apart from one small
+// device->host read of the solver scalars per chunk of PCG iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ariel.h"
+#include "ariel_device.h"
+#include "ariel_internal.h"
+
+using namespace ariel;
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+struct Stage {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    bool recorded = false;
+};
+
+}  // namespace
+
+struct ariel_sim {
+    ariel_scene_desc d{};
+    int N = 0;
+    int64_t nc = 0, nf = 0, np = 0;
+    float maxd = 0.f;
+    float max_density = 0.f;
+    int frame = 0;
+    bool initialized = false;
+    int last_iters = 0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // particles (sorted by (cell, id)); b = current buffer
+    float4* pos[2] = {nullptr, nullptr};
+    float4* vel[2] = {nullptr, nullptr};
+    int cur = 0;
+    int32_t* cell_of = nullptr;
+    int32_t* slots = nullptr;
+    int32_t* counts = nullptr;
+    int32_t* offsets = nullptr;
+    int32_t* scan_sums = nullptr;
+    uint8_t* accept = nullptr;
+    int64_t np_cap = 0;
+
+    // grids
+    float* u[3] = {nullptr, nullptr, nullptr};
+    float* du[3] = {nullptr, nullptr, nullptr};
+    float *P = nullptr, *D = nullptr, *L = nullptr;
+    uint8_t* A = nullptr;
+    float *R = nullptr, *Z = nullptr, *S = nullptr, *Q = nullptr, *pc = nullptr, *diag = nullptr;
+    uint8_t* nup = nullptr;
+    double* partials = nullptr;
+    uint32_t* counter = nullptr;
+    SolverScalars* sc = nullptr;
+    SolverScalars* sc_host = nullptr;   // pinned
+
+    Stage stages[ARIEL_NUM_STAGES];
+    hipEvent_t step_t0 = nullptr, step_t1 = nullptr, solve_t0 = nullptr, solve_t1 = nullptr;
+    ariel_step_stats stats{};
+
+    Params params() const {
+        Params p;
+        p.N = N;
+        p.maxd = maxd;
+        p.density = d.density;
+        p.dt = d.dt;
+        p.picflip = d.picflip;
+        p.sharpen_re = d.sharpen_re;
+        p.spring = d.spring;
+        p.splat = d.splat;
+        p.wall = d.wall;
+        p.seed = d.seed;
+        p.n_forces = d.n_forces;
+        for (int f = 0; f < 4; ++f)
+            for (int c = 0; c < 3; ++c) p.forces[f][c] = d.forces[f][c];
+        p.max_density = max_density;
+        p.frame = frame;
+        p.mic_tau = d.mic_tau;
+        p.mic_sigma = d.mic_sigma;
+        p.precond = d.precond;
+        return p;
+    }
+    Reduce reduce() const { return Reduce{partials, counter, sc}; }
+};
+
+namespace {
+
+int fail(ariel_sim* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((h), ARIEL_ERR_HIP,                                               \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+template <class T>
+int dalloc(ariel_sim* h, T** p, int64_t n) {
+    *p = nullptr;
+    if (n <= 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(h, ARIEL_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return ARIEL_OK;
+}
+
+#define TRY(expr)                       \
+    do {                                \
+        int rc_ = (expr);               \
+        if (rc_ != ARIEL_OK) return rc_; \
+    } while (0)
+
+int check_launch(ariel_sim* h, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(h, ARIEL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return ARIEL_OK;
+}
+
+int validate(const ariel_scene_desc* d, std::string& msg) {
+    if (!d) { msg = "null descriptor"; return ARIEL_ERR_INVALID; }
+    const int n = d->dims[0];
+    if (n != d->dims[1] || n != d->dims[2]) { msg = "grid must be cubic (dims equal)"; return ARIEL_ERR_INVALID; }
+    if (n < 8 || n % 8 != 0) { msg = "grid edge must be a positive multiple of 8"; return ARIEL_ERR_UNSUPPORTED; }
+    if (n > 1024) { msg = "grid edge above 1024"; return ARIEL_ERR_UNSUPPORTED; }
+    if (!(d->density > 0.f)) { msg = "density must be positive"; return ARIEL_ERR_INVALID; }
+    const double m = 1.0 / d->density;
+    if (std::fabs(m - std::round(m)) > 1e-4 || std::round(m) < 1) {
+        msg = "density must be 1/m for an integer m >= 1"; return ARIEL_ERR_INVALID;
+    }
+    if (!(d->dt > 0.f)) { msg = "dt must be positive"; return ARIEL_ERR_INVALID; }
+    if (d->n_forces < 0 || d->n_forces > ARIEL_MAX_FORCES) { msg = "n_forces out of range"; return ARIEL_ERR_INVALID; }
+    if (d->n_liquids < 0 || d->n_liquids > ARIEL_MAX_LIQUIDS) { msg = "n_liquids out of range"; return ARIEL_ERR_INVALID; }
+    for (int l = 0; l < d->n_liquids; ++l)
+        if (d->liquids[l].kind != ARIEL_LIQUID_BOX && d->liquids[l].kind != ARIEL_LIQUID_SPHERE) {
+            msg = "unknown liquid kind"; return ARIEL_ERR_INVALID;
+        }
+    if (d->wall < 0 || 2 * d->wall >= n) { msg = "wall thickness out of range"; return ARIEL_ERR_INVALID; }
+    if (d->precond == ARIEL_PRECOND_MIC0 || d->precond == ARIEL_PRECOND_REFERENCE ||
+        d->precond == ARIEL_PRECOND_LEARNED) {
+        msg = "preconditioner not available on the GPU path (use MIC0_TILED, MIC0_BLOCK or JACOBI)";
+        return ARIEL_ERR_UNSUPPORTED;
+    }
+    if (d->precond != ARIEL_PRECOND_MIC0_TILED && d->precond != ARIEL_PRECOND_MIC0_BLOCK &&
+        d->precond != ARIEL_PRECOND_JACOBI) {
+        msg = "unknown preconditioner"; return ARIEL_ERR_INVALID;
+    }
+    if ((d->precond == ARIEL_PRECOND_MIC0_TILED || d->precond == ARIEL_PRECOND_MIC0_BLOCK) &&
+        d->tile != 8) {
+        msg = "tiled MIC(0) uses 8^3 tiles (tile must be 8)"; return ARIEL_ERR_UNSUPPORTED;
+    }
+    if (!(d->tol > 0.f) && d->stop_rule == ARIEL_STOP_RELATIVE) { msg = "tol must be positive"; return ARIEL_ERR_INVALID; }
+    if (d->max_iters < 1) { msg = "max_iters must be >= 1"; return ARIEL_ERR_INVALID; }
+    if (d->refine_every < 0) { msg = "refine_every must be >= 0"; return ARIEL_ERR_INVALID; }
+    if (d->stop_rule != ARIEL_STOP_RELATIVE && d->stop_rule != ARIEL_STOP_REFERENCE_ABS) { msg = "unknown stop rule"; return ARIEL_ERR_INVALID; }
+    if (d->splat != ARIEL_SPLAT_SHARPEN && d->splat != ARIEL_SPLAT_TRILINEAR) { msg = "unknown splat kernel"; return ARIEL_ERR_INVALID; }
+    return ARIEL_OK;
+}
+
+// ----------------------------------------------------------------- seeding
+// Scene::GenerateParticles (scene.cpp:257-360) + AddLiquidParticle (436-455) with
+// the deterministic order: liquids in descriptor order, interior cells in i,j,k
+// order, lattice octants lexicographically; lattice spacing w = density*(1/maxd),
+// point a at a*w + u*w, u = 0.5 (reference lattice) or a per-point uniform jitter
+// u = (hash4(seed, 3l+axis, a*N*m+b, c) >> 8) * 2^-24.
+struct Seeded {
+    std::vector<float4> pos, vel;
+};
+
+void seed_range(const ariel_scene_desc& d, int N, int l, int ci0, int ci1, Seeded& out) {
+    const float maxd = (float)N;
+    const float thickness = 1.0f / maxd;
+    const float w = d.density * thickness;
+    const int m = (int)std::lround(1.0 / d.density);
+    const uint32_t Nm = (uint32_t)(N * m);
+    const int wall = d.wall;
+    const ariel_liquid& lq = d.liquids[l];
+    for (int ci = ci0; ci < ci1; ++ci)
+        for (int cj = wall; cj < N - wall; ++cj)
+            for (int ck = wall; ck < N - wall; ++ck)
+                for (int oa = 0; oa < m; ++oa)
+                    for (int ob = 0; ob < m; ++ob)
+                        for (int oc = 0; oc < m; ++oc) {
+                            const uint32_t a = (uint32_t)(ci * m + oa), b = (uint32_t)(cj * m + ob),
+                                           c = (uint32_t)(ck * m + oc);
+                            float jx = 0.5f, jy = 0.5f, jz = 0.5f;
+                            if (d.jitter) {
+                                const uint32_t key = a * Nm + b;
+                                jx = (float)(hash4(d.seed, 3u * l + 0u, key, c) >> 8) * (1.0f / 16777216.0f);
+                                jy = (float)(hash4(d.seed, 3u * l + 1u, key, c) >> 8) * (1.0f / 16777216.0f);
+                                jz = (float)(hash4(d.seed, 3u * l + 2u, key, c) >> 8) * (1.0f / 16777216.0f);
+                            }
+                            const float x = (float)a * w + jx * w, y = (float)b * w + jy * w,
+                                        z = (float)c * w + jz * w;
+                            bool inside;
+                            if (lq.kind == ARIEL_LIQUID_BOX) {
+                                inside = x >= lq.lo[0] && x < lq.hi[0] && y >= lq.lo[1] && y < lq.hi[1] &&
+                                         z >= lq.lo[2] && z < lq.hi[2];
+                            } else {
+                                const float dx = x - lq.center[0], dy = y - lq.center[1], dz = z - lq.center[2];
+                                inside = dx * dx + dy * dy + dz * dz < lq.radius * lq.radius;
+                            }
+                            if (!inside) continue;
+                            out.pos.push_back(make_float4(x, y, z, 0.0f));
+                            out.vel.push_back(make_float4(lq.velocity[0], lq.velocity[1], lq.velocity[2], 10.0f));
+                        }
+}
+
+void seed_particles(const ariel_scene_desc& d, int N, std::vector<float4>& pos, std::vector<float4>& vel) {
+    pos.clear();
+    vel.clear();
+    const int wall = d.wall;
+    unsigned hw = std::thread::hardware_concurrency();
+    const int nthreads = (int)std::max(1u, std::min(hw, 16u));
+    for (int l = 0; l < d.n_liquids; ++l) {
+        const int span = N - 2 * wall;
+        const int chunks = std::max(1, std::min(nthreads, span));
+        std::vector<Seeded> parts(chunks);
+        std::vector<std::thread> ts;
+        for (int t = 0; t < chunks; ++t) {
+            const int a0 = wall + span * t / chunks, a1 = wall + span * (t + 1) / chunks;
+            ts.emplace_back([&, t, a0, a1] { seed_range(d, N, l, a0, a1, parts[t]); });
+        }
+        for (auto& th : ts) th.join();
+        for (auto& p : parts) {
+            pos.insert(pos.end(), p.pos.begin(), p.pos.end());
+            vel.insert(vel.end(), p.vel.begin(), p.vel.end());
+        }
+    }
+    for (size_t i = 0; i < pos.size(); ++i) {
+        int32_t id = (int32_t)i;
+        float f;
+        std::memcpy(&f, &id, 4);
+        pos[i].w = f;
+    }
+}
+
+int ensure_particles(ariel_sim* h, int64_t np) {
+    if (np <= h->np_cap && h->pos[0]) return ARIEL_OK;
+    for (int b = 0; b < 2; ++b) {
+        hipFree(h->pos[b]);
+        hipFree(h->vel[b]);
+        h->pos[b] = nullptr;
+        h->vel[b] = nullptr;
+    }
+    hipFree(h->cell_of);
+    hipFree(h->slots);
+    hipFree(h->accept);
+    h->cell_of = h->slots = nullptr;
+    h->accept = nullptr;
+    for (int b = 0; b < 2; ++b) {
+        TRY(dalloc(h, &h->pos[b], np));
+        TRY(dalloc(h, &h->vel[b], np));
+    }
+    TRY(dalloc(h, &h->cell_of, np));
+    TRY(dalloc(h, &h->slots, np));
+    TRY(dalloc(h, &h->accept, np));
+    HIPCHK(h, hipMemsetAsync(h->accept, 1, (size_t)std::max<int64_t>(np, 1), h->stream));
+    h->np_cap = np;
+    return ARIEL_OK;
+}
+
+// ----------------------------------------------------------------- stages
+int stage_sort(ariel_sim* h) {
+    const int b = h->cur, o = 1 - b;
+    launch_bin(h->stream, h->pos[b], h->np, h->N, h->maxd, h->cell_of, h->counts);
+    launch_scan(h->stream, h->counts, h->offsets, h->nc, h->scan_sums);
+    launch_scatter(h->stream, h->cell_of, h->np, h->offsets, h->counts, h->slots);
+    launch_cellsort(h->stream, h->offsets, h->nc, h->pos[b], h->slots);
+    launch_gather(h->stream, h->slots, h->np, h->pos[b], h->vel[b], h->pos[o], h->vel[o]);
+    h->cur = o;
+    return check_launch(h, "sort");
+}
+
+int stage_density(ariel_sim* h, int forces) {
+    launch_density(h->stream, h->params(), h->pos[h->cur], h->vel[h->cur], h->np, h->offsets, forces);
+    return check_launch(h, "density");
+}
+
+int stage_mark(ariel_sim* h) {
+    launch_mark(h->stream, h->params(), h->vel[h->cur], h->offsets, h->A, h->L);
+    return check_launch(h, "mark");
+}
+
+int stage_p2g(ariel_sim* h) {
+    launch_p2g(h->stream, h->params(), h->pos[h->cur], h->vel[h->cur], h->offsets, h->A, h->u, h->du);
+    return check_launch(h, "p2g");
+}
+
+int stage_divergence(ariel_sim* h) {
+    launch_divergence(h->stream, h->params(), h->u, h->D);
+    return check_launch(h, "divergence");
+}
+
+int read_scalars(ariel_sim* h) {
+    HIPCHK(h, hipMemcpyAsync(h->sc_host, h->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+// BuildPreconditioner + SolveConjugateGradient (solver.inl:104-135, 289-354) with
+// the relative stop rule, FP64 reductions and residual replacement.
+int stage_solve(ariel_sim* h) {
+    const Params p = h->params();
+    const Reduce red = h->reduce();
+    hipStream_t s = h->stream;
+    HIPCHK(h, hipEventRecord(h->solve_t0, s));
+    launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
+    launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc);
+    launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every);
+    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule);
+    launch_precond_apply(s, p, h->A, h->pc, h->R, h->Q, h->Z, red, 0);
+    launch_update_s(s, h->nc, h->Z, h->S, red, 1);
+    TRY(check_launch(h, "solve setup"));
+    int queued = 0;
+    const int k = h->d.refine_every;
+    int chunk = std::max(4, h->last_iters - 3);
+    for (;;) {
+        const int n = std::min(chunk, h->d.max_iters - queued);
+        for (int j = 0; j < n; ++j) {
+            const int it = queued + j + 1;
+            launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red);
+            launch_axpy2(s, h->nc, h->P, h->S, h->R, h->Z, red);
+            if (k > 0 && it % k == 0) launch_true_residual(s, p, h->A, h->L, h->P, h->D, h->R, red);
+            launch_precond_apply(s, p, h->A, h->pc, h->R, h->Q, h->Z, red, 0);
+            launch_update_s(s, h->nc, h->Z, h->S, red, 0);
+        }
+        TRY(check_launch(h, "pcg"));
+        queued += n;
+        TRY(read_scalars(h));
+        if (h->sc_host->done || queued >= h->d.max_iters) break;
+        chunk = 4;
+    }
+    HIPCHK(h, hipEventRecord(h->solve_t1, s));
+    h->last_iters = h->sc_host->it;
+    h->stats.iterations = h->sc_host->it;
+    h->stats.solve_status = h->sc_host->status;
+    h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
+    h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
+    // true residual of the final pressure, for the report only
+    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red);
+    TRY(check_launch(h, "true residual"));
+    TRY(read_scalars(h));
+    h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
+    return ARIEL_OK;
+}
+
+int stage_gradient(ariel_sim* h) {
+    launch_gradient(h->stream, h->params(), h->P, h->L, h->A, h->u);
+    launch_grid_stats(h->stream, h->params(), h->u, h->A, h->sc);
+    return check_launch(h, "gradient");
+}
+
+int stage_extrapolate(ariel_sim* h) {
+    launch_extrapolate(h->stream, h->params(), h->A, h->u, h->du);
+    return check_launch(h, "extrapolate");
+}
+
+int stage_advect(ariel_sim* h) {
+    launch_g2p_advect(h->stream, h->params(), h->pos[h->cur], h->vel[h->cur], h->np, h->u, h->du);
+    return check_launch(h, "advect");
+}
+
+int stage_resample(ariel_sim* h) {
+    if (!h->d.resample) return ARIEL_OK;
+    TRY(stage_sort(h));
+    const int b = h->cur, o = 1 - b;
+    launch_resample(h->stream, h->params(), h->pos[b], h->vel[b], h->np, h->offsets, h->pos[o],
+                    h->vel[o], h->accept);
+    h->cur = o;
+    return check_launch(h, "resample");
+}
+
+int run_stage(ariel_sim* h, int stage) {
+    Stage& st = h->stages[stage];
+    HIPCHK(h, hipEventRecord(st.t0, h->stream));
+    int rc = ARIEL_OK;
+    switch (stage) {
+        case ARIEL_STAGE_SORT: rc = stage_sort(h); break;
+        case ARIEL_STAGE_DENSITY: rc = stage_density(h, 1); break;
+        case ARIEL_STAGE_MARK: rc = stage_mark(h); break;
+        case ARIEL_STAGE_P2G: rc = stage_p2g(h); break;
+        case ARIEL_STAGE_DIVERGENCE: rc = stage_divergence(h); break;
+        case ARIEL_STAGE_SOLVE: rc = stage_solve(h); break;
+        case ARIEL_STAGE_GRADIENT: rc = stage_gradient(h); break;
+        case ARIEL_STAGE_EXTRAPOLATE: rc = stage_extrapolate(h); break;
+        case ARIEL_STAGE_ADVECT: rc = stage_advect(h); break;
+        case ARIEL_STAGE_RESAMPLE: rc = stage_resample(h); break;
+        default: return fail(h, ARIEL_ERR_INVALID, "unknown stage");
+    }
+    if (rc != ARIEL_OK) return rc;
+    HIPCHK(h, hipEventRecord(st.t1, h->stream));
+    st.recorded = true;
+    return ARIEL_OK;
+}
+
+int collect_stats(ariel_sim* h) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < ARIEL_NUM_STAGES; ++i) {
+        float ms = 0.f;
+        if (h->stages[i].recorded && hipEventElapsedTime(&ms, h->stages[i].t0, h->stages[i].t1) == hipSuccess)
+            h->stats.stage_ms[i] = ms;
+    }
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->step_t0, h->step_t1) == hipSuccess) h->stats.step_ms = ms;
+    if (hipEventElapsedTime(&ms, h->solve_t0, h->solve_t1) == hipSuccess) h->stats.solve_ms = ms;
+    TRY(read_scalars(h));
+    h->stats.fluid_cells = h->sc_host->fluid_cells;
+    float md;
+    uint32_t bits = h->sc_host->maxdiv_bits;
+    std::memcpy(&md, &bits, 4);
+    h->stats.max_divergence = md;
+    (void)hipGetLastError();
+    return ARIEL_OK;
+}
+
+int check_handle(ariel_sim* h) {
+    if (!h) return fail(nullptr, ARIEL_ERR_INVALID, "null handle");
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return fail(h, ARIEL_ERR_HIP, hipGetErrorString(e));
+    return ARIEL_OK;
+}
+
+int64_t field_size(const ariel_sim* h, int which) { return which <= ARIEL_FIELD_DUZ ? h->nf : h->nc; }
+
+float* field_ptr(ariel_sim* h, int which) {
+    switch (which) {
+        case ARIEL_FIELD_UX: case ARIEL_FIELD_UY: case ARIEL_FIELD_UZ: return h->u[which];
+        case ARIEL_FIELD_DUX: case ARIEL_FIELD_DUY: case ARIEL_FIELD_DUZ: return h->du[which - 3];
+        case ARIEL_FIELD_P: return h->P;
+        case ARIEL_FIELD_D: return h->D;
+        case ARIEL_FIELD_L: return h->L;
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int ariel_desc_defaults(ariel_scene_desc* d) {
+    if (!d) return fail(nullptr, ARIEL_ERR_INVALID, "null descriptor");
+    std::memset(d, 0, sizeof(*d));
+    d->dims[0] = d->dims[1] = d->dims[2] = 32;
+    d->density = 0.5f;
+    d->dt = 0.005f;
+    d->picflip = 0.95f;
+    d->precond = ARIEL_PRECOND_MIC0_TILED;
+    d->mic_tau = 0.97f;
+    d->mic_sigma = 0.25f;
+    d->tile = 8;
+    d->tol = 1e-6f;
+    d->max_iters = 500;
+    d->refine_every = 0;
+    d->stop_rule = ARIEL_STOP_RELATIVE;
+    d->splat = ARIEL_SPLAT_SHARPEN;
+    d->resample = 1;
+    d->sharpen_re = 1.4f;
+    d->spring = 50.0f;
+    return ARIEL_OK;
+}
+
+int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
+    if (!out) return fail(nullptr, ARIEL_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    std::string msg;
+    int rc = validate(d, msg);
+    if (rc != ARIEL_OK) return fail(nullptr, rc, msg);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, ARIEL_ERR_HIP, "no HIP device visible");
+    ariel_sim* h = new ariel_sim();
+    h->d = *d;
+    h->N = d->dims[0];
+    h->nc = (int64_t)h->N * h->N * h->N;
+    h->nf = (int64_t)(h->N + 1) * h->N * h->N;
+    h->maxd = std::max(std::max((float)d->dims[0], (float)d->dims[2]), (float)d->dims[1]);
+    hipGetDevice(&h->device);
+    auto bail = [&](int code) {
+        ariel_destroy(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, ARIEL_ERR_HIP, "hipStreamCreate failed"));
+    for (int c = 0; c < 3; ++c) {
+        if ((rc = dalloc(h, &h->u[c], h->nf)) || (rc = dalloc(h, &h->du[c], h->nf))) return bail(rc);
+    }
+    if ((rc = dalloc(h, &h->P, h->nc)) || (rc = dalloc(h, &h->D, h->nc)) || (rc = dalloc(h, &h->L, h->nc)) ||
+        (rc = dalloc(h, &h->A, h->nc)) || (rc = dalloc(h, &h->R, h->nc)) || (rc = dalloc(h, &h->Z, h->nc)) ||
+        (rc = dalloc(h, &h->S, h->nc)) || (rc = dalloc(h, &h->Q, h->nc)) || (rc = dalloc(h, &h->pc, h->nc)) ||
+        (rc = dalloc(h, &h->diag, h->nc)) || (rc = dalloc(h, &h->nup, h->nc)) ||
+        (rc = dalloc(h, &h->counts, h->nc)) || (rc = dalloc(h, &h->offsets, h->nc + 1)) ||
+        (rc = dalloc(h, &h->scan_sums, h->nc / 4096 + 4)) ||
+        (rc = dalloc(h, &h->partials, 2 * (int64_t)kMaxPartials)) || (rc = dalloc(h, &h->counter, 1)) ||
+        (rc = dalloc(h, &h->sc, 1)))
+        return bail(rc);
+    if (hipHostMalloc((void**)&h->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(nullptr, ARIEL_ERR_NOMEM, "hipHostMalloc failed"));
+    for (int i = 0; i < ARIEL_NUM_STAGES; ++i) {
+        hipEventCreate(&h->stages[i].t0);
+        hipEventCreate(&h->stages[i].t1);
+    }
+    hipEventCreate(&h->step_t0);
+    hipEventCreate(&h->step_t1);
+    hipEventCreate(&h->solve_t0);
+    hipEventCreate(&h->solve_t1);
+    hipStream_t s = h->stream;
+    for (int c = 0; c < 3; ++c) {
+        launch_fill_f32(s, h->u[c], h->nf, 0.0f);
+        launch_fill_f32(s, h->du[c], h->nf, 0.0f);
+    }
+    launch_fill_f32(s, h->P, h->nc, 0.0f);
+    launch_fill_f32(s, h->D, h->nc, 0.0f);
+    launch_fill_f32(s, h->L, h->nc, 1.6f);   // macgrid.inl:82
+    for (float* v : {h->R, h->Z, h->S, h->Q, h->pc, h->diag}) launch_fill_f32(s, v, h->nc, 0.0f);
+    launch_fill_i32(s, h->counts, h->nc, 0);
+    launch_fill_i32(s, h->offsets, h->nc + 1, 0);
+    if (hipMemsetAsync(h->A, 0, (size_t)h->nc, s) != hipSuccess ||
+        hipMemsetAsync(h->nup, 0, (size_t)h->nc, s) != hipSuccess ||
+        hipMemsetAsync(h->counter, 0, sizeof(uint32_t), s) != hipSuccess ||
+        hipMemsetAsync(h->sc, 0, sizeof(SolverScalars), s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return bail(fail(nullptr, ARIEL_ERR_HIP, "device initialisation failed"));
+    *out = h;
+    return ARIEL_OK;
+}
+
+int ariel_destroy(ariel_sim* h) {
+    if (!h) return ARIEL_OK;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (int b = 0; b < 2; ++b) {
+        hipFree(h->pos[b]);
+        hipFree(h->vel[b]);
+    }
+    for (void* p : {(void*)h->cell_of, (void*)h->slots, (void*)h->counts, (void*)h->offsets,
+                    (void*)h->scan_sums, (void*)h->accept, (void*)h->P, (void*)h->D, (void*)h->L,
+                    (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
+                    (void*)h->diag, (void*)h->nup, (void*)h->partials, (void*)h->counter, (void*)h->sc})
+        hipFree(p);
+    for (int c = 0; c < 3; ++c) {
+        hipFree(h->u[c]);
+        hipFree(h->du[c]);
+    }
+    if (h->sc_host) hipHostFree(h->sc_host);
+    for (int i = 0; i < ARIEL_NUM_STAGES; ++i) {
+        if (h->stages[i].t0) hipEventDestroy(h->stages[i].t0);
+        if (h->stages[i].t1) hipEventDestroy(h->stages[i].t1);
+    }
+    for (hipEvent_t e : {h->step_t0, h->step_t1, h->solve_t0, h->solve_t1})
+        if (e) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return ARIEL_OK;
+}
+
+const char* ariel_last_error(const ariel_sim* h) {
+    if (h) return h->err.c_str();
+    return g_last_error.c_str();
+}
+
+int ariel_init(ariel_sim* h) {
+    TRY(check_handle(h));
+    // Max-density calibration (flip.cpp:42-71): a 10^3 lattice at spacing
+    // density/maxd, sorted and run through ComputeDensity with m_max_density = 1.
+    const float hsp = h->d.density / h->maxd;
+    std::vector<float4> tp(1000), tv(1000, make_float4(0, 0, 0, 0));
+    for (int i = 0, n = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j)
+            for (int k = 0; k < 10; ++k, ++n) {
+                int32_t id = n;
+                float f;
+                std::memcpy(&f, &id, 4);
+                tp[n] = make_float4(((float)i + 0.5f) * hsp, ((float)j + 0.5f) * hsp, ((float)k + 0.5f) * hsp, f);
+            }
+    TRY(ensure_particles(h, 1000));
+    h->cur = 0;
+    h->np = 1000;
+    HIPCHK(h, hipMemcpyAsync(h->pos[0], tp.data(), 1000 * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->vel[0], tv.data(), 1000 * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    TRY(stage_sort(h));
+    h->max_density = 1.0f;
+    TRY(stage_density(h, 0));
+    HIPCHK(h, hipMemcpyAsync(tv.data(), h->vel[h->cur], 1000 * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    float md = 0.0f;
+    for (const float4& v : tv) md = gmax(md, v.w);
+    h->max_density = md;
+
+    // Scene::GenerateParticles at frame 0, Sort, MarkCellTypes (flip.cpp:74-76).
+    std::vector<float4> pos, vel;
+    seed_particles(h->d, h->N, pos, vel);
+    if (pos.size() >= (size_t)INT32_MAX) return fail(h, ARIEL_ERR_UNSUPPORTED, "too many particles");
+    h->np = (int64_t)pos.size();
+    h->np_cap = 0;
+    TRY(ensure_particles(h, std::max<int64_t>(h->np, 1)));
+    h->cur = 0;
+    if (h->np) {
+        HIPCHK(h, hipMemcpyAsync(h->pos[0], pos.data(), h->np * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->vel[0], vel.data(), h->np * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    }
+    TRY(stage_sort(h));
+    TRY(stage_mark(h));
+    launch_fill_f32(h->stream, h->L, h->nc, 1.6f);   // Init marks A only; L keeps its background
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->frame = 0;
+    h->initialized = true;
+    h->stats.particles = h->np;
+    h->stats.max_density = h->max_density;
+    return check_launch(h, "init");
+}
+
+int ariel_step(ariel_sim* h, int32_t nsteps) {
+    TRY(check_handle(h));
+    if (!h->initialized) return fail(h, ARIEL_ERR_STATE, "ariel_step before ariel_init");
+    if (nsteps < 0) return fail(h, ARIEL_ERR_INVALID, "negative step count");
+    for (int n = 0; n < nsteps; ++n) {
+        h->frame++;
+        HIPCHK(h, hipEventRecord(h->step_t0, h->stream));
+        for (int st = 0; st < ARIEL_NUM_STAGES; ++st) TRY(run_stage(h, st));
+        HIPCHK(h, hipEventRecord(h->step_t1, h->stream));
+    }
+    if (nsteps > 0) TRY(collect_stats(h));
+    h->stats.frame = h->frame;
+    return ARIEL_OK;
+}
+
+int ariel_run_stage(ariel_sim* h, int32_t stage) {
+    TRY(check_handle(h));
+    if (!h->initialized && h->np == 0 && stage != ARIEL_STAGE_SOLVE)
+        return fail(h, ARIEL_ERR_STATE, "no particles: call ariel_init or ariel_set_particles");
+    if (stage < 0 || stage >= ARIEL_NUM_STAGES) return fail(h, ARIEL_ERR_INVALID, "unknown stage");
+    TRY(run_stage(h, stage));
+    TRY(collect_stats(h));
+    return ARIEL_OK;
+}
+
+int ariel_sync(ariel_sim* h) {
+    TRY(check_handle(h));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+int ariel_get_frame(ariel_sim* h, int32_t* frame) {
+    TRY(check_handle(h));
+    if (!frame) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    *frame = h->frame;
+    return ARIEL_OK;
+}
+
+int ariel_get_dimensions(ariel_sim* h, int32_t dims[3]) {
+    TRY(check_handle(h));
+    if (!dims) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    for (int i = 0; i < 3; ++i) dims[i] = h->d.dims[i];
+    return ARIEL_OK;
+}
+
+int ariel_particle_count(ariel_sim* h, int64_t* n) {
+    TRY(check_handle(h));
+    if (!n) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    *n = h->np;
+    return ARIEL_OK;
+}
+
+int ariel_get_particles(ariel_sim* h, float* pos, float* vel, float* density, int64_t cap, int64_t* n) {
+    TRY(check_handle(h));
+    if (n) *n = h->np;
+    if (!pos && !vel && !density) return ARIEL_OK;
+    if (cap < h->np) return fail(h, ARIEL_ERR_CAPACITY, "particle buffer too small");
+    if (h->np == 0) return ARIEL_OK;
+    const int b = h->cur, o = 1 - b;
+    // scatter to id order into the idle buffers
+    launch_unsort(h->stream, h->pos[b], h->vel[b], nullptr, h->np, h->pos[o], h->vel[o], nullptr, nullptr);
+    TRY(check_launch(h, "unsort"));
+    std::vector<float4> hp(h->np), hv(h->np);
+    HIPCHK(h, hipMemcpyAsync(hp.data(), h->pos[o], h->np * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(hv.data(), h->vel[o], h->np * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int64_t i = 0; i < h->np; ++i) {
+        if (pos) { pos[3 * i] = hp[i].x; pos[3 * i + 1] = hp[i].y; pos[3 * i + 2] = hp[i].z; }
+        if (vel) { vel[3 * i] = hv[i].x; vel[3 * i + 1] = hv[i].y; vel[3 * i + 2] = hv[i].z; }
+        if (density) density[i] = hv[i].w;
+    }
+    return ARIEL_OK;
+}
+
+int ariel_set_particles(ariel_sim* h, const float* pos, const float* vel, const float* density, int64_t n) {
+    TRY(check_handle(h));
+    if (n < 0 || (n > 0 && !pos)) return fail(h, ARIEL_ERR_INVALID, "bad particle arrays");
+    if (n >= INT32_MAX) return fail(h, ARIEL_ERR_UNSUPPORTED, "too many particles");
+    TRY(ensure_particles(h, std::max<int64_t>(n, 1)));
+    std::vector<float4> hp(n), hv(n);
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t id = (int32_t)i;
+        float f;
+        std::memcpy(&f, &id, 4);
+        hp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], f);
+        hv[i] = vel ? make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], density ? density[i] : 10.0f)
+                    : make_float4(0.f, 0.f, 0.f, density ? density[i] : 10.0f);
+    }
+    h->cur = 0;
+    h->np = n;
+    if (n) {
+        HIPCHK(h, hipMemcpyAsync(h->pos[0], hp.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->vel[0], hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->initialized = true;
+    h->stats.particles = n;
+    return ARIEL_OK;
+}
+
+int ariel_get_macgrid(ariel_sim* h, int32_t which, float* dst, int64_t cap) {
+    TRY(check_handle(h));
+    float* src = field_ptr(h, which);
+    if (!src) return fail(h, ARIEL_ERR_INVALID, "unknown field");
+    const int64_t n = field_size(h, which);
+    if (!dst || cap < n) return fail(h, ARIEL_ERR_CAPACITY, "field buffer too small");
+    HIPCHK(h, hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+int ariel_set_macgrid(ariel_sim* h, int32_t which, const float* src, int64_t n) {
+    TRY(check_handle(h));
+    float* dst = field_ptr(h, which);
+    if (!dst) return fail(h, ARIEL_ERR_INVALID, "unknown field");
+    if (!src || n != field_size(h, which)) return fail(h, ARIEL_ERR_INVALID, "field size mismatch");
+    HIPCHK(h, hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+int ariel_get_flags(ariel_sim* h, uint8_t* dst, int64_t cap) {
+    TRY(check_handle(h));
+    if (!dst || cap < h->nc) return fail(h, ARIEL_ERR_CAPACITY, "flag buffer too small");
+    HIPCHK(h, hipMemcpyAsync(dst, h->A, h->nc, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+int ariel_set_flags(ariel_sim* h, const uint8_t* src, int64_t n) {
+    TRY(check_handle(h));
+    if (!src || n != h->nc) return fail(h, ARIEL_ERR_INVALID, "flag size mismatch");
+    for (int64_t i = 0; i < n; ++i)
+        if (src[i] > 2) return fail(h, ARIEL_ERR_INVALID, "flag values must be 0, 1 or 2");
+    HIPCHK(h, hipMemcpyAsync(h->A, src, n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+int ariel_get_sort(ariel_sim* h, int32_t* ids, int64_t ids_cap, int32_t* offsets, int64_t offsets_cap) {
+    TRY(check_handle(h));
+    if (ids) {
+        if (ids_cap < h->np) return fail(h, ARIEL_ERR_CAPACITY, "ids buffer too small");
+        if (h->np) {
+            launch_unsort(h->stream, h->pos[h->cur], nullptr, nullptr, h->np, nullptr, nullptr, nullptr, h->slots);
+            TRY(check_launch(h, "ids"));
+            HIPCHK(h, hipMemcpyAsync(ids, h->slots, h->np * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        }
+    }
+    if (offsets) {
+        if (offsets_cap < h->nc + 1) return fail(h, ARIEL_ERR_CAPACITY, "offsets buffer too small");
+        HIPCHK(h, hipMemcpyAsync(offsets, h->offsets, (h->nc + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+int ariel_get_resample_accept(ariel_sim* h, uint8_t* dst, int64_t cap) {
+    TRY(check_handle(h));
+    if (!dst || cap < h->np) return fail(h, ARIEL_ERR_CAPACITY, "accept buffer too small");
+    if (h->np == 0) return ARIEL_OK;
+    // accept[] is indexed by the slot order of the resample pass, whose ids are
+    // those of the current buffer (resample keeps the slot order).
+    uint8_t* tmp = nullptr;
+    TRY(dalloc(h, &tmp, h->np));
+    launch_unsort(h->stream, h->pos[h->cur], nullptr, h->accept, h->np, nullptr, nullptr, tmp, nullptr);
+    int rc = check_launch(h, "accept");
+    if (rc == ARIEL_OK) {
+        hipError_t e = hipMemcpyAsync(dst, tmp, h->np, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(h, ARIEL_ERR_HIP, hipGetErrorString(e));
+    }
+    hipFree(tmp);
+    return rc;
+}
+
+int ariel_get_max_density(ariel_sim* h, float* v) {
+    TRY(check_handle(h));
+    if (!v) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    *v = h->max_density;
+    return ARIEL_OK;
+}
+
+int ariel_set_max_density(ariel_sim* h, float v) {
+    TRY(check_handle(h));
+    if (!(v > 0.f)) return fail(h, ARIEL_ERR_INVALID, "max density must be positive");
+    h->max_density = v;
+    return ARIEL_OK;
+}
+
+int ariel_get_stats(ariel_sim* h, ariel_step_stats* s) {
+    TRY(check_handle(h));
+    if (!s) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    h->stats.frame = h->frame;
+    h->stats.particles = h->np;
+    h->stats.max_density = h->max_density;
+    *s = h->stats;
+    return ARIEL_OK;
+}
+
+int ariel_apply_laplacian(ariel_sim* h, const float* x, float* y) {
+    TRY(check_handle(h));
+    if (!x || !y) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    HIPCHK(h, hipMemcpyAsync(h->S, x, h->nc * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    launch_spmv_plain(h->stream, h->params(), h->A, h->L, h->S, h->Z);
+    TRY(check_launch(h, "laplacian"));
+    HIPCHK(h, hipMemcpyAsync(y, h->Z, h->nc * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
+    TRY(check_handle(h));
+    if (!r || !z) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    const Params p = h->params();
+    HIPCHK(h, hipMemcpyAsync(h->R, r, h->nc * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    launch_precond_setup(h->stream, p, h->A, h->L, h->diag, h->nup, h->pc);
+    launch_precond_build(h->stream, p, h->A, h->diag, h->nup, h->pc);
+    launch_solver_init(h->stream, h->sc, h->d.max_iters, h->d.refine_every);
+    launch_fill_f32(h->stream, h->Z, h->nc, 0.0f);
+    launch_precond_apply(h->stream, p, h->A, h->pc, h->R, h->Q, h->Z, h->reduce(), 0);
+    TRY(check_launch(h, "preconditioner"));
+    HIPCHK(h, hipMemcpyAsync(z, h->Z, h->nc * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+// ------------------------------------------------------------ state files
+static const char kMagic[8] = {'A', 'R', 'I', 'E', 'L', 'S', 'T', '1'};
+
+int ariel_save_state(ariel_sim* h, const char* path) {
+    TRY(check_handle(h));
+    if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
+    std::vector<float> pos(3 * h->np), vel(3 * h->np), rho(h->np);
+    int64_t n = 0;
+    TRY(ariel_get_particles(h, pos.data(), vel.data(), rho.data(), h->np, &n));
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(h, ARIEL_ERR_IO, std::string("cannot open ") + path);
+    bool ok = std::fwrite(kMagic, 1, 8, f) == 8;
+    ok = ok && std::fwrite(&h->d, sizeof(h->d), 1, f) == 1;
+    ok = ok && std::fwrite(&h->frame, sizeof(int), 1, f) == 1;
+    ok = ok && std::fwrite(&h->max_density, sizeof(float), 1, f) == 1;
+    ok = ok && std::fwrite(&h->np, sizeof(int64_t), 1, f) == 1;
+    ok = ok && std::fwrite(pos.data(), sizeof(float), pos.size(), f) == pos.size();
+    ok = ok && std::fwrite(vel.data(), sizeof(float), vel.size(), f) == vel.size();
+    ok = ok && std::fwrite(rho.data(), sizeof(float), rho.size(), f) == rho.size();
+    std::vector<float> buf;
+    for (int which = 0; ok && which <= ARIEL_FIELD_L; ++which) {
+        buf.resize(field_size(h, which));
+        if (ariel_get_macgrid(h, which, buf.data(), (int64_t)buf.size()) != ARIEL_OK) { ok = false; break; }
+        ok = std::fwrite(buf.data(), sizeof(float), buf.size(), f) == buf.size();
+    }
+    std::vector<uint8_t> flags(h->nc);
+    if (ok && ariel_get_flags(h, flags.data(), h->nc) != ARIEL_OK) ok = false;
+    ok = ok && std::fwrite(flags.data(), 1, flags.size(), f) == flags.size();
+    std::fclose(f);
+    if (!ok) return fail(h, ARIEL_ERR_IO, std::string("write failed: ") + path);
+    return ARIEL_OK;
+}
+
+int ariel_load_state(ariel_sim* h, const char* path) {
+    TRY(check_handle(h));
+    if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(h, ARIEL_ERR_IO, std::string("cannot open ") + path);
+    char magic[8];
+    ariel_scene_desc d;
+    int frame = 0;
+    float md = 0.f;
+    int64_t np = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMagic, 8) == 0;
+    ok = ok && std::fread(&d, sizeof(d), 1, f) == 1;
+    ok = ok && std::memcmp(&d, &h->d, sizeof(d)) == 0;
+    ok = ok && std::fread(&frame, sizeof(int), 1, f) == 1;
+    ok = ok && std::fread(&md, sizeof(float), 1, f) == 1;
+    ok = ok && std::fread(&np, sizeof(int64_t), 1, f) == 1 && np >= 0 && np < INT32_MAX;
+    if (!ok) {
+        std::fclose(f);
+        return fail(h, ARIEL_ERR_IO, "state file header mismatch");
+    }
+    std::vector<float> pos(3 * np), vel(3 * np), rho(np);
+    ok = std::fread(pos.data(), sizeof(float), pos.size(), f) == pos.size();
+    ok = ok && std::fread(vel.data(), sizeof(float), vel.size(), f) == vel.size();
+    ok = ok && std::fread(rho.data(), sizeof(float), rho.size(), f) == rho.size();
+    std::vector<std::vector<float>> fields(ARIEL_FIELD_L + 1);
+    for (int which = 0; ok && which <= ARIEL_FIELD_L; ++which) {
+        fields[which].resize(field_size(h, which));
+        ok = std::fread(fields[which].data(), sizeof(float), fields[which].size(), f) == fields[which].size();
+    }
+    std::vector<uint8_t> flags(h->nc);
+    ok = ok && std::fread(flags.data(), 1, flags.size(), f) == flags.size();
+    std::fclose(f);
+    if (!ok) return fail(h, ARIEL_ERR_IO, "state file truncated");
+    TRY(ariel_set_particles(h, pos.data(), vel.data(), rho.data(), np));
+    for (int which = 0; which <= ARIEL_FIELD_L; ++which)
+        TRY(ariel_set_macgrid(h, which, fields[which].data(), (int64_t)fields[which].size()));
+    TRY(ariel_set_flags(h, flags.data(), h->nc));
+    h->frame = frame;
+    h->max_density = md;
+    // particles are unsorted after a load: sort so that every stage may follow
+    TRY(stage_sort(h));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+// Narrow-band level set of the FLUID particles: spheres of radius 0.5 in grid
+// units (levelset.cpp:101-111, ParticlesToLevelSet with r = 0.5 at voxel size 1,
+// positions scaled by maxd) sampled on the (N+1)^3 lattice of integer points.
+// phi = min over particles of |x - p| - 0.5, clamped to the band of 3 voxels.
+int ariel_export_levelset(ariel_sim* h, const char* path) {
+    TRY(check_handle(h));
+    if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
+    std::vector<float> pos(3 * h->np);
+    int64_t n = 0;
+    TRY(ariel_get_particles(h, pos.data(), nullptr, nullptr, h->np, &n));
+    const int M = h->N + 1;
+    const float band = 3.0f, rad = 0.5f;
+    std::vector<float> phi((size_t)M * M * M, band);
+    for (int64_t p = 0; p < n; ++p) {
+        const float x = pos[3 * p] * h->maxd, y = pos[3 * p + 1] * h->maxd, z = pos[3 * p + 2] * h->maxd;
+        const int r = (int)std::ceil(band + rad);
+        for (int i = std::max(0, (int)x - r); i <= std::min(M - 1, (int)x + r); ++i)
+            for (int j = std::max(0, (int)y - r); j <= std::min(M - 1, (int)y + r); ++j)
+                for (int k = std::max(0, (int)z - r); k <= std::min(M - 1, (int)z + r); ++k) {
+                    const float dx = i - x, dy = j - y, dz = k - z;
+                    const float d = std::sqrt(dx * dx + dy * dy + dz * dz) - rad;
+                    float& v = phi[((size_t)i * M + j) * M + k];
+                    if (d < v) v = d;
+                }
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(h, ARIEL_ERR_IO, std::string("cannot open ") + path);
+    char header[256];
+    int len = std::snprintf(header, sizeof(header),
+                            "ARIEL_LEVELSET 1\ndims %d %d %d\nvoxel 1.0\nbackground %.6f\nfloat32 le\n", M, M, M, band);
+    bool ok = std::fwrite(header, 1, (size_t)len, f) == (size_t)len;
+    ok = ok && std::fwrite(phi.data(), sizeof(float), phi.size(), f) == phi.size();
+    std::fclose(f);
+    if (!ok) return fail(h, ARIEL_ERR_IO, std::string("write failed: ") + path);
+    return ARIEL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// ariel_device.h — device-side arithmetic shared by the FLIP kernels.
+//
+// Every helper reproduces the reference's float expression and evaluation order
+// so that the kernels agree bit for bit with a serial evaluation (sources are
+// compiled with -ffp-contract=off and IEEE division/sqrt):
+//   glm::max(x,y) = (x<y)?y:x, glm::min(x,y) = (y<x)?y:x.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ariel {
+
+__host__ __device__ __forceinline__ float gmax(float x, float y) { return (x < y) ? y : x; }
+__host__ __device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
+
+// Sqrlength (src/math/kernels.inl:25-30)
+__device__ __forceinline__ float sqrlength(float ax, float ay, float az, float bx, float by,
+                                           float bz) {
+    float a = ax - bx, b = ay - by, c = az - bz;
+    return a * a + b * b + c * c;
+}
+// Smooth (kernels.inl:32-34)
+__device__ __forceinline__ float smooth(float r2, float h) {
+    return gmax(1.0f - r2 / (h * h), 0.0f);
+}
+// Sharpen (kernels.inl:36-38)
+__device__ __forceinline__ float sharpen(float r2, float h) {
+    return gmax(h * h / gmax(r2, 1.0e-5f) - 1.0f, 0.0f);
+}
+
+// 32-bit integer finaliser (lowbias32) and the 4-key hash used for the octant
+// jitter and the resampler's coincident-neighbour jitter.
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+__host__ __device__ __forceinline__ uint32_t hash4(uint32_t seed, uint32_t a, uint32_t b,
+                                                   uint32_t c) {
+    return mix32(mix32(mix32(mix32(seed) ^ a) ^ b) ^ c);
+}
+
+// Particle cell (ParticleGrid::Sort, particlegrid.cpp:156-158; ComputeDensity,
+// flip.cpp:711-713): (int)max(0, min((int)maxd-1, (int)maxd * p)).
+__device__ __forceinline__ int cell_coord(float p, float maxd) {
+    float fm = (float)(int)maxd;
+    return (int)gmax(0.0f, gmin(fm - 1.0f, fm * p));
+}
+
+__host__ __device__ __forceinline__ int64_t cidx(int N, int i, int j, int k) {
+    return ((int64_t)i * N + j) * N + k;
+}
+// face index of component c at (i,j,k), see include/ariel.h
+__host__ __device__ __forceinline__ int64_t fidx(int N, int c, int i, int j, int k) {
+    if (c == 0) return ((int64_t)i * N + j) * N + k;
+    if (c == 1) return ((int64_t)i * (N + 1) + j) * N + k;
+    return ((int64_t)i * N + j) * (N + 1) + k;
+}
+
+// Interpolate (particlegridoperations.inl:106-118) on one staggered component with
+// extents (nx, ny, nz) = n (floats).
+__device__ __forceinline__ float interpolate(const float* __restrict__ q, int N, int comp, float px,
+                                             float py, float pz, float nx, float ny, float nz) {
+    float x = gmax(0.0f, gmin(nx, px));
+    float y = gmax(0.0f, gmin(ny, py));
+    float z = gmax(0.0f, gmin(nz, pz));
+    int i = (int)gmin(x, nx - 2);
+    int j = (int)gmin(y, ny - 2);
+    int k = (int)gmin(z, nz - 2);
+    float fi1 = (float)(i + 1) - x, fi0 = x - (float)i;
+    float fj1 = (float)(j + 1) - y, fj0 = y - (float)j;
+    float q000 = q[fidx(N, comp, i, j, k)], q100 = q[fidx(N, comp, i + 1, j, k)];
+    float q010 = q[fidx(N, comp, i, j + 1, k)], q110 = q[fidx(N, comp, i + 1, j + 1, k)];
+    float q001 = q[fidx(N, comp, i, j, k + 1)], q101 = q[fidx(N, comp, i + 1, j, k + 1)];
+    float q011 = q[fidx(N, comp, i, j + 1, k + 1)], q111 = q[fidx(N, comp, i + 1, j + 1, k + 1)];
+    float term1 = (fi1 * q000 + fi0 * q100) * fj1;
+    float term2 = (fi1 * q010 + fi0 * q110) * fj0;
+    float term3 = (fi1 * q001 + fi0 * q101) * fj1;
+    float term4 = (fi1 * q011 + fi0 * q111) * fj0;
+    return ((float)(k + 1) - z) * (term1 + term2) + (z - (float)k) * (term3 + term4);
+}
+
+// InterpolateVelocity (particlegridoperations.inl:120-133): maxd on every axis.
+__device__ __forceinline__ void interpolate_velocity(float* const* g, int N, float px, float py,
+                                                     float pz, float& ux, float& uy, float& uz) {
+    const float x = (float)N, y = (float)N, z = (float)N;
+    ux = interpolate(g[0], N, 0, x * px, y * py - 0.5f, z * pz - 0.5f, x + 1, y, z);
+    uy = interpolate(g[1], N, 1, x * px - 0.5f, y * py, z * pz - 0.5f, x, y + 1, z);
+    uz = interpolate(g[2], N, 2, x * px - 0.5f, y * py - 0.5f, z * pz, x, y, z + 1);
+}
+
+// Wave-level FP64 sum (64-lane wavefront) via DPP-capable shuffles.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+}  // namespace ariel

@@ -1,0 +1,136 @@
+// ariel_internal.h — device state and kernel launchers of the FLIP step.
+//
+// Layout in HBM (one simulation handle, N = grid edge, Np particles):
+//   particles, sorted by (cell, id), ping-pong pair:
+//     pos[b] : float4 (x, y, z, id as int bits)          16 B/particle
+//     vel[b] : float4 (ux, uy, uz, density)              16 B/particle
+//   sort: cell_of[Np], slots[Np] (int32), counts[N^3] (int32, zero between
+//         sorts), offsets[N^3+1] (int32)
+//   MAC grid: u[3], du[3] (faces, float), P, D, L (cells, float), A (cells, u8)
+//   solver: R, Z, S, Q, pc, diag (float), nup (u8), reduction partials (double)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ariel {
+
+constexpr int kAir = 0, kFluid = 1, kSolid = 2;
+constexpr int kBlock = 256;      // default threads per block of streaming kernels
+constexpr int kMaxPartials = 65536;
+
+// Reduction/solver scalars living in device memory. Written by the last block of
+// a reduction ("last block finalises"), read by later kernels; no host round
+// trip inside a PCG iteration.
+struct SolverScalars {
+    double bnorm2;      // ||b||^2 over fluid cells
+    double rnorm2;      // ||r||^2
+    double thr;         // stop threshold on rnorm2
+    double rho;         // z.r
+    double sigma;       // s.As
+    double alpha;       // rho / sigma
+    double beta;        // rho_new / rho
+    double tmp;         // scratch
+    int32_t done;       // 1 once converged / failed / out of iterations
+    int32_t it;         // completed iterations
+    int32_t status;     // ARIEL_SOLVE_*
+    int32_t max_iters;
+    int32_t refine_every;
+    int32_t fluid_cells;
+    uint32_t maxdiv_bits;  // max |div| as float bits (non-negative floats order as ints)
+    int32_t pad;
+};
+
+struct Params {
+    int N;
+    float maxd;
+    float density;     // lattice spacing in cells
+    float dt;
+    float picflip;
+    float sharpen_re;
+    float spring;
+    int splat;
+    int wall;
+    uint32_t seed;
+    int n_forces;
+    float forces[4][3];
+    float max_density;
+    int frame;
+    float mic_tau, mic_sigma;
+    int precond;
+};
+
+// Reduction workspace: per-block double partials + an arrival counter.
+struct Reduce {
+    double* partials;      // kMaxPartials * 2
+    uint32_t* counter;     // 1
+    SolverScalars* sc;     // device scalars
+};
+
+// ---------------------------------------------------------------- launchers
+// particles (kernels_particles.hip)
+void launch_bin(hipStream_t s, const float4* pos, int64_t np, int N, float maxd, int32_t* cell_of,
+                int32_t* counts);
+void launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, int32_t* block_sums);
+void launch_scatter(hipStream_t s, const int32_t* cell_of, int64_t np, const int32_t* offsets,
+                    int32_t* counts, int32_t* slots);
+void launch_cellsort(hipStream_t s, const int32_t* offsets, int64_t nc, const float4* pos,
+                     int32_t* slots);
+void launch_gather(hipStream_t s, const int32_t* slots, int64_t np, const float4* pos_in,
+                   const float4* vel_in, float4* pos_out, float4* vel_out);
+void launch_density(hipStream_t s, const Params& p, const float4* pos, float4* vel, int64_t np,
+                    const int32_t* offsets, int apply_forces);
+void launch_mark(hipStream_t s, const Params& p, const float4* vel, const int32_t* offsets,
+                 uint8_t* A, float* L);
+void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
+                const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3]);
+void launch_g2p_advect(hipStream_t s, const Params& p, float4* pos, float4* vel, int64_t np,
+                       float* const u[3], float* const du[3]);
+void launch_resample(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
+                     int64_t np, const int32_t* offsets, float4* pos_out, float4* vel_out,
+                     uint8_t* accept);
+void launch_unsort(hipStream_t s, const float4* pos, const float4* vel, const uint8_t* accept,
+                   int64_t np, float4* pos_out, float4* vel_out, uint8_t* accept_out,
+                   int32_t* ids_out);
+void launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+void launch_fill_f32(hipStream_t s, float* p, int64_t n, float v);
+
+// grid (kernels_grid.hip)
+void launch_divergence(hipStream_t s, const Params& p, float* const u[3], float* D);
+void launch_gradient(hipStream_t s, const Params& p, const float* P, const float* L,
+                     const uint8_t* A, float* const u[3]);
+void launch_extrapolate(hipStream_t s, const Params& p, const uint8_t* A, float* const u[3],
+                        float* const du[3]);
+void launch_grid_stats(hipStream_t s, const Params& p, float* const u[3], const uint8_t* A,
+                       SolverScalars* sc);
+
+// pressure solve (kernels_pcg.hip)
+void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                          float* diag, uint8_t* nup, float* pc);
+void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
+                          const uint8_t* nup, float* pc);
+// z = M^-1 r; also finalises rho_new = z.r and beta when `finalize_rho` is set.
+void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* A, const float* pc,
+                          const float* r, float* q, float* z, const Reduce& red, int mode);
+void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                      float* P, const float* D, float* R, const Reduce& red, double tol,
+                      int stop_rule);
+void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                     const float* x, float* y, const Reduce& red);
+void launch_axpy2(hipStream_t s, int64_t n, float* P, const float* S, float* R, const float* Z,
+                  const Reduce& red);
+void launch_true_residual(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                          const float* P, const float* D, float* R, const Reduce& red);
+void launch_update_s(hipStream_t s, int64_t n, const float* Z, float* S, const Reduce& red,
+                     int first);
+void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                       const float* x, float* y);
+void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every);
+// ||b - A p||^2 in FP64 into sc->tmp (R untouched).
+void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                               const float* P, const float* D, const Reduce& red);
+
+// reduction helpers
+int reduce_blocks(int64_t n, int per_thread);
+
+}  // namespace ariel

@@ -1,0 +1,192 @@
+// kernels_grid.hip — MAC-grid kernels of the projection for gfx950.
+//
+// Each kernel is one streaming pass over faces or cells (z-fastest, coalesced);
+// the reference's separate loops over x/y/z faces (flip.cpp:324-680) are fused
+// where the result of a face depends only on that face and its two cells.
+#include "ariel_device.h"
+#include "ariel_internal.h"
+
+namespace ariel {
+
+namespace {
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+__device__ __forceinline__ void decode_face(int N, int comp, int64_t f, int& i, int& j, int& k) {
+    if (comp == 0) { k = (int)(f % N); j = (int)((f / N) % N); i = (int)(f / ((int64_t)N * N)); }
+    else if (comp == 1) { k = (int)(f % N); j = (int)((f / N) % (N + 1)); i = (int)(f / ((int64_t)N * (N + 1))); }
+    else { k = (int)(f % (N + 1)); j = (int)((f / (N + 1)) % N); i = (int)(f / ((int64_t)N * (N + 1))); }
+}
+
+// Project, divergence per cell (flip.cpp:417-437) and FlipGrid (solver.inl:42-56):
+// D = -((ux(i+1)-ux(i) + uy(j+1)-uy(j) + uz(k+1)-uz(k)) / h), h = 1/maxd.
+__global__ void k_divergence(Params p, const float* __restrict__ ux, const float* __restrict__ uy,
+                             const float* __restrict__ uz, float* __restrict__ D) {
+    const int N = p.N;
+    const int64_t nc = (int64_t)N * N * N;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int k = (int)(c % N), j = (int)((c / N) % N), i = (int)(c / ((int64_t)N * N));
+    const float h = 1.0f / p.maxd;
+    float dv = (ux[fidx(N, 0, i + 1, j, k)] - ux[fidx(N, 0, i, j, k)] + uy[fidx(N, 1, i, j + 1, k)] -
+                uy[fidx(N, 1, i, j, k)] + uz[fidx(N, 2, i, j, k + 1)] - uz[fidx(N, 2, i, j, k)]) / h;
+    D[c] = -dv;
+}
+
+// SubtractPressureGradient (flip.cpp:568-680, subcell = 1) + EnforceBoundaryVelocity
+// (particlegridoperations.inl:45-104) per face.
+__global__ void k_gradient(Params p, const float* __restrict__ P, const float* __restrict__ L,
+                           const uint8_t* __restrict__ A, float* __restrict__ ux,
+                           float* __restrict__ uy, float* __restrict__ uz) {
+    const int N = p.N;
+    const int64_t nf = (int64_t)(N + 1) * N * N;
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 3 * nf) return;
+    const int comp = (int)(g / nf);
+    const int64_t f = g - comp * nf;
+    int i, j, k;
+    decode_face(N, comp, f, i, j, k);
+    float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
+    const int a = comp == 0 ? i : (comp == 1 ? j : k);
+    if (a == 0 || a == N) {
+        u[f] = 0.0f;
+        return;
+    }
+    const float h = 1.0f / p.maxd;
+    const int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
+    const int64_t cf = cidx(N, i, j, k), cb = cidx(N, bi, bj, bk);
+    float pf = P[cf], pb = P[cb];
+    const float lf = L[cf], lb = L[cb];
+    if (lf * lb < 0.0f) {
+        if (lf < 0.0f) pf = P[cf];
+        else pf = lf / gmin(1.0e-3f, lb) * P[cb];
+        if (lb < 0.0f) pb = P[cb];
+        else pb = lb / gmin(1.0e-6f, lf) * P[cf];
+    }
+    float val = u[f];
+    val -= (pf - pb) / h;
+    const bool s0 = A[cf] == kSolid, s1 = A[cb] == kSolid;
+    u[f] = (s0 != s1) ? 0.0f : val;
+}
+
+// ExtrapolateVelocity (flip.cpp:452-566) + SubtractPreviousGrid (365-410). A face
+// that touches no FLUID cell but has SOLID (or out-of-domain) cells on both sides
+// takes the mean of its same-component neighbours that touch FLUID. Only such
+// faces are written and only fluid-touching faces are read, so the update is in
+// place without a hazard. Then du = u - prev for every face.
+__device__ __forceinline__ bool face_mark(const uint8_t* A, int N, int comp, int i, int j, int k) {
+    const int a = comp == 0 ? i : (comp == 1 ? j : k);
+    const int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
+    return (a > 0 && A[cidx(N, bi, bj, bk)] == kFluid) || (a < N && A[cidx(N, i, j, k)] == kFluid);
+}
+__device__ __forceinline__ bool face_wallmark(const uint8_t* A, int N, int comp, int i, int j, int k) {
+    const int a = comp == 0 ? i : (comp == 1 ? j : k);
+    const int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
+    return (a <= 0 || A[cidx(N, bi, bj, bk)] == kSolid) && (a >= N || A[cidx(N, i, j, k)] == kSolid);
+}
+
+__global__ void k_extrapolate(Params p, const uint8_t* __restrict__ A, float* ux, float* uy,
+                              float* uz, float* dux, float* duy, float* duz) {
+    const int N = p.N;
+    const int64_t nf = (int64_t)(N + 1) * N * N;
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 3 * nf) return;
+    const int comp = (int)(g / nf);
+    const int64_t f = g - comp * nf;
+    int i, j, k;
+    decode_face(N, comp, f, i, j, k);
+    float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
+    float* du = comp == 0 ? dux : (comp == 1 ? duy : duz);
+    float val = u[f];
+    if (!face_mark(A, N, comp, i, j, k) && face_wallmark(A, N, comp, i, j, k)) {
+        unsigned wsum = 0;
+        float sum = 0.0f;
+        const int q[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                             {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+        const int ex = N + (comp == 0), ey = N + (comp == 1), ez = N + (comp == 2);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const int a = q[m][0], b = q[m][1], c = q[m][2];
+            if (a >= 0 && a < ex && b >= 0 && b < ey && c >= 0 && c < ez) {
+                if (face_mark(A, N, comp, a, b, c)) {
+                    wsum++;
+                    sum += u[fidx(N, comp, a, b, c)];
+                }
+            }
+        }
+        if (wsum) {
+            val = sum / (float)wsum;
+            u[f] = val;
+        }
+    }
+    du[f] = val - du[f];
+}
+
+// Post-projection diagnostics: max |div u| over FLUID cells and the FLUID count.
+__global__ void k_grid_stats(Params p, const float* __restrict__ ux, const float* __restrict__ uy,
+                             const float* __restrict__ uz, const uint8_t* __restrict__ A,
+                             SolverScalars* sc) {
+    const int N = p.N;
+    const int64_t nc = (int64_t)N * N * N;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float mx = 0.0f;
+    int cnt = 0;
+    if (c < nc && A[c] == kFluid) {
+        const int k = (int)(c % N), j = (int)((c / N) % N), i = (int)(c / ((int64_t)N * N));
+        const float h = 1.0f / p.maxd;
+        float dv = (ux[fidx(N, 0, i + 1, j, k)] - ux[fidx(N, 0, i, j, k)] +
+                    uy[fidx(N, 1, i, j + 1, k)] - uy[fidx(N, 1, i, j, k)] +
+                    uz[fidx(N, 2, i, j, k + 1)] - uz[fidx(N, 2, i, j, k)]) / h;
+        mx = fabsf(dv);
+        cnt = 1;
+    }
+    // wave reduction, then one atomic per wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = fmaxf(mx, __shfl_down(mx, off, 64));
+        cnt += __shfl_down(cnt, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (cnt) atomicAdd(&sc->fluid_cells, cnt);
+        if (mx > 0.0f) atomicMax(&sc->maxdiv_bits, __float_as_uint(mx));
+    }
+}
+
+__global__ void k_stats_reset(SolverScalars* sc) {
+    if (threadIdx.x == 0) {
+        sc->fluid_cells = 0;
+        sc->maxdiv_bits = 0u;
+    }
+}
+
+}  // namespace
+
+void launch_divergence(hipStream_t s, const Params& p, float* const u[3], float* D) {
+    int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_divergence, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, u[0], u[1],
+                       u[2], D);
+}
+
+void launch_gradient(hipStream_t s, const Params& p, const float* P, const float* L,
+                     const uint8_t* A, float* const u[3]) {
+    int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
+    hipLaunchKernelGGL(k_gradient, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), 0, s, p, P, L, A,
+                       u[0], u[1], u[2]);
+}
+
+void launch_extrapolate(hipStream_t s, const Params& p, const uint8_t* A, float* const u[3],
+                        float* const du[3]) {
+    int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
+    hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), 0, s, p, A,
+                       u[0], u[1], u[2], du[0], du[1], du[2]);
+}
+
+void launch_grid_stats(hipStream_t s, const Params& p, float* const u[3], const uint8_t* A,
+                       SolverScalars* sc) {
+    int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_stats_reset, dim3(1), dim3(64), 0, s, sc);
+    hipLaunchKernelGGL(k_grid_stats, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, u[0], u[1],
+                       u[2], A, sc);
+}
+
+}  // namespace ariel

@@ -1,0 +1,551 @@
+// kernels_particles.hip — particle-side kernels of the FLIP step for gfx950.
+//
+// Particles live sorted by (cell, id) in float4 pairs (pos.xyz + id bits,
+// vel.xyz + density). A cell's particles are contiguous and cells are laid out
+// z-fastest, so every neighbour query of the reference — 3x3x3 cells visited
+// sx -> sy -> sz, each cell's list in particle-vector order
+// (ParticleGrid::GetCellNeighbors, particlegrid.cpp:28-50) — becomes 9 (or 8)
+// contiguous index ranges, walked in that same order. Sums are therefore formed
+// in the reference's order and agree bit for bit with a serial evaluation.
+#include "ariel_device.h"
+#include "ariel_internal.h"
+
+namespace ariel {
+
+namespace {
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// ------------------------------------------------------------------ sort
+// ParticleGrid::Sort (particlegrid.cpp:140-172) as a counting sort:
+// bin (histogram) -> exclusive scan -> scatter -> per-cell order by id -> gather.
+__global__ void k_bin(const float4* __restrict__ pos, int64_t np, int N, float maxd,
+                      int32_t* __restrict__ cell_of, int32_t* __restrict__ counts) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    float4 p = pos[s];
+    int cx = cell_coord(p.x, maxd), cy = cell_coord(p.y, maxd), cz = cell_coord(p.z, maxd);
+    int32_t c = (int32_t)cidx(N, cx, cy, cz);
+    cell_of[s] = c;
+    atomicAdd(&counts[c], 1);
+}
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ int32_t block_exclusive_scan(int32_t v, int32_t* total) {
+    // 256 threads = 4 waves
+    __shared__ int32_t wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int32_t before = 0, tot = 0;
+    for (int i = 0; i < kScanThreads / 64; ++i) {
+        if (i < w) before += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return before + x - v;
+}
+
+__global__ void k_scan_reduce(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ bsum) {
+    int64_t base = (int64_t)blockIdx.x * kScanTile;
+    int32_t acc = 0;
+    for (int t = 0; t < kScanItems; ++t) {
+        int64_t i = base + (int64_t)t * kScanThreads + threadIdx.x;
+        if (i < n) acc += in[i];
+    }
+    int32_t tot;
+    block_exclusive_scan(acc, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Single block: exclusive scan of nb block sums in place, bsum[nb] = total.
+__global__ void k_scan_bsums(int32_t* __restrict__ bsum, int nb) {
+    const int per = (nb + kScanThreads - 1) / kScanThreads;
+    const int lo = threadIdx.x * per;
+    int32_t acc = 0;
+    for (int i = 0; i < per; ++i)
+        if (lo + i < nb) acc += bsum[lo + i];
+    int32_t tot;
+    int32_t ex = block_exclusive_scan(acc, &tot);
+    for (int i = 0; i < per; ++i) {
+        if (lo + i < nb) {
+            int32_t v = bsum[lo + i];
+            bsum[lo + i] = ex;
+            ex += v;
+        }
+    }
+    if (threadIdx.x == 0) bsum[nb] = tot;
+}
+
+__global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const int32_t* __restrict__ bsum,
+                             int32_t* __restrict__ out, int nb) {
+    int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int32_t acc = 0;
+#pragma unroll
+    for (int t = 0; t < kScanItems; ++t) {
+        int64_t i = base + t;
+        v[t] = (i < n) ? in[i] : 0;
+        acc += v[t];
+    }
+    int32_t tot;
+    int32_t ex = block_exclusive_scan(acc, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int t = 0; t < kScanItems; ++t) {
+        int64_t i = base + t;
+        if (i < n) out[i] = ex;
+        ex += v[t];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+__global__ void k_scatter(const int32_t* __restrict__ cell_of, int64_t np,
+                          const int32_t* __restrict__ offsets, int32_t* __restrict__ counts,
+                          int32_t* __restrict__ slots) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    int32_t c = cell_of[s];
+    int32_t r = atomicSub(&counts[c], 1) - 1;   // leaves counts at 0 for the next sort
+    slots[offsets[c] + r] = (int32_t)s;
+}
+
+// Per-cell insertion sort of the scattered slots by particle id: restores the
+// reference's per-cell order (push_back in particle-vector order).
+__global__ void k_cellsort(const int32_t* __restrict__ offsets, int64_t nc,
+                           const float4* __restrict__ pos, int32_t* __restrict__ slots) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int32_t o0 = offsets[c], o1 = offsets[c + 1];
+    for (int32_t a = o0 + 1; a < o1; ++a) {
+        int32_t sv = slots[a];
+        int32_t key = __float_as_int(pos[sv].w);
+        int32_t b = a - 1;
+        while (b >= o0) {
+            int32_t sb = slots[b];
+            if (__float_as_int(pos[sb].w) <= key) break;
+            slots[b + 1] = sb;
+            --b;
+        }
+        slots[b + 1] = sv;
+    }
+}
+
+__global__ void k_gather(const int32_t* __restrict__ slots, int64_t np, const float4* __restrict__ pin,
+                         const float4* __restrict__ vin, float4* __restrict__ pout,
+                         float4* __restrict__ vout) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    int32_t src = slots[s];
+    pout[s] = pin[src];
+    vout[s] = vin[src];
+}
+
+// ---------------------------------------------------------- ComputeDensity
+// flip.cpp:697-733 + ApplyExternalForces (flip.cpp:682-695) fused: each particle
+// gathers Smooth weights (h = 4*density/maxd) over the 3x3x3 cells around its
+// cell; the 3 cells along z of each (sx,sy) column are one contiguous range.
+__global__ void k_density(Params p, const float4* __restrict__ pos, float4* __restrict__ vel,
+                          int64_t np, const int32_t* __restrict__ offsets, int apply_forces) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const int N = p.N;
+    const float maxd = p.maxd;
+    const float4 me = pos[s];
+    const int cx = cell_coord(me.x, maxd), cy = cell_coord(me.y, maxd), cz = cell_coord(me.z, maxd);
+    const float hsm = 4.0f * p.density / maxd;
+    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, N - 1);
+    float weightsum = 0.0f;
+    for (int sx = cx - 1; sx <= cx + 1; ++sx) {
+        if (sx < 0 || sx > N - 1) continue;
+        for (int sy = cy - 1; sy <= cy + 1; ++sy) {
+            if (sy < 0 || sy > N - 1) continue;
+            const int64_t col = cidx(N, sx, sy, 0);
+            const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
+            for (int32_t q = q0; q < q1; ++q) {
+                const float4 o = pos[q];
+                float sqd = sqrlength(o.x, o.y, o.z, me.x, me.y, me.z);
+                float weight = 1.0f * smooth(sqd, hsm);
+                weightsum = weightsum + weight;
+            }
+        }
+    }
+    float4 v = vel[s];
+    v.w = weightsum / p.max_density;
+    if (apply_forces) {
+        for (int f = 0; f < p.n_forces; ++f) {
+            v.x = v.x + p.forces[f][0] * p.dt;
+            v.y = v.y + p.forces[f][1] * p.dt;
+            v.z = v.z + p.forces[f][2] * p.dt;
+        }
+    }
+    vel[s] = v;
+}
+
+// ------------------------------------------------- MarkCellTypes + BuildSDF
+// CellSDF (particlegrid.cpp:75-90): L = 0.2/density^3 - sum of the cell's
+// densities in list order; A = SOLID on the static walls, else FLUID iff L < 0
+// (MarkCellTypes, particlegrid.cpp:108-138). BuildSDF (92-106) stores L.
+__global__ void k_mark(Params p, const float4* __restrict__ vel, const int32_t* __restrict__ offsets,
+                       uint8_t* __restrict__ A, float* __restrict__ L) {
+    const int N = p.N;
+    const int64_t nc = (int64_t)N * N * N;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int k = (int)(c % N), j = (int)((c / N) % N), i = (int)(c / ((int64_t)N * N));
+    float accm = 0.0f;
+    const int32_t o0 = offsets[c], o1 = offsets[c + 1];
+    for (int32_t a = o0; a < o1; ++a) accm += vel[a].w;
+    const float dd = p.density;
+    const float n0 = 1.0f / (dd * dd * dd);
+    const float lc = 0.2f * n0 - accm;
+    const int w = p.wall;
+    const bool wall = w > 0 && (i < w || j < w || k < w || i >= N - w || j >= N - w || k >= N - w);
+    uint8_t a = kAir;
+    if (wall) a = kSolid;
+    else a = (lc < 0.0f) ? kFluid : kAir;
+    A[c] = a;
+    L[c] = lc;
+}
+
+// -------------------------------------------------- SplatParticlesToMACGrid
+// particlegridoperations.inl:146-249: each face gathers the Sharpen-weighted
+// (RE = 1.4 cells), mass-weighted mean of particle velocities over the cells of
+// GetWallNeighbors' asymmetric window (particlegrid.cpp:52-73): 2 cells along the
+// face normal, 4 along the others. Fused: StorePreviousGrid (flip.cpp:324-363,
+// copy before the boundary pass) and EnforceBoundaryVelocity
+// (particlegridoperations.inl:45-104).
+__global__ void k_p2g(Params p, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                      const int32_t* __restrict__ offsets, const uint8_t* __restrict__ A,
+                      float* __restrict__ ux, float* __restrict__ uy, float* __restrict__ uz,
+                      float* __restrict__ dux, float* __restrict__ duy, float* __restrict__ duz) {
+    const int N = p.N;
+    const int64_t nf = (int64_t)(N + 1) * N * N;
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 3 * nf) return;
+    const int comp = (int)(g / nf);
+    const int64_t f = g - comp * nf;
+    int i, j, k;
+    if (comp == 0) { k = (int)(f % N); j = (int)((f / N) % N); i = (int)(f / ((int64_t)N * N)); }
+    else if (comp == 1) { k = (int)(f % N); j = (int)((f / N) % (N + 1)); i = (int)(f / ((int64_t)N * (N + 1))); }
+    else { k = (int)(f % (N + 1)); j = (int)((f / (N + 1)) % N); i = (int)(f / ((int64_t)N * (N + 1))); }
+    const float maxd = (float)N;
+    const float RE = p.sharpen_re;
+    const bool tri = p.splat == 1;
+    float fx = (float)i, fy = (float)j, fz = (float)k;
+    int xlo, xhi, ylo, yhi, zlo, zhi;
+    if (comp == 0) { fy += 0.5f; fz += 0.5f; xlo = i - 1; xhi = i; ylo = j - 2; yhi = j + 1; zlo = k - 2; zhi = k + 1; }
+    else if (comp == 1) { fx += 0.5f; fz += 0.5f; xlo = i - 2; xhi = i + 1; ylo = j - 1; yhi = j; zlo = k - 2; zhi = k + 1; }
+    else { fx += 0.5f; fy += 0.5f; xlo = i - 2; xhi = i + 1; ylo = j - 2; yhi = j + 1; zlo = k - 1; zhi = k; }
+    zlo = max(zlo, 0);
+    zhi = min(zhi, N - 1);
+    float sumw = 0.0f, sumc = 0.0f;
+    if (zlo <= zhi) {
+        for (int sx = max(xlo, 0); sx <= min(xhi, N - 1); ++sx) {
+            for (int sy = max(ylo, 0); sy <= min(yhi, N - 1); ++sy) {
+                const int64_t col = cidx(N, sx, sy, 0);
+                const int32_t q0 = offsets[col + zlo], q1 = offsets[col + zhi + 1];
+                for (int32_t q = q0; q < q1; ++q) {
+                    const float4 o = pos[q];
+                    float px = gmax(0.0f, gmin(maxd, maxd * o.x));
+                    float py = gmax(0.0f, gmin(maxd, maxd * o.y));
+                    float pz = gmax(0.0f, gmin(maxd, maxd * o.z));
+                    float w;
+                    if (tri) {
+                        w = (gmax(1.0f - fabsf(px - fx), 0.0f) * gmax(1.0f - fabsf(py - fy), 0.0f)) *
+                            gmax(1.0f - fabsf(pz - fz), 0.0f);
+                    } else {
+                        w = 1.0f * sharpen(sqrlength(px, py, pz, fx, fy, fz), RE);
+                    }
+                    const float4 v = vel[q];
+                    const float vc = comp == 0 ? v.x : (comp == 1 ? v.y : v.z);
+                    sumc += w * vc;
+                    sumw += w;
+                }
+            }
+        }
+    }
+    float val = 0.0f;
+    if (sumw > 0) val = sumc / sumw;
+    float* du = comp == 0 ? dux : (comp == 1 ? duy : duz);
+    float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
+    du[f] = val;
+    // boundary: domain faces, and faces between a SOLID and a non-SOLID cell
+    int a = comp == 0 ? i : (comp == 1 ? j : k);
+    bool zero = (a == 0 || a == N);
+    if (!zero) {
+        int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
+        bool s0 = A[cidx(N, i, j, k)] == kSolid, s1 = A[cidx(N, bi, bj, bk)] == kSolid;
+        zero = s0 != s1;
+    }
+    u[f] = zero ? 0.0f : val;
+}
+
+// ------------------------------------------- SolvePicFlip + AdvectParticles
+// flip.cpp:287-322 and 229-285: FLIP = u + I(du), PIC = I(u), blend
+// (1-r)*PIC + r*FLIP; p += dt * I(u)(p) (the PIC velocity at the same position);
+// wall clamp to [1/maxd, 1-1/maxd]. The re-sort inside AdvectParticles only
+// serves the solid-particle push, which is inert without SOLID particles.
+__global__ void k_g2p_advect(Params p, float4* __restrict__ pos, float4* __restrict__ vel,
+                             int64_t np, float* ux, float* uy, float* uz, float* dux, float* duy,
+                             float* duz) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const int N = p.N;
+    float* g[3] = {ux, uy, uz};
+    float* d[3] = {dux, duy, duz};
+    float4 me = pos[s];
+    float4 v = vel[s];
+    float dx, dy, dz, px, py, pz;
+    interpolate_velocity(d, N, me.x, me.y, me.z, dx, dy, dz);
+    float tx = dx + v.x, ty = dy + v.y, tz = dz + v.z;
+    interpolate_velocity(g, N, me.x, me.y, me.z, px, py, pz);
+    const float r = p.picflip;
+    v.x = (1.0f - r) * px + r * tx;
+    v.y = (1.0f - r) * py + r * ty;
+    v.z = (1.0f - r) * pz + r * tz;
+    me.x = me.x + p.dt * px;
+    me.y = me.y + p.dt * py;
+    me.z = me.z + p.dt * pz;
+    const float rr = 1.0f / p.maxd;
+    me.x = gmax(rr, gmin(1.0f - rr, me.x));
+    me.y = gmax(rr, gmin(1.0f - rr, me.y));
+    me.z = gmax(rr, gmin(1.0f - rr, me.z));
+    pos[s] = me;
+    vel[s] = v;
+}
+
+// ------------------------------------------------------- ResampleParticles
+// particleresampler.inl:35-148. Neighbour cells come from GetCellNeighbors with
+// a float index: loop variables start at (int)(x - 1) and run while
+// (float)s <= x + 1, skipping cells outside [0, N-1].
+__device__ __forceinline__ void cell_range(float x, int N, int& lo, int& hi) {
+    lo = (int)(x - 1.0f);
+    hi = lo;
+    while ((float)(hi + 1) <= x + 1.0f) ++hi;
+    if (lo < 0) lo = 0;
+    if (hi > N - 1) hi = N - 1;
+}
+
+__global__ void k_resample(Params p, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                           int64_t np, const int32_t* __restrict__ offsets, float4* __restrict__ pout,
+                           float4* __restrict__ vout, uint8_t* __restrict__ accept) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const int N = p.N;
+    const float maxd = p.maxd;
+    const float re = p.density / maxd;
+    const float dt = p.dt;
+    const float springforce = p.spring;
+    const float4 me = pos[s];
+    const float4 mv = vel[s];
+    const uint32_t my_id = (uint32_t)__float_as_int(me.w);
+    const uint32_t fr = (uint32_t)p.frame;
+    float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
+    {
+        float x = gmax(0.0f, gmin(maxd, maxd * me.x));
+        float y = gmax(0.0f, gmin(maxd, maxd * me.y));
+        float z = gmax(0.0f, gmin(maxd, maxd * me.z));
+        int xl, xh, yl, yh, zl, zh;
+        cell_range(x, N, xl, xh);
+        cell_range(y, N, yl, yh);
+        cell_range(z, N, zl, zh);
+        for (int cx = xl; cx <= xh; ++cx)
+            for (int cy = yl; cy <= yh; ++cy) {
+                if (zl > zh) continue;
+                const int64_t col = cidx(N, cx, cy, 0);
+                const int32_t q0 = offsets[col + zl], q1 = offsets[col + zh + 1];
+                for (int32_t q = q0; q < q1; ++q) {
+                    if (q == (int32_t)s) continue;
+                    const float4 o = pos[q];
+                    float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+                    float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                    float w = springforce * 1.0f * smooth(dist * dist, re);
+                    if (dist > 0.1f * re) {
+                        sx_ += w * (me.x - o.x) / dist * re;
+                        sy_ += w * (me.y - o.y) / dist * re;
+                        sz_ += w * (me.z - o.z) / dist * re;
+                    } else {
+                        const uint32_t qid = (uint32_t)__float_as_int(o.w);
+                        uint32_t hx = hash4(p.seed, 3u * fr + 0u, my_id, qid);
+                        uint32_t hy = hash4(p.seed, 3u * fr + 1u, my_id, qid);
+                        uint32_t hz = hash4(p.seed, 3u * fr + 2u, my_id, qid);
+                        sx_ += 0.01f * re / dt * (float)(int)(hx % 101u) / 100.0f;
+                        sy_ += 0.01f * re / dt * (float)(int)(hy % 101u) / 100.0f;
+                        sz_ += 0.01f * re / dt * (float)(int)(hz % 101u) / 100.0f;
+                    }
+                }
+            }
+    }
+    const float tx = me.x + dt * sx_, ty = me.y + dt * sy_, tz = me.z + dt * sz_;
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f, wsum = 0.0f;
+    {
+        float x = gmax(0.0f, gmin(maxd - 1, maxd * tx));
+        float y = gmax(0.0f, gmin(maxd - 1, maxd * ty));
+        float z = gmax(0.0f, gmin(maxd - 1, maxd * tz));
+        int xl, xh, yl, yh, zl, zh;
+        cell_range(x, N, xl, xh);
+        cell_range(y, N, yl, yh);
+        cell_range(z, N, zl, zh);
+        for (int cx = xl; cx <= xh; ++cx)
+            for (int cy = yl; cy <= yh; ++cy) {
+                if (zl > zh) continue;
+                const int64_t col = cidx(N, cx, cy, 0);
+                const int32_t q0 = offsets[col + zl], q1 = offsets[col + zh + 1];
+                for (int32_t q = q0; q < q1; ++q) {
+                    const float4 o = pos[q];
+                    float dist2 = sqrlength(tx, ty, tz, o.x, o.y, o.z);
+                    float w = 1.0f * sharpen(dist2, re);
+                    const float4 ov = vel[q];
+                    rx = rx + w * ov.x;
+                    ry = ry + w * ov.y;
+                    rz = rz + w * ov.z;
+                    wsum += w;
+                }
+            }
+    }
+    if (wsum != 0.0f) {
+        rx = rx / wsum; ry = ry / wsum; rz = rz / wsum;
+    } else {
+        rx = mv.x; ry = mv.y; rz = mv.z;
+    }
+    // accept unless t lies inside a solid (the static walls)
+    bool ok = true;
+    if (p.wall > 0) {
+        const float lo = (float)p.wall, hi = (float)(N - p.wall);
+        float gx = tx * maxd, gy = ty * maxd, gz = tz * maxd;
+        ok = !(gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi);
+    }
+    accept[s] = ok ? 1 : 0;
+    if (ok) {
+        pout[s] = make_float4(tx, ty, tz, me.w);
+        vout[s] = make_float4(rx, ry, rz, mv.w);
+    } else {
+        pout[s] = me;
+        vout[s] = mv;
+    }
+}
+
+// Scatter sorted particles back to particle-vector (id) order for the accessors.
+__global__ void k_unsort(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                         const uint8_t* __restrict__ accept, int64_t np, float4* __restrict__ pout,
+                         float4* __restrict__ vout, uint8_t* __restrict__ aout,
+                         int32_t* __restrict__ ids_out) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    float4 pp = pos[s];
+    int32_t id = __float_as_int(pp.w);
+    if (pout) pout[id] = pp;
+    if (vout) vout[id] = vel[s];
+    if (aout && accept) aout[id] = accept[s];
+    if (ids_out) ids_out[s] = id;
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void launch_bin(hipStream_t s, const float4* pos, int64_t np, int N, float maxd, int32_t* cell_of,
+                int32_t* counts) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_bin, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, pos, np, N, maxd,
+                       cell_of, counts);
+}
+
+void launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, int32_t* block_sums) {
+    int nb = (int)((n + kScanTile - 1) / kScanTile);
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, n, block_sums);
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(kScanThreads), 0, s, block_sums, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, n, block_sums, out, nb);
+}
+
+void launch_scatter(hipStream_t s, const int32_t* cell_of, int64_t np, const int32_t* offsets,
+                    int32_t* counts, int32_t* slots) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, cell_of, np,
+                       offsets, counts, slots);
+}
+
+void launch_cellsort(hipStream_t s, const int32_t* offsets, int64_t nc, const float4* pos,
+                     int32_t* slots) {
+    hipLaunchKernelGGL(k_cellsort, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, offsets, nc, pos,
+                       slots);
+}
+
+void launch_gather(hipStream_t s, const int32_t* slots, int64_t np, const float4* pos_in,
+                   const float4* vel_in, float4* pos_out, float4* vel_out) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, slots, np, pos_in,
+                       vel_in, pos_out, vel_out);
+}
+
+void launch_density(hipStream_t s, const Params& p, const float4* pos, float4* vel, int64_t np,
+                    const int32_t* offsets, int apply_forces) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_density, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, p, pos, vel, np,
+                       offsets, apply_forces);
+}
+
+void launch_mark(hipStream_t s, const Params& p, const float4* vel, const int32_t* offsets,
+                 uint8_t* A, float* L) {
+    int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_mark, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, vel, offsets, A, L);
+}
+
+void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
+                const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3]) {
+    int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
+    hipLaunchKernelGGL(k_p2g, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), 0, s, p, pos, vel,
+                       offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]);
+}
+
+void launch_g2p_advect(hipStream_t s, const Params& p, float4* pos, float4* vel, int64_t np,
+                       float* const u[3], float* const du[3]) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_g2p_advect, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, p, pos, vel,
+                       np, u[0], u[1], u[2], du[0], du[1], du[2]);
+}
+
+void launch_resample(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
+                     int64_t np, const int32_t* offsets, float4* pos_out, float4* vel_out,
+                     uint8_t* accept) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_resample, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, p, pos, vel, np,
+                       offsets, pos_out, vel_out, accept);
+}
+
+void launch_unsort(hipStream_t s, const float4* pos, const float4* vel, const uint8_t* accept,
+                   int64_t np, float4* pos_out, float4* vel_out, uint8_t* accept_out,
+                   int32_t* ids_out) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_unsort, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, pos, vel, accept,
+                       np, pos_out, vel_out, accept_out, ids_out);
+}
+
+void launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, p, n, v);
+}
+void launch_fill_f32(hipStream_t s, float* p, int64_t n, float v) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, p, n, v);
+}
+
+}  // namespace ariel

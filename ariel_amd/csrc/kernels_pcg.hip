@@ -1,0 +1,871 @@
+// kernels_pcg.hip — pressure solve for gfx950: matrix-free 7-point operator,
+// FP32 vectors with FP64 reductions, and MIC(0) under a tiled red-black ordering.
+//
+// PCG state never leaves the device inside an iteration: every reduction writes
+// per-block FP64 partials; the block that arrives last (atomic counter) sums them
+// in a fixed order and updates the scalars (alpha, beta, residual, stop flag) in
+// SolverScalars. Later kernels read those scalars and return early once `done`
+// is set, so the host can queue iterations in chunks without synchronising.
+//
+// MIC(0) (Bridson, "Fluid Simulation for Computer Graphics", ch. 5; the
+// reference's BuildPreconditioner/ApplyPreconditioner, solver.inl:104-286) is
+// factored under an ordering where 8^3 tiles are coloured red/black by
+// (ti+tj+tk)&1, red tiles precede black ones and cells inside a tile follow the
+// natural order. A black tile's cross-tile neighbours are all red (lower), a red
+// tile's all black (upper). One 64-lane wave owns a tile, lane = (il,jl) owns the
+// z-line of 8 cells, and the in-tile triangular solves run as a 22-step
+// wavefront over i+j+k through LDS.
+#include "ariel_device.h"
+#include "ariel_internal.h"
+
+namespace ariel {
+
+namespace {
+
+constexpr int kRedBlocks = 1024;   // cap of grid-stride reduction grids
+constexpr int kTile = 8;
+constexpr int kPad = 9;            // LDS stride per z-line (bank-conflict padding)
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// ------------------------------------------------------------ reductions
+// Block-reduce NV doubles; thread 0 stores the block partials at part[slot*NV+v],
+// fences, and bumps the counter. Returns true (block-uniform) in the block that
+// arrives last among `expected` arrivals.
+template <int NV>
+__device__ bool block_arrive(double (&v)[NV], double* part, int slot, uint32_t* counter,
+                             int expected) {
+    __shared__ double sm[NV][16];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double t = wave_sum(v[i]);
+        if (lane == 0) sm[i][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double t = 0.0;
+            for (int ww = 0; ww < nw; ++ww) t += sm[i][ww];
+            part[slot * NV + i] = t;
+        }
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        is_last = (prev == (unsigned)(expected - 1));
+    }
+    __syncthreads();
+    return is_last != 0;
+}
+
+// In the last block: sum part[0..n) per value in a fixed order (deterministic).
+template <int NV>
+__device__ void sum_partials(const double* part, int n, double (&out)[NV], uint32_t* counter) {
+    __threadfence();
+    __shared__ double sm[NV][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (int b = threadIdx.x; b < n; b += blockDim.x) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            acc[i] += __hip_atomic_load(&part[b * NV + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double t = wave_sum(acc[i]);
+        if (lane == 0) sm[i][w] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double t = 0.0;
+        for (int ww = 0; ww < nw; ++ww) t += sm[i][ww];
+        out[i] = t;
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+}
+
+__device__ __forceinline__ bool finite(double x) { return isfinite(x); }
+
+// ------------------------------------------------------------ operator
+// ComputeAx + XRef (solver.inl:138-154, 196-230): y_c = (6 x_c - sum XRef_q) / h,
+// h = 1/(n*n); XRef = x_q (FLUID), x_c (SOLID or clamped out of range),
+// L_q / min(1e-6, L_c) * x_c (AIR); terms in the order +x,-x,+y,-y,+z,-z.
+__device__ __forceinline__ float xref(const uint8_t* __restrict__ A, const float* __restrict__ L,
+                                      const float* __restrict__ x, int N, int pi, int pj, int pk,
+                                      float xc, int64_t c) {
+    const int a = min(max(0, pi), N - 1), b = min(max(0, pj), N - 1), cc = min(max(0, pk), N - 1);
+    const int64_t q = cidx(N, a, b, cc);
+    const uint8_t t = A[q];
+    if (t == kFluid) return x[q];
+    if (t == kSolid) return xc;
+    return L[q] / gmin(1.0e-6f, L[c]) * xc;
+}
+
+__device__ __forceinline__ float spmv_cell(const uint8_t* __restrict__ A, const float* __restrict__ L,
+                                           const float* __restrict__ x, int N, int i, int j, int k,
+                                           int64_t c, float h) {
+    const float xc = x[c];
+    return (6.0f * xc - xref(A, L, x, N, i + 1, j, k, xc, c) - xref(A, L, x, N, i - 1, j, k, xc, c) -
+            xref(A, L, x, N, i, j + 1, k, xc, c) - xref(A, L, x, N, i, j - 1, k, xc, c) -
+            xref(A, L, x, N, i, j, k + 1, xc, c) - xref(A, L, x, N, i, j, k - 1, xc, c)) / h;
+}
+
+// The same operator in FP64 from FP32 inputs (true residual).
+__device__ __forceinline__ double spmv_cell_f64(const uint8_t* __restrict__ A,
+                                                const float* __restrict__ L,
+                                                const float* __restrict__ x, int N, int i, int j,
+                                                int k, int64_t c) {
+    const double xc = x[c];
+    double acc = 6.0 * xc;
+    const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k},
+                         {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const int a = min(max(0, q[m][0]), N - 1), b = min(max(0, q[m][1]), N - 1),
+                  cc = min(max(0, q[m][2]), N - 1);
+        const int64_t n = cidx(N, a, b, cc);
+        const uint8_t t = A[n];
+        if (t == kFluid) acc -= (double)x[n];
+        else if (t == kSolid) acc -= xc;
+        else acc -= (double)L[n] / (double)gmin(1.0e-6f, L[c]) * xc;
+    }
+    return (double)N * (double)N * acc;
+}
+
+__device__ __forceinline__ void decode_cell(int N, int64_t c, int& i, int& j, int& k) {
+    k = (int)(c % N);
+    j = (int)((c / N) % N);
+    i = (int)(c / ((int64_t)N * N));
+}
+
+// ------------------------------------------------------------ solver kernels
+__global__ void k_solver_init(SolverScalars* sc, int max_iters, int refine_every) {
+    if (threadIdx.x != 0) return;
+    sc->bnorm2 = sc->rnorm2 = sc->thr = sc->rho = sc->sigma = 0.0;
+    sc->alpha = sc->beta = sc->tmp = 0.0;
+    sc->done = 0;
+    sc->it = 0;
+    sc->status = 1;
+    sc->max_iters = max_iters;
+    sc->refine_every = refine_every;
+    sc->fluid_cells = 0;
+    sc->maxdiv_bits = 0u;
+}
+
+// Warm start: P := 0 off FLUID (what the reference's first Op does), r = b - A p,
+// ||b||^2 and ||r||^2 over FLUID cells; stop threshold; early exit if converged.
+__global__ void k_residual0(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
+                            float* __restrict__ P, const float* __restrict__ D, float* __restrict__ R,
+                            Reduce red, double tol, int stop_rule) {
+    const int N = p.N;
+    const int64_t nc = (int64_t)N * N * N;
+    const float n = (float)N;
+    const float h = 1.0f / (n * n);
+    double v[2] = {0.0, 0.0};
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (A[c] != kFluid) {
+            P[c] = 0.0f;
+            R[c] = 0.0f;
+            continue;
+        }
+        int i, j, k;
+        decode_cell(N, c, i, j, k);
+        const float ax = spmv_cell(A, L, P, N, i, j, k, c, h);
+        const float d = D[c];
+        const float r = d + (-1.0f) * ax;
+        R[c] = r;
+        v[0] += (double)d * (double)d;
+        v[1] += (double)r * (double)r;
+    }
+    if (block_arrive<2>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
+        double s[2];
+        sum_partials<2>(red.partials, gridDim.x, s, red.counter);
+        if (threadIdx.x == 0) {
+            SolverScalars* sc = red.sc;
+            sc->bnorm2 = s[0];
+            sc->rnorm2 = s[1];
+            if (stop_rule == 1) sc->thr = (double)(1.0e-2f * (float)(N * N * N));
+            else sc->thr = tol * tol * s[0];
+            if (s[1] <= sc->thr) {
+                sc->done = 1;
+                sc->status = 0;
+            }
+        }
+    }
+}
+
+// z = A s, sigma = z.s, alpha = rho / sigma.
+__global__ void k_spmv_dot(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
+                           const float* __restrict__ x, float* __restrict__ y, Reduce red) {
+    if (red.sc->done) return;
+    const int N = p.N;
+    const int64_t nc = (int64_t)N * N * N;
+    const float n = (float)N;
+    const float h = 1.0f / (n * n);
+    double v[1] = {0.0};
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (A[c] != kFluid) {
+            y[c] = 0.0f;
+            continue;
+        }
+        int i, j, k;
+        decode_cell(N, c, i, j, k);
+        const float yc = spmv_cell(A, L, x, N, i, j, k, c, h);
+        y[c] = yc;
+        v[0] += (double)yc * (double)x[c];
+    }
+    if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
+        double s[1];
+        sum_partials<1>(red.partials, gridDim.x, s, red.counter);
+        if (threadIdx.x == 0) {
+            SolverScalars* sc = red.sc;
+            sc->sigma = s[0];
+            if (!finite(s[0]) || !finite(sc->rho)) {
+                sc->status = 3;
+                sc->done = 1;
+            } else if (!(s[0] > 0.0)) {
+                sc->status = 2;
+                sc->done = 1;
+            } else {
+                sc->alpha = sc->rho / s[0];
+            }
+        }
+    }
+}
+
+// Stop decision after the residual norm of iteration `it` is known.
+__device__ __forceinline__ void decide(SolverScalars* sc) {
+    const double r2 = sc->rnorm2;
+    if (!finite(r2)) {
+        sc->status = 3;
+        sc->done = 1;
+    } else if (r2 <= sc->thr) {
+        sc->status = 0;
+        sc->done = 1;
+    } else if (sc->it >= sc->max_iters) {
+        sc->status = 1;
+        sc->done = 1;
+    }
+}
+__device__ __forceinline__ bool refine_due(const SolverScalars* sc) {
+    return sc->refine_every > 0 && (sc->it % sc->refine_every) == 0 && sc->rnorm2 > 100.0 * sc->thr;
+}
+
+// p += alpha s; r += (-alpha) z (Op, solver.inl:157-177); ||r||^2; stop test.
+__global__ void k_axpy2(int64_t n, float* __restrict__ P, const float* __restrict__ S,
+                        float* __restrict__ R, const float* __restrict__ Z, Reduce red) {
+    if (red.sc->done) return;
+    const float af = (float)red.sc->alpha;
+    double v[1] = {0.0};
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        P[c] = P[c] + af * S[c];
+        const float r = R[c] + (-af) * Z[c];
+        R[c] = r;
+        v[0] += (double)r * (double)r;
+    }
+    if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
+        double s[1];
+        sum_partials<1>(red.partials, gridDim.x, s, red.counter);
+        if (threadIdx.x == 0) {
+            SolverScalars* sc = red.sc;
+            sc->rnorm2 = s[0];
+            sc->it += 1;
+            if (!refine_due(sc)) decide(sc);
+        }
+    }
+}
+
+// Residual replacement: r = b - A p evaluated in FP64, stored in FP32.
+__global__ void k_true_residual(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
+                                const float* __restrict__ P, const float* __restrict__ D,
+                                float* __restrict__ R, Reduce red, int force) {
+    if (red.sc->done && !force) return;
+    if (!force && !refine_due(red.sc)) return;
+    const int N = p.N;
+    const int64_t nc = (int64_t)N * N * N;
+    double v[1] = {0.0};
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (A[c] != kFluid) {
+            if (!force) R[c] = 0.0f;
+            continue;
+        }
+        int i, j, k;
+        decode_cell(N, c, i, j, k);
+        const double r = (double)D[c] - spmv_cell_f64(A, L, P, N, i, j, k, c);
+        v[0] += r * r;
+        if (!force) R[c] = (float)r;
+    }
+    if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
+        double s[1];
+        sum_partials<1>(red.partials, gridDim.x, s, red.counter);
+        if (threadIdx.x == 0) {
+            SolverScalars* sc = red.sc;
+            if (force) {
+                sc->tmp = s[0];
+            } else {
+                sc->rnorm2 = s[0];
+                decide(sc);
+            }
+        }
+    }
+}
+
+__global__ void k_update_s(int64_t n, const float* __restrict__ Z, float* __restrict__ S,
+                           const SolverScalars* sc, int first) {
+    if (sc->done) return;
+    const float bf = (float)sc->beta;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        S[c] = first ? Z[c] : Z[c] + bf * S[c];
+    }
+}
+
+__global__ void k_spmv_plain(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
+                             const float* __restrict__ x, float* __restrict__ y) {
+    const int N = p.N;
+    const int64_t nc = (int64_t)N * N * N;
+    const float n = (float)N;
+    const float h = 1.0f / (n * n);
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    if (A[c] != kFluid) {
+        y[c] = 0.0f;
+        return;
+    }
+    int i, j, k;
+    decode_cell(N, c, i, j, k);
+    y[c] = spmv_cell(A, L, x, N, i, j, k, c, h);
+}
+
+// rho_new = z.r finalisation shared by the preconditioner kernels.
+__device__ __forceinline__ void finish_rho(SolverScalars* sc, double rho_new) {
+    if (sc->it == 0) {
+        sc->rho = rho_new;
+    } else {
+        sc->beta = rho_new / sc->rho;
+        sc->rho = rho_new;
+    }
+}
+
+// ------------------------------------------------------------ preconditioner
+// ADiag (solver.inl:81-101) and the number of FLUID upper neighbours under the
+// ordering (mode 0: tiled red-black, 4: block/no coupling); Jacobi inverse diag.
+__global__ void k_precond_setup(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
+                                float* __restrict__ diag, uint8_t* __restrict__ nup,
+                                float* __restrict__ pc) {
+    const int N = p.N;
+    const int64_t nc = (int64_t)N * N * N;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int i, j, k;
+    decode_cell(N, c, i, j, k);
+    if (A[c] != kFluid) {
+        diag[c] = 6.0f;
+        nup[c] = 0;
+        pc[c] = 0.0f;
+        return;
+    }
+    float dg = 6.0f;
+    const int q[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                         {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+    int cnt = 0;
+    const int color = ((i >> 3) + (j >> 3) + (k >> 3)) & 1;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const int a = q[m][0], b = q[m][1], cc = q[m][2];
+        const bool in = a >= 0 && a < N && b >= 0 && b < N && cc >= 0 && cc < N;
+        const uint8_t t = in ? A[cidx(N, a, b, cc)] : (uint8_t)kSolid;
+        if (!in || t == kSolid) {
+            dg -= 1.0f;
+        } else if (t == kAir) {
+            dg -= L[cidx(N, a, b, cc)] / gmin(1.0e-6f, L[c]);
+        }
+        if (in && t == kFluid) {
+            const bool same = (a >> 3) == (i >> 3) && (b >> 3) == (j >> 3) && (cc >> 3) == (k >> 3);
+            const bool plus = (m & 1) != 0;
+            if (same) cnt += plus ? 1 : 0;
+            else if (p.precond == 0 && color == 0) cnt += 1;
+        }
+    }
+    diag[c] = dg;
+    nup[c] = (uint8_t)cnt;
+    pc[c] = (p.precond == 2 && dg > 0) ? 1.0f / dg : 0.0f;
+}
+
+struct TileCtx {
+    int N, TN;
+    int ti, tj, tk;
+    int il, jl, lane;
+    int gi, gj, k0;
+    int64_t line;   // cell index of (gi, gj, k0)
+};
+
+__device__ __forceinline__ bool tile_of(int N, int64_t t, int color, TileCtx& c) {
+    const int TN = N / kTile;
+    c.N = N;
+    c.TN = TN;
+    c.tk = (int)(t % TN);
+    c.tj = (int)((t / TN) % TN);
+    c.ti = (int)(t / ((int64_t)TN * TN));
+    if (color >= 0 && ((c.ti + c.tj + c.tk) & 1) != color) return false;
+    c.lane = threadIdx.x & 63;
+    c.il = c.lane >> 3;
+    c.jl = c.lane & 7;
+    c.gi = c.ti * kTile + c.il;
+    c.gj = c.tj * kTile + c.jl;
+    c.k0 = c.tk * kTile;
+    c.line = cidx(N, c.gi, c.gj, c.k0);
+    return true;
+}
+
+__device__ __forceinline__ uint32_t fluid_bits(const uint8_t* __restrict__ A, int64_t line) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int kl = 0; kl < kTile; ++kl) m |= (A[line + kl] == kFluid ? 1u : 0u) << kl;
+    return m;
+}
+
+// Load a neighbouring line (8 cells) into LDS slot `dst` (stride kPad) as v = a*b
+// (b may be null -> v = a) together with its fluid bits.
+__device__ __forceinline__ uint32_t load_halo_line(const uint8_t* __restrict__ A,
+                                                   const float* __restrict__ a,
+                                                   const float* __restrict__ b, int64_t line,
+                                                   float* dst) {
+    uint32_t m = fluid_bits(A, line);
+#pragma unroll
+    for (int kl = 0; kl < kTile; ++kl) {
+        const float x = a[line + kl];
+        dst[kl] = b ? x * b[line + kl] : x;
+    }
+    return m;
+}
+
+// MIC(0) factor for the tiles of one colour (mode 0) or all tiles (mode 4).
+// pivot: e = diag - sum_j pc_j^2 - tau * sum_j (nup_j - 1) pc_j^2 over FLUID lower
+// neighbours j in the order -x,-y,-z,+x,+y,+z; e = diag if e < sigma*diag;
+// pc = 1/sqrt(e).
+__global__ __launch_bounds__(64) void k_mic_build(Params p, const uint8_t* __restrict__ A,
+                                                   const float* __restrict__ diag,
+                                                   const uint8_t* __restrict__ nup,
+                                                   float* __restrict__ pc, int color) {
+    __shared__ float s_pc[64 * kPad];
+    __shared__ float s_nm[64 * kPad];
+    __shared__ float s_hxp[64 * kPad];   // cross-tile x neighbour line: pc
+    __shared__ float s_hxn[64 * kPad];   //                              nup-1
+    __shared__ float s_hyp[64 * kPad];
+    __shared__ float s_hyn[64 * kPad];
+    const int N = p.N;
+    const int TN = N / kTile;
+    const int64_t ntiles = (int64_t)TN * TN * TN;
+    const bool black = color == 1;
+    const float tau = p.mic_tau, sigma = p.mic_sigma;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        TileCtx c;
+        if (!tile_of(N, t, color, c)) continue;
+        const uint32_t fl = fluid_bits(A, c.line);
+        const uint32_t fl_im = __shfl(fl, (c.lane - 8) & 63, 64);
+        const uint32_t fl_jm = __shfl(fl, (c.lane - 1) & 63, 64);
+        float dg[kTile];
+#pragma unroll
+        for (int kl = 0; kl < kTile; ++kl) dg[kl] = diag[c.line + kl];
+        // halos (black tiles): x line (il==0: -x, il==7: +x), y line, z singles
+        uint32_t hx_fl = 0, hy_fl = 0, hzm_fl = 0, hzp_fl = 0;
+        float hzm_pc = 0.f, hzm_nm = 0.f, hzp_pc = 0.f, hzp_nm = 0.f;
+        if (black) {
+            int xi = -1;
+            if (c.il == 0 && c.gi > 0) xi = c.gi - 1;
+            if (c.il == 7 && c.gi + 1 < N) xi = c.gi + 1;
+            if (xi >= 0) {
+                const int64_t ln = cidx(N, xi, c.gj, c.k0);
+                hx_fl = fluid_bits(A, ln);
+#pragma unroll
+                for (int kl = 0; kl < kTile; ++kl) {
+                    s_hxp[c.lane * kPad + kl] = pc[ln + kl];
+                    s_hxn[c.lane * kPad + kl] = (float)((int)nup[ln + kl] - 1);
+                }
+            }
+            int yj = -1;
+            if (c.jl == 0 && c.gj > 0) yj = c.gj - 1;
+            if (c.jl == 7 && c.gj + 1 < N) yj = c.gj + 1;
+            if (yj >= 0) {
+                const int64_t ln = cidx(N, c.gi, yj, c.k0);
+                hy_fl = fluid_bits(A, ln);
+#pragma unroll
+                for (int kl = 0; kl < kTile; ++kl) {
+                    s_hyp[c.lane * kPad + kl] = pc[ln + kl];
+                    s_hyn[c.lane * kPad + kl] = (float)((int)nup[ln + kl] - 1);
+                }
+            }
+            if (c.k0 > 0) {
+                const int64_t n = c.line - 1;
+                hzm_fl = A[n] == kFluid;
+                hzm_pc = pc[n];
+                hzm_nm = (float)((int)nup[n] - 1);
+            }
+            if (c.k0 + kTile < N) {
+                const int64_t n = c.line + kTile;
+                hzp_fl = A[n] == kFluid;
+                hzp_pc = pc[n];
+                hzp_nm = (float)((int)nup[n] - 1);
+            }
+        }
+#pragma unroll
+        for (int kl = 0; kl < kTile; ++kl) s_nm[c.lane * kPad + kl] = (float)((int)nup[c.line + kl] - 1);
+        __syncthreads();
+        for (int st = 0; st < 3 * kTile - 2; ++st) {
+            const int kl = st - c.il - c.jl;
+            if (kl >= 0 && kl < kTile) {
+                float res = 0.0f;
+                if ((fl >> kl) & 1u) {
+                    // gather the lower neighbours' (pc, nup-1) in order -x,-y,-z,+x,+y,+z
+                    float pj[6], nm[6];
+                    bool has[6];
+                    // -x
+                    has[0] = false;
+                    if (c.il > 0) {
+                        if ((fl_im >> kl) & 1u) { has[0] = true; pj[0] = s_pc[(c.lane - 8) * kPad + kl]; nm[0] = s_nm[(c.lane - 8) * kPad + kl]; }
+                    } else if (black && ((hx_fl >> kl) & 1u)) {
+                        has[0] = true; pj[0] = s_hxp[c.lane * kPad + kl]; nm[0] = s_hxn[c.lane * kPad + kl];
+                    }
+                    // -y
+                    has[1] = false;
+                    if (c.jl > 0) {
+                        if ((fl_jm >> kl) & 1u) { has[1] = true; pj[1] = s_pc[(c.lane - 1) * kPad + kl]; nm[1] = s_nm[(c.lane - 1) * kPad + kl]; }
+                    } else if (black && ((hy_fl >> kl) & 1u)) {
+                        has[1] = true; pj[1] = s_hyp[c.lane * kPad + kl]; nm[1] = s_hyn[c.lane * kPad + kl];
+                    }
+                    // -z
+                    has[2] = false;
+                    if (kl > 0) {
+                        if ((fl >> (kl - 1)) & 1u) { has[2] = true; pj[2] = s_pc[c.lane * kPad + kl - 1]; nm[2] = s_nm[c.lane * kPad + kl - 1]; }
+                    } else if (black && hzm_fl) {
+                        has[2] = true; pj[2] = hzm_pc; nm[2] = hzm_nm;
+                    }
+                    // +x, +y, +z (cross-tile, black only)
+                    has[3] = black && c.il == 7 && ((hx_fl >> kl) & 1u);
+                    if (has[3]) { pj[3] = s_hxp[c.lane * kPad + kl]; nm[3] = s_hxn[c.lane * kPad + kl]; }
+                    has[4] = black && c.jl == 7 && ((hy_fl >> kl) & 1u);
+                    if (has[4]) { pj[4] = s_hyp[c.lane * kPad + kl]; nm[4] = s_hyn[c.lane * kPad + kl]; }
+                    has[5] = black && kl == 7 && hzp_fl;
+                    if (has[5]) { pj[5] = hzp_pc; nm[5] = hzp_nm; }
+                    const float d = dg[kl];
+                    float e = d;
+#pragma unroll
+                    for (int m = 0; m < 6; ++m)
+                        if (has[m]) e = e - pj[m] * pj[m];
+                    float ssum = 0.0f;
+#pragma unroll
+                    for (int m = 0; m < 6; ++m)
+                        if (has[m]) ssum = ssum + nm[m] * (pj[m] * pj[m]);
+                    e = e - tau * ssum;
+                    if (d > 0) {
+                        if (e < sigma * d) e = d;
+                        res = 1.0f / sqrtf(e);
+                    }
+                }
+                s_pc[c.lane * kPad + kl] = res;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int kl = 0; kl < kTile; ++kl) pc[c.line + kl] = s_pc[c.lane * kPad + kl];
+        __syncthreads();
+    }
+}
+
+// MIC(0) apply, one pass over the tiles of one colour.
+//   phase 0 (red, forward):          q = L^-1 r inside the tile          -> Q
+//   phase 1 (black, fwd + bwd):      forward with red halos, backward     -> Z, z.r
+//   phase 2 (red, backward):         backward with black halos            -> Z, z.r (+finalise)
+//   phase 3 (block, fwd + bwd):      no cross-tile coupling               -> Z, z.r (+finalise)
+// forward:  q_i = (r_i + sum_{j lower} pc_j q_j) pc_i   (order -x,-y,-z,+x,+y,+z)
+// backward: z_i = (q_i + sum_{k upper} pc_i z_k) pc_i   (order +x,+y,+z,-x,-y,-z)
+__global__ __launch_bounds__(64) void k_mic_apply(Params p, const uint8_t* __restrict__ A,
+                                                   const float* __restrict__ pc,
+                                                   const float* __restrict__ r,
+                                                   float* __restrict__ Q, float* __restrict__ Z,
+                                                   Reduce red, int phase, int slot_base,
+                                                   int finalize_total) {
+    if (red.sc->done) return;
+    __shared__ float s_a[64 * kPad];    // r, then q
+    __shared__ float s_pc[64 * kPad];
+    __shared__ float s_b[64 * kPad];    // pc*q (forward), then z (backward)
+    __shared__ float s_hx[64 * kPad];
+    __shared__ float s_hy[64 * kPad];
+    const int N = p.N;
+    const int TN = N / kTile;
+    const int64_t ntiles = (int64_t)TN * TN * TN;
+    const int color = phase == 3 ? -1 : (phase == 1 ? 1 : 0);
+    const bool do_fwd = phase != 2;
+    const bool do_bwd = phase != 0;
+    const bool halo_fwd = phase == 1;   // black forward sees red neighbours
+    const bool halo_bwd = phase == 2;   // red backward sees black neighbours
+    double dot[1] = {0.0};
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        TileCtx c;
+        if (!tile_of(N, t, color, c)) continue;
+        const uint32_t fl = fluid_bits(A, c.line);
+        const uint32_t fl_im = __shfl(fl, (c.lane - 8) & 63, 64);
+        const uint32_t fl_jm = __shfl(fl, (c.lane - 1) & 63, 64);
+        const uint32_t fl_ip = __shfl(fl, (c.lane + 8) & 63, 64);
+        const uint32_t fl_jp = __shfl(fl, (c.lane + 1) & 63, 64);
+        float* sa = s_a + c.lane * kPad;
+        float* spc = s_pc + c.lane * kPad;
+        float* sb = s_b + c.lane * kPad;
+#pragma unroll
+        for (int kl = 0; kl < kTile; ++kl) {
+            spc[kl] = pc[c.line + kl];
+            sa[kl] = do_fwd ? r[c.line + kl] : Q[c.line + kl];
+        }
+        // cross-tile halos
+        uint32_t hx_fl = 0, hy_fl = 0, hzm_fl = 0, hzp_fl = 0;
+        float hzm = 0.f, hzp = 0.f;
+        if (halo_fwd || halo_bwd) {
+            // forward halo values are pc*Q of red neighbours; backward ones are Z of black
+            const float* va = halo_fwd ? pc : Z;
+            const float* vb = halo_fwd ? Q : nullptr;
+            int xi = -1;
+            if (c.il == 0 && c.gi > 0) xi = c.gi - 1;
+            if (c.il == 7 && c.gi + 1 < N) xi = c.gi + 1;
+            if (xi >= 0) hx_fl = load_halo_line(A, va, vb, cidx(N, xi, c.gj, c.k0), s_hx + c.lane * kPad);
+            int yj = -1;
+            if (c.jl == 0 && c.gj > 0) yj = c.gj - 1;
+            if (c.jl == 7 && c.gj + 1 < N) yj = c.gj + 1;
+            if (yj >= 0) hy_fl = load_halo_line(A, va, vb, cidx(N, c.gi, yj, c.k0), s_hy + c.lane * kPad);
+            if (c.k0 > 0) {
+                const int64_t n = c.line - 1;
+                hzm_fl = A[n] == kFluid;
+                hzm = vb ? va[n] * vb[n] : va[n];
+            }
+            if (c.k0 + kTile < N) {
+                const int64_t n = c.line + kTile;
+                hzp_fl = A[n] == kFluid;
+                hzp = vb ? va[n] * vb[n] : va[n];
+            }
+        }
+        __syncthreads();
+        if (do_fwd) {
+            for (int st = 0; st < 3 * kTile - 2; ++st) {
+                const int kl = st - c.il - c.jl;
+                if (kl >= 0 && kl < kTile) {
+                    float q = 0.0f;
+                    if ((fl >> kl) & 1u) {
+                        float acc = sa[kl];
+                        if (c.il > 0) {
+                            if ((fl_im >> kl) & 1u) acc = acc + s_b[(c.lane - 8) * kPad + kl];
+                        } else if (halo_fwd && ((hx_fl >> kl) & 1u)) {
+                            acc = acc + s_hx[c.lane * kPad + kl];
+                        }
+                        if (c.jl > 0) {
+                            if ((fl_jm >> kl) & 1u) acc = acc + s_b[(c.lane - 1) * kPad + kl];
+                        } else if (halo_fwd && ((hy_fl >> kl) & 1u)) {
+                            acc = acc + s_hy[c.lane * kPad + kl];
+                        }
+                        if (kl > 0) {
+                            if ((fl >> (kl - 1)) & 1u) acc = acc + sb[kl - 1];
+                        } else if (halo_fwd && hzm_fl) {
+                            acc = acc + hzm;
+                        }
+                        if (halo_fwd && c.il == 7 && ((hx_fl >> kl) & 1u)) acc = acc + s_hx[c.lane * kPad + kl];
+                        if (halo_fwd && c.jl == 7 && ((hy_fl >> kl) & 1u)) acc = acc + s_hy[c.lane * kPad + kl];
+                        if (halo_fwd && kl == 7 && hzp_fl) acc = acc + hzp;
+                        q = acc * spc[kl];
+                    }
+                    sa[kl] = q;
+                    sb[kl] = spc[kl] * q;
+                }
+                __syncthreads();
+            }
+            if (!do_bwd) {
+#pragma unroll
+                for (int kl = 0; kl < kTile; ++kl) Q[c.line + kl] = sa[kl];
+            }
+        }
+        if (do_bwd) {
+            for (int st = 0; st < 3 * kTile - 2; ++st) {
+                const int m = st - (7 - c.il) - (7 - c.jl);
+                const int kl = 7 - m;
+                if (m >= 0 && m < kTile) {
+                    float z = 0.0f;
+                    if ((fl >> kl) & 1u) {
+                        const float pci = spc[kl];
+                        float acc = sa[kl];
+                        // +x
+                        if (c.il < 7) {
+                            if ((fl_ip >> kl) & 1u) acc = acc + pci * s_b[(c.lane + 8) * kPad + kl];
+                        } else if (halo_bwd && ((hx_fl >> kl) & 1u)) {
+                            acc = acc + pci * s_hx[c.lane * kPad + kl];
+                        }
+                        // +y
+                        if (c.jl < 7) {
+                            if ((fl_jp >> kl) & 1u) acc = acc + pci * s_b[(c.lane + 1) * kPad + kl];
+                        } else if (halo_bwd && ((hy_fl >> kl) & 1u)) {
+                            acc = acc + pci * s_hy[c.lane * kPad + kl];
+                        }
+                        // +z
+                        if (kl < 7) {
+                            if ((fl >> (kl + 1)) & 1u) acc = acc + pci * sb[kl + 1];
+                        } else if (halo_bwd && hzp_fl) {
+                            acc = acc + pci * hzp;
+                        }
+                        // -x, -y, -z across the tile boundary (red backward only)
+                        if (halo_bwd && c.il == 0 && ((hx_fl >> kl) & 1u)) acc = acc + pci * s_hx[c.lane * kPad + kl];
+                        if (halo_bwd && c.jl == 0 && ((hy_fl >> kl) & 1u)) acc = acc + pci * s_hy[c.lane * kPad + kl];
+                        if (halo_bwd && kl == 0 && hzm_fl) acc = acc + pci * hzm;
+                        z = acc * pci;
+                    }
+                    sb[kl] = z;
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int kl = 0; kl < kTile; ++kl) {
+                const float z = sb[kl];
+                Z[c.line + kl] = z;
+                dot[0] += (double)z * (double)r[c.line + kl];
+            }
+        }
+        __syncthreads();
+    }
+    if (phase == 0) return;
+    if (phase == 1) {
+        // partials only; the red backward pass finalises
+        double t = wave_sum(dot[0]);
+        if (threadIdx.x == 0) red.partials[slot_base + blockIdx.x] = t;
+        return;
+    }
+    if (block_arrive<1>(dot, red.partials, slot_base + blockIdx.x, red.counter, gridDim.x)) {
+        double s[1];
+        sum_partials<1>(red.partials, finalize_total, s, red.counter);
+        if (threadIdx.x == 0) finish_rho(red.sc, s[0]);
+    }
+}
+
+// Jacobi: z = r * (1/diag), rho = z.r
+__global__ void k_jacobi_apply(int64_t n, const float* __restrict__ pc, const float* __restrict__ r,
+                               float* __restrict__ Z, Reduce red) {
+    if (red.sc->done) return;
+    double v[1] = {0.0};
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const float z = r[c] * pc[c];
+        Z[c] = z;
+        v[0] += (double)z * (double)r[c];
+    }
+    if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
+        double s[1];
+        sum_partials<1>(red.partials, gridDim.x, s, red.counter);
+        if (threadIdx.x == 0) finish_rho(red.sc, s[0]);
+    }
+}
+
+int stream_blocks(int64_t n) {
+    int64_t b = (n + kBlock - 1) / kBlock;
+    return (int)(b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks);
+}
+
+int tile_blocks(int N) {
+    const int64_t TN = N / kTile;
+    const int64_t nt = TN * TN * TN;
+    return (int)(nt < 4096 ? nt : 4096);
+}
+
+}  // namespace
+
+int reduce_blocks(int64_t n, int) { return stream_blocks(n); }
+
+void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every) {
+    hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, s, sc, max_iters, refine_every);
+}
+
+void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                          float* diag, uint8_t* nup, float* pc) {
+    int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_precond_setup, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, A, L,
+                       diag, nup, pc);
+}
+
+void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
+                          const uint8_t* nup, float* pc) {
+    if (p.precond == 2) return;   // Jacobi: built by setup
+    const int nb = tile_blocks(p.N);
+    if (p.precond == 0) {
+        hipLaunchKernelGGL(k_mic_build, dim3(nb), dim3(64), 0, s, p, A, diag, nup, pc, 0);
+        hipLaunchKernelGGL(k_mic_build, dim3(nb), dim3(64), 0, s, p, A, diag, nup, pc, 1);
+    } else {
+        hipLaunchKernelGGL(k_mic_build, dim3(nb), dim3(64), 0, s, p, A, diag, nup, pc, -1);
+    }
+}
+
+void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* A, const float* pc,
+                          const float* r, float* q, float* z, const Reduce& red, int) {
+    const int64_t nc = (int64_t)p.N * p.N * p.N;
+    if (p.precond == 2) {
+        hipLaunchKernelGGL(k_jacobi_apply, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, nc, pc, r,
+                           z, red);
+        return;
+    }
+    const int nb = tile_blocks(p.N);
+    if (p.precond == 0) {
+        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(64), 0, s, p, A, pc, r, q, z, red, 0, 0, 0);
+        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(64), 0, s, p, A, pc, r, q, z, red, 1, 0, 0);
+        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(64), 0, s, p, A, pc, r, q, z, red, 2, nb,
+                           2 * nb);
+    } else {
+        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(64), 0, s, p, A, pc, r, q, z, red, 3, 0, nb);
+    }
+}
+
+void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L, float* P,
+                      const float* D, float* R, const Reduce& red, double tol, int stop_rule) {
+    const int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_residual0, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P, D, R,
+                       red, tol, stop_rule);
+}
+
+void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                     const float* x, float* y, const Reduce& red) {
+    const int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_spmv_dot, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, x, y, red);
+}
+
+void launch_axpy2(hipStream_t s, int64_t n, float* P, const float* S, float* R, const float* Z,
+                  const Reduce& red) {
+    hipLaunchKernelGGL(k_axpy2, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, P, S, R, Z, red);
+}
+
+void launch_true_residual(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                          const float* P, const float* D, float* R, const Reduce& red) {
+    const int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_true_residual, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P, D,
+                       R, red, 0);
+}
+
+void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                               const float* P, const float* D, const Reduce& red) {
+    const int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_true_residual, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P, D,
+                       (float*)nullptr, red, 1);
+}
+
+void launch_update_s(hipStream_t s, int64_t n, const float* Z, float* S, const Reduce& red,
+                     int first) {
+    hipLaunchKernelGGL(k_update_s, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, Z, S, red.sc,
+                       first);
+}
+
+void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                       const float* x, float* y) {
+    const int64_t nc = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_spmv_plain, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, A, L, x, y);
+}
+
+}  // namespace ariel

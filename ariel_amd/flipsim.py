@@ -1,0 +1,181 @@
+"""Python mirror of fluidCore::FlipSim (src/sim/flip.hpp:20-34) over the C ABI.
+
+``FlipSim(desc)`` creates the device state (FlipSim::FlipSim, flip.cpp:15-30);
+``Init()``/``Step()`` follow FlipSim::Init/Step; ``GetParticles()``,
+``GetDimensions()`` and ``m_frame`` are the reference accessors. The MAC-grid,
+flag and sort accessors have no reference counterpart (m_mgrid is private).
+Errors raise :class:`ArielError` carrying the ABI status and message.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import SceneDesc, StepStats
+
+
+class ArielError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ariel error {code}: {msg}")
+        self.code = code
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class FlipSim:
+    def __init__(self, desc):
+        self._lib = _abi.load()
+        self._h = ctypes.c_void_p()
+        self.desc = desc
+        rc = self._lib.ariel_create(ctypes.byref(desc), ctypes.byref(self._h))
+        if rc != 0:
+            raise ArielError(rc, self._lib.ariel_last_error(None).decode())
+        self.n = desc.dims[0]
+
+    # ------------------------------------------------------------ lifecycle
+    def _check(self, rc):
+        if rc != 0:
+            raise ArielError(rc, self._lib.ariel_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._lib.ariel_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------- reference interface
+    def Init(self):
+        self._check(self._lib.ariel_init(self._h))
+
+    def Step(self, nsteps=1):
+        self._check(self._lib.ariel_step(self._h, nsteps))
+
+    def GetParticles(self):
+        """(pos[n,3], vel[n,3], density[n]) in particle-vector (id) order."""
+        n = self.num_particles
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        rho = np.empty(n, np.float32)
+        cnt = ctypes.c_int64()
+        self._check(self._lib.ariel_get_particles(self._h, _p(pos), _p(vel), _p(rho), n,
+                                                  ctypes.byref(cnt)))
+        return pos, vel, rho
+
+    def GetDimensions(self):
+        d = (ctypes.c_int32 * 3)()
+        self._check(self._lib.ariel_get_dimensions(self._h, d))
+        return tuple(d)
+
+    @property
+    def m_frame(self):
+        f = ctypes.c_int32()
+        self._check(self._lib.ariel_get_frame(self._h, ctypes.byref(f)))
+        return f.value
+
+    # ------------------------------------------------------------ extras
+    init = Init
+    step = Step
+
+    @property
+    def num_particles(self):
+        n = ctypes.c_int64()
+        self._check(self._lib.ariel_particle_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def run_stage(self, stage):
+        self._check(self._lib.ariel_run_stage(self._h, stage))
+
+    def sync(self):
+        self._check(self._lib.ariel_sync(self._h))
+
+    def set_particles(self, pos, vel=None, density=None):
+        pos = np.ascontiguousarray(pos, np.float32)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float32)
+        d = None if density is None else np.ascontiguousarray(density, np.float32)
+        self._check(self._lib.ariel_set_particles(self._h, _p(pos), None if v is None else _p(v),
+                                                  None if d is None else _p(d), len(pos)))
+
+    def field_size(self, which):
+        n = self.n
+        return (n + 1) * n * n if which <= _abi.FIELD_DUZ else n ** 3
+
+    def get_field(self, which):
+        out = np.empty(self.field_size(which), np.float32)
+        self._check(self._lib.ariel_get_macgrid(self._h, which, _p(out), out.size))
+        return out
+
+    def set_field(self, which, arr):
+        arr = np.ascontiguousarray(arr, np.float32).ravel()
+        self._check(self._lib.ariel_set_macgrid(self._h, which, _p(arr), arr.size))
+
+    def get_flags(self):
+        out = np.empty(self.n ** 3, np.uint8)
+        self._check(self._lib.ariel_get_flags(self._h, _p(out), out.size))
+        return out
+
+    def set_flags(self, flags):
+        flags = np.ascontiguousarray(flags, np.uint8).ravel()
+        self._check(self._lib.ariel_set_flags(self._h, _p(flags), flags.size))
+
+    def get_sort(self):
+        ids = np.empty(self.num_particles, np.int32)
+        offs = np.empty(self.n ** 3 + 1, np.int32)
+        self._check(self._lib.ariel_get_sort(self._h, _p(ids), ids.size, _p(offs), offs.size))
+        return ids, offs
+
+    def get_accept(self):
+        out = np.empty(self.num_particles, np.uint8)
+        self._check(self._lib.ariel_get_resample_accept(self._h, _p(out), out.size))
+        return out
+
+    @property
+    def max_density(self):
+        v = ctypes.c_float()
+        self._check(self._lib.ariel_get_max_density(self._h, ctypes.byref(v)))
+        return v.value
+
+    @max_density.setter
+    def max_density(self, v):
+        self._check(self._lib.ariel_set_max_density(self._h, float(v)))
+
+    def stats(self):
+        st = StepStats()
+        self._check(self._lib.ariel_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def apply_laplacian(self, x):
+        x = np.ascontiguousarray(x, np.float32).ravel()
+        y = np.empty_like(x)
+        self._check(self._lib.ariel_apply_laplacian(self._h, _p(x), _p(y)))
+        return y
+
+    def apply_preconditioner(self, r):
+        r = np.ascontiguousarray(r, np.float32).ravel()
+        z = np.empty_like(r)
+        self._check(self._lib.ariel_apply_preconditioner(self._h, _p(r), _p(z)))
+        return z
+
+    def save_state(self, path):
+        self._check(self._lib.ariel_save_state(self._h, str(path).encode()))
+
+    def load_state(self, path):
+        self._check(self._lib.ariel_load_state(self._h, str(path).encode()))
+
+    def export_levelset(self, path):
+        self._check(self._lib.ariel_export_levelset(self._h, str(path).encode()))
+
+
+__all__ = ["FlipSim", "ArielError", "SceneDesc"]

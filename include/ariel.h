@@ -1,0 +1,256 @@
+/*
+ * ariel.h — C ABI of the MI355X-native FLIP step.
+ *
+ * This is the drop-in boundary for Ariel's `fluidCore::FlipSim`
+ * (reference: src/sim/flip.hpp:20-34). Every entry point below names the
+ * reference interface it replaces. The library owns all device memory of a
+ * simulation handle; host buffers are caller-owned and copied. Calls on one
+ * handle are serialised on that handle's HIP stream; distinct handles may be
+ * driven from distinct threads.
+ *
+ * Every function returns an int status: ARIEL_OK (0) or a negative
+ * ARIEL_ERR_* code; ariel_last_error() returns the message of the last failure
+ * on that handle (or the last handle-less failure when h is NULL).
+ *
+ * Units follow the reference: particle positions live in the unit box [0,1]^3
+ * (grid coordinates are maxd * p), velocities in unit-box lengths per second.
+ * Grids are cubic, N = dims[0] = dims[1] = dims[2]; storage is x-slowest,
+ * z-fastest (Grid<T>::m_rawgrid[x][y][z], src/grid/grid.inl:11-62):
+ *   cell (i,j,k)   -> (i*N + j)*N + k                       (N^3 values)
+ *   x-face (i,j,k) -> (i*N + j)*N + k,     i in [0,N]       ((N+1)N^2 values)
+ *   y-face (i,j,k) -> (i*(N+1) + j)*N + k, j in [0,N]       ((N+1)N^2 values)
+ *   z-face (i,j,k) -> (i*N + j)*(N+1) + k, k in [0,N]       ((N+1)N^2 values)
+ */
+#ifndef ARIEL_H
+#define ARIEL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ARIEL_OK               0
+#define ARIEL_ERR_INVALID     -1   /* bad argument or descriptor            */
+#define ARIEL_ERR_HIP         -2   /* HIP runtime error                     */
+#define ARIEL_ERR_NOMEM       -3   /* device allocation failed              */
+#define ARIEL_ERR_STATE       -4   /* call out of order (e.g. step before init) */
+#define ARIEL_ERR_CAPACITY    -5   /* caller buffer too small               */
+#define ARIEL_ERR_UNSUPPORTED -6   /* feature outside this build            */
+#define ARIEL_ERR_IO          -7   /* file open/read/write failed           */
+
+/* ---- cell classes (src/grid/macgrid.inl:13) ---------------------------- */
+#define ARIEL_AIR   0
+#define ARIEL_FLUID 1
+#define ARIEL_SOLID 2
+
+/* ---- liquid primitives (replace JSON liquid geoms, sceneloader.cpp:153-171) */
+#define ARIEL_LIQUID_BOX    0      /* lo <= p < hi on every axis            */
+#define ARIEL_LIQUID_SPHERE 1      /* |p - center|^2 < radius^2             */
+
+/* ---- pressure preconditioners ------------------------------------------ */
+#define ARIEL_PRECOND_MIC0_TILED   0  /* MIC(0), tile red-black ordering       */
+#define ARIEL_PRECOND_MIC0         1  /* MIC(0), natural (lexicographic) order */
+#define ARIEL_PRECOND_JACOBI       2  /* diagonal                              */
+#define ARIEL_PRECOND_REFERENCE    3  /* solver.inl:104-135,232-286 verbatim   */
+#define ARIEL_PRECOND_MIC0_BLOCK   4  /* MIC(0) inside tiles, no coupling      */
+#define ARIEL_PRECOND_LEARNED      5  /* learned conv net inside flexible PCG  */
+
+/* ---- PCG stop rules ----------------------------------------------------- */
+#define ARIEL_STOP_RELATIVE      0   /* ||r||_2 <= tol * ||b||_2 (FP64 norms) */
+#define ARIEL_STOP_REFERENCE_ABS 1   /* r.r <= 1e-2 * N^3 (solver.inl:322,339) */
+
+/* ---- P2G kernels --------------------------------------------------------- */
+#define ARIEL_SPLAT_SHARPEN   0      /* reference (particlegridoperations.inl:146-249) */
+#define ARIEL_SPLAT_TRILINEAR 1      /* hat weights over the same window       */
+
+/* ---- grid fields for ariel_get_macgrid / ariel_set_macgrid -------------- */
+#define ARIEL_FIELD_UX   0   /* m_u_x                                   */
+#define ARIEL_FIELD_UY   1   /* m_u_y                                   */
+#define ARIEL_FIELD_UZ   2   /* m_u_z                                   */
+#define ARIEL_FIELD_DUX  3   /* m_mgrid_previous.m_u_x (prev, then du)  */
+#define ARIEL_FIELD_DUY  4
+#define ARIEL_FIELD_DUZ  5
+#define ARIEL_FIELD_P    6   /* pressure, warm start across steps       */
+#define ARIEL_FIELD_D    7   /* divergence, negated (solver RHS)         */
+#define ARIEL_FIELD_L    8   /* CellSDF level set                        */
+
+/* ---- stages of FlipSim::Step, for stage-wise parity -------------------- */
+#define ARIEL_STAGE_SORT        0  /* ParticleGrid::Sort                    */
+#define ARIEL_STAGE_DENSITY     1  /* ComputeDensity + ApplyExternalForces  */
+#define ARIEL_STAGE_MARK        2  /* MarkCellTypes + BuildSDF (A and L)    */
+#define ARIEL_STAGE_P2G         3  /* Splat + StorePreviousGrid + boundary  */
+#define ARIEL_STAGE_DIVERGENCE  4  /* Project divergence + FlipGrid         */
+#define ARIEL_STAGE_SOLVE       5  /* BuildPreconditioner + PCG             */
+#define ARIEL_STAGE_GRADIENT    6  /* SubtractPressureGradient + boundary   */
+#define ARIEL_STAGE_EXTRAPOLATE 7  /* ExtrapolateVelocity + SubtractPrevious*/
+#define ARIEL_STAGE_ADVECT      8  /* SolvePicFlip + AdvectParticles        */
+#define ARIEL_STAGE_RESAMPLE    9  /* ResampleParticles                     */
+#define ARIEL_NUM_STAGES       10
+
+/* ---- solver status ------------------------------------------------------- */
+#define ARIEL_SOLVE_CONVERGED  0
+#define ARIEL_SOLVE_MAX_ITERS  1
+#define ARIEL_SOLVE_BREAKDOWN  2   /* s.As <= 0                              */
+#define ARIEL_SOLVE_NONFINITE  3   /* NaN/Inf in a reduction                 */
+
+#define ARIEL_MAX_FORCES  4
+#define ARIEL_MAX_LIQUIDS 8
+
+typedef struct ariel_liquid {
+    int32_t kind;           /* ARIEL_LIQUID_*                               */
+    float   lo[3];          /* box corner (unit coordinates)                */
+    float   hi[3];          /* box corner                                   */
+    float   center[3];      /* sphere centre                                */
+    float   radius;         /* sphere radius                                */
+    float   velocity[3];    /* initial velocity (sceneloader.cpp:158-167)   */
+} ariel_liquid;
+
+/* Scene + solver description. ariel_desc_defaults() fills the reference's
+ * defaults (sceneloader.cpp:411-443, flip.cpp:15-30, solver.inl:107). */
+typedef struct ariel_scene_desc {
+    int32_t  dims[3];       /* grid edge per axis; must be equal            */
+    float    density;       /* lattice spacing in cells, 1/m for integer m  */
+    float    dt;            /* step size                                    */
+    float    picflip;       /* FLIP fraction of the PIC/FLIP blend          */
+    int32_t  n_forces;
+    float    forces[ARIEL_MAX_FORCES][3];
+    int32_t  n_liquids;
+    ariel_liquid liquids[ARIEL_MAX_LIQUIDS];
+    int32_t  wall;          /* static SOLID wall thickness in cells         */
+    uint32_t seed;          /* seed of the octant jitter and resampler RNG  */
+    int32_t  jitter;        /* 1: uniform jitter inside each lattice octant */
+    int32_t  precond;       /* ARIEL_PRECOND_*                              */
+    float    mic_tau;       /* MIC(0) modification (0.97)                   */
+    float    mic_sigma;     /* MIC(0) safety (0.25)                         */
+    int32_t  tile;          /* tile edge of the tiled preconditioners       */
+    float    tol;           /* relative residual target                     */
+    int32_t  max_iters;
+    int32_t  refine_every;  /* true-residual recompute period (0 = never)   */
+    int32_t  stop_rule;     /* ARIEL_STOP_*                                 */
+    int32_t  splat;         /* ARIEL_SPLAT_*                                */
+    int32_t  resample;      /* 1: run ResampleParticles each step           */
+    float    sharpen_re;    /* P2G Sharpen radius in cells (1.4)            */
+    float    spring;        /* resampler spring force (50)                  */
+    int32_t  reserved[8];
+} ariel_scene_desc;
+
+typedef struct ariel_step_stats {
+    int32_t frame;              /* FlipSim::m_frame                          */
+    int32_t iterations;         /* PCG iterations of the last solve          */
+    int32_t solve_status;       /* ARIEL_SOLVE_*                             */
+    int32_t fluid_cells;        /* cells classified FLUID                    */
+    int64_t particles;
+    double  rhs_norm;           /* ||b||_2 over fluid cells                  */
+    double  residual_norm;      /* recursive ||r||_2 at exit                 */
+    double  true_residual_norm; /* ||b - A p||_2 recomputed at exit          */
+    double  max_divergence;     /* max |div u| over fluid cells after projection */
+    double  max_density;        /* FlipSim::m_max_density                    */
+    double  stage_ms[ARIEL_NUM_STAGES]; /* device time of each stage, last step */
+    double  step_ms;            /* device time of the last step              */
+    double  solve_ms;           /* device time of the last pressure solve    */
+} ariel_step_stats;
+
+typedef struct ariel_sim ariel_sim;
+
+/* Fill d with the reference defaults: 32^3, density 0.5, dt 0.005
+ * (sceneloader.cpp:411-443), FLIP 0.95 (flip.cpp:27), Sharpen RE 1.4
+ * (particlegridoperations.inl:149), spring 50 (particleresampler.inl:42),
+ * MIC(0) tau 0.97 sigma 0.25, tol 1e-6, 500 iterations, no liquids. */
+int ariel_desc_defaults(ariel_scene_desc* d);
+
+/* FlipSim::FlipSim(maxres, density, stepsize, scene, verbose)
+ * (src/sim/flip.cpp:15-30): validates d and allocates device state. */
+int ariel_create(const ariel_scene_desc* d, ariel_sim** out);
+
+/* FlipSim::~FlipSim (src/sim/flip.cpp:32-40). NULL is accepted. */
+int ariel_destroy(ariel_sim* h);
+
+/* Message of the last failure on h (h may be NULL). Never NULL. */
+const char* ariel_last_error(const ariel_sim* h);
+
+/* FlipSim::Init (src/sim/flip.cpp:42-77): max-density calibration, seeding
+ * (Scene::GenerateParticles, src/scene/scene.cpp:257-360), sort, cell marks. */
+int ariel_init(ariel_sim* h);
+
+/* nsteps x FlipSim::Step (src/sim/flip.cpp:92-128), without export. */
+int ariel_step(ariel_sim* h, int32_t nsteps);
+
+/* One stage of Step on the current device state (ARIEL_STAGE_*). Stages after
+ * ARIEL_STAGE_SORT assume the particles are sorted. Used by parity tests. */
+int ariel_run_stage(ariel_sim* h, int32_t stage);
+
+/* Block until all work queued on h has finished. */
+int ariel_sync(ariel_sim* h);
+
+/* FlipSim::m_frame (src/sim/flip.hpp:33). */
+int ariel_get_frame(ariel_sim* h, int32_t* frame);
+
+/* FlipSim::GetDimensions (src/sim/flip.cpp:747-749). */
+int ariel_get_dimensions(ariel_sim* h, int32_t dims[3]);
+
+/* Number of particles (size of FlipSim::GetParticles()). */
+int ariel_particle_count(ariel_sim* h, int64_t* n);
+
+/* FlipSim::GetParticles (src/sim/flip.cpp:743-745): copies particles out in
+ * particle-vector order (particle id order). pos/vel are n*3 floats, density
+ * n floats; any pointer may be NULL. cap is the capacity in particles. */
+int ariel_get_particles(ariel_sim* h, float* pos, float* vel, float* density,
+                        int64_t cap, int64_t* n);
+
+/* Replace all particles (id = index). Used to load a state; the particles are
+ * unsorted afterwards, so the next stage to run must be ARIEL_STAGE_SORT. */
+int ariel_set_particles(ariel_sim* h, const float* pos, const float* vel,
+                        const float* density, int64_t n);
+
+/* MAC-grid accessor (no reference counterpart: m_mgrid is private,
+ * src/sim/flip.hpp:52). which = ARIEL_FIELD_*; cap in floats. */
+int ariel_get_macgrid(ariel_sim* h, int32_t which, float* dst, int64_t cap);
+int ariel_set_macgrid(ariel_sim* h, int32_t which, const float* src, int64_t n);
+
+/* Cell classes m_A (src/grid/macgrid.inl:21-34) as one byte per cell. */
+int ariel_get_flags(ariel_sim* h, uint8_t* dst, int64_t cap);
+int ariel_set_flags(ariel_sim* h, const uint8_t* src, int64_t n);
+
+/* Result of the last particle sort (ParticleGrid::Sort, particlegrid.cpp:140-172):
+ * ids[np] lists particle ids cell by cell, each cell in id order; offsets[N^3+1]
+ * is the exclusive prefix sum of per-cell counts. Either pointer may be NULL. */
+int ariel_get_sort(ariel_sim* h, int32_t* ids, int64_t ids_cap,
+                   int32_t* offsets, int64_t offsets_cap);
+
+/* Accept mask of the last ResampleParticles (particleresampler.inl:103-117),
+ * one byte per particle in id order. */
+int ariel_get_resample_accept(ariel_sim* h, uint8_t* dst, int64_t cap);
+
+/* FlipSim::m_max_density (src/sim/flip.cpp:42-71). */
+int ariel_get_max_density(ariel_sim* h, float* v);
+int ariel_set_max_density(ariel_sim* h, float v);
+
+int ariel_get_stats(ariel_sim* h, ariel_step_stats* s);
+
+/* Reference matrix-free operator y = A x (ComputeAx, solver.inl:196-230) on
+ * the current cell classes and level set; x and y are N^3 host floats. */
+int ariel_apply_laplacian(ariel_sim* h, const float* x, float* y);
+
+/* Preconditioner of the configured kind built on the current A and L
+ * (BuildPreconditioner, solver.inl:104-135) and applied: z = M^-1 r. */
+int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z);
+
+/* Binary state dump/restore: particles, all grid fields, flags, max density,
+ * frame counter. The descriptor must match the file's. */
+int ariel_save_state(ariel_sim* h, const char* path);
+int ariel_load_state(ariel_sim* h, const char* path);
+
+/* Scene::ExportParticles VDB/OBJ path (src/scene/scene.cpp:84-101): writes the
+ * fluid particles as a narrow-band level set sampled on a dense grid of
+ * (N+1)^3 voxels of size 1 in grid units, sphere radius 0.5 (levelset.cpp:
+ * 101-111), as a raw little-endian float32 volume with a text header. */
+int ariel_export_levelset(ariel_sim* h, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARIEL_H */
