@@ -25,6 +25,11 @@ STAGE_NAMES = ["sort", "density", "mark", "p2g", "divergence", "solve", "gradien
                "extrapolate", "advect", "resample"]
 SOLVE_CONVERGED, SOLVE_MAX_ITERS, SOLVE_BREAKDOWN, SOLVE_NONFINITE = range(4)
 MAX_FORCES, MAX_LIQUIDS = 4, 8
+(KERNEL_SORT, KERNEL_DENSITY, KERNEL_MARK, KERNEL_P2G, KERNEL_SPMV, KERNEL_PRECOND, KERNEL_AXPY,
+ KERNEL_GRADIENT, KERNEL_EXTRAPOLATE, KERNEL_ADVECT, KERNEL_RESAMPLE) = range(11)
+NUM_KERNELS = 11
+KERNEL_NAMES = ["sort", "density", "mark", "p2g", "spmv", "precond", "axpy", "gradient",
+                "extrapolate", "advect", "resample"]
 
 
 class Liquid(ctypes.Structure):
@@ -55,6 +60,11 @@ class StepStats(ctypes.Structure):
                 ("max_divergence", ctypes.c_double), ("max_density", ctypes.c_double),
                 ("stage_ms", ctypes.c_double * NUM_STAGES), ("step_ms", ctypes.c_double),
                 ("solve_ms", ctypes.c_double)]
+
+
+class KernelStats(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_int64), ("total_ms", ctypes.c_double),
+                ("bytes_per_launch", ctypes.c_double)]
 
 
 _P = ctypes.c_void_p
@@ -88,6 +98,9 @@ PROTOTYPES = {
     "ariel_get_stats": (_I32, [_P, ctypes.POINTER(StepStats)]),
     "ariel_apply_laplacian": (_I32, [_P, _P, _P]),
     "ariel_apply_preconditioner": (_I32, [_P, _P, _P]),
+    "ariel_set_profiling": (_I32, [_P, _I32]),
+    "ariel_reset_kernel_stats": (_I32, [_P]),
+    "ariel_get_kernel_stats": (_I32, [_P, _I32, ctypes.POINTER(KernelStats)]),
     "ariel_save_state": (_I32, [_P, ctypes.c_char_p]),
     "ariel_load_state": (_I32, [_P, ctypes.c_char_p]),
     "ariel_export_levelset": (_I32, [_P, ctypes.c_char_p]),

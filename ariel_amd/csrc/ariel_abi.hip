@@ -28,6 +28,11 @@ struct Stage {
     bool recorded = false;
 };
 
+struct KTimer {
+    int kernel;
+    hipEvent_t e0, e1;
+};
+
 }  // namespace
 
 struct ariel_sim {
@@ -68,6 +73,10 @@ struct ariel_sim {
     SolverScalars* sc_host = nullptr;   // pinned
 
     Stage stages[ARIEL_NUM_STAGES];
+    bool profile = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<KTimer> pending;
+    ariel_kernel_stats kstats[ARIEL_NUM_KERNELS]{};
     hipEvent_t step_t0 = nullptr, step_t1 = nullptr, solve_t0 = nullptr, solve_t1 = nullptr;
     ariel_step_stats stats{};
 
@@ -134,6 +143,69 @@ int check_launch(ariel_sim* h, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(h, ARIEL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     return ARIEL_OK;
+}
+
+hipEvent_t take_event(ariel_sim* h) {
+    if (!h->ev_pool.empty()) {
+        hipEvent_t e = h->ev_pool.back();
+        h->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Algorithmic HBM bytes of one launch group: per-unit figures x units.
+double kernel_bytes(const ariel_sim* h, int k) {
+    const double np = (double)h->np, nc = (double)h->nc;
+    switch (k) {
+        case ARIEL_KERNEL_SORT: return 60.0 * np + 8.0 * nc;
+        case ARIEL_KERNEL_DENSITY: return 40.0 * np;
+        case ARIEL_KERNEL_MARK: return 4.0 * np + 9.0 * nc;
+        case ARIEL_KERNEL_P2G: return 24.0 * np + 12.0 * nc;
+        case ARIEL_KERNEL_SPMV: return 13.0 * nc;
+        case ARIEL_KERNEL_PRECOND: return 26.0 * nc;
+        case ARIEL_KERNEL_AXPY: return 24.0 * nc;
+        case ARIEL_KERNEL_GRADIENT: return 33.0 * nc;
+        case ARIEL_KERNEL_EXTRAPOLATE: return 36.0 * nc;
+        case ARIEL_KERNEL_ADVECT: return 48.0 * np + 24.0 * nc;
+        case ARIEL_KERNEL_RESAMPLE: return 48.0 * np;
+        default: return 0.0;
+    }
+}
+
+struct Prof {
+    ariel_sim* h;
+    int k;
+    hipEvent_t e0 = nullptr;
+    Prof(ariel_sim* hh, int kk) : h(hh), k(kk) {
+        if (h->profile) {
+            e0 = take_event(h);
+            if (e0) (void)hipEventRecord(e0, h->stream);
+        }
+    }
+    ~Prof() {
+        if (!e0) return;
+        hipEvent_t e1 = take_event(h);
+        if (!e1) return;
+        (void)hipEventRecord(e1, h->stream);
+        h->pending.push_back(KTimer{k, e0, e1});
+    }
+};
+
+// Called after a stream synchronisation: fold finished timers into kstats.
+void drain_timers(ariel_sim* h) {
+    for (const KTimer& t : h->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.e0, t.e1) == hipSuccess) {
+            h->kstats[t.kernel].launches += 1;
+            h->kstats[t.kernel].total_ms += ms;
+        }
+        h->ev_pool.push_back(t.e0);
+        h->ev_pool.push_back(t.e1);
+    }
+    h->pending.clear();
 }
 
 int validate(const ariel_scene_desc* d, std::string& msg) {
@@ -282,6 +354,7 @@ int ensure_particles(ariel_sim* h, int64_t np) {
 // ----------------------------------------------------------------- stages
 int stage_sort(ariel_sim* h) {
     const int b = h->cur, o = 1 - b;
+    Prof prof(h, ARIEL_KERNEL_SORT);
     launch_bin(h->stream, h->pos[b], h->np, h->N, h->maxd, h->cell_of, h->counts);
     launch_scan(h->stream, h->counts, h->offsets, h->nc, h->scan_sums);
     launch_scatter(h->stream, h->cell_of, h->np, h->offsets, h->counts, h->slots);
@@ -292,16 +365,19 @@ int stage_sort(ariel_sim* h) {
 }
 
 int stage_density(ariel_sim* h, int forces) {
+    Prof prof(h, ARIEL_KERNEL_DENSITY);
     launch_density(h->stream, h->params(), h->pos[h->cur], h->vel[h->cur], h->np, h->offsets, forces);
     return check_launch(h, "density");
 }
 
 int stage_mark(ariel_sim* h) {
+    Prof prof(h, ARIEL_KERNEL_MARK);
     launch_mark(h->stream, h->params(), h->vel[h->cur], h->offsets, h->A, h->L);
     return check_launch(h, "mark");
 }
 
 int stage_p2g(ariel_sim* h) {
+    Prof prof(h, ARIEL_KERNEL_P2G);
     launch_p2g(h->stream, h->params(), h->pos[h->cur], h->vel[h->cur], h->offsets, h->A, h->u, h->du);
     return check_launch(h, "p2g");
 }
@@ -338,10 +414,19 @@ int stage_solve(ariel_sim* h) {
         const int n = std::min(chunk, h->d.max_iters - queued);
         for (int j = 0; j < n; ++j) {
             const int it = queued + j + 1;
-            launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red);
-            launch_axpy2(s, h->nc, h->P, h->S, h->R, h->Z, red);
+            {
+                Prof prof(h, ARIEL_KERNEL_SPMV);
+                launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red);
+            }
+            {
+                Prof prof(h, ARIEL_KERNEL_AXPY);
+                launch_axpy2(s, h->nc, h->P, h->S, h->R, h->Z, red);
+            }
             if (k > 0 && it % k == 0) launch_true_residual(s, p, h->A, h->L, h->P, h->D, h->R, red);
-            launch_precond_apply(s, p, h->A, h->pc, h->R, h->Q, h->Z, red, 0);
+            {
+                Prof prof(h, ARIEL_KERNEL_PRECOND);
+                launch_precond_apply(s, p, h->A, h->pc, h->R, h->Q, h->Z, red, 0);
+            }
             launch_update_s(s, h->nc, h->Z, h->S, red, 0);
         }
         TRY(check_launch(h, "pcg"));
@@ -365,17 +450,20 @@ int stage_solve(ariel_sim* h) {
 }
 
 int stage_gradient(ariel_sim* h) {
+    Prof prof(h, ARIEL_KERNEL_GRADIENT);
     launch_gradient(h->stream, h->params(), h->P, h->L, h->A, h->u);
     launch_grid_stats(h->stream, h->params(), h->u, h->A, h->sc);
     return check_launch(h, "gradient");
 }
 
 int stage_extrapolate(ariel_sim* h) {
+    Prof prof(h, ARIEL_KERNEL_EXTRAPOLATE);
     launch_extrapolate(h->stream, h->params(), h->A, h->u, h->du);
     return check_launch(h, "extrapolate");
 }
 
 int stage_advect(ariel_sim* h) {
+    Prof prof(h, ARIEL_KERNEL_ADVECT);
     launch_g2p_advect(h->stream, h->params(), h->pos[h->cur], h->vel[h->cur], h->np, h->u, h->du);
     return check_launch(h, "advect");
 }
@@ -384,6 +472,7 @@ int stage_resample(ariel_sim* h) {
     if (!h->d.resample) return ARIEL_OK;
     TRY(stage_sort(h));
     const int b = h->cur, o = 1 - b;
+    Prof prof(h, ARIEL_KERNEL_RESAMPLE);
     launch_resample(h->stream, h->params(), h->pos[b], h->vel[b], h->np, h->offsets, h->pos[o],
                     h->vel[o], h->accept);
     h->cur = o;
@@ -415,6 +504,7 @@ int run_stage(ariel_sim* h, int stage) {
 
 int collect_stats(ariel_sim* h) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    drain_timers(h);
     for (int i = 0; i < ARIEL_NUM_STAGES; ++i) {
         float ms = 0.f;
         if (h->stages[i].recorded && hipEventElapsedTime(&ms, h->stages[i].t0, h->stages[i].t1) == hipSuccess)
@@ -569,6 +659,8 @@ int ariel_destroy(ariel_sim* h) {
     }
     for (hipEvent_t e : {h->step_t0, h->step_t1, h->solve_t0, h->solve_t1})
         if (e) hipEventDestroy(e);
+    drain_timers(h);
+    for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return ARIEL_OK;
@@ -854,6 +946,30 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
     TRY(check_launch(h, "preconditioner"));
     HIPCHK(h, hipMemcpyAsync(z, h->Z, h->nc * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ARIEL_OK;
+}
+
+int ariel_set_profiling(ariel_sim* h, int32_t enable) {
+    TRY(check_handle(h));
+    h->profile = enable != 0;
+    return ARIEL_OK;
+}
+
+int ariel_reset_kernel_stats(ariel_sim* h) {
+    TRY(check_handle(h));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drain_timers(h);
+    for (auto& k : h->kstats) k = ariel_kernel_stats{};
+    return ARIEL_OK;
+}
+
+int ariel_get_kernel_stats(ariel_sim* h, int32_t kernel, ariel_kernel_stats* out) {
+    TRY(check_handle(h));
+    if (kernel < 0 || kernel >= ARIEL_NUM_KERNELS || !out) return fail(h, ARIEL_ERR_INVALID, "bad kernel id");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drain_timers(h);
+    *out = h->kstats[kernel];
+    out->bytes_per_launch = kernel_bytes(h, kernel);
     return ARIEL_OK;
 }
 

@@ -168,6 +168,17 @@ class FlipSim:
         self._check(self._lib.ariel_apply_preconditioner(self._h, _p(r), _p(z)))
         return z
 
+    def set_profiling(self, enable=True):
+        self._check(self._lib.ariel_set_profiling(self._h, 1 if enable else 0))
+
+    def reset_kernel_stats(self):
+        self._check(self._lib.ariel_reset_kernel_stats(self._h))
+
+    def kernel_stats(self, kernel):
+        ks = _abi.KernelStats()
+        self._check(self._lib.ariel_get_kernel_stats(self._h, kernel, ctypes.byref(ks)))
+        return ks
+
     def save_state(self, path):
         self._check(self._lib.ariel_save_state(self._h, str(path).encode()))
 

@@ -96,6 +96,20 @@ extern "C" {
 #define ARIEL_SOLVE_BREAKDOWN  2   /* s.As <= 0                              */
 #define ARIEL_SOLVE_NONFINITE  3   /* NaN/Inf in a reduction                 */
 
+/* ---- kernel groups for per-kernel timing (ariel_get_kernel_stats) -------- */
+#define ARIEL_KERNEL_SORT        0   /* bin + scan + scatter + cell order + gather */
+#define ARIEL_KERNEL_DENSITY     1
+#define ARIEL_KERNEL_MARK        2
+#define ARIEL_KERNEL_P2G         3
+#define ARIEL_KERNEL_SPMV        4   /* z = A s with the s.z reduction (PCG)    */
+#define ARIEL_KERNEL_PRECOND     5   /* one preconditioner apply (all passes)   */
+#define ARIEL_KERNEL_AXPY        6   /* p += a s, r -= a z with ||r||^2         */
+#define ARIEL_KERNEL_GRADIENT    7
+#define ARIEL_KERNEL_EXTRAPOLATE 8
+#define ARIEL_KERNEL_ADVECT      9   /* G2P FLIP/PIC blend + advection          */
+#define ARIEL_KERNEL_RESAMPLE   10   /* spring + resample pass (sort excluded)  */
+#define ARIEL_NUM_KERNELS       11
+
 #define ARIEL_MAX_FORCES  4
 #define ARIEL_MAX_LIQUIDS 8
 
@@ -152,6 +166,12 @@ typedef struct ariel_step_stats {
     double  step_ms;            /* device time of the last step              */
     double  solve_ms;           /* device time of the last pressure solve    */
 } ariel_step_stats;
+
+typedef struct ariel_kernel_stats {
+    int64_t launches;          /* timed launches (groups) since the last reset */
+    double  total_ms;          /* summed device time (HIP events on the stream) */
+    double  bytes_per_launch;  /* algorithmic HBM bytes of one launch          */
+} ariel_kernel_stats;
 
 typedef struct ariel_sim ariel_sim;
 
@@ -237,6 +257,13 @@ int ariel_apply_laplacian(ariel_sim* h, const float* x, float* y);
 /* Preconditioner of the configured kind built on the current A and L
  * (BuildPreconditioner, solver.inl:104-135) and applied: z = M^-1 r. */
 int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z);
+
+/* Per-kernel device timing with HIP events recorded on the handle's stream
+ * around each launch group (no reference counterpart: measurement only).
+ * Enabling adds two event records per launch; stats accumulate until reset. */
+int ariel_set_profiling(ariel_sim* h, int32_t enable);
+int ariel_reset_kernel_stats(ariel_sim* h);
+int ariel_get_kernel_stats(ariel_sim* h, int32_t kernel, ariel_kernel_stats* out);
 
 /* Binary state dump/restore: particles, all grid fields, flags, max density,
  * frame counter. The descriptor must match the file's. */
