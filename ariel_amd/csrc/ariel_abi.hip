@@ -67,6 +67,7 @@ struct ariel_sim {
     uint8_t* A = nullptr;
     float *R = nullptr, *Z = nullptr, *S = nullptr, *Q = nullptr, *pc = nullptr, *diag = nullptr;
     uint8_t* nup = nullptr;
+    double* P64 = nullptr;        // FP64 solution of the refined solve
     double* partials = nullptr;
     uint32_t* counter = nullptr;
     SolverScalars* sc = nullptr;
@@ -403,7 +404,10 @@ int stage_solve(ariel_sim* h) {
     launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
     launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc);
     launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every);
-    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule);
+    const int refine = h->d.refine_every > 0;
+    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
+                     h->P64, refine);
+    if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
     launch_precond_apply(s, p, h->A, h->pc, h->R, h->Q, h->Z, red, 0);
     launch_update_s(s, h->nc, h->Z, h->S, red, 1);
     TRY(check_launch(h, "solve setup"));
@@ -422,7 +426,8 @@ int stage_solve(ariel_sim* h) {
                 Prof prof(h, ARIEL_KERNEL_AXPY);
                 launch_axpy2(s, h->nc, h->P, h->S, h->R, h->Z, red);
             }
-            if (k > 0 && it % k == 0) launch_true_residual(s, p, h->A, h->L, h->P, h->D, h->R, red);
+            if (k > 0 && it % k == 0)
+                launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red);
             {
                 Prof prof(h, ARIEL_KERNEL_PRECOND);
                 launch_precond_apply(s, p, h->A, h->pc, h->R, h->Q, h->Z, red, 0);
@@ -435,6 +440,7 @@ int stage_solve(ariel_sim* h) {
         if (h->sc_host->done || queued >= h->d.max_iters) break;
         chunk = 4;
     }
+    if (refine) launch_refine_finish(s, h->nc, h->A, h->P, h->P64);
     HIPCHK(h, hipEventRecord(h->solve_t1, s));
     h->last_iters = h->sc_host->it;
     h->stats.iterations = h->sc_host->it;
@@ -601,6 +607,7 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->diag, h->nc)) || (rc = dalloc(h, &h->nup, h->nc)) ||
         (rc = dalloc(h, &h->counts, h->nc)) || (rc = dalloc(h, &h->offsets, h->nc + 1)) ||
         (rc = dalloc(h, &h->scan_sums, h->nc / 4096 + 4)) ||
+        (rc = dalloc(h, &h->P64, h->d.refine_every > 0 ? h->nc : 1)) ||
         (rc = dalloc(h, &h->partials, 2 * (int64_t)kMaxPartials)) || (rc = dalloc(h, &h->counter, 1)) ||
         (rc = dalloc(h, &h->sc, 1)))
         return bail(rc);
@@ -646,7 +653,7 @@ int ariel_destroy(ariel_sim* h) {
     for (void* p : {(void*)h->cell_of, (void*)h->slots, (void*)h->counts, (void*)h->offsets,
                     (void*)h->scan_sums, (void*)h->accept, (void*)h->P, (void*)h->D, (void*)h->L,
                     (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
-                    (void*)h->diag, (void*)h->nup, (void*)h->partials, (void*)h->counter, (void*)h->sc})
+                    (void*)h->diag, (void*)h->nup, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc})
         hipFree(p);
     for (int c = 0; c < 3; ++c) {
         hipFree(h->u[c]);

@@ -114,13 +114,15 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* A, cons
                           const float* r, float* q, float* z, const Reduce& red, int mode);
 void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                       float* P, const float* D, float* R, const Reduce& red, double tol,
-                      int stop_rule);
+                      int stop_rule, double* P64, int refine);
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                      const float* x, float* y, const Reduce& red);
 void launch_axpy2(hipStream_t s, int64_t n, float* P, const float* S, float* R, const float* Z,
                   const Reduce& red);
-void launch_true_residual(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                          const float* P, const float* D, float* R, const Reduce& red);
+// Reliable update every refine_every iterations: x64 += p, p = 0, r = b - A x64 (FP64).
+void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                            float* P, double* P64, const float* D, float* R, const Reduce& red);
+void launch_refine_finish(hipStream_t s, int64_t n, const uint8_t* A, float* P, const double* P64);
 void launch_update_s(hipStream_t s, int64_t n, const float* Z, float* S, const Reduce& red,
                      int first);
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,

@@ -22,7 +22,7 @@ namespace ariel {
 
 namespace {
 
-constexpr int kRedBlocks = 1024;   // cap of grid-stride reduction grids
+constexpr int kRedBlocks = 8192;   // cap of grid-stride reduction grids
 constexpr int kTile = 8;
 constexpr int kPad = 9;            // LDS stride per z-line (bank-conflict padding)
 
@@ -136,6 +136,28 @@ __device__ __forceinline__ double spmv_cell_f64(const uint8_t* __restrict__ A,
     return (double)N * (double)N * acc;
 }
 
+// The same operator on an FP64 vector (solution accumulated by reliable updates).
+__device__ __forceinline__ double spmv_cell_f64d(const uint8_t* __restrict__ A,
+                                                 const float* __restrict__ L,
+                                                 const double* __restrict__ x, int N, int i,
+                                                 int j, int k, int64_t c) {
+    const double xc = x[c];
+    double acc = 6.0 * xc;
+    const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k},
+                         {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const int a = min(max(0, q[m][0]), N - 1), b = min(max(0, q[m][1]), N - 1),
+                  cc = min(max(0, q[m][2]), N - 1);
+        const int64_t n = cidx(N, a, b, cc);
+        const uint8_t t = A[n];
+        if (t == kFluid) acc -= x[n];
+        else if (t == kSolid) acc -= xc;
+        else acc -= (double)L[n] / (double)gmin(1.0e-6f, L[c]) * xc;
+    }
+    return (double)N * (double)N * acc;
+}
+
 __device__ __forceinline__ void decode_cell(int N, int64_t c, int& i, int& j, int& k) {
     k = (int)(c % N);
     j = (int)((c / N) % N);
@@ -158,9 +180,12 @@ __global__ void k_solver_init(SolverScalars* sc, int max_iters, int refine_every
 
 // Warm start: P := 0 off FLUID (what the reference's first Op does), r = b - A p,
 // ||b||^2 and ||r||^2 over FLUID cells; stop threshold; early exit if converged.
+// With reliable updates (refine): x64 = P, r = b - A x64 in FP64; P is cleared
+// by a separate fill once every thread has read it.
 __global__ void k_residual0(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
                             float* __restrict__ P, const float* __restrict__ D, float* __restrict__ R,
-                            Reduce red, double tol, int stop_rule) {
+                            Reduce red, double tol, int stop_rule, double* __restrict__ P64,
+                            int refine) {
     const int N = p.N;
     const int64_t nc = (int64_t)N * N * N;
     const float n = (float)N;
@@ -169,18 +194,26 @@ __global__ void k_residual0(Params p, const uint8_t* __restrict__ A, const float
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
          c += (int64_t)gridDim.x * blockDim.x) {
         if (A[c] != kFluid) {
-            P[c] = 0.0f;
+            if (!refine) P[c] = 0.0f;
+            else P64[c] = 0.0;
             R[c] = 0.0f;
             continue;
         }
         int i, j, k;
         decode_cell(N, c, i, j, k);
-        const float ax = spmv_cell(A, L, P, N, i, j, k, c, h);
         const float d = D[c];
-        const float r = d + (-1.0f) * ax;
-        R[c] = r;
         v[0] += (double)d * (double)d;
-        v[1] += (double)r * (double)r;
+        if (refine) {
+            const double r = (double)d - spmv_cell_f64(A, L, P, N, i, j, k, c);
+            P64[c] = (double)P[c];
+            R[c] = (float)r;
+            v[1] += r * r;
+        } else {
+            const float ax = spmv_cell(A, L, P, N, i, j, k, c, h);
+            const float r = d + (-1.0f) * ax;
+            R[c] = r;
+            v[1] += (double)r * (double)r;
+        }
     }
     if (block_arrive<2>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
         double s[2];
@@ -254,7 +287,7 @@ __device__ __forceinline__ void decide(SolverScalars* sc) {
     }
 }
 __device__ __forceinline__ bool refine_due(const SolverScalars* sc) {
-    return sc->refine_every > 0 && (sc->it % sc->refine_every) == 0 && sc->rnorm2 > 100.0 * sc->thr;
+    return sc->refine_every > 0 && (sc->it % sc->refine_every) == 0;
 }
 
 // p += alpha s; r += (-alpha) z (Op, solver.inl:157-177); ||r||^2; stop test.
@@ -282,33 +315,47 @@ __global__ void k_axpy2(int64_t n, float* __restrict__ P, const float* __restric
     }
 }
 
-// Residual replacement: r = b - A p evaluated in FP64, stored in FP32.
+// Reliable update, part 1: x64 += p, p = 0 (every refine_every iterations).
+__global__ void k_flush(int64_t n, const uint8_t* __restrict__ A, float* __restrict__ P,
+                        double* __restrict__ P64, const SolverScalars* sc) {
+    if (sc->done || !refine_due(sc)) return;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (A[c] == kFluid) P64[c] += (double)P[c];
+        P[c] = 0.0f;
+    }
+}
+
+// mode 0 (reliable update, part 2): r = b - A x64 in FP64, stored in FP32, then the
+// stop test. mode 1 (report): ||b - A p||^2 of the FP32 pressure into sc->tmp.
 __global__ void k_true_residual(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
-                                const float* __restrict__ P, const float* __restrict__ D,
-                                float* __restrict__ R, Reduce red, int force) {
-    if (red.sc->done && !force) return;
-    if (!force && !refine_due(red.sc)) return;
+                                const float* __restrict__ P, const double* __restrict__ P64,
+                                const float* __restrict__ D, float* __restrict__ R, Reduce red,
+                                int mode) {
+    if (mode == 0 && (red.sc->done || !refine_due(red.sc))) return;
     const int N = p.N;
     const int64_t nc = (int64_t)N * N * N;
     double v[1] = {0.0};
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
          c += (int64_t)gridDim.x * blockDim.x) {
         if (A[c] != kFluid) {
-            if (!force) R[c] = 0.0f;
+            if (mode == 0) R[c] = 0.0f;
             continue;
         }
         int i, j, k;
         decode_cell(N, c, i, j, k);
-        const double r = (double)D[c] - spmv_cell_f64(A, L, P, N, i, j, k, c);
+        const double ax = mode == 0 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
+                                    : spmv_cell_f64(A, L, P, N, i, j, k, c);
+        const double r = (double)D[c] - ax;
         v[0] += r * r;
-        if (!force) R[c] = (float)r;
+        if (mode == 0) R[c] = (float)r;
     }
     if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
         double s[1];
         sum_partials<1>(red.partials, gridDim.x, s, red.counter);
         if (threadIdx.x == 0) {
             SolverScalars* sc = red.sc;
-            if (force) {
+            if (mode == 1) {
                 sc->tmp = s[0];
             } else {
                 sc->rnorm2 = s[0];
@@ -316,6 +363,14 @@ __global__ void k_true_residual(Params p, const uint8_t* __restrict__ A, const f
             }
         }
     }
+}
+
+// End of a refined solve: p = (float)(x64 + p) on FLUID cells, 0 elsewhere.
+__global__ void k_refine_finish(int64_t n, const uint8_t* __restrict__ A, float* __restrict__ P,
+                                const double* __restrict__ P64) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x)
+        P[c] = A[c] == kFluid ? (float)(P64[c] + (double)P[c]) : 0.0f;
 }
 
 __global__ void k_update_s(int64_t n, const float* __restrict__ Z, float* __restrict__ S,
@@ -825,10 +880,11 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* A, cons
 }
 
 void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L, float* P,
-                      const float* D, float* R, const Reduce& red, double tol, int stop_rule) {
+                      const float* D, float* R, const Reduce& red, double tol, int stop_rule,
+                      double* P64, int refine) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
     hipLaunchKernelGGL(k_residual0, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P, D, R,
-                       red, tol, stop_rule);
+                       red, tol, stop_rule, P64, refine);
 }
 
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
@@ -842,18 +898,23 @@ void launch_axpy2(hipStream_t s, int64_t n, float* P, const float* S, float* R, 
     hipLaunchKernelGGL(k_axpy2, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, P, S, R, Z, red);
 }
 
-void launch_true_residual(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                          const float* P, const float* D, float* R, const Reduce& red) {
+void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                            float* P, double* P64, const float* D, float* R, const Reduce& red) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_true_residual, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P, D,
-                       R, red, 0);
+    hipLaunchKernelGGL(k_flush, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, nc, A, P, P64, red.sc);
+    hipLaunchKernelGGL(k_true_residual, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L,
+                       (const float*)P, (const double*)P64, D, R, red, 0);
+}
+
+void launch_refine_finish(hipStream_t s, int64_t n, const uint8_t* A, float* P, const double* P64) {
+    hipLaunchKernelGGL(k_refine_finish, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, A, P, P64);
 }
 
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                                const float* P, const float* D, const Reduce& red) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_true_residual, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P, D,
-                       (float*)nullptr, red, 1);
+    hipLaunchKernelGGL(k_true_residual, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P,
+                       (const double*)nullptr, D, (float*)nullptr, red, 1);
 }
 
 void launch_update_s(hipStream_t s, int64_t n, const float* Z, float* S, const Reduce& red,

@@ -597,19 +597,20 @@ void apply_reference_precond(oracle_sim& s, const float* R, float* Z) {
 //   e = diag_i - sum_j pc_j^2 - tau * sum_j (nup_j - 1) pc_j^2,
 //   e = diag_i if e < sigma*diag_i;  pc_i = 1/sqrt(e);
 // nup_j = number of fluid upper neighbours of j. Off-diagonals are -1 (ARef).
-// Orderings: natural (lexicographic), tiled red-black (tiles of edge T coloured by
-// (ti+tj+tk)&1, red tiles first, natural order inside a tile), block (tiles
-// independent: cross-tile couplings dropped). Neighbour order for sums:
+// Orderings: natural (lexicographic), tiled red-black (tiles of 8 x 8 x Tz cells,
+// Tz = desc.tile, coloured by (ti+tj+tk)&1, red tiles first, natural order inside
+// a tile), block (tiles independent: cross-tile couplings dropped). Neighbour order for sums:
 // lower pass -x,-y,-z,+x,+y,+z; upper pass +x,+y,+z,-x,-y,-z.
 const int DIRS[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 const int UDIRS[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
 
 struct Ordering {
     int kind;  // 1 natural, 0 tiled red-black, 4 block
-    int T;
-    int color(int i, int j, int k) const { return ((i / T) + (j / T) + (k / T)) & 1; }
+    int T;     // tile edge along x and y (8)
+    int Tz;    // tile edge along z
+    int color(int i, int j, int k) const { return ((i / T) + (j / T) + (k / Tz)) & 1; }
     bool same_tile(int i, int j, int k, int a, int b, int c) const {
-        return i / T == a / T && j / T == b / T && k / T == c / T;
+        return i / T == a / T && j / T == b / T && k / Tz == c / Tz;
     }
     // 1 if (a,b,c) precedes (i,j,k), 0 if it follows, -1 if uncoupled.
     int lower(int i, int j, int k, int a, int b, int c) const {
@@ -624,7 +625,8 @@ struct Ordering {
 Ordering ordering_of(const oracle_sim& s) {
     Ordering o;
     o.kind = s.d.precond;
-    o.T = s.d.tile > 0 ? s.d.tile : 8;
+    o.T = 8;
+    o.Tz = s.d.tile > 0 ? s.d.tile : 8;
     return o;
 }
 
@@ -813,18 +815,75 @@ void solve_faithful(oracle_sim& s) {
     s.st.residual_norm = std::sqrt((double)error1);
 }
 
+// The operator on an FP64 vector (same terms and order as compute_ax_f64).
+void compute_ax_f64d(const oracle_sim& s, const double* X, double* Y, Pool& pool) {
+    const int N = s.N;
+    const double scale = (double)N * (double)N;
+    pfor(pool, 0, N, [&](int64_t ii) {
+        int i = (int)ii;
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                int64_t c = s.cidx(i, j, k);
+                if (s.A[c] != FLUID) { Y[c] = 0.0; continue; }
+                const double xc = X[c];
+                double acc = 6.0 * xc;
+                const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k},
+                                     {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+                for (int m = 0; m < 6; ++m) {
+                    int a = std::min(std::max(0, q[m][0]), N - 1);
+                    int b = std::min(std::max(0, q[m][1]), N - 1);
+                    int cc = std::min(std::max(0, q[m][2]), N - 1);
+                    int t = s.a_at(a, b, cc);
+                    if (t == FLUID) acc -= X[s.cidx(a, b, cc)];
+                    else if (t == SOLID) acc -= xc;
+                    else acc -= (double)s.L[s.cidx(a, b, cc)] / (double)gmin(1.0e-6f, s.L[c]) * xc;
+                }
+                Y[c] = scale * acc;
+            }
+    });
+}
+
+// r = b - A x for an FP64 solution x; returns ||r||^2 and stores r in FP32.
+double true_residual_d(const oracle_sim& s, const double* X, float* R, Pool& pool) {
+    std::vector<double> ax(s.nc(), 0.0);
+    compute_ax_f64d(s, X, ax.data(), pool);
+    double acc = 0.0;
+    for (int64_t c = 0; c < s.nc(); ++c) {
+        if (s.A[c] != FLUID) { R[c] = 0.0f; continue; }
+        double r = (double)s.D[c] - ax[c];
+        acc += r * r;
+        R[c] = (float)r;
+    }
+    return acc;
+}
+
 // E3-E5 PCG: warm start (non-fluid P cleared, as the reference's first Op does),
-// FP64 dots/norms, FP32 vectors, optional true-residual recompute every k.
+// FP64 dots/norms, FP32 vectors. With refinement period k > 0 the solution is
+// accumulated in FP64 by group-wise reliable updates (van der Vorst & Ye,
+// "Residual replacement strategies for Krylov subspace iterative methods", 2000):
+// x64 = warm start, the FP32 PCG iterate P holds the correction since the last
+// flush; every k iterations x64 += P, P = 0 and r = b - A x64 evaluated in FP64;
+// at exit P = (float)(x64 + P).
 void solve_extended(oracle_sim& s) {
     const int64_t n = s.nc();
     std::vector<float> R(n, 0.0f), Z(n, 0.0f), S(n, 0.0f), AX(n, 0.0f);
+    std::vector<double> X64;
+    const int kref = s.d.refine_every;
     build_precond(s);
     for (int64_t c = 0; c < n; ++c)
         if (s.A[c] != FLUID) s.P[c] = 0.0f;
-    compute_ax(s, s.P.data(), AX.data(), *s.pool);
-    op(s, s.D.data(), AX.data(), R.data(), -1.0f, *s.pool);
     double bnorm2 = product_f64(s, s.D.data(), s.D.data());
-    double rnorm2 = product_f64(s, R.data(), R.data());
+    double rnorm2;
+    if (kref > 0) {
+        X64.assign(n, 0.0);
+        for (int64_t c = 0; c < n; ++c) X64[c] = (double)s.P[c];
+        rnorm2 = true_residual_d(s, X64.data(), R.data(), *s.pool);
+        std::fill(s.P.begin(), s.P.end(), 0.0f);
+    } else {
+        compute_ax(s, s.P.data(), AX.data(), *s.pool);
+        op(s, s.D.data(), AX.data(), R.data(), -1.0f, *s.pool);
+        rnorm2 = product_f64(s, R.data(), R.data());
+    }
     double thr;
     if (s.d.stop_rule == 1) thr = (double)(1.0e-2f * (float)(s.N * s.N * s.N));
     else thr = (double)s.d.tol * (double)s.d.tol * bnorm2;
@@ -843,10 +902,13 @@ void solve_extended(oracle_sim& s) {
             op(s, s.P.data(), S.data(), s.P.data(), af, *s.pool);
             op(s, R.data(), Z.data(), R.data(), -af, *s.pool);
             rnorm2 = product_f64(s, R.data(), R.data());
-            // Residual replacement (E5): every k iterations, while the recursive
-            // residual is still above 10x the stop threshold, r <- b - A p in FP64.
-            if (s.d.refine_every > 0 && it % s.d.refine_every == 0 && rnorm2 > 100.0 * thr)
-                rnorm2 = true_residual(s, s.P.data(), R.data(), *s.pool);
+            if (kref > 0 && it % kref == 0) {
+                for (int64_t c = 0; c < n; ++c) {
+                    if (s.A[c] == FLUID) X64[c] += (double)s.P[c];
+                    s.P[c] = 0.0f;
+                }
+                rnorm2 = true_residual_d(s, X64.data(), R.data(), *s.pool);
+            }
             if (!std::isfinite(rnorm2)) { status = 3; break; }
             if (rnorm2 <= thr) { status = 0; break; }
             apply_precond(s, R.data(), Z.data());
@@ -859,6 +921,9 @@ void solve_extended(oracle_sim& s) {
     } else {
         status = 0;
     }
+    if (kref > 0)
+        for (int64_t c = 0; c < n; ++c)
+            s.P[c] = (s.A[c] == FLUID) ? (float)(X64[c] + (double)s.P[c]) : 0.0f;
     s.st.iterations = it;
     s.st.solve_status = status;
     s.st.residual_norm = std::sqrt(rnorm2);
