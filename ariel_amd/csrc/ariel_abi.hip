@@ -67,6 +67,7 @@ struct ariel_sim {
     uint8_t* A = nullptr;
     float *R = nullptr, *Z = nullptr, *S = nullptr, *Q = nullptr, *pc = nullptr, *diag = nullptr;
     uint8_t* nup = nullptr;
+    uint8_t* fl8 = nullptr;       // fluid bits per 8-cell z-segment
     double* P64 = nullptr;        // FP64 solution of the refined solve
     double* partials = nullptr;
     uint32_t* counter = nullptr;
@@ -402,13 +403,14 @@ int stage_solve(ariel_sim* h) {
     hipStream_t s = h->stream;
     HIPCHK(h, hipEventRecord(h->solve_t0, s));
     launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
+    launch_fluid_segments(s, h->N, h->A, h->fl8);
     launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc);
     launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every);
     const int refine = h->d.refine_every > 0;
     launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
                      h->P64, refine);
     if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
-    launch_precond_apply(s, p, h->A, h->pc, h->R, h->Q, h->Z, red, 0);
+    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, 0);
     launch_update_s(s, h->nc, h->Z, h->S, red, 1);
     TRY(check_launch(h, "solve setup"));
     int queued = 0;
@@ -430,7 +432,7 @@ int stage_solve(ariel_sim* h) {
                 launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red);
             {
                 Prof prof(h, ARIEL_KERNEL_PRECOND);
-                launch_precond_apply(s, p, h->A, h->pc, h->R, h->Q, h->Z, red, 0);
+                launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, 0);
             }
             launch_update_s(s, h->nc, h->Z, h->S, red, 0);
         }
@@ -605,6 +607,7 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->A, h->nc)) || (rc = dalloc(h, &h->R, h->nc)) || (rc = dalloc(h, &h->Z, h->nc)) ||
         (rc = dalloc(h, &h->S, h->nc)) || (rc = dalloc(h, &h->Q, h->nc)) || (rc = dalloc(h, &h->pc, h->nc)) ||
         (rc = dalloc(h, &h->diag, h->nc)) || (rc = dalloc(h, &h->nup, h->nc)) ||
+        (rc = dalloc(h, &h->fl8, h->nc / 8)) ||
         (rc = dalloc(h, &h->counts, h->nc)) || (rc = dalloc(h, &h->offsets, h->nc + 1)) ||
         (rc = dalloc(h, &h->scan_sums, h->nc / 4096 + 4)) ||
         (rc = dalloc(h, &h->P64, h->d.refine_every > 0 ? h->nc : 1)) ||
@@ -653,7 +656,7 @@ int ariel_destroy(ariel_sim* h) {
     for (void* p : {(void*)h->cell_of, (void*)h->slots, (void*)h->counts, (void*)h->offsets,
                     (void*)h->scan_sums, (void*)h->accept, (void*)h->P, (void*)h->D, (void*)h->L,
                     (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
-                    (void*)h->diag, (void*)h->nup, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc})
+                    (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc})
         hipFree(p);
     for (int c = 0; c < 3; ++c) {
         hipFree(h->u[c]);
@@ -946,10 +949,11 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
     const Params p = h->params();
     HIPCHK(h, hipMemcpyAsync(h->R, r, h->nc * sizeof(float), hipMemcpyHostToDevice, h->stream));
     launch_precond_setup(h->stream, p, h->A, h->L, h->diag, h->nup, h->pc);
+    launch_fluid_segments(h->stream, h->N, h->A, h->fl8);
     launch_precond_build(h->stream, p, h->A, h->diag, h->nup, h->pc);
     launch_solver_init(h->stream, h->sc, h->d.max_iters, h->d.refine_every);
     launch_fill_f32(h->stream, h->Z, h->nc, 0.0f);
-    launch_precond_apply(h->stream, p, h->A, h->pc, h->R, h->Q, h->Z, h->reduce(), 0);
+    launch_precond_apply(h->stream, p, h->fl8, h->pc, h->R, h->Q, h->Z, h->reduce(), 0);
     TRY(check_launch(h, "preconditioner"));
     HIPCHK(h, hipMemcpyAsync(z, h->Z, h->nc * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -109,8 +109,10 @@ void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, cons
                           float* diag, uint8_t* nup, float* pc);
 void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
                           const uint8_t* nup, float* pc);
-// z = M^-1 r; also finalises rho_new = z.r and beta when `finalize_rho` is set.
-void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* A, const float* pc,
+// Fluid bits of each 8-cell z-segment (input of the tiled preconditioner apply).
+void launch_fluid_segments(hipStream_t s, int N, const uint8_t* A, uint8_t* fl8);
+// z = M^-1 r; the last pass finalises rho_new = z.r and beta in the scalars.
+void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, const float* pc,
                           const float* r, float* q, float* z, const Reduce& red, int mode);
 void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                       float* P, const float* D, float* R, const Reduce& red, double tol,

@@ -290,6 +290,145 @@ __global__ void k_p2g(Params p, const float4* __restrict__ pos, const float4* __
     u[f] = zero ? 0.0f : val;
 }
 
+// P2G through LDS. A 64-lane block owns the cells (i, j, k0..k0+63) and computes
+// the x-, y- and z-faces of each (faces at i = N, j = N, k = N included). The
+// union of the three reference windows is the 4x4 block of particle columns
+// (sx, sy) in [i-2, i+1] x [j-2, j+1], cells k-2..k+1 along z. Columns are visited
+// in (sx, sy) order; each column's particles (contiguous in sorted order) are
+// staged in LDS with their clamped grid positions, and every face adds the
+// particles of its own window in ascending order: per face the additions happen
+// in exactly the reference's order (sx -> sy -> sz -> list), so the sums are the
+// same as the per-face gather's. Weights beyond the kernel radius are exactly
+// zero and skipped (adding +0 to a sum that starts at +0 never changes it).
+constexpr int kP2GCap = 512;
+
+__device__ __forceinline__ float p2g_weight(bool tri, float px, float py, float pz, float fx,
+                                            float fy, float fz, float RE, float hh) {
+    const float dx = px - fx, dy = py - fy, dz = pz - fz;
+    if (tri) {
+        const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+        if (ax >= 1.0f || ay >= 1.0f || az >= 1.0f) return 0.0f;
+        return (gmax(1.0f - ax, 0.0f) * gmax(1.0f - ay, 0.0f)) * gmax(1.0f - az, 0.0f);
+    }
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 >= hh) return 0.0f;
+    return 1.0f * sharpen(r2, RE);
+}
+
+__global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restrict__ pos,
+                                                  const float4* __restrict__ vel,
+                                                  const int32_t* __restrict__ offsets,
+                                                  const uint8_t* __restrict__ A,
+                                                  float* __restrict__ ux, float* __restrict__ uy,
+                                                  float* __restrict__ uz, float* __restrict__ dux,
+                                                  float* __restrict__ duy, float* __restrict__ duz) {
+    __shared__ float sp[6][kP2GCap];
+    const int N = p.N;
+    const int nkc = (N + 1 + 63) / 64;
+    int64_t b = blockIdx.x;
+    const int kc = (int)(b % nkc);
+    b /= nkc;
+    const int j = (int)(b % (N + 1));
+    const int i = (int)(b / (N + 1));
+    const int lane = threadIdx.x;
+    const int k0 = kc * 64;
+    const int k = k0 + lane;
+    const bool kval = k <= N;
+    const bool vx = kval && j < N && k < N;
+    const bool vy = kval && i < N && k < N;
+    const bool vz = kval && i < N && j < N;
+    const float maxd = (float)N;
+    const float RE = p.sharpen_re;
+    const float hh = RE * RE;
+    const bool tri = p.splat == 1;
+    const float fi = (float)i, fj = (float)j, fk = (float)k;
+    float swx = 0.0f, sux = 0.0f, swy = 0.0f, suy = 0.0f, swz = 0.0f, suz = 0.0f;
+    const int zlo_blk = max(0, k0 - 2), zhi_blk = min(N - 1, k0 + 64);
+    const int a0 = max(k - 2, 0), a1 = min(k + 1, N - 1);
+    const int b0 = max(k - 1, 0), b1 = min(k, N - 1);
+    for (int sx = max(i - 2, 0); sx <= min(i + 1, N - 1); ++sx) {
+        for (int sy = max(j - 2, 0); sy <= min(j + 1, N - 1); ++sy) {
+            const bool colx = (sx >= i - 1 && sx <= i) && j < N;
+            const bool coly = (sy >= j - 1 && sy <= j) && i < N;
+            const bool colz = i < N && j < N;
+            if (!(colx || coly || colz)) continue;
+            const int64_t col = cidx(N, sx, sy, 0);
+            const int32_t c_begin = offsets[col + zlo_blk], c_end = offsets[col + zhi_blk + 1];
+            int32_t ra0 = 0, ra1 = 0, rb0 = 0, rb1 = 0;
+            if (kval && a0 <= a1) { ra0 = offsets[col + a0]; ra1 = offsets[col + a1 + 1]; }
+            if (kval && b0 <= b1) { rb0 = offsets[col + b0]; rb1 = offsets[col + b1 + 1]; }
+            const bool dox = vx && colx, doy = vy && coly, doz = vz && colz;
+            for (int32_t c0 = c_begin; c0 < c_end; c0 += kP2GCap) {
+                const int32_t c1 = min(c_end, c0 + kP2GCap);
+                __syncthreads();
+                for (int32_t q = c0 + lane; q < c1; q += 64) {
+                    const float4 o = pos[q];
+                    const float4 v = vel[q];
+                    const int t = q - c0;
+                    sp[0][t] = gmax(0.0f, gmin(maxd, maxd * o.x));
+                    sp[1][t] = gmax(0.0f, gmin(maxd, maxd * o.y));
+                    sp[2][t] = gmax(0.0f, gmin(maxd, maxd * o.z));
+                    sp[3][t] = v.x;
+                    sp[4][t] = v.y;
+                    sp[5][t] = v.z;
+                }
+                __syncthreads();
+                const int32_t q0 = max(ra0, c0), q1 = min(ra1, c1);
+                for (int32_t q = q0; q < q1; ++q) {
+                    const int t = q - c0;
+                    const float px = sp[0][t], py = sp[1][t], pz = sp[2][t];
+                    if (dox) {
+                        const float w = p2g_weight(tri, px, py, pz, fi, fj + 0.5f, fk + 0.5f, RE, hh);
+                        if (w != 0.0f) {
+                            sux += w * sp[3][t];
+                            swx += w;
+                        }
+                    }
+                    if (doy) {
+                        const float w = p2g_weight(tri, px, py, pz, fi + 0.5f, fj, fk + 0.5f, RE, hh);
+                        if (w != 0.0f) {
+                            suy += w * sp[4][t];
+                            swy += w;
+                        }
+                    }
+                    if (doz && q >= rb0 && q < rb1) {
+                        const float w = p2g_weight(tri, px, py, pz, fi + 0.5f, fj + 0.5f, fk, RE, hh);
+                        if (w != 0.0f) {
+                            suz += w * sp[5][t];
+                            swz += w;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // StorePreviousGrid copy, then EnforceBoundaryVelocity
+    if (vx) {
+        const int64_t f = fidx(N, 0, i, j, k);
+        const float val = swx > 0 ? sux / swx : 0.0f;
+        dux[f] = val;
+        bool zero = (i == 0 || i == N);
+        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i - 1, j, k)] == kSolid);
+        ux[f] = zero ? 0.0f : val;
+    }
+    if (vy) {
+        const int64_t f = fidx(N, 1, i, j, k);
+        const float val = swy > 0 ? suy / swy : 0.0f;
+        duy[f] = val;
+        bool zero = (j == 0 || j == N);
+        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j - 1, k)] == kSolid);
+        uy[f] = zero ? 0.0f : val;
+    }
+    if (vz) {
+        const int64_t f = fidx(N, 2, i, j, k);
+        const float val = swz > 0 ? suz / swz : 0.0f;
+        duz[f] = val;
+        bool zero = (k == 0 || k == N);
+        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j, k - 1)] == kSolid);
+        uz[f] = zero ? 0.0f : val;
+    }
+}
+
 // ------------------------------------------- SolvePicFlip + AdvectParticles
 // flip.cpp:287-322 and 229-285: FLIP = u + I(du), PIC = I(u), blend
 // (1-r)*PIC + r*FLIP; p += dt * I(u)(p) (the PIC velocity at the same position);
@@ -349,6 +488,8 @@ __global__ void k_resample(Params p, const float4* __restrict__ pos, const float
     const float4 me = pos[s];
     const float4 mv = vel[s];
     const uint32_t my_id = (uint32_t)__float_as_int(me.w);
+    const float re2 = re * re;
+    const float re2_skip = re2 * 1.0001f;
     const uint32_t fr = (uint32_t)p.frame;
     float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
     {
@@ -368,7 +509,13 @@ __global__ void k_resample(Params p, const float4* __restrict__ pos, const float
                     if (q == (int32_t)s) continue;
                     const float4 o = pos[q];
                     float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
-                    float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and then
+                    // dist > 0.1 re: the term adds +-0 to the spring. d2 above re^2
+                    // by a margin far beyond the rounding of sqrt and the square
+                    // guarantees that, so the pair is skipped without changing a bit.
+                    if (d2 > re2_skip) continue;
+                    float dist = sqrtf(d2);
                     float w = springforce * 1.0f * smooth(dist * dist, re);
                     if (dist > 0.1f * re) {
                         sx_ += w * (me.x - o.x) / dist * re;
@@ -404,6 +551,9 @@ __global__ void k_resample(Params p, const float4* __restrict__ pos, const float
                 for (int32_t q = q0; q < q1; ++q) {
                     const float4 o = pos[q];
                     float dist2 = sqrlength(tx, ty, tz, o.x, o.y, o.z);
+                    // Sharpen(dist2, re) = max(re^2/max(dist2,1e-5) - 1, 0) is exactly 0
+                    // when max(dist2, 1e-5) >= re^2; the +0 term is skipped.
+                    if (gmax(dist2, 1.0e-5f) >= re2) continue;
                     float w = 1.0f * sharpen(dist2, re);
                     const float4 ov = vel[q];
                     rx = rx + w * ov.x;
@@ -511,6 +661,16 @@ void launch_mark(hipStream_t s, const Params& p, const float4* vel, const int32_
 
 void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
                 const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3]) {
+    const int64_t nkc = (p.N + 1 + 63) / 64;
+    const int64_t nb = (int64_t)(p.N + 1) * (p.N + 1) * nkc;
+    hipLaunchKernelGGL(k_p2g_cols, dim3((unsigned)nb), dim3(64), 0, s, p, pos, vel, offsets, A,
+                       u[0], u[1], u[2], du[0], du[1], du[2]);
+}
+
+// Reference-shaped per-face gather (one thread per face), kept for cross-checks.
+void launch_p2g_faces(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
+                      const int32_t* offsets, const uint8_t* A, float* const u[3],
+                      float* const du[3]) {
     int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
     hipLaunchKernelGGL(k_p2g, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), 0, s, p, pos, vel,
                        offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]);

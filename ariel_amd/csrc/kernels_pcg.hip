@@ -22,7 +22,7 @@ namespace ariel {
 
 namespace {
 
-constexpr int kRedBlocks = 8192;   // cap of grid-stride reduction grids
+constexpr int kRedBlocks = 2048;   // cap of grid-stride reduction grids
 constexpr int kTile = 8;
 constexpr int kPad = 9;            // LDS stride per z-line (bank-conflict padding)
 
@@ -86,6 +86,19 @@ __device__ void sum_partials(const double* part, int n, double (&out)[NV], uint3
         out[i] = t;
     }
     if (threadIdx.x == 0) *counter = 0u;
+}
+
+// Block-level FP64 sum, valid in thread 0.
+__device__ double block_sum_only(double v) {
+    __shared__ double sm[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    double t = wave_sum(v);
+    if (lane == 0) sm[w] = t;
+    __syncthreads();
+    double acc = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nw; ++i) acc += sm[i];
+    return acc;
 }
 
 __device__ __forceinline__ bool finite(double x) { return isfinite(x); }
@@ -636,171 +649,260 @@ __global__ __launch_bounds__(64) void k_mic_build(Params p, const uint8_t* __res
     }
 }
 
-// MIC(0) apply, one pass over the tiles of one colour.
-//   phase 0 (red, forward):          q = L^-1 r inside the tile          -> Q
-//   phase 1 (black, fwd + bwd):      forward with red halos, backward     -> Z, z.r
-//   phase 2 (red, backward):         backward with black halos            -> Z, z.r (+finalise)
-//   phase 3 (block, fwd + bwd):      no cross-tile coupling               -> Z, z.r (+finalise)
+// Fluid bits of every z-segment of 8 cells: fl8[(i*N + j)*(N/8) + k/8], bit k%8.
+__global__ void k_fluid_segments(int N, const uint8_t* __restrict__ A, uint8_t* __restrict__ fl8) {
+    const int64_t nseg = (int64_t)N * N * (N / kTile);
+    int64_t sgi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sgi >= nseg) return;
+    const uint2 v = *reinterpret_cast<const uint2*>(A + sgi * kTile);
+    uint32_t m = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        m |= (((v.x >> (8 * b)) & 0xFFu) == (uint32_t)kFluid ? 1u : 0u) << b;
+        m |= (((v.y >> (8 * b)) & 0xFFu) == (uint32_t)kFluid ? 1u : 0u) << (b + 4);
+    }
+    fl8[sgi] = (uint8_t)m;
+}
+
+constexpr int kTilesPerBlock = 4;
+constexpr int kLine = 64 * kPad;          // floats of one per-line LDS array
+constexpr int kHalo = 2 * kTile * kPad;   // two halo faces (minus / plus side) of 8 lines
+
+// Tile `idx` of colour `color` (0 red, 1 black, -1 all) -> tile coordinates.
+// With an even tile count per axis every (ti,tj) row holds TN/2 tiles of each
+// colour; otherwise tiles are enumerated linearly and off-colour ones are idle.
+__device__ __forceinline__ bool tile_coords(int TN, int64_t idx, int color, int& ti, int& tj,
+                                            int& tk) {
+    if (color >= 0 && (TN & 1) == 0) {
+        const int half = TN >> 1;
+        const int64_t row = idx / half;
+        const int m = (int)(idx % half);
+        ti = (int)(row / TN);
+        tj = (int)(row % TN);
+        tk = 2 * m + ((ti + tj + color) & 1);
+        return ti < TN;
+    }
+    tk = (int)(idx % TN);
+    tj = (int)((idx / TN) % TN);
+    ti = (int)(idx / ((int64_t)TN * TN));
+    if (ti >= TN) return false;
+    return color < 0 || ((ti + tj + tk) & 1) == color;
+}
+
+__device__ __forceinline__ int64_t tiles_to_launch(int TN, int color) {
+    const int64_t all = (int64_t)TN * TN * TN;
+    if (color >= 0 && (TN & 1) == 0) return all / 2;
+    return all;
+}
+
+// MIC(0) apply, one pass over the tiles of one colour; 4 tiles per 256-thread
+// block, one wave per tile.
+//   phase 0 (red, forward):       q = L^-1 r inside the tile          -> Q
+//   phase 1 (black, fwd + bwd):   forward with red halos, backward     -> Z, z.r partials
+//   phase 2 (red, backward):      backward with black halos            -> Z, z.r (+finalise)
+//   phase 3 (block, fwd + bwd):   no cross-tile coupling               -> Z, z.r (+finalise)
 // forward:  q_i = (r_i + sum_{j lower} pc_j q_j) pc_i   (order -x,-y,-z,+x,+y,+z)
 // backward: z_i = (q_i + sum_{k upper} pc_i z_k) pc_i   (order +x,+y,+z,-x,-y,-z)
-__global__ __launch_bounds__(64) void k_mic_apply(Params p, const uint8_t* __restrict__ A,
-                                                   const float* __restrict__ pc,
-                                                   const float* __restrict__ r,
-                                                   float* __restrict__ Q, float* __restrict__ Z,
-                                                   Reduce red, int phase, int slot_base,
-                                                   int finalize_total) {
+// Tile data moves through LDS with coalesced accesses: in load round m the 64
+// lanes read the 8 lines with il = m, 8 lanes per 32-byte line segment.
+__global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __restrict__ fl8,
+                                                    const float* __restrict__ pc,
+                                                    const float* __restrict__ r,
+                                                    float* __restrict__ Q, float* __restrict__ Z,
+                                                    Reduce red, int phase, int slot_base,
+                                                    int finalize_total) {
     if (red.sc->done) return;
-    __shared__ float s_a[64 * kPad];    // r, then q
-    __shared__ float s_pc[64 * kPad];
-    __shared__ float s_b[64 * kPad];    // pc*q (forward), then z (backward)
-    __shared__ float s_hx[64 * kPad];
-    __shared__ float s_hy[64 * kPad];
+    __shared__ float lds[kTilesPerBlock][3 * kLine + 2 * kHalo];
     const int N = p.N;
     const int TN = N / kTile;
-    const int64_t ntiles = (int64_t)TN * TN * TN;
+    const int NS = N / kTile;   // segments per z-line
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int color = phase == 3 ? -1 : (phase == 1 ? 1 : 0);
     const bool do_fwd = phase != 2;
     const bool do_bwd = phase != 0;
-    const bool halo_fwd = phase == 1;   // black forward sees red neighbours
-    const bool halo_bwd = phase == 2;   // red backward sees black neighbours
-    double dot[1] = {0.0};
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        TileCtx c;
-        if (!tile_of(N, t, color, c)) continue;
-        const uint32_t fl = fluid_bits(A, c.line);
-        const uint32_t fl_im = __shfl(fl, (c.lane - 8) & 63, 64);
-        const uint32_t fl_jm = __shfl(fl, (c.lane - 1) & 63, 64);
-        const uint32_t fl_ip = __shfl(fl, (c.lane + 8) & 63, 64);
-        const uint32_t fl_jp = __shfl(fl, (c.lane + 1) & 63, 64);
-        float* sa = s_a + c.lane * kPad;
-        float* spc = s_pc + c.lane * kPad;
-        float* sb = s_b + c.lane * kPad;
+    const bool halo_fwd = phase == 1;
+    const bool halo_bwd = phase == 2;
+    float* sa = lds[wave];                // r, then q          [line*9 + kl]
+    float* spc = sa + kLine;              // pc
+    float* sb = spc + kLine;              // pc*q (forward), z (backward)
+    float* hx = sb + kLine;               // x halo [(il==7)*72 + jl*9 + kl]
+    float* hy = hx + kHalo;               // y halo [(jl==7)*72 + il*9 + kl]
+    int ti = 0, tj = 0, tk = 0;
+    const int64_t idx = (int64_t)blockIdx.x * kTilesPerBlock + wave;
+    const bool active = idx < tiles_to_launch(TN, color) && tile_coords(TN, idx, color, ti, tj, tk);
+    const int il = lane >> 3, jl = lane & 7;
+    const int gi = ti * kTile + il, gj = tj * kTile + jl, k0 = tk * kTile;
+    uint32_t fl = 0, fl_im = 0, fl_jm = 0, fl_ip = 0, fl_jp = 0;
+    uint32_t hx_fl = 0, hy_fl = 0, hzm_fl = 0, hzp_fl = 0;
+    float hzm = 0.f, hzp = 0.f;
+    float rreg[kTile];
+    // coalesced mapping: round m covers line (m, lane>>3), element lane&7
+    const int ce = lane & 7, cj = lane >> 3;
+    if (active) {
+        fl = fl8[((int64_t)gi * N + gj) * NS + tk];
 #pragma unroll
-        for (int kl = 0; kl < kTile; ++kl) {
-            spc[kl] = pc[c.line + kl];
-            sa[kl] = do_fwd ? r[c.line + kl] : Q[c.line + kl];
+        for (int m = 0; m < kTile; ++m) {
+            const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
+            const int li = (m * 8 + cj) * kPad + ce;
+            spc[li] = pc[g];
+            rreg[m] = r[g];
+            sa[li] = do_fwd ? rreg[m] : Q[g];
         }
-        // cross-tile halos
-        uint32_t hx_fl = 0, hy_fl = 0, hzm_fl = 0, hzp_fl = 0;
-        float hzm = 0.f, hzp = 0.f;
         if (halo_fwd || halo_bwd) {
-            // forward halo values are pc*Q of red neighbours; backward ones are Z of black
             const float* va = halo_fwd ? pc : Z;
             const float* vb = halo_fwd ? Q : nullptr;
             int xi = -1;
-            if (c.il == 0 && c.gi > 0) xi = c.gi - 1;
-            if (c.il == 7 && c.gi + 1 < N) xi = c.gi + 1;
-            if (xi >= 0) hx_fl = load_halo_line(A, va, vb, cidx(N, xi, c.gj, c.k0), s_hx + c.lane * kPad);
+            if (il == 0 && gi > 0) xi = gi - 1;
+            if (il == 7 && gi + 1 < N) xi = gi + 1;
+            if (xi >= 0) {
+                const int64_t ln = cidx(N, xi, gj, k0);
+                hx_fl = fl8[((int64_t)xi * N + gj) * NS + tk];
+                float* d = hx + (il == 7) * kTile * kPad + jl * kPad;
+                const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
+                const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
+                float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                if (vb) {
+                    const float4 b0 = *reinterpret_cast<const float4*>(vb + ln);
+                    const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
+                    float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = av[kl] * bv[kl];
+                } else {
+#pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = av[kl];
+                }
+            }
             int yj = -1;
-            if (c.jl == 0 && c.gj > 0) yj = c.gj - 1;
-            if (c.jl == 7 && c.gj + 1 < N) yj = c.gj + 1;
-            if (yj >= 0) hy_fl = load_halo_line(A, va, vb, cidx(N, c.gi, yj, c.k0), s_hy + c.lane * kPad);
-            if (c.k0 > 0) {
-                const int64_t n = c.line - 1;
-                hzm_fl = A[n] == kFluid;
-                hzm = vb ? va[n] * vb[n] : va[n];
-            }
-            if (c.k0 + kTile < N) {
-                const int64_t n = c.line + kTile;
-                hzp_fl = A[n] == kFluid;
-                hzp = vb ? va[n] * vb[n] : va[n];
-            }
-        }
-        __syncthreads();
-        if (do_fwd) {
-            for (int st = 0; st < 3 * kTile - 2; ++st) {
-                const int kl = st - c.il - c.jl;
-                if (kl >= 0 && kl < kTile) {
-                    float q = 0.0f;
-                    if ((fl >> kl) & 1u) {
-                        float acc = sa[kl];
-                        if (c.il > 0) {
-                            if ((fl_im >> kl) & 1u) acc = acc + s_b[(c.lane - 8) * kPad + kl];
-                        } else if (halo_fwd && ((hx_fl >> kl) & 1u)) {
-                            acc = acc + s_hx[c.lane * kPad + kl];
-                        }
-                        if (c.jl > 0) {
-                            if ((fl_jm >> kl) & 1u) acc = acc + s_b[(c.lane - 1) * kPad + kl];
-                        } else if (halo_fwd && ((hy_fl >> kl) & 1u)) {
-                            acc = acc + s_hy[c.lane * kPad + kl];
-                        }
-                        if (kl > 0) {
-                            if ((fl >> (kl - 1)) & 1u) acc = acc + sb[kl - 1];
-                        } else if (halo_fwd && hzm_fl) {
-                            acc = acc + hzm;
-                        }
-                        if (halo_fwd && c.il == 7 && ((hx_fl >> kl) & 1u)) acc = acc + s_hx[c.lane * kPad + kl];
-                        if (halo_fwd && c.jl == 7 && ((hy_fl >> kl) & 1u)) acc = acc + s_hy[c.lane * kPad + kl];
-                        if (halo_fwd && kl == 7 && hzp_fl) acc = acc + hzp;
-                        q = acc * spc[kl];
-                    }
-                    sa[kl] = q;
-                    sb[kl] = spc[kl] * q;
-                }
-                __syncthreads();
-            }
-            if (!do_bwd) {
+            if (jl == 0 && gj > 0) yj = gj - 1;
+            if (jl == 7 && gj + 1 < N) yj = gj + 1;
+            if (yj >= 0) {
+                const int64_t ln = cidx(N, gi, yj, k0);
+                hy_fl = fl8[((int64_t)gi * N + yj) * NS + tk];
+                float* d = hy + (jl == 7) * kTile * kPad + il * kPad;
+                const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
+                const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
+                float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                if (vb) {
+                    const float4 b0 = *reinterpret_cast<const float4*>(vb + ln);
+                    const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
+                    float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
-                for (int kl = 0; kl < kTile; ++kl) Q[c.line + kl] = sa[kl];
-            }
-        }
-        if (do_bwd) {
-            for (int st = 0; st < 3 * kTile - 2; ++st) {
-                const int m = st - (7 - c.il) - (7 - c.jl);
-                const int kl = 7 - m;
-                if (m >= 0 && m < kTile) {
-                    float z = 0.0f;
-                    if ((fl >> kl) & 1u) {
-                        const float pci = spc[kl];
-                        float acc = sa[kl];
-                        // +x
-                        if (c.il < 7) {
-                            if ((fl_ip >> kl) & 1u) acc = acc + pci * s_b[(c.lane + 8) * kPad + kl];
-                        } else if (halo_bwd && ((hx_fl >> kl) & 1u)) {
-                            acc = acc + pci * s_hx[c.lane * kPad + kl];
-                        }
-                        // +y
-                        if (c.jl < 7) {
-                            if ((fl_jp >> kl) & 1u) acc = acc + pci * s_b[(c.lane + 1) * kPad + kl];
-                        } else if (halo_bwd && ((hy_fl >> kl) & 1u)) {
-                            acc = acc + pci * s_hy[c.lane * kPad + kl];
-                        }
-                        // +z
-                        if (kl < 7) {
-                            if ((fl >> (kl + 1)) & 1u) acc = acc + pci * sb[kl + 1];
-                        } else if (halo_bwd && hzp_fl) {
-                            acc = acc + pci * hzp;
-                        }
-                        // -x, -y, -z across the tile boundary (red backward only)
-                        if (halo_bwd && c.il == 0 && ((hx_fl >> kl) & 1u)) acc = acc + pci * s_hx[c.lane * kPad + kl];
-                        if (halo_bwd && c.jl == 0 && ((hy_fl >> kl) & 1u)) acc = acc + pci * s_hy[c.lane * kPad + kl];
-                        if (halo_bwd && kl == 0 && hzm_fl) acc = acc + pci * hzm;
-                        z = acc * pci;
-                    }
-                    sb[kl] = z;
-                }
-                __syncthreads();
-            }
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = av[kl] * bv[kl];
+                } else {
 #pragma unroll
-            for (int kl = 0; kl < kTile; ++kl) {
-                const float z = sb[kl];
-                Z[c.line + kl] = z;
-                dot[0] += (double)z * (double)r[c.line + kl];
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = av[kl];
+                }
+            }
+            const int64_t line = cidx(N, gi, gj, k0);
+            if (k0 > 0) {
+                hzm_fl = (fl8[((int64_t)gi * N + gj) * NS + tk - 1] >> 7) & 1u;
+                hzm = vb ? va[line - 1] * vb[line - 1] : va[line - 1];
+            }
+            if (k0 + kTile < N) {
+                hzp_fl = fl8[((int64_t)gi * N + gj) * NS + tk + 1] & 1u;
+                hzp = vb ? va[line + kTile] * vb[line + kTile] : va[line + kTile];
             }
         }
-        __syncthreads();
+    }
+    fl_im = __shfl(fl, (lane - 8) & 63, 64);
+    fl_jm = __shfl(fl, (lane - 1) & 63, 64);
+    fl_ip = __shfl(fl, (lane + 8) & 63, 64);
+    fl_jp = __shfl(fl, (lane + 1) & 63, 64);
+    __syncthreads();
+    const int me = lane * kPad;
+    if (do_fwd) {
+        for (int st = 0; st < 3 * kTile - 2; ++st) {
+            const int kl = st - il - jl;
+            if (active && kl >= 0 && kl < kTile) {
+                float q = 0.0f;
+                if ((fl >> kl) & 1u) {
+                    float acc = sa[me + kl];
+                    if (il > 0) {
+                        if ((fl_im >> kl) & 1u) acc = acc + sb[me - 8 * kPad + kl];
+                    } else if (halo_fwd && ((hx_fl >> kl) & 1u)) {
+                        acc = acc + hx[jl * kPad + kl];
+                    }
+                    if (jl > 0) {
+                        if ((fl_jm >> kl) & 1u) acc = acc + sb[me - kPad + kl];
+                    } else if (halo_fwd && ((hy_fl >> kl) & 1u)) {
+                        acc = acc + hy[il * kPad + kl];
+                    }
+                    if (kl > 0) {
+                        if ((fl >> (kl - 1)) & 1u) acc = acc + sb[me + kl - 1];
+                    } else if (halo_fwd && hzm_fl) {
+                        acc = acc + hzm;
+                    }
+                    if (halo_fwd && il == 7 && ((hx_fl >> kl) & 1u)) acc = acc + hx[kTile * kPad + jl * kPad + kl];
+                    if (halo_fwd && jl == 7 && ((hy_fl >> kl) & 1u)) acc = acc + hy[kTile * kPad + il * kPad + kl];
+                    if (halo_fwd && kl == 7 && hzp_fl) acc = acc + hzp;
+                    q = acc * spc[me + kl];
+                }
+                sa[me + kl] = q;
+                sb[me + kl] = spc[me + kl] * q;
+            }
+            __syncthreads();
+        }
+    }
+    if (do_bwd) {
+        for (int st = 0; st < 3 * kTile - 2; ++st) {
+            const int m = st - (7 - il) - (7 - jl);
+            const int kl = 7 - m;
+            if (active && m >= 0 && m < kTile) {
+                float z = 0.0f;
+                if ((fl >> kl) & 1u) {
+                    const float pci = spc[me + kl];
+                    float acc = sa[me + kl];
+                    if (il < 7) {
+                        if ((fl_ip >> kl) & 1u) acc = acc + pci * sb[me + 8 * kPad + kl];
+                    } else if (halo_bwd && ((hx_fl >> kl) & 1u)) {
+                        acc = acc + pci * hx[kTile * kPad + jl * kPad + kl];
+                    }
+                    if (jl < 7) {
+                        if ((fl_jp >> kl) & 1u) acc = acc + pci * sb[me + kPad + kl];
+                    } else if (halo_bwd && ((hy_fl >> kl) & 1u)) {
+                        acc = acc + pci * hy[kTile * kPad + il * kPad + kl];
+                    }
+                    if (kl < 7) {
+                        if ((fl >> (kl + 1)) & 1u) acc = acc + pci * sb[me + kl + 1];
+                    } else if (halo_bwd && hzp_fl) {
+                        acc = acc + pci * hzp;
+                    }
+                    if (halo_bwd && il == 0 && ((hx_fl >> kl) & 1u)) acc = acc + pci * hx[jl * kPad + kl];
+                    if (halo_bwd && jl == 0 && ((hy_fl >> kl) & 1u)) acc = acc + pci * hy[il * kPad + kl];
+                    if (halo_bwd && kl == 0 && hzm_fl) acc = acc + pci * hzm;
+                    z = acc * pci;
+                }
+                sb[me + kl] = z;
+            }
+            __syncthreads();
+        }
+    }
+    // coalesced write-back
+    double dot[1] = {0.0};
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < kTile; ++m) {
+            const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
+            const int li = (m * 8 + cj) * kPad + ce;
+            if (do_bwd) {
+                const float z = sb[li];
+                Z[g] = z;
+                dot[0] += (double)z * (double)rreg[m];
+            } else {
+                Q[g] = sa[li];
+            }
+        }
     }
     if (phase == 0) return;
     if (phase == 1) {
-        // partials only; the red backward pass finalises
-        double t = wave_sum(dot[0]);
+        double t = block_sum_only(dot[0]);
         if (threadIdx.x == 0) red.partials[slot_base + blockIdx.x] = t;
         return;
     }
     if (block_arrive<1>(dot, red.partials, slot_base + blockIdx.x, red.counter, gridDim.x)) {
-        double s[1];
-        sum_partials<1>(red.partials, finalize_total, s, red.counter);
-        if (threadIdx.x == 0) finish_rho(red.sc, s[0]);
+        double sres[1];
+        sum_partials<1>(red.partials, finalize_total, sres, red.counter);
+        if (threadIdx.x == 0) finish_rho(red.sc, sres[0]);
     }
 }
 
@@ -860,7 +962,12 @@ void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, cons
     }
 }
 
-void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* A, const float* pc,
+void launch_fluid_segments(hipStream_t s, int N, const uint8_t* A, uint8_t* fl8) {
+    const int64_t nseg = (int64_t)N * N * (N / kTile);
+    hipLaunchKernelGGL(k_fluid_segments, dim3(grid_for(nseg, kBlock)), dim3(kBlock), 0, s, N, A, fl8);
+}
+
+void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, const float* pc,
                           const float* r, float* q, float* z, const Reduce& red, int) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
     if (p.precond == 2) {
@@ -868,14 +975,18 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* A, cons
                            z, red);
         return;
     }
-    const int nb = tile_blocks(p.N);
+    const int TN = p.N / kTile;
     if (p.precond == 0) {
-        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(64), 0, s, p, A, pc, r, q, z, red, 0, 0, 0);
-        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(64), 0, s, p, A, pc, r, q, z, red, 1, 0, 0);
-        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(64), 0, s, p, A, pc, r, q, z, red, 2, nb,
+        const int64_t per_color = (TN & 1) == 0 ? (int64_t)TN * TN * TN / 2 : (int64_t)TN * TN * TN;
+        const int nb = (int)((per_color + kTilesPerBlock - 1) / kTilesPerBlock);
+        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 0, 0, 0);
+        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 1, 0, 0);
+        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 2, nb,
                            2 * nb);
     } else {
-        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(64), 0, s, p, A, pc, r, q, z, red, 3, 0, nb);
+        const int64_t all = (int64_t)TN * TN * TN;
+        const int nb = (int)((all + kTilesPerBlock - 1) / kTilesPerBlock);
+        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 3, 0, nb);
     }
 }
 
