@@ -611,7 +611,7 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->counts, h->nc)) || (rc = dalloc(h, &h->offsets, h->nc + 1)) ||
         (rc = dalloc(h, &h->scan_sums, h->nc / 4096 + 4)) ||
         (rc = dalloc(h, &h->P64, h->d.refine_every > 0 ? h->nc : 1)) ||
-        (rc = dalloc(h, &h->partials, 2 * (int64_t)kMaxPartials)) || (rc = dalloc(h, &h->counter, 1)) ||
+        (rc = dalloc(h, &h->partials, 2 * (int64_t)kMaxPartials)) || (rc = dalloc(h, &h->counter, kCounterWords)) ||
         (rc = dalloc(h, &h->sc, 1)))
         return bail(rc);
     if (hipHostMalloc((void**)&h->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)
@@ -637,7 +637,7 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
     launch_fill_i32(s, h->offsets, h->nc + 1, 0);
     if (hipMemsetAsync(h->A, 0, (size_t)h->nc, s) != hipSuccess ||
         hipMemsetAsync(h->nup, 0, (size_t)h->nc, s) != hipSuccess ||
-        hipMemsetAsync(h->counter, 0, sizeof(uint32_t), s) != hipSuccess ||
+        hipMemsetAsync(h->counter, 0, sizeof(uint32_t) * kCounterWords, s) != hipSuccess ||
         hipMemsetAsync(h->sc, 0, sizeof(SolverScalars), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return bail(fail(nullptr, ARIEL_ERR_HIP, "device initialisation failed"));

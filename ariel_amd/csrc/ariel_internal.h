@@ -18,6 +18,7 @@ namespace ariel {
 constexpr int kAir = 0, kFluid = 1, kSolid = 2;
 constexpr int kBlock = 256;      // default threads per block of streaming kernels
 constexpr int kMaxPartials = 65536;
+constexpr int kCounterWords = 33 * 32;   // arrival counters of the reductions
 
 // Reduction/solver scalars living in device memory. Written by the last block of
 // a reduction ("last block finalises"), read by later kernels; no host round
@@ -63,7 +64,7 @@ struct Params {
 // Reduction workspace: per-block double partials + an arrival counter.
 struct Reduce {
     double* partials;      // kMaxPartials * 2
-    uint32_t* counter;     // 1
+    uint32_t* counter;     // 33 arrival counters, 128 bytes apart (zero between reductions)
     SolverScalars* sc;     // device scalars
 };
 

@@ -29,14 +29,15 @@ constexpr int kPad = 9;            // LDS stride per z-line (bank-conflict paddi
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // ------------------------------------------------------------ reductions
-// Block-reduce NV doubles; thread 0 stores the block partials at part[slot*NV+v],
-// fences, and bumps the counter. Returns true (block-uniform) in the block that
-// arrives last among `expected` arrivals.
+// Each reducing kernel stores one FP64 partial per block and value
+// (part[slot*NV + v], plain stores); a one-block finalize kernel launched after
+// it sums the partials in a fixed order and updates the solver scalars. The
+// kernel boundary makes the partials visible. (A "last block finalises" scheme
+// needs a device-scope release per block, which on gfx950 writes back the L2 of
+// the block's XCD each time; it made every reducing kernel several times slower.)
 template <int NV>
-__device__ bool block_arrive(double (&v)[NV], double* part, int slot, uint32_t* counter,
-                             int expected) {
+__device__ void block_partials(double (&v)[NV], double* part, int slot) {
     __shared__ double sm[NV][16];
-    __shared__ int is_last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -51,54 +52,7 @@ __device__ bool block_arrive(double (&v)[NV], double* part, int slot, uint32_t* 
             for (int ww = 0; ww < nw; ++ww) t += sm[i][ww];
             part[slot * NV + i] = t;
         }
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == (unsigned)(expected - 1));
     }
-    __syncthreads();
-    return is_last != 0;
-}
-
-// In the last block: sum part[0..n) per value in a fixed order (deterministic).
-template <int NV>
-__device__ void sum_partials(const double* part, int n, double (&out)[NV], uint32_t* counter) {
-    __threadfence();
-    __shared__ double sm[NV][16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    double acc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-    for (int b = threadIdx.x; b < n; b += blockDim.x) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
-            acc[i] += __hip_atomic_load(&part[b * NV + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double t = wave_sum(acc[i]);
-        if (lane == 0) sm[i][w] = t;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double t = 0.0;
-        for (int ww = 0; ww < nw; ++ww) t += sm[i][ww];
-        out[i] = t;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-}
-
-// Block-level FP64 sum, valid in thread 0.
-__device__ double block_sum_only(double v) {
-    __shared__ double sm[16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    double t = wave_sum(v);
-    if (lane == 0) sm[w] = t;
-    __syncthreads();
-    double acc = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < nw; ++i) acc += sm[i];
-    return acc;
 }
 
 __device__ __forceinline__ bool finite(double x) { return isfinite(x); }
@@ -228,61 +182,94 @@ __global__ void k_residual0(Params p, const uint8_t* __restrict__ A, const float
             v[1] += (double)r * (double)r;
         }
     }
-    if (block_arrive<2>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
-        double s[2];
-        sum_partials<2>(red.partials, gridDim.x, s, red.counter);
-        if (threadIdx.x == 0) {
-            SolverScalars* sc = red.sc;
-            sc->bnorm2 = s[0];
-            sc->rnorm2 = s[1];
-            if (stop_rule == 1) sc->thr = (double)(1.0e-2f * (float)(N * N * N));
-            else sc->thr = tol * tol * s[0];
-            if (s[1] <= sc->thr) {
-                sc->done = 1;
-                sc->status = 0;
-            }
-        }
-    }
+    block_partials<2>(v, red.partials, blockIdx.x);
 }
 
-// z = A s, sigma = z.s, alpha = rho / sigma.
+// One XRef term for a neighbour with class t, value xq, level set lq; cell value
+// xc with level set lc (solver.inl:138-154).
+__device__ __forceinline__ float xref_term(uint8_t t, float xq, const float* __restrict__ L,
+                                           int64_t q, float lc, float xc) {
+    if (t == kFluid) return xq;
+    if (t == kSolid) return xc;
+    return L[q] / gmin(1.0e-6f, lc) * xc;
+}
+
+// z = A s, sigma = z.s, alpha = rho / sigma. Each thread evaluates 4 consecutive
+// cells along z with vector loads (x and the classes of the cell quad and of its
+// +-y and +-x neighbour quads) and the reference expression per cell.
 __global__ void k_spmv_dot(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
                            const float* __restrict__ x, float* __restrict__ y, Reduce red) {
     if (red.sc->done) return;
     const int N = p.N;
     const int64_t nc = (int64_t)N * N * N;
+    const int64_t n4 = nc / 4;
+    const int64_t NN = (int64_t)N * N;
     const float n = (float)N;
     const float h = 1.0f / (n * n);
+    const float4* __restrict__ X4 = reinterpret_cast<const float4*>(x);
+    const uchar4* __restrict__ A4 = reinterpret_cast<const uchar4*>(A);
+    float4* __restrict__ Y4 = reinterpret_cast<float4*>(y);
     double v[1] = {0.0};
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        if (A[c] != kFluid) {
-            y[c] = 0.0f;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = 4 * t;
+        const uchar4 ac = A4[t];
+        if (ac.x != kFluid && ac.y != kFluid && ac.z != kFluid && ac.w != kFluid) {
+            Y4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
             continue;
         }
-        int i, j, k;
-        decode_cell(N, c, i, j, k);
-        const float yc = spmv_cell(A, L, x, N, i, j, k, c, h);
-        y[c] = yc;
-        v[0] += (double)yc * (double)x[c];
-    }
-    if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
-        double s[1];
-        sum_partials<1>(red.partials, gridDim.x, s, red.counter);
-        if (threadIdx.x == 0) {
-            SolverScalars* sc = red.sc;
-            sc->sigma = s[0];
-            if (!finite(s[0]) || !finite(sc->rho)) {
-                sc->status = 3;
-                sc->done = 1;
-            } else if (!(s[0] > 0.0)) {
-                sc->status = 2;
-                sc->done = 1;
-            } else {
-                sc->alpha = sc->rho / s[0];
+        const int k0 = (int)(c % N);
+        const int j = (int)((c / N) % N);
+        const int i = (int)(c / NN);
+        const float4 xc4 = X4[t];
+        const float xs[4] = {xc4.x, xc4.y, xc4.z, xc4.w};
+        const uint8_t as[4] = {ac.x, ac.y, ac.z, ac.w};
+        // neighbour quads (clamped to self at the domain boundary)
+        const bool hxp = i + 1 < N, hxm = i > 0, hyp = j + 1 < N, hym = j > 0;
+        float4 xxp = xc4, xxm = xc4, xyp = xc4, xym = xc4;
+        uchar4 axp = ac, axm = ac, ayp = ac, aym = ac;
+        if (hxp) { xxp = X4[t + NN / 4]; axp = A4[t + NN / 4]; }
+        if (hxm) { xxm = X4[t - NN / 4]; axm = A4[t - NN / 4]; }
+        if (hyp) { xyp = X4[t + N / 4]; ayp = A4[t + N / 4]; }
+        if (hym) { xym = X4[t - N / 4]; aym = A4[t - N / 4]; }
+        float xzm = 0.f, xzp = 0.f;
+        uint8_t azm = kSolid, azp = kSolid;
+        if (k0 > 0) { xzm = x[c - 1]; azm = A[c - 1]; }
+        if (k0 + 4 < N) { xzp = x[c + 4]; azp = A[c + 4]; }
+        const float vxp[4] = {xxp.x, xxp.y, xxp.z, xxp.w}, vxm[4] = {xxm.x, xxm.y, xxm.z, xxm.w};
+        const float vyp[4] = {xyp.x, xyp.y, xyp.z, xyp.w}, vym[4] = {xym.x, xym.y, xym.z, xym.w};
+        const uint8_t txp[4] = {axp.x, axp.y, axp.z, axp.w}, txm[4] = {axm.x, axm.y, axm.z, axm.w};
+        const uint8_t typ[4] = {ayp.x, ayp.y, ayp.z, ayp.w}, tym[4] = {aym.x, aym.y, aym.z, aym.w};
+        float out[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            if (as[m] != kFluid) {
+                out[m] = 0.0f;
+                continue;
             }
+            const int64_t cm = c + m;
+            const float xcm = xs[m];
+            const bool need_l = txp[m] == kAir || txm[m] == kAir || typ[m] == kAir || tym[m] == kAir ||
+                                (m < 3 ? as[m + 1] == kAir : azp == kAir) ||
+                                (m > 0 ? as[m - 1] == kAir : azm == kAir);
+            const float lc = need_l ? L[cm] : 0.0f;
+            // +x, -x, +y, -y: a missing neighbour clamps to the cell itself (FLUID)
+            const float t1 = hxp ? xref_term(txp[m], vxp[m], L, cm + NN, lc, xcm) : xcm;
+            const float t2 = hxm ? xref_term(txm[m], vxm[m], L, cm - NN, lc, xcm) : xcm;
+            const float t3 = hyp ? xref_term(typ[m], vyp[m], L, cm + N, lc, xcm) : xcm;
+            const float t4 = hym ? xref_term(tym[m], vym[m], L, cm - N, lc, xcm) : xcm;
+            float t5, t6;
+            if (m < 3) t5 = xref_term(as[m + 1], xs[m + 1], L, cm + 1, lc, xcm);
+            else t5 = (k0 + 4 < N) ? xref_term(azp, xzp, L, cm + 1, lc, xcm) : xcm;
+            if (m > 0) t6 = xref_term(as[m - 1], xs[m - 1], L, cm - 1, lc, xcm);
+            else t6 = (k0 > 0) ? xref_term(azm, xzm, L, cm - 1, lc, xcm) : xcm;
+            const float yc = (6.0f * xcm - t1 - t2 - t3 - t4 - t5 - t6) / h;
+            out[m] = yc;
+            v[0] += (double)yc * (double)xcm;
         }
+        Y4[t] = make_float4(out[0], out[1], out[2], out[3]);
     }
+    block_partials<1>(v, red.partials, blockIdx.x);
 }
 
 // Stop decision after the residual norm of iteration `it` is known.
@@ -304,28 +291,41 @@ __device__ __forceinline__ bool refine_due(const SolverScalars* sc) {
 }
 
 // p += alpha s; r += (-alpha) z (Op, solver.inl:157-177); ||r||^2; stop test.
+// float4 per thread, two loads of each operand in flight per iteration.
 __global__ void k_axpy2(int64_t n, float* __restrict__ P, const float* __restrict__ S,
                         float* __restrict__ R, const float* __restrict__ Z, Reduce red) {
     if (red.sc->done) return;
     const float af = (float)red.sc->alpha;
+    const float naf = -af;
+    const int64_t n4 = n / 4;
+    float4* __restrict__ P4 = reinterpret_cast<float4*>(P);
+    float4* __restrict__ R4 = reinterpret_cast<float4*>(R);
+    const float4* __restrict__ S4 = reinterpret_cast<const float4*>(S);
+    const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
     double v[1] = {0.0};
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        P[c] = P[c] + af * S[c];
-        const float r = R[c] + (-af) * Z[c];
-        R[c] = r;
-        v[0] += (double)r * (double)r;
-    }
-    if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
-        double s[1];
-        sum_partials<1>(red.partials, gridDim.x, s, red.counter);
-        if (threadIdx.x == 0) {
-            SolverScalars* sc = red.sc;
-            sc->rnorm2 = s[0];
-            sc->it += 1;
-            if (!refine_due(sc)) decide(sc);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n4; c += 2 * stride) {
+        const int64_t c2 = c + stride;
+        const bool two = c2 < n4;
+        float4 p0 = P4[c], s0 = S4[c], r0 = R4[c], z0 = Z4[c];
+        float4 p1, s1, r1, z1;
+        if (two) { p1 = P4[c2]; s1 = S4[c2]; r1 = R4[c2]; z1 = Z4[c2]; }
+        p0.x = p0.x + af * s0.x; p0.y = p0.y + af * s0.y; p0.z = p0.z + af * s0.z; p0.w = p0.w + af * s0.w;
+        r0.x = r0.x + naf * z0.x; r0.y = r0.y + naf * z0.y; r0.z = r0.z + naf * z0.z; r0.w = r0.w + naf * z0.w;
+        P4[c] = p0;
+        R4[c] = r0;
+        v[0] += (double)r0.x * (double)r0.x + (double)r0.y * (double)r0.y +
+                (double)r0.z * (double)r0.z + (double)r0.w * (double)r0.w;
+        if (two) {
+            p1.x = p1.x + af * s1.x; p1.y = p1.y + af * s1.y; p1.z = p1.z + af * s1.z; p1.w = p1.w + af * s1.w;
+            r1.x = r1.x + naf * z1.x; r1.y = r1.y + naf * z1.y; r1.z = r1.z + naf * z1.z; r1.w = r1.w + naf * z1.w;
+            P4[c2] = p1;
+            R4[c2] = r1;
+            v[0] += (double)r1.x * (double)r1.x + (double)r1.y * (double)r1.y +
+                    (double)r1.z * (double)r1.z + (double)r1.w * (double)r1.w;
         }
     }
+    block_partials<1>(v, red.partials, blockIdx.x);
 }
 
 // Reliable update, part 1: x64 += p, p = 0 (every refine_every iterations).
@@ -363,19 +363,7 @@ __global__ void k_true_residual(Params p, const uint8_t* __restrict__ A, const f
         v[0] += r * r;
         if (mode == 0) R[c] = (float)r;
     }
-    if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
-        double s[1];
-        sum_partials<1>(red.partials, gridDim.x, s, red.counter);
-        if (threadIdx.x == 0) {
-            SolverScalars* sc = red.sc;
-            if (mode == 1) {
-                sc->tmp = s[0];
-            } else {
-                sc->rnorm2 = s[0];
-                decide(sc);
-            }
-        }
-    }
+    block_partials<1>(v, red.partials, blockIdx.x);
 }
 
 // End of a refined solve: p = (float)(x64 + p) on FLUID cells, 0 elsewhere.
@@ -390,9 +378,19 @@ __global__ void k_update_s(int64_t n, const float* __restrict__ Z, float* __rest
                            const SolverScalars* sc, int first) {
     if (sc->done) return;
     const float bf = (float)sc->beta;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        S[c] = first ? Z[c] : Z[c] + bf * S[c];
+    const int64_t n4 = n / 4;
+    const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
+    float4* __restrict__ S4 = reinterpret_cast<float4*>(S);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n4; c += stride) {
+        const float4 z = Z4[c];
+        if (first) {
+            S4[c] = z;
+        } else {
+            float4 sv = S4[c];
+            sv.x = z.x + bf * sv.x; sv.y = z.y + bf * sv.y; sv.z = z.z + bf * sv.z; sv.w = z.w + bf * sv.w;
+            S4[c] = sv;
+        }
     }
 }
 
@@ -420,6 +418,79 @@ __device__ __forceinline__ void finish_rho(SolverScalars* sc, double rho_new) {
     } else {
         sc->beta = rho_new / sc->rho;
         sc->rho = rho_new;
+    }
+}
+
+// ------------------------------------------------------------ finalize
+enum { FIN_RESID0 = 0, FIN_SIGMA = 1, FIN_RNORM = 2, FIN_TRUE = 3, FIN_REPORT = 4, FIN_RHO = 5 };
+
+// One block: sum the partials of the preceding reducing kernel in a fixed order
+// and update the scalars. Gated exactly like its producer, so a skipped producer
+// (solve finished) is matched by a skipped finalize.
+__global__ __launch_bounds__(256) void k_finalize(Reduce red, int kind, int nparts, double tol,
+                                                  int stop_rule, int N) {
+    SolverScalars* sc = red.sc;
+    if ((kind == FIN_SIGMA || kind == FIN_RNORM || kind == FIN_RHO) && sc->done) return;
+    if (kind == FIN_TRUE && (sc->done || !refine_due(sc))) return;
+    const int NV = kind == FIN_RESID0 ? 2 : 1;
+    double a0 = 0.0, a1 = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        a0 += red.partials[b * NV];
+        if (NV == 2) a1 += red.partials[b * NV + 1];
+    }
+    __shared__ double sm[2][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    if (lane == 0) {
+        sm[0][w] = a0;
+        sm[1][w] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        s0 += sm[0][i];
+        s1 += sm[1][i];
+    }
+    switch (kind) {
+        case FIN_RESID0:
+            sc->bnorm2 = s0;
+            sc->rnorm2 = s1;
+            if (stop_rule == 1) sc->thr = (double)(1.0e-2f * (float)(N * N * N));
+            else sc->thr = tol * tol * s0;
+            if (s1 <= sc->thr) {
+                sc->done = 1;
+                sc->status = 0;
+            }
+            break;
+        case FIN_SIGMA:
+            sc->sigma = s0;
+            if (!finite(s0) || !finite(sc->rho)) {
+                sc->status = 3;
+                sc->done = 1;
+            } else if (!(s0 > 0.0)) {
+                sc->status = 2;
+                sc->done = 1;
+            } else {
+                sc->alpha = sc->rho / s0;
+            }
+            break;
+        case FIN_RNORM:
+            sc->rnorm2 = s0;
+            sc->it += 1;
+            if (!refine_due(sc)) decide(sc);
+            break;
+        case FIN_TRUE:
+            sc->rnorm2 = s0;
+            decide(sc);
+            break;
+        case FIN_REPORT:
+            sc->tmp = s0;
+            break;
+        case FIN_RHO:
+            finish_rho(sc, s0);
+            break;
     }
 }
 
@@ -894,16 +965,7 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
         }
     }
     if (phase == 0) return;
-    if (phase == 1) {
-        double t = block_sum_only(dot[0]);
-        if (threadIdx.x == 0) red.partials[slot_base + blockIdx.x] = t;
-        return;
-    }
-    if (block_arrive<1>(dot, red.partials, slot_base + blockIdx.x, red.counter, gridDim.x)) {
-        double sres[1];
-        sum_partials<1>(red.partials, finalize_total, sres, red.counter);
-        if (threadIdx.x == 0) finish_rho(red.sc, sres[0]);
-    }
+    block_partials<1>(dot, red.partials, slot_base + blockIdx.x);
 }
 
 // Jacobi: z = r * (1/diag), rho = z.r
@@ -917,11 +979,7 @@ __global__ void k_jacobi_apply(int64_t n, const float* __restrict__ pc, const fl
         Z[c] = z;
         v[0] += (double)z * (double)r[c];
     }
-    if (block_arrive<1>(v, red.partials, blockIdx.x, red.counter, gridDim.x)) {
-        double s[1];
-        sum_partials<1>(red.partials, gridDim.x, s, red.counter);
-        if (threadIdx.x == 0) finish_rho(red.sc, s[0]);
-    }
+    block_partials<1>(v, red.partials, blockIdx.x);
 }
 
 int stream_blocks(int64_t n) {
@@ -938,6 +996,11 @@ int tile_blocks(int N) {
 }  // namespace
 
 int reduce_blocks(int64_t n, int) { return stream_blocks(n); }
+
+void launch_finalize(hipStream_t s, const Reduce& red, int kind, int nparts, double tol = 0.0,
+                     int stop_rule = 0, int N = 0) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, red, kind, nparts, tol, stop_rule, N);
+}
 
 void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every) {
     hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, s, sc, max_iters, refine_every);
@@ -971,8 +1034,9 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, co
                           const float* r, float* q, float* z, const Reduce& red, int) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
     if (p.precond == 2) {
-        hipLaunchKernelGGL(k_jacobi_apply, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, nc, pc, r,
-                           z, red);
+        const int nb = stream_blocks(nc);
+        hipLaunchKernelGGL(k_jacobi_apply, dim3(nb), dim3(kBlock), 0, s, nc, pc, r, z, red);
+        launch_finalize(s, red, FIN_RHO, nb);
         return;
     }
     const int TN = p.N / kTile;
@@ -983,10 +1047,12 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, co
         hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 1, 0, 0);
         hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 2, nb,
                            2 * nb);
+        launch_finalize(s, red, FIN_RHO, 2 * nb);
     } else {
         const int64_t all = (int64_t)TN * TN * TN;
         const int nb = (int)((all + kTilesPerBlock - 1) / kTilesPerBlock);
         hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 3, 0, nb);
+        launch_finalize(s, red, FIN_RHO, nb);
     }
 }
 
@@ -994,27 +1060,35 @@ void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const fl
                       const float* D, float* R, const Reduce& red, double tol, int stop_rule,
                       double* P64, int refine) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_residual0, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P, D, R,
-                       red, tol, stop_rule, P64, refine);
+    const int nb = stream_blocks(nc);
+    hipLaunchKernelGGL(k_residual0, dim3(nb), dim3(kBlock), 0, s, p, A, L, P, D, R, red, tol,
+                       stop_rule, P64, refine);
+    launch_finalize(s, red, FIN_RESID0, nb, tol, stop_rule, p.N);
 }
 
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                      const float* x, float* y, const Reduce& red) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_spmv_dot, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, x, y, red);
+    const int nb = stream_blocks(nc / 4);
+    hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(kBlock), 0, s, p, A, L, x, y, red);
+    launch_finalize(s, red, FIN_SIGMA, nb);
 }
 
 void launch_axpy2(hipStream_t s, int64_t n, float* P, const float* S, float* R, const float* Z,
                   const Reduce& red) {
-    hipLaunchKernelGGL(k_axpy2, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, P, S, R, Z, red);
+    const int nb = stream_blocks(n / 8);
+    hipLaunchKernelGGL(k_axpy2, dim3(nb), dim3(kBlock), 0, s, n, P, S, R, Z, red);
+    launch_finalize(s, red, FIN_RNORM, nb);
 }
 
 void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                             float* P, double* P64, const float* D, float* R, const Reduce& red) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
+    const int nb = stream_blocks(nc);
     hipLaunchKernelGGL(k_flush, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, nc, A, P, P64, red.sc);
-    hipLaunchKernelGGL(k_true_residual, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L,
+    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, p, A, L,
                        (const float*)P, (const double*)P64, D, R, red, 0);
+    launch_finalize(s, red, FIN_TRUE, nb);
 }
 
 void launch_refine_finish(hipStream_t s, int64_t n, const uint8_t* A, float* P, const double* P64) {
@@ -1024,13 +1098,15 @@ void launch_refine_finish(hipStream_t s, int64_t n, const uint8_t* A, float* P, 
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                                const float* P, const float* D, const Reduce& red) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_true_residual, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, p, A, L, P,
+    const int nb = stream_blocks(nc);
+    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, p, A, L, P,
                        (const double*)nullptr, D, (float*)nullptr, red, 1);
+    launch_finalize(s, red, FIN_REPORT, nb);
 }
 
 void launch_update_s(hipStream_t s, int64_t n, const float* Z, float* S, const Reduce& red,
                      int first) {
-    hipLaunchKernelGGL(k_update_s, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, Z, S, red.sc,
+    hipLaunchKernelGGL(k_update_s, dim3(stream_blocks(n / 4)), dim3(kBlock), 0, s, n, Z, S, red.sc,
                        first);
 }
 

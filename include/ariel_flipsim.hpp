@@ -1,0 +1,128 @@
+// ariel_flipsim.hpp — header-only C++ facade over the C ABI with the shape of
+// Ariel's fluidCore::FlipSim (src/sim/flip.hpp:20-34), so that the reference's
+// callers (main.cpp:52-54 constructs it; viewer.cpp:66-86 calls Init() once and
+// Step() in a loop, reading GetParticles() after each step) keep their calls.
+//
+// Differences that a caller sees:
+//  * the scene is an ariel_scene_desc (liquid primitives, forces, seed) instead
+//    of sceneCore::Scene; JSON loading and mesh geometry stay with the caller;
+//  * GetParticles() returns a snapshot by value (device memory stays on the GPU)
+//    in particle-vector order, not a pointer to live particles;
+//  * failures throw ariel::Error carrying the ABI status and message.
+#ifndef ARIEL_FLIPSIM_HPP
+#define ARIEL_FLIPSIM_HPP
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ariel.h"
+
+namespace ariel {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+// One particle as the reference's Particle exposes it (macgrid.inl:36-50):
+// position, velocity, density; type is always FLUID here.
+struct Particle {
+    float p[3];
+    float u[3];
+    float density;
+};
+
+struct Dimensions {
+    int x, y, z;
+};
+
+class FlipSim {
+public:
+    // FlipSim(maxres, density, stepsize, scene, verbose) (flip.cpp:15-30):
+    // dims, density and dt live in the descriptor.
+    explicit FlipSim(const ariel_scene_desc& desc) : m_frame(0) {
+        check(ariel_create(&desc, &h_), nullptr);
+    }
+    ~FlipSim() { ariel_destroy(h_); }
+    FlipSim(const FlipSim&) = delete;
+    FlipSim& operator=(const FlipSim&) = delete;
+
+    void Init() {
+        check(ariel_init(h_), h_);
+        sync_frame();
+    }
+
+    // Step(saveVDB, saveOBJ, savePARTIO) (flip.cpp:92-128). Export writes the
+    // particle level set to `export_path` when saveVDB is set.
+    void Step(bool saveVDB = false, bool saveOBJ = false, bool savePARTIO = false,
+              const char* export_path = nullptr) {
+        (void)saveOBJ;
+        (void)savePARTIO;
+        check(ariel_step(h_, 1), h_);
+        sync_frame();
+        if (saveVDB && export_path) check(ariel_export_levelset(h_, export_path), h_);
+    }
+
+    std::vector<Particle> GetParticles() {
+        int64_t n = 0;
+        check(ariel_particle_count(h_, &n), h_);
+        std::vector<float> pos(3 * n), vel(3 * n), rho(n);
+        check(ariel_get_particles(h_, pos.data(), vel.data(), rho.data(), n, &n), h_);
+        std::vector<Particle> out((size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                out[i].p[c] = pos[3 * i + c];
+                out[i].u[c] = vel[3 * i + c];
+            }
+            out[i].density = rho[i];
+        }
+        return out;
+    }
+
+    Dimensions GetDimensions() {
+        int32_t d[3];
+        check(ariel_get_dimensions(h_, d), h_);
+        return Dimensions{d[0], d[1], d[2]};
+    }
+
+    // MAC-grid accessor (no counterpart in the reference).
+    std::vector<float> GetMacGrid(int which) {
+        const Dimensions d = GetDimensions();
+        const int64_t n = which <= ARIEL_FIELD_DUZ ? (int64_t)(d.x + 1) * d.y * d.z
+                                                   : (int64_t)d.x * d.y * d.z;
+        std::vector<float> out((size_t)n);
+        check(ariel_get_macgrid(h_, which, out.data(), n), h_);
+        return out;
+    }
+
+    ariel_step_stats GetStats() {
+        ariel_step_stats s;
+        check(ariel_get_stats(h_, &s), h_);
+        return s;
+    }
+
+    ariel_sim* handle() { return h_; }
+
+    int m_frame;   // FlipSim::m_frame (flip.hpp:33)
+
+private:
+    void sync_frame() {
+        int32_t f = 0;
+        check(ariel_get_frame(h_, &f), h_);
+        m_frame = f;
+    }
+    static void check(int rc, const ariel_sim* h) {
+        if (rc != ARIEL_OK) throw Error(rc, ariel_last_error(h));
+    }
+    ariel_sim* h_ = nullptr;
+};
+
+}  // namespace ariel
+
+#endif  // ARIEL_FLIPSIM_HPP
