@@ -54,6 +54,7 @@ struct ariel_sim {
     int cur = 0;
     int32_t* cell_of = nullptr;
     int32_t* slots = nullptr;
+    int32_t* sid = nullptr;       // particle ids next to slots (per-cell ordering)
     int32_t* counts = nullptr;
     int32_t* offsets = nullptr;
     int32_t* scan_sums = nullptr;
@@ -338,8 +339,9 @@ int ensure_particles(ariel_sim* h, int64_t np) {
     }
     hipFree(h->cell_of);
     hipFree(h->slots);
+    hipFree(h->sid);
     hipFree(h->accept);
-    h->cell_of = h->slots = nullptr;
+    h->cell_of = h->slots = h->sid = nullptr;
     h->accept = nullptr;
     for (int b = 0; b < 2; ++b) {
         TRY(dalloc(h, &h->pos[b], np));
@@ -347,6 +349,7 @@ int ensure_particles(ariel_sim* h, int64_t np) {
     }
     TRY(dalloc(h, &h->cell_of, np));
     TRY(dalloc(h, &h->slots, np));
+    TRY(dalloc(h, &h->sid, np));
     TRY(dalloc(h, &h->accept, np));
     HIPCHK(h, hipMemsetAsync(h->accept, 1, (size_t)std::max<int64_t>(np, 1), h->stream));
     h->np_cap = np;
@@ -359,8 +362,8 @@ int stage_sort(ariel_sim* h) {
     Prof prof(h, ARIEL_KERNEL_SORT);
     launch_bin(h->stream, h->pos[b], h->np, h->N, h->maxd, h->cell_of, h->counts);
     launch_scan(h->stream, h->counts, h->offsets, h->nc, h->scan_sums);
-    launch_scatter(h->stream, h->cell_of, h->np, h->offsets, h->counts, h->slots);
-    launch_cellsort(h->stream, h->offsets, h->nc, h->pos[b], h->slots);
+    launch_scatter(h->stream, h->cell_of, h->np, h->offsets, h->counts, h->slots, h->pos[b], h->sid);
+    launch_cellsort(h->stream, h->offsets, h->nc, h->sid, h->slots);
     launch_gather(h->stream, h->slots, h->np, h->pos[b], h->vel[b], h->pos[o], h->vel[o]);
     h->cur = o;
     return check_launch(h, "sort");
@@ -653,7 +656,7 @@ int ariel_destroy(ariel_sim* h) {
         hipFree(h->pos[b]);
         hipFree(h->vel[b]);
     }
-    for (void* p : {(void*)h->cell_of, (void*)h->slots, (void*)h->counts, (void*)h->offsets,
+    for (void* p : {(void*)h->cell_of, (void*)h->slots, (void*)h->sid, (void*)h->counts, (void*)h->offsets,
                     (void*)h->scan_sums, (void*)h->accept, (void*)h->P, (void*)h->D, (void*)h->L,
                     (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
                     (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc})

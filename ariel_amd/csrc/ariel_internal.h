@@ -74,8 +74,8 @@ void launch_bin(hipStream_t s, const float4* pos, int64_t np, int N, float maxd,
                 int32_t* counts);
 void launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, int32_t* block_sums);
 void launch_scatter(hipStream_t s, const int32_t* cell_of, int64_t np, const int32_t* offsets,
-                    int32_t* counts, int32_t* slots);
-void launch_cellsort(hipStream_t s, const int32_t* offsets, int64_t nc, const float4* pos,
+                    int32_t* counts, int32_t* slots, const float4* pos, int32_t* sid);
+void launch_cellsort(hipStream_t s, const int32_t* offsets, int64_t nc, int32_t* sid,
                      int32_t* slots);
 void launch_gather(hipStream_t s, const int32_t* slots, int64_t np, const float4* pos_in,
                    const float4* vel_in, float4* pos_out, float4* vel_out);

@@ -12,10 +12,20 @@ namespace {
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-__device__ __forceinline__ void decode_face(int N, int comp, int64_t f, int& i, int& j, int& k) {
-    if (comp == 0) { k = (int)(f % N); j = (int)((f / N) % N); i = (int)(f / ((int64_t)N * N)); }
-    else if (comp == 1) { k = (int)(f % N); j = (int)((f / N) % (N + 1)); i = (int)(f / ((int64_t)N * (N + 1))); }
-    else { k = (int)(f % (N + 1)); j = (int)((f / (N + 1)) % N); i = (int)(f / ((int64_t)N * (N + 1))); }
+// Face and cell indices stay below 2^31 for N <= 1024: decode in 32-bit.
+__device__ __forceinline__ void decode_face(int N, int comp, int64_t f64, int& i, int& j, int& k) {
+    const uint32_t f = (uint32_t)f64, n = (uint32_t)N, n1 = n + 1;
+    if (comp == 0) { k = (int)(f % n); const uint32_t q = f / n; j = (int)(q % n); i = (int)(q / n); }
+    else if (comp == 1) { k = (int)(f % n); const uint32_t q = f / n; j = (int)(q % n1); i = (int)(q / n1); }
+    else { k = (int)(f % n1); const uint32_t q = f / n1; j = (int)(q % n); i = (int)(q / n); }
+}
+
+__device__ __forceinline__ void decode_cell32(int N, int64_t c64, int& i, int& j, int& k) {
+    const uint32_t c = (uint32_t)c64, n = (uint32_t)N;
+    k = (int)(c % n);
+    const uint32_t q = c / n;
+    j = (int)(q % n);
+    i = (int)(q / n);
 }
 
 // Project, divergence per cell (flip.cpp:417-437) and FlipGrid (solver.inl:42-56):
@@ -26,7 +36,8 @@ __global__ void k_divergence(Params p, const float* __restrict__ ux, const float
     const int64_t nc = (int64_t)N * N * N;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    const int k = (int)(c % N), j = (int)((c / N) % N), i = (int)(c / ((int64_t)N * N));
+    int i, j, k;
+    decode_cell32(N, c, i, j, k);
     const float h = 1.0f / p.maxd;
     float dv = (ux[fidx(N, 0, i + 1, j, k)] - ux[fidx(N, 0, i, j, k)] + uy[fidx(N, 1, i, j + 1, k)] -
                 uy[fidx(N, 1, i, j, k)] + uz[fidx(N, 2, i, j, k + 1)] - uz[fidx(N, 2, i, j, k)]) / h;
@@ -123,30 +134,44 @@ __global__ void k_extrapolate(Params p, const uint8_t* __restrict__ A, float* ux
 }
 
 // Post-projection diagnostics: max |div u| over FLUID cells and the FLUID count.
+// Grid-stride over a capped grid; one pair of atomics per block.
 __global__ void k_grid_stats(Params p, const float* __restrict__ ux, const float* __restrict__ uy,
                              const float* __restrict__ uz, const uint8_t* __restrict__ A,
                              SolverScalars* sc) {
     const int N = p.N;
     const int64_t nc = (int64_t)N * N * N;
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float h = 1.0f / p.maxd;
     float mx = 0.0f;
     int cnt = 0;
-    if (c < nc && A[c] == kFluid) {
-        const int k = (int)(c % N), j = (int)((c / N) % N), i = (int)(c / ((int64_t)N * N));
-        const float h = 1.0f / p.maxd;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (A[c] != kFluid) continue;
+        int i, j, k;
+        decode_cell32(N, c, i, j, k);
         float dv = (ux[fidx(N, 0, i + 1, j, k)] - ux[fidx(N, 0, i, j, k)] +
                     uy[fidx(N, 1, i, j + 1, k)] - uy[fidx(N, 1, i, j, k)] +
                     uz[fidx(N, 2, i, j, k + 1)] - uz[fidx(N, 2, i, j, k)]) / h;
-        mx = fabsf(dv);
-        cnt = 1;
+        mx = fmaxf(mx, fabsf(dv));
+        cnt += 1;
     }
-    // wave reduction, then one atomic per wave
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         mx = fmaxf(mx, __shfl_down(mx, off, 64));
         cnt += __shfl_down(cnt, off, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ float smx[16];
+    __shared__ int scnt[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) {
+        smx[w] = mx;
+        scnt[w] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < nw; ++i) {
+            mx = fmaxf(mx, smx[i]);
+            cnt += scnt[i];
+        }
         if (cnt) atomicAdd(&sc->fluid_cells, cnt);
         if (mx > 0.0f) atomicMax(&sc->maxdiv_bits, __float_as_uint(mx));
     }
@@ -185,8 +210,8 @@ void launch_grid_stats(hipStream_t s, const Params& p, float* const u[3], const 
                        SolverScalars* sc) {
     int64_t nc = (int64_t)p.N * p.N * p.N;
     hipLaunchKernelGGL(k_stats_reset, dim3(1), dim3(64), 0, s, sc);
-    hipLaunchKernelGGL(k_grid_stats, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, u[0], u[1],
-                       u[2], A, sc);
+    const int64_t nb = grid_for(nc, kBlock) < 1024 ? grid_for(nc, kBlock) : 1024;
+    hipLaunchKernelGGL(k_grid_stats, dim3((unsigned)nb), dim3(kBlock), 0, s, p, u[0], u[1], u[2], A, sc);
 }
 
 }  // namespace ariel

@@ -19,15 +19,37 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 // ------------------------------------------------------------------ sort
 // ParticleGrid::Sort (particlegrid.cpp:140-172) as a counting sort:
 // bin (histogram) -> exclusive scan -> scatter -> per-cell order by id -> gather.
+// Runs of equal cells among consecutive lanes (the input is the previous sorted
+// order, so most of a wave shares a handful of cells): the lowest lane of a run
+// is its leader; returns the leader lane and the run length (valid in leaders).
+__device__ __forceinline__ void wave_runs(int32_t c, bool valid, int& leader, int& len) {
+    const int lane = threadIdx.x & 63;
+    const int32_t prev = __shfl_up(c, 1, 64);
+    const bool head = valid && (lane == 0 || prev != c);
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    leader = 63 - __clzll(heads & upto);
+    const uint64_t after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+    const uint64_t vmask = __ballot(valid);
+    const int nvalid = __popcll(vmask);
+    const int next = after ? lane + 1 + __ffsll((unsigned long long)after) - 1 : nvalid;
+    len = next - lane;
+}
+
 __global__ void k_bin(const float4* __restrict__ pos, int64_t np, int N, float maxd,
                       int32_t* __restrict__ cell_of, int32_t* __restrict__ counts) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= np) return;
-    float4 p = pos[s];
-    int cx = cell_coord(p.x, maxd), cy = cell_coord(p.y, maxd), cz = cell_coord(p.z, maxd);
-    int32_t c = (int32_t)cidx(N, cx, cy, cz);
-    cell_of[s] = c;
-    atomicAdd(&counts[c], 1);
+    const bool valid = s < np;
+    int32_t c = -1;
+    if (valid) {
+        float4 p = pos[s];
+        int cx = cell_coord(p.x, maxd), cy = cell_coord(p.y, maxd), cz = cell_coord(p.z, maxd);
+        c = (int32_t)cidx(N, cx, cy, cz);
+        cell_of[s] = c;
+    }
+    int leader, len;
+    wave_runs(c, valid, leader, len);
+    if (valid && leader == (int)(threadIdx.x & 63)) atomicAdd(&counts[c], len);
 }
 
 constexpr int kScanThreads = 256;
@@ -109,33 +131,47 @@ __global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const in
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
 }
 
+// A run leader reserves len slots at the top of its cell with one atomic
+// (counts return to 0 for the next sort); members take consecutive slots in lane
+// order. The particle id is written next to its slot for the per-cell ordering.
 __global__ void k_scatter(const int32_t* __restrict__ cell_of, int64_t np,
                           const int32_t* __restrict__ offsets, int32_t* __restrict__ counts,
-                          int32_t* __restrict__ slots) {
+                          int32_t* __restrict__ slots, const float4* __restrict__ pos,
+                          int32_t* __restrict__ sid) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= np) return;
-    int32_t c = cell_of[s];
-    int32_t r = atomicSub(&counts[c], 1) - 1;   // leaves counts at 0 for the next sort
-    slots[offsets[c] + r] = (int32_t)s;
+    const bool valid = s < np;
+    const int32_t c = valid ? cell_of[s] : -1;
+    int leader, len;
+    wave_runs(c, valid, leader, len);
+    const int lane = threadIdx.x & 63;
+    int32_t base = 0;
+    if (valid && leader == lane) base = atomicSub(&counts[c], len) - len;
+    base = __shfl(base, leader, 64);
+    if (!valid) return;
+    const int32_t dst = offsets[c] + base + (lane - leader);
+    slots[dst] = (int32_t)s;
+    sid[dst] = __float_as_int(pos[s].w);
 }
 
-// Per-cell insertion sort of the scattered slots by particle id: restores the
-// reference's per-cell order (push_back in particle-vector order).
+// Per-cell insertion sort of (id, slot) pairs by id: restores the reference's
+// per-cell order (push_back in particle-vector order). Particles that stayed in
+// their cell arrive in order already, so most segments need no move.
 __global__ void k_cellsort(const int32_t* __restrict__ offsets, int64_t nc,
-                           const float4* __restrict__ pos, int32_t* __restrict__ slots) {
+                           int32_t* __restrict__ sid, int32_t* __restrict__ slots) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const int32_t o0 = offsets[c], o1 = offsets[c + 1];
     for (int32_t a = o0 + 1; a < o1; ++a) {
-        int32_t sv = slots[a];
-        int32_t key = __float_as_int(pos[sv].w);
+        const int32_t key = sid[a];
+        if (sid[a - 1] <= key) continue;
+        const int32_t sv = slots[a];
         int32_t b = a - 1;
-        while (b >= o0) {
-            int32_t sb = slots[b];
-            if (__float_as_int(pos[sb].w) <= key) break;
-            slots[b + 1] = sb;
+        while (b >= o0 && sid[b] > key) {
+            sid[b + 1] = sid[b];
+            slots[b + 1] = slots[b];
             --b;
         }
+        sid[b + 1] = key;
         slots[b + 1] = sv;
     }
 }
@@ -201,7 +237,8 @@ __global__ void k_mark(Params p, const float4* __restrict__ vel, const int32_t* 
     const int64_t nc = (int64_t)N * N * N;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    const int k = (int)(c % N), j = (int)((c / N) % N), i = (int)(c / ((int64_t)N * N));
+    const uint32_t cu = (uint32_t)c, nu = (uint32_t)N;
+    const int k = (int)(cu % nu), j = (int)((cu / nu) % nu), i = (int)(cu / nu / nu);
     float accm = 0.0f;
     const int32_t o0 = offsets[c], o1 = offsets[c + 1];
     for (int32_t a = o0; a < o1; ++a) accm += vel[a].w;
@@ -535,7 +572,9 @@ __global__ void k_resample(Params p, const float4* __restrict__ pos, const float
     }
     const float tx = me.x + dt * sx_, ty = me.y + dt * sy_, tz = me.z + dt * sz_;
     float rx = 0.0f, ry = 0.0f, rz = 0.0f, wsum = 0.0f;
-    {
+    // With re^2 <= 1e-5 every Sharpen weight max(re^2/max(d2,1e-5) - 1, 0) is 0:
+    // the velocity is kept (Resample's "else ru = u"), without visiting neighbours.
+    if (re2 > 1.0e-5f) {
         float x = gmax(0.0f, gmin(maxd - 1, maxd * tx));
         float y = gmax(0.0f, gmin(maxd - 1, maxd * ty));
         float z = gmax(0.0f, gmin(maxd - 1, maxd * tz));
@@ -627,15 +666,15 @@ void launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, int3
 }
 
 void launch_scatter(hipStream_t s, const int32_t* cell_of, int64_t np, const int32_t* offsets,
-                    int32_t* counts, int32_t* slots) {
+                    int32_t* counts, int32_t* slots, const float4* pos, int32_t* sid) {
     if (np == 0) return;
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, cell_of, np,
-                       offsets, counts, slots);
+                       offsets, counts, slots, pos, sid);
 }
 
-void launch_cellsort(hipStream_t s, const int32_t* offsets, int64_t nc, const float4* pos,
+void launch_cellsort(hipStream_t s, const int32_t* offsets, int64_t nc, int32_t* sid,
                      int32_t* slots) {
-    hipLaunchKernelGGL(k_cellsort, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, offsets, nc, pos,
+    hipLaunchKernelGGL(k_cellsort, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, offsets, nc, sid,
                        slots);
 }
 

@@ -125,10 +125,12 @@ __device__ __forceinline__ double spmv_cell_f64d(const uint8_t* __restrict__ A,
     return (double)N * (double)N * acc;
 }
 
-__device__ __forceinline__ void decode_cell(int N, int64_t c, int& i, int& j, int& k) {
-    k = (int)(c % N);
-    j = (int)((c / N) % N);
-    i = (int)(c / ((int64_t)N * N));
+__device__ __forceinline__ void decode_cell(int N, int64_t c64, int& i, int& j, int& k) {
+    const uint32_t c = (uint32_t)c64, n = (uint32_t)N;   // N^3 < 2^31 for N <= 1024
+    k = (int)(c % n);
+    const uint32_t q = c / n;
+    j = (int)(q % n);
+    i = (int)(q / n);
 }
 
 // ------------------------------------------------------------ solver kernels
@@ -218,9 +220,8 @@ __global__ void k_spmv_dot(Params p, const uint8_t* __restrict__ A, const float*
             Y4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
             continue;
         }
-        const int k0 = (int)(c % N);
-        const int j = (int)((c / N) % N);
-        const int i = (int)(c / NN);
+        int i, j, k0;
+        decode_cell(N, c, i, j, k0);
         const float4 xc4 = X4[t];
         const float xs[4] = {xc4.x, xc4.y, xc4.z, xc4.w};
         const uint8_t as[4] = {ac.x, ac.y, ac.z, ac.w};
@@ -766,6 +767,16 @@ __device__ __forceinline__ int64_t tiles_to_launch(int TN, int color) {
     return all;
 }
 
+// Each wave owns its tile and its LDS region, so the wavefront steps only need
+// the wave's own LDS writes to be visible to its other lanes: a wavefront-scope
+// release/acquire pair (waits for the wave's LDS operations) around a wave
+// barrier, instead of a barrier over the whole block.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // MIC(0) apply, one pass over the tiles of one colour; 4 tiles per 256-thread
 // block, one wave per tile.
 //   phase 0 (red, forward):       q = L^-1 r inside the tile          -> Q
@@ -879,7 +890,7 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
     fl_jm = __shfl(fl, (lane - 1) & 63, 64);
     fl_ip = __shfl(fl, (lane + 8) & 63, 64);
     fl_jp = __shfl(fl, (lane + 1) & 63, 64);
-    __syncthreads();
+    wave_sync();
     const int me = lane * kPad;
     if (do_fwd) {
         for (int st = 0; st < 3 * kTile - 2; ++st) {
@@ -911,7 +922,7 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
                 sa[me + kl] = q;
                 sb[me + kl] = spc[me + kl] * q;
             }
-            __syncthreads();
+            wave_sync();
         }
     }
     if (do_bwd) {
@@ -945,7 +956,7 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
                 }
                 sb[me + kl] = z;
             }
-            __syncthreads();
+            wave_sync();
         }
     }
     // coalesced write-back
