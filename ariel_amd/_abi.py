@@ -104,6 +104,8 @@ PROTOTYPES = {
     "ariel_save_state": (_I32, [_P, ctypes.c_char_p]),
     "ariel_load_state": (_I32, [_P, ctypes.c_char_p]),
     "ariel_export_levelset": (_I32, [_P, ctypes.c_char_p]),
+    "ariel_selftest_division": (_I32, [ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, ctypes.POINTER(ctypes.c_int64)]),
 }
 
 _lib = None

@@ -70,6 +70,10 @@ struct ariel_sim {
     uint8_t* nup = nullptr;
     uint8_t* fl8 = nullptr;       // fluid bits per 8-cell z-segment
     double* P64 = nullptr;        // FP64 solution of the refined solve
+    // active sets of the pressure solve (segments, tiles), rebuilt per solve
+    int32_t *seg_flag = nullptr, *seg_scan = nullptr, *seg_list = nullptr;
+    int32_t *tile_any = nullptr, *tile_lists = nullptr;
+    Active act;
     double* partials = nullptr;
     uint32_t* counter = nullptr;
     SolverScalars* sc = nullptr;
@@ -398,23 +402,51 @@ int read_scalars(ariel_sim* h) {
     return ARIEL_OK;
 }
 
+// Preconditioner set-up on the current cell classes: diagonal, fluid segment
+// bits, the active segment and tile lists (one host round trip for their
+// counts), then the factorisation over the active tiles. Leaves the solver
+// scalars initialised.
+int prepare_solve(ariel_sim* h) {
+    const Params p = h->params();
+    hipStream_t s = h->stream;
+    launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
+    launch_fluid_segments(s, h->N, h->A, h->fl8);
+    launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every);
+    launch_active_sets(s, h->N, h->fl8, h->seg_flag, h->seg_scan, h->seg_list, h->tile_any,
+                       h->tile_lists, h->scan_sums, h->sc);
+    TRY(check_launch(h, "active sets"));
+    TRY(read_scalars(h));
+    const int64_t nt = h->nc / 512;
+    h->act.segs = h->seg_list;
+    h->act.nseg = h->sc_host->nseg;
+    for (int c = 0; c < 3; ++c) {
+        h->act.tiles[c] = h->tile_lists + c * nt;
+        h->act.ntiles[c] = h->sc_host->ntile[c];
+    }
+    launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc, h->act);
+    return check_launch(h, "preconditioner build");
+}
+
 // BuildPreconditioner + SolveConjugateGradient (solver.inl:104-135, 289-354) with
-// the relative stop rule, FP64 reductions and residual replacement.
+// the relative stop rule, FP64 reductions and residual replacement. The
+// iterations touch the active segments and tiles only. The full-grid set-up
+// below leaves P and R zero off-FLUID and Z zero everywhere; S and Z are then
+// written on every active segment before they are read, and values off the
+// active sets are never read as FLUID values.
 int stage_solve(ariel_sim* h) {
     const Params p = h->params();
     const Reduce red = h->reduce();
     hipStream_t s = h->stream;
     HIPCHK(h, hipEventRecord(h->solve_t0, s));
-    launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
-    launch_fluid_segments(s, h->N, h->A, h->fl8);
-    launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc);
-    launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every);
+    TRY(prepare_solve(h));
+    const Active& act = h->act;
     const int refine = h->d.refine_every > 0;
     launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
                      h->P64, refine);
     if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, 0);
-    launch_update_s(s, h->nc, h->Z, h->S, red, 1);
+    launch_fill_f32(s, h->Z, h->nc, 0.0f);
+    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, act);
+    launch_update_s(s, h->Z, h->S, red, 1, act);
     TRY(check_launch(h, "solve setup"));
     int queued = 0;
     const int k = h->d.refine_every;
@@ -425,19 +457,19 @@ int stage_solve(ariel_sim* h) {
             const int it = queued + j + 1;
             {
                 Prof prof(h, ARIEL_KERNEL_SPMV);
-                launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red);
+                launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, act);
             }
             {
                 Prof prof(h, ARIEL_KERNEL_AXPY);
-                launch_axpy2(s, h->nc, h->P, h->S, h->R, h->Z, red);
+                launch_axpy2(s, h->P, h->S, h->R, h->Z, red, act);
             }
             if (k > 0 && it % k == 0)
-                launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red);
+                launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, act);
             {
                 Prof prof(h, ARIEL_KERNEL_PRECOND);
-                launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, 0);
+                launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, act);
             }
-            launch_update_s(s, h->nc, h->Z, h->S, red, 0);
+            launch_update_s(s, h->Z, h->S, red, 0, act);
         }
         TRY(check_launch(h, "pcg"));
         queued += n;
@@ -445,7 +477,7 @@ int stage_solve(ariel_sim* h) {
         if (h->sc_host->done || queued >= h->d.max_iters) break;
         chunk = 4;
     }
-    if (refine) launch_refine_finish(s, h->nc, h->A, h->P, h->P64);
+    if (refine) launch_refine_finish(s, h->A, h->P, h->P64, act);
     HIPCHK(h, hipEventRecord(h->solve_t1, s));
     h->last_iters = h->sc_host->it;
     h->stats.iterations = h->sc_host->it;
@@ -453,7 +485,7 @@ int stage_solve(ariel_sim* h) {
     h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
     h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
     // true residual of the final pressure, for the report only
-    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red);
+    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, act);
     TRY(check_launch(h, "true residual"));
     TRY(read_scalars(h));
     h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
@@ -614,6 +646,9 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->counts, h->nc)) || (rc = dalloc(h, &h->offsets, h->nc + 1)) ||
         (rc = dalloc(h, &h->scan_sums, h->nc / 4096 + 4)) ||
         (rc = dalloc(h, &h->P64, h->d.refine_every > 0 ? h->nc : 1)) ||
+        (rc = dalloc(h, &h->seg_flag, h->nc / 8)) || (rc = dalloc(h, &h->seg_scan, h->nc / 8 + 1)) ||
+        (rc = dalloc(h, &h->seg_list, h->nc / 8)) || (rc = dalloc(h, &h->tile_any, h->nc / 512)) ||
+        (rc = dalloc(h, &h->tile_lists, 3 * (h->nc / 512))) ||
         (rc = dalloc(h, &h->partials, 2 * (int64_t)kMaxPartials)) || (rc = dalloc(h, &h->counter, kCounterWords)) ||
         (rc = dalloc(h, &h->sc, 1)))
         return bail(rc);
@@ -659,7 +694,9 @@ int ariel_destroy(ariel_sim* h) {
     for (void* p : {(void*)h->cell_of, (void*)h->slots, (void*)h->sid, (void*)h->counts, (void*)h->offsets,
                     (void*)h->scan_sums, (void*)h->accept, (void*)h->P, (void*)h->D, (void*)h->L,
                     (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
-                    (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc})
+                    (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc,
+                    (void*)h->seg_flag, (void*)h->seg_scan, (void*)h->seg_list, (void*)h->tile_any,
+                    (void*)h->tile_lists})
         hipFree(p);
     for (int c = 0; c < 3; ++c) {
         hipFree(h->u[c]);
@@ -951,12 +988,9 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
     if (!r || !z) return fail(h, ARIEL_ERR_INVALID, "null pointer");
     const Params p = h->params();
     HIPCHK(h, hipMemcpyAsync(h->R, r, h->nc * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    launch_precond_setup(h->stream, p, h->A, h->L, h->diag, h->nup, h->pc);
-    launch_fluid_segments(h->stream, h->N, h->A, h->fl8);
-    launch_precond_build(h->stream, p, h->A, h->diag, h->nup, h->pc);
-    launch_solver_init(h->stream, h->sc, h->d.max_iters, h->d.refine_every);
+    TRY(prepare_solve(h));
     launch_fill_f32(h->stream, h->Z, h->nc, 0.0f);
-    launch_precond_apply(h->stream, p, h->fl8, h->pc, h->R, h->Q, h->Z, h->reduce(), 0);
+    launch_precond_apply(h->stream, p, h->fl8, h->pc, h->R, h->Q, h->Z, h->reduce(), h->act);
     TRY(check_launch(h, "preconditioner"));
     HIPCHK(h, hipMemcpyAsync(z, h->Z, h->nc * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1099,6 +1133,17 @@ int ariel_export_levelset(ariel_sim* h, const char* path) {
     ok = ok && std::fwrite(phi.data(), sizeof(float), phi.size(), f) == phi.size();
     std::fclose(f);
     if (!ok) return fail(h, ARIEL_ERR_IO, std::string("write failed: ") + path);
+    return ARIEL_OK;
+}
+
+int ariel_selftest_division(int32_t which, float a, float lo, float hi, int64_t* mismatches) {
+    if (!mismatches || (which != 0 && which != 1) || !(lo >= 0.0f) || !(hi >= lo) ||
+        !std::isfinite(hi) || !(a > 0.0f) || !std::isfinite(a))
+        return fail(nullptr, ARIEL_ERR_INVALID, "selftest_division: bad arguments");
+    *mismatches = -1;
+    hipError_t e = run_division_selftest(which, a, lo, hi, mismatches);
+    if (e != hipSuccess)
+        return fail(nullptr, ARIEL_ERR_HIP, std::string("selftest_division: ") + hipGetErrorString(e));
     return ARIEL_OK;
 }
 

@@ -29,6 +29,65 @@ __device__ __forceinline__ float sharpen(float r2, float h) {
     return gmax(h * h / gmax(r2, 1.0e-5f) - 1.0f, 0.0f);
 }
 
+// Correctly rounded x / d for a divisor that is fixed over many calls, given
+// yd = 1.0f / d (itself correctly rounded). q = x*yd lies within two ulps of
+// x/d; one residual correction brings it within one ulp, and then, yd being the
+// correctly rounded reciprocal, a second correction yields the correctly rounded
+// quotient (Markstein's theorem, IBM J. Res. Dev. 34(1), 1990; Muller et al.,
+// Handbook of Floating-Point Arithmetic, 2nd ed., sec. 4.7). The residuals
+// r = x - q*d are exact fmas unless they underflow, which needs x below about
+// 2^-100 for the divisors used here; such quotients are below 2^-80 and
+// smooth_fixed() returns exactly 1 for them either way.
+// ariel_selftest_division compares it with IEEE division for every float x in
+// a range.
+__device__ __forceinline__ float div_fixed(float x, float d, float yd) {
+    float q = x * yd;
+    float r = fmaf(-q, d, x);
+    q = fmaf(r, yd, q);
+    r = fmaf(-q, d, x);
+    return fmaf(r, yd, q);
+}
+// smooth(r2, h) with hh = h*h and yhh = 1.0f / hh: same value, bit for bit.
+__device__ __forceinline__ float smooth_fixed(float r2, float hh, float yhh) {
+    return gmax(1.0f - div_fixed(r2, hh, yhh), 0.0f);
+}
+
+// Correctly rounded n / d: Newton-Raphson on the hardware reciprocal (one
+// reciprocal refinement, a quotient, two residual corrections), the arithmetic
+// of the compiler's IEEE expansion without its range scaling and special-value
+// fix-up, so it holds while n, d and n/d are finite, nonzero and within
+// [2^-60, 2^60], where that scaling is the identity.
+__device__ __forceinline__ float div_nr(float n, float d) {
+    float y = __builtin_amdgcn_rcpf(d);
+    const float e = fmaf(-d, y, 1.0f);
+    y = fmaf(e, y, y);
+    float q = n * y;
+    float r = fmaf(-d, q, n);
+    q = fmaf(r, y, q);
+    r = fmaf(-d, q, n);
+    return fmaf(r, y, q);
+}
+// sharpen(r2, h) with hh = h*h and the division by div_nr: same value for
+// 2^-60 <= hh <= 2^60 (max(r2, 1e-5) keeps the divisor in range).
+__device__ __forceinline__ float sharpen_nr(float r2, float hh) {
+    return gmax(div_nr(hh, gmax(r2, 1.0e-5f)) - 1.0f, 0.0f);
+}
+
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD b % 8).
+// xcd_block maps them so that each XCD takes a contiguous eighth of the logical
+// blocks: blocks that run together then share neighbour cells in their XCD's
+// L2 instead of spreading every neighbourhood over all eight. The grid must be
+// xcd_grid(nb) blocks; logical blocks >= nb return early.
+constexpr int kXcds = 8;
+__host__ __device__ __forceinline__ int64_t xcd_grid(int64_t nb) {
+    return (nb + kXcds - 1) / kXcds * kXcds;
+}
+__device__ __forceinline__ int64_t xcd_block(int64_t nb) {
+    const int64_t per = (nb + kXcds - 1) / kXcds;
+    const int64_t b = blockIdx.x;
+    return (b % kXcds) * per + b / kXcds;
+}
+
 // 32-bit integer finaliser (lowbias32) and the 4-key hash used for the octant
 // jitter and the resampler's coincident-neighbour jitter.
 __host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {

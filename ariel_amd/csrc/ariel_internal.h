@@ -40,6 +40,22 @@ struct SolverScalars {
     int32_t fluid_cells;
     uint32_t maxdiv_bits;  // max |div| as float bits (non-negative floats order as ints)
     int32_t pad;
+    int64_t nseg;          // active z-segments (counted by the active-set build)
+    int32_t ntile[3];      // active tiles: red, black, all
+    int32_t pad2;
+};
+
+// Cells the pressure solve touches. A z-segment is 8 cells (cells 8s..8s+7) and
+// is active when one of them is FLUID; an 8^3 tile is active when one of its 64
+// segments is. Lists are ascending, so block partials keep a fixed order.
+// Everything outside them stays as the solve's full-grid set-up left it
+// (zero off-FLUID), which the operator, the preconditioner halos and the dot
+// products never read as FLUID values.
+struct Active {
+    const int32_t* segs = nullptr;
+    int64_t nseg = 0;
+    const int32_t* tiles[3] = {nullptr, nullptr, nullptr};   // red, black, all
+    int64_t ntiles[3] = {0, 0, 0};
 };
 
 struct Params {
@@ -67,6 +83,9 @@ struct Reduce {
     uint32_t* counter;     // 33 arrival counters, 128 bytes apart (zero between reductions)
     SolverScalars* sc;     // device scalars
 };
+
+// Exhaustive division self-check (selftest.hip); lo, hi non-negative.
+hipError_t run_division_selftest(int which, float a, float lo, float hi, int64_t* mismatches);
 
 // ---------------------------------------------------------------- launchers
 // particles (kernels_particles.hip)
@@ -109,31 +128,41 @@ void launch_grid_stats(hipStream_t s, const Params& p, float* const u[3], const 
 void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                           float* diag, uint8_t* nup, float* pc);
 void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
-                          const uint8_t* nup, float* pc);
+                          const uint8_t* nup, float* pc, const Active& act);
 // Fluid bits of each 8-cell z-segment (input of the tiled preconditioner apply).
 void launch_fluid_segments(hipStream_t s, int N, const uint8_t* A, uint8_t* fl8);
+// Active segment and tile lists from fl8; counts into sc->nseg / sc->ntile.
+// seg_flag, seg_scan: N^3/8 (+1) ints; tile_any: (N/8)^3 ints; scan_sums: the
+// block sums of launch_scan.
+void launch_active_sets(hipStream_t s, int N, const uint8_t* fl8, int32_t* seg_flag,
+                        int32_t* seg_scan, int32_t* seg_list, int32_t* tile_any,
+                        int32_t* tile_lists, int32_t* scan_sums, SolverScalars* sc);
 // z = M^-1 r; the last pass finalises rho_new = z.r and beta in the scalars.
 void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, const float* pc,
-                          const float* r, float* q, float* z, const Reduce& red, int mode);
+                          const float* r, float* q, float* z, const Reduce& red,
+                          const Active& act);
 void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                       float* P, const float* D, float* R, const Reduce& red, double tol,
                       int stop_rule, double* P64, int refine);
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                     const float* x, float* y, const Reduce& red);
-void launch_axpy2(hipStream_t s, int64_t n, float* P, const float* S, float* R, const float* Z,
-                  const Reduce& red);
+                     const float* x, float* y, const Reduce& red, const Active& act);
+void launch_axpy2(hipStream_t s, float* P, const float* S, float* R, const float* Z,
+                  const Reduce& red, const Active& act);
 // Reliable update every refine_every iterations: x64 += p, p = 0, r = b - A x64 (FP64).
 void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                            float* P, double* P64, const float* D, float* R, const Reduce& red);
-void launch_refine_finish(hipStream_t s, int64_t n, const uint8_t* A, float* P, const double* P64);
-void launch_update_s(hipStream_t s, int64_t n, const float* Z, float* S, const Reduce& red,
-                     int first);
+                            float* P, double* P64, const float* D, float* R, const Reduce& red,
+                            const Active& act);
+void launch_refine_finish(hipStream_t s, const uint8_t* A, float* P, const double* P64,
+                          const Active& act);
+void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,
+                     const Active& act);
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                        const float* x, float* y);
 void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every);
 // ||b - A p||^2 in FP64 into sc->tmp (R untouched).
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                               const float* P, const float* D, const Reduce& red);
+                               const float* P, const float* D, const Reduce& red,
+                               const Active& act);
 
 // reduction helpers
 int reduce_blocks(int64_t n, int per_thread);

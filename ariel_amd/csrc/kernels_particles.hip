@@ -192,13 +192,14 @@ __global__ void k_gather(const int32_t* __restrict__ slots, int64_t np, const fl
 // cell; the 3 cells along z of each (sx,sy) column are one contiguous range.
 __global__ void k_density(Params p, const float4* __restrict__ pos, float4* __restrict__ vel,
                           int64_t np, const int32_t* __restrict__ offsets, int apply_forces) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
     if (s >= np) return;
     const int N = p.N;
     const float maxd = p.maxd;
     const float4 me = pos[s];
     const int cx = cell_coord(me.x, maxd), cy = cell_coord(me.y, maxd), cz = cell_coord(me.z, maxd);
     const float hsm = 4.0f * p.density / maxd;
+    const float hh = hsm * hsm, yhh = 1.0f / hh;
     const int z0 = max(cz - 1, 0), z1 = min(cz + 1, N - 1);
     float weightsum = 0.0f;
     for (int sx = cx - 1; sx <= cx + 1; ++sx) {
@@ -207,11 +208,19 @@ __global__ void k_density(Params p, const float4* __restrict__ pos, float4* __re
             if (sy < 0 || sy > N - 1) continue;
             const int64_t col = cidx(N, sx, sy, 0);
             const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
-            for (int32_t q = q0; q < q1; ++q) {
-                const float4 o = pos[q];
-                float sqd = sqrlength(o.x, o.y, o.z, me.x, me.y, me.z);
-                float weight = 1.0f * smooth(sqd, hsm);
-                weightsum = weightsum + weight;
+            // four loads in flight, terms added in list order
+            for (int32_t q = q0; q < q1; q += 4) {
+                float4 o4[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) o4[u] = pos[min(q + u, q1 - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (q + u >= q1) break;
+                    const float4 o = o4[u];
+                    float sqd = sqrlength(o.x, o.y, o.z, me.x, me.y, me.z);
+                    float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
+                    weightsum = weightsum + weight;
+                }
             }
         }
     }
@@ -349,7 +358,7 @@ __device__ __forceinline__ float p2g_weight(bool tri, float px, float py, float 
     }
     const float r2 = dx * dx + dy * dy + dz * dz;
     if (r2 >= hh) return 0.0f;
-    return 1.0f * sharpen(r2, RE);
+    return 1.0f * sharpen_nr(r2, hh);
 }
 
 __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restrict__ pos,
@@ -362,7 +371,9 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
     __shared__ float sp[6][kP2GCap];
     const int N = p.N;
     const int nkc = (N + 1 + 63) / 64;
-    int64_t b = blockIdx.x;
+    const int64_t nbl = (int64_t)(N + 1) * (N + 1) * nkc;
+    int64_t b = xcd_block(nbl);
+    if (b >= nbl) return;
     const int kc = (int)(b % nkc);
     b /= nkc;
     const int j = (int)(b % (N + 1));
@@ -380,11 +391,17 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
     const bool tri = p.splat == 1;
     const float fi = (float)i, fj = (float)j, fk = (float)k;
     float swx = 0.0f, sux = 0.0f, swy = 0.0f, suy = 0.0f, swz = 0.0f, suz = 0.0f;
-    const int zlo_blk = max(0, k0 - 2), zhi_blk = min(N - 1, k0 + 64);
-    const int a0 = max(k - 2, 0), a1 = min(k + 1, N - 1);
+    // Along a face's tangential axis the window's first cell ([f-2.5, f-1.5]
+    // from the face coordinate f) is at least 1.5 cells away: every weight
+    // there is exactly 0 for the trilinear kernel and for Sharpen with
+    // RE^2 <= 2.25 (the weight is 0 once r2 >= RE^2), so those cells are not
+    // visited. The window then starts one cell later on every axis.
+    const int lo = (tri || hh <= 2.25f) ? 1 : 2;
+    const int zlo_blk = max(0, k0 - lo), zhi_blk = min(N - 1, k0 + 64);
+    const int a0 = max(k - lo, 0), a1 = min(k + 1, N - 1);
     const int b0 = max(k - 1, 0), b1 = min(k, N - 1);
-    for (int sx = max(i - 2, 0); sx <= min(i + 1, N - 1); ++sx) {
-        for (int sy = max(j - 2, 0); sy <= min(j + 1, N - 1); ++sy) {
+    for (int sx = max(i - lo, 0); sx <= min(i + 1, N - 1); ++sx) {
+        for (int sy = max(j - lo, 0); sy <= min(j + 1, N - 1); ++sy) {
             const bool colx = (sx >= i - 1 && sx <= i) && j < N;
             const bool coly = (sy >= j - 1 && sy <= j) && i < N;
             const bool colz = i < N && j < N;
@@ -474,7 +491,7 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
 __global__ void k_g2p_advect(Params p, float4* __restrict__ pos, float4* __restrict__ vel,
                              int64_t np, float* ux, float* uy, float* uz, float* dux, float* duy,
                              float* duz) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
     if (s >= np) return;
     const int N = p.N;
     float* g[3] = {ux, uy, uz};
@@ -512,10 +529,45 @@ __device__ __forceinline__ void cell_range(float x, int N, int& lo, int& hi) {
     if (hi > N - 1) hi = N - 1;
 }
 
-__global__ void k_resample(Params p, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                           int64_t np, const int32_t* __restrict__ offsets, float4* __restrict__ pout,
-                           float4* __restrict__ vout, uint8_t* __restrict__ accept) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Distance from the grid coordinate x to the cell interval [c, c+1], less a
+// margin of 1e-3 cells plus a few ulps of the coordinate that covers the
+// rounding of x and of the particles' own grid coordinates.
+__device__ __forceinline__ float cell_gap(float x, int c, float margin) {
+    const float g = fmaxf(fmaxf((float)c - x, x - (float)(c + 1)), 0.0f);
+    return fmaxf(g - margin, 0.0f);
+}
+
+// Narrows the z cells [zl, zh] of column (gx2 + gy2) to those whose squared
+// distance bound does not exceed gbound. The bound grows away from the point,
+// so the cells dropped are at the ends of the contiguous range.
+__device__ __forceinline__ void narrow_z(float z, float gxy2, float gbound, float margin, int& zl,
+                                        int& zh) {
+    while (zl <= zh) {
+        const float g = cell_gap(z, zl, margin);
+        if (gxy2 + g * g <= gbound) break;
+        ++zl;
+    }
+    while (zh >= zl) {
+        const float g = cell_gap(z, zh, margin);
+        if (gxy2 + g * g <= gbound) break;
+        --zh;
+    }
+}
+
+constexpr int kSpringList = 6;   // queued spring neighbours per thread (LDS)
+
+__global__ __launch_bounds__(kBlock) void k_resample(Params p, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ vel, int64_t np,
+                                                     const int32_t* __restrict__ offsets,
+                                                     float4* __restrict__ pout,
+                                                     float4* __restrict__ vout,
+                                                     uint8_t* __restrict__ accept) {
+    // Neighbours inside the spring radius are queued per thread (in visiting
+    // order) and their terms evaluated in batches, so that the wave does not run
+    // the square root and divisions on every visit for the few lanes that need
+    // them. Each thread owns column threadIdx.x of the queue.
+    __shared__ float4 s_nb[kSpringList][kBlock];
+    const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
     if (s >= np) return;
     const int N = p.N;
     const float maxd = p.maxd;
@@ -526,49 +578,85 @@ __global__ void k_resample(Params p, const float4* __restrict__ pos, const float
     const float4 mv = vel[s];
     const uint32_t my_id = (uint32_t)__float_as_int(me.w);
     const float re2 = re * re;
+    const float yre2 = 1.0f / re2;
     const float re2_skip = re2 * 1.0001f;
+    // Cells are visited only when the distance bound to them (grid units, see
+    // cell_gap) is within re*1.0005: every pair in a skipped cell has
+    // dist^2 > re^2*1.0001 and would be skipped by the pair tests below.
+    const float gbound = re2 * (maxd * maxd) * 1.001f;
+    const float margin = 1.0e-3f + 4.0e-7f * maxd;
     const uint32_t fr = (uint32_t)p.frame;
     float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
+    int queued = 0;
+    // Spring term of one neighbour (flip.cpp Resample's first loop): dx = me - o.
+    auto spring_terms = [&]() {
+        for (int t = 0; t < queued; ++t) {
+            const float4 e = s_nb[t][threadIdx.x];
+            const float dx = e.x, dy = e.y, dz = e.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            float dist = sqrtf(d2);
+            float w = springforce * 1.0f * smooth_fixed(dist * dist, re2, yre2);
+            if (dist > 0.1f * re) {
+                sx_ += w * dx / dist * re;
+                sy_ += w * dy / dist * re;
+                sz_ += w * dz / dist * re;
+            } else {
+                const uint32_t qid = (uint32_t)__float_as_int(e.w);
+                uint32_t hx = hash4(p.seed, 3u * fr + 0u, my_id, qid);
+                uint32_t hy = hash4(p.seed, 3u * fr + 1u, my_id, qid);
+                uint32_t hz = hash4(p.seed, 3u * fr + 2u, my_id, qid);
+                sx_ += 0.01f * re / dt * (float)(int)(hx % 101u) / 100.0f;
+                sy_ += 0.01f * re / dt * (float)(int)(hy % 101u) / 100.0f;
+                sz_ += 0.01f * re / dt * (float)(int)(hz % 101u) / 100.0f;
+            }
+        }
+        queued = 0;
+    };
     {
         float x = gmax(0.0f, gmin(maxd, maxd * me.x));
         float y = gmax(0.0f, gmin(maxd, maxd * me.y));
         float z = gmax(0.0f, gmin(maxd, maxd * me.z));
+        const float xg = maxd * me.x, yg = maxd * me.y, zg = maxd * me.z;
         int xl, xh, yl, yh, zl, zh;
         cell_range(x, N, xl, xh);
         cell_range(y, N, yl, yh);
         cell_range(z, N, zl, zh);
-        for (int cx = xl; cx <= xh; ++cx)
+        for (int cx = xl; cx <= xh; ++cx) {
+            const float gx = cell_gap(xg, cx, margin);
             for (int cy = yl; cy <= yh; ++cy) {
-                if (zl > zh) continue;
+                const float gy = cell_gap(yg, cy, margin);
+                const float gxy2 = gx * gx + gy * gy;
+                if (gxy2 > gbound) continue;
+                int za = zl, zb = zh;
+                narrow_z(zg, gxy2, gbound, margin, za, zb);
+                if (za > zb) continue;
                 const int64_t col = cidx(N, cx, cy, 0);
-                const int32_t q0 = offsets[col + zl], q1 = offsets[col + zh + 1];
-                for (int32_t q = q0; q < q1; ++q) {
-                    if (q == (int32_t)s) continue;
-                    const float4 o = pos[q];
-                    float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and then
-                    // dist > 0.1 re: the term adds +-0 to the spring. d2 above re^2
-                    // by a margin far beyond the rounding of sqrt and the square
-                    // guarantees that, so the pair is skipped without changing a bit.
-                    if (d2 > re2_skip) continue;
-                    float dist = sqrtf(d2);
-                    float w = springforce * 1.0f * smooth(dist * dist, re);
-                    if (dist > 0.1f * re) {
-                        sx_ += w * (me.x - o.x) / dist * re;
-                        sy_ += w * (me.y - o.y) / dist * re;
-                        sz_ += w * (me.z - o.z) / dist * re;
-                    } else {
-                        const uint32_t qid = (uint32_t)__float_as_int(o.w);
-                        uint32_t hx = hash4(p.seed, 3u * fr + 0u, my_id, qid);
-                        uint32_t hy = hash4(p.seed, 3u * fr + 1u, my_id, qid);
-                        uint32_t hz = hash4(p.seed, 3u * fr + 2u, my_id, qid);
-                        sx_ += 0.01f * re / dt * (float)(int)(hx % 101u) / 100.0f;
-                        sy_ += 0.01f * re / dt * (float)(int)(hy % 101u) / 100.0f;
-                        sz_ += 0.01f * re / dt * (float)(int)(hz % 101u) / 100.0f;
+                const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
+                // four loads in flight, neighbours taken in list order
+                for (int32_t q = q0; q < q1; q += 4) {
+                    float4 o4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) o4[u] = pos[min(q + u, q1 - 1)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int32_t qq = q + u;
+                        if (qq >= q1) break;
+                        if (qq == (int32_t)s) continue;
+                        const float4 o = o4[u];
+                        float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and then
+                        // dist > 0.1 re: the term adds +-0 to the spring. d2 above re^2
+                        // by a margin far beyond the rounding of sqrt and the square
+                        // guarantees that, so the pair is skipped without changing a bit.
+                        if (d2 > re2_skip) continue;
+                        s_nb[queued][threadIdx.x] = make_float4(dx, dy, dz, o.w);
+                        if (++queued == kSpringList) spring_terms();
                     }
                 }
             }
+        }
+        spring_terms();
     }
     const float tx = me.x + dt * sx_, ty = me.y + dt * sy_, tz = me.z + dt * sz_;
     float rx = 0.0f, ry = 0.0f, rz = 0.0f, wsum = 0.0f;
@@ -578,22 +666,29 @@ __global__ void k_resample(Params p, const float4* __restrict__ pos, const float
         float x = gmax(0.0f, gmin(maxd - 1, maxd * tx));
         float y = gmax(0.0f, gmin(maxd - 1, maxd * ty));
         float z = gmax(0.0f, gmin(maxd - 1, maxd * tz));
+        const float xg = maxd * tx, yg = maxd * ty, zg = maxd * tz;
         int xl, xh, yl, yh, zl, zh;
         cell_range(x, N, xl, xh);
         cell_range(y, N, yl, yh);
         cell_range(z, N, zl, zh);
-        for (int cx = xl; cx <= xh; ++cx)
+        for (int cx = xl; cx <= xh; ++cx) {
+            const float gx = cell_gap(xg, cx, margin);
             for (int cy = yl; cy <= yh; ++cy) {
-                if (zl > zh) continue;
+                const float gy = cell_gap(yg, cy, margin);
+                const float gxy2 = gx * gx + gy * gy;
+                if (gxy2 > gbound) continue;
+                int za = zl, zb = zh;
+                narrow_z(zg, gxy2, gbound, margin, za, zb);
+                if (za > zb) continue;
                 const int64_t col = cidx(N, cx, cy, 0);
-                const int32_t q0 = offsets[col + zl], q1 = offsets[col + zh + 1];
+                const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
                 for (int32_t q = q0; q < q1; ++q) {
                     const float4 o = pos[q];
                     float dist2 = sqrlength(tx, ty, tz, o.x, o.y, o.z);
                     // Sharpen(dist2, re) = max(re^2/max(dist2,1e-5) - 1, 0) is exactly 0
                     // when max(dist2, 1e-5) >= re^2; the +0 term is skipped.
                     if (gmax(dist2, 1.0e-5f) >= re2) continue;
-                    float w = 1.0f * sharpen(dist2, re);
+                    float w = 1.0f * sharpen_nr(dist2, re2);
                     const float4 ov = vel[q];
                     rx = rx + w * ov.x;
                     ry = ry + w * ov.y;
@@ -601,6 +696,7 @@ __global__ void k_resample(Params p, const float4* __restrict__ pos, const float
                     wsum += w;
                 }
             }
+        }
     }
     if (wsum != 0.0f) {
         rx = rx / wsum; ry = ry / wsum; rz = rz / wsum;
@@ -688,7 +784,7 @@ void launch_gather(hipStream_t s, const int32_t* slots, int64_t np, const float4
 void launch_density(hipStream_t s, const Params& p, const float4* pos, float4* vel, int64_t np,
                     const int32_t* offsets, int apply_forces) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_density, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, p, pos, vel, np,
+    hipLaunchKernelGGL(k_density, dim3((unsigned)xcd_grid(grid_for(np, kBlock))), dim3(kBlock), 0, s, p, pos, vel, np,
                        offsets, apply_forces);
 }
 
@@ -702,7 +798,7 @@ void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4*
                 const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3]) {
     const int64_t nkc = (p.N + 1 + 63) / 64;
     const int64_t nb = (int64_t)(p.N + 1) * (p.N + 1) * nkc;
-    hipLaunchKernelGGL(k_p2g_cols, dim3((unsigned)nb), dim3(64), 0, s, p, pos, vel, offsets, A,
+    hipLaunchKernelGGL(k_p2g_cols, dim3((unsigned)xcd_grid(nb)), dim3(64), 0, s, p, pos, vel, offsets, A,
                        u[0], u[1], u[2], du[0], du[1], du[2]);
 }
 
@@ -718,7 +814,7 @@ void launch_p2g_faces(hipStream_t s, const Params& p, const float4* pos, const f
 void launch_g2p_advect(hipStream_t s, const Params& p, float4* pos, float4* vel, int64_t np,
                        float* const u[3], float* const du[3]) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_g2p_advect, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, p, pos, vel,
+    hipLaunchKernelGGL(k_g2p_advect, dim3((unsigned)xcd_grid(grid_for(np, kBlock))), dim3(kBlock), 0, s, p, pos, vel,
                        np, u[0], u[1], u[2], du[0], du[1], du[2]);
 }
 
@@ -726,7 +822,7 @@ void launch_resample(hipStream_t s, const Params& p, const float4* pos, const fl
                      int64_t np, const int32_t* offsets, float4* pos_out, float4* vel_out,
                      uint8_t* accept) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_resample, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, p, pos, vel, np,
+    hipLaunchKernelGGL(k_resample, dim3((unsigned)xcd_grid(grid_for(np, kBlock))), dim3(kBlock), 0, s, p, pos, vel, np,
                        offsets, pos_out, vel_out, accept);
 }
 

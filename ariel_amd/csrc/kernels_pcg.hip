@@ -196,79 +196,89 @@ __device__ __forceinline__ float xref_term(uint8_t t, float xq, const float* __r
     return L[q] / gmin(1.0e-6f, lc) * xc;
 }
 
-// z = A s, sigma = z.s, alpha = rho / sigma. Each thread evaluates 4 consecutive
-// cells along z with vector loads (x and the classes of the cell quad and of its
-// +-y and +-x neighbour quads) and the reference expression per cell.
-__global__ void k_spmv_dot(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
-                           const float* __restrict__ x, float* __restrict__ y, Reduce red) {
-    if (red.sc->done) return;
+// y = A x on the quad of 4 consecutive cells t (cells 4t..4t+3) with vector loads
+// (x and the classes of the quad and of its +-y and +-x neighbour quads) and the
+// reference expression per cell; adds y.x over its FLUID cells to acc.
+__device__ __forceinline__ void spmv_quad(const Params& p, const uint8_t* __restrict__ A,
+                                          const float* __restrict__ L,
+                                          const float* __restrict__ x, float* __restrict__ y,
+                                          int64_t t, double& acc) {
     const int N = p.N;
-    const int64_t nc = (int64_t)N * N * N;
-    const int64_t n4 = nc / 4;
     const int64_t NN = (int64_t)N * N;
     const float n = (float)N;
     const float h = 1.0f / (n * n);
     const float4* __restrict__ X4 = reinterpret_cast<const float4*>(x);
     const uchar4* __restrict__ A4 = reinterpret_cast<const uchar4*>(A);
     float4* __restrict__ Y4 = reinterpret_cast<float4*>(y);
-    double v[1] = {0.0};
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = 4 * t;
-        const uchar4 ac = A4[t];
-        if (ac.x != kFluid && ac.y != kFluid && ac.z != kFluid && ac.w != kFluid) {
-            Y4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t c = 4 * t;
+    const uchar4 ac = A4[t];
+    if (ac.x != kFluid && ac.y != kFluid && ac.z != kFluid && ac.w != kFluid) {
+        Y4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    int i, j, k0;
+    decode_cell(N, c, i, j, k0);
+    const float4 xc4 = X4[t];
+    const float xs[4] = {xc4.x, xc4.y, xc4.z, xc4.w};
+    const uint8_t as[4] = {ac.x, ac.y, ac.z, ac.w};
+    // neighbour quads (clamped to self at the domain boundary)
+    const bool hxp = i + 1 < N, hxm = i > 0, hyp = j + 1 < N, hym = j > 0;
+    float4 xxp = xc4, xxm = xc4, xyp = xc4, xym = xc4;
+    uchar4 axp = ac, axm = ac, ayp = ac, aym = ac;
+    if (hxp) { xxp = X4[t + NN / 4]; axp = A4[t + NN / 4]; }
+    if (hxm) { xxm = X4[t - NN / 4]; axm = A4[t - NN / 4]; }
+    if (hyp) { xyp = X4[t + N / 4]; ayp = A4[t + N / 4]; }
+    if (hym) { xym = X4[t - N / 4]; aym = A4[t - N / 4]; }
+    float xzm = 0.f, xzp = 0.f;
+    uint8_t azm = kSolid, azp = kSolid;
+    if (k0 > 0) { xzm = x[c - 1]; azm = A[c - 1]; }
+    if (k0 + 4 < N) { xzp = x[c + 4]; azp = A[c + 4]; }
+    const float vxp[4] = {xxp.x, xxp.y, xxp.z, xxp.w}, vxm[4] = {xxm.x, xxm.y, xxm.z, xxm.w};
+    const float vyp[4] = {xyp.x, xyp.y, xyp.z, xyp.w}, vym[4] = {xym.x, xym.y, xym.z, xym.w};
+    const uint8_t txp[4] = {axp.x, axp.y, axp.z, axp.w}, txm[4] = {axm.x, axm.y, axm.z, axm.w};
+    const uint8_t typ[4] = {ayp.x, ayp.y, ayp.z, ayp.w}, tym[4] = {aym.x, aym.y, aym.z, aym.w};
+    float out[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        if (as[m] != kFluid) {
+            out[m] = 0.0f;
             continue;
         }
-        int i, j, k0;
-        decode_cell(N, c, i, j, k0);
-        const float4 xc4 = X4[t];
-        const float xs[4] = {xc4.x, xc4.y, xc4.z, xc4.w};
-        const uint8_t as[4] = {ac.x, ac.y, ac.z, ac.w};
-        // neighbour quads (clamped to self at the domain boundary)
-        const bool hxp = i + 1 < N, hxm = i > 0, hyp = j + 1 < N, hym = j > 0;
-        float4 xxp = xc4, xxm = xc4, xyp = xc4, xym = xc4;
-        uchar4 axp = ac, axm = ac, ayp = ac, aym = ac;
-        if (hxp) { xxp = X4[t + NN / 4]; axp = A4[t + NN / 4]; }
-        if (hxm) { xxm = X4[t - NN / 4]; axm = A4[t - NN / 4]; }
-        if (hyp) { xyp = X4[t + N / 4]; ayp = A4[t + N / 4]; }
-        if (hym) { xym = X4[t - N / 4]; aym = A4[t - N / 4]; }
-        float xzm = 0.f, xzp = 0.f;
-        uint8_t azm = kSolid, azp = kSolid;
-        if (k0 > 0) { xzm = x[c - 1]; azm = A[c - 1]; }
-        if (k0 + 4 < N) { xzp = x[c + 4]; azp = A[c + 4]; }
-        const float vxp[4] = {xxp.x, xxp.y, xxp.z, xxp.w}, vxm[4] = {xxm.x, xxm.y, xxm.z, xxm.w};
-        const float vyp[4] = {xyp.x, xyp.y, xyp.z, xyp.w}, vym[4] = {xym.x, xym.y, xym.z, xym.w};
-        const uint8_t txp[4] = {axp.x, axp.y, axp.z, axp.w}, txm[4] = {axm.x, axm.y, axm.z, axm.w};
-        const uint8_t typ[4] = {ayp.x, ayp.y, ayp.z, ayp.w}, tym[4] = {aym.x, aym.y, aym.z, aym.w};
-        float out[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            if (as[m] != kFluid) {
-                out[m] = 0.0f;
-                continue;
-            }
-            const int64_t cm = c + m;
-            const float xcm = xs[m];
-            const bool need_l = txp[m] == kAir || txm[m] == kAir || typ[m] == kAir || tym[m] == kAir ||
-                                (m < 3 ? as[m + 1] == kAir : azp == kAir) ||
-                                (m > 0 ? as[m - 1] == kAir : azm == kAir);
-            const float lc = need_l ? L[cm] : 0.0f;
-            // +x, -x, +y, -y: a missing neighbour clamps to the cell itself (FLUID)
-            const float t1 = hxp ? xref_term(txp[m], vxp[m], L, cm + NN, lc, xcm) : xcm;
-            const float t2 = hxm ? xref_term(txm[m], vxm[m], L, cm - NN, lc, xcm) : xcm;
-            const float t3 = hyp ? xref_term(typ[m], vyp[m], L, cm + N, lc, xcm) : xcm;
-            const float t4 = hym ? xref_term(tym[m], vym[m], L, cm - N, lc, xcm) : xcm;
-            float t5, t6;
-            if (m < 3) t5 = xref_term(as[m + 1], xs[m + 1], L, cm + 1, lc, xcm);
-            else t5 = (k0 + 4 < N) ? xref_term(azp, xzp, L, cm + 1, lc, xcm) : xcm;
-            if (m > 0) t6 = xref_term(as[m - 1], xs[m - 1], L, cm - 1, lc, xcm);
-            else t6 = (k0 > 0) ? xref_term(azm, xzm, L, cm - 1, lc, xcm) : xcm;
-            const float yc = (6.0f * xcm - t1 - t2 - t3 - t4 - t5 - t6) / h;
-            out[m] = yc;
-            v[0] += (double)yc * (double)xcm;
-        }
-        Y4[t] = make_float4(out[0], out[1], out[2], out[3]);
+        const int64_t cm = c + m;
+        const float xcm = xs[m];
+        const bool need_l = txp[m] == kAir || txm[m] == kAir || typ[m] == kAir || tym[m] == kAir ||
+                            (m < 3 ? as[m + 1] == kAir : azp == kAir) ||
+                            (m > 0 ? as[m - 1] == kAir : azm == kAir);
+        const float lc = need_l ? L[cm] : 0.0f;
+        // +x, -x, +y, -y: a missing neighbour clamps to the cell itself (FLUID)
+        const float t1 = hxp ? xref_term(txp[m], vxp[m], L, cm + NN, lc, xcm) : xcm;
+        const float t2 = hxm ? xref_term(txm[m], vxm[m], L, cm - NN, lc, xcm) : xcm;
+        const float t3 = hyp ? xref_term(typ[m], vyp[m], L, cm + N, lc, xcm) : xcm;
+        const float t4 = hym ? xref_term(tym[m], vym[m], L, cm - N, lc, xcm) : xcm;
+        float t5, t6;
+        if (m < 3) t5 = xref_term(as[m + 1], xs[m + 1], L, cm + 1, lc, xcm);
+        else t5 = (k0 + 4 < N) ? xref_term(azp, xzp, L, cm + 1, lc, xcm) : xcm;
+        if (m > 0) t6 = xref_term(as[m - 1], xs[m - 1], L, cm - 1, lc, xcm);
+        else t6 = (k0 > 0) ? xref_term(azm, xzm, L, cm - 1, lc, xcm) : xcm;
+        const float yc = (6.0f * xcm - t1 - t2 - t3 - t4 - t5 - t6) / h;
+        out[m] = yc;
+        acc += (double)yc * (double)xcm;
+    }
+    Y4[t] = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// z = A s, sigma = z.s, alpha = rho / sigma over the active segments (two quads
+// each); y elsewhere keeps the zeros of the solve's set-up.
+__global__ void k_spmv_dot(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
+                           const float* __restrict__ x, float* __restrict__ y,
+                           const int32_t* __restrict__ segs, int64_t nseg, Reduce red) {
+    if (red.sc->done) return;
+    double v[1] = {0.0};
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t sg = segs[u];
+        spmv_quad(p, A, L, x, y, 2 * sg, v[0]);
+        spmv_quad(p, A, L, x, y, 2 * sg + 1, v[0]);
     }
     block_partials<1>(v, red.partials, blockIdx.x);
 }
@@ -292,51 +302,53 @@ __device__ __forceinline__ bool refine_due(const SolverScalars* sc) {
 }
 
 // p += alpha s; r += (-alpha) z (Op, solver.inl:157-177); ||r||^2; stop test.
-// float4 per thread, two loads of each operand in flight per iteration.
-__global__ void k_axpy2(int64_t n, float* __restrict__ P, const float* __restrict__ S,
-                        float* __restrict__ R, const float* __restrict__ Z, Reduce red) {
+// One active segment (two float4 of each operand) per thread and iteration.
+__global__ void k_axpy2(float* __restrict__ P, const float* __restrict__ S, float* __restrict__ R,
+                        const float* __restrict__ Z, const int32_t* __restrict__ segs,
+                        int64_t nseg, Reduce red) {
     if (red.sc->done) return;
     const float af = (float)red.sc->alpha;
     const float naf = -af;
-    const int64_t n4 = n / 4;
     float4* __restrict__ P4 = reinterpret_cast<float4*>(P);
     float4* __restrict__ R4 = reinterpret_cast<float4*>(R);
     const float4* __restrict__ S4 = reinterpret_cast<const float4*>(S);
     const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
     double v[1] = {0.0};
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n4; c += 2 * stride) {
-        const int64_t c2 = c + stride;
-        const bool two = c2 < n4;
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = 2 * (int64_t)segs[u];
         float4 p0 = P4[c], s0 = S4[c], r0 = R4[c], z0 = Z4[c];
-        float4 p1, s1, r1, z1;
-        if (two) { p1 = P4[c2]; s1 = S4[c2]; r1 = R4[c2]; z1 = Z4[c2]; }
+        float4 p1 = P4[c + 1], s1 = S4[c + 1], r1 = R4[c + 1], z1 = Z4[c + 1];
         p0.x = p0.x + af * s0.x; p0.y = p0.y + af * s0.y; p0.z = p0.z + af * s0.z; p0.w = p0.w + af * s0.w;
         r0.x = r0.x + naf * z0.x; r0.y = r0.y + naf * z0.y; r0.z = r0.z + naf * z0.z; r0.w = r0.w + naf * z0.w;
+        p1.x = p1.x + af * s1.x; p1.y = p1.y + af * s1.y; p1.z = p1.z + af * s1.z; p1.w = p1.w + af * s1.w;
+        r1.x = r1.x + naf * z1.x; r1.y = r1.y + naf * z1.y; r1.z = r1.z + naf * z1.z; r1.w = r1.w + naf * z1.w;
         P4[c] = p0;
         R4[c] = r0;
+        P4[c + 1] = p1;
+        R4[c + 1] = r1;
         v[0] += (double)r0.x * (double)r0.x + (double)r0.y * (double)r0.y +
                 (double)r0.z * (double)r0.z + (double)r0.w * (double)r0.w;
-        if (two) {
-            p1.x = p1.x + af * s1.x; p1.y = p1.y + af * s1.y; p1.z = p1.z + af * s1.z; p1.w = p1.w + af * s1.w;
-            r1.x = r1.x + naf * z1.x; r1.y = r1.y + naf * z1.y; r1.z = r1.z + naf * z1.z; r1.w = r1.w + naf * z1.w;
-            P4[c2] = p1;
-            R4[c2] = r1;
-            v[0] += (double)r1.x * (double)r1.x + (double)r1.y * (double)r1.y +
-                    (double)r1.z * (double)r1.z + (double)r1.w * (double)r1.w;
-        }
+        v[0] += (double)r1.x * (double)r1.x + (double)r1.y * (double)r1.y +
+                (double)r1.z * (double)r1.z + (double)r1.w * (double)r1.w;
     }
     block_partials<1>(v, red.partials, blockIdx.x);
 }
 
 // Reliable update, part 1: x64 += p, p = 0 (every refine_every iterations).
-__global__ void k_flush(int64_t n, const uint8_t* __restrict__ A, float* __restrict__ P,
-                        double* __restrict__ P64, const SolverScalars* sc) {
+__global__ void k_flush(const uint8_t* __restrict__ A, float* __restrict__ P,
+                        double* __restrict__ P64, const int32_t* __restrict__ segs, int64_t nseg,
+                        const SolverScalars* sc) {
     if (sc->done || !refine_due(sc)) return;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        if (A[c] == kFluid) P64[c] += (double)P[c];
-        P[c] = 0.0f;
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c0 = 8 * (int64_t)segs[u];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int64_t c = c0 + m;
+            if (A[c] == kFluid) P64[c] += (double)P[c];
+            P[c] = 0.0f;
+        }
     }
 }
 
@@ -344,53 +356,68 @@ __global__ void k_flush(int64_t n, const uint8_t* __restrict__ A, float* __restr
 // stop test. mode 1 (report): ||b - A p||^2 of the FP32 pressure into sc->tmp.
 __global__ void k_true_residual(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
                                 const float* __restrict__ P, const double* __restrict__ P64,
-                                const float* __restrict__ D, float* __restrict__ R, Reduce red,
+                                const float* __restrict__ D, float* __restrict__ R,
+                                const int32_t* __restrict__ segs, int64_t nseg, Reduce red,
                                 int mode) {
     if (mode == 0 && (red.sc->done || !refine_due(red.sc))) return;
     const int N = p.N;
-    const int64_t nc = (int64_t)N * N * N;
     double v[1] = {0.0};
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        if (A[c] != kFluid) {
-            if (mode == 0) R[c] = 0.0f;
-            continue;
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c0 = 8 * (int64_t)segs[u];
+        for (int m = 0; m < 8; ++m) {
+            const int64_t c = c0 + m;
+            if (A[c] != kFluid) {
+                if (mode == 0) R[c] = 0.0f;
+                continue;
+            }
+            int i, j, k;
+            decode_cell(N, c, i, j, k);
+            const double ax = mode == 0 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
+                                        : spmv_cell_f64(A, L, P, N, i, j, k, c);
+            const double r = (double)D[c] - ax;
+            v[0] += r * r;
+            if (mode == 0) R[c] = (float)r;
         }
-        int i, j, k;
-        decode_cell(N, c, i, j, k);
-        const double ax = mode == 0 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
-                                    : spmv_cell_f64(A, L, P, N, i, j, k, c);
-        const double r = (double)D[c] - ax;
-        v[0] += r * r;
-        if (mode == 0) R[c] = (float)r;
     }
     block_partials<1>(v, red.partials, blockIdx.x);
 }
 
 // End of a refined solve: p = (float)(x64 + p) on FLUID cells, 0 elsewhere.
-__global__ void k_refine_finish(int64_t n, const uint8_t* __restrict__ A, float* __restrict__ P,
-                                const double* __restrict__ P64) {
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-         c += (int64_t)gridDim.x * blockDim.x)
-        P[c] = A[c] == kFluid ? (float)(P64[c] + (double)P[c]) : 0.0f;
+__global__ void k_refine_finish(const uint8_t* __restrict__ A, float* __restrict__ P,
+                                const double* __restrict__ P64, const int32_t* __restrict__ segs,
+                                int64_t nseg) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c0 = 8 * (int64_t)segs[u];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int64_t c = c0 + m;
+            P[c] = A[c] == kFluid ? (float)(P64[c] + (double)P[c]) : 0.0f;
+        }
+    }
 }
 
-__global__ void k_update_s(int64_t n, const float* __restrict__ Z, float* __restrict__ S,
-                           const SolverScalars* sc, int first) {
+__global__ void k_update_s(const float* __restrict__ Z, float* __restrict__ S,
+                           const int32_t* __restrict__ segs, int64_t nseg, const SolverScalars* sc,
+                           int first) {
     if (sc->done) return;
     const float bf = (float)sc->beta;
-    const int64_t n4 = n / 4;
     const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
     float4* __restrict__ S4 = reinterpret_cast<float4*>(S);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n4; c += stride) {
-        const float4 z = Z4[c];
-        if (first) {
-            S4[c] = z;
-        } else {
-            float4 sv = S4[c];
-            sv.x = z.x + bf * sv.x; sv.y = z.y + bf * sv.y; sv.z = z.z + bf * sv.z; sv.w = z.w + bf * sv.w;
-            S4[c] = sv;
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = 2 * (int64_t)segs[u];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float4 z = Z4[c + h];
+            if (first) {
+                S4[c + h] = z;
+            } else {
+                float4 sv = S4[c + h];
+                sv.x = z.x + bf * sv.x; sv.y = z.y + bf * sv.y; sv.z = z.z + bf * sv.z; sv.w = z.w + bf * sv.w;
+                S4[c + h] = sv;
+            }
         }
     }
 }
@@ -595,7 +622,9 @@ __device__ __forceinline__ uint32_t load_halo_line(const uint8_t* __restrict__ A
 __global__ __launch_bounds__(64) void k_mic_build(Params p, const uint8_t* __restrict__ A,
                                                    const float* __restrict__ diag,
                                                    const uint8_t* __restrict__ nup,
-                                                   float* __restrict__ pc, int color) {
+                                                   float* __restrict__ pc, int color,
+                                                   const int32_t* __restrict__ tiles,
+                                                   int64_t ntl) {
     __shared__ float s_pc[64 * kPad];
     __shared__ float s_nm[64 * kPad];
     __shared__ float s_hxp[64 * kPad];   // cross-tile x neighbour line: pc
@@ -604,10 +633,11 @@ __global__ __launch_bounds__(64) void k_mic_build(Params p, const uint8_t* __res
     __shared__ float s_hyn[64 * kPad];
     const int N = p.N;
     const int TN = N / kTile;
-    const int64_t ntiles = (int64_t)TN * TN * TN;
+    (void)TN;
     const bool black = color == 1;
     const float tau = p.mic_tau, sigma = p.mic_sigma;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int64_t u = blockIdx.x; u < ntl; u += gridDim.x) {
+        const int64_t t = tiles[u];
         TileCtx c;
         if (!tile_of(N, t, color, c)) continue;
         const uint32_t fl = fluid_bits(A, c.line);
@@ -740,33 +770,6 @@ constexpr int kTilesPerBlock = 4;
 constexpr int kLine = 64 * kPad;          // floats of one per-line LDS array
 constexpr int kHalo = 2 * kTile * kPad;   // two halo faces (minus / plus side) of 8 lines
 
-// Tile `idx` of colour `color` (0 red, 1 black, -1 all) -> tile coordinates.
-// With an even tile count per axis every (ti,tj) row holds TN/2 tiles of each
-// colour; otherwise tiles are enumerated linearly and off-colour ones are idle.
-__device__ __forceinline__ bool tile_coords(int TN, int64_t idx, int color, int& ti, int& tj,
-                                            int& tk) {
-    if (color >= 0 && (TN & 1) == 0) {
-        const int half = TN >> 1;
-        const int64_t row = idx / half;
-        const int m = (int)(idx % half);
-        ti = (int)(row / TN);
-        tj = (int)(row % TN);
-        tk = 2 * m + ((ti + tj + color) & 1);
-        return ti < TN;
-    }
-    tk = (int)(idx % TN);
-    tj = (int)((idx / TN) % TN);
-    ti = (int)(idx / ((int64_t)TN * TN));
-    if (ti >= TN) return false;
-    return color < 0 || ((ti + tj + tk) & 1) == color;
-}
-
-__device__ __forceinline__ int64_t tiles_to_launch(int TN, int color) {
-    const int64_t all = (int64_t)TN * TN * TN;
-    if (color >= 0 && (TN & 1) == 0) return all / 2;
-    return all;
-}
-
 // Each wave owns its tile and its LDS region, so the wavefront steps only need
 // the wave's own LDS writes to be visible to its other lanes: a wavefront-scope
 // release/acquire pair (waits for the wave's LDS operations) around a wave
@@ -778,68 +781,104 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // MIC(0) apply, one pass over the tiles of one colour; 4 tiles per 256-thread
-// block, one wave per tile.
-//   phase 0 (red, forward):       q = L^-1 r inside the tile          -> Q
-//   phase 1 (black, fwd + bwd):   forward with red halos, backward     -> Z, z.r partials
-//   phase 2 (red, backward):      backward with black halos            -> Z, z.r (+finalise)
-//   phase 3 (block, fwd + bwd):   no cross-tile coupling               -> Z, z.r (+finalise)
+// block, one wave per tile, lane (il, jl) owning the z-line of 8 cells.
+//   PH 0 (red, forward):       q = L^-1 r inside the tile          -> Q
+//   PH 1 (black, fwd + bwd):   forward with red halos, backward     -> Z, z.r partials
+//   PH 2 (red, backward):      backward with black halos            -> Z, z.r partials
+//   PH 3 (block, fwd + bwd):   no cross-tile coupling               -> Z, z.r partials
 // forward:  q_i = (r_i + sum_{j lower} pc_j q_j) pc_i   (order -x,-y,-z,+x,+y,+z)
 // backward: z_i = (q_i + sum_{k upper} pc_i z_k) pc_i   (order +x,+y,+z,-x,-y,-z)
-// Tile data moves through LDS with coalesced accesses: in load round m the 64
-// lanes read the 8 lines with il = m, 8 lanes per 32-byte line segment.
+// The sweeps are wavefronts over il+jl+kl (22 steps). Every step is branch-free:
+// the neighbour values kept in LDS for the sums (pc*q forward, z backward) are
+// -0.0 for cells that are not FLUID, and absent neighbours (tile edges without
+// a halo) read a line of -0.0, so each term is added unconditionally:
+// x + (-0.0) == x for every x, and pc * (-0.0) == -0.0 for pc >= 0, which
+// leaves every sum bit-identical to one that skips the term.
+// Lane addresses are set up once as base - (il+jl); step st then reads base + st
+// (forward) or base + 21 - st (backward) with the step in the instruction's
+// immediate offset. Lanes outside their 8 cells at a step read inside the
+// guarded block array and do not write.
+constexpr int kSteps = 3 * kTile - 2;                // 22
+constexpr int kGuardLo = 16;                          // >= 14 = max il+jl
+constexpr int kWaveFloats = 3 * kLine + 2 * kHalo + kTile;
+constexpr int kGuardHi = 32;                          // >= 21 past the last region
+
+template <int PH>
 __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __restrict__ fl8,
                                                     const float* __restrict__ pc,
                                                     const float* __restrict__ r,
                                                     float* __restrict__ Q, float* __restrict__ Z,
-                                                    Reduce red, int phase, int slot_base,
-                                                    int finalize_total) {
+                                                    const int32_t* __restrict__ tiles,
+                                                    int64_t ntiles, Reduce red, int slot_base) {
+    constexpr bool do_fwd = PH != 2;
+    constexpr bool do_bwd = PH != 0;
+    constexpr bool halo_fwd = PH == 1;
+    constexpr bool halo_bwd = PH == 2;
+    constexpr bool halo = halo_fwd || halo_bwd;
     if (red.sc->done) return;
-    __shared__ float lds[kTilesPerBlock][3 * kLine + 2 * kHalo];
+    __shared__ float lds[kGuardLo + kTilesPerBlock * kWaveFloats + kGuardHi];
     const int N = p.N;
     const int TN = N / kTile;
     const int NS = N / kTile;   // segments per z-line
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int color = phase == 3 ? -1 : (phase == 1 ? 1 : 0);
-    const bool do_fwd = phase != 2;
-    const bool do_bwd = phase != 0;
-    const bool halo_fwd = phase == 1;
-    const bool halo_bwd = phase == 2;
-    float* sa = lds[wave];                // r, then q          [line*9 + kl]
-    float* spc = sa + kLine;              // pc
-    float* sb = spc + kLine;              // pc*q (forward), z (backward)
-    float* hx = sb + kLine;               // x halo [(il==7)*72 + jl*9 + kl]
-    float* hy = hx + kHalo;               // y halo [(jl==7)*72 + il*9 + kl]
+    // float offsets of this wave's arrays inside lds[]
+    const int o_sa = kGuardLo + wave * kWaveFloats;   // r / q, then q / z      [line*9 + kl]
+    const int o_pc = o_sa + kLine;                    // pc
+    const int o_sb = o_pc + kLine;                    // pc*q (forward), z (backward); -0 off-fluid
+    const int o_hx = o_sb + kLine;                    // x halo [(il==7)*72 + jl*9 + kl]
+    const int o_hy = o_hx + kHalo;                    // y halo [(jl==7)*72 + il*9 + kl]
+    const int o_zero = o_hy + kHalo;                  // 8 x -0.0
     int ti = 0, tj = 0, tk = 0;
     const int64_t idx = (int64_t)blockIdx.x * kTilesPerBlock + wave;
-    const bool active = idx < tiles_to_launch(TN, color) && tile_coords(TN, idx, color, ti, tj, tk);
+    const bool active = idx < ntiles;
+    if (active) {
+        const int t = tiles[idx];
+        tk = t % TN;
+        tj = (t / TN) % TN;
+        ti = t / (TN * TN);
+    }
     const int il = lane >> 3, jl = lane & 7;
     const int gi = ti * kTile + il, gj = tj * kTile + jl, k0 = tk * kTile;
-    uint32_t fl = 0, fl_im = 0, fl_jm = 0, fl_ip = 0, fl_jp = 0;
-    uint32_t hx_fl = 0, hy_fl = 0, hzm_fl = 0, hzp_fl = 0;
-    float hzm = 0.f, hzp = 0.f;
+    uint32_t fl = 0;
+    float hzm = -0.0f, hzp = -0.0f;   // z-halo terms (-0.0 when absent)
     float rreg[kTile];
     // coalesced mapping: round m covers line (m, lane>>3), element lane&7
     const int ce = lane & 7, cj = lane >> 3;
+    if (lane < kTile) lds[o_zero + lane] = -0.0f;
     if (active) {
         fl = fl8[((int64_t)gi * N + gj) * NS + tk];
 #pragma unroll
         for (int m = 0; m < kTile; ++m) {
             const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
             const int li = (m * 8 + cj) * kPad + ce;
-            spc[li] = pc[g];
+            lds[o_pc + li] = pc[g];
             rreg[m] = r[g];
-            sa[li] = do_fwd ? rreg[m] : Q[g];
+            lds[o_sa + li] = do_fwd ? rreg[m] : Q[g];
         }
-        if (halo_fwd || halo_bwd) {
+        if constexpr (halo) {
+            // neighbour lines across the tile faces: forward halos hold pc*Q of
+            // the red neighbours, backward halos their Z; -0.0 off-fluid
             const float* va = halo_fwd ? pc : Z;
             const float* vb = halo_fwd ? Q : nullptr;
+            // lanes on a tile face own one halo line; at the domain boundary it
+            // holds -0.0
+            if (il == 0 || il == 7) {
+                float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
+#pragma unroll
+                for (int kl = 0; kl < kTile; ++kl) d[kl] = -0.0f;
+            }
+            if (jl == 0 || jl == 7) {
+                float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
+#pragma unroll
+                for (int kl = 0; kl < kTile; ++kl) d[kl] = -0.0f;
+            }
             int xi = -1;
             if (il == 0 && gi > 0) xi = gi - 1;
             if (il == 7 && gi + 1 < N) xi = gi + 1;
             if (xi >= 0) {
                 const int64_t ln = cidx(N, xi, gj, k0);
-                hx_fl = fl8[((int64_t)xi * N + gj) * NS + tk];
-                float* d = hx + (il == 7) * kTile * kPad + jl * kPad;
+                const uint32_t hf = fl8[((int64_t)xi * N + gj) * NS + tk];
+                float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
                 const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
                 const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
                 float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
@@ -848,19 +887,18 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
                     const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
                     float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) d[kl] = av[kl] * bv[kl];
-                } else {
-#pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) d[kl] = av[kl];
+                    for (int kl = 0; kl < kTile; ++kl) av[kl] = av[kl] * bv[kl];
                 }
+#pragma unroll
+                for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
             }
             int yj = -1;
             if (jl == 0 && gj > 0) yj = gj - 1;
             if (jl == 7 && gj + 1 < N) yj = gj + 1;
             if (yj >= 0) {
                 const int64_t ln = cidx(N, gi, yj, k0);
-                hy_fl = fl8[((int64_t)gi * N + yj) * NS + tk];
-                float* d = hy + (jl == 7) * kTile * kPad + il * kPad;
+                const uint32_t hf = fl8[((int64_t)gi * N + yj) * NS + tk];
+                float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
                 const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
                 const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
                 float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
@@ -869,92 +907,82 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
                     const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
                     float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) d[kl] = av[kl] * bv[kl];
-                } else {
-#pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) d[kl] = av[kl];
+                    for (int kl = 0; kl < kTile; ++kl) av[kl] = av[kl] * bv[kl];
                 }
+#pragma unroll
+                for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
             }
             const int64_t line = cidx(N, gi, gj, k0);
-            if (k0 > 0) {
-                hzm_fl = (fl8[((int64_t)gi * N + gj) * NS + tk - 1] >> 7) & 1u;
+            if (k0 > 0 && ((fl8[((int64_t)gi * N + gj) * NS + tk - 1] >> 7) & 1u))
                 hzm = vb ? va[line - 1] * vb[line - 1] : va[line - 1];
-            }
-            if (k0 + kTile < N) {
-                hzp_fl = fl8[((int64_t)gi * N + gj) * NS + tk + 1] & 1u;
+            if (k0 + kTile < N && (fl8[((int64_t)gi * N + gj) * NS + tk + 1] & 1u))
                 hzp = vb ? va[line + kTile] * vb[line + kTile] : va[line + kTile];
-            }
         }
     }
-    fl_im = __shfl(fl, (lane - 8) & 63, 64);
-    fl_jm = __shfl(fl, (lane - 1) & 63, 64);
-    fl_ip = __shfl(fl, (lane + 8) & 63, 64);
-    fl_jp = __shfl(fl, (lane + 1) & 63, 64);
     wave_sync();
+    const int sl = il + jl;
     const int me = lane * kPad;
-    if (do_fwd) {
-        for (int st = 0; st < 3 * kTile - 2; ++st) {
-            const int kl = st - il - jl;
-            if (active && kl >= 0 && kl < kTile) {
-                float q = 0.0f;
-                if ((fl >> kl) & 1u) {
-                    float acc = sa[me + kl];
-                    if (il > 0) {
-                        if ((fl_im >> kl) & 1u) acc = acc + sb[me - 8 * kPad + kl];
-                    } else if (halo_fwd && ((hx_fl >> kl) & 1u)) {
-                        acc = acc + hx[jl * kPad + kl];
-                    }
-                    if (jl > 0) {
-                        if ((fl_jm >> kl) & 1u) acc = acc + sb[me - kPad + kl];
-                    } else if (halo_fwd && ((hy_fl >> kl) & 1u)) {
-                        acc = acc + hy[il * kPad + kl];
-                    }
-                    if (kl > 0) {
-                        if ((fl >> (kl - 1)) & 1u) acc = acc + sb[me + kl - 1];
-                    } else if (halo_fwd && hzm_fl) {
-                        acc = acc + hzm;
-                    }
-                    if (halo_fwd && il == 7 && ((hx_fl >> kl) & 1u)) acc = acc + hx[kTile * kPad + jl * kPad + kl];
-                    if (halo_fwd && jl == 7 && ((hy_fl >> kl) & 1u)) acc = acc + hy[kTile * kPad + il * kPad + kl];
-                    if (halo_fwd && kl == 7 && hzp_fl) acc = acc + hzp;
-                    q = acc * spc[me + kl];
-                }
-                sa[me + kl] = q;
-                sb[me + kl] = spc[me + kl] * q;
+    const uint32_t fls = fl << sl;    // bit (kl + sl) = FLUID bit of cell kl
+    const int a_sa = o_sa + me - sl, a_pc = o_pc + me - sl, a_sb = o_sb + me - sl;
+    const int a_zero = o_zero - sl;
+    if constexpr (do_fwd) {
+        // lower neighbours: -x, -y, -z (in tile), +x, +y (halo, PH 1)
+        const int f_xm = il > 0 ? a_sb - 8 * kPad : (halo_fwd ? o_hx + jl * kPad - sl : a_zero);
+        const int f_ym = jl > 0 ? a_sb - kPad : (halo_fwd ? o_hy + il * kPad - sl : a_zero);
+        const int f_xp = (halo_fwd && il == 7) ? o_hx + kTile * kPad + jl * kPad - sl : a_zero;
+        const int f_yp = (halo_fwd && jl == 7) ? o_hy + kTile * kPad + il * kPad - sl : a_zero;
+#pragma unroll
+        for (int st = 0; st < kSteps; ++st) {
+            float acc = lds[a_sa + st];
+            acc = acc + lds[f_xm + st];
+            acc = acc + lds[f_ym + st];
+            const float vzm = lds[a_sb - 1 + st];         // sb[kl - 1]
+            acc = acc + (sl == st ? hzm : vzm);
+            if constexpr (halo_fwd) {
+                acc = acc + lds[f_xp + st];
+                acc = acc + lds[f_yp + st];
+                acc = acc + (sl + 7 == st ? hzp : -0.0f);
+            }
+            const float pcv = lds[a_pc + st];
+            const bool fluid = (fls >> st) & 1u;
+            const float q = fluid ? acc * pcv : 0.0f;
+            const float pq = fluid ? pcv * q : -0.0f;
+            if ((unsigned)(st - sl) < (unsigned)kTile && active) {
+                lds[a_sa + st] = q;
+                lds[a_sb + st] = pq;
             }
             wave_sync();
         }
     }
-    if (do_bwd) {
-        for (int st = 0; st < 3 * kTile - 2; ++st) {
-            const int m = st - (7 - il) - (7 - jl);
-            const int kl = 7 - m;
-            if (active && m >= 0 && m < kTile) {
-                float z = 0.0f;
-                if ((fl >> kl) & 1u) {
-                    const float pci = spc[me + kl];
-                    float acc = sa[me + kl];
-                    if (il < 7) {
-                        if ((fl_ip >> kl) & 1u) acc = acc + pci * sb[me + 8 * kPad + kl];
-                    } else if (halo_bwd && ((hx_fl >> kl) & 1u)) {
-                        acc = acc + pci * hx[kTile * kPad + jl * kPad + kl];
-                    }
-                    if (jl < 7) {
-                        if ((fl_jp >> kl) & 1u) acc = acc + pci * sb[me + kPad + kl];
-                    } else if (halo_bwd && ((hy_fl >> kl) & 1u)) {
-                        acc = acc + pci * hy[kTile * kPad + il * kPad + kl];
-                    }
-                    if (kl < 7) {
-                        if ((fl >> (kl + 1)) & 1u) acc = acc + pci * sb[me + kl + 1];
-                    } else if (halo_bwd && hzp_fl) {
-                        acc = acc + pci * hzp;
-                    }
-                    if (halo_bwd && il == 0 && ((hx_fl >> kl) & 1u)) acc = acc + pci * hx[jl * kPad + kl];
-                    if (halo_bwd && jl == 0 && ((hy_fl >> kl) & 1u)) acc = acc + pci * hy[il * kPad + kl];
-                    if (halo_bwd && kl == 0 && hzm_fl) acc = acc + pci * hzm;
-                    z = acc * pci;
-                }
-                sb[me + kl] = z;
+    if constexpr (do_bwd) {
+        // upper neighbours: +x, +y, +z (in tile), -x, -y, -z (halo, PH 2)
+        const int b_xp = il < 7 ? a_sb + 8 * kPad
+                                : (halo_bwd ? o_hx + kTile * kPad + jl * kPad - sl : a_zero);
+        const int b_yp = jl < 7 ? a_sb + kPad
+                                : (halo_bwd ? o_hy + kTile * kPad + il * kPad - sl : a_zero);
+        const int b_xm = (halo_bwd && il == 0) ? o_hx + jl * kPad - sl : a_zero;
+        const int b_ym = (halo_bwd && jl == 0) ? o_hy + il * kPad - sl : a_zero;
+        const float hzp_b = halo_bwd ? hzp : -0.0f;
+#pragma unroll
+        for (int st = 0; st < kSteps; ++st) {
+            constexpr int kLast = kSteps - 1;             // kl = 21 - st - sl
+            const int o = kLast - st;
+            const float pci = lds[a_pc + o];
+            float acc = lds[a_sa + o];
+            acc = acc + pci * lds[b_xp + o];
+            acc = acc + pci * lds[b_yp + o];
+            const float vzp = lds[a_sb + 1 + o];          // sb[kl + 1]
+            acc = acc + pci * (sl + 7 == o ? hzp_b : vzp);
+            if constexpr (halo_bwd) {
+                acc = acc + pci * lds[b_xm + o];
+                acc = acc + pci * lds[b_ym + o];
+                acc = acc + pci * (sl == o ? hzm : -0.0f);
+            }
+            const bool fluid = (fls >> o) & 1u;
+            const float z = fluid ? acc * pci : 0.0f;
+            if ((unsigned)(o - sl) < (unsigned)kTile && active) {
+                lds[a_sa + o] = z;
+                lds[a_sb + o] = fluid ? z : -0.0f;
             }
             wave_sync();
         }
@@ -966,42 +994,117 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
         for (int m = 0; m < kTile; ++m) {
             const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
             const int li = (m * 8 + cj) * kPad + ce;
-            if (do_bwd) {
-                const float z = sb[li];
-                Z[g] = z;
-                dot[0] += (double)z * (double)rreg[m];
+            const float v = lds[o_sa + li];
+            if constexpr (do_bwd) {
+                Z[g] = v;
+                dot[0] += (double)v * (double)rreg[m];
             } else {
-                Q[g] = sa[li];
+                Q[g] = v;
             }
         }
     }
-    if (phase == 0) return;
+    if constexpr (PH == 0) return;
     block_partials<1>(dot, red.partials, slot_base + blockIdx.x);
 }
 
-// Jacobi: z = r * (1/diag), rho = z.r
-__global__ void k_jacobi_apply(int64_t n, const float* __restrict__ pc, const float* __restrict__ r,
-                               float* __restrict__ Z, Reduce red) {
+// Jacobi: z = r * (1/diag), rho = z.r over the active segments
+__global__ void k_jacobi_apply(const float* __restrict__ pc, const float* __restrict__ r,
+                               float* __restrict__ Z, const int32_t* __restrict__ segs,
+                               int64_t nseg, Reduce red) {
     if (red.sc->done) return;
     double v[1] = {0.0};
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        const float z = r[c] * pc[c];
-        Z[c] = z;
-        v[0] += (double)z * (double)r[c];
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c0 = 8 * (int64_t)segs[u];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int64_t c = c0 + m;
+            const float z = r[c] * pc[c];
+            Z[c] = z;
+            v[0] += (double)z * (double)r[c];
+        }
     }
     block_partials<1>(v, red.partials, blockIdx.x);
+}
+
+// ------------------------------------------------------------ active sets
+__global__ void k_seg_flags(int64_t nseg, const uint8_t* __restrict__ fl8, int32_t* __restrict__ f) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nseg) f[t] = fl8[t] != 0 ? 1 : 0;
+}
+
+__global__ void k_seg_compact(int64_t nseg, const int32_t* __restrict__ f,
+                              const int32_t* __restrict__ pos, int32_t* __restrict__ list,
+                              SolverScalars* sc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nseg) return;
+    if (f[t]) list[pos[t]] = (int32_t)t;
+    if (t == nseg - 1) sc->nseg = (int64_t)pos[t] + f[t];
+}
+
+// any[t] = 1 when one of tile t's 64 z-segments holds a FLUID cell
+__global__ void k_tile_any(int N, const uint8_t* __restrict__ fl8, int32_t* __restrict__ any) {
+    const int TN = N / kTile;
+    const int64_t nt = (int64_t)TN * TN * TN;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int tk = (int)(t % TN), tj = (int)((t / TN) % TN), ti = (int)(t / ((int64_t)TN * TN));
+    const int NS = N / kTile;
+    uint32_t m = 0;
+    for (int il = 0; il < kTile; ++il)
+        for (int jl = 0; jl < kTile; ++jl)
+            m |= fl8[((int64_t)(ti * kTile + il) * N + tj * kTile + jl) * NS + tk];
+    any[t] = m != 0 ? 1 : 0;
+}
+
+// One block: ascending lists of the active red, black and all tiles
+// (lists + 0, + nt, + 2 nt) and their counts.
+__global__ __launch_bounds__(1024) void k_tile_lists(int N, const int32_t* __restrict__ any,
+                                                      int32_t* __restrict__ lists,
+                                                      SolverScalars* sc) {
+    const int TN = N / kTile;
+    const int nt = TN * TN * TN;
+    __shared__ int wsum[3][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int base[3] = {0, 0, 0};
+    for (int t0 = 0; t0 < nt; t0 += blockDim.x) {
+        const int t = t0 + threadIdx.x;
+        int f[3] = {0, 0, 0};
+        if (t < nt && any[t]) {
+            const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
+            const int col = (ti + tj + tk) & 1;
+            f[col] = 1;
+            f[2] = 1;
+        }
+        int pre[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint64_t b = __ballot(f[c]);
+            pre[c] = __popcll(b & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[c][w] = __popcll(b);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            int off = base[c];
+            for (int ww = 0; ww < w; ++ww) off += wsum[c][ww];
+            if (f[c]) lists[(int64_t)c * nt + off + pre[c]] = t;
+            int tot = 0;
+            for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) tot += wsum[c][ww];
+            base[c] += tot;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sc->ntile[0] = base[0];
+        sc->ntile[1] = base[1];
+        sc->ntile[2] = base[2];
+    }
 }
 
 int stream_blocks(int64_t n) {
     int64_t b = (n + kBlock - 1) / kBlock;
     return (int)(b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks);
-}
-
-int tile_blocks(int N) {
-    const int64_t TN = N / kTile;
-    const int64_t nt = TN * TN * TN;
-    return (int)(nt < 4096 ? nt : 4096);
 }
 
 }  // namespace
@@ -1025,14 +1128,17 @@ void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, cons
 }
 
 void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
-                          const uint8_t* nup, float* pc) {
+                          const uint8_t* nup, float* pc, const Active& act) {
     if (p.precond == 2) return;   // Jacobi: built by setup
-    const int nb = tile_blocks(p.N);
+    auto nb = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(n, 4096)); };
     if (p.precond == 0) {
-        hipLaunchKernelGGL(k_mic_build, dim3(nb), dim3(64), 0, s, p, A, diag, nup, pc, 0);
-        hipLaunchKernelGGL(k_mic_build, dim3(nb), dim3(64), 0, s, p, A, diag, nup, pc, 1);
+        hipLaunchKernelGGL(k_mic_build, dim3(nb(act.ntiles[0])), dim3(64), 0, s, p, A, diag, nup, pc,
+                           0, act.tiles[0], act.ntiles[0]);
+        hipLaunchKernelGGL(k_mic_build, dim3(nb(act.ntiles[1])), dim3(64), 0, s, p, A, diag, nup, pc,
+                           1, act.tiles[1], act.ntiles[1]);
     } else {
-        hipLaunchKernelGGL(k_mic_build, dim3(nb), dim3(64), 0, s, p, A, diag, nup, pc, -1);
+        hipLaunchKernelGGL(k_mic_build, dim3(nb(act.ntiles[2])), dim3(64), 0, s, p, A, diag, nup, pc,
+                           -1, act.tiles[2], act.ntiles[2]);
     }
 }
 
@@ -1041,28 +1147,53 @@ void launch_fluid_segments(hipStream_t s, int N, const uint8_t* A, uint8_t* fl8)
     hipLaunchKernelGGL(k_fluid_segments, dim3(grid_for(nseg, kBlock)), dim3(kBlock), 0, s, N, A, fl8);
 }
 
+void launch_active_sets(hipStream_t s, int N, const uint8_t* fl8, int32_t* seg_flag,
+                        int32_t* seg_scan, int32_t* seg_list, int32_t* tile_any,
+                        int32_t* tile_lists, int32_t* scan_sums, SolverScalars* sc) {
+    const int64_t nseg = (int64_t)N * N * (N / kTile);
+    hipLaunchKernelGGL(k_seg_flags, dim3(grid_for(nseg, kBlock)), dim3(kBlock), 0, s, nseg, fl8,
+                       seg_flag);
+    launch_scan(s, seg_flag, seg_scan, nseg, scan_sums);
+    hipLaunchKernelGGL(k_seg_compact, dim3(grid_for(nseg, kBlock)), dim3(kBlock), 0, s, nseg,
+                       (const int32_t*)seg_flag, (const int32_t*)seg_scan, seg_list, sc);
+    const int64_t TN = N / kTile;
+    const int64_t nt = TN * TN * TN;
+    hipLaunchKernelGGL(k_tile_any, dim3(grid_for(nt, kBlock)), dim3(kBlock), 0, s, N, fl8, tile_any);
+    hipLaunchKernelGGL(k_tile_lists, dim3(1), dim3(1024), 0, s, N, (const int32_t*)tile_any,
+                       tile_lists, sc);
+}
+
+static int list_blocks(int64_t n) {
+    const int64_t b = (n + kBlock - 1) / kBlock;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
+}
+
 void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, const float* pc,
-                          const float* r, float* q, float* z, const Reduce& red, int) {
-    const int64_t nc = (int64_t)p.N * p.N * p.N;
+                          const float* r, float* q, float* z, const Reduce& red,
+                          const Active& act) {
     if (p.precond == 2) {
-        const int nb = stream_blocks(nc);
-        hipLaunchKernelGGL(k_jacobi_apply, dim3(nb), dim3(kBlock), 0, s, nc, pc, r, z, red);
+        const int nb = list_blocks(act.nseg);
+        hipLaunchKernelGGL(k_jacobi_apply, dim3(nb), dim3(kBlock), 0, s, pc, r, z, act.segs,
+                           act.nseg, red);
         launch_finalize(s, red, FIN_RHO, nb);
         return;
     }
-    const int TN = p.N / kTile;
+    auto blocks = [](int64_t nt) {
+        return (int)std::max<int64_t>(1, (nt + kTilesPerBlock - 1) / kTilesPerBlock);
+    };
     if (p.precond == 0) {
-        const int64_t per_color = (TN & 1) == 0 ? (int64_t)TN * TN * TN / 2 : (int64_t)TN * TN * TN;
-        const int nb = (int)((per_color + kTilesPerBlock - 1) / kTilesPerBlock);
-        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 0, 0, 0);
-        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 1, 0, 0);
-        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 2, nb,
-                           2 * nb);
-        launch_finalize(s, red, FIN_RHO, 2 * nb);
+        const int nr = blocks(act.ntiles[0]), nbk = blocks(act.ntiles[1]);
+        hipLaunchKernelGGL(k_mic_apply<0>, dim3(nr), dim3(256), 0, s, p, fl8, pc, r, q, z,
+                           act.tiles[0], act.ntiles[0], red, 0);
+        hipLaunchKernelGGL(k_mic_apply<1>, dim3(nbk), dim3(256), 0, s, p, fl8, pc, r, q, z,
+                           act.tiles[1], act.ntiles[1], red, 0);
+        hipLaunchKernelGGL(k_mic_apply<2>, dim3(nr), dim3(256), 0, s, p, fl8, pc, r, q, z,
+                           act.tiles[0], act.ntiles[0], red, nbk);
+        launch_finalize(s, red, FIN_RHO, nbk + nr);
     } else {
-        const int64_t all = (int64_t)TN * TN * TN;
-        const int nb = (int)((all + kTilesPerBlock - 1) / kTilesPerBlock);
-        hipLaunchKernelGGL(k_mic_apply, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z, red, 3, 0, nb);
+        const int nb = blocks(act.ntiles[2]);
+        hipLaunchKernelGGL(k_mic_apply<3>, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z,
+                           act.tiles[2], act.ntiles[2], red, 0);
         launch_finalize(s, red, FIN_RHO, nb);
     }
 }
@@ -1078,47 +1209,50 @@ void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const fl
 }
 
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                     const float* x, float* y, const Reduce& red) {
-    const int64_t nc = (int64_t)p.N * p.N * p.N;
-    const int nb = stream_blocks(nc / 4);
-    hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(kBlock), 0, s, p, A, L, x, y, red);
+                     const float* x, float* y, const Reduce& red, const Active& act) {
+    const int nb = list_blocks(act.nseg);
+    hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(kBlock), 0, s, p, A, L, x, y, act.segs, act.nseg,
+                       red);
     launch_finalize(s, red, FIN_SIGMA, nb);
 }
 
-void launch_axpy2(hipStream_t s, int64_t n, float* P, const float* S, float* R, const float* Z,
-                  const Reduce& red) {
-    const int nb = stream_blocks(n / 8);
-    hipLaunchKernelGGL(k_axpy2, dim3(nb), dim3(kBlock), 0, s, n, P, S, R, Z, red);
+void launch_axpy2(hipStream_t s, float* P, const float* S, float* R, const float* Z,
+                  const Reduce& red, const Active& act) {
+    const int nb = list_blocks(act.nseg);
+    hipLaunchKernelGGL(k_axpy2, dim3(nb), dim3(kBlock), 0, s, P, S, R, Z, act.segs, act.nseg, red);
     launch_finalize(s, red, FIN_RNORM, nb);
 }
 
 void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                            float* P, double* P64, const float* D, float* R, const Reduce& red) {
-    const int64_t nc = (int64_t)p.N * p.N * p.N;
-    const int nb = stream_blocks(nc);
-    hipLaunchKernelGGL(k_flush, dim3(stream_blocks(nc)), dim3(kBlock), 0, s, nc, A, P, P64, red.sc);
-    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, p, A, L,
-                       (const float*)P, (const double*)P64, D, R, red, 0);
+                            float* P, double* P64, const float* D, float* R, const Reduce& red,
+                            const Active& act) {
+    const int nb = list_blocks(act.nseg);
+    hipLaunchKernelGGL(k_flush, dim3(nb), dim3(kBlock), 0, s, A, P, P64, act.segs, act.nseg,
+                       (const SolverScalars*)red.sc);
+    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, p, A, L, (const float*)P,
+                       (const double*)P64, D, R, act.segs, act.nseg, red, 0);
     launch_finalize(s, red, FIN_TRUE, nb);
 }
 
-void launch_refine_finish(hipStream_t s, int64_t n, const uint8_t* A, float* P, const double* P64) {
-    hipLaunchKernelGGL(k_refine_finish, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, A, P, P64);
+void launch_refine_finish(hipStream_t s, const uint8_t* A, float* P, const double* P64,
+                          const Active& act) {
+    hipLaunchKernelGGL(k_refine_finish, dim3(list_blocks(act.nseg)), dim3(kBlock), 0, s, A, P, P64,
+                       act.segs, act.nseg);
 }
 
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                               const float* P, const float* D, const Reduce& red) {
-    const int64_t nc = (int64_t)p.N * p.N * p.N;
-    const int nb = stream_blocks(nc);
+                               const float* P, const float* D, const Reduce& red,
+                               const Active& act) {
+    const int nb = list_blocks(act.nseg);
     hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, p, A, L, P,
-                       (const double*)nullptr, D, (float*)nullptr, red, 1);
+                       (const double*)nullptr, D, (float*)nullptr, act.segs, act.nseg, red, 1);
     launch_finalize(s, red, FIN_REPORT, nb);
 }
 
-void launch_update_s(hipStream_t s, int64_t n, const float* Z, float* S, const Reduce& red,
-                     int first) {
-    hipLaunchKernelGGL(k_update_s, dim3(stream_blocks(n / 4)), dim3(kBlock), 0, s, n, Z, S, red.sc,
-                       first);
+void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,
+                     const Active& act) {
+    hipLaunchKernelGGL(k_update_s, dim3(list_blocks(act.nseg)), dim3(kBlock), 0, s, Z, S, act.segs,
+                       act.nseg, (const SolverScalars*)red.sc, first);
 }
 
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,

@@ -276,6 +276,16 @@ int ariel_load_state(ariel_sim* h, const char* path);
  * 101-111), as a raw little-endian float32 volume with a text header. */
 int ariel_export_levelset(ariel_sim* h, const char* path);
 
+/* Self-check of the device division routines against IEEE division on the
+ * current device (no counterpart in the reference: it guards the bit-exactness
+ * of Smooth and Sharpen inside the kernels). For every float x with
+ * lo <= x <= hi (lo >= 0):
+ *   which = 0: x / a by the fixed-divisor routine (ComputeDensity's Smooth);
+ *              for x < 2^-100 the compared value is max(1 - x/a, 0);
+ *   which = 1: a / x by the reciprocal-based routine (Sharpen).
+ * *mismatches receives the number of x whose result differs in any bit. */
+int ariel_selftest_division(int32_t which, float a, float lo, float hi, int64_t* mismatches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,7 @@ def test_init_matches(n, precond):
     lambda: scenes.dam_break(32, splat=_abi.SPLAT_TRILINEAR, precond=_abi.PRECOND_JACOBI),
     lambda: scenes.dam_break_sphere(48, precond=_abi.PRECOND_MIC0_BLOCK),
     lambda: scenes.pool_and_cube(32, refine_every=7),
+    lambda: scenes.pool_and_cube(64),
 ])
 def test_stagewise_parity(desc_fn):
     d = desc_fn()
