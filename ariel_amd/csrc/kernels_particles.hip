@@ -186,37 +186,173 @@ __global__ void k_gather(const int32_t* __restrict__ slots, int64_t np, const fl
     vout[s] = vin[src];
 }
 
+// ------------------------------------------------ neighbourhood staging
+// A block of 256 consecutive sorted particles sits in a few cells of one or two
+// grid columns, so the cells its particles query form a small box. When the box
+// spans at most kStageCols columns and its particles fit kStageCap, each
+// column's z-range of the box is copied (x, y, z) into LDS with coalesced loads,
+// all in flight at once, and the neighbour loops read LDS instead of issuing a
+// dependent global load per candidate. Candidate order and values are unchanged.
+constexpr int kStageBlock = 256;   // threads (particles) per staging block
+constexpr int kStageCap = 2560;
+constexpr int kStageCols = 16;
+constexpr int kStagePer = kStageCap / kStageBlock;   // staged particles per thread
+
+struct StageLds {
+    float x[kStageCap], y[kStageCap], z[kStageCap];
+    int q0[kStageCols];     // first particle of each column's staged range
+    int base[kStageCols];   // its LDS index
+    int box[6];             // x0, x1, y0, y1, z0, z1 (cells, inclusive)
+    int ncols, ny, ok, total;
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Every thread of the block calls this; valid threads pass their query box.
+// Returns with st.ok set (block-uniform) and, if set, the columns staged.
+__device__ void stage_neighbourhood(StageLds& st, bool valid, int x0, int x1, int y0, int y1,
+                                    int z0, int z1, int N, const int32_t* __restrict__ offsets,
+                                    const float4* __restrict__ pos) {
+    if (threadIdx.x == 0) {
+        st.box[0] = st.box[2] = st.box[4] = 1 << 30;
+        st.box[1] = st.box[3] = st.box[5] = -1;
+    }
+    __syncthreads();
+    const int big = 1 << 30;
+    const int a0 = wave_min(valid ? x0 : big), a1 = wave_max(valid ? x1 : -1);
+    const int b0 = wave_min(valid ? y0 : big), b1 = wave_max(valid ? y1 : -1);
+    const int c0 = wave_min(valid ? z0 : big), c1 = wave_max(valid ? z1 : -1);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&st.box[0], a0);
+        atomicMax(&st.box[1], a1);
+        atomicMin(&st.box[2], b0);
+        atomicMax(&st.box[3], b1);
+        atomicMin(&st.box[4], c0);
+        atomicMax(&st.box[5], c1);
+    }
+    __syncthreads();
+    const int bx0 = st.box[0], bx1 = st.box[1], by0 = st.box[2], by1 = st.box[3];
+    const int bz0 = st.box[4], bz1 = st.box[5];
+    const int ny = by1 - by0 + 1;
+    const int ncols = bx1 < bx0 ? 0 : (bx1 - bx0 + 1) * ny;
+    if (ncols <= 0 || ncols > kStageCols) {
+        if (threadIdx.x == 0) st.ok = 0;
+        __syncthreads();
+        return;
+    }
+    int len = 0;
+    if ((int)threadIdx.x < ncols) {
+        const int cx = bx0 + (int)threadIdx.x / ny, cy = by0 + (int)threadIdx.x % ny;
+        const int64_t col = cidx(N, cx, cy, 0);
+        const int q0 = offsets[col + bz0];
+        len = offsets[col + bz1 + 1] - q0;
+        st.q0[threadIdx.x] = q0;
+        st.base[threadIdx.x] = len;   // lengths, turned into bases below
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int c = 0; c < ncols; ++c) {
+            const int l = st.base[c];
+            st.base[c] = acc;
+            acc += l;
+        }
+        st.ok = acc <= kStageCap;
+        st.ncols = ncols;
+        st.ny = ny;
+        st.total = acc;
+    }
+    __syncthreads();
+    if (!st.ok) return;
+    const int total = st.total;
+    float4 t[kStagePer];
+#pragma unroll
+    for (int m = 0; m < kStagePer; ++m) {
+        const int u = (int)threadIdx.x + m * kStageBlock;
+        if (u < total) {
+            int c = 0;
+            while (c + 1 < ncols && st.base[c + 1] <= u) ++c;
+            t[m] = pos[st.q0[c] + (u - st.base[c])];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kStagePer; ++m) {
+        const int u = (int)threadIdx.x + m * kStageBlock;
+        if (u < total) {
+            st.x[u] = t[m].x;
+            st.y[u] = t[m].y;
+            st.z[u] = t[m].z;
+        }
+    }
+    __syncthreads();
+}
+
+// Particle index of LDS slot l of a staged box.
+__device__ __forceinline__ int staged_particle(const StageLds& st, int l) {
+    int c = 0;
+    while (c + 1 < st.ncols && st.base[c + 1] <= l) ++c;
+    return st.q0[c] + (l - st.base[c]);
+}
+
+// LDS index of particle q of column (cx, cy) inside a staged box.
+__device__ __forceinline__ int staged_index(const StageLds& st, int cx, int cy, int q) {
+    const int c = (cx - st.box[0]) * st.ny + (cy - st.box[2]);
+    return st.base[c] + (q - st.q0[c]);
+}
+
 // ---------------------------------------------------------- ComputeDensity
 // flip.cpp:697-733 + ApplyExternalForces (flip.cpp:682-695) fused: each particle
 // gathers Smooth weights (h = 4*density/maxd) over the 3x3x3 cells around its
 // cell; the 3 cells along z of each (sx,sy) column are one contiguous range.
-__global__ void k_density(Params p, const float4* __restrict__ pos, float4* __restrict__ vel,
-                          int64_t np, const int32_t* __restrict__ offsets, int apply_forces) {
+__global__ __launch_bounds__(kStageBlock) void k_density(Params p, const float4* __restrict__ pos,
+                                                    float4* __restrict__ vel, int64_t np,
+                                                    const int32_t* __restrict__ offsets,
+                                                    int apply_forces) {
+    __shared__ StageLds st;
     const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
-    if (s >= np) return;
+    const bool valid = s < np;
     const int N = p.N;
     const float maxd = p.maxd;
-    const float4 me = pos[s];
+    const float4 me = valid ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int cx = cell_coord(me.x, maxd), cy = cell_coord(me.y, maxd), cz = cell_coord(me.z, maxd);
     const float hsm = 4.0f * p.density / maxd;
     const float hh = hsm * hsm, yhh = 1.0f / hh;
+    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, N - 1);
+    const int y0 = max(cy - 1, 0), y1 = min(cy + 1, N - 1);
     const int z0 = max(cz - 1, 0), z1 = min(cz + 1, N - 1);
+    stage_neighbourhood(st, valid, x0, x1, y0, y1, z0, z1, N, offsets, pos);
+    if (!valid) return;
     float weightsum = 0.0f;
-    for (int sx = cx - 1; sx <= cx + 1; ++sx) {
-        if (sx < 0 || sx > N - 1) continue;
-        for (int sy = cy - 1; sy <= cy + 1; ++sy) {
-            if (sy < 0 || sy > N - 1) continue;
-            const int64_t col = cidx(N, sx, sy, 0);
-            const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
-            // four loads in flight, terms added in list order
-            for (int32_t q = q0; q < q1; q += 4) {
-                float4 o4[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) o4[u] = pos[min(q + u, q1 - 1)];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (q + u >= q1) break;
-                    const float4 o = o4[u];
+    if (st.ok) {
+        for (int sx = x0; sx <= x1; ++sx) {
+            for (int sy = y0; sy <= y1; ++sy) {
+                const int64_t col = cidx(N, sx, sy, 0);
+                const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
+                const int l0 = staged_index(st, sx, sy, q0);
+                const int l1 = l0 + (q1 - q0);
+                for (int l = l0; l < l1; ++l) {
+                    float sqd = sqrlength(st.x[l], st.y[l], st.z[l], me.x, me.y, me.z);
+                    float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
+                    weightsum = weightsum + weight;
+                }
+            }
+        }
+    } else {
+        for (int sx = x0; sx <= x1; ++sx) {
+            for (int sy = y0; sy <= y1; ++sy) {
+                const int64_t col = cidx(N, sx, sy, 0);
+                const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
+                for (int32_t q = q0; q < q1; ++q) {
+                    const float4 o = pos[q];
                     float sqd = sqrlength(o.x, o.y, o.z, me.x, me.y, me.z);
                     float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
                     weightsum = weightsum + weight;
@@ -347,6 +483,7 @@ __global__ void k_p2g(Params p, const float4* __restrict__ pos, const float4* __
 // same as the per-face gather's. Weights beyond the kernel radius are exactly
 // zero and skipped (adding +0 to a sum that starts at +0 never changes it).
 constexpr int kP2GCap = 512;
+__device__ __forceinline__ int p2g_slot(int t) { return t + (t >> 3); }
 
 __device__ __forceinline__ float p2g_weight(bool tri, float px, float py, float pz, float fx,
                                             float fy, float fz, float RE, float hh) {
@@ -368,16 +505,28 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
                                                   float* __restrict__ ux, float* __restrict__ uy,
                                                   float* __restrict__ uz, float* __restrict__ dux,
                                                   float* __restrict__ duy, float* __restrict__ duz) {
-    __shared__ float sp[6][kP2GCap];
+    // staged particle t lives at p2g_slot(t): one pad word per 8 particles keeps
+    // lanes whose ranges start a cell (about 8 particles) apart on distinct banks
+    __shared__ float sp[6][kP2GCap + kP2GCap / 8];
     const int N = p.N;
     const int nkc = (N + 1 + 63) / 64;
-    const int64_t nbl = (int64_t)(N + 1) * (N + 1) * nkc;
+    // Blocks run over 8x8 tiles of face columns (i, j), k-chunks innermost, so
+    // the particle columns that neighbouring face columns share are staged while
+    // still in L2; each XCD takes a contiguous run of tiles.
+    constexpr int kT = 8;
+    const int nti = (N + 1 + kT - 1) / kT;
+    const int64_t nbl = (int64_t)nti * nti * kT * kT * nkc;
     int64_t b = xcd_block(nbl);
     if (b >= nbl) return;
     const int kc = (int)(b % nkc);
     b /= nkc;
-    const int j = (int)(b % (N + 1));
-    const int i = (int)(b / (N + 1));
+    const int jj = (int)(b % kT);
+    b /= kT;
+    const int ii = (int)(b % kT);
+    b /= kT;
+    const int j = (int)(b % nti) * kT + jj;
+    const int i = (int)(b / nti) * kT + ii;
+    if (i > N || j > N) return;
     const int lane = threadIdx.x;
     const int k0 = kc * 64;
     const int k = k0 + lane;
@@ -418,7 +567,7 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
                 for (int32_t q = c0 + lane; q < c1; q += 64) {
                     const float4 o = pos[q];
                     const float4 v = vel[q];
-                    const int t = q - c0;
+                    const int t = p2g_slot(q - c0);
                     sp[0][t] = gmax(0.0f, gmin(maxd, maxd * o.x));
                     sp[1][t] = gmax(0.0f, gmin(maxd, maxd * o.y));
                     sp[2][t] = gmax(0.0f, gmin(maxd, maxd * o.z));
@@ -429,7 +578,7 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
                 __syncthreads();
                 const int32_t q0 = max(ra0, c0), q1 = min(ra1, c1);
                 for (int32_t q = q0; q < q1; ++q) {
-                    const int t = q - c0;
+                    const int t = p2g_slot(q - c0);
                     const float px = sp[0][t], py = sp[1][t], pz = sp[2][t];
                     if (dox) {
                         const float w = p2g_weight(tri, px, py, pz, fi, fj + 0.5f, fk + 0.5f, RE, hh);
@@ -554,28 +703,24 @@ __device__ __forceinline__ void narrow_z(float z, float gxy2, float gbound, floa
     }
 }
 
-constexpr int kSpringList = 6;   // queued spring neighbours per thread (LDS)
+constexpr int kSpringQueue = 8;   // queued in-radius neighbours per thread
 
-__global__ __launch_bounds__(kBlock) void k_resample(Params p, const float4* __restrict__ pos,
+__global__ __launch_bounds__(kStageBlock) void k_resample(Params p, const float4* __restrict__ pos,
                                                      const float4* __restrict__ vel, int64_t np,
                                                      const int32_t* __restrict__ offsets,
                                                      float4* __restrict__ pout,
                                                      float4* __restrict__ vout,
                                                      uint8_t* __restrict__ accept) {
-    // Neighbours inside the spring radius are queued per thread (in visiting
-    // order) and their terms evaluated in batches, so that the wave does not run
-    // the square root and divisions on every visit for the few lanes that need
-    // them. Each thread owns column threadIdx.x of the queue.
-    __shared__ float4 s_nb[kSpringList][kBlock];
+    __shared__ StageLds st;
+    __shared__ int s_queue[kSpringQueue][kStageBlock];
     const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
-    if (s >= np) return;
+    const bool valid = s < np;
     const int N = p.N;
     const float maxd = p.maxd;
     const float re = p.density / maxd;
     const float dt = p.dt;
     const float springforce = p.spring;
-    const float4 me = pos[s];
-    const float4 mv = vel[s];
+    const float4 me = valid ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t my_id = (uint32_t)__float_as_int(me.w);
     const float re2 = re * re;
     const float yre2 = 1.0f / re2;
@@ -587,30 +732,26 @@ __global__ __launch_bounds__(kBlock) void k_resample(Params p, const float4* __r
     const float margin = 1.0e-3f + 4.0e-7f * maxd;
     const uint32_t fr = (uint32_t)p.frame;
     float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
-    int queued = 0;
-    // Spring term of one neighbour (flip.cpp Resample's first loop): dx = me - o.
-    auto spring_terms = [&]() {
-        for (int t = 0; t < queued; ++t) {
-            const float4 e = s_nb[t][threadIdx.x];
-            const float dx = e.x, dy = e.y, dz = e.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            float dist = sqrtf(d2);
-            float w = springforce * 1.0f * smooth_fixed(dist * dist, re2, yre2);
-            if (dist > 0.1f * re) {
-                sx_ += w * dx / dist * re;
-                sy_ += w * dy / dist * re;
-                sz_ += w * dz / dist * re;
-            } else {
-                const uint32_t qid = (uint32_t)__float_as_int(e.w);
-                uint32_t hx = hash4(p.seed, 3u * fr + 0u, my_id, qid);
-                uint32_t hy = hash4(p.seed, 3u * fr + 1u, my_id, qid);
-                uint32_t hz = hash4(p.seed, 3u * fr + 2u, my_id, qid);
-                sx_ += 0.01f * re / dt * (float)(int)(hx % 101u) / 100.0f;
-                sy_ += 0.01f * re / dt * (float)(int)(hy % 101u) / 100.0f;
-                sz_ += 0.01f * re / dt * (float)(int)(hz % 101u) / 100.0f;
-            }
+    // spring term of one neighbour within re (first loop of Resample); dx = me - o
+    // q is the neighbour's particle index, or its LDS slot when `slot` is set
+    // (resolved only on the rare coincident-pair path)
+    auto spring = [&](float dx, float dy, float dz, float d2, int32_t q, bool slot) {
+        float dist = sqrtf(d2);
+        float w = springforce * 1.0f * smooth_fixed(dist * dist, re2, yre2);
+        if (dist > 0.1f * re) {
+            sx_ += w * dx / dist * re;
+            sy_ += w * dy / dist * re;
+            sz_ += w * dz / dist * re;
+        } else {
+            const int32_t qq = slot ? staged_particle(st, q) : q;
+            const uint32_t qid = (uint32_t)__float_as_int(pos[qq].w);
+            uint32_t hx = hash4(p.seed, 3u * fr + 0u, my_id, qid);
+            uint32_t hy = hash4(p.seed, 3u * fr + 1u, my_id, qid);
+            uint32_t hz = hash4(p.seed, 3u * fr + 2u, my_id, qid);
+            sx_ += 0.01f * re / dt * (float)(int)(hx % 101u) / 100.0f;
+            sy_ += 0.01f * re / dt * (float)(int)(hy % 101u) / 100.0f;
+            sz_ += 0.01f * re / dt * (float)(int)(hz % 101u) / 100.0f;
         }
-        queued = 0;
     };
     {
         float x = gmax(0.0f, gmin(maxd, maxd * me.x));
@@ -621,43 +762,108 @@ __global__ __launch_bounds__(kBlock) void k_resample(Params p, const float4* __r
         cell_range(x, N, xl, xh);
         cell_range(y, N, yl, yh);
         cell_range(z, N, zl, zh);
-        for (int cx = xl; cx <= xh; ++cx) {
-            const float gx = cell_gap(xg, cx, margin);
-            for (int cy = yl; cy <= yh; ++cy) {
-                const float gy = cell_gap(yg, cy, margin);
-                const float gxy2 = gx * gx + gy * gy;
+        stage_neighbourhood(st, valid && zl <= zh, xl, xh, yl, yh, zl, zh, N, offsets, pos);
+        if (!valid) return;
+        // With the neighbourhood staged, in-radius neighbours are queued (LDS
+        // index, in visiting order) and their spring terms evaluated in batches:
+        // evaluated inline, nearly every iteration of the wave would run the
+        // square root and divisions for the few lanes with a neighbour in range.
+        int queued = 0;
+        // An entry is the neighbour's LDS slot when the block is staged and its
+        // particle index otherwise.
+        const bool staged = st.ok;
+        auto flush = [&]() {
+            for (int t = 0; t < queued; ++t) {
+                const int e = s_queue[t][threadIdx.x];
+                float ox, oy, oz;
+                if (staged) {
+                    ox = st.x[e]; oy = st.y[e]; oz = st.z[e];
+                } else {
+                    const float4 o = pos[e];
+                    ox = o.x; oy = o.y; oz = o.z;
+                }
+                const float dx = me.x - ox, dy = me.y - oy, dz = me.z - oz;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                spring(dx, dy, dz, d2, e, staged);
+            }
+            queued = 0;
+        };
+        // Per axis, the cells of the range whose distance bound is within the
+        // spring radius, ascending (at most 3, normally 2). Columns are taken in
+        // the reference's order (cx, then cy, ascending) from these lists, so a
+        // wave iterates about 2 x 2 columns instead of 3 x 3 of which each lane
+        // needs some.
+        int cxs[3], cys[3], nx = 0, ny = 0, za0 = 0, zb0 = -1;
+        float gxs[3], gys[3], gzs[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int c = xl + m;
+            const float g = cell_gap(xg, c, margin);
+            if (c <= xh && g * g <= gbound) { cxs[nx] = c; gxs[nx] = g; ++nx; }
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int c = yl + m;
+            const float g = cell_gap(yg, c, margin);
+            if (c <= yh && g * g <= gbound) { cys[ny] = c; gys[ny] = g; ++ny; }
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int c = zl + m;
+            const float g = cell_gap(zg, c, margin);
+            gzs[m] = g;
+            if (c <= zh && g * g <= gbound) {
+                if (zb0 < za0) za0 = c;
+                zb0 = c;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a >= nx) break;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                if (b >= ny) break;
+                const int cx = cxs[a], cy = cys[b];
+                const float gxy2 = gxs[a] * gxs[a] + gys[b] * gys[b];
                 if (gxy2 > gbound) continue;
-                int za = zl, zb = zh;
-                narrow_z(zg, gxy2, gbound, margin, za, zb);
+                // z cells of this column within the bound (ends of [za0, zb0])
+                int za = za0, zb = zb0;
+                while (za <= zb && gxy2 + gzs[za - zl] * gzs[za - zl] > gbound) ++za;
+                while (zb >= za && gxy2 + gzs[zb - zl] * gzs[zb - zl] > gbound) --zb;
                 if (za > zb) continue;
                 const int64_t col = cidx(N, cx, cy, 0);
                 const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
-                // four loads in flight, neighbours taken in list order
-                for (int32_t q = q0; q < q1; q += 4) {
-                    float4 o4[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) o4[u] = pos[min(q + u, q1 - 1)];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int32_t qq = q + u;
-                        if (qq >= q1) break;
-                        if (qq == (int32_t)s) continue;
-                        const float4 o = o4[u];
-                        float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+                if (staged) {
+                    const int l0 = staged_index(st, cx, cy, q0);
+                    for (int32_t q = q0, l = l0; q < q1; ++q, ++l) {
+                        if (q == (int32_t)s) continue;
+                        const float dx = me.x - st.x[l], dy = me.y - st.y[l], dz = me.z - st.z[l];
                         const float d2 = dx * dx + dy * dy + dz * dz;
-                        // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and then
-                        // dist > 0.1 re: the term adds +-0 to the spring. d2 above re^2
-                        // by a margin far beyond the rounding of sqrt and the square
-                        // guarantees that, so the pair is skipped without changing a bit.
+                        // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and
+                        // then dist > 0.1 re: the term adds +-0 to the spring. d2
+                        // above re^2 by a margin far beyond the rounding of sqrt and
+                        // the square guarantees that, so the pair is skipped without
+                        // changing a bit.
                         if (d2 > re2_skip) continue;
-                        s_nb[queued][threadIdx.x] = make_float4(dx, dy, dz, o.w);
-                        if (++queued == kSpringList) spring_terms();
+                        s_queue[queued][threadIdx.x] = l;
+                        if (++queued == kSpringQueue) flush();
+                    }
+                } else {
+                    for (int32_t q = q0; q < q1; ++q) {
+                        if (q == (int32_t)s) continue;
+                        const float4 o = pos[q];
+                        const float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        if (d2 > re2_skip) continue;
+                        s_queue[queued][threadIdx.x] = q;
+                        if (++queued == kSpringQueue) flush();
                     }
                 }
             }
         }
-        spring_terms();
+        flush();
     }
+    const float4 mv = vel[s];
     const float tx = me.x + dt * sx_, ty = me.y + dt * sy_, tz = me.z + dt * sz_;
     float rx = 0.0f, ry = 0.0f, rz = 0.0f, wsum = 0.0f;
     // With re^2 <= 1e-5 every Sharpen weight max(re^2/max(d2,1e-5) - 1, 0) is 0:
@@ -784,7 +990,7 @@ void launch_gather(hipStream_t s, const int32_t* slots, int64_t np, const float4
 void launch_density(hipStream_t s, const Params& p, const float4* pos, float4* vel, int64_t np,
                     const int32_t* offsets, int apply_forces) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_density, dim3((unsigned)xcd_grid(grid_for(np, kBlock))), dim3(kBlock), 0, s, p, pos, vel, np,
+    hipLaunchKernelGGL(k_density, dim3((unsigned)xcd_grid(grid_for(np, kStageBlock))), dim3(kStageBlock), 0, s, p, pos, vel, np,
                        offsets, apply_forces);
 }
 
@@ -797,7 +1003,8 @@ void launch_mark(hipStream_t s, const Params& p, const float4* vel, const int32_
 void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
                 const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3]) {
     const int64_t nkc = (p.N + 1 + 63) / 64;
-    const int64_t nb = (int64_t)(p.N + 1) * (p.N + 1) * nkc;
+    const int64_t nti = (p.N + 1 + 7) / 8;
+    const int64_t nb = nti * nti * 64 * nkc;
     hipLaunchKernelGGL(k_p2g_cols, dim3((unsigned)xcd_grid(nb)), dim3(64), 0, s, p, pos, vel, offsets, A,
                        u[0], u[1], u[2], du[0], du[1], du[2]);
 }
@@ -822,7 +1029,7 @@ void launch_resample(hipStream_t s, const Params& p, const float4* pos, const fl
                      int64_t np, const int32_t* offsets, float4* pos_out, float4* vel_out,
                      uint8_t* accept) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_resample, dim3((unsigned)xcd_grid(grid_for(np, kBlock))), dim3(kBlock), 0, s, p, pos, vel, np,
+    hipLaunchKernelGGL(k_resample, dim3((unsigned)xcd_grid(grid_for(np, kStageBlock))), dim3(kStageBlock), 0, s, p, pos, vel, np,
                        offsets, pos_out, vel_out, accept);
 }
 
