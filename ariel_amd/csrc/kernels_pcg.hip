@@ -2,10 +2,12 @@
 // FP32 vectors with FP64 reductions, and MIC(0) under a tiled red-black ordering.
 //
 // PCG state never leaves the device inside an iteration: every reduction writes
-// per-block FP64 partials; the block that arrives last (atomic counter) sums them
-// in a fixed order and updates the scalars (alpha, beta, residual, stop flag) in
+// per-block FP64 partials, and a one-block finalize kernel sums them in a fixed
+// order and updates the scalars (alpha, beta, residual, stop flag) in
 // SolverScalars. Later kernels read those scalars and return early once `done`
 // is set, so the host can queue iterations in chunks without synchronising.
+// The iterations run over ascending lists of the active z-segments and 8^3 tiles
+// (those holding a FLUID cell), built once per solve.
 //
 // MIC(0) (Bridson, "Fluid Simulation for Computer Graphics", ch. 5; the
 // reference's BuildPreconditioner/ApplyPreconditioner, solver.inl:104-286) is
@@ -202,7 +204,7 @@ __device__ __forceinline__ float xref_term(uint8_t t, float xq, const float* __r
 __device__ __forceinline__ void spmv_quad(const Params& p, const uint8_t* __restrict__ A,
                                           const float* __restrict__ L,
                                           const float* __restrict__ x, float* __restrict__ y,
-                                          int64_t t, double& acc) {
+                                          int64_t t, int i, int j, int k0, double& acc) {
     const int N = p.N;
     const int64_t NN = (int64_t)N * N;
     const float n = (float)N;
@@ -216,8 +218,6 @@ __device__ __forceinline__ void spmv_quad(const Params& p, const uint8_t* __rest
         Y4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
-    int i, j, k0;
-    decode_cell(N, c, i, j, k0);
     const float4 xc4 = X4[t];
     const float xs[4] = {xc4.x, xc4.y, xc4.z, xc4.w};
     const uint8_t as[4] = {ac.x, ac.y, ac.z, ac.w};
@@ -269,18 +269,33 @@ __device__ __forceinline__ void spmv_quad(const Params& p, const uint8_t* __rest
 
 // z = A s, sigma = z.s, alpha = rho / sigma over the active segments (two quads
 // each); y elsewhere keeps the zeros of the solve's set-up.
-__global__ void k_spmv_dot(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
-                           const float* __restrict__ x, float* __restrict__ y,
-                           const int32_t* __restrict__ segs, int64_t nseg, Reduce red) {
+// One active segment per thread; blocks take contiguous runs of the list and
+// each XCD a contiguous eighth of the blocks, so the +-x neighbour planes of a
+// segment are read by the same XCD that reads them as its own (shared L2).
+__global__ __launch_bounds__(256) void k_spmv_dot(Params p, const uint8_t* __restrict__ A,
+                                                   const float* __restrict__ L,
+                                                   const float* __restrict__ x,
+                                                   float* __restrict__ y,
+                                                   const int32_t* __restrict__ segs, int64_t nseg,
+                                                   int per_thread, Reduce red) {
     if (red.sc->done) return;
+    const int64_t per_block = (int64_t)blockDim.x * per_thread;
+    const int64_t nb = (nseg + per_block - 1) / per_block;
+    const int64_t b = xcd_block(nb);
+    if (b >= nb) return;
     double v[1] = {0.0};
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-         u += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t sg = segs[u];
-        spmv_quad(p, A, L, x, y, 2 * sg, v[0]);
-        spmv_quad(p, A, L, x, y, 2 * sg + 1, v[0]);
+    for (int m = 0; m < per_thread; ++m) {
+        const int64_t u = b * per_block + (int64_t)m * blockDim.x + threadIdx.x;
+        if (u >= nseg) break;
+        const uint32_t sg = (uint32_t)segs[u];
+        const uint32_t NS = (uint32_t)p.N / kTile;
+        const uint32_t line = sg / NS;   // i*N + j
+        const int k0 = (int)(sg - line * NS) * kTile;
+        const int j = (int)(line % (uint32_t)p.N), i = (int)(line / (uint32_t)p.N);
+        spmv_quad(p, A, L, x, y, 2 * (int64_t)sg, i, j, k0, v[0]);
+        spmv_quad(p, A, L, x, y, 2 * (int64_t)sg + 1, i, j, k0 + 4, v[0]);
     }
-    block_partials<1>(v, red.partials, blockIdx.x);
+    block_partials<1>(v, red.partials, b);
 }
 
 // Stop decision after the residual norm of iteration `it` is known.
@@ -815,7 +830,6 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
     constexpr bool halo_fwd = PH == 1;
     constexpr bool halo_bwd = PH == 2;
     constexpr bool halo = halo_fwd || halo_bwd;
-    if (red.sc->done) return;
     __shared__ float lds[kGuardLo + kTilesPerBlock * kWaveFloats + kGuardHi];
     const int N = p.N;
     const int TN = N / kTile;
@@ -831,8 +845,10 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
     int ti = 0, tj = 0, tk = 0;
     const int64_t idx = (int64_t)blockIdx.x * kTilesPerBlock + wave;
     const bool active = idx < ntiles;
+    // the tile index and the stop flag are fetched together
+    const int t = active ? tiles[idx] : 0;
+    if (red.sc->done) return;
     if (active) {
-        const int t = tiles[idx];
         tk = t % TN;
         tj = (t / TN) % TN;
         ti = t / (TN * TN);
@@ -912,11 +928,16 @@ __global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __re
 #pragma unroll
                 for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
             }
+            // z neighbours: values and fluid bits loaded together, masked after
             const int64_t line = cidx(N, gi, gj, k0);
-            if (k0 > 0 && ((fl8[((int64_t)gi * N + gj) * NS + tk - 1] >> 7) & 1u))
-                hzm = vb ? va[line - 1] * vb[line - 1] : va[line - 1];
-            if (k0 + kTile < N && (fl8[((int64_t)gi * N + gj) * NS + tk + 1] & 1u))
-                hzp = vb ? va[line + kTile] * vb[line + kTile] : va[line + kTile];
+            const int64_t sgl = ((int64_t)gi * N + gj) * NS + tk;
+            const bool has_m = k0 > 0, has_p = k0 + kTile < N;
+            const int64_t lm = has_m ? line - 1 : line, lp = has_p ? line + kTile : line;
+            const uint32_t fm = fl8[has_m ? sgl - 1 : sgl], fp = fl8[has_p ? sgl + 1 : sgl];
+            const float vm = vb ? va[lm] * vb[lm] : va[lm];
+            const float vp = vb ? va[lp] * vb[lp] : va[lp];
+            if (has_m && ((fm >> 7) & 1u)) hzm = vm;
+            if (has_p && (fp & 1u)) hzp = vp;
         }
     }
     wave_sync();
@@ -1210,10 +1231,14 @@ void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const fl
 
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                      const float* x, float* y, const Reduce& red, const Active& act) {
-    const int nb = list_blocks(act.nseg);
-    hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(kBlock), 0, s, p, A, L, x, y, act.segs, act.nseg,
-                       red);
-    launch_finalize(s, red, FIN_SIGMA, nb);
+    // one segment per thread while the block partials fit, more beyond that
+    const int per = (int)std::max<int64_t>(1, (act.nseg + (int64_t)kBlock * kMaxPartials - 1) /
+                                                  ((int64_t)kBlock * kMaxPartials));
+    const int64_t per_block = (int64_t)kBlock * per;
+    const int64_t nb = std::max<int64_t>(1, (act.nseg + per_block - 1) / per_block);
+    hipLaunchKernelGGL(k_spmv_dot, dim3((unsigned)xcd_grid(nb)), dim3(kBlock), 0, s, p, A, L, x, y,
+                       act.segs, act.nseg, per, red);
+    launch_finalize(s, red, FIN_SIGMA, (int)nb);
 }
 
 void launch_axpy2(hipStream_t s, float* P, const float* S, float* R, const float* Z,
