@@ -164,16 +164,23 @@ hipEvent_t take_event(ariel_sim* h) {
 }
 
 // Algorithmic HBM bytes of one launch group: per-unit figures x units.
+// Algorithmic HBM bytes of one launch group: each input read once and each
+// output written once. The solver groups count the cells they touch: the
+// active segments (8 cells each) and, for the preconditioner, the active tiles
+// (512 cells each; r and pc read, z written, and q written and read again on
+// the red half).
 double kernel_bytes(const ariel_sim* h, int k) {
     const double np = (double)h->np, nc = (double)h->nc;
+    const double seg_cells = 8.0 * (double)h->act.nseg;
+    const double tile_cells = 512.0 * (double)h->act.ntiles[2];
     switch (k) {
         case ARIEL_KERNEL_SORT: return 60.0 * np + 8.0 * nc;
         case ARIEL_KERNEL_DENSITY: return 40.0 * np;
         case ARIEL_KERNEL_MARK: return 4.0 * np + 9.0 * nc;
         case ARIEL_KERNEL_P2G: return 24.0 * np + 12.0 * nc;
-        case ARIEL_KERNEL_SPMV: return 13.0 * nc;
-        case ARIEL_KERNEL_PRECOND: return 26.0 * nc;
-        case ARIEL_KERNEL_AXPY: return 24.0 * nc;
+        case ARIEL_KERNEL_SPMV: return 9.0 * seg_cells;
+        case ARIEL_KERNEL_PRECOND: return 20.0 * tile_cells;
+        case ARIEL_KERNEL_AXPY: return 24.0 * seg_cells;
         case ARIEL_KERNEL_GRADIENT: return 33.0 * nc;
         case ARIEL_KERNEL_EXTRAPOLATE: return 36.0 * nc;
         case ARIEL_KERNEL_ADVECT: return 48.0 * np + 24.0 * nc;

@@ -47,7 +47,28 @@ def parse():
     ap.add_argument("--precond", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (a plain copy "
+                         "kernel of known traffic, to scale FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """Five hipMemcpy device-to-device copies of `mib` MiB: a plain copy kernel
+    that reads and writes exactly mib MiB per launch."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(5):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # DeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes_per_launch": n}), flush=True)
 
 
 def cpu_baseline(sim, desc, gpu_iters, cpu_iters):
@@ -91,6 +112,9 @@ def cpu_baseline(sim, desc, gpu_iters, cpu_iters):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         sys.stderr.write("bench.py: this build runs on one GPU only; --gpus must be 1 "
