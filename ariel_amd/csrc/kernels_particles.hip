@@ -482,22 +482,26 @@ __global__ void k_p2g(Params p, const float4* __restrict__ pos, const float4* __
 // in exactly the reference's order (sx -> sy -> sz -> list), so the sums are the
 // same as the per-face gather's. Weights beyond the kernel radius are exactly
 // zero and skipped (adding +0 to a sum that starts at +0 never changes it).
-constexpr int kP2GCap = 512;
-__device__ __forceinline__ int p2g_slot(int t) { return t + (t >> 3); }
+constexpr int kP2GCap = 256;   // particles staged per pass (small: occupancy hides the staging)
 
-__device__ __forceinline__ float p2g_weight(bool tri, float px, float py, float pz, float fx,
-                                            float fy, float fz, float RE, float hh) {
-    const float dx = px - fx, dy = py - fy, dz = pz - fz;
-    if (tri) {
+// Weight of one particle at offset (dx, dy, dz) from a face, branch-free: the
+// Sharpen or hat weight, 0 outside its support. Terms of weight 0 are added to
+// the sums like the others (as the reference does): they add +-0 to sums that
+// start at +0 and so never change a bit.
+template <bool TRI>
+__device__ __forceinline__ float p2g_weight(float dx, float dy, float dz, float hh) {
+    if constexpr (TRI) {
         const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
-        if (ax >= 1.0f || ay >= 1.0f || az >= 1.0f) return 0.0f;
-        return (gmax(1.0f - ax, 0.0f) * gmax(1.0f - ay, 0.0f)) * gmax(1.0f - az, 0.0f);
+        const float w = (gmax(1.0f - ax, 0.0f) * gmax(1.0f - ay, 0.0f)) * gmax(1.0f - az, 0.0f);
+        return (ax >= 1.0f || ay >= 1.0f || az >= 1.0f) ? 0.0f : w;
+    } else {
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float w = 1.0f * sharpen_nr(r2, hh);
+        return r2 >= hh ? 0.0f : w;
     }
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 >= hh) return 0.0f;
-    return 1.0f * sharpen_nr(r2, hh);
 }
 
+template <bool TRI>
 __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restrict__ pos,
                                                   const float4* __restrict__ vel,
                                                   const int32_t* __restrict__ offsets,
@@ -505,9 +509,7 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
                                                   float* __restrict__ ux, float* __restrict__ uy,
                                                   float* __restrict__ uz, float* __restrict__ dux,
                                                   float* __restrict__ duy, float* __restrict__ duz) {
-    // staged particle t lives at p2g_slot(t): one pad word per 8 particles keeps
-    // lanes whose ranges start a cell (about 8 particles) apart on distinct banks
-    __shared__ float sp[6][kP2GCap + kP2GCap / 8];
+    __shared__ float sp[6][kP2GCap];
     const int N = p.N;
     const int nkc = (N + 1 + 63) / 64;
     // Blocks run over 8x8 tiles of face columns (i, j), k-chunks innermost, so
@@ -537,8 +539,9 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
     const float maxd = (float)N;
     const float RE = p.sharpen_re;
     const float hh = RE * RE;
-    const bool tri = p.splat == 1;
+    const bool tri = TRI;
     const float fi = (float)i, fj = (float)j, fk = (float)k;
+    const float fih = fi + 0.5f, fjh = fj + 0.5f, fkh = fk + 0.5f;
     float swx = 0.0f, sux = 0.0f, swy = 0.0f, suy = 0.0f, swz = 0.0f, suz = 0.0f;
     // Along a face's tangential axis the window's first cell ([f-2.5, f-1.5]
     // from the face coordinate f) is at least 1.5 cells away: every weight
@@ -567,7 +570,7 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
                 for (int32_t q = c0 + lane; q < c1; q += 64) {
                     const float4 o = pos[q];
                     const float4 v = vel[q];
-                    const int t = p2g_slot(q - c0);
+                    const int t = q - c0;
                     sp[0][t] = gmax(0.0f, gmin(maxd, maxd * o.x));
                     sp[1][t] = gmax(0.0f, gmin(maxd, maxd * o.y));
                     sp[2][t] = gmax(0.0f, gmin(maxd, maxd * o.z));
@@ -578,28 +581,26 @@ __global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restr
                 __syncthreads();
                 const int32_t q0 = max(ra0, c0), q1 = min(ra1, c1);
                 for (int32_t q = q0; q < q1; ++q) {
-                    const int t = p2g_slot(q - c0);
+                    const int t = q - c0;
                     const float px = sp[0][t], py = sp[1][t], pz = sp[2][t];
+                    // offsets to the three faces (x face at (i, j+.5, k+.5), ...)
+                    const float ax = px - fi, bx = px - fih;
+                    const float ay = py - fj, by = py - fjh;
+                    const float az = pz - fk, bz = pz - fkh;
                     if (dox) {
-                        const float w = p2g_weight(tri, px, py, pz, fi, fj + 0.5f, fk + 0.5f, RE, hh);
-                        if (w != 0.0f) {
-                            sux += w * sp[3][t];
-                            swx += w;
-                        }
+                        const float w = p2g_weight<TRI>(ax, by, bz, hh);
+                        sux += w * sp[3][t];
+                        swx += w;
                     }
                     if (doy) {
-                        const float w = p2g_weight(tri, px, py, pz, fi + 0.5f, fj, fk + 0.5f, RE, hh);
-                        if (w != 0.0f) {
-                            suy += w * sp[4][t];
-                            swy += w;
-                        }
+                        const float w = p2g_weight<TRI>(bx, ay, bz, hh);
+                        suy += w * sp[4][t];
+                        swy += w;
                     }
-                    if (doz && q >= rb0 && q < rb1) {
-                        const float w = p2g_weight(tri, px, py, pz, fi + 0.5f, fj + 0.5f, fk, RE, hh);
-                        if (w != 0.0f) {
-                            suz += w * sp[5][t];
-                            swz += w;
-                        }
+                    if (doz) {
+                        const float w = (q >= rb0 && q < rb1) ? p2g_weight<TRI>(bx, by, az, hh) : 0.0f;
+                        suz += w * sp[5][t];
+                        swz += w;
                     }
                 }
             }
@@ -1005,8 +1006,12 @@ void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4*
     const int64_t nkc = (p.N + 1 + 63) / 64;
     const int64_t nti = (p.N + 1 + 7) / 8;
     const int64_t nb = nti * nti * 64 * nkc;
-    hipLaunchKernelGGL(k_p2g_cols, dim3((unsigned)xcd_grid(nb)), dim3(64), 0, s, p, pos, vel, offsets, A,
-                       u[0], u[1], u[2], du[0], du[1], du[2]);
+    if (p.splat == 1)
+        hipLaunchKernelGGL(k_p2g_cols<true>, dim3((unsigned)xcd_grid(nb)), dim3(64), 0, s, p, pos, vel,
+                           offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]);
+    else
+        hipLaunchKernelGGL(k_p2g_cols<false>, dim3((unsigned)xcd_grid(nb)), dim3(64), 0, s, p, pos, vel,
+                           offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]);
 }
 
 // Reference-shaped per-face gather (one thread per face), kept for cross-checks.
