@@ -112,6 +112,17 @@ struct ariel_sim {
     Reduce reduce() const { return Reduce{partials, counter, sc}; }
 };
 
+namespace ariel {
+Recorder*& recorder() {
+    static thread_local Recorder* r = nullptr;
+    return r;
+}
+bool& batch_sizing() {
+    static thread_local bool b = false;
+    return b;
+}
+}  // namespace ariel
+
 namespace {
 
 int fail(ariel_sim* h, int code, const std::string& msg) {

@@ -13,6 +13,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
+#include <vector>
+
 namespace ariel {
 
 constexpr int kAir = 0, kFluid = 1, kSolid = 2;
@@ -20,9 +23,9 @@ constexpr int kBlock = 256;      // default threads per block of streaming kerne
 constexpr int kMaxPartials = 65536;
 constexpr int kCounterWords = 33 * 32;   // arrival counters of the reductions
 
-// Reduction/solver scalars living in device memory. Written by the last block of
-// a reduction ("last block finalises"), read by later kernels; no host round
-// trip inside a PCG iteration.
+// Reduction/solver scalars living in device memory. Written by the one-block
+// finalize kernel of each reduction, read by later kernels; no host round trip
+// inside a PCG iteration.
 struct SolverScalars {
     double bnorm2;      // ||b||^2 over fluid cells
     double rnorm2;      // ||r||^2
@@ -86,6 +89,53 @@ struct Reduce {
 
 // Exhaustive division self-check (selftest.hip); lo, hi non-negative.
 hipError_t run_division_selftest(int which, float a, float lo, float hi, int64_t* mismatches);
+
+// ---------------------------------------------------------------- launching
+// Every kernel is a functor K { static constexpr int kThreads; struct Args;
+// static __device__ void run(const Args&); }. For one simulation, launch<K>
+// passes the Args by value (k_one<K>). A batch of independent tanks runs the
+// same launch sequence once for all of them: k_many<K> is launched with
+// gridDim.y = number of tanks and reads tank blockIdx.y's Args from a device
+// table. While a Recorder is installed on the calling thread, launch<K> records
+// the launch (kernel, geometry, Args bytes) instead of issuing it.
+struct LaunchRecord {
+    const void* many = nullptr;   // host stub of k_many<K>
+    dim3 grid, block;
+    std::vector<unsigned char> args;
+};
+struct Recorder {
+    std::vector<LaunchRecord> seq;
+};
+Recorder*& recorder();        // thread-local (ariel_abi.hip)
+// Batch sizing (thread-local): launchers give reductions and active-set loops
+// grids that do not depend on the tank, and the kernels take the active-set
+// counts from the tank's device scalars, so one recorded sequence serves every
+// tank and every step.
+bool& batch_sizing();
+
+template <class K>
+__global__ __launch_bounds__(K::kThreads) void k_one(typename K::Args a) {
+    K::run(a);
+}
+template <class K>
+__global__ __launch_bounds__(K::kThreads) void k_many(const typename K::Args* __restrict__ t) {
+    K::run(t[blockIdx.y]);
+}
+
+template <class K>
+inline void launch(hipStream_t s, dim3 grid, dim3 block, const typename K::Args& a) {
+    if (Recorder* r = recorder()) {
+        LaunchRecord rec;
+        rec.many = reinterpret_cast<const void*>(&k_many<K>);
+        rec.grid = grid;
+        rec.block = block;
+        rec.args.resize(sizeof(a));
+        std::memcpy(rec.args.data(), &a, sizeof(a));
+        r->seq.push_back(std::move(rec));
+        return;
+    }
+    hipLaunchKernelGGL(k_one<K>, grid, block, 0, s, a);
+}
 
 // ---------------------------------------------------------------- launchers
 // particles (kernels_particles.hip)

@@ -30,55 +30,88 @@ __device__ __forceinline__ void decode_cell32(int N, int64_t c64, int& i, int& j
 
 // Project, divergence per cell (flip.cpp:417-437) and FlipGrid (solver.inl:42-56):
 // D = -((ux(i+1)-ux(i) + uy(j+1)-uy(j) + uz(k+1)-uz(k)) / h), h = 1/maxd.
-__global__ void k_divergence(Params p, const float* __restrict__ ux, const float* __restrict__ uy,
-                             const float* __restrict__ uz, float* __restrict__ D) {
-    const int N = p.N;
-    const int64_t nc = (int64_t)N * N * N;
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    int i, j, k;
-    decode_cell32(N, c, i, j, k);
-    const float h = 1.0f / p.maxd;
-    float dv = (ux[fidx(N, 0, i + 1, j, k)] - ux[fidx(N, 0, i, j, k)] + uy[fidx(N, 1, i, j + 1, k)] -
-                uy[fidx(N, 1, i, j, k)] + uz[fidx(N, 2, i, j, k + 1)] - uz[fidx(N, 2, i, j, k)]) / h;
-    D[c] = -dv;
-}
+struct KDivergence {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const float* ux;
+        const float* uy;
+        const float* uz;
+        float* D;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const float* __restrict__ ux = kargs.ux;
+        const float* __restrict__ uy = kargs.uy;
+        const float* __restrict__ uz = kargs.uz;
+        float* __restrict__ D = kargs.D;
+
+        const int N = p.N;
+        const int64_t nc = (int64_t)N * N * N;
+        int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= nc) return;
+        int i, j, k;
+        decode_cell32(N, c, i, j, k);
+        const float h = 1.0f / p.maxd;
+        float dv = (ux[fidx(N, 0, i + 1, j, k)] - ux[fidx(N, 0, i, j, k)] + uy[fidx(N, 1, i, j + 1, k)] -
+                    uy[fidx(N, 1, i, j, k)] + uz[fidx(N, 2, i, j, k + 1)] - uz[fidx(N, 2, i, j, k)]) / h;
+        D[c] = -dv;
+    }
+};
 
 // SubtractPressureGradient (flip.cpp:568-680, subcell = 1) + EnforceBoundaryVelocity
 // (particlegridoperations.inl:45-104) per face.
-__global__ void k_gradient(Params p, const float* __restrict__ P, const float* __restrict__ L,
-                           const uint8_t* __restrict__ A, float* __restrict__ ux,
-                           float* __restrict__ uy, float* __restrict__ uz) {
-    const int N = p.N;
-    const int64_t nf = (int64_t)(N + 1) * N * N;
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= 3 * nf) return;
-    const int comp = (int)(g / nf);
-    const int64_t f = g - comp * nf;
-    int i, j, k;
-    decode_face(N, comp, f, i, j, k);
-    float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
-    const int a = comp == 0 ? i : (comp == 1 ? j : k);
-    if (a == 0 || a == N) {
-        u[f] = 0.0f;
-        return;
+struct KGradient {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const float* P;
+        const float* L;
+        const uint8_t* A;
+        float* ux;
+        float* uy;
+        float* uz;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const float* __restrict__ P = kargs.P;
+        const float* __restrict__ L = kargs.L;
+        const uint8_t* __restrict__ A = kargs.A;
+        float* __restrict__ ux = kargs.ux;
+        float* __restrict__ uy = kargs.uy;
+        float* __restrict__ uz = kargs.uz;
+
+        const int N = p.N;
+        const int64_t nf = (int64_t)(N + 1) * N * N;
+        int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (g >= 3 * nf) return;
+        const int comp = (int)(g / nf);
+        const int64_t f = g - comp * nf;
+        int i, j, k;
+        decode_face(N, comp, f, i, j, k);
+        float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
+        const int a = comp == 0 ? i : (comp == 1 ? j : k);
+        if (a == 0 || a == N) {
+            u[f] = 0.0f;
+            return;
+        }
+        const float h = 1.0f / p.maxd;
+        const int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
+        const int64_t cf = cidx(N, i, j, k), cb = cidx(N, bi, bj, bk);
+        float pf = P[cf], pb = P[cb];
+        const float lf = L[cf], lb = L[cb];
+        if (lf * lb < 0.0f) {
+            if (lf < 0.0f) pf = P[cf];
+            else pf = lf / gmin(1.0e-3f, lb) * P[cb];
+            if (lb < 0.0f) pb = P[cb];
+            else pb = lb / gmin(1.0e-6f, lf) * P[cf];
+        }
+        float val = u[f];
+        val -= (pf - pb) / h;
+        const bool s0 = A[cf] == kSolid, s1 = A[cb] == kSolid;
+        u[f] = (s0 != s1) ? 0.0f : val;
     }
-    const float h = 1.0f / p.maxd;
-    const int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
-    const int64_t cf = cidx(N, i, j, k), cb = cidx(N, bi, bj, bk);
-    float pf = P[cf], pb = P[cb];
-    const float lf = L[cf], lb = L[cb];
-    if (lf * lb < 0.0f) {
-        if (lf < 0.0f) pf = P[cf];
-        else pf = lf / gmin(1.0e-3f, lb) * P[cb];
-        if (lb < 0.0f) pb = P[cb];
-        else pb = lb / gmin(1.0e-6f, lf) * P[cf];
-    }
-    float val = u[f];
-    val -= (pf - pb) / h;
-    const bool s0 = A[cf] == kSolid, s1 = A[cb] == kSolid;
-    u[f] = (s0 != s1) ? 0.0f : val;
-}
+};
 
 // ExtrapolateVelocity (flip.cpp:452-566) + SubtractPreviousGrid (365-410). A face
 // that touches no FLUID cell but has SOLID (or out-of-domain) cells on both sides
@@ -96,122 +129,164 @@ __device__ __forceinline__ bool face_wallmark(const uint8_t* A, int N, int comp,
     return (a <= 0 || A[cidx(N, bi, bj, bk)] == kSolid) && (a >= N || A[cidx(N, i, j, k)] == kSolid);
 }
 
-__global__ void k_extrapolate(Params p, const uint8_t* __restrict__ A, float* ux, float* uy,
-                              float* uz, float* dux, float* duy, float* duz) {
-    const int N = p.N;
-    const int64_t nf = (int64_t)(N + 1) * N * N;
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= 3 * nf) return;
-    const int comp = (int)(g / nf);
-    const int64_t f = g - comp * nf;
-    int i, j, k;
-    decode_face(N, comp, f, i, j, k);
-    float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
-    float* du = comp == 0 ? dux : (comp == 1 ? duy : duz);
-    float val = u[f];
-    if (!face_mark(A, N, comp, i, j, k) && face_wallmark(A, N, comp, i, j, k)) {
-        unsigned wsum = 0;
-        float sum = 0.0f;
-        const int q[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
-                             {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
-        const int ex = N + (comp == 0), ey = N + (comp == 1), ez = N + (comp == 2);
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            const int a = q[m][0], b = q[m][1], c = q[m][2];
-            if (a >= 0 && a < ex && b >= 0 && b < ey && c >= 0 && c < ez) {
-                if (face_mark(A, N, comp, a, b, c)) {
-                    wsum++;
-                    sum += u[fidx(N, comp, a, b, c)];
+struct KExtrapolate {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const uint8_t* A;
+        float* ux;
+        float* uy;
+        float* uz;
+        float* dux;
+        float* duy;
+        float* duz;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const uint8_t* __restrict__ A = kargs.A;
+        float* ux = kargs.ux;
+        float* uy = kargs.uy;
+        float* uz = kargs.uz;
+        float* dux = kargs.dux;
+        float* duy = kargs.duy;
+        float* duz = kargs.duz;
+
+        const int N = p.N;
+        const int64_t nf = (int64_t)(N + 1) * N * N;
+        int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (g >= 3 * nf) return;
+        const int comp = (int)(g / nf);
+        const int64_t f = g - comp * nf;
+        int i, j, k;
+        decode_face(N, comp, f, i, j, k);
+        float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
+        float* du = comp == 0 ? dux : (comp == 1 ? duy : duz);
+        float val = u[f];
+        if (!face_mark(A, N, comp, i, j, k) && face_wallmark(A, N, comp, i, j, k)) {
+            unsigned wsum = 0;
+            float sum = 0.0f;
+            const int q[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                                 {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+            const int ex = N + (comp == 0), ey = N + (comp == 1), ez = N + (comp == 2);
+    #pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                const int a = q[m][0], b = q[m][1], c = q[m][2];
+                if (a >= 0 && a < ex && b >= 0 && b < ey && c >= 0 && c < ez) {
+                    if (face_mark(A, N, comp, a, b, c)) {
+                        wsum++;
+                        sum += u[fidx(N, comp, a, b, c)];
+                    }
                 }
             }
+            if (wsum) {
+                val = sum / (float)wsum;
+                u[f] = val;
+            }
         }
-        if (wsum) {
-            val = sum / (float)wsum;
-            u[f] = val;
-        }
+        du[f] = val - du[f];
     }
-    du[f] = val - du[f];
-}
+};
 
 // Post-projection diagnostics: max |div u| over FLUID cells and the FLUID count.
 // Grid-stride over a capped grid; one pair of atomics per block.
-__global__ void k_grid_stats(Params p, const float* __restrict__ ux, const float* __restrict__ uy,
-                             const float* __restrict__ uz, const uint8_t* __restrict__ A,
-                             SolverScalars* sc) {
-    const int N = p.N;
-    const int64_t nc = (int64_t)N * N * N;
-    const float h = 1.0f / p.maxd;
-    float mx = 0.0f;
-    int cnt = 0;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        if (A[c] != kFluid) continue;
-        int i, j, k;
-        decode_cell32(N, c, i, j, k);
-        float dv = (ux[fidx(N, 0, i + 1, j, k)] - ux[fidx(N, 0, i, j, k)] +
-                    uy[fidx(N, 1, i, j + 1, k)] - uy[fidx(N, 1, i, j, k)] +
-                    uz[fidx(N, 2, i, j, k + 1)] - uz[fidx(N, 2, i, j, k)]) / h;
-        mx = fmaxf(mx, fabsf(dv));
-        cnt += 1;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        mx = fmaxf(mx, __shfl_down(mx, off, 64));
-        cnt += __shfl_down(cnt, off, 64);
-    }
-    __shared__ float smx[16];
-    __shared__ int scnt[16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (lane == 0) {
-        smx[w] = mx;
-        scnt[w] = cnt;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 1; i < nw; ++i) {
-            mx = fmaxf(mx, smx[i]);
-            cnt += scnt[i];
-        }
-        if (cnt) atomicAdd(&sc->fluid_cells, cnt);
-        if (mx > 0.0f) atomicMax(&sc->maxdiv_bits, __float_as_uint(mx));
-    }
-}
+struct KGridStats {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const float* ux;
+        const float* uy;
+        const float* uz;
+        const uint8_t* A;
+        SolverScalars* sc;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const float* __restrict__ ux = kargs.ux;
+        const float* __restrict__ uy = kargs.uy;
+        const float* __restrict__ uz = kargs.uz;
+        const uint8_t* __restrict__ A = kargs.A;
+        SolverScalars* sc = kargs.sc;
 
-__global__ void k_stats_reset(SolverScalars* sc) {
-    if (threadIdx.x == 0) {
-        sc->fluid_cells = 0;
-        sc->maxdiv_bits = 0u;
+        const int N = p.N;
+        const int64_t nc = (int64_t)N * N * N;
+        const float h = 1.0f / p.maxd;
+        float mx = 0.0f;
+        int cnt = 0;
+        for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+             c += (int64_t)gridDim.x * blockDim.x) {
+            if (A[c] != kFluid) continue;
+            int i, j, k;
+            decode_cell32(N, c, i, j, k);
+            float dv = (ux[fidx(N, 0, i + 1, j, k)] - ux[fidx(N, 0, i, j, k)] +
+                        uy[fidx(N, 1, i, j + 1, k)] - uy[fidx(N, 1, i, j, k)] +
+                        uz[fidx(N, 2, i, j, k + 1)] - uz[fidx(N, 2, i, j, k)]) / h;
+            mx = fmaxf(mx, fabsf(dv));
+            cnt += 1;
+        }
+    #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mx = fmaxf(mx, __shfl_down(mx, off, 64));
+            cnt += __shfl_down(cnt, off, 64);
+        }
+        __shared__ float smx[16];
+        __shared__ int scnt[16];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        if (lane == 0) {
+            smx[w] = mx;
+            scnt[w] = cnt;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < nw; ++i) {
+                mx = fmaxf(mx, smx[i]);
+                cnt += scnt[i];
+            }
+            if (cnt) atomicAdd(&sc->fluid_cells, cnt);
+            if (mx > 0.0f) atomicMax(&sc->maxdiv_bits, __float_as_uint(mx));
+        }
     }
-}
+};
+
+struct KStatsReset {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        SolverScalars* sc;
+    };
+    static __device__ void run(const Args& kargs) {
+        SolverScalars* sc = kargs.sc;
+
+        if (threadIdx.x == 0) {
+            sc->fluid_cells = 0;
+            sc->maxdiv_bits = 0u;
+        }
+    }
+};
 
 }  // namespace
 
 void launch_divergence(hipStream_t s, const Params& p, float* const u[3], float* D) {
     int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_divergence, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, u[0], u[1],
-                       u[2], D);
+    launch<KDivergence>(s, dim3(grid_for(nc, kBlock)), dim3(kBlock), {p, u[0], u[1], u[2], D});
 }
 
 void launch_gradient(hipStream_t s, const Params& p, const float* P, const float* L,
                      const uint8_t* A, float* const u[3]) {
     int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
-    hipLaunchKernelGGL(k_gradient, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), 0, s, p, P, L, A,
-                       u[0], u[1], u[2]);
+    launch<KGradient>(s, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), {p, P, L, A, u[0], u[1], u[2]});
 }
 
 void launch_extrapolate(hipStream_t s, const Params& p, const uint8_t* A, float* const u[3],
                         float* const du[3]) {
     int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
-    hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), 0, s, p, A,
-                       u[0], u[1], u[2], du[0], du[1], du[2]);
+    launch<KExtrapolate>(s, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), {p, A, u[0], u[1], u[2], du[0], du[1], du[2]});
 }
 
 void launch_grid_stats(hipStream_t s, const Params& p, float* const u[3], const uint8_t* A,
                        SolverScalars* sc) {
     int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_stats_reset, dim3(1), dim3(64), 0, s, sc);
+    launch<KStatsReset>(s, dim3(1), dim3(64), {sc});
     const int64_t nb = grid_for(nc, kBlock) < 1024 ? grid_for(nc, kBlock) : 1024;
-    hipLaunchKernelGGL(k_grid_stats, dim3((unsigned)nb), dim3(kBlock), 0, s, p, u[0], u[1], u[2], A, sc);
+    launch<KGridStats>(s, dim3((unsigned)nb), dim3(kBlock), {p, u[0], u[1], u[2], A, sc});
 }
 
 }  // namespace ariel

@@ -36,21 +36,38 @@ __device__ __forceinline__ void wave_runs(int32_t c, bool valid, int& leader, in
     len = next - lane;
 }
 
-__global__ void k_bin(const float4* __restrict__ pos, int64_t np, int N, float maxd,
-                      int32_t* __restrict__ cell_of, int32_t* __restrict__ counts) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = s < np;
-    int32_t c = -1;
-    if (valid) {
-        float4 p = pos[s];
-        int cx = cell_coord(p.x, maxd), cy = cell_coord(p.y, maxd), cz = cell_coord(p.z, maxd);
-        c = (int32_t)cidx(N, cx, cy, cz);
-        cell_of[s] = c;
+struct KBin {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const float4* pos;
+        int64_t np;
+        int N;
+        float maxd;
+        int32_t* cell_of;
+        int32_t* counts;
+    };
+    static __device__ void run(const Args& kargs) {
+        const float4* __restrict__ pos = kargs.pos;
+        int64_t np = kargs.np;
+        int N = kargs.N;
+        float maxd = kargs.maxd;
+        int32_t* __restrict__ cell_of = kargs.cell_of;
+        int32_t* __restrict__ counts = kargs.counts;
+
+        int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const bool valid = s < np;
+        int32_t c = -1;
+        if (valid) {
+            float4 p = pos[s];
+            int cx = cell_coord(p.x, maxd), cy = cell_coord(p.y, maxd), cz = cell_coord(p.z, maxd);
+            c = (int32_t)cidx(N, cx, cy, cz);
+            cell_of[s] = c;
+        }
+        int leader, len;
+        wave_runs(c, valid, leader, len);
+        if (valid && leader == (int)(threadIdx.x & 63)) atomicAdd(&counts[c], len);
     }
-    int leader, len;
-    wave_runs(c, valid, leader, len);
-    if (valid && leader == (int)(threadIdx.x & 63)) atomicAdd(&counts[c], len);
-}
+};
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
@@ -78,113 +95,196 @@ __device__ int32_t block_exclusive_scan(int32_t v, int32_t* total) {
     return before + x - v;
 }
 
-__global__ void k_scan_reduce(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ bsum) {
-    int64_t base = (int64_t)blockIdx.x * kScanTile;
-    int32_t acc = 0;
-    for (int t = 0; t < kScanItems; ++t) {
-        int64_t i = base + (int64_t)t * kScanThreads + threadIdx.x;
-        if (i < n) acc += in[i];
+struct KScanReduce {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const int32_t* in;
+        int64_t n;
+        int32_t* bsum;
+    };
+    static __device__ void run(const Args& kargs) {
+        const int32_t* __restrict__ in = kargs.in;
+        int64_t n = kargs.n;
+        int32_t* __restrict__ bsum = kargs.bsum;
+
+        int64_t base = (int64_t)blockIdx.x * kScanTile;
+        int32_t acc = 0;
+        for (int t = 0; t < kScanItems; ++t) {
+            int64_t i = base + (int64_t)t * kScanThreads + threadIdx.x;
+            if (i < n) acc += in[i];
+        }
+        int32_t tot;
+        block_exclusive_scan(acc, &tot);
+        if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
     }
-    int32_t tot;
-    block_exclusive_scan(acc, &tot);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
-}
+};
 
 // Single block: exclusive scan of nb block sums in place, bsum[nb] = total.
-__global__ void k_scan_bsums(int32_t* __restrict__ bsum, int nb) {
-    const int per = (nb + kScanThreads - 1) / kScanThreads;
-    const int lo = threadIdx.x * per;
-    int32_t acc = 0;
-    for (int i = 0; i < per; ++i)
-        if (lo + i < nb) acc += bsum[lo + i];
-    int32_t tot;
-    int32_t ex = block_exclusive_scan(acc, &tot);
-    for (int i = 0; i < per; ++i) {
-        if (lo + i < nb) {
-            int32_t v = bsum[lo + i];
-            bsum[lo + i] = ex;
-            ex += v;
-        }
-    }
-    if (threadIdx.x == 0) bsum[nb] = tot;
-}
+struct KScanBsums {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int32_t* bsum;
+        int nb;
+    };
+    static __device__ void run(const Args& kargs) {
+        int32_t* __restrict__ bsum = kargs.bsum;
+        int nb = kargs.nb;
 
-__global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const int32_t* __restrict__ bsum,
-                             int32_t* __restrict__ out, int nb) {
-    int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    int32_t v[kScanItems];
-    int32_t acc = 0;
-#pragma unroll
-    for (int t = 0; t < kScanItems; ++t) {
-        int64_t i = base + t;
-        v[t] = (i < n) ? in[i] : 0;
-        acc += v[t];
+        const int per = (nb + kScanThreads - 1) / kScanThreads;
+        const int lo = threadIdx.x * per;
+        int32_t acc = 0;
+        for (int i = 0; i < per; ++i)
+            if (lo + i < nb) acc += bsum[lo + i];
+        int32_t tot;
+        int32_t ex = block_exclusive_scan(acc, &tot);
+        for (int i = 0; i < per; ++i) {
+            if (lo + i < nb) {
+                int32_t v = bsum[lo + i];
+                bsum[lo + i] = ex;
+                ex += v;
+            }
+        }
+        if (threadIdx.x == 0) bsum[nb] = tot;
     }
-    int32_t tot;
-    int32_t ex = block_exclusive_scan(acc, &tot) + bsum[blockIdx.x];
-#pragma unroll
-    for (int t = 0; t < kScanItems; ++t) {
-        int64_t i = base + t;
-        if (i < n) out[i] = ex;
-        ex += v[t];
+};
+
+struct KScanApply {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const int32_t* in;
+        int64_t n;
+        const int32_t* bsum;
+        int32_t* out;
+        int nb;
+    };
+    static __device__ void run(const Args& kargs) {
+        const int32_t* __restrict__ in = kargs.in;
+        int64_t n = kargs.n;
+        const int32_t* __restrict__ bsum = kargs.bsum;
+        int32_t* __restrict__ out = kargs.out;
+        int nb = kargs.nb;
+
+        int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+        int32_t v[kScanItems];
+        int32_t acc = 0;
+    #pragma unroll
+        for (int t = 0; t < kScanItems; ++t) {
+            int64_t i = base + t;
+            v[t] = (i < n) ? in[i] : 0;
+            acc += v[t];
+        }
+        int32_t tot;
+        int32_t ex = block_exclusive_scan(acc, &tot) + bsum[blockIdx.x];
+    #pragma unroll
+        for (int t = 0; t < kScanItems; ++t) {
+            int64_t i = base + t;
+            if (i < n) out[i] = ex;
+            ex += v[t];
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
-}
+};
 
 // A run leader reserves len slots at the top of its cell with one atomic
 // (counts return to 0 for the next sort); members take consecutive slots in lane
 // order. The particle id is written next to its slot for the per-cell ordering.
-__global__ void k_scatter(const int32_t* __restrict__ cell_of, int64_t np,
-                          const int32_t* __restrict__ offsets, int32_t* __restrict__ counts,
-                          int32_t* __restrict__ slots, const float4* __restrict__ pos,
-                          int32_t* __restrict__ sid) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = s < np;
-    const int32_t c = valid ? cell_of[s] : -1;
-    int leader, len;
-    wave_runs(c, valid, leader, len);
-    const int lane = threadIdx.x & 63;
-    int32_t base = 0;
-    if (valid && leader == lane) base = atomicSub(&counts[c], len) - len;
-    base = __shfl(base, leader, 64);
-    if (!valid) return;
-    const int32_t dst = offsets[c] + base + (lane - leader);
-    slots[dst] = (int32_t)s;
-    sid[dst] = __float_as_int(pos[s].w);
-}
+struct KScatter {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const int32_t* cell_of;
+        int64_t np;
+        const int32_t* offsets;
+        int32_t* counts;
+        int32_t* slots;
+        const float4* pos;
+        int32_t* sid;
+    };
+    static __device__ void run(const Args& kargs) {
+        const int32_t* __restrict__ cell_of = kargs.cell_of;
+        int64_t np = kargs.np;
+        const int32_t* __restrict__ offsets = kargs.offsets;
+        int32_t* __restrict__ counts = kargs.counts;
+        int32_t* __restrict__ slots = kargs.slots;
+        const float4* __restrict__ pos = kargs.pos;
+        int32_t* __restrict__ sid = kargs.sid;
+
+        int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const bool valid = s < np;
+        const int32_t c = valid ? cell_of[s] : -1;
+        int leader, len;
+        wave_runs(c, valid, leader, len);
+        const int lane = threadIdx.x & 63;
+        int32_t base = 0;
+        if (valid && leader == lane) base = atomicSub(&counts[c], len) - len;
+        base = __shfl(base, leader, 64);
+        if (!valid) return;
+        const int32_t dst = offsets[c] + base + (lane - leader);
+        slots[dst] = (int32_t)s;
+        sid[dst] = __float_as_int(pos[s].w);
+    }
+};
 
 // Per-cell insertion sort of (id, slot) pairs by id: restores the reference's
 // per-cell order (push_back in particle-vector order). Particles that stayed in
 // their cell arrive in order already, so most segments need no move.
-__global__ void k_cellsort(const int32_t* __restrict__ offsets, int64_t nc,
-                           int32_t* __restrict__ sid, int32_t* __restrict__ slots) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const int32_t o0 = offsets[c], o1 = offsets[c + 1];
-    for (int32_t a = o0 + 1; a < o1; ++a) {
-        const int32_t key = sid[a];
-        if (sid[a - 1] <= key) continue;
-        const int32_t sv = slots[a];
-        int32_t b = a - 1;
-        while (b >= o0 && sid[b] > key) {
-            sid[b + 1] = sid[b];
-            slots[b + 1] = slots[b];
-            --b;
-        }
-        sid[b + 1] = key;
-        slots[b + 1] = sv;
-    }
-}
+struct KCellsort {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const int32_t* offsets;
+        int64_t nc;
+        int32_t* sid;
+        int32_t* slots;
+    };
+    static __device__ void run(const Args& kargs) {
+        const int32_t* __restrict__ offsets = kargs.offsets;
+        int64_t nc = kargs.nc;
+        int32_t* __restrict__ sid = kargs.sid;
+        int32_t* __restrict__ slots = kargs.slots;
 
-__global__ void k_gather(const int32_t* __restrict__ slots, int64_t np, const float4* __restrict__ pin,
-                         const float4* __restrict__ vin, float4* __restrict__ pout,
-                         float4* __restrict__ vout) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= np) return;
-    int32_t src = slots[s];
-    pout[s] = pin[src];
-    vout[s] = vin[src];
-}
+        int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= nc) return;
+        const int32_t o0 = offsets[c], o1 = offsets[c + 1];
+        for (int32_t a = o0 + 1; a < o1; ++a) {
+            const int32_t key = sid[a];
+            if (sid[a - 1] <= key) continue;
+            const int32_t sv = slots[a];
+            int32_t b = a - 1;
+            while (b >= o0 && sid[b] > key) {
+                sid[b + 1] = sid[b];
+                slots[b + 1] = slots[b];
+                --b;
+            }
+            sid[b + 1] = key;
+            slots[b + 1] = sv;
+        }
+    }
+};
+
+struct KGather {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const int32_t* slots;
+        int64_t np;
+        const float4* pin;
+        const float4* vin;
+        float4* pout;
+        float4* vout;
+    };
+    static __device__ void run(const Args& kargs) {
+        const int32_t* __restrict__ slots = kargs.slots;
+        int64_t np = kargs.np;
+        const float4* __restrict__ pin = kargs.pin;
+        const float4* __restrict__ vin = kargs.vin;
+        float4* __restrict__ pout = kargs.pout;
+        float4* __restrict__ vout = kargs.vout;
+
+        int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (s >= np) return;
+        int32_t src = slots[s];
+        pout[s] = pin[src];
+        vout[s] = vin[src];
+    }
+};
 
 // ------------------------------------------------ neighbourhood staging
 // A block of 256 consecutive sorted particles sits in a few cells of one or two
@@ -313,91 +413,121 @@ __device__ __forceinline__ int staged_index(const StageLds& st, int cx, int cy, 
 // flip.cpp:697-733 + ApplyExternalForces (flip.cpp:682-695) fused: each particle
 // gathers Smooth weights (h = 4*density/maxd) over the 3x3x3 cells around its
 // cell; the 3 cells along z of each (sx,sy) column are one contiguous range.
-__global__ __launch_bounds__(kStageBlock) void k_density(Params p, const float4* __restrict__ pos,
-                                                    float4* __restrict__ vel, int64_t np,
-                                                    const int32_t* __restrict__ offsets,
-                                                    int apply_forces) {
-    __shared__ StageLds st;
-    const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
-    const bool valid = s < np;
-    const int N = p.N;
-    const float maxd = p.maxd;
-    const float4 me = valid ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int cx = cell_coord(me.x, maxd), cy = cell_coord(me.y, maxd), cz = cell_coord(me.z, maxd);
-    const float hsm = 4.0f * p.density / maxd;
-    const float hh = hsm * hsm, yhh = 1.0f / hh;
-    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, N - 1);
-    const int y0 = max(cy - 1, 0), y1 = min(cy + 1, N - 1);
-    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, N - 1);
-    stage_neighbourhood(st, valid, x0, x1, y0, y1, z0, z1, N, offsets, pos);
-    if (!valid) return;
-    float weightsum = 0.0f;
-    if (st.ok) {
-        for (int sx = x0; sx <= x1; ++sx) {
-            for (int sy = y0; sy <= y1; ++sy) {
-                const int64_t col = cidx(N, sx, sy, 0);
-                const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
-                const int l0 = staged_index(st, sx, sy, q0);
-                const int l1 = l0 + (q1 - q0);
-                for (int l = l0; l < l1; ++l) {
-                    float sqd = sqrlength(st.x[l], st.y[l], st.z[l], me.x, me.y, me.z);
-                    float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
-                    weightsum = weightsum + weight;
+struct KDensity {
+    static constexpr int kThreads = kStageBlock;
+    struct Args {
+        Params p;
+        const float4* pos;
+        float4* vel;
+        int64_t np;
+        const int32_t* offsets;
+        int apply_forces;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const float4* __restrict__ pos = kargs.pos;
+        float4* __restrict__ vel = kargs.vel;
+        int64_t np = kargs.np;
+        const int32_t* __restrict__ offsets = kargs.offsets;
+        int apply_forces = kargs.apply_forces;
+
+        __shared__ StageLds st;
+        const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
+        const bool valid = s < np;
+        const int N = p.N;
+        const float maxd = p.maxd;
+        const float4 me = valid ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int cx = cell_coord(me.x, maxd), cy = cell_coord(me.y, maxd), cz = cell_coord(me.z, maxd);
+        const float hsm = 4.0f * p.density / maxd;
+        const float hh = hsm * hsm, yhh = 1.0f / hh;
+        const int x0 = max(cx - 1, 0), x1 = min(cx + 1, N - 1);
+        const int y0 = max(cy - 1, 0), y1 = min(cy + 1, N - 1);
+        const int z0 = max(cz - 1, 0), z1 = min(cz + 1, N - 1);
+        stage_neighbourhood(st, valid, x0, x1, y0, y1, z0, z1, N, offsets, pos);
+        if (!valid) return;
+        float weightsum = 0.0f;
+        if (st.ok) {
+            for (int sx = x0; sx <= x1; ++sx) {
+                for (int sy = y0; sy <= y1; ++sy) {
+                    const int64_t col = cidx(N, sx, sy, 0);
+                    const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
+                    const int l0 = staged_index(st, sx, sy, q0);
+                    const int l1 = l0 + (q1 - q0);
+                    for (int l = l0; l < l1; ++l) {
+                        float sqd = sqrlength(st.x[l], st.y[l], st.z[l], me.x, me.y, me.z);
+                        float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
+                        weightsum = weightsum + weight;
+                    }
+                }
+            }
+        } else {
+            for (int sx = x0; sx <= x1; ++sx) {
+                for (int sy = y0; sy <= y1; ++sy) {
+                    const int64_t col = cidx(N, sx, sy, 0);
+                    const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
+                    for (int32_t q = q0; q < q1; ++q) {
+                        const float4 o = pos[q];
+                        float sqd = sqrlength(o.x, o.y, o.z, me.x, me.y, me.z);
+                        float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
+                        weightsum = weightsum + weight;
+                    }
                 }
             }
         }
-    } else {
-        for (int sx = x0; sx <= x1; ++sx) {
-            for (int sy = y0; sy <= y1; ++sy) {
-                const int64_t col = cidx(N, sx, sy, 0);
-                const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
-                for (int32_t q = q0; q < q1; ++q) {
-                    const float4 o = pos[q];
-                    float sqd = sqrlength(o.x, o.y, o.z, me.x, me.y, me.z);
-                    float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
-                    weightsum = weightsum + weight;
-                }
+        float4 v = vel[s];
+        v.w = weightsum / p.max_density;
+        if (apply_forces) {
+            for (int f = 0; f < p.n_forces; ++f) {
+                v.x = v.x + p.forces[f][0] * p.dt;
+                v.y = v.y + p.forces[f][1] * p.dt;
+                v.z = v.z + p.forces[f][2] * p.dt;
             }
         }
+        vel[s] = v;
     }
-    float4 v = vel[s];
-    v.w = weightsum / p.max_density;
-    if (apply_forces) {
-        for (int f = 0; f < p.n_forces; ++f) {
-            v.x = v.x + p.forces[f][0] * p.dt;
-            v.y = v.y + p.forces[f][1] * p.dt;
-            v.z = v.z + p.forces[f][2] * p.dt;
-        }
-    }
-    vel[s] = v;
-}
+};
 
 // ------------------------------------------------- MarkCellTypes + BuildSDF
 // CellSDF (particlegrid.cpp:75-90): L = 0.2/density^3 - sum of the cell's
 // densities in list order; A = SOLID on the static walls, else FLUID iff L < 0
 // (MarkCellTypes, particlegrid.cpp:108-138). BuildSDF (92-106) stores L.
-__global__ void k_mark(Params p, const float4* __restrict__ vel, const int32_t* __restrict__ offsets,
-                       uint8_t* __restrict__ A, float* __restrict__ L) {
-    const int N = p.N;
-    const int64_t nc = (int64_t)N * N * N;
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const uint32_t cu = (uint32_t)c, nu = (uint32_t)N;
-    const int k = (int)(cu % nu), j = (int)((cu / nu) % nu), i = (int)(cu / nu / nu);
-    float accm = 0.0f;
-    const int32_t o0 = offsets[c], o1 = offsets[c + 1];
-    for (int32_t a = o0; a < o1; ++a) accm += vel[a].w;
-    const float dd = p.density;
-    const float n0 = 1.0f / (dd * dd * dd);
-    const float lc = 0.2f * n0 - accm;
-    const int w = p.wall;
-    const bool wall = w > 0 && (i < w || j < w || k < w || i >= N - w || j >= N - w || k >= N - w);
-    uint8_t a = kAir;
-    if (wall) a = kSolid;
-    else a = (lc < 0.0f) ? kFluid : kAir;
-    A[c] = a;
-    L[c] = lc;
-}
+struct KMark {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const float4* vel;
+        const int32_t* offsets;
+        uint8_t* A;
+        float* L;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const float4* __restrict__ vel = kargs.vel;
+        const int32_t* __restrict__ offsets = kargs.offsets;
+        uint8_t* __restrict__ A = kargs.A;
+        float* __restrict__ L = kargs.L;
+
+        const int N = p.N;
+        const int64_t nc = (int64_t)N * N * N;
+        int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= nc) return;
+        const uint32_t cu = (uint32_t)c, nu = (uint32_t)N;
+        const int k = (int)(cu % nu), j = (int)((cu / nu) % nu), i = (int)(cu / nu / nu);
+        float accm = 0.0f;
+        const int32_t o0 = offsets[c], o1 = offsets[c + 1];
+        for (int32_t a = o0; a < o1; ++a) accm += vel[a].w;
+        const float dd = p.density;
+        const float n0 = 1.0f / (dd * dd * dd);
+        const float lc = 0.2f * n0 - accm;
+        const int w = p.wall;
+        const bool wall = w > 0 && (i < w || j < w || k < w || i >= N - w || j >= N - w || k >= N - w);
+        uint8_t a = kAir;
+        if (wall) a = kSolid;
+        else a = (lc < 0.0f) ? kFluid : kAir;
+        A[c] = a;
+        L[c] = lc;
+    }
+};
 
 // -------------------------------------------------- SplatParticlesToMACGrid
 // particlegridoperations.inl:146-249: each face gathers the Sharpen-weighted
@@ -406,71 +536,96 @@ __global__ void k_mark(Params p, const float4* __restrict__ vel, const int32_t* 
 // face normal, 4 along the others. Fused: StorePreviousGrid (flip.cpp:324-363,
 // copy before the boundary pass) and EnforceBoundaryVelocity
 // (particlegridoperations.inl:45-104).
-__global__ void k_p2g(Params p, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                      const int32_t* __restrict__ offsets, const uint8_t* __restrict__ A,
-                      float* __restrict__ ux, float* __restrict__ uy, float* __restrict__ uz,
-                      float* __restrict__ dux, float* __restrict__ duy, float* __restrict__ duz) {
-    const int N = p.N;
-    const int64_t nf = (int64_t)(N + 1) * N * N;
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= 3 * nf) return;
-    const int comp = (int)(g / nf);
-    const int64_t f = g - comp * nf;
-    int i, j, k;
-    if (comp == 0) { k = (int)(f % N); j = (int)((f / N) % N); i = (int)(f / ((int64_t)N * N)); }
-    else if (comp == 1) { k = (int)(f % N); j = (int)((f / N) % (N + 1)); i = (int)(f / ((int64_t)N * (N + 1))); }
-    else { k = (int)(f % (N + 1)); j = (int)((f / (N + 1)) % N); i = (int)(f / ((int64_t)N * (N + 1))); }
-    const float maxd = (float)N;
-    const float RE = p.sharpen_re;
-    const bool tri = p.splat == 1;
-    float fx = (float)i, fy = (float)j, fz = (float)k;
-    int xlo, xhi, ylo, yhi, zlo, zhi;
-    if (comp == 0) { fy += 0.5f; fz += 0.5f; xlo = i - 1; xhi = i; ylo = j - 2; yhi = j + 1; zlo = k - 2; zhi = k + 1; }
-    else if (comp == 1) { fx += 0.5f; fz += 0.5f; xlo = i - 2; xhi = i + 1; ylo = j - 1; yhi = j; zlo = k - 2; zhi = k + 1; }
-    else { fx += 0.5f; fy += 0.5f; xlo = i - 2; xhi = i + 1; ylo = j - 2; yhi = j + 1; zlo = k - 1; zhi = k; }
-    zlo = max(zlo, 0);
-    zhi = min(zhi, N - 1);
-    float sumw = 0.0f, sumc = 0.0f;
-    if (zlo <= zhi) {
-        for (int sx = max(xlo, 0); sx <= min(xhi, N - 1); ++sx) {
-            for (int sy = max(ylo, 0); sy <= min(yhi, N - 1); ++sy) {
-                const int64_t col = cidx(N, sx, sy, 0);
-                const int32_t q0 = offsets[col + zlo], q1 = offsets[col + zhi + 1];
-                for (int32_t q = q0; q < q1; ++q) {
-                    const float4 o = pos[q];
-                    float px = gmax(0.0f, gmin(maxd, maxd * o.x));
-                    float py = gmax(0.0f, gmin(maxd, maxd * o.y));
-                    float pz = gmax(0.0f, gmin(maxd, maxd * o.z));
-                    float w;
-                    if (tri) {
-                        w = (gmax(1.0f - fabsf(px - fx), 0.0f) * gmax(1.0f - fabsf(py - fy), 0.0f)) *
-                            gmax(1.0f - fabsf(pz - fz), 0.0f);
-                    } else {
-                        w = 1.0f * sharpen(sqrlength(px, py, pz, fx, fy, fz), RE);
+struct KP2g {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const float4* pos;
+        const float4* vel;
+        const int32_t* offsets;
+        const uint8_t* A;
+        float* ux;
+        float* uy;
+        float* uz;
+        float* dux;
+        float* duy;
+        float* duz;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const float4* __restrict__ pos = kargs.pos;
+        const float4* __restrict__ vel = kargs.vel;
+        const int32_t* __restrict__ offsets = kargs.offsets;
+        const uint8_t* __restrict__ A = kargs.A;
+        float* __restrict__ ux = kargs.ux;
+        float* __restrict__ uy = kargs.uy;
+        float* __restrict__ uz = kargs.uz;
+        float* __restrict__ dux = kargs.dux;
+        float* __restrict__ duy = kargs.duy;
+        float* __restrict__ duz = kargs.duz;
+
+        const int N = p.N;
+        const int64_t nf = (int64_t)(N + 1) * N * N;
+        int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (g >= 3 * nf) return;
+        const int comp = (int)(g / nf);
+        const int64_t f = g - comp * nf;
+        int i, j, k;
+        if (comp == 0) { k = (int)(f % N); j = (int)((f / N) % N); i = (int)(f / ((int64_t)N * N)); }
+        else if (comp == 1) { k = (int)(f % N); j = (int)((f / N) % (N + 1)); i = (int)(f / ((int64_t)N * (N + 1))); }
+        else { k = (int)(f % (N + 1)); j = (int)((f / (N + 1)) % N); i = (int)(f / ((int64_t)N * (N + 1))); }
+        const float maxd = (float)N;
+        const float RE = p.sharpen_re;
+        const bool tri = p.splat == 1;
+        float fx = (float)i, fy = (float)j, fz = (float)k;
+        int xlo, xhi, ylo, yhi, zlo, zhi;
+        if (comp == 0) { fy += 0.5f; fz += 0.5f; xlo = i - 1; xhi = i; ylo = j - 2; yhi = j + 1; zlo = k - 2; zhi = k + 1; }
+        else if (comp == 1) { fx += 0.5f; fz += 0.5f; xlo = i - 2; xhi = i + 1; ylo = j - 1; yhi = j; zlo = k - 2; zhi = k + 1; }
+        else { fx += 0.5f; fy += 0.5f; xlo = i - 2; xhi = i + 1; ylo = j - 2; yhi = j + 1; zlo = k - 1; zhi = k; }
+        zlo = max(zlo, 0);
+        zhi = min(zhi, N - 1);
+        float sumw = 0.0f, sumc = 0.0f;
+        if (zlo <= zhi) {
+            for (int sx = max(xlo, 0); sx <= min(xhi, N - 1); ++sx) {
+                for (int sy = max(ylo, 0); sy <= min(yhi, N - 1); ++sy) {
+                    const int64_t col = cidx(N, sx, sy, 0);
+                    const int32_t q0 = offsets[col + zlo], q1 = offsets[col + zhi + 1];
+                    for (int32_t q = q0; q < q1; ++q) {
+                        const float4 o = pos[q];
+                        float px = gmax(0.0f, gmin(maxd, maxd * o.x));
+                        float py = gmax(0.0f, gmin(maxd, maxd * o.y));
+                        float pz = gmax(0.0f, gmin(maxd, maxd * o.z));
+                        float w;
+                        if (tri) {
+                            w = (gmax(1.0f - fabsf(px - fx), 0.0f) * gmax(1.0f - fabsf(py - fy), 0.0f)) *
+                                gmax(1.0f - fabsf(pz - fz), 0.0f);
+                        } else {
+                            w = 1.0f * sharpen(sqrlength(px, py, pz, fx, fy, fz), RE);
+                        }
+                        const float4 v = vel[q];
+                        const float vc = comp == 0 ? v.x : (comp == 1 ? v.y : v.z);
+                        sumc += w * vc;
+                        sumw += w;
                     }
-                    const float4 v = vel[q];
-                    const float vc = comp == 0 ? v.x : (comp == 1 ? v.y : v.z);
-                    sumc += w * vc;
-                    sumw += w;
                 }
             }
         }
+        float val = 0.0f;
+        if (sumw > 0) val = sumc / sumw;
+        float* du = comp == 0 ? dux : (comp == 1 ? duy : duz);
+        float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
+        du[f] = val;
+        // boundary: domain faces, and faces between a SOLID and a non-SOLID cell
+        int a = comp == 0 ? i : (comp == 1 ? j : k);
+        bool zero = (a == 0 || a == N);
+        if (!zero) {
+            int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
+            bool s0 = A[cidx(N, i, j, k)] == kSolid, s1 = A[cidx(N, bi, bj, bk)] == kSolid;
+            zero = s0 != s1;
+        }
+        u[f] = zero ? 0.0f : val;
     }
-    float val = 0.0f;
-    if (sumw > 0) val = sumc / sumw;
-    float* du = comp == 0 ? dux : (comp == 1 ? duy : duz);
-    float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
-    du[f] = val;
-    // boundary: domain faces, and faces between a SOLID and a non-SOLID cell
-    int a = comp == 0 ? i : (comp == 1 ? j : k);
-    bool zero = (a == 0 || a == N);
-    if (!zero) {
-        int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
-        bool s0 = A[cidx(N, i, j, k)] == kSolid, s1 = A[cidx(N, bi, bj, bk)] == kSolid;
-        zero = s0 != s1;
-    }
-    u[f] = zero ? 0.0f : val;
-}
+};
 
 // P2G through LDS. A 64-lane block owns the cells (i, j, k0..k0+63) and computes
 // the x-, y- and z-faces of each (faces at i = N, j = N, k = N included). The
@@ -502,170 +657,216 @@ __device__ __forceinline__ float p2g_weight(float dx, float dy, float dz, float 
 }
 
 template <bool TRI>
-__global__ __launch_bounds__(64) void k_p2g_cols(Params p, const float4* __restrict__ pos,
-                                                  const float4* __restrict__ vel,
-                                                  const int32_t* __restrict__ offsets,
-                                                  const uint8_t* __restrict__ A,
-                                                  float* __restrict__ ux, float* __restrict__ uy,
-                                                  float* __restrict__ uz, float* __restrict__ dux,
-                                                  float* __restrict__ duy, float* __restrict__ duz) {
-    __shared__ float sp[6][kP2GCap];
-    const int N = p.N;
-    const int nkc = (N + 1 + 63) / 64;
-    // Blocks run over 8x8 tiles of face columns (i, j), k-chunks innermost, so
-    // the particle columns that neighbouring face columns share are staged while
-    // still in L2; each XCD takes a contiguous run of tiles.
-    constexpr int kT = 8;
-    const int nti = (N + 1 + kT - 1) / kT;
-    const int64_t nbl = (int64_t)nti * nti * kT * kT * nkc;
-    int64_t b = xcd_block(nbl);
-    if (b >= nbl) return;
-    const int kc = (int)(b % nkc);
-    b /= nkc;
-    const int jj = (int)(b % kT);
-    b /= kT;
-    const int ii = (int)(b % kT);
-    b /= kT;
-    const int j = (int)(b % nti) * kT + jj;
-    const int i = (int)(b / nti) * kT + ii;
-    if (i > N || j > N) return;
-    const int lane = threadIdx.x;
-    const int k0 = kc * 64;
-    const int k = k0 + lane;
-    const bool kval = k <= N;
-    const bool vx = kval && j < N && k < N;
-    const bool vy = kval && i < N && k < N;
-    const bool vz = kval && i < N && j < N;
-    const float maxd = (float)N;
-    const float RE = p.sharpen_re;
-    const float hh = RE * RE;
-    const bool tri = TRI;
-    const float fi = (float)i, fj = (float)j, fk = (float)k;
-    const float fih = fi + 0.5f, fjh = fj + 0.5f, fkh = fk + 0.5f;
-    float swx = 0.0f, sux = 0.0f, swy = 0.0f, suy = 0.0f, swz = 0.0f, suz = 0.0f;
-    // Along a face's tangential axis the window's first cell ([f-2.5, f-1.5]
-    // from the face coordinate f) is at least 1.5 cells away: every weight
-    // there is exactly 0 for the trilinear kernel and for Sharpen with
-    // RE^2 <= 2.25 (the weight is 0 once r2 >= RE^2), so those cells are not
-    // visited. The window then starts one cell later on every axis.
-    const int lo = (tri || hh <= 2.25f) ? 1 : 2;
-    const int zlo_blk = max(0, k0 - lo), zhi_blk = min(N - 1, k0 + 64);
-    const int a0 = max(k - lo, 0), a1 = min(k + 1, N - 1);
-    const int b0 = max(k - 1, 0), b1 = min(k, N - 1);
-    for (int sx = max(i - lo, 0); sx <= min(i + 1, N - 1); ++sx) {
-        for (int sy = max(j - lo, 0); sy <= min(j + 1, N - 1); ++sy) {
-            const bool colx = (sx >= i - 1 && sx <= i) && j < N;
-            const bool coly = (sy >= j - 1 && sy <= j) && i < N;
-            const bool colz = i < N && j < N;
-            if (!(colx || coly || colz)) continue;
-            const int64_t col = cidx(N, sx, sy, 0);
-            const int32_t c_begin = offsets[col + zlo_blk], c_end = offsets[col + zhi_blk + 1];
-            int32_t ra0 = 0, ra1 = 0, rb0 = 0, rb1 = 0;
-            if (kval && a0 <= a1) { ra0 = offsets[col + a0]; ra1 = offsets[col + a1 + 1]; }
-            if (kval && b0 <= b1) { rb0 = offsets[col + b0]; rb1 = offsets[col + b1 + 1]; }
-            const bool dox = vx && colx, doy = vy && coly, doz = vz && colz;
-            for (int32_t c0 = c_begin; c0 < c_end; c0 += kP2GCap) {
-                const int32_t c1 = min(c_end, c0 + kP2GCap);
-                __syncthreads();
-                for (int32_t q = c0 + lane; q < c1; q += 64) {
-                    const float4 o = pos[q];
-                    const float4 v = vel[q];
-                    const int t = q - c0;
-                    sp[0][t] = gmax(0.0f, gmin(maxd, maxd * o.x));
-                    sp[1][t] = gmax(0.0f, gmin(maxd, maxd * o.y));
-                    sp[2][t] = gmax(0.0f, gmin(maxd, maxd * o.z));
-                    sp[3][t] = v.x;
-                    sp[4][t] = v.y;
-                    sp[5][t] = v.z;
-                }
-                __syncthreads();
-                const int32_t q0 = max(ra0, c0), q1 = min(ra1, c1);
-                for (int32_t q = q0; q < q1; ++q) {
-                    const int t = q - c0;
-                    const float px = sp[0][t], py = sp[1][t], pz = sp[2][t];
-                    // offsets to the three faces (x face at (i, j+.5, k+.5), ...)
-                    const float ax = px - fi, bx = px - fih;
-                    const float ay = py - fj, by = py - fjh;
-                    const float az = pz - fk, bz = pz - fkh;
-                    if (dox) {
-                        const float w = p2g_weight<TRI>(ax, by, bz, hh);
-                        sux += w * sp[3][t];
-                        swx += w;
+struct KP2gCols {
+    static constexpr int kThreads = 64;
+    struct Args {
+        Params p;
+        const float4* pos;
+        const float4* vel;
+        const int32_t* offsets;
+        const uint8_t* A;
+        float* ux;
+        float* uy;
+        float* uz;
+        float* dux;
+        float* duy;
+        float* duz;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const float4* __restrict__ pos = kargs.pos;
+        const float4* __restrict__ vel = kargs.vel;
+        const int32_t* __restrict__ offsets = kargs.offsets;
+        const uint8_t* __restrict__ A = kargs.A;
+        float* __restrict__ ux = kargs.ux;
+        float* __restrict__ uy = kargs.uy;
+        float* __restrict__ uz = kargs.uz;
+        float* __restrict__ dux = kargs.dux;
+        float* __restrict__ duy = kargs.duy;
+        float* __restrict__ duz = kargs.duz;
+
+        __shared__ float sp[6][kP2GCap];
+        const int N = p.N;
+        const int nkc = (N + 1 + 63) / 64;
+        // Blocks run over 8x8 tiles of face columns (i, j), k-chunks innermost, so
+        // the particle columns that neighbouring face columns share are staged while
+        // still in L2; each XCD takes a contiguous run of tiles.
+        constexpr int kT = 8;
+        const int nti = (N + 1 + kT - 1) / kT;
+        const int64_t nbl = (int64_t)nti * nti * kT * kT * nkc;
+        int64_t b = xcd_block(nbl);
+        if (b >= nbl) return;
+        const int kc = (int)(b % nkc);
+        b /= nkc;
+        const int jj = (int)(b % kT);
+        b /= kT;
+        const int ii = (int)(b % kT);
+        b /= kT;
+        const int j = (int)(b % nti) * kT + jj;
+        const int i = (int)(b / nti) * kT + ii;
+        if (i > N || j > N) return;
+        const int lane = threadIdx.x;
+        const int k0 = kc * 64;
+        const int k = k0 + lane;
+        const bool kval = k <= N;
+        const bool vx = kval && j < N && k < N;
+        const bool vy = kval && i < N && k < N;
+        const bool vz = kval && i < N && j < N;
+        const float maxd = (float)N;
+        const float RE = p.sharpen_re;
+        const float hh = RE * RE;
+        const bool tri = TRI;
+        const float fi = (float)i, fj = (float)j, fk = (float)k;
+        const float fih = fi + 0.5f, fjh = fj + 0.5f, fkh = fk + 0.5f;
+        float swx = 0.0f, sux = 0.0f, swy = 0.0f, suy = 0.0f, swz = 0.0f, suz = 0.0f;
+        // Along a face's tangential axis the window's first cell ([f-2.5, f-1.5]
+        // from the face coordinate f) is at least 1.5 cells away: every weight
+        // there is exactly 0 for the trilinear kernel and for Sharpen with
+        // RE^2 <= 2.25 (the weight is 0 once r2 >= RE^2), so those cells are not
+        // visited. The window then starts one cell later on every axis.
+        const int lo = (tri || hh <= 2.25f) ? 1 : 2;
+        const int zlo_blk = max(0, k0 - lo), zhi_blk = min(N - 1, k0 + 64);
+        const int a0 = max(k - lo, 0), a1 = min(k + 1, N - 1);
+        const int b0 = max(k - 1, 0), b1 = min(k, N - 1);
+        for (int sx = max(i - lo, 0); sx <= min(i + 1, N - 1); ++sx) {
+            for (int sy = max(j - lo, 0); sy <= min(j + 1, N - 1); ++sy) {
+                const bool colx = (sx >= i - 1 && sx <= i) && j < N;
+                const bool coly = (sy >= j - 1 && sy <= j) && i < N;
+                const bool colz = i < N && j < N;
+                if (!(colx || coly || colz)) continue;
+                const int64_t col = cidx(N, sx, sy, 0);
+                const int32_t c_begin = offsets[col + zlo_blk], c_end = offsets[col + zhi_blk + 1];
+                int32_t ra0 = 0, ra1 = 0, rb0 = 0, rb1 = 0;
+                if (kval && a0 <= a1) { ra0 = offsets[col + a0]; ra1 = offsets[col + a1 + 1]; }
+                if (kval && b0 <= b1) { rb0 = offsets[col + b0]; rb1 = offsets[col + b1 + 1]; }
+                const bool dox = vx && colx, doy = vy && coly, doz = vz && colz;
+                for (int32_t c0 = c_begin; c0 < c_end; c0 += kP2GCap) {
+                    const int32_t c1 = min(c_end, c0 + kP2GCap);
+                    __syncthreads();
+                    for (int32_t q = c0 + lane; q < c1; q += 64) {
+                        const float4 o = pos[q];
+                        const float4 v = vel[q];
+                        const int t = q - c0;
+                        sp[0][t] = gmax(0.0f, gmin(maxd, maxd * o.x));
+                        sp[1][t] = gmax(0.0f, gmin(maxd, maxd * o.y));
+                        sp[2][t] = gmax(0.0f, gmin(maxd, maxd * o.z));
+                        sp[3][t] = v.x;
+                        sp[4][t] = v.y;
+                        sp[5][t] = v.z;
                     }
-                    if (doy) {
-                        const float w = p2g_weight<TRI>(bx, ay, bz, hh);
-                        suy += w * sp[4][t];
-                        swy += w;
-                    }
-                    if (doz) {
-                        const float w = (q >= rb0 && q < rb1) ? p2g_weight<TRI>(bx, by, az, hh) : 0.0f;
-                        suz += w * sp[5][t];
-                        swz += w;
+                    __syncthreads();
+                    const int32_t q0 = max(ra0, c0), q1 = min(ra1, c1);
+                    for (int32_t q = q0; q < q1; ++q) {
+                        const int t = q - c0;
+                        const float px = sp[0][t], py = sp[1][t], pz = sp[2][t];
+                        // offsets to the three faces (x face at (i, j+.5, k+.5), ...)
+                        const float ax = px - fi, bx = px - fih;
+                        const float ay = py - fj, by = py - fjh;
+                        const float az = pz - fk, bz = pz - fkh;
+                        if (dox) {
+                            const float w = p2g_weight<TRI>(ax, by, bz, hh);
+                            sux += w * sp[3][t];
+                            swx += w;
+                        }
+                        if (doy) {
+                            const float w = p2g_weight<TRI>(bx, ay, bz, hh);
+                            suy += w * sp[4][t];
+                            swy += w;
+                        }
+                        if (doz) {
+                            const float w = (q >= rb0 && q < rb1) ? p2g_weight<TRI>(bx, by, az, hh) : 0.0f;
+                            suz += w * sp[5][t];
+                            swz += w;
+                        }
                     }
                 }
             }
         }
+        // StorePreviousGrid copy, then EnforceBoundaryVelocity
+        if (vx) {
+            const int64_t f = fidx(N, 0, i, j, k);
+            const float val = swx > 0 ? sux / swx : 0.0f;
+            dux[f] = val;
+            bool zero = (i == 0 || i == N);
+            if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i - 1, j, k)] == kSolid);
+            ux[f] = zero ? 0.0f : val;
+        }
+        if (vy) {
+            const int64_t f = fidx(N, 1, i, j, k);
+            const float val = swy > 0 ? suy / swy : 0.0f;
+            duy[f] = val;
+            bool zero = (j == 0 || j == N);
+            if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j - 1, k)] == kSolid);
+            uy[f] = zero ? 0.0f : val;
+        }
+        if (vz) {
+            const int64_t f = fidx(N, 2, i, j, k);
+            const float val = swz > 0 ? suz / swz : 0.0f;
+            duz[f] = val;
+            bool zero = (k == 0 || k == N);
+            if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j, k - 1)] == kSolid);
+            uz[f] = zero ? 0.0f : val;
+        }
     }
-    // StorePreviousGrid copy, then EnforceBoundaryVelocity
-    if (vx) {
-        const int64_t f = fidx(N, 0, i, j, k);
-        const float val = swx > 0 ? sux / swx : 0.0f;
-        dux[f] = val;
-        bool zero = (i == 0 || i == N);
-        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i - 1, j, k)] == kSolid);
-        ux[f] = zero ? 0.0f : val;
-    }
-    if (vy) {
-        const int64_t f = fidx(N, 1, i, j, k);
-        const float val = swy > 0 ? suy / swy : 0.0f;
-        duy[f] = val;
-        bool zero = (j == 0 || j == N);
-        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j - 1, k)] == kSolid);
-        uy[f] = zero ? 0.0f : val;
-    }
-    if (vz) {
-        const int64_t f = fidx(N, 2, i, j, k);
-        const float val = swz > 0 ? suz / swz : 0.0f;
-        duz[f] = val;
-        bool zero = (k == 0 || k == N);
-        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j, k - 1)] == kSolid);
-        uz[f] = zero ? 0.0f : val;
-    }
-}
+};
 
 // ------------------------------------------- SolvePicFlip + AdvectParticles
 // flip.cpp:287-322 and 229-285: FLIP = u + I(du), PIC = I(u), blend
 // (1-r)*PIC + r*FLIP; p += dt * I(u)(p) (the PIC velocity at the same position);
 // wall clamp to [1/maxd, 1-1/maxd]. The re-sort inside AdvectParticles only
 // serves the solid-particle push, which is inert without SOLID particles.
-__global__ void k_g2p_advect(Params p, float4* __restrict__ pos, float4* __restrict__ vel,
-                             int64_t np, float* ux, float* uy, float* uz, float* dux, float* duy,
-                             float* duz) {
-    const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
-    if (s >= np) return;
-    const int N = p.N;
-    float* g[3] = {ux, uy, uz};
-    float* d[3] = {dux, duy, duz};
-    float4 me = pos[s];
-    float4 v = vel[s];
-    float dx, dy, dz, px, py, pz;
-    interpolate_velocity(d, N, me.x, me.y, me.z, dx, dy, dz);
-    float tx = dx + v.x, ty = dy + v.y, tz = dz + v.z;
-    interpolate_velocity(g, N, me.x, me.y, me.z, px, py, pz);
-    const float r = p.picflip;
-    v.x = (1.0f - r) * px + r * tx;
-    v.y = (1.0f - r) * py + r * ty;
-    v.z = (1.0f - r) * pz + r * tz;
-    me.x = me.x + p.dt * px;
-    me.y = me.y + p.dt * py;
-    me.z = me.z + p.dt * pz;
-    const float rr = 1.0f / p.maxd;
-    me.x = gmax(rr, gmin(1.0f - rr, me.x));
-    me.y = gmax(rr, gmin(1.0f - rr, me.y));
-    me.z = gmax(rr, gmin(1.0f - rr, me.z));
-    pos[s] = me;
-    vel[s] = v;
-}
+struct KG2pAdvect {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        float4* pos;
+        float4* vel;
+        int64_t np;
+        float* ux;
+        float* uy;
+        float* uz;
+        float* dux;
+        float* duy;
+        float* duz;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        float4* __restrict__ pos = kargs.pos;
+        float4* __restrict__ vel = kargs.vel;
+        int64_t np = kargs.np;
+        float* ux = kargs.ux;
+        float* uy = kargs.uy;
+        float* uz = kargs.uz;
+        float* dux = kargs.dux;
+        float* duy = kargs.duy;
+        float* duz = kargs.duz;
+
+        const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
+        if (s >= np) return;
+        const int N = p.N;
+        float* g[3] = {ux, uy, uz};
+        float* d[3] = {dux, duy, duz};
+        float4 me = pos[s];
+        float4 v = vel[s];
+        float dx, dy, dz, px, py, pz;
+        interpolate_velocity(d, N, me.x, me.y, me.z, dx, dy, dz);
+        float tx = dx + v.x, ty = dy + v.y, tz = dz + v.z;
+        interpolate_velocity(g, N, me.x, me.y, me.z, px, py, pz);
+        const float r = p.picflip;
+        v.x = (1.0f - r) * px + r * tx;
+        v.y = (1.0f - r) * py + r * ty;
+        v.z = (1.0f - r) * pz + r * tz;
+        me.x = me.x + p.dt * px;
+        me.y = me.y + p.dt * py;
+        me.z = me.z + p.dt * pz;
+        const float rr = 1.0f / p.maxd;
+        me.x = gmax(rr, gmin(1.0f - rr, me.x));
+        me.y = gmax(rr, gmin(1.0f - rr, me.y));
+        me.z = gmax(rr, gmin(1.0f - rr, me.z));
+        pos[s] = me;
+        vel[s] = v;
+    }
+};
 
 // ------------------------------------------------------- ResampleParticles
 // particleresampler.inl:35-148. Neighbour cells come from GetCellNeighbors with
@@ -706,299 +907,354 @@ __device__ __forceinline__ void narrow_z(float z, float gxy2, float gbound, floa
 
 constexpr int kSpringQueue = 8;   // queued in-radius neighbours per thread
 
-__global__ __launch_bounds__(kStageBlock) void k_resample(Params p, const float4* __restrict__ pos,
-                                                     const float4* __restrict__ vel, int64_t np,
-                                                     const int32_t* __restrict__ offsets,
-                                                     float4* __restrict__ pout,
-                                                     float4* __restrict__ vout,
-                                                     uint8_t* __restrict__ accept) {
-    __shared__ StageLds st;
-    __shared__ int s_queue[kSpringQueue][kStageBlock];
-    const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
-    const bool valid = s < np;
-    const int N = p.N;
-    const float maxd = p.maxd;
-    const float re = p.density / maxd;
-    const float dt = p.dt;
-    const float springforce = p.spring;
-    const float4 me = valid ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint32_t my_id = (uint32_t)__float_as_int(me.w);
-    const float re2 = re * re;
-    const float yre2 = 1.0f / re2;
-    const float re2_skip = re2 * 1.0001f;
-    // Cells are visited only when the distance bound to them (grid units, see
-    // cell_gap) is within re*1.0005: every pair in a skipped cell has
-    // dist^2 > re^2*1.0001 and would be skipped by the pair tests below.
-    const float gbound = re2 * (maxd * maxd) * 1.001f;
-    const float margin = 1.0e-3f + 4.0e-7f * maxd;
-    const uint32_t fr = (uint32_t)p.frame;
-    float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
-    // spring term of one neighbour within re (first loop of Resample); dx = me - o
-    // q is the neighbour's particle index, or its LDS slot when `slot` is set
-    // (resolved only on the rare coincident-pair path)
-    auto spring = [&](float dx, float dy, float dz, float d2, int32_t q, bool slot) {
-        float dist = sqrtf(d2);
-        float w = springforce * 1.0f * smooth_fixed(dist * dist, re2, yre2);
-        if (dist > 0.1f * re) {
-            sx_ += w * dx / dist * re;
-            sy_ += w * dy / dist * re;
-            sz_ += w * dz / dist * re;
-        } else {
-            const int32_t qq = slot ? staged_particle(st, q) : q;
-            const uint32_t qid = (uint32_t)__float_as_int(pos[qq].w);
-            uint32_t hx = hash4(p.seed, 3u * fr + 0u, my_id, qid);
-            uint32_t hy = hash4(p.seed, 3u * fr + 1u, my_id, qid);
-            uint32_t hz = hash4(p.seed, 3u * fr + 2u, my_id, qid);
-            sx_ += 0.01f * re / dt * (float)(int)(hx % 101u) / 100.0f;
-            sy_ += 0.01f * re / dt * (float)(int)(hy % 101u) / 100.0f;
-            sz_ += 0.01f * re / dt * (float)(int)(hz % 101u) / 100.0f;
-        }
+struct KResample {
+    static constexpr int kThreads = kStageBlock;
+    struct Args {
+        Params p;
+        const float4* pos;
+        const float4* vel;
+        int64_t np;
+        const int32_t* offsets;
+        float4* pout;
+        float4* vout;
+        uint8_t* accept;
     };
-    {
-        float x = gmax(0.0f, gmin(maxd, maxd * me.x));
-        float y = gmax(0.0f, gmin(maxd, maxd * me.y));
-        float z = gmax(0.0f, gmin(maxd, maxd * me.z));
-        const float xg = maxd * me.x, yg = maxd * me.y, zg = maxd * me.z;
-        int xl, xh, yl, yh, zl, zh;
-        cell_range(x, N, xl, xh);
-        cell_range(y, N, yl, yh);
-        cell_range(z, N, zl, zh);
-        stage_neighbourhood(st, valid && zl <= zh, xl, xh, yl, yh, zl, zh, N, offsets, pos);
-        if (!valid) return;
-        // With the neighbourhood staged, in-radius neighbours are queued (LDS
-        // index, in visiting order) and their spring terms evaluated in batches:
-        // evaluated inline, nearly every iteration of the wave would run the
-        // square root and divisions for the few lanes with a neighbour in range.
-        int queued = 0;
-        // An entry is the neighbour's LDS slot when the block is staged and its
-        // particle index otherwise.
-        const bool staged = st.ok;
-        auto flush = [&]() {
-            for (int t = 0; t < queued; ++t) {
-                const int e = s_queue[t][threadIdx.x];
-                float ox, oy, oz;
-                if (staged) {
-                    ox = st.x[e]; oy = st.y[e]; oz = st.z[e];
-                } else {
-                    const float4 o = pos[e];
-                    ox = o.x; oy = o.y; oz = o.z;
-                }
-                const float dx = me.x - ox, dy = me.y - oy, dz = me.z - oz;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                spring(dx, dy, dz, d2, e, staged);
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const float4* __restrict__ pos = kargs.pos;
+        const float4* __restrict__ vel = kargs.vel;
+        int64_t np = kargs.np;
+        const int32_t* __restrict__ offsets = kargs.offsets;
+        float4* __restrict__ pout = kargs.pout;
+        float4* __restrict__ vout = kargs.vout;
+        uint8_t* __restrict__ accept = kargs.accept;
+
+        __shared__ StageLds st;
+        __shared__ int s_queue[kSpringQueue][kStageBlock];
+        const int64_t s = xcd_block((np + blockDim.x - 1) / blockDim.x) * blockDim.x + threadIdx.x;
+        const bool valid = s < np;
+        const int N = p.N;
+        const float maxd = p.maxd;
+        const float re = p.density / maxd;
+        const float dt = p.dt;
+        const float springforce = p.spring;
+        const float4 me = valid ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const uint32_t my_id = (uint32_t)__float_as_int(me.w);
+        const float re2 = re * re;
+        const float yre2 = 1.0f / re2;
+        const float re2_skip = re2 * 1.0001f;
+        // Cells are visited only when the distance bound to them (grid units, see
+        // cell_gap) is within re*1.0005: every pair in a skipped cell has
+        // dist^2 > re^2*1.0001 and would be skipped by the pair tests below.
+        const float gbound = re2 * (maxd * maxd) * 1.001f;
+        const float margin = 1.0e-3f + 4.0e-7f * maxd;
+        const uint32_t fr = (uint32_t)p.frame;
+        float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
+        // spring term of one neighbour within re (first loop of Resample); dx = me - o
+        // q is the neighbour's particle index, or its LDS slot when `slot` is set
+        // (resolved only on the rare coincident-pair path)
+        auto spring = [&](float dx, float dy, float dz, float d2, int32_t q, bool slot) {
+            float dist = sqrtf(d2);
+            float w = springforce * 1.0f * smooth_fixed(dist * dist, re2, yre2);
+            if (dist > 0.1f * re) {
+                sx_ += w * dx / dist * re;
+                sy_ += w * dy / dist * re;
+                sz_ += w * dz / dist * re;
+            } else {
+                const int32_t qq = slot ? staged_particle(st, q) : q;
+                const uint32_t qid = (uint32_t)__float_as_int(pos[qq].w);
+                uint32_t hx = hash4(p.seed, 3u * fr + 0u, my_id, qid);
+                uint32_t hy = hash4(p.seed, 3u * fr + 1u, my_id, qid);
+                uint32_t hz = hash4(p.seed, 3u * fr + 2u, my_id, qid);
+                sx_ += 0.01f * re / dt * (float)(int)(hx % 101u) / 100.0f;
+                sy_ += 0.01f * re / dt * (float)(int)(hy % 101u) / 100.0f;
+                sz_ += 0.01f * re / dt * (float)(int)(hz % 101u) / 100.0f;
             }
-            queued = 0;
         };
-        // Per axis, the cells of the range whose distance bound is within the
-        // spring radius, ascending (at most 3, normally 2). Columns are taken in
-        // the reference's order (cx, then cy, ascending) from these lists, so a
-        // wave iterates about 2 x 2 columns instead of 3 x 3 of which each lane
-        // needs some.
-        int cxs[3], cys[3], nx = 0, ny = 0, za0 = 0, zb0 = -1;
-        float gxs[3], gys[3], gzs[3];
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            const int c = xl + m;
-            const float g = cell_gap(xg, c, margin);
-            if (c <= xh && g * g <= gbound) { cxs[nx] = c; gxs[nx] = g; ++nx; }
-        }
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            const int c = yl + m;
-            const float g = cell_gap(yg, c, margin);
-            if (c <= yh && g * g <= gbound) { cys[ny] = c; gys[ny] = g; ++ny; }
-        }
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            const int c = zl + m;
-            const float g = cell_gap(zg, c, margin);
-            gzs[m] = g;
-            if (c <= zh && g * g <= gbound) {
-                if (zb0 < za0) za0 = c;
-                zb0 = c;
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (a >= nx) break;
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                if (b >= ny) break;
-                const int cx = cxs[a], cy = cys[b];
-                const float gxy2 = gxs[a] * gxs[a] + gys[b] * gys[b];
-                if (gxy2 > gbound) continue;
-                // z cells of this column within the bound (ends of [za0, zb0])
-                int za = za0, zb = zb0;
-                while (za <= zb && gxy2 + gzs[za - zl] * gzs[za - zl] > gbound) ++za;
-                while (zb >= za && gxy2 + gzs[zb - zl] * gzs[zb - zl] > gbound) --zb;
-                if (za > zb) continue;
-                const int64_t col = cidx(N, cx, cy, 0);
-                const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
-                if (staged) {
-                    const int l0 = staged_index(st, cx, cy, q0);
-                    for (int32_t q = q0, l = l0; q < q1; ++q, ++l) {
-                        if (q == (int32_t)s) continue;
-                        const float dx = me.x - st.x[l], dy = me.y - st.y[l], dz = me.z - st.z[l];
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and
-                        // then dist > 0.1 re: the term adds +-0 to the spring. d2
-                        // above re^2 by a margin far beyond the rounding of sqrt and
-                        // the square guarantees that, so the pair is skipped without
-                        // changing a bit.
-                        if (d2 > re2_skip) continue;
-                        s_queue[queued][threadIdx.x] = l;
-                        if (++queued == kSpringQueue) flush();
+        {
+            float x = gmax(0.0f, gmin(maxd, maxd * me.x));
+            float y = gmax(0.0f, gmin(maxd, maxd * me.y));
+            float z = gmax(0.0f, gmin(maxd, maxd * me.z));
+            const float xg = maxd * me.x, yg = maxd * me.y, zg = maxd * me.z;
+            int xl, xh, yl, yh, zl, zh;
+            cell_range(x, N, xl, xh);
+            cell_range(y, N, yl, yh);
+            cell_range(z, N, zl, zh);
+            stage_neighbourhood(st, valid && zl <= zh, xl, xh, yl, yh, zl, zh, N, offsets, pos);
+            if (!valid) return;
+            // With the neighbourhood staged, in-radius neighbours are queued (LDS
+            // index, in visiting order) and their spring terms evaluated in batches:
+            // evaluated inline, nearly every iteration of the wave would run the
+            // square root and divisions for the few lanes with a neighbour in range.
+            int queued = 0;
+            // An entry is the neighbour's LDS slot when the block is staged and its
+            // particle index otherwise.
+            const bool staged = st.ok;
+            auto flush = [&]() {
+                for (int t = 0; t < queued; ++t) {
+                    const int e = s_queue[t][threadIdx.x];
+                    float ox, oy, oz;
+                    if (staged) {
+                        ox = st.x[e]; oy = st.y[e]; oz = st.z[e];
+                    } else {
+                        const float4 o = pos[e];
+                        ox = o.x; oy = o.y; oz = o.z;
                     }
-                } else {
+                    const float dx = me.x - ox, dy = me.y - oy, dz = me.z - oz;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    spring(dx, dy, dz, d2, e, staged);
+                }
+                queued = 0;
+            };
+            // Per axis, the cells of the range whose distance bound is within the
+            // spring radius, ascending (at most 3, normally 2). Columns are taken in
+            // the reference's order (cx, then cy, ascending) from these lists, so a
+            // wave iterates about 2 x 2 columns instead of 3 x 3 of which each lane
+            // needs some.
+            int cxs[3], cys[3], nx = 0, ny = 0, za0 = 0, zb0 = -1;
+            float gxs[3], gys[3], gzs[3];
+    #pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const int c = xl + m;
+                const float g = cell_gap(xg, c, margin);
+                if (c <= xh && g * g <= gbound) { cxs[nx] = c; gxs[nx] = g; ++nx; }
+            }
+    #pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const int c = yl + m;
+                const float g = cell_gap(yg, c, margin);
+                if (c <= yh && g * g <= gbound) { cys[ny] = c; gys[ny] = g; ++ny; }
+            }
+    #pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const int c = zl + m;
+                const float g = cell_gap(zg, c, margin);
+                gzs[m] = g;
+                if (c <= zh && g * g <= gbound) {
+                    if (zb0 < za0) za0 = c;
+                    zb0 = c;
+                }
+            }
+    #pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (a >= nx) break;
+    #pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    if (b >= ny) break;
+                    const int cx = cxs[a], cy = cys[b];
+                    const float gxy2 = gxs[a] * gxs[a] + gys[b] * gys[b];
+                    if (gxy2 > gbound) continue;
+                    // z cells of this column within the bound (ends of [za0, zb0])
+                    int za = za0, zb = zb0;
+                    while (za <= zb && gxy2 + gzs[za - zl] * gzs[za - zl] > gbound) ++za;
+                    while (zb >= za && gxy2 + gzs[zb - zl] * gzs[zb - zl] > gbound) --zb;
+                    if (za > zb) continue;
+                    const int64_t col = cidx(N, cx, cy, 0);
+                    const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
+                    if (staged) {
+                        const int l0 = staged_index(st, cx, cy, q0);
+                        for (int32_t q = q0, l = l0; q < q1; ++q, ++l) {
+                            if (q == (int32_t)s) continue;
+                            const float dx = me.x - st.x[l], dy = me.y - st.y[l], dz = me.z - st.z[l];
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and
+                            // then dist > 0.1 re: the term adds +-0 to the spring. d2
+                            // above re^2 by a margin far beyond the rounding of sqrt and
+                            // the square guarantees that, so the pair is skipped without
+                            // changing a bit.
+                            if (d2 > re2_skip) continue;
+                            s_queue[queued][threadIdx.x] = l;
+                            if (++queued == kSpringQueue) flush();
+                        }
+                    } else {
+                        for (int32_t q = q0; q < q1; ++q) {
+                            if (q == (int32_t)s) continue;
+                            const float4 o = pos[q];
+                            const float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 > re2_skip) continue;
+                            s_queue[queued][threadIdx.x] = q;
+                            if (++queued == kSpringQueue) flush();
+                        }
+                    }
+                }
+            }
+            flush();
+        }
+        const float4 mv = vel[s];
+        const float tx = me.x + dt * sx_, ty = me.y + dt * sy_, tz = me.z + dt * sz_;
+        float rx = 0.0f, ry = 0.0f, rz = 0.0f, wsum = 0.0f;
+        // With re^2 <= 1e-5 every Sharpen weight max(re^2/max(d2,1e-5) - 1, 0) is 0:
+        // the velocity is kept (Resample's "else ru = u"), without visiting neighbours.
+        if (re2 > 1.0e-5f) {
+            float x = gmax(0.0f, gmin(maxd - 1, maxd * tx));
+            float y = gmax(0.0f, gmin(maxd - 1, maxd * ty));
+            float z = gmax(0.0f, gmin(maxd - 1, maxd * tz));
+            const float xg = maxd * tx, yg = maxd * ty, zg = maxd * tz;
+            int xl, xh, yl, yh, zl, zh;
+            cell_range(x, N, xl, xh);
+            cell_range(y, N, yl, yh);
+            cell_range(z, N, zl, zh);
+            for (int cx = xl; cx <= xh; ++cx) {
+                const float gx = cell_gap(xg, cx, margin);
+                for (int cy = yl; cy <= yh; ++cy) {
+                    const float gy = cell_gap(yg, cy, margin);
+                    const float gxy2 = gx * gx + gy * gy;
+                    if (gxy2 > gbound) continue;
+                    int za = zl, zb = zh;
+                    narrow_z(zg, gxy2, gbound, margin, za, zb);
+                    if (za > zb) continue;
+                    const int64_t col = cidx(N, cx, cy, 0);
+                    const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
                     for (int32_t q = q0; q < q1; ++q) {
-                        if (q == (int32_t)s) continue;
                         const float4 o = pos[q];
-                        const float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        if (d2 > re2_skip) continue;
-                        s_queue[queued][threadIdx.x] = q;
-                        if (++queued == kSpringQueue) flush();
+                        float dist2 = sqrlength(tx, ty, tz, o.x, o.y, o.z);
+                        // Sharpen(dist2, re) = max(re^2/max(dist2,1e-5) - 1, 0) is exactly 0
+                        // when max(dist2, 1e-5) >= re^2; the +0 term is skipped.
+                        if (gmax(dist2, 1.0e-5f) >= re2) continue;
+                        float w = 1.0f * sharpen_nr(dist2, re2);
+                        const float4 ov = vel[q];
+                        rx = rx + w * ov.x;
+                        ry = ry + w * ov.y;
+                        rz = rz + w * ov.z;
+                        wsum += w;
                     }
                 }
             }
         }
-        flush();
-    }
-    const float4 mv = vel[s];
-    const float tx = me.x + dt * sx_, ty = me.y + dt * sy_, tz = me.z + dt * sz_;
-    float rx = 0.0f, ry = 0.0f, rz = 0.0f, wsum = 0.0f;
-    // With re^2 <= 1e-5 every Sharpen weight max(re^2/max(d2,1e-5) - 1, 0) is 0:
-    // the velocity is kept (Resample's "else ru = u"), without visiting neighbours.
-    if (re2 > 1.0e-5f) {
-        float x = gmax(0.0f, gmin(maxd - 1, maxd * tx));
-        float y = gmax(0.0f, gmin(maxd - 1, maxd * ty));
-        float z = gmax(0.0f, gmin(maxd - 1, maxd * tz));
-        const float xg = maxd * tx, yg = maxd * ty, zg = maxd * tz;
-        int xl, xh, yl, yh, zl, zh;
-        cell_range(x, N, xl, xh);
-        cell_range(y, N, yl, yh);
-        cell_range(z, N, zl, zh);
-        for (int cx = xl; cx <= xh; ++cx) {
-            const float gx = cell_gap(xg, cx, margin);
-            for (int cy = yl; cy <= yh; ++cy) {
-                const float gy = cell_gap(yg, cy, margin);
-                const float gxy2 = gx * gx + gy * gy;
-                if (gxy2 > gbound) continue;
-                int za = zl, zb = zh;
-                narrow_z(zg, gxy2, gbound, margin, za, zb);
-                if (za > zb) continue;
-                const int64_t col = cidx(N, cx, cy, 0);
-                const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
-                for (int32_t q = q0; q < q1; ++q) {
-                    const float4 o = pos[q];
-                    float dist2 = sqrlength(tx, ty, tz, o.x, o.y, o.z);
-                    // Sharpen(dist2, re) = max(re^2/max(dist2,1e-5) - 1, 0) is exactly 0
-                    // when max(dist2, 1e-5) >= re^2; the +0 term is skipped.
-                    if (gmax(dist2, 1.0e-5f) >= re2) continue;
-                    float w = 1.0f * sharpen_nr(dist2, re2);
-                    const float4 ov = vel[q];
-                    rx = rx + w * ov.x;
-                    ry = ry + w * ov.y;
-                    rz = rz + w * ov.z;
-                    wsum += w;
-                }
-            }
+        if (wsum != 0.0f) {
+            rx = rx / wsum; ry = ry / wsum; rz = rz / wsum;
+        } else {
+            rx = mv.x; ry = mv.y; rz = mv.z;
+        }
+        // accept unless t lies inside a solid (the static walls)
+        bool ok = true;
+        if (p.wall > 0) {
+            const float lo = (float)p.wall, hi = (float)(N - p.wall);
+            float gx = tx * maxd, gy = ty * maxd, gz = tz * maxd;
+            ok = !(gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi);
+        }
+        accept[s] = ok ? 1 : 0;
+        if (ok) {
+            pout[s] = make_float4(tx, ty, tz, me.w);
+            vout[s] = make_float4(rx, ry, rz, mv.w);
+        } else {
+            pout[s] = me;
+            vout[s] = mv;
         }
     }
-    if (wsum != 0.0f) {
-        rx = rx / wsum; ry = ry / wsum; rz = rz / wsum;
-    } else {
-        rx = mv.x; ry = mv.y; rz = mv.z;
-    }
-    // accept unless t lies inside a solid (the static walls)
-    bool ok = true;
-    if (p.wall > 0) {
-        const float lo = (float)p.wall, hi = (float)(N - p.wall);
-        float gx = tx * maxd, gy = ty * maxd, gz = tz * maxd;
-        ok = !(gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi);
-    }
-    accept[s] = ok ? 1 : 0;
-    if (ok) {
-        pout[s] = make_float4(tx, ty, tz, me.w);
-        vout[s] = make_float4(rx, ry, rz, mv.w);
-    } else {
-        pout[s] = me;
-        vout[s] = mv;
-    }
-}
+};
 
 // Scatter sorted particles back to particle-vector (id) order for the accessors.
-__global__ void k_unsort(const float4* __restrict__ pos, const float4* __restrict__ vel,
-                         const uint8_t* __restrict__ accept, int64_t np, float4* __restrict__ pout,
-                         float4* __restrict__ vout, uint8_t* __restrict__ aout,
-                         int32_t* __restrict__ ids_out) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= np) return;
-    float4 pp = pos[s];
-    int32_t id = __float_as_int(pp.w);
-    if (pout) pout[id] = pp;
-    if (vout) vout[id] = vel[s];
-    if (aout && accept) aout[id] = accept[s];
-    if (ids_out) ids_out[s] = id;
-}
+struct KUnsort {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const float4* pos;
+        const float4* vel;
+        const uint8_t* accept;
+        int64_t np;
+        float4* pout;
+        float4* vout;
+        uint8_t* aout;
+        int32_t* ids_out;
+    };
+    static __device__ void run(const Args& kargs) {
+        const float4* __restrict__ pos = kargs.pos;
+        const float4* __restrict__ vel = kargs.vel;
+        const uint8_t* __restrict__ accept = kargs.accept;
+        int64_t np = kargs.np;
+        float4* __restrict__ pout = kargs.pout;
+        float4* __restrict__ vout = kargs.vout;
+        uint8_t* __restrict__ aout = kargs.aout;
+        int32_t* __restrict__ ids_out = kargs.ids_out;
 
-__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-__global__ void k_fill_f32(float* p, int64_t n, float v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
+        int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (s >= np) return;
+        float4 pp = pos[s];
+        int32_t id = __float_as_int(pp.w);
+        if (pout) pout[id] = pp;
+        if (vout) vout[id] = vel[s];
+        if (aout && accept) aout[id] = accept[s];
+        if (ids_out) ids_out[s] = id;
+    }
+};
+
+struct KFillI32 {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int32_t* p;
+        int64_t n;
+        int32_t v;
+    };
+    static __device__ void run(const Args& kargs) {
+        int32_t* p = kargs.p;
+        int64_t n = kargs.n;
+        int32_t v = kargs.v;
+
+        int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < n) p[i] = v;
+    }
+};
+struct KFillF32 {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        float* p;
+        int64_t n;
+        float v;
+    };
+    static __device__ void run(const Args& kargs) {
+        float* p = kargs.p;
+        int64_t n = kargs.n;
+        float v = kargs.v;
+
+        int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < n) p[i] = v;
+    }
+};
 
 }  // namespace
 
 void launch_bin(hipStream_t s, const float4* pos, int64_t np, int N, float maxd, int32_t* cell_of,
                 int32_t* counts) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_bin, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, pos, np, N, maxd,
-                       cell_of, counts);
+    launch<KBin>(s, dim3(grid_for(np, kBlock)), dim3(kBlock), {pos, np, N, maxd, cell_of, counts});
 }
 
 void launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, int32_t* block_sums) {
     int nb = (int)((n + kScanTile - 1) / kScanTile);
     if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, n, block_sums);
-    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(kScanThreads), 0, s, block_sums, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, n, block_sums, out, nb);
+    launch<KScanReduce>(s, dim3(nb), dim3(kScanThreads), {in, n, block_sums});
+    launch<KScanBsums>(s, dim3(1), dim3(kScanThreads), {block_sums, nb});
+    launch<KScanApply>(s, dim3(nb), dim3(kScanThreads), {in, n, block_sums, out, nb});
 }
 
 void launch_scatter(hipStream_t s, const int32_t* cell_of, int64_t np, const int32_t* offsets,
                     int32_t* counts, int32_t* slots, const float4* pos, int32_t* sid) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, cell_of, np,
-                       offsets, counts, slots, pos, sid);
+    launch<KScatter>(s, dim3(grid_for(np, kBlock)), dim3(kBlock), {cell_of, np, offsets, counts, slots, pos, sid});
 }
 
 void launch_cellsort(hipStream_t s, const int32_t* offsets, int64_t nc, int32_t* sid,
                      int32_t* slots) {
-    hipLaunchKernelGGL(k_cellsort, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, offsets, nc, sid,
-                       slots);
+    launch<KCellsort>(s, dim3(grid_for(nc, kBlock)), dim3(kBlock), {offsets, nc, sid, slots});
 }
 
 void launch_gather(hipStream_t s, const int32_t* slots, int64_t np, const float4* pos_in,
                    const float4* vel_in, float4* pos_out, float4* vel_out) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, slots, np, pos_in,
-                       vel_in, pos_out, vel_out);
+    launch<KGather>(s, dim3(grid_for(np, kBlock)), dim3(kBlock), {slots, np, pos_in, vel_in, pos_out, vel_out});
 }
 
 void launch_density(hipStream_t s, const Params& p, const float4* pos, float4* vel, int64_t np,
                     const int32_t* offsets, int apply_forces) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_density, dim3((unsigned)xcd_grid(grid_for(np, kStageBlock))), dim3(kStageBlock), 0, s, p, pos, vel, np,
-                       offsets, apply_forces);
+    launch<KDensity>(s, dim3((unsigned)xcd_grid(grid_for(np, kStageBlock))), dim3(kStageBlock), {p, pos, vel, np, offsets, apply_forces});
 }
 
 void launch_mark(hipStream_t s, const Params& p, const float4* vel, const int32_t* offsets,
                  uint8_t* A, float* L) {
     int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_mark, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, vel, offsets, A, L);
+    launch<KMark>(s, dim3(grid_for(nc, kBlock)), dim3(kBlock), {p, vel, offsets, A, L});
 }
 
 void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
@@ -1007,11 +1263,9 @@ void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4*
     const int64_t nti = (p.N + 1 + 7) / 8;
     const int64_t nb = nti * nti * 64 * nkc;
     if (p.splat == 1)
-        hipLaunchKernelGGL(k_p2g_cols<true>, dim3((unsigned)xcd_grid(nb)), dim3(64), 0, s, p, pos, vel,
-                           offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]);
+        launch<KP2gCols<true>>(s, dim3((unsigned)xcd_grid(nb)), dim3(64), {p, pos, vel, offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]});
     else
-        hipLaunchKernelGGL(k_p2g_cols<false>, dim3((unsigned)xcd_grid(nb)), dim3(64), 0, s, p, pos, vel,
-                           offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]);
+        launch<KP2gCols<false>>(s, dim3((unsigned)xcd_grid(nb)), dim3(64), {p, pos, vel, offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]});
 }
 
 // Reference-shaped per-face gather (one thread per face), kept for cross-checks.
@@ -1019,40 +1273,36 @@ void launch_p2g_faces(hipStream_t s, const Params& p, const float4* pos, const f
                       const int32_t* offsets, const uint8_t* A, float* const u[3],
                       float* const du[3]) {
     int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
-    hipLaunchKernelGGL(k_p2g, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), 0, s, p, pos, vel,
-                       offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]);
+    launch<KP2g>(s, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), {p, pos, vel, offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]});
 }
 
 void launch_g2p_advect(hipStream_t s, const Params& p, float4* pos, float4* vel, int64_t np,
                        float* const u[3], float* const du[3]) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_g2p_advect, dim3((unsigned)xcd_grid(grid_for(np, kBlock))), dim3(kBlock), 0, s, p, pos, vel,
-                       np, u[0], u[1], u[2], du[0], du[1], du[2]);
+    launch<KG2pAdvect>(s, dim3((unsigned)xcd_grid(grid_for(np, kBlock))), dim3(kBlock), {p, pos, vel, np, u[0], u[1], u[2], du[0], du[1], du[2]});
 }
 
 void launch_resample(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
                      int64_t np, const int32_t* offsets, float4* pos_out, float4* vel_out,
                      uint8_t* accept) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_resample, dim3((unsigned)xcd_grid(grid_for(np, kStageBlock))), dim3(kStageBlock), 0, s, p, pos, vel, np,
-                       offsets, pos_out, vel_out, accept);
+    launch<KResample>(s, dim3((unsigned)xcd_grid(grid_for(np, kStageBlock))), dim3(kStageBlock), {p, pos, vel, np, offsets, pos_out, vel_out, accept});
 }
 
 void launch_unsort(hipStream_t s, const float4* pos, const float4* vel, const uint8_t* accept,
                    int64_t np, float4* pos_out, float4* vel_out, uint8_t* accept_out,
                    int32_t* ids_out) {
     if (np == 0) return;
-    hipLaunchKernelGGL(k_unsort, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, pos, vel, accept,
-                       np, pos_out, vel_out, accept_out, ids_out);
+    launch<KUnsort>(s, dim3(grid_for(np, kBlock)), dim3(kBlock), {pos, vel, accept, np, pos_out, vel_out, accept_out, ids_out});
 }
 
 void launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, p, n, v);
+    launch<KFillI32>(s, dim3(grid_for(n, kBlock)), dim3(kBlock), {p, n, v});
 }
 void launch_fill_f32(hipStream_t s, float* p, int64_t n, float v) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, p, n, v);
+    launch<KFillF32>(s, dim3(grid_for(n, kBlock)), dim3(kBlock), {p, n, v});
 }
 
 }  // namespace ariel

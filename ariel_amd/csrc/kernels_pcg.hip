@@ -136,58 +136,95 @@ __device__ __forceinline__ void decode_cell(int N, int64_t c64, int& i, int& j, 
 }
 
 // ------------------------------------------------------------ solver kernels
-__global__ void k_solver_init(SolverScalars* sc, int max_iters, int refine_every) {
-    if (threadIdx.x != 0) return;
-    sc->bnorm2 = sc->rnorm2 = sc->thr = sc->rho = sc->sigma = 0.0;
-    sc->alpha = sc->beta = sc->tmp = 0.0;
-    sc->done = 0;
-    sc->it = 0;
-    sc->status = 1;
-    sc->max_iters = max_iters;
-    sc->refine_every = refine_every;
-    sc->fluid_cells = 0;
-    sc->maxdiv_bits = 0u;
-}
+struct KSolverInit {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        SolverScalars* sc;
+        int max_iters;
+        int refine_every;
+    };
+    static __device__ void run(const Args& kargs) {
+        SolverScalars* sc = kargs.sc;
+        int max_iters = kargs.max_iters;
+        int refine_every = kargs.refine_every;
+
+        if (threadIdx.x != 0) return;
+        sc->bnorm2 = sc->rnorm2 = sc->thr = sc->rho = sc->sigma = 0.0;
+        sc->alpha = sc->beta = sc->tmp = 0.0;
+        sc->done = 0;
+        sc->it = 0;
+        sc->status = 1;
+        sc->max_iters = max_iters;
+        sc->refine_every = refine_every;
+        sc->fluid_cells = 0;
+        sc->maxdiv_bits = 0u;
+    }
+};
 
 // Warm start: P := 0 off FLUID (what the reference's first Op does), r = b - A p,
 // ||b||^2 and ||r||^2 over FLUID cells; stop threshold; early exit if converged.
 // With reliable updates (refine): x64 = P, r = b - A x64 in FP64; P is cleared
 // by a separate fill once every thread has read it.
-__global__ void k_residual0(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
-                            float* __restrict__ P, const float* __restrict__ D, float* __restrict__ R,
-                            Reduce red, double tol, int stop_rule, double* __restrict__ P64,
-                            int refine) {
-    const int N = p.N;
-    const int64_t nc = (int64_t)N * N * N;
-    const float n = (float)N;
-    const float h = 1.0f / (n * n);
-    double v[2] = {0.0, 0.0};
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        if (A[c] != kFluid) {
-            if (!refine) P[c] = 0.0f;
-            else P64[c] = 0.0;
-            R[c] = 0.0f;
-            continue;
+struct KResidual0 {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const uint8_t* A;
+        const float* L;
+        float* P;
+        const float* D;
+        float* R;
+        Reduce red;
+        double tol;
+        int stop_rule;
+        double* P64;
+        int refine;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const uint8_t* __restrict__ A = kargs.A;
+        const float* __restrict__ L = kargs.L;
+        float* __restrict__ P = kargs.P;
+        const float* __restrict__ D = kargs.D;
+        float* __restrict__ R = kargs.R;
+        const Reduce& red = kargs.red;
+        double tol = kargs.tol;
+        int stop_rule = kargs.stop_rule;
+        double* __restrict__ P64 = kargs.P64;
+        int refine = kargs.refine;
+
+        const int N = p.N;
+        const int64_t nc = (int64_t)N * N * N;
+        const float n = (float)N;
+        const float h = 1.0f / (n * n);
+        double v[2] = {0.0, 0.0};
+        for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+             c += (int64_t)gridDim.x * blockDim.x) {
+            if (A[c] != kFluid) {
+                if (!refine) P[c] = 0.0f;
+                else P64[c] = 0.0;
+                R[c] = 0.0f;
+                continue;
+            }
+            int i, j, k;
+            decode_cell(N, c, i, j, k);
+            const float d = D[c];
+            v[0] += (double)d * (double)d;
+            if (refine) {
+                const double r = (double)d - spmv_cell_f64(A, L, P, N, i, j, k, c);
+                P64[c] = (double)P[c];
+                R[c] = (float)r;
+                v[1] += r * r;
+            } else {
+                const float ax = spmv_cell(A, L, P, N, i, j, k, c, h);
+                const float r = d + (-1.0f) * ax;
+                R[c] = r;
+                v[1] += (double)r * (double)r;
+            }
         }
-        int i, j, k;
-        decode_cell(N, c, i, j, k);
-        const float d = D[c];
-        v[0] += (double)d * (double)d;
-        if (refine) {
-            const double r = (double)d - spmv_cell_f64(A, L, P, N, i, j, k, c);
-            P64[c] = (double)P[c];
-            R[c] = (float)r;
-            v[1] += r * r;
-        } else {
-            const float ax = spmv_cell(A, L, P, N, i, j, k, c, h);
-            const float r = d + (-1.0f) * ax;
-            R[c] = r;
-            v[1] += (double)r * (double)r;
-        }
+        block_partials<2>(v, red.partials, blockIdx.x);
     }
-    block_partials<2>(v, red.partials, blockIdx.x);
-}
+};
 
 // One XRef term for a neighbour with class t, value xq, level set lq; cell value
 // xc with level set lc (solver.inl:138-154).
@@ -272,31 +309,50 @@ __device__ __forceinline__ void spmv_quad(const Params& p, const uint8_t* __rest
 // One active segment per thread; blocks take contiguous runs of the list and
 // each XCD a contiguous eighth of the blocks, so the +-x neighbour planes of a
 // segment are read by the same XCD that reads them as its own (shared L2).
-__global__ __launch_bounds__(256) void k_spmv_dot(Params p, const uint8_t* __restrict__ A,
-                                                   const float* __restrict__ L,
-                                                   const float* __restrict__ x,
-                                                   float* __restrict__ y,
-                                                   const int32_t* __restrict__ segs, int64_t nseg,
-                                                   int per_thread, Reduce red) {
-    if (red.sc->done) return;
-    const int64_t per_block = (int64_t)blockDim.x * per_thread;
-    const int64_t nb = (nseg + per_block - 1) / per_block;
-    const int64_t b = xcd_block(nb);
-    if (b >= nb) return;
-    double v[1] = {0.0};
-    for (int m = 0; m < per_thread; ++m) {
-        const int64_t u = b * per_block + (int64_t)m * blockDim.x + threadIdx.x;
-        if (u >= nseg) break;
-        const uint32_t sg = (uint32_t)segs[u];
-        const uint32_t NS = (uint32_t)p.N / kTile;
-        const uint32_t line = sg / NS;   // i*N + j
-        const int k0 = (int)(sg - line * NS) * kTile;
-        const int j = (int)(line % (uint32_t)p.N), i = (int)(line / (uint32_t)p.N);
-        spmv_quad(p, A, L, x, y, 2 * (int64_t)sg, i, j, k0, v[0]);
-        spmv_quad(p, A, L, x, y, 2 * (int64_t)sg + 1, i, j, k0 + 4, v[0]);
+struct KSpmvDot {
+    static constexpr int kThreads = 256;
+    struct Args {
+        Params p;
+        const uint8_t* A;
+        const float* L;
+        const float* x;
+        float* y;
+        const int32_t* segs;
+        int64_t nseg;
+        int per_thread;
+        Reduce red;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const uint8_t* __restrict__ A = kargs.A;
+        const float* __restrict__ L = kargs.L;
+        const float* __restrict__ x = kargs.x;
+        float* __restrict__ y = kargs.y;
+        const int32_t* __restrict__ segs = kargs.segs;
+        int64_t nseg = kargs.nseg;
+        int per_thread = kargs.per_thread;
+        const Reduce& red = kargs.red;
+
+        if (red.sc->done) return;
+        const int64_t per_block = (int64_t)blockDim.x * per_thread;
+        const int64_t nb = (nseg + per_block - 1) / per_block;
+        const int64_t b = xcd_block(nb);
+        if (b >= nb) return;
+        double v[1] = {0.0};
+        for (int m = 0; m < per_thread; ++m) {
+            const int64_t u = b * per_block + (int64_t)m * blockDim.x + threadIdx.x;
+            if (u >= nseg) break;
+            const uint32_t sg = (uint32_t)segs[u];
+            const uint32_t NS = (uint32_t)p.N / kTile;
+            const uint32_t line = sg / NS;   // i*N + j
+            const int k0 = (int)(sg - line * NS) * kTile;
+            const int j = (int)(line % (uint32_t)p.N), i = (int)(line / (uint32_t)p.N);
+            spmv_quad(p, A, L, x, y, 2 * (int64_t)sg, i, j, k0, v[0]);
+            spmv_quad(p, A, L, x, y, 2 * (int64_t)sg + 1, i, j, k0 + 4, v[0]);
+        }
+        block_partials<1>(v, red.partials, b);
     }
-    block_partials<1>(v, red.partials, b);
-}
+};
 
 // Stop decision after the residual norm of iteration `it` is known.
 __device__ __forceinline__ void decide(SolverScalars* sc) {
@@ -318,141 +374,244 @@ __device__ __forceinline__ bool refine_due(const SolverScalars* sc) {
 
 // p += alpha s; r += (-alpha) z (Op, solver.inl:157-177); ||r||^2; stop test.
 // One active segment (two float4 of each operand) per thread and iteration.
-__global__ void k_axpy2(float* __restrict__ P, const float* __restrict__ S, float* __restrict__ R,
-                        const float* __restrict__ Z, const int32_t* __restrict__ segs,
-                        int64_t nseg, Reduce red) {
-    if (red.sc->done) return;
-    const float af = (float)red.sc->alpha;
-    const float naf = -af;
-    float4* __restrict__ P4 = reinterpret_cast<float4*>(P);
-    float4* __restrict__ R4 = reinterpret_cast<float4*>(R);
-    const float4* __restrict__ S4 = reinterpret_cast<const float4*>(S);
-    const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
-    double v[1] = {0.0};
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-         u += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = 2 * (int64_t)segs[u];
-        float4 p0 = P4[c], s0 = S4[c], r0 = R4[c], z0 = Z4[c];
-        float4 p1 = P4[c + 1], s1 = S4[c + 1], r1 = R4[c + 1], z1 = Z4[c + 1];
-        p0.x = p0.x + af * s0.x; p0.y = p0.y + af * s0.y; p0.z = p0.z + af * s0.z; p0.w = p0.w + af * s0.w;
-        r0.x = r0.x + naf * z0.x; r0.y = r0.y + naf * z0.y; r0.z = r0.z + naf * z0.z; r0.w = r0.w + naf * z0.w;
-        p1.x = p1.x + af * s1.x; p1.y = p1.y + af * s1.y; p1.z = p1.z + af * s1.z; p1.w = p1.w + af * s1.w;
-        r1.x = r1.x + naf * z1.x; r1.y = r1.y + naf * z1.y; r1.z = r1.z + naf * z1.z; r1.w = r1.w + naf * z1.w;
-        P4[c] = p0;
-        R4[c] = r0;
-        P4[c + 1] = p1;
-        R4[c + 1] = r1;
-        v[0] += (double)r0.x * (double)r0.x + (double)r0.y * (double)r0.y +
-                (double)r0.z * (double)r0.z + (double)r0.w * (double)r0.w;
-        v[0] += (double)r1.x * (double)r1.x + (double)r1.y * (double)r1.y +
-                (double)r1.z * (double)r1.z + (double)r1.w * (double)r1.w;
+struct KAxpy2 {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        float* P;
+        const float* S;
+        float* R;
+        const float* Z;
+        const int32_t* segs;
+        int64_t nseg;
+        Reduce red;
+    };
+    static __device__ void run(const Args& kargs) {
+        float* __restrict__ P = kargs.P;
+        const float* __restrict__ S = kargs.S;
+        float* __restrict__ R = kargs.R;
+        const float* __restrict__ Z = kargs.Z;
+        const int32_t* __restrict__ segs = kargs.segs;
+        int64_t nseg = kargs.nseg;
+        const Reduce& red = kargs.red;
+
+        if (red.sc->done) return;
+        const float af = (float)red.sc->alpha;
+        const float naf = -af;
+        float4* __restrict__ P4 = reinterpret_cast<float4*>(P);
+        float4* __restrict__ R4 = reinterpret_cast<float4*>(R);
+        const float4* __restrict__ S4 = reinterpret_cast<const float4*>(S);
+        const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
+        double v[1] = {0.0};
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+             u += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t c = 2 * (int64_t)segs[u];
+            float4 p0 = P4[c], s0 = S4[c], r0 = R4[c], z0 = Z4[c];
+            float4 p1 = P4[c + 1], s1 = S4[c + 1], r1 = R4[c + 1], z1 = Z4[c + 1];
+            p0.x = p0.x + af * s0.x; p0.y = p0.y + af * s0.y; p0.z = p0.z + af * s0.z; p0.w = p0.w + af * s0.w;
+            r0.x = r0.x + naf * z0.x; r0.y = r0.y + naf * z0.y; r0.z = r0.z + naf * z0.z; r0.w = r0.w + naf * z0.w;
+            p1.x = p1.x + af * s1.x; p1.y = p1.y + af * s1.y; p1.z = p1.z + af * s1.z; p1.w = p1.w + af * s1.w;
+            r1.x = r1.x + naf * z1.x; r1.y = r1.y + naf * z1.y; r1.z = r1.z + naf * z1.z; r1.w = r1.w + naf * z1.w;
+            P4[c] = p0;
+            R4[c] = r0;
+            P4[c + 1] = p1;
+            R4[c + 1] = r1;
+            v[0] += (double)r0.x * (double)r0.x + (double)r0.y * (double)r0.y +
+                    (double)r0.z * (double)r0.z + (double)r0.w * (double)r0.w;
+            v[0] += (double)r1.x * (double)r1.x + (double)r1.y * (double)r1.y +
+                    (double)r1.z * (double)r1.z + (double)r1.w * (double)r1.w;
+        }
+        block_partials<1>(v, red.partials, blockIdx.x);
     }
-    block_partials<1>(v, red.partials, blockIdx.x);
-}
+};
 
 // Reliable update, part 1: x64 += p, p = 0 (every refine_every iterations).
-__global__ void k_flush(const uint8_t* __restrict__ A, float* __restrict__ P,
-                        double* __restrict__ P64, const int32_t* __restrict__ segs, int64_t nseg,
-                        const SolverScalars* sc) {
-    if (sc->done || !refine_due(sc)) return;
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-         u += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c0 = 8 * (int64_t)segs[u];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const int64_t c = c0 + m;
-            if (A[c] == kFluid) P64[c] += (double)P[c];
-            P[c] = 0.0f;
+struct KFlush {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const uint8_t* A;
+        float* P;
+        double* P64;
+        const int32_t* segs;
+        int64_t nseg;
+        const SolverScalars* sc;
+    };
+    static __device__ void run(const Args& kargs) {
+        const uint8_t* __restrict__ A = kargs.A;
+        float* __restrict__ P = kargs.P;
+        double* __restrict__ P64 = kargs.P64;
+        const int32_t* __restrict__ segs = kargs.segs;
+        int64_t nseg = kargs.nseg;
+        const SolverScalars* sc = kargs.sc;
+
+        if (sc->done || !refine_due(sc)) return;
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+             u += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t c0 = 8 * (int64_t)segs[u];
+    #pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int64_t c = c0 + m;
+                if (A[c] == kFluid) P64[c] += (double)P[c];
+                P[c] = 0.0f;
+            }
         }
     }
-}
+};
 
 // mode 0 (reliable update, part 2): r = b - A x64 in FP64, stored in FP32, then the
 // stop test. mode 1 (report): ||b - A p||^2 of the FP32 pressure into sc->tmp.
-__global__ void k_true_residual(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
-                                const float* __restrict__ P, const double* __restrict__ P64,
-                                const float* __restrict__ D, float* __restrict__ R,
-                                const int32_t* __restrict__ segs, int64_t nseg, Reduce red,
-                                int mode) {
-    if (mode == 0 && (red.sc->done || !refine_due(red.sc))) return;
-    const int N = p.N;
-    double v[1] = {0.0};
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-         u += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c0 = 8 * (int64_t)segs[u];
-        for (int m = 0; m < 8; ++m) {
-            const int64_t c = c0 + m;
-            if (A[c] != kFluid) {
-                if (mode == 0) R[c] = 0.0f;
-                continue;
+struct KTrueResidual {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const uint8_t* A;
+        const float* L;
+        const float* P;
+        const double* P64;
+        const float* D;
+        float* R;
+        const int32_t* segs;
+        int64_t nseg;
+        Reduce red;
+        int mode;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const uint8_t* __restrict__ A = kargs.A;
+        const float* __restrict__ L = kargs.L;
+        const float* __restrict__ P = kargs.P;
+        const double* __restrict__ P64 = kargs.P64;
+        const float* __restrict__ D = kargs.D;
+        float* __restrict__ R = kargs.R;
+        const int32_t* __restrict__ segs = kargs.segs;
+        int64_t nseg = kargs.nseg;
+        const Reduce& red = kargs.red;
+        int mode = kargs.mode;
+
+        if (mode == 0 && (red.sc->done || !refine_due(red.sc))) return;
+        const int N = p.N;
+        double v[1] = {0.0};
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+             u += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t c0 = 8 * (int64_t)segs[u];
+            for (int m = 0; m < 8; ++m) {
+                const int64_t c = c0 + m;
+                if (A[c] != kFluid) {
+                    if (mode == 0) R[c] = 0.0f;
+                    continue;
+                }
+                int i, j, k;
+                decode_cell(N, c, i, j, k);
+                const double ax = mode == 0 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
+                                            : spmv_cell_f64(A, L, P, N, i, j, k, c);
+                const double r = (double)D[c] - ax;
+                v[0] += r * r;
+                if (mode == 0) R[c] = (float)r;
             }
-            int i, j, k;
-            decode_cell(N, c, i, j, k);
-            const double ax = mode == 0 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
-                                        : spmv_cell_f64(A, L, P, N, i, j, k, c);
-            const double r = (double)D[c] - ax;
-            v[0] += r * r;
-            if (mode == 0) R[c] = (float)r;
         }
+        block_partials<1>(v, red.partials, blockIdx.x);
     }
-    block_partials<1>(v, red.partials, blockIdx.x);
-}
+};
 
 // End of a refined solve: p = (float)(x64 + p) on FLUID cells, 0 elsewhere.
-__global__ void k_refine_finish(const uint8_t* __restrict__ A, float* __restrict__ P,
-                                const double* __restrict__ P64, const int32_t* __restrict__ segs,
-                                int64_t nseg) {
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-         u += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c0 = 8 * (int64_t)segs[u];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const int64_t c = c0 + m;
-            P[c] = A[c] == kFluid ? (float)(P64[c] + (double)P[c]) : 0.0f;
-        }
-    }
-}
+struct KRefineFinish {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const uint8_t* A;
+        float* P;
+        const double* P64;
+        const int32_t* segs;
+        int64_t nseg;
+    };
+    static __device__ void run(const Args& kargs) {
+        const uint8_t* __restrict__ A = kargs.A;
+        float* __restrict__ P = kargs.P;
+        const double* __restrict__ P64 = kargs.P64;
+        const int32_t* __restrict__ segs = kargs.segs;
+        int64_t nseg = kargs.nseg;
 
-__global__ void k_update_s(const float* __restrict__ Z, float* __restrict__ S,
-                           const int32_t* __restrict__ segs, int64_t nseg, const SolverScalars* sc,
-                           int first) {
-    if (sc->done) return;
-    const float bf = (float)sc->beta;
-    const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
-    float4* __restrict__ S4 = reinterpret_cast<float4*>(S);
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-         u += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = 2 * (int64_t)segs[u];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const float4 z = Z4[c + h];
-            if (first) {
-                S4[c + h] = z;
-            } else {
-                float4 sv = S4[c + h];
-                sv.x = z.x + bf * sv.x; sv.y = z.y + bf * sv.y; sv.z = z.z + bf * sv.z; sv.w = z.w + bf * sv.w;
-                S4[c + h] = sv;
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+             u += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t c0 = 8 * (int64_t)segs[u];
+    #pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int64_t c = c0 + m;
+                P[c] = A[c] == kFluid ? (float)(P64[c] + (double)P[c]) : 0.0f;
             }
         }
     }
-}
+};
 
-__global__ void k_spmv_plain(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
-                             const float* __restrict__ x, float* __restrict__ y) {
-    const int N = p.N;
-    const int64_t nc = (int64_t)N * N * N;
-    const float n = (float)N;
-    const float h = 1.0f / (n * n);
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    if (A[c] != kFluid) {
-        y[c] = 0.0f;
-        return;
+struct KUpdateS {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const float* Z;
+        float* S;
+        const int32_t* segs;
+        int64_t nseg;
+        const SolverScalars* sc;
+        int first;
+    };
+    static __device__ void run(const Args& kargs) {
+        const float* __restrict__ Z = kargs.Z;
+        float* __restrict__ S = kargs.S;
+        const int32_t* __restrict__ segs = kargs.segs;
+        int64_t nseg = kargs.nseg;
+        const SolverScalars* sc = kargs.sc;
+        int first = kargs.first;
+
+        if (sc->done) return;
+        const float bf = (float)sc->beta;
+        const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
+        float4* __restrict__ S4 = reinterpret_cast<float4*>(S);
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+             u += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t c = 2 * (int64_t)segs[u];
+    #pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float4 z = Z4[c + h];
+                if (first) {
+                    S4[c + h] = z;
+                } else {
+                    float4 sv = S4[c + h];
+                    sv.x = z.x + bf * sv.x; sv.y = z.y + bf * sv.y; sv.z = z.z + bf * sv.z; sv.w = z.w + bf * sv.w;
+                    S4[c + h] = sv;
+                }
+            }
+        }
     }
-    int i, j, k;
-    decode_cell(N, c, i, j, k);
-    y[c] = spmv_cell(A, L, x, N, i, j, k, c, h);
-}
+};
+
+struct KSpmvPlain {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const uint8_t* A;
+        const float* L;
+        const float* x;
+        float* y;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const uint8_t* __restrict__ A = kargs.A;
+        const float* __restrict__ L = kargs.L;
+        const float* __restrict__ x = kargs.x;
+        float* __restrict__ y = kargs.y;
+
+        const int N = p.N;
+        const int64_t nc = (int64_t)N * N * N;
+        const float n = (float)N;
+        const float h = 1.0f / (n * n);
+        int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= nc) return;
+        if (A[c] != kFluid) {
+            y[c] = 0.0f;
+            return;
+        }
+        int i, j, k;
+        decode_cell(N, c, i, j, k);
+        y[c] = spmv_cell(A, L, x, N, i, j, k, c, h);
+    }
+};
 
 // rho_new = z.r finalisation shared by the preconditioner kernels.
 __device__ __forceinline__ void finish_rho(SolverScalars* sc, double rho_new) {
@@ -470,117 +629,150 @@ enum { FIN_RESID0 = 0, FIN_SIGMA = 1, FIN_RNORM = 2, FIN_TRUE = 3, FIN_REPORT = 
 // One block: sum the partials of the preceding reducing kernel in a fixed order
 // and update the scalars. Gated exactly like its producer, so a skipped producer
 // (solve finished) is matched by a skipped finalize.
-__global__ __launch_bounds__(256) void k_finalize(Reduce red, int kind, int nparts, double tol,
-                                                  int stop_rule, int N) {
-    SolverScalars* sc = red.sc;
-    if ((kind == FIN_SIGMA || kind == FIN_RNORM || kind == FIN_RHO) && sc->done) return;
-    if (kind == FIN_TRUE && (sc->done || !refine_due(sc))) return;
-    const int NV = kind == FIN_RESID0 ? 2 : 1;
-    double a0 = 0.0, a1 = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-        a0 += red.partials[b * NV];
-        if (NV == 2) a1 += red.partials[b * NV + 1];
+struct KFinalize {
+    static constexpr int kThreads = 256;
+    struct Args {
+        Reduce red;
+        int kind;
+        int nparts;
+        double tol;
+        int stop_rule;
+        int N;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Reduce& red = kargs.red;
+        int kind = kargs.kind;
+        int nparts = kargs.nparts;
+        double tol = kargs.tol;
+        int stop_rule = kargs.stop_rule;
+        int N = kargs.N;
+
+        SolverScalars* sc = red.sc;
+        if ((kind == FIN_SIGMA || kind == FIN_RNORM || kind == FIN_RHO) && sc->done) return;
+        if (kind == FIN_TRUE && (sc->done || !refine_due(sc))) return;
+        const int NV = kind == FIN_RESID0 ? 2 : 1;
+        double a0 = 0.0, a1 = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+            a0 += red.partials[b * NV];
+            if (NV == 2) a1 += red.partials[b * NV + 1];
+        }
+        __shared__ double sm[2][4];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        if (lane == 0) {
+            sm[0][w] = a0;
+            sm[1][w] = a1;
+        }
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            s0 += sm[0][i];
+            s1 += sm[1][i];
+        }
+        switch (kind) {
+            case FIN_RESID0:
+                sc->bnorm2 = s0;
+                sc->rnorm2 = s1;
+                if (stop_rule == 1) sc->thr = (double)(1.0e-2f * (float)(N * N * N));
+                else sc->thr = tol * tol * s0;
+                if (s1 <= sc->thr) {
+                    sc->done = 1;
+                    sc->status = 0;
+                }
+                break;
+            case FIN_SIGMA:
+                sc->sigma = s0;
+                if (!finite(s0) || !finite(sc->rho)) {
+                    sc->status = 3;
+                    sc->done = 1;
+                } else if (!(s0 > 0.0)) {
+                    sc->status = 2;
+                    sc->done = 1;
+                } else {
+                    sc->alpha = sc->rho / s0;
+                }
+                break;
+            case FIN_RNORM:
+                sc->rnorm2 = s0;
+                sc->it += 1;
+                if (!refine_due(sc)) decide(sc);
+                break;
+            case FIN_TRUE:
+                sc->rnorm2 = s0;
+                decide(sc);
+                break;
+            case FIN_REPORT:
+                sc->tmp = s0;
+                break;
+            case FIN_RHO:
+                finish_rho(sc, s0);
+                break;
+        }
     }
-    __shared__ double sm[2][4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    a0 = wave_sum(a0);
-    a1 = wave_sum(a1);
-    if (lane == 0) {
-        sm[0][w] = a0;
-        sm[1][w] = a1;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double s0 = 0.0, s1 = 0.0;
-    for (int i = 0; i < 4; ++i) {
-        s0 += sm[0][i];
-        s1 += sm[1][i];
-    }
-    switch (kind) {
-        case FIN_RESID0:
-            sc->bnorm2 = s0;
-            sc->rnorm2 = s1;
-            if (stop_rule == 1) sc->thr = (double)(1.0e-2f * (float)(N * N * N));
-            else sc->thr = tol * tol * s0;
-            if (s1 <= sc->thr) {
-                sc->done = 1;
-                sc->status = 0;
-            }
-            break;
-        case FIN_SIGMA:
-            sc->sigma = s0;
-            if (!finite(s0) || !finite(sc->rho)) {
-                sc->status = 3;
-                sc->done = 1;
-            } else if (!(s0 > 0.0)) {
-                sc->status = 2;
-                sc->done = 1;
-            } else {
-                sc->alpha = sc->rho / s0;
-            }
-            break;
-        case FIN_RNORM:
-            sc->rnorm2 = s0;
-            sc->it += 1;
-            if (!refine_due(sc)) decide(sc);
-            break;
-        case FIN_TRUE:
-            sc->rnorm2 = s0;
-            decide(sc);
-            break;
-        case FIN_REPORT:
-            sc->tmp = s0;
-            break;
-        case FIN_RHO:
-            finish_rho(sc, s0);
-            break;
-    }
-}
+};
 
 // ------------------------------------------------------------ preconditioner
 // ADiag (solver.inl:81-101) and the number of FLUID upper neighbours under the
 // ordering (mode 0: tiled red-black, 4: block/no coupling); Jacobi inverse diag.
-__global__ void k_precond_setup(Params p, const uint8_t* __restrict__ A, const float* __restrict__ L,
-                                float* __restrict__ diag, uint8_t* __restrict__ nup,
-                                float* __restrict__ pc) {
-    const int N = p.N;
-    const int64_t nc = (int64_t)N * N * N;
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    int i, j, k;
-    decode_cell(N, c, i, j, k);
-    if (A[c] != kFluid) {
-        diag[c] = 6.0f;
-        nup[c] = 0;
-        pc[c] = 0.0f;
-        return;
-    }
-    float dg = 6.0f;
-    const int q[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
-                         {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
-    int cnt = 0;
-    const int color = ((i >> 3) + (j >> 3) + (k >> 3)) & 1;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        const int a = q[m][0], b = q[m][1], cc = q[m][2];
-        const bool in = a >= 0 && a < N && b >= 0 && b < N && cc >= 0 && cc < N;
-        const uint8_t t = in ? A[cidx(N, a, b, cc)] : (uint8_t)kSolid;
-        if (!in || t == kSolid) {
-            dg -= 1.0f;
-        } else if (t == kAir) {
-            dg -= L[cidx(N, a, b, cc)] / gmin(1.0e-6f, L[c]);
+struct KPrecondSetup {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        Params p;
+        const uint8_t* A;
+        const float* L;
+        float* diag;
+        uint8_t* nup;
+        float* pc;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const uint8_t* __restrict__ A = kargs.A;
+        const float* __restrict__ L = kargs.L;
+        float* __restrict__ diag = kargs.diag;
+        uint8_t* __restrict__ nup = kargs.nup;
+        float* __restrict__ pc = kargs.pc;
+
+        const int N = p.N;
+        const int64_t nc = (int64_t)N * N * N;
+        int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= nc) return;
+        int i, j, k;
+        decode_cell(N, c, i, j, k);
+        if (A[c] != kFluid) {
+            diag[c] = 6.0f;
+            nup[c] = 0;
+            pc[c] = 0.0f;
+            return;
         }
-        if (in && t == kFluid) {
-            const bool same = (a >> 3) == (i >> 3) && (b >> 3) == (j >> 3) && (cc >> 3) == (k >> 3);
-            const bool plus = (m & 1) != 0;
-            if (same) cnt += plus ? 1 : 0;
-            else if (p.precond == 0 && color == 0) cnt += 1;
+        float dg = 6.0f;
+        const int q[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                             {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+        int cnt = 0;
+        const int color = ((i >> 3) + (j >> 3) + (k >> 3)) & 1;
+    #pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const int a = q[m][0], b = q[m][1], cc = q[m][2];
+            const bool in = a >= 0 && a < N && b >= 0 && b < N && cc >= 0 && cc < N;
+            const uint8_t t = in ? A[cidx(N, a, b, cc)] : (uint8_t)kSolid;
+            if (!in || t == kSolid) {
+                dg -= 1.0f;
+            } else if (t == kAir) {
+                dg -= L[cidx(N, a, b, cc)] / gmin(1.0e-6f, L[c]);
+            }
+            if (in && t == kFluid) {
+                const bool same = (a >> 3) == (i >> 3) && (b >> 3) == (j >> 3) && (cc >> 3) == (k >> 3);
+                const bool plus = (m & 1) != 0;
+                if (same) cnt += plus ? 1 : 0;
+                else if (p.precond == 0 && color == 0) cnt += 1;
+            }
         }
+        diag[c] = dg;
+        nup[c] = (uint8_t)cnt;
+        pc[c] = (p.precond == 2 && dg > 0) ? 1.0f / dg : 0.0f;
     }
-    diag[c] = dg;
-    nup[c] = (uint8_t)cnt;
-    pc[c] = (p.precond == 2 && dg > 0) ? 1.0f / dg : 0.0f;
-}
+};
 
 struct TileCtx {
     int N, TN;
@@ -634,152 +826,181 @@ __device__ __forceinline__ uint32_t load_halo_line(const uint8_t* __restrict__ A
 // pivot: e = diag - sum_j pc_j^2 - tau * sum_j (nup_j - 1) pc_j^2 over FLUID lower
 // neighbours j in the order -x,-y,-z,+x,+y,+z; e = diag if e < sigma*diag;
 // pc = 1/sqrt(e).
-__global__ __launch_bounds__(64) void k_mic_build(Params p, const uint8_t* __restrict__ A,
-                                                   const float* __restrict__ diag,
-                                                   const uint8_t* __restrict__ nup,
-                                                   float* __restrict__ pc, int color,
-                                                   const int32_t* __restrict__ tiles,
-                                                   int64_t ntl) {
-    __shared__ float s_pc[64 * kPad];
-    __shared__ float s_nm[64 * kPad];
-    __shared__ float s_hxp[64 * kPad];   // cross-tile x neighbour line: pc
-    __shared__ float s_hxn[64 * kPad];   //                              nup-1
-    __shared__ float s_hyp[64 * kPad];
-    __shared__ float s_hyn[64 * kPad];
-    const int N = p.N;
-    const int TN = N / kTile;
-    (void)TN;
-    const bool black = color == 1;
-    const float tau = p.mic_tau, sigma = p.mic_sigma;
-    for (int64_t u = blockIdx.x; u < ntl; u += gridDim.x) {
-        const int64_t t = tiles[u];
-        TileCtx c;
-        if (!tile_of(N, t, color, c)) continue;
-        const uint32_t fl = fluid_bits(A, c.line);
-        const uint32_t fl_im = __shfl(fl, (c.lane - 8) & 63, 64);
-        const uint32_t fl_jm = __shfl(fl, (c.lane - 1) & 63, 64);
-        float dg[kTile];
-#pragma unroll
-        for (int kl = 0; kl < kTile; ++kl) dg[kl] = diag[c.line + kl];
-        // halos (black tiles): x line (il==0: -x, il==7: +x), y line, z singles
-        uint32_t hx_fl = 0, hy_fl = 0, hzm_fl = 0, hzp_fl = 0;
-        float hzm_pc = 0.f, hzm_nm = 0.f, hzp_pc = 0.f, hzp_nm = 0.f;
-        if (black) {
-            int xi = -1;
-            if (c.il == 0 && c.gi > 0) xi = c.gi - 1;
-            if (c.il == 7 && c.gi + 1 < N) xi = c.gi + 1;
-            if (xi >= 0) {
-                const int64_t ln = cidx(N, xi, c.gj, c.k0);
-                hx_fl = fluid_bits(A, ln);
-#pragma unroll
-                for (int kl = 0; kl < kTile; ++kl) {
-                    s_hxp[c.lane * kPad + kl] = pc[ln + kl];
-                    s_hxn[c.lane * kPad + kl] = (float)((int)nup[ln + kl] - 1);
-                }
-            }
-            int yj = -1;
-            if (c.jl == 0 && c.gj > 0) yj = c.gj - 1;
-            if (c.jl == 7 && c.gj + 1 < N) yj = c.gj + 1;
-            if (yj >= 0) {
-                const int64_t ln = cidx(N, c.gi, yj, c.k0);
-                hy_fl = fluid_bits(A, ln);
-#pragma unroll
-                for (int kl = 0; kl < kTile; ++kl) {
-                    s_hyp[c.lane * kPad + kl] = pc[ln + kl];
-                    s_hyn[c.lane * kPad + kl] = (float)((int)nup[ln + kl] - 1);
-                }
-            }
-            if (c.k0 > 0) {
-                const int64_t n = c.line - 1;
-                hzm_fl = A[n] == kFluid;
-                hzm_pc = pc[n];
-                hzm_nm = (float)((int)nup[n] - 1);
-            }
-            if (c.k0 + kTile < N) {
-                const int64_t n = c.line + kTile;
-                hzp_fl = A[n] == kFluid;
-                hzp_pc = pc[n];
-                hzp_nm = (float)((int)nup[n] - 1);
-            }
-        }
-#pragma unroll
-        for (int kl = 0; kl < kTile; ++kl) s_nm[c.lane * kPad + kl] = (float)((int)nup[c.line + kl] - 1);
-        __syncthreads();
-        for (int st = 0; st < 3 * kTile - 2; ++st) {
-            const int kl = st - c.il - c.jl;
-            if (kl >= 0 && kl < kTile) {
-                float res = 0.0f;
-                if ((fl >> kl) & 1u) {
-                    // gather the lower neighbours' (pc, nup-1) in order -x,-y,-z,+x,+y,+z
-                    float pj[6], nm[6];
-                    bool has[6];
-                    // -x
-                    has[0] = false;
-                    if (c.il > 0) {
-                        if ((fl_im >> kl) & 1u) { has[0] = true; pj[0] = s_pc[(c.lane - 8) * kPad + kl]; nm[0] = s_nm[(c.lane - 8) * kPad + kl]; }
-                    } else if (black && ((hx_fl >> kl) & 1u)) {
-                        has[0] = true; pj[0] = s_hxp[c.lane * kPad + kl]; nm[0] = s_hxn[c.lane * kPad + kl];
-                    }
-                    // -y
-                    has[1] = false;
-                    if (c.jl > 0) {
-                        if ((fl_jm >> kl) & 1u) { has[1] = true; pj[1] = s_pc[(c.lane - 1) * kPad + kl]; nm[1] = s_nm[(c.lane - 1) * kPad + kl]; }
-                    } else if (black && ((hy_fl >> kl) & 1u)) {
-                        has[1] = true; pj[1] = s_hyp[c.lane * kPad + kl]; nm[1] = s_hyn[c.lane * kPad + kl];
-                    }
-                    // -z
-                    has[2] = false;
-                    if (kl > 0) {
-                        if ((fl >> (kl - 1)) & 1u) { has[2] = true; pj[2] = s_pc[c.lane * kPad + kl - 1]; nm[2] = s_nm[c.lane * kPad + kl - 1]; }
-                    } else if (black && hzm_fl) {
-                        has[2] = true; pj[2] = hzm_pc; nm[2] = hzm_nm;
-                    }
-                    // +x, +y, +z (cross-tile, black only)
-                    has[3] = black && c.il == 7 && ((hx_fl >> kl) & 1u);
-                    if (has[3]) { pj[3] = s_hxp[c.lane * kPad + kl]; nm[3] = s_hxn[c.lane * kPad + kl]; }
-                    has[4] = black && c.jl == 7 && ((hy_fl >> kl) & 1u);
-                    if (has[4]) { pj[4] = s_hyp[c.lane * kPad + kl]; nm[4] = s_hyn[c.lane * kPad + kl]; }
-                    has[5] = black && kl == 7 && hzp_fl;
-                    if (has[5]) { pj[5] = hzp_pc; nm[5] = hzp_nm; }
-                    const float d = dg[kl];
-                    float e = d;
-#pragma unroll
-                    for (int m = 0; m < 6; ++m)
-                        if (has[m]) e = e - pj[m] * pj[m];
-                    float ssum = 0.0f;
-#pragma unroll
-                    for (int m = 0; m < 6; ++m)
-                        if (has[m]) ssum = ssum + nm[m] * (pj[m] * pj[m]);
-                    e = e - tau * ssum;
-                    if (d > 0) {
-                        if (e < sigma * d) e = d;
-                        res = 1.0f / sqrtf(e);
+struct KMicBuild {
+    static constexpr int kThreads = 64;
+    struct Args {
+        Params p;
+        const uint8_t* A;
+        const float* diag;
+        const uint8_t* nup;
+        float* pc;
+        int color;
+        const int32_t* tiles;
+        int64_t ntl;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const uint8_t* __restrict__ A = kargs.A;
+        const float* __restrict__ diag = kargs.diag;
+        const uint8_t* __restrict__ nup = kargs.nup;
+        float* __restrict__ pc = kargs.pc;
+        int color = kargs.color;
+        const int32_t* __restrict__ tiles = kargs.tiles;
+        int64_t ntl = kargs.ntl;
+
+        __shared__ float s_pc[64 * kPad];
+        __shared__ float s_nm[64 * kPad];
+        __shared__ float s_hxp[64 * kPad];   // cross-tile x neighbour line: pc
+        __shared__ float s_hxn[64 * kPad];   //                              nup-1
+        __shared__ float s_hyp[64 * kPad];
+        __shared__ float s_hyn[64 * kPad];
+        const int N = p.N;
+        const int TN = N / kTile;
+        (void)TN;
+        const bool black = color == 1;
+        const float tau = p.mic_tau, sigma = p.mic_sigma;
+        for (int64_t u = blockIdx.x; u < ntl; u += gridDim.x) {
+            const int64_t t = tiles[u];
+            TileCtx c;
+            if (!tile_of(N, t, color, c)) continue;
+            const uint32_t fl = fluid_bits(A, c.line);
+            const uint32_t fl_im = __shfl(fl, (c.lane - 8) & 63, 64);
+            const uint32_t fl_jm = __shfl(fl, (c.lane - 1) & 63, 64);
+            float dg[kTile];
+    #pragma unroll
+            for (int kl = 0; kl < kTile; ++kl) dg[kl] = diag[c.line + kl];
+            // halos (black tiles): x line (il==0: -x, il==7: +x), y line, z singles
+            uint32_t hx_fl = 0, hy_fl = 0, hzm_fl = 0, hzp_fl = 0;
+            float hzm_pc = 0.f, hzm_nm = 0.f, hzp_pc = 0.f, hzp_nm = 0.f;
+            if (black) {
+                int xi = -1;
+                if (c.il == 0 && c.gi > 0) xi = c.gi - 1;
+                if (c.il == 7 && c.gi + 1 < N) xi = c.gi + 1;
+                if (xi >= 0) {
+                    const int64_t ln = cidx(N, xi, c.gj, c.k0);
+                    hx_fl = fluid_bits(A, ln);
+    #pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) {
+                        s_hxp[c.lane * kPad + kl] = pc[ln + kl];
+                        s_hxn[c.lane * kPad + kl] = (float)((int)nup[ln + kl] - 1);
                     }
                 }
-                s_pc[c.lane * kPad + kl] = res;
+                int yj = -1;
+                if (c.jl == 0 && c.gj > 0) yj = c.gj - 1;
+                if (c.jl == 7 && c.gj + 1 < N) yj = c.gj + 1;
+                if (yj >= 0) {
+                    const int64_t ln = cidx(N, c.gi, yj, c.k0);
+                    hy_fl = fluid_bits(A, ln);
+    #pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) {
+                        s_hyp[c.lane * kPad + kl] = pc[ln + kl];
+                        s_hyn[c.lane * kPad + kl] = (float)((int)nup[ln + kl] - 1);
+                    }
+                }
+                if (c.k0 > 0) {
+                    const int64_t n = c.line - 1;
+                    hzm_fl = A[n] == kFluid;
+                    hzm_pc = pc[n];
+                    hzm_nm = (float)((int)nup[n] - 1);
+                }
+                if (c.k0 + kTile < N) {
+                    const int64_t n = c.line + kTile;
+                    hzp_fl = A[n] == kFluid;
+                    hzp_pc = pc[n];
+                    hzp_nm = (float)((int)nup[n] - 1);
+                }
             }
+    #pragma unroll
+            for (int kl = 0; kl < kTile; ++kl) s_nm[c.lane * kPad + kl] = (float)((int)nup[c.line + kl] - 1);
+            __syncthreads();
+            for (int st = 0; st < 3 * kTile - 2; ++st) {
+                const int kl = st - c.il - c.jl;
+                if (kl >= 0 && kl < kTile) {
+                    float res = 0.0f;
+                    if ((fl >> kl) & 1u) {
+                        // gather the lower neighbours' (pc, nup-1) in order -x,-y,-z,+x,+y,+z
+                        float pj[6], nm[6];
+                        bool has[6];
+                        // -x
+                        has[0] = false;
+                        if (c.il > 0) {
+                            if ((fl_im >> kl) & 1u) { has[0] = true; pj[0] = s_pc[(c.lane - 8) * kPad + kl]; nm[0] = s_nm[(c.lane - 8) * kPad + kl]; }
+                        } else if (black && ((hx_fl >> kl) & 1u)) {
+                            has[0] = true; pj[0] = s_hxp[c.lane * kPad + kl]; nm[0] = s_hxn[c.lane * kPad + kl];
+                        }
+                        // -y
+                        has[1] = false;
+                        if (c.jl > 0) {
+                            if ((fl_jm >> kl) & 1u) { has[1] = true; pj[1] = s_pc[(c.lane - 1) * kPad + kl]; nm[1] = s_nm[(c.lane - 1) * kPad + kl]; }
+                        } else if (black && ((hy_fl >> kl) & 1u)) {
+                            has[1] = true; pj[1] = s_hyp[c.lane * kPad + kl]; nm[1] = s_hyn[c.lane * kPad + kl];
+                        }
+                        // -z
+                        has[2] = false;
+                        if (kl > 0) {
+                            if ((fl >> (kl - 1)) & 1u) { has[2] = true; pj[2] = s_pc[c.lane * kPad + kl - 1]; nm[2] = s_nm[c.lane * kPad + kl - 1]; }
+                        } else if (black && hzm_fl) {
+                            has[2] = true; pj[2] = hzm_pc; nm[2] = hzm_nm;
+                        }
+                        // +x, +y, +z (cross-tile, black only)
+                        has[3] = black && c.il == 7 && ((hx_fl >> kl) & 1u);
+                        if (has[3]) { pj[3] = s_hxp[c.lane * kPad + kl]; nm[3] = s_hxn[c.lane * kPad + kl]; }
+                        has[4] = black && c.jl == 7 && ((hy_fl >> kl) & 1u);
+                        if (has[4]) { pj[4] = s_hyp[c.lane * kPad + kl]; nm[4] = s_hyn[c.lane * kPad + kl]; }
+                        has[5] = black && kl == 7 && hzp_fl;
+                        if (has[5]) { pj[5] = hzp_pc; nm[5] = hzp_nm; }
+                        const float d = dg[kl];
+                        float e = d;
+    #pragma unroll
+                        for (int m = 0; m < 6; ++m)
+                            if (has[m]) e = e - pj[m] * pj[m];
+                        float ssum = 0.0f;
+    #pragma unroll
+                        for (int m = 0; m < 6; ++m)
+                            if (has[m]) ssum = ssum + nm[m] * (pj[m] * pj[m]);
+                        e = e - tau * ssum;
+                        if (d > 0) {
+                            if (e < sigma * d) e = d;
+                            res = 1.0f / sqrtf(e);
+                        }
+                    }
+                    s_pc[c.lane * kPad + kl] = res;
+                }
+                __syncthreads();
+            }
+    #pragma unroll
+            for (int kl = 0; kl < kTile; ++kl) pc[c.line + kl] = s_pc[c.lane * kPad + kl];
             __syncthreads();
         }
-#pragma unroll
-        for (int kl = 0; kl < kTile; ++kl) pc[c.line + kl] = s_pc[c.lane * kPad + kl];
-        __syncthreads();
     }
-}
+};
 
 // Fluid bits of every z-segment of 8 cells: fl8[(i*N + j)*(N/8) + k/8], bit k%8.
-__global__ void k_fluid_segments(int N, const uint8_t* __restrict__ A, uint8_t* __restrict__ fl8) {
-    const int64_t nseg = (int64_t)N * N * (N / kTile);
-    int64_t sgi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (sgi >= nseg) return;
-    const uint2 v = *reinterpret_cast<const uint2*>(A + sgi * kTile);
-    uint32_t m = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        m |= (((v.x >> (8 * b)) & 0xFFu) == (uint32_t)kFluid ? 1u : 0u) << b;
-        m |= (((v.y >> (8 * b)) & 0xFFu) == (uint32_t)kFluid ? 1u : 0u) << (b + 4);
+struct KFluidSegments {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int N;
+        const uint8_t* A;
+        uint8_t* fl8;
+    };
+    static __device__ void run(const Args& kargs) {
+        int N = kargs.N;
+        const uint8_t* __restrict__ A = kargs.A;
+        uint8_t* __restrict__ fl8 = kargs.fl8;
+
+        const int64_t nseg = (int64_t)N * N * (N / kTile);
+        int64_t sgi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (sgi >= nseg) return;
+        const uint2 v = *reinterpret_cast<const uint2*>(A + sgi * kTile);
+        uint32_t m = 0;
+    #pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            m |= (((v.x >> (8 * b)) & 0xFFu) == (uint32_t)kFluid ? 1u : 0u) << b;
+            m |= (((v.y >> (8 * b)) & 0xFFu) == (uint32_t)kFluid ? 1u : 0u) << (b + 4);
+        }
+        fl8[sgi] = (uint8_t)m;
     }
-    fl8[sgi] = (uint8_t)m;
-}
+};
 
 constexpr int kTilesPerBlock = 4;
 constexpr int kLine = 64 * kPad;          // floats of one per-line LDS array
@@ -819,309 +1040,396 @@ constexpr int kWaveFloats = 3 * kLine + 2 * kHalo + kTile;
 constexpr int kGuardHi = 32;                          // >= 21 past the last region
 
 template <int PH>
-__global__ __launch_bounds__(256) void k_mic_apply(Params p, const uint8_t* __restrict__ fl8,
-                                                    const float* __restrict__ pc,
-                                                    const float* __restrict__ r,
-                                                    float* __restrict__ Q, float* __restrict__ Z,
-                                                    const int32_t* __restrict__ tiles,
-                                                    int64_t ntiles, Reduce red, int slot_base) {
-    constexpr bool do_fwd = PH != 2;
-    constexpr bool do_bwd = PH != 0;
-    constexpr bool halo_fwd = PH == 1;
-    constexpr bool halo_bwd = PH == 2;
-    constexpr bool halo = halo_fwd || halo_bwd;
-    __shared__ float lds[kGuardLo + kTilesPerBlock * kWaveFloats + kGuardHi];
-    const int N = p.N;
-    const int TN = N / kTile;
-    const int NS = N / kTile;   // segments per z-line
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // float offsets of this wave's arrays inside lds[]
-    const int o_sa = kGuardLo + wave * kWaveFloats;   // r / q, then q / z      [line*9 + kl]
-    const int o_pc = o_sa + kLine;                    // pc
-    const int o_sb = o_pc + kLine;                    // pc*q (forward), z (backward); -0 off-fluid
-    const int o_hx = o_sb + kLine;                    // x halo [(il==7)*72 + jl*9 + kl]
-    const int o_hy = o_hx + kHalo;                    // y halo [(jl==7)*72 + il*9 + kl]
-    const int o_zero = o_hy + kHalo;                  // 8 x -0.0
-    int ti = 0, tj = 0, tk = 0;
-    const int64_t idx = (int64_t)blockIdx.x * kTilesPerBlock + wave;
-    const bool active = idx < ntiles;
-    // the tile index and the stop flag are fetched together
-    const int t = active ? tiles[idx] : 0;
-    if (red.sc->done) return;
-    if (active) {
-        tk = t % TN;
-        tj = (t / TN) % TN;
-        ti = t / (TN * TN);
-    }
-    const int il = lane >> 3, jl = lane & 7;
-    const int gi = ti * kTile + il, gj = tj * kTile + jl, k0 = tk * kTile;
-    uint32_t fl = 0;
-    float hzm = -0.0f, hzp = -0.0f;   // z-halo terms (-0.0 when absent)
-    float rreg[kTile];
-    // coalesced mapping: round m covers line (m, lane>>3), element lane&7
-    const int ce = lane & 7, cj = lane >> 3;
-    if (lane < kTile) lds[o_zero + lane] = -0.0f;
-    if (active) {
-        fl = fl8[((int64_t)gi * N + gj) * NS + tk];
-#pragma unroll
-        for (int m = 0; m < kTile; ++m) {
-            const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
-            const int li = (m * 8 + cj) * kPad + ce;
-            lds[o_pc + li] = pc[g];
-            rreg[m] = r[g];
-            lds[o_sa + li] = do_fwd ? rreg[m] : Q[g];
+struct KMicApply {
+    static constexpr int kThreads = 256;
+    struct Args {
+        Params p;
+        const uint8_t* fl8;
+        const float* pc;
+        const float* r;
+        float* Q;
+        float* Z;
+        const int32_t* tiles;
+        int64_t ntiles;
+        Reduce red;
+        int slot_base;
+    };
+    static __device__ void run(const Args& kargs) {
+        const Params& p = kargs.p;
+        const uint8_t* __restrict__ fl8 = kargs.fl8;
+        const float* __restrict__ pc = kargs.pc;
+        const float* __restrict__ r = kargs.r;
+        float* __restrict__ Q = kargs.Q;
+        float* __restrict__ Z = kargs.Z;
+        const int32_t* __restrict__ tiles = kargs.tiles;
+        int64_t ntiles = kargs.ntiles;
+        const Reduce& red = kargs.red;
+        int slot_base = kargs.slot_base;
+
+        constexpr bool do_fwd = PH != 2;
+        constexpr bool do_bwd = PH != 0;
+        constexpr bool halo_fwd = PH == 1;
+        constexpr bool halo_bwd = PH == 2;
+        constexpr bool halo = halo_fwd || halo_bwd;
+        __shared__ float lds[kGuardLo + kTilesPerBlock * kWaveFloats + kGuardHi];
+        const int N = p.N;
+        const int TN = N / kTile;
+        const int NS = N / kTile;   // segments per z-line
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        // float offsets of this wave's arrays inside lds[]
+        const int o_sa = kGuardLo + wave * kWaveFloats;   // r / q, then q / z      [line*9 + kl]
+        const int o_pc = o_sa + kLine;                    // pc
+        const int o_sb = o_pc + kLine;                    // pc*q (forward), z (backward); -0 off-fluid
+        const int o_hx = o_sb + kLine;                    // x halo [(il==7)*72 + jl*9 + kl]
+        const int o_hy = o_hx + kHalo;                    // y halo [(jl==7)*72 + il*9 + kl]
+        const int o_zero = o_hy + kHalo;                  // 8 x -0.0
+        int ti = 0, tj = 0, tk = 0;
+        const int64_t idx = (int64_t)blockIdx.x * kTilesPerBlock + wave;
+        const bool active = idx < ntiles;
+        // the tile index and the stop flag are fetched together
+        const int t = active ? tiles[idx] : 0;
+        if (red.sc->done) return;
+        if (active) {
+            tk = t % TN;
+            tj = (t / TN) % TN;
+            ti = t / (TN * TN);
         }
-        if constexpr (halo) {
-            // neighbour lines across the tile faces: forward halos hold pc*Q of
-            // the red neighbours, backward halos their Z; -0.0 off-fluid
-            const float* va = halo_fwd ? pc : Z;
-            const float* vb = halo_fwd ? Q : nullptr;
-            // lanes on a tile face own one halo line; at the domain boundary it
-            // holds -0.0
-            if (il == 0 || il == 7) {
-                float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
-#pragma unroll
-                for (int kl = 0; kl < kTile; ++kl) d[kl] = -0.0f;
+        const int il = lane >> 3, jl = lane & 7;
+        const int gi = ti * kTile + il, gj = tj * kTile + jl, k0 = tk * kTile;
+        uint32_t fl = 0;
+        float hzm = -0.0f, hzp = -0.0f;   // z-halo terms (-0.0 when absent)
+        float rreg[kTile];
+        // coalesced mapping: round m covers line (m, lane>>3), element lane&7
+        const int ce = lane & 7, cj = lane >> 3;
+        if (lane < kTile) lds[o_zero + lane] = -0.0f;
+        if (active) {
+            fl = fl8[((int64_t)gi * N + gj) * NS + tk];
+    #pragma unroll
+            for (int m = 0; m < kTile; ++m) {
+                const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
+                const int li = (m * 8 + cj) * kPad + ce;
+                lds[o_pc + li] = pc[g];
+                rreg[m] = r[g];
+                lds[o_sa + li] = do_fwd ? rreg[m] : Q[g];
             }
-            if (jl == 0 || jl == 7) {
-                float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
-#pragma unroll
-                for (int kl = 0; kl < kTile; ++kl) d[kl] = -0.0f;
-            }
-            int xi = -1;
-            if (il == 0 && gi > 0) xi = gi - 1;
-            if (il == 7 && gi + 1 < N) xi = gi + 1;
-            if (xi >= 0) {
-                const int64_t ln = cidx(N, xi, gj, k0);
-                const uint32_t hf = fl8[((int64_t)xi * N + gj) * NS + tk];
-                float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
-                const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
-                const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
-                float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-                if (vb) {
-                    const float4 b0 = *reinterpret_cast<const float4*>(vb + ln);
-                    const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
-                    float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-#pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) av[kl] = av[kl] * bv[kl];
+            if constexpr (halo) {
+                // neighbour lines across the tile faces: forward halos hold pc*Q of
+                // the red neighbours, backward halos their Z; -0.0 off-fluid
+                const float* va = halo_fwd ? pc : Z;
+                const float* vb = halo_fwd ? Q : nullptr;
+                // lanes on a tile face own one halo line; at the domain boundary it
+                // holds -0.0
+                if (il == 0 || il == 7) {
+                    float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
+    #pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = -0.0f;
                 }
-#pragma unroll
-                for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
-            }
-            int yj = -1;
-            if (jl == 0 && gj > 0) yj = gj - 1;
-            if (jl == 7 && gj + 1 < N) yj = gj + 1;
-            if (yj >= 0) {
-                const int64_t ln = cidx(N, gi, yj, k0);
-                const uint32_t hf = fl8[((int64_t)gi * N + yj) * NS + tk];
-                float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
-                const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
-                const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
-                float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-                if (vb) {
-                    const float4 b0 = *reinterpret_cast<const float4*>(vb + ln);
-                    const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
-                    float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-#pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) av[kl] = av[kl] * bv[kl];
+                if (jl == 0 || jl == 7) {
+                    float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
+    #pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = -0.0f;
                 }
-#pragma unroll
-                for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
-            }
-            // z neighbours: values and fluid bits loaded together, masked after
-            const int64_t line = cidx(N, gi, gj, k0);
-            const int64_t sgl = ((int64_t)gi * N + gj) * NS + tk;
-            const bool has_m = k0 > 0, has_p = k0 + kTile < N;
-            const int64_t lm = has_m ? line - 1 : line, lp = has_p ? line + kTile : line;
-            const uint32_t fm = fl8[has_m ? sgl - 1 : sgl], fp = fl8[has_p ? sgl + 1 : sgl];
-            const float vm = vb ? va[lm] * vb[lm] : va[lm];
-            const float vp = vb ? va[lp] * vb[lp] : va[lp];
-            if (has_m && ((fm >> 7) & 1u)) hzm = vm;
-            if (has_p && (fp & 1u)) hzp = vp;
-        }
-    }
-    wave_sync();
-    const int sl = il + jl;
-    const int me = lane * kPad;
-    const uint32_t fls = fl << sl;    // bit (kl + sl) = FLUID bit of cell kl
-    const int a_sa = o_sa + me - sl, a_pc = o_pc + me - sl, a_sb = o_sb + me - sl;
-    const int a_zero = o_zero - sl;
-    if constexpr (do_fwd) {
-        // lower neighbours: -x, -y, -z (in tile), +x, +y (halo, PH 1)
-        const int f_xm = il > 0 ? a_sb - 8 * kPad : (halo_fwd ? o_hx + jl * kPad - sl : a_zero);
-        const int f_ym = jl > 0 ? a_sb - kPad : (halo_fwd ? o_hy + il * kPad - sl : a_zero);
-        const int f_xp = (halo_fwd && il == 7) ? o_hx + kTile * kPad + jl * kPad - sl : a_zero;
-        const int f_yp = (halo_fwd && jl == 7) ? o_hy + kTile * kPad + il * kPad - sl : a_zero;
-#pragma unroll
-        for (int st = 0; st < kSteps; ++st) {
-            float acc = lds[a_sa + st];
-            acc = acc + lds[f_xm + st];
-            acc = acc + lds[f_ym + st];
-            const float vzm = lds[a_sb - 1 + st];         // sb[kl - 1]
-            acc = acc + (sl == st ? hzm : vzm);
-            if constexpr (halo_fwd) {
-                acc = acc + lds[f_xp + st];
-                acc = acc + lds[f_yp + st];
-                acc = acc + (sl + 7 == st ? hzp : -0.0f);
-            }
-            const float pcv = lds[a_pc + st];
-            const bool fluid = (fls >> st) & 1u;
-            const float q = fluid ? acc * pcv : 0.0f;
-            const float pq = fluid ? pcv * q : -0.0f;
-            if ((unsigned)(st - sl) < (unsigned)kTile && active) {
-                lds[a_sa + st] = q;
-                lds[a_sb + st] = pq;
-            }
-            wave_sync();
-        }
-    }
-    if constexpr (do_bwd) {
-        // upper neighbours: +x, +y, +z (in tile), -x, -y, -z (halo, PH 2)
-        const int b_xp = il < 7 ? a_sb + 8 * kPad
-                                : (halo_bwd ? o_hx + kTile * kPad + jl * kPad - sl : a_zero);
-        const int b_yp = jl < 7 ? a_sb + kPad
-                                : (halo_bwd ? o_hy + kTile * kPad + il * kPad - sl : a_zero);
-        const int b_xm = (halo_bwd && il == 0) ? o_hx + jl * kPad - sl : a_zero;
-        const int b_ym = (halo_bwd && jl == 0) ? o_hy + il * kPad - sl : a_zero;
-        const float hzp_b = halo_bwd ? hzp : -0.0f;
-#pragma unroll
-        for (int st = 0; st < kSteps; ++st) {
-            constexpr int kLast = kSteps - 1;             // kl = 21 - st - sl
-            const int o = kLast - st;
-            const float pci = lds[a_pc + o];
-            float acc = lds[a_sa + o];
-            acc = acc + pci * lds[b_xp + o];
-            acc = acc + pci * lds[b_yp + o];
-            const float vzp = lds[a_sb + 1 + o];          // sb[kl + 1]
-            acc = acc + pci * (sl + 7 == o ? hzp_b : vzp);
-            if constexpr (halo_bwd) {
-                acc = acc + pci * lds[b_xm + o];
-                acc = acc + pci * lds[b_ym + o];
-                acc = acc + pci * (sl == o ? hzm : -0.0f);
-            }
-            const bool fluid = (fls >> o) & 1u;
-            const float z = fluid ? acc * pci : 0.0f;
-            if ((unsigned)(o - sl) < (unsigned)kTile && active) {
-                lds[a_sa + o] = z;
-                lds[a_sb + o] = fluid ? z : -0.0f;
-            }
-            wave_sync();
-        }
-    }
-    // coalesced write-back
-    double dot[1] = {0.0};
-    if (active) {
-#pragma unroll
-        for (int m = 0; m < kTile; ++m) {
-            const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
-            const int li = (m * 8 + cj) * kPad + ce;
-            const float v = lds[o_sa + li];
-            if constexpr (do_bwd) {
-                Z[g] = v;
-                dot[0] += (double)v * (double)rreg[m];
-            } else {
-                Q[g] = v;
+                int xi = -1;
+                if (il == 0 && gi > 0) xi = gi - 1;
+                if (il == 7 && gi + 1 < N) xi = gi + 1;
+                if (xi >= 0) {
+                    const int64_t ln = cidx(N, xi, gj, k0);
+                    const uint32_t hf = fl8[((int64_t)xi * N + gj) * NS + tk];
+                    float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
+                    const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
+                    const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
+                    float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                    if (vb) {
+                        const float4 b0 = *reinterpret_cast<const float4*>(vb + ln);
+                        const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
+                        float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+    #pragma unroll
+                        for (int kl = 0; kl < kTile; ++kl) av[kl] = av[kl] * bv[kl];
+                    }
+    #pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
+                }
+                int yj = -1;
+                if (jl == 0 && gj > 0) yj = gj - 1;
+                if (jl == 7 && gj + 1 < N) yj = gj + 1;
+                if (yj >= 0) {
+                    const int64_t ln = cidx(N, gi, yj, k0);
+                    const uint32_t hf = fl8[((int64_t)gi * N + yj) * NS + tk];
+                    float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
+                    const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
+                    const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
+                    float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                    if (vb) {
+                        const float4 b0 = *reinterpret_cast<const float4*>(vb + ln);
+                        const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
+                        float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+    #pragma unroll
+                        for (int kl = 0; kl < kTile; ++kl) av[kl] = av[kl] * bv[kl];
+                    }
+    #pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
+                }
+                // z neighbours: values and fluid bits loaded together, masked after
+                const int64_t line = cidx(N, gi, gj, k0);
+                const int64_t sgl = ((int64_t)gi * N + gj) * NS + tk;
+                const bool has_m = k0 > 0, has_p = k0 + kTile < N;
+                const int64_t lm = has_m ? line - 1 : line, lp = has_p ? line + kTile : line;
+                const uint32_t fm = fl8[has_m ? sgl - 1 : sgl], fp = fl8[has_p ? sgl + 1 : sgl];
+                const float vm = vb ? va[lm] * vb[lm] : va[lm];
+                const float vp = vb ? va[lp] * vb[lp] : va[lp];
+                if (has_m && ((fm >> 7) & 1u)) hzm = vm;
+                if (has_p && (fp & 1u)) hzp = vp;
             }
         }
+        wave_sync();
+        const int sl = il + jl;
+        const int me = lane * kPad;
+        const uint32_t fls = fl << sl;    // bit (kl + sl) = FLUID bit of cell kl
+        const int a_sa = o_sa + me - sl, a_pc = o_pc + me - sl, a_sb = o_sb + me - sl;
+        const int a_zero = o_zero - sl;
+        if constexpr (do_fwd) {
+            // lower neighbours: -x, -y, -z (in tile), +x, +y (halo, PH 1)
+            const int f_xm = il > 0 ? a_sb - 8 * kPad : (halo_fwd ? o_hx + jl * kPad - sl : a_zero);
+            const int f_ym = jl > 0 ? a_sb - kPad : (halo_fwd ? o_hy + il * kPad - sl : a_zero);
+            const int f_xp = (halo_fwd && il == 7) ? o_hx + kTile * kPad + jl * kPad - sl : a_zero;
+            const int f_yp = (halo_fwd && jl == 7) ? o_hy + kTile * kPad + il * kPad - sl : a_zero;
+    #pragma unroll
+            for (int st = 0; st < kSteps; ++st) {
+                float acc = lds[a_sa + st];
+                acc = acc + lds[f_xm + st];
+                acc = acc + lds[f_ym + st];
+                const float vzm = lds[a_sb - 1 + st];         // sb[kl - 1]
+                acc = acc + (sl == st ? hzm : vzm);
+                if constexpr (halo_fwd) {
+                    acc = acc + lds[f_xp + st];
+                    acc = acc + lds[f_yp + st];
+                    acc = acc + (sl + 7 == st ? hzp : -0.0f);
+                }
+                const float pcv = lds[a_pc + st];
+                const bool fluid = (fls >> st) & 1u;
+                const float q = fluid ? acc * pcv : 0.0f;
+                const float pq = fluid ? pcv * q : -0.0f;
+                if ((unsigned)(st - sl) < (unsigned)kTile && active) {
+                    lds[a_sa + st] = q;
+                    lds[a_sb + st] = pq;
+                }
+                wave_sync();
+            }
+        }
+        if constexpr (do_bwd) {
+            // upper neighbours: +x, +y, +z (in tile), -x, -y, -z (halo, PH 2)
+            const int b_xp = il < 7 ? a_sb + 8 * kPad
+                                    : (halo_bwd ? o_hx + kTile * kPad + jl * kPad - sl : a_zero);
+            const int b_yp = jl < 7 ? a_sb + kPad
+                                    : (halo_bwd ? o_hy + kTile * kPad + il * kPad - sl : a_zero);
+            const int b_xm = (halo_bwd && il == 0) ? o_hx + jl * kPad - sl : a_zero;
+            const int b_ym = (halo_bwd && jl == 0) ? o_hy + il * kPad - sl : a_zero;
+            const float hzp_b = halo_bwd ? hzp : -0.0f;
+    #pragma unroll
+            for (int st = 0; st < kSteps; ++st) {
+                constexpr int kLast = kSteps - 1;             // kl = 21 - st - sl
+                const int o = kLast - st;
+                const float pci = lds[a_pc + o];
+                float acc = lds[a_sa + o];
+                acc = acc + pci * lds[b_xp + o];
+                acc = acc + pci * lds[b_yp + o];
+                const float vzp = lds[a_sb + 1 + o];          // sb[kl + 1]
+                acc = acc + pci * (sl + 7 == o ? hzp_b : vzp);
+                if constexpr (halo_bwd) {
+                    acc = acc + pci * lds[b_xm + o];
+                    acc = acc + pci * lds[b_ym + o];
+                    acc = acc + pci * (sl == o ? hzm : -0.0f);
+                }
+                const bool fluid = (fls >> o) & 1u;
+                const float z = fluid ? acc * pci : 0.0f;
+                if ((unsigned)(o - sl) < (unsigned)kTile && active) {
+                    lds[a_sa + o] = z;
+                    lds[a_sb + o] = fluid ? z : -0.0f;
+                }
+                wave_sync();
+            }
+        }
+        // coalesced write-back
+        double dot[1] = {0.0};
+        if (active) {
+    #pragma unroll
+            for (int m = 0; m < kTile; ++m) {
+                const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
+                const int li = (m * 8 + cj) * kPad + ce;
+                const float v = lds[o_sa + li];
+                if constexpr (do_bwd) {
+                    Z[g] = v;
+                    dot[0] += (double)v * (double)rreg[m];
+                } else {
+                    Q[g] = v;
+                }
+            }
+        }
+        if constexpr (PH == 0) return;
+        block_partials<1>(dot, red.partials, slot_base + blockIdx.x);
     }
-    if constexpr (PH == 0) return;
-    block_partials<1>(dot, red.partials, slot_base + blockIdx.x);
-}
+};
 
 // Jacobi: z = r * (1/diag), rho = z.r over the active segments
-__global__ void k_jacobi_apply(const float* __restrict__ pc, const float* __restrict__ r,
-                               float* __restrict__ Z, const int32_t* __restrict__ segs,
-                               int64_t nseg, Reduce red) {
-    if (red.sc->done) return;
-    double v[1] = {0.0};
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-         u += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c0 = 8 * (int64_t)segs[u];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const int64_t c = c0 + m;
-            const float z = r[c] * pc[c];
-            Z[c] = z;
-            v[0] += (double)z * (double)r[c];
+struct KJacobiApply {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const float* pc;
+        const float* r;
+        float* Z;
+        const int32_t* segs;
+        int64_t nseg;
+        Reduce red;
+    };
+    static __device__ void run(const Args& kargs) {
+        const float* __restrict__ pc = kargs.pc;
+        const float* __restrict__ r = kargs.r;
+        float* __restrict__ Z = kargs.Z;
+        const int32_t* __restrict__ segs = kargs.segs;
+        int64_t nseg = kargs.nseg;
+        const Reduce& red = kargs.red;
+
+        if (red.sc->done) return;
+        double v[1] = {0.0};
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+             u += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t c0 = 8 * (int64_t)segs[u];
+    #pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int64_t c = c0 + m;
+                const float z = r[c] * pc[c];
+                Z[c] = z;
+                v[0] += (double)z * (double)r[c];
+            }
         }
+        block_partials<1>(v, red.partials, blockIdx.x);
     }
-    block_partials<1>(v, red.partials, blockIdx.x);
-}
+};
 
 // ------------------------------------------------------------ active sets
-__global__ void k_seg_flags(int64_t nseg, const uint8_t* __restrict__ fl8, int32_t* __restrict__ f) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nseg) f[t] = fl8[t] != 0 ? 1 : 0;
-}
+struct KSegFlags {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int64_t nseg;
+        const uint8_t* fl8;
+        int32_t* f;
+    };
+    static __device__ void run(const Args& kargs) {
+        int64_t nseg = kargs.nseg;
+        const uint8_t* __restrict__ fl8 = kargs.fl8;
+        int32_t* __restrict__ f = kargs.f;
 
-__global__ void k_seg_compact(int64_t nseg, const int32_t* __restrict__ f,
-                              const int32_t* __restrict__ pos, int32_t* __restrict__ list,
-                              SolverScalars* sc) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nseg) return;
-    if (f[t]) list[pos[t]] = (int32_t)t;
-    if (t == nseg - 1) sc->nseg = (int64_t)pos[t] + f[t];
-}
+        const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t < nseg) f[t] = fl8[t] != 0 ? 1 : 0;
+    }
+};
+
+struct KSegCompact {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int64_t nseg;
+        const int32_t* f;
+        const int32_t* pos;
+        int32_t* list;
+        SolverScalars* sc;
+    };
+    static __device__ void run(const Args& kargs) {
+        int64_t nseg = kargs.nseg;
+        const int32_t* __restrict__ f = kargs.f;
+        const int32_t* __restrict__ pos = kargs.pos;
+        int32_t* __restrict__ list = kargs.list;
+        SolverScalars* sc = kargs.sc;
+
+        const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= nseg) return;
+        if (f[t]) list[pos[t]] = (int32_t)t;
+        if (t == nseg - 1) sc->nseg = (int64_t)pos[t] + f[t];
+    }
+};
 
 // any[t] = 1 when one of tile t's 64 z-segments holds a FLUID cell
-__global__ void k_tile_any(int N, const uint8_t* __restrict__ fl8, int32_t* __restrict__ any) {
-    const int TN = N / kTile;
-    const int64_t nt = (int64_t)TN * TN * TN;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    const int tk = (int)(t % TN), tj = (int)((t / TN) % TN), ti = (int)(t / ((int64_t)TN * TN));
-    const int NS = N / kTile;
-    uint32_t m = 0;
-    for (int il = 0; il < kTile; ++il)
-        for (int jl = 0; jl < kTile; ++jl)
-            m |= fl8[((int64_t)(ti * kTile + il) * N + tj * kTile + jl) * NS + tk];
-    any[t] = m != 0 ? 1 : 0;
-}
+struct KTileAny {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int N;
+        const uint8_t* fl8;
+        int32_t* any;
+    };
+    static __device__ void run(const Args& kargs) {
+        int N = kargs.N;
+        const uint8_t* __restrict__ fl8 = kargs.fl8;
+        int32_t* __restrict__ any = kargs.any;
+
+        const int TN = N / kTile;
+        const int64_t nt = (int64_t)TN * TN * TN;
+        const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= nt) return;
+        const int tk = (int)(t % TN), tj = (int)((t / TN) % TN), ti = (int)(t / ((int64_t)TN * TN));
+        const int NS = N / kTile;
+        uint32_t m = 0;
+        for (int il = 0; il < kTile; ++il)
+            for (int jl = 0; jl < kTile; ++jl)
+                m |= fl8[((int64_t)(ti * kTile + il) * N + tj * kTile + jl) * NS + tk];
+        any[t] = m != 0 ? 1 : 0;
+    }
+};
 
 // One block: ascending lists of the active red, black and all tiles
 // (lists + 0, + nt, + 2 nt) and their counts.
-__global__ __launch_bounds__(1024) void k_tile_lists(int N, const int32_t* __restrict__ any,
-                                                      int32_t* __restrict__ lists,
-                                                      SolverScalars* sc) {
-    const int TN = N / kTile;
-    const int nt = TN * TN * TN;
-    __shared__ int wsum[3][16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int base[3] = {0, 0, 0};
-    for (int t0 = 0; t0 < nt; t0 += blockDim.x) {
-        const int t = t0 + threadIdx.x;
-        int f[3] = {0, 0, 0};
-        if (t < nt && any[t]) {
-            const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
-            const int col = (ti + tj + tk) & 1;
-            f[col] = 1;
-            f[2] = 1;
+struct KTileLists {
+    static constexpr int kThreads = 1024;
+    struct Args {
+        int N;
+        const int32_t* any;
+        int32_t* lists;
+        SolverScalars* sc;
+    };
+    static __device__ void run(const Args& kargs) {
+        int N = kargs.N;
+        const int32_t* __restrict__ any = kargs.any;
+        int32_t* __restrict__ lists = kargs.lists;
+        SolverScalars* sc = kargs.sc;
+
+        const int TN = N / kTile;
+        const int nt = TN * TN * TN;
+        __shared__ int wsum[3][16];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        int base[3] = {0, 0, 0};
+        for (int t0 = 0; t0 < nt; t0 += blockDim.x) {
+            const int t = t0 + threadIdx.x;
+            int f[3] = {0, 0, 0};
+            if (t < nt && any[t]) {
+                const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
+                const int col = (ti + tj + tk) & 1;
+                f[col] = 1;
+                f[2] = 1;
+            }
+            int pre[3];
+    #pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const uint64_t b = __ballot(f[c]);
+                pre[c] = __popcll(b & ((1ull << lane) - 1ull));
+                if (lane == 0) wsum[c][w] = __popcll(b);
+            }
+            __syncthreads();
+    #pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                int off = base[c];
+                for (int ww = 0; ww < w; ++ww) off += wsum[c][ww];
+                if (f[c]) lists[(int64_t)c * nt + off + pre[c]] = t;
+                int tot = 0;
+                for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) tot += wsum[c][ww];
+                base[c] += tot;
+            }
+            __syncthreads();
         }
-        int pre[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const uint64_t b = __ballot(f[c]);
-            pre[c] = __popcll(b & ((1ull << lane) - 1ull));
-            if (lane == 0) wsum[c][w] = __popcll(b);
+        if (threadIdx.x == 0) {
+            sc->ntile[0] = base[0];
+            sc->ntile[1] = base[1];
+            sc->ntile[2] = base[2];
         }
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            int off = base[c];
-            for (int ww = 0; ww < w; ++ww) off += wsum[c][ww];
-            if (f[c]) lists[(int64_t)c * nt + off + pre[c]] = t;
-            int tot = 0;
-            for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) tot += wsum[c][ww];
-            base[c] += tot;
-        }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        sc->ntile[0] = base[0];
-        sc->ntile[1] = base[1];
-        sc->ntile[2] = base[2];
-    }
-}
+};
 
 int stream_blocks(int64_t n) {
     int64_t b = (n + kBlock - 1) / kBlock;
@@ -1134,18 +1442,17 @@ int reduce_blocks(int64_t n, int) { return stream_blocks(n); }
 
 void launch_finalize(hipStream_t s, const Reduce& red, int kind, int nparts, double tol = 0.0,
                      int stop_rule = 0, int N = 0) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, red, kind, nparts, tol, stop_rule, N);
+    launch<KFinalize>(s, dim3(1), dim3(256), {red, kind, nparts, tol, stop_rule, N});
 }
 
 void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every) {
-    hipLaunchKernelGGL(k_solver_init, dim3(1), dim3(64), 0, s, sc, max_iters, refine_every);
+    launch<KSolverInit>(s, dim3(1), dim3(64), {sc, max_iters, refine_every});
 }
 
 void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                           float* diag, uint8_t* nup, float* pc) {
     int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_precond_setup, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, A, L,
-                       diag, nup, pc);
+    launch<KPrecondSetup>(s, dim3(grid_for(nc, kBlock)), dim3(kBlock), {p, A, L, diag, nup, pc});
 }
 
 void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
@@ -1153,35 +1460,29 @@ void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, cons
     if (p.precond == 2) return;   // Jacobi: built by setup
     auto nb = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(n, 4096)); };
     if (p.precond == 0) {
-        hipLaunchKernelGGL(k_mic_build, dim3(nb(act.ntiles[0])), dim3(64), 0, s, p, A, diag, nup, pc,
-                           0, act.tiles[0], act.ntiles[0]);
-        hipLaunchKernelGGL(k_mic_build, dim3(nb(act.ntiles[1])), dim3(64), 0, s, p, A, diag, nup, pc,
-                           1, act.tiles[1], act.ntiles[1]);
+        launch<KMicBuild>(s, dim3(nb(act.ntiles[0])), dim3(64), {p, A, diag, nup, pc, 0, act.tiles[0], act.ntiles[0]});
+        launch<KMicBuild>(s, dim3(nb(act.ntiles[1])), dim3(64), {p, A, diag, nup, pc, 1, act.tiles[1], act.ntiles[1]});
     } else {
-        hipLaunchKernelGGL(k_mic_build, dim3(nb(act.ntiles[2])), dim3(64), 0, s, p, A, diag, nup, pc,
-                           -1, act.tiles[2], act.ntiles[2]);
+        launch<KMicBuild>(s, dim3(nb(act.ntiles[2])), dim3(64), {p, A, diag, nup, pc, -1, act.tiles[2], act.ntiles[2]});
     }
 }
 
 void launch_fluid_segments(hipStream_t s, int N, const uint8_t* A, uint8_t* fl8) {
     const int64_t nseg = (int64_t)N * N * (N / kTile);
-    hipLaunchKernelGGL(k_fluid_segments, dim3(grid_for(nseg, kBlock)), dim3(kBlock), 0, s, N, A, fl8);
+    launch<KFluidSegments>(s, dim3(grid_for(nseg, kBlock)), dim3(kBlock), {N, A, fl8});
 }
 
 void launch_active_sets(hipStream_t s, int N, const uint8_t* fl8, int32_t* seg_flag,
                         int32_t* seg_scan, int32_t* seg_list, int32_t* tile_any,
                         int32_t* tile_lists, int32_t* scan_sums, SolverScalars* sc) {
     const int64_t nseg = (int64_t)N * N * (N / kTile);
-    hipLaunchKernelGGL(k_seg_flags, dim3(grid_for(nseg, kBlock)), dim3(kBlock), 0, s, nseg, fl8,
-                       seg_flag);
+    launch<KSegFlags>(s, dim3(grid_for(nseg, kBlock)), dim3(kBlock), {nseg, fl8, seg_flag});
     launch_scan(s, seg_flag, seg_scan, nseg, scan_sums);
-    hipLaunchKernelGGL(k_seg_compact, dim3(grid_for(nseg, kBlock)), dim3(kBlock), 0, s, nseg,
-                       (const int32_t*)seg_flag, (const int32_t*)seg_scan, seg_list, sc);
+    launch<KSegCompact>(s, dim3(grid_for(nseg, kBlock)), dim3(kBlock), {nseg, (const int32_t*)seg_flag, (const int32_t*)seg_scan, seg_list, sc});
     const int64_t TN = N / kTile;
     const int64_t nt = TN * TN * TN;
-    hipLaunchKernelGGL(k_tile_any, dim3(grid_for(nt, kBlock)), dim3(kBlock), 0, s, N, fl8, tile_any);
-    hipLaunchKernelGGL(k_tile_lists, dim3(1), dim3(1024), 0, s, N, (const int32_t*)tile_any,
-                       tile_lists, sc);
+    launch<KTileAny>(s, dim3(grid_for(nt, kBlock)), dim3(kBlock), {N, fl8, tile_any});
+    launch<KTileLists>(s, dim3(1), dim3(1024), {N, (const int32_t*)tile_any, tile_lists, sc});
 }
 
 static int list_blocks(int64_t n) {
@@ -1194,8 +1495,7 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, co
                           const Active& act) {
     if (p.precond == 2) {
         const int nb = list_blocks(act.nseg);
-        hipLaunchKernelGGL(k_jacobi_apply, dim3(nb), dim3(kBlock), 0, s, pc, r, z, act.segs,
-                           act.nseg, red);
+        launch<KJacobiApply>(s, dim3(nb), dim3(kBlock), {pc, r, z, act.segs, act.nseg, red});
         launch_finalize(s, red, FIN_RHO, nb);
         return;
     }
@@ -1204,17 +1504,13 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, co
     };
     if (p.precond == 0) {
         const int nr = blocks(act.ntiles[0]), nbk = blocks(act.ntiles[1]);
-        hipLaunchKernelGGL(k_mic_apply<0>, dim3(nr), dim3(256), 0, s, p, fl8, pc, r, q, z,
-                           act.tiles[0], act.ntiles[0], red, 0);
-        hipLaunchKernelGGL(k_mic_apply<1>, dim3(nbk), dim3(256), 0, s, p, fl8, pc, r, q, z,
-                           act.tiles[1], act.ntiles[1], red, 0);
-        hipLaunchKernelGGL(k_mic_apply<2>, dim3(nr), dim3(256), 0, s, p, fl8, pc, r, q, z,
-                           act.tiles[0], act.ntiles[0], red, nbk);
+        launch<KMicApply<0>>(s, dim3(nr), dim3(256), {p, fl8, pc, r, q, z, act.tiles[0], act.ntiles[0], red, 0});
+        launch<KMicApply<1>>(s, dim3(nbk), dim3(256), {p, fl8, pc, r, q, z, act.tiles[1], act.ntiles[1], red, 0});
+        launch<KMicApply<2>>(s, dim3(nr), dim3(256), {p, fl8, pc, r, q, z, act.tiles[0], act.ntiles[0], red, nbk});
         launch_finalize(s, red, FIN_RHO, nbk + nr);
     } else {
         const int nb = blocks(act.ntiles[2]);
-        hipLaunchKernelGGL(k_mic_apply<3>, dim3(nb), dim3(256), 0, s, p, fl8, pc, r, q, z,
-                           act.tiles[2], act.ntiles[2], red, 0);
+        launch<KMicApply<3>>(s, dim3(nb), dim3(256), {p, fl8, pc, r, q, z, act.tiles[2], act.ntiles[2], red, 0});
         launch_finalize(s, red, FIN_RHO, nb);
     }
 }
@@ -1224,8 +1520,7 @@ void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const fl
                       double* P64, int refine) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
     const int nb = stream_blocks(nc);
-    hipLaunchKernelGGL(k_residual0, dim3(nb), dim3(kBlock), 0, s, p, A, L, P, D, R, red, tol,
-                       stop_rule, P64, refine);
+    launch<KResidual0>(s, dim3(nb), dim3(kBlock), {p, A, L, P, D, R, red, tol, stop_rule, P64, refine});
     launch_finalize(s, red, FIN_RESID0, nb, tol, stop_rule, p.N);
 }
 
@@ -1236,15 +1531,14 @@ void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const flo
                                                   ((int64_t)kBlock * kMaxPartials));
     const int64_t per_block = (int64_t)kBlock * per;
     const int64_t nb = std::max<int64_t>(1, (act.nseg + per_block - 1) / per_block);
-    hipLaunchKernelGGL(k_spmv_dot, dim3((unsigned)xcd_grid(nb)), dim3(kBlock), 0, s, p, A, L, x, y,
-                       act.segs, act.nseg, per, red);
+    launch<KSpmvDot>(s, dim3((unsigned)xcd_grid(nb)), dim3(kBlock), {p, A, L, x, y, act.segs, act.nseg, per, red});
     launch_finalize(s, red, FIN_SIGMA, (int)nb);
 }
 
 void launch_axpy2(hipStream_t s, float* P, const float* S, float* R, const float* Z,
                   const Reduce& red, const Active& act) {
     const int nb = list_blocks(act.nseg);
-    hipLaunchKernelGGL(k_axpy2, dim3(nb), dim3(kBlock), 0, s, P, S, R, Z, act.segs, act.nseg, red);
+    launch<KAxpy2>(s, dim3(nb), dim3(kBlock), {P, S, R, Z, act.segs, act.nseg, red});
     launch_finalize(s, red, FIN_RNORM, nb);
 }
 
@@ -1252,38 +1546,33 @@ void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, co
                             float* P, double* P64, const float* D, float* R, const Reduce& red,
                             const Active& act) {
     const int nb = list_blocks(act.nseg);
-    hipLaunchKernelGGL(k_flush, dim3(nb), dim3(kBlock), 0, s, A, P, P64, act.segs, act.nseg,
-                       (const SolverScalars*)red.sc);
-    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, p, A, L, (const float*)P,
-                       (const double*)P64, D, R, act.segs, act.nseg, red, 0);
+    launch<KFlush>(s, dim3(nb), dim3(kBlock), {A, P, P64, act.segs, act.nseg, (const SolverScalars*)red.sc});
+    launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, (const float*)P, (const double*)P64, D, R, act.segs, act.nseg, red, 0});
     launch_finalize(s, red, FIN_TRUE, nb);
 }
 
 void launch_refine_finish(hipStream_t s, const uint8_t* A, float* P, const double* P64,
                           const Active& act) {
-    hipLaunchKernelGGL(k_refine_finish, dim3(list_blocks(act.nseg)), dim3(kBlock), 0, s, A, P, P64,
-                       act.segs, act.nseg);
+    launch<KRefineFinish>(s, dim3(list_blocks(act.nseg)), dim3(kBlock), {A, P, P64, act.segs, act.nseg});
 }
 
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                                const float* P, const float* D, const Reduce& red,
                                const Active& act) {
     const int nb = list_blocks(act.nseg);
-    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, p, A, L, P,
-                       (const double*)nullptr, D, (float*)nullptr, act.segs, act.nseg, red, 1);
+    launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, P, (const double*)nullptr, D, (float*)nullptr, act.segs, act.nseg, red, 1});
     launch_finalize(s, red, FIN_REPORT, nb);
 }
 
 void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,
                      const Active& act) {
-    hipLaunchKernelGGL(k_update_s, dim3(list_blocks(act.nseg)), dim3(kBlock), 0, s, Z, S, act.segs,
-                       act.nseg, (const SolverScalars*)red.sc, first);
+    launch<KUpdateS>(s, dim3(list_blocks(act.nseg)), dim3(kBlock), {Z, S, act.segs, act.nseg, (const SolverScalars*)red.sc, first});
 }
 
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                        const float* x, float* y) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_spmv_plain, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, s, p, A, L, x, y);
+    launch<KSpmvPlain>(s, dim3(grid_for(nc, kBlock)), dim3(kBlock), {p, A, L, x, y});
 }
 
 }  // namespace ariel
