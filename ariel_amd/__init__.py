@@ -5,3 +5,4 @@ declared in include/ariel.h; this package is its Python mirror.
 """
 from . import _abi, scenes  # noqa: F401
 from .flipsim import ArielError, FlipSim  # noqa: F401
+from .batch import FlipBatch  # noqa: F401

@@ -11,6 +11,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libariel_amd.so")
 
 ARIEL_OK = 0
+(ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_CAPACITY, ERR_UNSUPPORTED,
+ ERR_IO) = -1, -2, -3, -4, -5, -6, -7
 AIR, FLUID, SOLID = 0, 1, 2
 LIQUID_BOX, LIQUID_SPHERE = 0, 1
 PRECOND_MIC0_TILED, PRECOND_MIC0, PRECOND_JACOBI, PRECOND_REFERENCE, PRECOND_MIC0_BLOCK, \
@@ -106,6 +108,17 @@ PROTOTYPES = {
     "ariel_export_levelset": (_I32, [_P, ctypes.c_char_p]),
     "ariel_selftest_division": (_I32, [ctypes.c_int32, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, ctypes.POINTER(ctypes.c_int64)]),
+    "ariel_batch_create": (_I32, [ctypes.POINTER(SceneDesc), _I32, ctypes.POINTER(_P)]),
+    "ariel_batch_destroy": (_I32, [_P]),
+    "ariel_batch_last_error": (ctypes.c_char_p, [_P]),
+    "ariel_batch_size": (_I32, [_P, ctypes.POINTER(_I32)]),
+    "ariel_batch_tank": (_I32, [_P, _I32, ctypes.POINTER(_P)]),
+    "ariel_batch_init": (_I32, [_P]),
+    "ariel_batch_autotune": (_I32, [_P]),
+    "ariel_batch_get_tuning": (_I32, [_P, _I32, ctypes.POINTER(_I32)]),
+    "ariel_batch_set_graphs": (_I32, [_P, _I32]),
+    "ariel_batch_step": (_I32, [_P, _I32]),
+    "ariel_batch_step_sequential": (_I32, [_P, _I32]),
 }
 
 _lib = None

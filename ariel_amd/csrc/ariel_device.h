@@ -85,6 +85,7 @@ __host__ __device__ __forceinline__ int64_t xcd_grid(int64_t nb) {
 __device__ __forceinline__ int64_t xcd_block(int64_t nb) {
     const int64_t per = (nb + kXcds - 1) / kXcds;
     const int64_t b = blockIdx.x;
+    if (b >= per * kXcds) return nb;   // beyond this grid (a launch sized for a larger one)
     return (b % kXcds) * per + b / kXcds;
 }
 

@@ -54,11 +54,15 @@ struct SolverScalars {
 // Everything outside them stays as the solve's full-grid set-up left it
 // (zero off-FLUID), which the operator, the preconditioner halos and the dot
 // products never read as FLUID values.
+// A count of -1 means "read it from the device scalars" (batched launches);
+// the caps are the largest possible counts, used to size such launches.
 struct Active {
     const int32_t* segs = nullptr;
     int64_t nseg = 0;
     const int32_t* tiles[3] = {nullptr, nullptr, nullptr};   // red, black, all
     int64_t ntiles[3] = {0, 0, 0};
+    int64_t seg_cap = 0;
+    int64_t tile_cap[3] = {0, 0, 0};
 };
 
 struct Params {
@@ -78,6 +82,7 @@ struct Params {
     int frame;
     float mic_tau, mic_sigma;
     int precond;
+    int p2g_tile;      // edge of the face-column tiles that order P2G blocks
 };
 
 // Reduction workspace: per-block double partials + an arrival counter.
@@ -178,7 +183,7 @@ void launch_grid_stats(hipStream_t s, const Params& p, float* const u[3], const 
 void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                           float* diag, uint8_t* nup, float* pc);
 void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
-                          const uint8_t* nup, float* pc, const Active& act);
+                          const uint8_t* nup, float* pc, const Active& act, SolverScalars* sc);
 // Fluid bits of each 8-cell z-segment (input of the tiled preconditioner apply).
 void launch_fluid_segments(hipStream_t s, int N, const uint8_t* A, uint8_t* fl8);
 // Active segment and tile lists from fl8; counts into sc->nseg / sc->ntile.
@@ -203,7 +208,7 @@ void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, co
                             float* P, double* P64, const float* D, float* R, const Reduce& red,
                             const Active& act);
 void launch_refine_finish(hipStream_t s, const uint8_t* A, float* P, const double* P64,
-                          const Active& act);
+                          const Active& act, const SolverScalars* sc);
 void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,
                      const Active& act);
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,

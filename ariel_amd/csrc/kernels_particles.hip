@@ -107,6 +107,7 @@ struct KScanReduce {
         int64_t n = kargs.n;
         int32_t* __restrict__ bsum = kargs.bsum;
 
+        if (blockIdx.x > 0 && (int64_t)blockIdx.x * kScanTile >= n) return;   // beyond this grid
         int64_t base = (int64_t)blockIdx.x * kScanTile;
         int32_t acc = 0;
         for (int t = 0; t < kScanItems; ++t) {
@@ -164,6 +165,7 @@ struct KScanApply {
         int32_t* __restrict__ out = kargs.out;
         int nb = kargs.nb;
 
+        if ((int)blockIdx.x >= nb) return;   // beyond this grid
         int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
         int32_t v[kScanItems];
         int32_t acc = 0;
@@ -690,8 +692,9 @@ struct KP2gCols {
         const int nkc = (N + 1 + 63) / 64;
         // Blocks run over 8x8 tiles of face columns (i, j), k-chunks innermost, so
         // the particle columns that neighbouring face columns share are staged while
-        // still in L2; each XCD takes a contiguous run of tiles.
-        constexpr int kT = 8;
+        // still in L2; each XCD takes a contiguous run of tiles. The tile edge is
+        // a launch parameter (autotuned per grid size for batched tanks).
+        const int kT = p.p2g_tile;
         const int nti = (N + 1 + kT - 1) / kT;
         const int64_t nbl = (int64_t)nti * nti * kT * kT * nkc;
         int64_t b = xcd_block(nbl);
@@ -1260,8 +1263,8 @@ void launch_mark(hipStream_t s, const Params& p, const float4* vel, const int32_
 void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
                 const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3]) {
     const int64_t nkc = (p.N + 1 + 63) / 64;
-    const int64_t nti = (p.N + 1 + 7) / 8;
-    const int64_t nb = nti * nti * 64 * nkc;
+    const int64_t nti = (p.N + 1 + p.p2g_tile - 1) / p.p2g_tile;
+    const int64_t nb = nti * nti * p.p2g_tile * p.p2g_tile * nkc;
     if (p.splat == 1)
         launch<KP2gCols<true>>(s, dim3((unsigned)xcd_grid(nb)), dim3(64), {p, pos, vel, offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]});
     else

@@ -30,6 +30,39 @@ constexpr int kPad = 9;            // LDS stride per z-line (bank-conflict paddi
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Grids of the solver kernels as functions of the work counts, shared by the
+// host launchers and the kernels. A kernel launched on more blocks than its own
+// grid (a batched launch sized for the largest tank) lets the extra blocks
+// return, strides by its own grid and writes partials only for it, so each
+// tank's reduction order is the one a launch of its own would have.
+__host__ __device__ inline int64_t list_grid(int64_t n) {
+    const int64_t b = (n + kBlock - 1) / kBlock;
+    return b < 1 ? 1 : (b < kRedBlocks ? b : kRedBlocks);
+}
+__host__ __device__ inline int spmv_per_thread(int64_t nseg) {
+    const int64_t cap = (int64_t)kBlock * kMaxPartials;
+    const int64_t per = (nseg + cap - 1) / cap;
+    return per < 1 ? 1 : (int)per;
+}
+__host__ __device__ inline int64_t spmv_grid(int64_t nseg) {
+    const int64_t per_block = (int64_t)kBlock * spmv_per_thread(nseg);
+    const int64_t nb = (nseg + per_block - 1) / per_block;
+    return nb < 1 ? 1 : nb;
+}
+constexpr int kTilesPerBlock = 4;
+__host__ __device__ inline int64_t mic_grid(int64_t ntiles) {
+    const int64_t nb = (ntiles + kTilesPerBlock - 1) / kTilesPerBlock;
+    return nb < 1 ? 1 : nb;
+}
+// Counts passed as -1 are read from the tank's device scalars (batched launches,
+// which are recorded before the counts exist).
+__device__ __forceinline__ int64_t seg_count(int64_t n, const SolverScalars* sc) {
+    return n >= 0 ? n : sc->nseg;
+}
+__device__ __forceinline__ int64_t tile_count(int64_t n, const SolverScalars* sc, int which) {
+    return n >= 0 ? n : (int64_t)sc->ntile[which];
+}
+
 // ------------------------------------------------------------ reductions
 // Each reducing kernel stores one FP64 partial per block and value
 // (part[slot*NV + v], plain stores); a one-block finalize kernel launched after
@@ -197,9 +230,11 @@ struct KResidual0 {
         const int64_t nc = (int64_t)N * N * N;
         const float n = (float)N;
         const float h = 1.0f / (n * n);
+        const int64_t nb = list_grid(nc);
+        if ((int64_t)blockIdx.x >= nb) return;
         double v[2] = {0.0, 0.0};
         for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-             c += (int64_t)gridDim.x * blockDim.x) {
+             c += nb * blockDim.x) {
             if (A[c] != kFluid) {
                 if (!refine) P[c] = 0.0f;
                 else P64[c] = 0.0;
@@ -334,8 +369,10 @@ struct KSpmvDot {
         const Reduce& red = kargs.red;
 
         if (red.sc->done) return;
+        nseg = seg_count(nseg, red.sc);
+        per_thread = spmv_per_thread(nseg);
         const int64_t per_block = (int64_t)blockDim.x * per_thread;
-        const int64_t nb = (nseg + per_block - 1) / per_block;
+        const int64_t nb = spmv_grid(nseg);
         const int64_t b = xcd_block(nb);
         if (b >= nb) return;
         double v[1] = {0.0};
@@ -395,6 +432,9 @@ struct KAxpy2 {
         const Reduce& red = kargs.red;
 
         if (red.sc->done) return;
+        nseg = seg_count(nseg, red.sc);
+        const int64_t nb = list_grid(nseg);
+        if ((int64_t)blockIdx.x >= nb) return;
         const float af = (float)red.sc->alpha;
         const float naf = -af;
         float4* __restrict__ P4 = reinterpret_cast<float4*>(P);
@@ -403,7 +443,7 @@ struct KAxpy2 {
         const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
         double v[1] = {0.0};
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += (int64_t)gridDim.x * blockDim.x) {
+             u += nb * blockDim.x) {
             const int64_t c = 2 * (int64_t)segs[u];
             float4 p0 = P4[c], s0 = S4[c], r0 = R4[c], z0 = Z4[c];
             float4 p1 = P4[c + 1], s1 = S4[c + 1], r1 = R4[c + 1], z1 = Z4[c + 1];
@@ -444,8 +484,11 @@ struct KFlush {
         const SolverScalars* sc = kargs.sc;
 
         if (sc->done || !refine_due(sc)) return;
+        nseg = seg_count(nseg, sc);
+        const int64_t nb = list_grid(nseg);
+        if ((int64_t)blockIdx.x >= nb) return;
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += (int64_t)gridDim.x * blockDim.x) {
+             u += nb * blockDim.x) {
             const int64_t c0 = 8 * (int64_t)segs[u];
     #pragma unroll
             for (int m = 0; m < 8; ++m) {
@@ -488,10 +531,13 @@ struct KTrueResidual {
         int mode = kargs.mode;
 
         if (mode == 0 && (red.sc->done || !refine_due(red.sc))) return;
+        nseg = seg_count(nseg, red.sc);
+        const int64_t nb = list_grid(nseg);
+        if ((int64_t)blockIdx.x >= nb) return;
         const int N = p.N;
         double v[1] = {0.0};
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += (int64_t)gridDim.x * blockDim.x) {
+             u += nb * blockDim.x) {
             const int64_t c0 = 8 * (int64_t)segs[u];
             for (int m = 0; m < 8; ++m) {
                 const int64_t c = c0 + m;
@@ -521,16 +567,19 @@ struct KRefineFinish {
         const double* P64;
         const int32_t* segs;
         int64_t nseg;
+        const SolverScalars* sc;
     };
     static __device__ void run(const Args& kargs) {
         const uint8_t* __restrict__ A = kargs.A;
         float* __restrict__ P = kargs.P;
         const double* __restrict__ P64 = kargs.P64;
         const int32_t* __restrict__ segs = kargs.segs;
-        int64_t nseg = kargs.nseg;
+        int64_t nseg = seg_count(kargs.nseg, kargs.sc);
 
+        const int64_t nb = list_grid(nseg);
+        if ((int64_t)blockIdx.x >= nb) return;
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += (int64_t)gridDim.x * blockDim.x) {
+             u += nb * blockDim.x) {
             const int64_t c0 = 8 * (int64_t)segs[u];
     #pragma unroll
             for (int m = 0; m < 8; ++m) {
@@ -560,11 +609,14 @@ struct KUpdateS {
         int first = kargs.first;
 
         if (sc->done) return;
+        nseg = seg_count(nseg, sc);
+        const int64_t nb = list_grid(nseg);
+        if ((int64_t)blockIdx.x >= nb) return;
         const float bf = (float)sc->beta;
         const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
         float4* __restrict__ S4 = reinterpret_cast<float4*>(S);
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += (int64_t)gridDim.x * blockDim.x) {
+             u += nb * blockDim.x) {
             const int64_t c = 2 * (int64_t)segs[u];
     #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -624,6 +676,18 @@ __device__ __forceinline__ void finish_rho(SolverScalars* sc, double rho_new) {
 }
 
 // ------------------------------------------------------------ finalize
+// Partial counts that depend on the active sets, for finalize launches recorded
+// before the counts exist (nparts < 0 selects the producer's grid rule).
+enum { PARTS_LIST = -1, PARTS_SPMV = -2, PARTS_MIC_RB = -3, PARTS_MIC_ALL = -4 };
+__device__ __forceinline__ int64_t parts_from_counts(int code, const SolverScalars* sc) {
+    switch (code) {
+        case PARTS_LIST: return list_grid(sc->nseg);
+        case PARTS_SPMV: return spmv_grid(sc->nseg);
+        case PARTS_MIC_RB: return mic_grid(sc->ntile[1]) + mic_grid(sc->ntile[0]);
+        default: return mic_grid(sc->ntile[2]);
+    }
+}
+
 enum { FIN_RESID0 = 0, FIN_SIGMA = 1, FIN_RNORM = 2, FIN_TRUE = 3, FIN_REPORT = 4, FIN_RHO = 5 };
 
 // One block: sum the partials of the preceding reducing kernel in a fixed order
@@ -650,6 +714,7 @@ struct KFinalize {
         SolverScalars* sc = red.sc;
         if ((kind == FIN_SIGMA || kind == FIN_RNORM || kind == FIN_RHO) && sc->done) return;
         if (kind == FIN_TRUE && (sc->done || !refine_due(sc))) return;
+        if (nparts < 0) nparts = (int)parts_from_counts(nparts, sc);
         const int NV = kind == FIN_RESID0 ? 2 : 1;
         double a0 = 0.0, a1 = 0.0;
         for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
@@ -837,6 +902,7 @@ struct KMicBuild {
         int color;
         const int32_t* tiles;
         int64_t ntl;
+        const SolverScalars* sc;
     };
     static __device__ void run(const Args& kargs) {
         const Params& p = kargs.p;
@@ -846,7 +912,7 @@ struct KMicBuild {
         float* __restrict__ pc = kargs.pc;
         int color = kargs.color;
         const int32_t* __restrict__ tiles = kargs.tiles;
-        int64_t ntl = kargs.ntl;
+        int64_t ntl = tile_count(kargs.ntl, kargs.sc, kargs.color < 0 ? 2 : kargs.color);
 
         __shared__ float s_pc[64 * kPad];
         __shared__ float s_nm[64 * kPad];
@@ -1002,7 +1068,6 @@ struct KFluidSegments {
     }
 };
 
-constexpr int kTilesPerBlock = 4;
 constexpr int kLine = 64 * kPad;          // floats of one per-line LDS array
 constexpr int kHalo = 2 * kTile * kPad;   // two halo faces (minus / plus side) of 8 lines
 
@@ -1084,6 +1149,9 @@ struct KMicApply {
         const int o_hy = o_hx + kHalo;                    // y halo [(jl==7)*72 + il*9 + kl]
         const int o_zero = o_hy + kHalo;                  // 8 x -0.0
         int ti = 0, tj = 0, tk = 0;
+        ntiles = tile_count(ntiles, red.sc, PH == 1 ? 1 : (PH == 3 ? 2 : 0));
+        if ((int64_t)blockIdx.x >= mic_grid(ntiles)) return;
+        if (slot_base < 0) slot_base = (int)mic_grid(tile_count(-1, red.sc, 1));
         const int64_t idx = (int64_t)blockIdx.x * kTilesPerBlock + wave;
         const bool active = idx < ntiles;
         // the tile index and the stop flag are fetched together
@@ -1290,9 +1358,12 @@ struct KJacobiApply {
         const Reduce& red = kargs.red;
 
         if (red.sc->done) return;
+        nseg = seg_count(nseg, red.sc);
+        const int64_t nb = list_grid(nseg);
+        if ((int64_t)blockIdx.x >= nb) return;
         double v[1] = {0.0};
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += (int64_t)gridDim.x * blockDim.x) {
+             u += nb * blockDim.x) {
             const int64_t c0 = 8 * (int64_t)segs[u];
     #pragma unroll
             for (int m = 0; m < 8; ++m) {
@@ -1456,14 +1527,17 @@ void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, cons
 }
 
 void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
-                          const uint8_t* nup, float* pc, const Active& act) {
+                          const uint8_t* nup, float* pc, const Active& act, SolverScalars* sc) {
     if (p.precond == 2) return;   // Jacobi: built by setup
-    auto nb = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(n, 4096)); };
+    auto nb = [&](int w) {
+        const int64_t n = act.ntiles[w] >= 0 ? act.ntiles[w] : act.tile_cap[w];
+        return (unsigned)std::max<int64_t>(1, std::min<int64_t>(n, 4096));
+    };
     if (p.precond == 0) {
-        launch<KMicBuild>(s, dim3(nb(act.ntiles[0])), dim3(64), {p, A, diag, nup, pc, 0, act.tiles[0], act.ntiles[0]});
-        launch<KMicBuild>(s, dim3(nb(act.ntiles[1])), dim3(64), {p, A, diag, nup, pc, 1, act.tiles[1], act.ntiles[1]});
+        launch<KMicBuild>(s, dim3(nb(0)), dim3(64), {p, A, diag, nup, pc, 0, act.tiles[0], act.ntiles[0], sc});
+        launch<KMicBuild>(s, dim3(nb(1)), dim3(64), {p, A, diag, nup, pc, 1, act.tiles[1], act.ntiles[1], sc});
     } else {
-        launch<KMicBuild>(s, dim3(nb(act.ntiles[2])), dim3(64), {p, A, diag, nup, pc, -1, act.tiles[2], act.ntiles[2]});
+        launch<KMicBuild>(s, dim3(nb(2)), dim3(64), {p, A, diag, nup, pc, -1, act.tiles[2], act.ntiles[2], sc});
     }
 }
 
@@ -1485,33 +1559,31 @@ void launch_active_sets(hipStream_t s, int N, const uint8_t* fl8, int32_t* seg_f
     launch<KTileLists>(s, dim3(1), dim3(1024), {N, (const int32_t*)tile_any, tile_lists, sc});
 }
 
-static int list_blocks(int64_t n) {
-    const int64_t b = (n + kBlock - 1) / kBlock;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
-}
+// Launch geometry for a count that may be -1 (device-side): size by the cap;
+// the matching finalize then derives its partial count on the device.
+static int64_t seg_or_cap(const Active& a) { return a.nseg >= 0 ? a.nseg : a.seg_cap; }
+static int64_t tiles_or_cap(const Active& a, int w) { return a.ntiles[w] >= 0 ? a.ntiles[w] : a.tile_cap[w]; }
 
 void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, const float* pc,
                           const float* r, float* q, float* z, const Reduce& red,
                           const Active& act) {
+    const bool dev = act.nseg < 0;
     if (p.precond == 2) {
-        const int nb = list_blocks(act.nseg);
+        const int nb = (int)list_grid(seg_or_cap(act));
         launch<KJacobiApply>(s, dim3(nb), dim3(kBlock), {pc, r, z, act.segs, act.nseg, red});
-        launch_finalize(s, red, FIN_RHO, nb);
+        launch_finalize(s, red, FIN_RHO, dev ? PARTS_LIST : nb);
         return;
     }
-    auto blocks = [](int64_t nt) {
-        return (int)std::max<int64_t>(1, (nt + kTilesPerBlock - 1) / kTilesPerBlock);
-    };
     if (p.precond == 0) {
-        const int nr = blocks(act.ntiles[0]), nbk = blocks(act.ntiles[1]);
+        const int nr = (int)mic_grid(tiles_or_cap(act, 0)), nbk = (int)mic_grid(tiles_or_cap(act, 1));
         launch<KMicApply<0>>(s, dim3(nr), dim3(256), {p, fl8, pc, r, q, z, act.tiles[0], act.ntiles[0], red, 0});
         launch<KMicApply<1>>(s, dim3(nbk), dim3(256), {p, fl8, pc, r, q, z, act.tiles[1], act.ntiles[1], red, 0});
-        launch<KMicApply<2>>(s, dim3(nr), dim3(256), {p, fl8, pc, r, q, z, act.tiles[0], act.ntiles[0], red, nbk});
-        launch_finalize(s, red, FIN_RHO, nbk + nr);
+        launch<KMicApply<2>>(s, dim3(nr), dim3(256), {p, fl8, pc, r, q, z, act.tiles[0], act.ntiles[0], red, dev ? -1 : nbk});
+        launch_finalize(s, red, FIN_RHO, dev ? PARTS_MIC_RB : nbk + nr);
     } else {
-        const int nb = blocks(act.ntiles[2]);
+        const int nb = (int)mic_grid(tiles_or_cap(act, 2));
         launch<KMicApply<3>>(s, dim3(nb), dim3(256), {p, fl8, pc, r, q, z, act.tiles[2], act.ntiles[2], red, 0});
-        launch_finalize(s, red, FIN_RHO, nb);
+        launch_finalize(s, red, FIN_RHO, dev ? PARTS_MIC_ALL : nb);
     }
 }
 
@@ -1519,54 +1591,51 @@ void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const fl
                       const float* D, float* R, const Reduce& red, double tol, int stop_rule,
                       double* P64, int refine) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
-    const int nb = stream_blocks(nc);
+    const int nb = (int)list_grid(nc);
     launch<KResidual0>(s, dim3(nb), dim3(kBlock), {p, A, L, P, D, R, red, tol, stop_rule, P64, refine});
     launch_finalize(s, red, FIN_RESID0, nb, tol, stop_rule, p.N);
 }
 
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                      const float* x, float* y, const Reduce& red, const Active& act) {
-    // one segment per thread while the block partials fit, more beyond that
-    const int per = (int)std::max<int64_t>(1, (act.nseg + (int64_t)kBlock * kMaxPartials - 1) /
-                                                  ((int64_t)kBlock * kMaxPartials));
-    const int64_t per_block = (int64_t)kBlock * per;
-    const int64_t nb = std::max<int64_t>(1, (act.nseg + per_block - 1) / per_block);
-    launch<KSpmvDot>(s, dim3((unsigned)xcd_grid(nb)), dim3(kBlock), {p, A, L, x, y, act.segs, act.nseg, per, red});
-    launch_finalize(s, red, FIN_SIGMA, (int)nb);
+    const int64_t n = seg_or_cap(act);
+    const int64_t nb = spmv_grid(n);
+    launch<KSpmvDot>(s, dim3((unsigned)xcd_grid(nb)), dim3(kBlock), {p, A, L, x, y, act.segs, act.nseg, spmv_per_thread(n), red});
+    launch_finalize(s, red, FIN_SIGMA, act.nseg < 0 ? PARTS_SPMV : (int)nb);
 }
 
 void launch_axpy2(hipStream_t s, float* P, const float* S, float* R, const float* Z,
                   const Reduce& red, const Active& act) {
-    const int nb = list_blocks(act.nseg);
+    const int nb = (int)list_grid(seg_or_cap(act));
     launch<KAxpy2>(s, dim3(nb), dim3(kBlock), {P, S, R, Z, act.segs, act.nseg, red});
-    launch_finalize(s, red, FIN_RNORM, nb);
+    launch_finalize(s, red, FIN_RNORM, act.nseg < 0 ? PARTS_LIST : nb);
 }
 
 void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                             float* P, double* P64, const float* D, float* R, const Reduce& red,
                             const Active& act) {
-    const int nb = list_blocks(act.nseg);
+    const int nb = (int)list_grid(seg_or_cap(act));
     launch<KFlush>(s, dim3(nb), dim3(kBlock), {A, P, P64, act.segs, act.nseg, (const SolverScalars*)red.sc});
     launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, (const float*)P, (const double*)P64, D, R, act.segs, act.nseg, red, 0});
-    launch_finalize(s, red, FIN_TRUE, nb);
+    launch_finalize(s, red, FIN_TRUE, act.nseg < 0 ? PARTS_LIST : nb);
 }
 
 void launch_refine_finish(hipStream_t s, const uint8_t* A, float* P, const double* P64,
-                          const Active& act) {
-    launch<KRefineFinish>(s, dim3(list_blocks(act.nseg)), dim3(kBlock), {A, P, P64, act.segs, act.nseg});
+                          const Active& act, const SolverScalars* sc) {
+    launch<KRefineFinish>(s, dim3((unsigned)list_grid(seg_or_cap(act))), dim3(kBlock), {A, P, P64, act.segs, act.nseg, sc});
 }
 
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                                const float* P, const float* D, const Reduce& red,
                                const Active& act) {
-    const int nb = list_blocks(act.nseg);
+    const int nb = (int)list_grid(seg_or_cap(act));
     launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, P, (const double*)nullptr, D, (float*)nullptr, act.segs, act.nseg, red, 1});
-    launch_finalize(s, red, FIN_REPORT, nb);
+    launch_finalize(s, red, FIN_REPORT, act.nseg < 0 ? PARTS_LIST : nb);
 }
 
 void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,
                      const Active& act) {
-    launch<KUpdateS>(s, dim3(list_blocks(act.nseg)), dim3(kBlock), {Z, S, act.segs, act.nseg, (const SolverScalars*)red.sc, first});
+    launch<KUpdateS>(s, dim3((unsigned)list_grid(seg_or_cap(act))), dim3(kBlock), {Z, S, act.segs, act.nseg, (const SolverScalars*)red.sc, first});
 }
 
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,

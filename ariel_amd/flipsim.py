@@ -25,14 +25,19 @@ def _p(a):
 
 
 class FlipSim:
-    def __init__(self, desc):
+    def __init__(self, desc, _handle=None):
         self._lib = _abi.load()
-        self._h = ctypes.c_void_p()
         self.desc = desc
+        self.n = desc.dims[0]
+        self._owned = _handle is None
+        if _handle is not None:
+            # a view of a handle owned elsewhere (a batch tank): never destroyed here
+            self._h = ctypes.c_void_p(_handle)
+            return
+        self._h = ctypes.c_void_p()
         rc = self._lib.ariel_create(ctypes.byref(desc), ctypes.byref(self._h))
         if rc != 0:
             raise ArielError(rc, self._lib.ariel_last_error(None).decode())
-        self.n = desc.dims[0]
 
     # ------------------------------------------------------------ lifecycle
     def _check(self, rc):
@@ -40,9 +45,9 @@ class FlipSim:
             raise ArielError(rc, self._lib.ariel_last_error(self._h).decode())
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             self._lib.ariel_destroy(self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -145,6 +145,20 @@ def tank(n, frac, seed=2, **overrides):
     return d
 
 
+def batched_tanks(ntanks=64, seed=2, sizes=(48, 64, 80, 96), lo=0.2, hi=0.6, **overrides):
+    """The batched workload: ntanks dam breaks, edges drawn uniformly from `sizes`
+    and liquid-block fractions uniformly from [lo, hi] per axis (numpy generator
+    seeded with `seed`), each tank seeded with jitter seed `seed`."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(ntanks):
+        n = int(rng.choice(sizes))
+        frac = tuple(float(x) for x in rng.uniform(lo, hi, size=3))
+        out.append(tank(n, frac, seed=seed, **overrides))
+    return out
+
+
 def copy(d):
     out = SceneDesc()
     out_bytes = bytes(d)

@@ -276,6 +276,35 @@ int ariel_load_state(ariel_sim* h, const char* path);
  * 101-111), as a raw little-endian float32 volume with a text header. */
 int ariel_export_levelset(ariel_sim* h, const char* path);
 
+/* ---------------------------------------------------------------- batches
+ * Independent tanks stepped together (no counterpart in the reference, which
+ * runs one FlipSim per process). Every stage of a step is one kernel launch
+ * over all tanks; the launch sequence is replayed from per-tank argument tables
+ * and can be captured as HIP graphs. Each tank's results equal those of
+ * stepping it alone with ariel_step. Tanks must share preconditioner, splat,
+ * refinement period, resampling switch and iteration limit; grid sizes and
+ * scenes may differ. */
+typedef struct ariel_batch ariel_batch;
+
+int ariel_batch_create(const ariel_scene_desc* descs, int32_t ntanks, ariel_batch** out);
+int ariel_batch_destroy(ariel_batch* b);
+const char* ariel_batch_last_error(const ariel_batch* b);
+int ariel_batch_size(ariel_batch* b, int32_t* ntanks);
+/* The tank's handle for the per-handle accessors (owned by the batch). */
+int ariel_batch_tank(ariel_batch* b, int32_t i, ariel_sim** tank);
+/* ariel_init on every tank. */
+int ariel_batch_init(ariel_batch* b);
+/* Autotuner: for each grid size, times P2G with face-column tile edges
+ * {4, 8, 16, 32} on one tank of that size and keeps the fastest for all. */
+int ariel_batch_autotune(ariel_batch* b);
+int ariel_batch_get_tuning(ariel_batch* b, int32_t i, int32_t* p2g_tile);
+/* Capture the replayed launches as HIP graphs (off by default). */
+int ariel_batch_set_graphs(ariel_batch* b, int32_t enable);
+/* nsteps batched steps (one launch per stage for all tanks). */
+int ariel_batch_step(ariel_batch* b, int32_t nsteps);
+/* nsteps steps as a loop of ariel_step over the tanks: the baseline. */
+int ariel_batch_step_sequential(ariel_batch* b, int32_t nsteps);
+
 /* Self-check of the device division routines against IEEE division on the
  * current device (no counterpart in the reference: it guards the bit-exactness
  * of Smooth and Sharpen inside the kernels). For every float x with

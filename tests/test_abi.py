@@ -136,3 +136,20 @@ int main() {
         if not os.path.exists("/dev/kfd"):
             out = subprocess.run([exe], capture_output=True, text=True, timeout=60).stdout
             assert out.startswith("error -2")
+
+
+def test_batch_rejects_mixed_solver_settings():
+    """Tanks of a batch share one launch sequence: mixed preconditioners are
+    rejected before any device work."""
+    import ctypes
+    from ariel_amd import _abi, scenes
+    lib = _abi.load()
+    a = scenes.tank(16, (0.3, 0.5, 0.5))
+    b = scenes.tank(24, (0.3, 0.5, 0.5))
+    b.precond = _abi.PRECOND_JACOBI
+    arr = (_abi.SceneDesc * 2)(a, b)
+    h = ctypes.c_void_p()
+    rc = lib.ariel_batch_create(arr, 2, ctypes.byref(h))
+    assert rc == _abi.ERR_INVALID
+    assert b"share" in lib.ariel_batch_last_error(None)
+    assert lib.ariel_batch_create(arr, 0, ctypes.byref(h)) == _abi.ERR_INVALID
