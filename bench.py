@@ -47,6 +47,10 @@ def parse():
     ap.add_argument("--precond", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--batched", action="store_true",
+                    help="the batched workload instead: 64 tanks (edges from {48,64,80,96}, "
+                         "seed 2), aggregate tank-steps/s of batched steps vs a per-tank loop")
+    ap.add_argument("--tanks", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (a plain copy "
                          "kernel of known traffic, to scale FETCH_SIZE/WRITE_SIZE)")
@@ -110,10 +114,71 @@ def cpu_baseline(sim, desc, gpu_iters, cpu_iters):
     }
 
 
+def batched(args):
+    """64 independent dam-break tanks: batched steps (one launch per stage for all
+    tanks, HIP graphs, autotuned P2G tiles) against the same tanks stepped by a
+    loop of single-tank steps. Tank-steps/s over the timed steps."""
+    from ariel_amd import FlipBatch, scenes
+
+    descs = scenes.batched_tanks(args.tanks, seed=2)
+    sizes = sorted({d.dims[0] for d in descs})
+    b = FlipBatch(descs)
+    b.Init()
+    nparticles = sum(b.tank(i).num_particles for i in range(len(descs)))
+    b.autotune()
+    tiles = {int(descs[i].dims[0]): b.tuning(i) for i in range(len(descs))}
+    log(f"batched: {len(descs)} tanks, sizes {sizes}, {nparticles} particles, P2G tiles {tiles}")
+    b.set_graphs(True)
+    b.Step(args.warmup)
+    t0 = time.perf_counter()
+    b.Step(args.steps)
+    t_batch = time.perf_counter() - t0
+    iters = [b.tank(i).stats().iterations for i in range(len(descs))]
+    conv = all(b.tank(i).stats().solve_status == 0 for i in range(len(descs)))
+    log(f"batched: {args.steps} steps in {t_batch:.3f} s")
+    b.step_sequential(args.warmup)
+    t0 = time.perf_counter()
+    b.step_sequential(args.steps)
+    t_seq = time.perf_counter() - t0
+    log(f"sequential loop: {args.steps} steps in {t_seq:.3f} s")
+    b.close()
+    tank_steps = len(descs) * args.steps
+    out = {
+        "metric": "batched tank-steps/sec (64 tanks, edges 48-96, to 1e-6 residual)",
+        "value": tank_steps / t_batch,
+        "unit": "tank-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_batch * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(descs)} dam-break tanks, edges from {{48,64,80,96}} and "
+                               "block fractions U[0.2,0.6] (seed 2), 8 ppc",
+                   "tanks": len(descs), "particles": int(nparticles), "p2g_tiles": tiles,
+                   "graphs": True},
+        "sequential_loop": {"value": tank_steps / t_seq, "unit": "tank-steps/s",
+                            "ms_per_step": t_seq * 1e3 / args.steps},
+        "speedup_vs_sequential": t_seq / t_batch,
+        "pcg_iterations_last_step": {"min": int(min(iters)), "max": int(max(iters))},
+        "pcg_all_converged": bool(conv),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.batched:
+        if args.gpus != 1:
+            sys.stderr.write("bench.py: this build runs on one GPU only; --gpus must be 1\n")
+            sys.exit(2)
+        batched(args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
