@@ -61,7 +61,8 @@ class StepStats(ctypes.Structure):
                 ("residual_norm", ctypes.c_double), ("true_residual_norm", ctypes.c_double),
                 ("max_divergence", ctypes.c_double), ("max_density", ctypes.c_double),
                 ("stage_ms", ctypes.c_double * NUM_STAGES), ("step_ms", ctypes.c_double),
-                ("solve_ms", ctypes.c_double)]
+                ("solve_ms", ctypes.c_double), ("precond_fallback", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
 
 
 class KernelStats(ctypes.Structure):
@@ -108,6 +109,8 @@ PROTOTYPES = {
     "ariel_export_levelset": (_I32, [_P, ctypes.c_char_p]),
     "ariel_selftest_division": (_I32, [ctypes.c_int32, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, ctypes.POINTER(ctypes.c_int64)]),
+    "ariel_set_learned_weights": (_I32, [_P, ctypes.c_void_p, ctypes.c_int64]),
+    "ariel_solve_rhs": (_I32, [_P, _PF, _PF, ctypes.POINTER(StepStats)]),
     "ariel_batch_create": (_I32, [ctypes.POINTER(SceneDesc), _I32, ctypes.POINTER(_P)]),
     "ariel_batch_destroy": (_I32, [_P]),
     "ariel_batch_last_error": (ctypes.c_char_p, [_P]),

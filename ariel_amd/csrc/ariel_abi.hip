@@ -75,6 +75,11 @@ struct ariel_sim {
     int32_t *seg_flag = nullptr, *seg_scan = nullptr, *seg_list = nullptr;
     int32_t *tile_any = nullptr, *tile_lists = nullptr;
     Active act;
+    // learned preconditioner: weights, activations, r_old (allocated on load)
+    LearnedNet net;
+    uint16_t* net_mem = nullptr;
+    float* R_old = nullptr;
+    bool net_ready = false;
     double* partials = nullptr;
     uint32_t* counter = nullptr;
     SolverScalars* sc = nullptr;
@@ -107,7 +112,8 @@ struct ariel_sim {
         p.frame = frame;
         p.mic_tau = d.mic_tau;
         p.mic_sigma = d.mic_sigma;
-        p.precond = d.precond;
+        // the learned preconditioner's fallback and set-up are the tiled MIC(0)
+        p.precond = d.precond == ARIEL_PRECOND_LEARNED ? ARIEL_PRECOND_MIC0_TILED : d.precond;
         p.p2g_tile = p2g_tile;
         return p;
     }
@@ -254,13 +260,13 @@ int validate(const ariel_scene_desc* d, std::string& msg) {
             msg = "unknown liquid kind"; return ARIEL_ERR_INVALID;
         }
     if (d->wall < 0 || 2 * d->wall >= n) { msg = "wall thickness out of range"; return ARIEL_ERR_INVALID; }
-    if (d->precond == ARIEL_PRECOND_MIC0 || d->precond == ARIEL_PRECOND_REFERENCE ||
-        d->precond == ARIEL_PRECOND_LEARNED) {
-        msg = "preconditioner not available on the GPU path (use MIC0_TILED, MIC0_BLOCK or JACOBI)";
+    if (d->precond == ARIEL_PRECOND_MIC0 || d->precond == ARIEL_PRECOND_REFERENCE) {
+        msg = "preconditioner not available on the GPU path (use MIC0_TILED, MIC0_BLOCK, JACOBI "
+              "or LEARNED)";
         return ARIEL_ERR_UNSUPPORTED;
     }
     if (d->precond != ARIEL_PRECOND_MIC0_TILED && d->precond != ARIEL_PRECOND_MIC0_BLOCK &&
-        d->precond != ARIEL_PRECOND_JACOBI) {
+        d->precond != ARIEL_PRECOND_JACOBI && d->precond != ARIEL_PRECOND_LEARNED) {
         msg = "unknown preconditioner"; return ARIEL_ERR_INVALID;
     }
     if ((d->precond == ARIEL_PRECOND_MIC0_TILED || d->precond == ARIEL_PRECOND_MIC0_BLOCK) &&
@@ -447,7 +453,8 @@ int prepare_solve(ariel_sim* h) {
     hipStream_t s = h->stream;
     launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
     launch_fluid_segments(s, h->N, h->A, h->fl8);
-    launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every);
+    launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every,
+                       h->d.precond == ARIEL_PRECOND_LEARNED ? 1 : 0);
     launch_active_sets(s, h->N, h->fl8, h->seg_flag, h->seg_scan, h->seg_list, h->tile_any,
                        h->tile_lists, h->scan_sums, h->sc);
     TRY(check_launch(h, "active sets"));
@@ -463,7 +470,10 @@ int prepare_solve(ariel_sim* h) {
 // below leaves P and R zero off-FLUID and Z zero everywhere; S and Z are then
 // written on every active segment before they are read, and values off the
 // active sets are never read as FLUID values.
+int stage_solve_learned(ariel_sim* h);
+
 int stage_solve(ariel_sim* h) {
+    if (h->d.precond == ARIEL_PRECOND_LEARNED) return stage_solve_learned(h);
     const Params p = h->params();
     const Reduce red = h->reduce();
     hipStream_t s = h->stream;
@@ -515,6 +525,89 @@ int stage_solve(ariel_sim* h) {
     h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
     h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
     // true residual of the final pressure, for the report only
+    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, act);
+    TRY(check_launch(h, "true residual"));
+    TRY(read_scalars(h));
+    h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
+    return ARIEL_OK;
+}
+
+// Flexible PCG (Notay 2000; Polak-Ribiere beta) with the learned network as
+// preconditioner and the tiled MIC(0) as fallback: over every window of
+// kFlexWindow iterations the residual must fall by 5% per iteration on average,
+// and every z of the network must be a descent direction (z.r > 0); otherwise
+// the solve switches to MIC(0) for good and restarts its search direction. The
+// network and MIC(0) kernels are both queued each iteration; the one not in use
+// returns at once. Once the host has seen the switch (at a readback), it
+// queues the MIC(0) kernels only.
+void learned_precond(ariel_sim* h, const Params& p, const Reduce& red, bool mic_only) {
+    hipStream_t s = h->stream;
+    if (!mic_only) {
+        launch_learned_apply(s, h->net, h->N, h->R, h->A, h->diag, h->Z, h->sc, h->act.nseg);
+        launch_pr_dot(s, h->Z, h->R, h->R_old, red, h->act, true);
+    }
+    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, h->act);
+    launch_pr_dot(s, h->Z, h->R, h->R_old, red, h->act, false);
+}
+
+int stage_solve_learned(ariel_sim* h) {
+    if (!h->net_ready)
+        return fail(h, ARIEL_ERR_STATE, "learned preconditioner: no weights (ariel_set_learned_weights)");
+    const Params p = h->params();
+    const Reduce red = h->reduce();
+    hipStream_t s = h->stream;
+    HIPCHK(h, hipEventRecord(h->solve_t0, s));
+    TRY(prepare_solve(h));
+    const Active& act = h->act;
+    const int refine = h->d.refine_every > 0;
+    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
+                     h->P64, refine);
+    if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
+    launch_fill_f32(s, h->Z, h->nc, 0.0f);
+    launch_fill_f32(s, h->R_old, h->nc, 0.0f);
+    learned_precond(h, p, red, false);
+    launch_update_s(s, h->Z, h->S, red, 1, act);
+    TRY(check_launch(h, "learned solve setup"));
+    bool on_mic = false;
+    int queued = 0;
+    const int k = h->d.refine_every;
+    int chunk = std::max(4, h->last_iters - 3);
+    for (;;) {
+        const int n = std::min(chunk, h->d.max_iters - queued);
+        for (int j = 0; j < n; ++j) {
+            const int it = queued + j + 1;
+            {
+                Prof prof(h, ARIEL_KERNEL_SPMV);
+                launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, act);
+            }
+            {
+                Prof prof(h, ARIEL_KERNEL_AXPY);
+                launch_copy_list(s, h->R, h->R_old, act, h->sc);
+                launch_axpy2(s, h->P, h->S, h->R, h->Z, red, act);
+            }
+            if (k > 0 && it % k == 0)
+                launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, act);
+            {
+                Prof prof(h, ARIEL_KERNEL_PRECOND);
+                learned_precond(h, p, red, on_mic);
+            }
+            launch_update_s(s, h->Z, h->S, red, 0, act);
+        }
+        TRY(check_launch(h, "learned pcg"));
+        queued += n;
+        TRY(read_scalars(h));
+        on_mic = on_mic || h->sc_host->use_mic;
+        if (h->sc_host->done || queued >= h->d.max_iters) break;
+        chunk = 4;
+    }
+    if (refine) launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
+    HIPCHK(h, hipEventRecord(h->solve_t1, s));
+    h->last_iters = h->sc_host->it;
+    h->stats.iterations = h->sc_host->it;
+    h->stats.solve_status = h->sc_host->status;
+    h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
+    h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
+    h->stats.precond_fallback = h->sc_host->use_mic;
     launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, act);
     TRY(check_launch(h, "true residual"));
     TRY(read_scalars(h));
@@ -726,7 +819,7 @@ int ariel_destroy(ariel_sim* h) {
                     (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
                     (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc,
                     (void*)h->seg_flag, (void*)h->seg_scan, (void*)h->seg_list, (void*)h->tile_any,
-                    (void*)h->tile_lists})
+                    (void*)h->tile_lists, (void*)h->net_mem, (void*)h->R_old})
         hipFree(p);
     for (int c = 0; c < 3; ++c) {
         hipFree(h->u[c]);
@@ -1020,7 +1113,20 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
     HIPCHK(h, hipMemcpyAsync(h->R, r, h->nc * sizeof(float), hipMemcpyHostToDevice, h->stream));
     TRY(prepare_solve(h));
     launch_fill_f32(h->stream, h->Z, h->nc, 0.0f);
-    launch_precond_apply(h->stream, p, h->fl8, h->pc, h->R, h->Q, h->Z, h->reduce(), h->act);
+    if (h->d.precond == ARIEL_PRECOND_LEARNED) {
+        if (!h->net_ready) return fail(h, ARIEL_ERR_STATE, "learned preconditioner: no weights");
+        // the network's input scale reads ||r||^2 over the FLUID cells
+        std::vector<uint8_t> flags(h->nc);
+        HIPCHK(h, hipMemcpyAsync(flags.data(), h->A, h->nc, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        double r2 = 0.0;
+        for (int64_t c = 0; c < h->nc; ++c)
+            if (flags[c] == ARIEL_FLUID) r2 += (double)r[c] * (double)r[c];
+        HIPCHK(h, hipMemcpyAsync(&h->sc->rnorm2, &r2, sizeof(double), hipMemcpyHostToDevice, h->stream));
+        launch_learned_apply(h->stream, h->net, h->N, h->R, h->A, h->diag, h->Z, h->sc, h->act.nseg);
+    } else {
+        launch_precond_apply(h->stream, p, h->fl8, h->pc, h->R, h->Q, h->Z, h->reduce(), h->act);
+    }
     TRY(check_launch(h, "preconditioner"));
     HIPCHK(h, hipMemcpyAsync(z, h->Z, h->nc * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1163,6 +1269,67 @@ int ariel_export_levelset(ariel_sim* h, const char* path) {
     ok = ok && std::fwrite(phi.data(), sizeof(float), phi.size(), f) == phi.size();
     std::fclose(f);
     if (!ok) return fail(h, ARIEL_ERR_IO, std::string("write failed: ") + path);
+    return ARIEL_OK;
+}
+
+int ariel_solve_rhs(ariel_sim* h, const float* rhs, float* x, ariel_step_stats* stats) {
+    TRY(check_handle(h));
+    if (!rhs || !x) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    HIPCHK(h, hipMemcpyAsync(h->D, rhs, h->nc * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    launch_fill_f32(h->stream, h->P, h->nc, 0.0f);
+    TRY(stage_solve(h));
+    HIPCHK(h, hipMemcpyAsync(x, h->P, h->nc * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->solve_t0, h->solve_t1) == hipSuccess) h->stats.solve_ms = ms;
+    if (stats) *stats = h->stats;
+    return ARIEL_OK;
+}
+
+int ariel_set_learned_weights(ariel_sim* h, const void* blob, int64_t bytes) {
+    TRY(check_handle(h));
+    if (!blob || bytes < 24) return fail(h, ARIEL_ERR_INVALID, "learned weights: short blob");
+    const unsigned char* b = static_cast<const unsigned char*>(blob);
+    if (std::memcmp(b, "ARIELNN1", 8) != 0) return fail(h, ARIEL_ERR_INVALID, "learned weights: bad magic");
+    int32_t layers;
+    int64_t count;
+    std::memcpy(&layers, b + 8, 4);
+    std::memcpy(&count, b + 16, 8);
+    int64_t off[kLearnedLayers];
+    const int64_t want = learned_layout(off);
+    if (layers != kLearnedLayers || count != want || bytes != 24 + 2 * count)
+        return fail(h, ARIEL_ERR_INVALID, "learned weights: layout does not match this build");
+    if (h->N % 16 != 0)
+        return fail(h, ARIEL_ERR_UNSUPPORTED, "learned preconditioner needs N divisible by 16");
+    if (!h->net_mem) {
+        const int64_t n3 = h->nc;
+        // weights, then e0 16n3, e1 4n3, e2 0.75n3, e3 0.125n3, p 2n3, t 16n3, d2 0.75n3,
+        // d1 4n3, d0 16n3 (elements, n3 = N^3; rounded up per buffer)
+        const int64_t sizes[] = {want, 16 * n3, 4 * n3, (3 * n3) / 4 + 64, n3 / 8 + 64, 2 * n3,
+                                 16 * n3, (3 * n3) / 4 + 64, 4 * n3, 16 * n3};
+        int64_t total = 0, pos[10];
+        for (int i = 0; i < 10; ++i) {
+            pos[i] = total;
+            total += (sizes[i] + 127) / 128 * 128;
+        }
+        TRY(dalloc(h, &h->net_mem, total));
+        TRY(dalloc(h, &h->R_old, h->nc));
+        uint16_t* m = h->net_mem;
+        h->net.weights = m + pos[0];
+        h->net.e0 = m + pos[1];
+        h->net.e1 = m + pos[2];
+        h->net.e2 = m + pos[3];
+        h->net.e3 = m + pos[4];
+        h->net.p = m + pos[5];
+        h->net.t = m + pos[6];
+        h->net.d2 = m + pos[7];
+        h->net.d1 = m + pos[8];
+        h->net.d0 = m + pos[9];
+        for (int i = 0; i < kLearnedLayers; ++i) h->net.off[i] = off[i];
+    }
+    HIPCHK(h, hipMemcpyAsync(h->net_mem, b + 24, (size_t)(2 * count), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->net_ready = true;
     return ARIEL_OK;
 }
 
@@ -1503,6 +1670,9 @@ extern "C" {
 int ariel_batch_create(const ariel_scene_desc* descs, int32_t ntanks, ariel_batch** out) {
     if (!descs || !out || ntanks <= 0) return bfail(nullptr, ARIEL_ERR_INVALID, "batch_create: bad arguments");
     *out = nullptr;
+    if (descs[0].precond == ARIEL_PRECOND_LEARNED)
+        return bfail(nullptr, ARIEL_ERR_UNSUPPORTED,
+                     "batch_create: the learned preconditioner runs on single tanks only");
     for (int32_t t = 1; t < ntanks; ++t) {
         const ariel_scene_desc& a = descs[0];
         const ariel_scene_desc& b = descs[t];

@@ -45,7 +45,10 @@ struct SolverScalars {
     int32_t pad;
     int64_t nseg;          // active z-segments (counted by the active-set build)
     int32_t ntile[3];      // active tiles: red, black, all
-    int32_t pad2;
+    int32_t flex;          // flexible PCG (learned preconditioner)
+    int32_t use_mic;       // flexible PCG has fallen back to MIC(0)
+    int32_t restart;       // next search direction restarts (beta = 0)
+    double rnorm2_prev;    // ||r||^2 of the previous iteration (fallback test)
 };
 
 // Cells the pressure solve touches. A z-segment is 8 cells (cells 8s..8s+7) and
@@ -94,6 +97,23 @@ struct Reduce {
 
 // Exhaustive division self-check (selftest.hip); lo, hi non-negative.
 hipError_t run_division_selftest(int which, float a, float lo, float hi, int64_t* mismatches);
+
+// Learned preconditioner (learned.hip): packed BF16 weights and activation
+// buffers of the 4-level network.
+constexpr int kLearnedLayers = 11;
+struct LearnedNet {
+    const uint16_t* weights = nullptr;
+    int64_t off[kLearnedLayers] = {};
+    uint16_t *e0 = nullptr, *e1 = nullptr, *e2 = nullptr, *e3 = nullptr;
+    uint16_t *p = nullptr, *t = nullptr, *d2 = nullptr, *d1 = nullptr, *d0 = nullptr;
+};
+// Offsets of the layers in the packed weights (BF16 elements); returns the total.
+int64_t learned_layout(int64_t* off);
+// z = net(r) on the FLUID cells (0 elsewhere); skipped when sc says done or
+// fallen back to MIC(0). nseg = active segments (-1: from sc) for the scale.
+void launch_learned_apply(hipStream_t s, const LearnedNet& net, int N, const float* r,
+                          const uint8_t* A, const float* diag, float* z, const SolverScalars* sc,
+                          int64_t nseg);
 
 // ---------------------------------------------------------------- launching
 // Every kernel is a functor K { static constexpr int kThreads; struct Args;
@@ -213,7 +233,14 @@ void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red,
                      const Active& act);
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                        const float* x, float* y);
-void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every);
+void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every,
+                        int flex = 0);
+// Flexible PCG helpers: dst = src on the active segments; z.r and z.r_old with
+// the Polak-Ribiere finalize (check: only the descent test of the network's z).
+void launch_copy_list(hipStream_t s, const float* src, float* dst, const Active& act,
+                      const SolverScalars* sc);
+void launch_pr_dot(hipStream_t s, const float* z, const float* r, const float* r_old,
+                   const Reduce& red, const Active& act, bool check);
 // ||b - A p||^2 in FP64 into sc->tmp (R untouched).
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                                const float* P, const float* D, const Reduce& red,

@@ -175,6 +175,7 @@ struct KSolverInit {
         SolverScalars* sc;
         int max_iters;
         int refine_every;
+        int flex;
     };
     static __device__ void run(const Args& kargs) {
         SolverScalars* sc = kargs.sc;
@@ -182,6 +183,10 @@ struct KSolverInit {
         int refine_every = kargs.refine_every;
 
         if (threadIdx.x != 0) return;
+        sc->flex = kargs.flex;
+        sc->use_mic = 0;
+        sc->restart = 0;
+        sc->rnorm2_prev = 0.0;
         sc->bnorm2 = sc->rnorm2 = sc->thr = sc->rho = sc->sigma = 0.0;
         sc->alpha = sc->beta = sc->tmp = 0.0;
         sc->done = 0;
@@ -688,7 +693,11 @@ __device__ __forceinline__ int64_t parts_from_counts(int code, const SolverScala
     }
 }
 
-enum { FIN_RESID0 = 0, FIN_SIGMA = 1, FIN_RNORM = 2, FIN_TRUE = 3, FIN_REPORT = 4, FIN_RHO = 5 };
+enum { FIN_RESID0 = 0, FIN_SIGMA = 1, FIN_RNORM = 2, FIN_TRUE = 3, FIN_REPORT = 4, FIN_RHO = 5,
+       FIN_PR = 6, FIN_PR_CHECK = 7 };
+// Flexible PCG falls back to MIC(0) when, over a window of this many
+// iterations, ||r|| fell by less than 5% per iteration on average.
+constexpr int kFlexWindow = 4;
 
 // One block: sum the partials of the preceding reducing kernel in a fixed order
 // and update the scalars. Gated exactly like its producer, so a skipped producer
@@ -712,10 +721,13 @@ struct KFinalize {
         int N = kargs.N;
 
         SolverScalars* sc = red.sc;
-        if ((kind == FIN_SIGMA || kind == FIN_RNORM || kind == FIN_RHO) && sc->done) return;
+        if ((kind == FIN_SIGMA || kind == FIN_RNORM || kind == FIN_RHO || kind == FIN_PR ||
+             kind == FIN_PR_CHECK) && sc->done) return;
         if (kind == FIN_TRUE && (sc->done || !refine_due(sc))) return;
+        if (kind == FIN_RHO && sc->flex) return;          // flexible PCG sets beta itself
+        if (kind == FIN_PR_CHECK && (!sc->flex || sc->use_mic)) return;
         if (nparts < 0) nparts = (int)parts_from_counts(nparts, sc);
-        const int NV = kind == FIN_RESID0 ? 2 : 1;
+        const int NV = (kind == FIN_RESID0 || kind == FIN_PR || kind == FIN_PR_CHECK) ? 2 : 1;
         double a0 = 0.0, a1 = 0.0;
         for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
             a0 += red.partials[b * NV];
@@ -740,6 +752,7 @@ struct KFinalize {
             case FIN_RESID0:
                 sc->bnorm2 = s0;
                 sc->rnorm2 = s1;
+                sc->rnorm2_prev = s1;
                 if (stop_rule == 1) sc->thr = (double)(1.0e-2f * (float)(N * N * N));
                 else sc->thr = tol * tol * s0;
                 if (s1 <= sc->thr) {
@@ -762,6 +775,15 @@ struct KFinalize {
             case FIN_RNORM:
                 sc->rnorm2 = s0;
                 sc->it += 1;
+                if (sc->flex && !sc->use_mic && sc->it % kFlexWindow == 0) {
+                    // average reduction per iteration over the window below 5%
+                    const double need = 0.9025 * 0.9025 * 0.9025 * 0.9025;   // 0.95^(2*4)
+                    if (!(s0 <= need * sc->rnorm2_prev)) {
+                        sc->use_mic = 1;
+                        sc->restart = 1;
+                    }
+                    sc->rnorm2_prev = s0;
+                }
                 if (!refine_due(sc)) decide(sc);
                 break;
             case FIN_TRUE:
@@ -773,6 +795,26 @@ struct KFinalize {
                 break;
             case FIN_RHO:
                 finish_rho(sc, s0);
+                break;
+            case FIN_PR_CHECK:
+                // the network's z must be a descent direction (z.r > 0); if not,
+                // fall back to MIC(0) before z is used
+                if (!(s0 > 0.0) || !finite(s0)) {
+                    sc->use_mic = 1;
+                    sc->restart = 1;
+                }
+                break;
+            case FIN_PR:
+                // Polak-Ribiere: beta = z.(r - r_old) / rho_old
+                if (!finite(s0) || !finite(s1)) {
+                    sc->status = 3;
+                    sc->done = 1;
+                    break;
+                }
+                if (sc->it == 0 || sc->restart) sc->beta = 0.0;
+                else sc->beta = (s0 - s1) / sc->rho;
+                sc->rho = s0;
+                sc->restart = 0;
                 break;
         }
     }
@@ -1151,6 +1193,7 @@ struct KMicApply {
         int ti = 0, tj = 0, tk = 0;
         ntiles = tile_count(ntiles, red.sc, PH == 1 ? 1 : (PH == 3 ? 2 : 0));
         if ((int64_t)blockIdx.x >= mic_grid(ntiles)) return;
+        if (red.sc->flex && !red.sc->use_mic) return;   // flexible PCG still on the network
         if (slot_base < 0) slot_base = (int)mic_grid(tile_count(-1, red.sc, 1));
         const int64_t idx = (int64_t)blockIdx.x * kTilesPerBlock + wave;
         const bool active = idx < ntiles;
@@ -1338,6 +1381,65 @@ struct KMicApply {
     }
 };
 
+// r_old = r on the active segments (flexible PCG).
+struct KCopyList {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const float* src;
+        float* dst;
+        const int32_t* segs;
+        int64_t nseg;
+        const SolverScalars* sc;
+    };
+    static __device__ void run(const Args& a) {
+        if (a.sc->done) return;
+        const int64_t nseg = seg_count(a.nseg, a.sc);
+        const int64_t nb = list_grid(nseg);
+        if ((int64_t)blockIdx.x >= nb) return;
+        const float4* __restrict__ S4 = reinterpret_cast<const float4*>(a.src);
+        float4* __restrict__ D4 = reinterpret_cast<float4*>(a.dst);
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+             u += nb * blockDim.x) {
+            const int64_t c = 2 * (int64_t)a.segs[u];
+            D4[c] = S4[c];
+            D4[c + 1] = S4[c + 1];
+        }
+    }
+};
+
+// z.r and z.r_old on the active segments (flexible PCG's Polak-Ribiere beta).
+struct KPRDot {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const float* z;
+        const float* r;
+        const float* r_old;
+        const int32_t* segs;
+        int64_t nseg;
+        Reduce red;
+        int check;   // 1: only while the network is in use
+    };
+    static __device__ void run(const Args& a) {
+        const SolverScalars* sc = a.red.sc;
+        if (sc->done || (a.check && sc->use_mic)) return;
+        const int64_t nseg = seg_count(a.nseg, sc);
+        const int64_t nb = list_grid(nseg);
+        if ((int64_t)blockIdx.x >= nb) return;
+        double v[2] = {0.0, 0.0};
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+             u += nb * blockDim.x) {
+            const int64_t c0 = 8 * (int64_t)a.segs[u];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const double zz = a.z[c0 + m];
+                v[0] += zz * (double)a.r[c0 + m];
+                v[1] += zz * (double)a.r_old[c0 + m];
+            }
+        }
+        block_partials<2>(v, a.red.partials, blockIdx.x);
+    }
+};
+
 // Jacobi: z = r * (1/diag), rho = z.r over the active segments
 struct KJacobiApply {
     static constexpr int kThreads = kBlock;
@@ -1516,8 +1618,8 @@ void launch_finalize(hipStream_t s, const Reduce& red, int kind, int nparts, dou
     launch<KFinalize>(s, dim3(1), dim3(256), {red, kind, nparts, tol, stop_rule, N});
 }
 
-void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every) {
-    launch<KSolverInit>(s, dim3(1), dim3(64), {sc, max_iters, refine_every});
+void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every, int flex) {
+    launch<KSolverInit>(s, dim3(1), dim3(64), {sc, max_iters, refine_every, flex});
 }
 
 void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
@@ -1636,6 +1738,18 @@ void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A,
 void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,
                      const Active& act) {
     launch<KUpdateS>(s, dim3((unsigned)list_grid(seg_or_cap(act))), dim3(kBlock), {Z, S, act.segs, act.nseg, (const SolverScalars*)red.sc, first});
+}
+
+void launch_copy_list(hipStream_t s, const float* src, float* dst, const Active& act,
+                      const SolverScalars* sc) {
+    launch<KCopyList>(s, dim3((unsigned)list_grid(seg_or_cap(act))), dim3(kBlock), {src, dst, act.segs, act.nseg, sc});
+}
+
+void launch_pr_dot(hipStream_t s, const float* z, const float* r, const float* r_old,
+                   const Reduce& red, const Active& act, bool check) {
+    const int nb = (int)list_grid(seg_or_cap(act));
+    launch<KPRDot>(s, dim3(nb), dim3(kBlock), {z, r, r_old, act.segs, act.nseg, red, check ? 1 : 0});
+    launch_finalize(s, red, check ? FIN_PR_CHECK : FIN_PR, act.nseg < 0 ? PARTS_LIST : nb);
 }
 
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,

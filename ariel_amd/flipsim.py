@@ -7,6 +7,7 @@ flag and sort accessors have no reference counterpart (m_mgrid is private).
 Errors raise :class:`ArielError` carrying the ABI status and message.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -38,6 +39,11 @@ class FlipSim:
         rc = self._lib.ariel_create(ctypes.byref(desc), ctypes.byref(self._h))
         if rc != 0:
             raise ArielError(rc, self._lib.ariel_last_error(None).decode())
+        if desc.precond == _abi.PRECOND_LEARNED and self.n % 16 == 0:
+            # the shipped weights, when present; set_learned_weights replaces them
+            from .learned import DEFAULT_WEIGHTS
+            if os.path.exists(DEFAULT_WEIGHTS):
+                self.set_learned_weights(DEFAULT_WEIGHTS)
 
     # ------------------------------------------------------------ lifecycle
     def _check(self, rc):
@@ -189,6 +195,25 @@ class FlipSim:
 
     def load_state(self, path):
         self._check(self._lib.ariel_load_state(self._h, str(path).encode()))
+
+    def set_learned_weights(self, path_or_bytes=None):
+        """Load the learned preconditioner's packed weights (default: the file
+        shipped in ariel_amd/learned/)."""
+        if path_or_bytes is None:
+            from .learned import DEFAULT_WEIGHTS
+            path_or_bytes = DEFAULT_WEIGHTS
+        blob = path_or_bytes if isinstance(path_or_bytes, (bytes, bytearray)) else open(path_or_bytes, "rb").read()
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        self._check(self._lib.ariel_set_learned_weights(self._h, buf, len(blob)))
+
+    def solve_rhs(self, rhs):
+        """x with A x = rhs (current classes and level set), from x = 0; returns (x, stats)."""
+        rhs = np.ascontiguousarray(rhs, np.float32).reshape(-1)
+        x = np.empty_like(rhs)
+        st = StepStats()
+        self._check(self._lib.ariel_solve_rhs(
+            self._h, rhs.ctypes.data_as(_abi._PF), x.ctypes.data_as(_abi._PF), ctypes.byref(st)))
+        return x, st
 
     def export_levelset(self, path):
         self._check(self._lib.ariel_export_levelset(self._h, str(path).encode()))

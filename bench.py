@@ -51,6 +51,11 @@ def parse():
                     help="the batched workload instead: 64 tanks (edges from {48,64,80,96}, "
                          "seed 2), aggregate tank-steps/s of batched steps vs a per-tank loop")
     ap.add_argument("--tanks", type=int, default=64)
+    ap.add_argument("--learned", action="store_true",
+                    help="the learned-preconditioner comparison instead: one right-hand side "
+                         "at 128^3 and one at 256^3 solved from zero to 1e-6 with MIC(0)-PCG "
+                         "and with flexible PCG on the network; iterations and solve time")
+    ap.add_argument("--weights", default=None, help="learned weights (default: shipped)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (a plain copy "
                          "kernel of known traffic, to scale FETCH_SIZE/WRITE_SIZE)")
@@ -169,8 +174,70 @@ def batched(args):
     print(json.dumps(out), flush=True)
 
 
+def learned(args):
+    """Learned preconditioner vs MIC(0) on the same right-hand side: the divergence
+    of step 20 of the 128^3 dam break + sphere and of step 10 of the 256^3 pool +
+    falling cube, solved from zero to 1e-6 relative residual by MIC(0)-PCG and by
+    flexible PCG on the network (MIC(0) fallback armed). Solve time is the GPU
+    time of the whole solve (set-up, iterations, host readbacks of the stop
+    test), best of 3."""
+    from ariel_amd import FlipSim, _abi, scenes
+
+    cases = [("dam break + sphere", scenes.dam_break_sphere, 128, 20),
+             ("pool + falling cube", scenes.pool_and_cube, 256, 10)]
+    out_cases = []
+    for name, fn, n, steps in cases:
+        mic = FlipSim(fn(n))
+        mic.Init()
+        mic.Step(steps)
+        rhs = mic.get_field(_abi.FIELD_D)
+        lrn = FlipSim(fn(n, precond=_abi.PRECOND_LEARNED))
+        if args.weights:
+            lrn.set_learned_weights(args.weights)
+        lrn.Init()
+        lrn.set_flags(mic.get_flags())
+        lrn.set_field(_abi.FIELD_L, mic.get_field(_abi.FIELD_L))
+        res = {"case": name, "n": n, "step": steps, "fluid_cells": int((mic.get_flags() == 1).sum())}
+        for tag, sim in (("mic0", mic), ("learned", lrn)):
+            best = None
+            for _ in range(4):
+                x, st = sim.solve_rhs(rhs)
+                if best is None or st.solve_ms < best.solve_ms:
+                    best = st
+            res[tag] = {"iterations": int(best.iterations), "solve_ms": float(best.solve_ms),
+                        "ms_per_iteration": float(best.solve_ms) / max(int(best.iterations), 1),
+                        "true_residual_rel": float(best.true_residual_norm / max(best.rhs_norm, 1e-30)),
+                        "status": int(best.solve_status)}
+            if tag == "learned":
+                res[tag]["fallback_to_mic0"] = int(best.precond_fallback)
+        log(f"{name} {n}^3: MIC(0) {res['mic0']['iterations']} it {res['mic0']['solve_ms']:.2f} ms, "
+            f"learned {res['learned']['iterations']} it {res['learned']['solve_ms']:.2f} ms "
+            f"(fallback {res['learned']['fallback_to_mic0']})")
+        mic.close()
+        lrn.close()
+        out_cases.append(res)
+    c0 = out_cases[0]
+    out = {
+        "metric": "pressure solve to 1e-6, learned preconditioner vs MIC(0): wall-time ratio at 128³",
+        "value": c0["mic0"]["solve_ms"] / c0["learned"]["solve_ms"],
+        "unit": "x (MIC(0) ms / learned ms)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "bf16 network (FP32 accumulate), f32 PCG with FP64 reductions",
+        "data": "synthetic",
+        "config": {"workload": "divergence of a simulated step, solved from zero; 128³ dam break "
+                               "+ sphere (step 20) and 256³ pool + falling cube (step 10)"},
+        "cases": out_cases,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.learned:
+        learned(args)
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

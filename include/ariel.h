@@ -165,6 +165,8 @@ typedef struct ariel_step_stats {
     double  stage_ms[ARIEL_NUM_STAGES]; /* device time of each stage, last step */
     double  step_ms;            /* device time of the last step              */
     double  solve_ms;           /* device time of the last pressure solve    */
+    int32_t precond_fallback;   /* learned preconditioner: 1 if the last solve fell back to MIC(0) */
+    int32_t pad_;
 } ariel_step_stats;
 
 typedef struct ariel_kernel_stats {
@@ -275,6 +277,20 @@ int ariel_load_state(ariel_sim* h, const char* path);
  * (N+1)^3 voxels of size 1 in grid units, sphere radius 0.5 (levelset.cpp:
  * 101-111), as a raw little-endian float32 volume with a text header. */
 int ariel_export_levelset(ariel_sim* h, const char* path);
+
+/* Weights of the learned preconditioner (ARIEL_PRECOND_LEARNED; no reference
+ * counterpart): the packed BF16 blob written by ariel_amd/learned/train.py --
+ * 8-byte magic "ARIELNN1", int32 layer count (11), int32 reserved, int64 value
+ * count, then that many little-endian BF16 values in the kernels' layout.
+ * Required before the first solve of a handle with that preconditioner;
+ * allocates the network's activation buffers. */
+int ariel_set_learned_weights(ariel_sim* h, const void* blob, int64_t bytes);
+
+/* Solve A x = rhs for the current cell classes and level set with the
+ * configured preconditioner, from x = 0 (no reference counterpart: it builds
+ * the learned preconditioner's training pairs and serves the tests). rhs and x
+ * are N^3 host floats; overwrites the handle's divergence and pressure. */
+int ariel_solve_rhs(ariel_sim* h, const float* rhs, float* x, ariel_step_stats* stats);
 
 /* ---------------------------------------------------------------- batches
  * Independent tanks stepped together (no counterpart in the reference, which

@@ -70,6 +70,8 @@ typedef struct oracle_stats {
     double  stage_ms[10];
     double  step_ms;
     double  solve_ms;
+    int32_t precond_fallback;   /* always 0: the oracle has no learned preconditioner */
+    int32_t pad_;
 } oracle_stats;
 
 typedef struct oracle_sim oracle_sim;

@@ -25,7 +25,8 @@ class OracleStats(ctypes.Structure):
                 ("residual_norm", ctypes.c_double), ("true_residual_norm", ctypes.c_double),
                 ("max_divergence", ctypes.c_double), ("max_density", ctypes.c_double),
                 ("stage_ms", ctypes.c_double * 10), ("step_ms", ctypes.c_double),
-                ("solve_ms", ctypes.c_double)]
+                ("solve_ms", ctypes.c_double), ("precond_fallback", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
 
 
 def build():

@@ -153,3 +153,6 @@ def test_batch_rejects_mixed_solver_settings():
     assert rc == _abi.ERR_INVALID
     assert b"share" in lib.ariel_batch_last_error(None)
     assert lib.ariel_batch_create(arr, 0, ctypes.byref(h)) == _abi.ERR_INVALID
+    arr[0].precond = _abi.PRECOND_LEARNED
+    assert lib.ariel_batch_create(arr, 1, ctypes.byref(h)) == _abi.ERR_UNSUPPORTED
+    assert b"single tanks" in lib.ariel_batch_last_error(None)

@@ -1,0 +1,160 @@
+"""Learned preconditioner on the GPU: the MFMA network against the PyTorch
+restatement (BF16 rounding emulated, FP32 on the CPU), and the flexible PCG
+built on it: converges to the same tolerance as MIC(0)-PCG, falls back to
+MIC(0) when the network's directions are useless, and runs inside a step."""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from ariel_amd import FlipSim, _abi, learned, scenes
+from ariel_amd.learned import net as lnet
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _sim(desc, blob):
+    s = FlipSim(desc)
+    s.set_learned_weights(blob)
+    s.Init()
+    return s
+
+
+def _rhs(sim, seed):
+    n = sim.GetDimensions()[0]
+    flags = sim.get_flags()
+    rng = np.random.default_rng(seed)
+    r = rng.standard_normal(n ** 3).astype(np.float32)
+    r[flags != lnet.FLUID] = 0.0
+    return r, flags
+
+
+def _shipped_or_random():
+    if os.path.exists(learned.DEFAULT_WEIGHTS):
+        return open(learned.DEFAULT_WEIGHTS, "rb").read()
+    return lnet.PressureNet(seed=11).blob()
+
+
+@pytest.mark.parametrize("n", [32, 48])
+def test_network_matches_torch(n):
+    blob = lnet.PressureNet(seed=7).blob()
+    _say(f"learned forward n={n}")
+    sim = _sim(scenes.dam_break_sphere(n, precond=_abi.PRECOND_LEARNED), blob)
+    sim.Step(2)
+    r, flags = _rhs(sim, n)
+    L = sim.get_field(_abi.FIELD_L)
+    z = sim.apply_preconditioner(r)
+    sim.close()
+    ref_net = lnet.PressureNet.from_blob(blob)
+    diag, fluid = lnet.laplacian(torch.from_numpy(flags.reshape(n, n, n)),
+                                 torch.from_numpy(L.reshape(n, n, n)))
+    fl = fluid.view(1, 1, n, n, n)
+    dw = lnet.diag_weight(diag.view(1, 1, n, n, n), fl)
+    with torch.no_grad():
+        ref = ref_net(torch.from_numpy(r).view(1, 1, n, n, n), fl, dw, emulate_bf16=True)
+    ref = ref.reshape(-1).numpy()
+    assert np.all(z[flags != lnet.FLUID] == 0.0)
+    rel = np.linalg.norm(z - ref) / np.linalg.norm(ref)
+    mx = np.abs(z - ref).max() / np.abs(ref).max()
+    _say(f"  rel L2 {rel:.3e} max {mx:.3e}")
+    # BF16 storage of every activation; accumulation order differs from torch's
+    assert rel < 1e-2 and mx < 5e-2
+
+
+def _solve_pair(n, blob, seed):
+    ds = scenes.dam_break_sphere(n, precond=_abi.PRECOND_LEARNED)
+    sim = _sim(ds, blob)
+    sim.Step(3)
+    r, flags = _rhs(sim, seed)
+    dm = scenes.dam_break_sphere(n)
+    mic = FlipSim(dm)
+    mic.Init()
+    mic.set_flags(flags)
+    mic.set_field(_abi.FIELD_L, sim.get_field(_abi.FIELD_L))
+    xl, sl = sim.solve_rhs(r)
+    xm, sm = mic.solve_rhs(r)
+    sim.close()
+    mic.close()
+    return xl, sl, xm, sm
+
+
+def _check_converged(x, st, ref):
+    assert st.solve_status == 0
+    assert st.true_residual_norm <= 2e-6 * st.rhs_norm
+    assert np.linalg.norm(x - ref) <= 1e-3 * np.linalg.norm(ref)
+
+
+def test_flexible_pcg_converges_like_mic():
+    _say("learned solve n=32")
+    xl, sl, xm, sm = _solve_pair(32, _shipped_or_random(), 3)
+    _say(f"  learned {sl.iterations} it (fallback {sl.precond_fallback}), mic {sm.iterations} it")
+    _check_converged(xm, sm, xm)
+    _check_converged(xl, sl, xm)
+    if os.path.exists(learned.DEFAULT_WEIGHTS):
+        # the shipped network stays in use and beats MIC(0) on iterations
+        assert sl.precond_fallback == 0
+        assert sl.iterations < sm.iterations
+
+
+def test_fallback_to_mic_when_network_is_useless():
+    """All-zero weights give z = 0, which is no descent direction: the solve
+    switches to MIC(0) at once and ends as plain MIC(0)-PCG does."""
+    zero = learned.pack([np.zeros_like(w) for w in learned.unpack(lnet.PressureNet().blob())])
+    _say("learned fallback n=32")
+    xl, sl, xm, sm = _solve_pair(32, zero, 4)
+    _say(f"  learned {sl.iterations} it (fallback {sl.precond_fallback}), mic {sm.iterations} it")
+    assert sl.precond_fallback == 1
+    _check_converged(xl, sl, xm)
+    assert sl.iterations <= sm.iterations + 2
+
+
+def test_negated_network_falls_back():
+    """z = -net(r) points uphill (z.r < 0): fallback, same answer."""
+    ws = learned.unpack(_shipped_or_random())
+    ws[-1] = -ws[-1]
+    _say("learned negated n=32")
+    xl, sl, xm, sm = _solve_pair(32, learned.pack(ws), 5)
+    assert sl.precond_fallback == 1
+    _check_converged(xl, sl, xm)
+
+
+def test_learned_step_runs():
+    _say("learned step n=48")
+    sim = _sim(scenes.dam_break(48, precond=_abi.PRECOND_LEARNED), _shipped_or_random())
+    mic = FlipSim(scenes.dam_break(48))
+    mic.Init()
+    # first step: same right-hand side in both; the stop rule is on the
+    # recurrence residual, the true residual of a smooth pressure has an FP32
+    # floor (A p cancels), which MIC(0)-PCG shows as well
+    sim.Step(1)
+    mic.Step(1)
+    st, sm = sim.stats(), mic.stats()
+    mic.close()
+    _say(f"  learned {st.iterations} it fb {st.precond_fallback} true {st.true_residual_norm:.3e}; "
+         f"mic {sm.iterations} it true {sm.true_residual_norm:.3e}; |b| {st.rhs_norm:.3e}")
+    assert st.solve_status == 0 and st.iterations > 0
+    assert st.rhs_norm == sm.rhs_norm
+    assert st.residual_norm <= 1e-6 * st.rhs_norm
+    assert st.true_residual_norm <= max(3.0 * sm.true_residual_norm, 2e-6 * st.rhs_norm)
+    sim.Step(2)
+    st = sim.stats()
+    assert st.solve_status == 0 and st.iterations > 0
+    assert st.residual_norm <= 1e-6 * st.rhs_norm
+    sim.close()
+
+
+def test_learned_needs_weights_and_divisible_grid():
+    sim = FlipSim(scenes.dam_break(24, precond=_abi.PRECOND_LEARNED))
+    with pytest.raises(Exception):
+        sim.set_learned_weights(lnet.PressureNet().blob())
+    sim.Init()
+    with pytest.raises(Exception):
+        sim.Step(1)
+    sim.close()
