@@ -565,6 +565,7 @@ int stage_solve_learned(ariel_sim* h) {
     if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
     launch_fill_f32(s, h->Z, h->nc, 0.0f);
     launch_fill_f32(s, h->R_old, h->nc, 0.0f);
+    launch_diag_weight(s, h->N, h->A, h->diag, h->net.wd);
     learned_precond(h, p, red, false);
     launch_update_s(s, h->Z, h->S, red, 1, act);
     TRY(check_launch(h, "learned solve setup"));
@@ -607,7 +608,8 @@ int stage_solve_learned(ariel_sim* h) {
     h->stats.solve_status = h->sc_host->status;
     h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
     h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
-    h->stats.precond_fallback = h->sc_host->use_mic;
+    h->stats.precond_fallback = h->sc_host->use_mic ? h->sc_host->fallback_reason : 0;
+    h->stats.fallback_iter = h->sc_host->use_mic ? h->sc_host->fallback_it : 0;
     launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, act);
     TRY(check_launch(h, "true residual"));
     TRY(read_scalars(h));
@@ -819,7 +821,7 @@ int ariel_destroy(ariel_sim* h) {
                     (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
                     (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc,
                     (void*)h->seg_flag, (void*)h->seg_scan, (void*)h->seg_list, (void*)h->tile_any,
-                    (void*)h->tile_lists, (void*)h->net_mem, (void*)h->R_old})
+                    (void*)h->tile_lists, (void*)h->net_mem, (void*)h->R_old, (void*)h->net.wd})
         hipFree(p);
     for (int c = 0; c < 3; ++c) {
         hipFree(h->u[c]);
@@ -1123,6 +1125,7 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
         for (int64_t c = 0; c < h->nc; ++c)
             if (flags[c] == ARIEL_FLUID) r2 += (double)r[c] * (double)r[c];
         HIPCHK(h, hipMemcpyAsync(&h->sc->rnorm2, &r2, sizeof(double), hipMemcpyHostToDevice, h->stream));
+        launch_diag_weight(h->stream, h->N, h->A, h->diag, h->net.wd);
         launch_learned_apply(h->stream, h->net, h->N, h->R, h->A, h->diag, h->Z, h->sc, h->act.nseg);
     } else {
         launch_precond_apply(h->stream, p, h->fl8, h->pc, h->R, h->Q, h->Z, h->reduce(), h->act);
@@ -1314,6 +1317,7 @@ int ariel_set_learned_weights(ariel_sim* h, const void* blob, int64_t bytes) {
         }
         TRY(dalloc(h, &h->net_mem, total));
         TRY(dalloc(h, &h->R_old, h->nc));
+        TRY(dalloc(h, &h->net.wd, h->nc));
         uint16_t* m = h->net_mem;
         h->net.weights = m + pos[0];
         h->net.e0 = m + pos[1];

@@ -48,7 +48,9 @@ struct SolverScalars {
     int32_t flex;          // flexible PCG (learned preconditioner)
     int32_t use_mic;       // flexible PCG has fallen back to MIC(0)
     int32_t restart;       // next search direction restarts (beta = 0)
-    double rnorm2_prev;    // ||r||^2 of the previous iteration (fallback test)
+    double rnorm2_prev;    // ||r||^2 at the start of the fallback test's window
+    int32_t fallback_it;   // iteration at which flexible PCG switched to MIC(0)
+    int32_t fallback_reason;  // 1: slow progress, 2: not a descent direction
 };
 
 // Cells the pressure solve touches. A z-segment is 8 cells (cells 8s..8s+7) and
@@ -106,6 +108,7 @@ struct LearnedNet {
     int64_t off[kLearnedLayers] = {};
     uint16_t *e0 = nullptr, *e1 = nullptr, *e2 = nullptr, *e3 = nullptr;
     uint16_t *p = nullptr, *t = nullptr, *d2 = nullptr, *d1 = nullptr, *d0 = nullptr;
+    float* wd = nullptr;   // sqrt(6/diag) on FLUID cells, 0 elsewhere (per solve)
 };
 // Offsets of the layers in the packed weights (BF16 elements); returns the total.
 int64_t learned_layout(int64_t* off);
@@ -114,6 +117,8 @@ int64_t learned_layout(int64_t* off);
 void launch_learned_apply(hipStream_t s, const LearnedNet& net, int N, const float* r,
                           const uint8_t* A, const float* diag, float* z, const SolverScalars* sc,
                           int64_t nseg);
+// net.wd from the cell classes and the diagonal of the current matrix.
+void launch_diag_weight(hipStream_t s, int N, const uint8_t* A, const float* diag, float* wd);
 
 // ---------------------------------------------------------------- launching
 // Every kernel is a functor K { static constexpr int kThreads; struct Args;

@@ -187,6 +187,8 @@ struct KSolverInit {
         sc->use_mic = 0;
         sc->restart = 0;
         sc->rnorm2_prev = 0.0;
+        sc->fallback_it = 0;
+        sc->fallback_reason = 0;
         sc->bnorm2 = sc->rnorm2 = sc->thr = sc->rho = sc->sigma = 0.0;
         sc->alpha = sc->beta = sc->tmp = 0.0;
         sc->done = 0;
@@ -697,7 +699,16 @@ enum { FIN_RESID0 = 0, FIN_SIGMA = 1, FIN_RNORM = 2, FIN_TRUE = 3, FIN_REPORT = 
        FIN_PR = 6, FIN_PR_CHECK = 7 };
 // Flexible PCG falls back to MIC(0) when, over a window of this many
 // iterations, ||r|| fell by less than 5% per iteration on average.
-constexpr int kFlexWindow = 4;
+// Flexible PCG's progress test: over every window of kFlexWindow iterations the
+// residual norm must fall by 5% per iteration on average. The CG residual norm
+// is not monotone (at 256^3 it barely moves in the first 4 iterations while the
+// error falls), so the window is long.
+constexpr int kFlexWindow = 16;
+constexpr double flex_need() {   // 0.95^(2 kFlexWindow), on ||r||^2
+    double v = 1.0;
+    for (int i = 0; i < kFlexWindow; ++i) v *= 0.9025;
+    return v;
+}
 
 // One block: sum the partials of the preceding reducing kernel in a fixed order
 // and update the scalars. Gated exactly like its producer, so a skipped producer
@@ -777,10 +788,12 @@ struct KFinalize {
                 sc->it += 1;
                 if (sc->flex && !sc->use_mic && sc->it % kFlexWindow == 0) {
                     // average reduction per iteration over the window below 5%
-                    const double need = 0.9025 * 0.9025 * 0.9025 * 0.9025;   // 0.95^(2*4)
+                    constexpr double need = flex_need();
                     if (!(s0 <= need * sc->rnorm2_prev)) {
                         sc->use_mic = 1;
                         sc->restart = 1;
+                        sc->fallback_it = sc->it;
+                        sc->fallback_reason = 1;
                     }
                     sc->rnorm2_prev = s0;
                 }
@@ -802,6 +815,8 @@ struct KFinalize {
                 if (!(s0 > 0.0) || !finite(s0)) {
                     sc->use_mic = 1;
                     sc->restart = 1;
+                    sc->fallback_it = sc->it;
+                    sc->fallback_reason = 2;
                 }
                 break;
             case FIN_PR:

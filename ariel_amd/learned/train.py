@@ -302,7 +302,8 @@ def library_check(blob, spec):
         xm, sm = mic.solve_rhs(D)
         res = {"scene": name, "n": n, "step": steps,
                "learned_iters": sl.iterations, "learned_ms": sl.solve_ms,
-               "fallback": sl.precond_fallback, "mic_iters": sm.iterations, "mic_ms": sm.solve_ms,
+               "fallback": sl.precond_fallback, "fallback_iter": sl.fallback_iter,
+               "mic_iters": sm.iterations, "mic_ms": sm.solve_ms,
                "learned_true_rel": sl.true_residual_norm / max(sl.rhs_norm, 1e-30),
                "mic_true_rel": sm.true_residual_norm / max(sm.rhs_norm, 1e-30)}
         sim.close()

@@ -210,9 +210,10 @@ def learned(args):
                         "status": int(best.solve_status)}
             if tag == "learned":
                 res[tag]["fallback_to_mic0"] = int(best.precond_fallback)
+                res[tag]["fallback_iteration"] = int(best.fallback_iter)
         log(f"{name} {n}^3: MIC(0) {res['mic0']['iterations']} it {res['mic0']['solve_ms']:.2f} ms, "
             f"learned {res['learned']['iterations']} it {res['learned']['solve_ms']:.2f} ms "
-            f"(fallback {res['learned']['fallback_to_mic0']})")
+            f"(fallback {res['learned']['fallback_to_mic0']} at {res['learned']['fallback_iteration']})")
         mic.close()
         lrn.close()
         out_cases.append(res)

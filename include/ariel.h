@@ -165,8 +165,10 @@ typedef struct ariel_step_stats {
     double  stage_ms[ARIEL_NUM_STAGES]; /* device time of each stage, last step */
     double  step_ms;            /* device time of the last step              */
     double  solve_ms;           /* device time of the last pressure solve    */
-    int32_t precond_fallback;   /* learned preconditioner: 1 if the last solve fell back to MIC(0) */
-    int32_t pad_;
+    int32_t precond_fallback;   /* learned preconditioner, last solve: 0 network used throughout,
+                                   1 switched to MIC(0) for slow progress, 2 for a network
+                                   output that was no descent direction */
+    int32_t fallback_iter;      /* iteration of that switch (0 if none) */
 } ariel_step_stats;
 
 typedef struct ariel_kernel_stats {
