@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _pair(desc, warm_steps=0):
-    o = OracleSim(desc, threads=8)
+    o = OracleSim(desc, threads=16)
     g = FlipSim(desc)
     o.init()
     g.Init()
@@ -48,6 +48,7 @@ def test_init_matches(n, precond):
     lambda: scenes.dam_break_sphere(48, precond=_abi.PRECOND_MIC0_BLOCK),
     lambda: scenes.pool_and_cube(32, refine_every=7),
     lambda: scenes.pool_and_cube(64),
+    lambda: scenes.dam_break_sphere(128),
 ])
 def test_stagewise_parity(desc_fn):
     d = desc_fn()
@@ -145,3 +146,25 @@ def test_free_running_steps_track_oracle():
     pg, vg, _ = g.GetParticles()
     assert np.abs(pg - po).max() < 1e-4
     assert rel_err(vg, vo) < 1e-3
+
+
+def test_dam_break_sphere_128_first_steps():
+    """The 128^3 dam break + sphere, free running: step 1's cell classes and
+    per-cell particle counts are bit-exact (every stage before them is); later
+    steps see pressures that differ in the last bits (FP64 reduction order), so
+    a few particles near cell faces may land differently. Particle totals are
+    conserved exactly."""
+    d = scenes.dam_break_sphere(128)
+    o, g = _pair(d)
+    n3 = 128 ** 3
+    for step in range(5):
+        o.step()
+        g.Step(1)
+        assert g.stats().solve_status == 0 and o.stats().solve_status == 0
+        assert g.num_particles == o.num_particles
+        flag_mism = int((g.get_flags() != o.get_flags()).sum())
+        count_mism = int((np.diff(g.get_sort()[1]) != np.diff(o.get_sort()[1])).sum())
+        print(f"step {step + 1}: {flag_mism} cell classes, {count_mism} cell counts differ", flush=True)
+        if step == 0:
+            assert flag_mism == 0
+        assert flag_mism <= 1e-4 * n3
