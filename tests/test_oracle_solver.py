@@ -199,3 +199,28 @@ def test_faithful_reference_solver_terminates():
     o.step(mode=1)
     st = o.stats()
     assert st.solve_status == 0 and 0 < st.iterations < 16 ** 3
+
+
+def test_precision_measure_on_an_oracle_step():
+    """The FP64 error measurement pinned on the oracle's own FP32 step: the FP32
+    pressure is within the solver tolerance's reach of the FP64 solution, the
+    error of the exact FP64 solution (rounded to FP32) is at FP32 round-off,
+    and a planted 1% pressure error is reported as 1%."""
+    from tests import precision
+    d = scenes.pool_and_cube(32)
+    o = OracleSim(d, threads=4)
+    o.init()
+    o.step()
+    flags, L, D, P = (o.get_flags(), o.get_field(_abi.FIELD_L), o.get_field(_abi.FIELD_D),
+                      o.get_field(_abi.FIELD_P))
+    chk = OracleSim(d, threads=4)
+    e = precision.errors(chk, flags, L, D, P)
+    assert e["fp64_rel_residual"] <= 1e-10
+    assert 0.0 < e["pressure_rel_error"] < 1e-4 and 0.0 < e["velocity_rel_error"] < 1e-3
+    it, p64, _ = chk.solve_fp64(tol=1e-10)
+    e64 = precision.errors(chk, flags, L, D, p64.astype(np.float32))
+    assert e64["pressure_rel_error"] < 1e-6
+    fl = flags == FLUID
+    e1 = precision.errors(chk, flags, L, D, np.where(fl, p64 * 1.01, 0.0).astype(np.float32))
+    assert e1["pressure_rel_error"] == pytest.approx(0.01, rel=1e-3)
+    assert e1["velocity_rel_error"] == pytest.approx(0.01, rel=1e-2)
