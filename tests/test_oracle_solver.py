@@ -224,3 +224,53 @@ def test_precision_measure_on_an_oracle_step():
     e1 = precision.errors(chk, flags, L, D, np.where(fl, p64 * 1.01, 0.0).astype(np.float32))
     assert e1["pressure_rel_error"] == pytest.approx(0.01, rel=1e-3)
     assert e1["velocity_rel_error"] == pytest.approx(0.01, rel=1e-2)
+
+
+def _fault_solve(flags, D, n=16, max_iters=500):
+    d = scenes.dam_break(n, max_iters=max_iters)
+    o = OracleSim(d, threads=4)
+    o.set_flags(flags)
+    o.set_field(_abi.FIELD_L, np.where(flags == FLUID, -0.5, 0.5).astype(np.float32))
+    o.set_field(_abi.FIELD_D, np.asarray(D, np.float32))
+    o.set_field(_abi.FIELD_P, np.zeros(n ** 3, np.float32))
+    o.stage(_abi.STAGE_SOLVE)
+    return o.stats(), o.get_field(_abi.FIELD_P)
+
+
+def fault_cases(n=16, seed=0):
+    """(name, flags, rhs, max_iters, expected status or set of statuses)."""
+    rng = np.random.default_rng(seed)
+    pocket = np.full((n, n, n), SOLID, np.uint8)
+    pocket[4:12, 4:12, 4:12] = FLUID
+    pocket = pocket.ravel()
+    b = rng.standard_normal(n ** 3) * (pocket == FLUID)
+    consistent = b.copy()
+    consistent[pocket == FLUID] -= consistent[pocket == FLUID].mean()
+    dam = np.zeros((n, n, n), np.uint8)
+    dam[1:-1, 1:8, 1:-1] = FLUID
+    dam = dam.ravel()
+    bn = rng.standard_normal(n ** 3) * (dam == FLUID)
+    bn[np.flatnonzero(dam == FLUID)[5]] = np.nan
+    return [
+        ("empty fluid", np.zeros(n ** 3, np.uint8), rng.standard_normal(n ** 3), 500, {0}),
+        ("closed pocket, zero-mean rhs", pocket, consistent, 500, {0}),
+        ("closed pocket, inconsistent rhs", pocket, b, 500, {1, 2}),
+        ("NaN in the rhs", dam, bn, 500, {3}),
+        ("iteration cap 1", dam, rng.standard_normal(n ** 3) * (dam == FLUID), 1, {1}),
+    ]
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_fault_injection(case):
+    """Empty fluid, the singular all-Neumann pocket (consistent and not), a NaN
+    right-hand side and the iteration cap: every solve terminates with the
+    matching status (converged / max-iters / breakdown / non-finite)."""
+    name, flags, b, max_iters, want = fault_cases()[case]
+    st, P = _fault_solve(flags, b, max_iters=max_iters)
+    assert st.solve_status in want, (name, st.solve_status)
+    if name == "empty fluid":
+        assert st.iterations == 0 and np.all(P == 0.0)
+    if name == "iteration cap 1":
+        assert st.iterations == 1
+    if st.solve_status != 3:
+        assert np.all(np.isfinite(P))
