@@ -114,6 +114,14 @@ PROTOTYPES = {
     "ariel_batch_create": (_I32, [ctypes.POINTER(SceneDesc), _I32, ctypes.POINTER(_P)]),
     "ariel_batch_destroy": (_I32, [_P]),
     "ariel_batch_last_error": (ctypes.c_char_p, [_P]),
+    "ariel_trainer_create": (_I32, [ctypes.POINTER(_P)]),
+    "ariel_trainer_destroy": (_I32, [_P]),
+    "ariel_trainer_last_error": (ctypes.c_char_p, [_P]),
+    "ariel_trainer_set_weights": (_I32, [_P, _PF, ctypes.c_int64]),
+    "ariel_trainer_get_weights": (_I32, [_P, _PF, ctypes.c_int64]),
+    "ariel_trainer_step": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
+                                  _I32, ctypes.POINTER(ctypes.c_double), _PF]),
+    "ariel_trainer_forward": (_I32, [_P, _I32, _I32, _P, _P, _P, _P]),
     "ariel_batch_size": (_I32, [_P, ctypes.POINTER(_I32)]),
     "ariel_batch_tank": (_I32, [_P, _I32, ctypes.POINTER(_P)]),
     "ariel_batch_init": (_I32, [_P]),

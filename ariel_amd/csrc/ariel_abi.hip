@@ -35,6 +35,8 @@ struct KTimer {
 
 }  // namespace
 
+void ariel::set_last_error(const std::string& msg) { g_last_error = msg; }
+
 struct ariel_sim {
     ariel_scene_desc d{};
     int N = 0;

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace ariel {
@@ -117,6 +118,9 @@ int64_t learned_layout(int64_t* off);
 void launch_learned_apply(hipStream_t s, const LearnedNet& net, int N, const float* r,
                           const uint8_t* A, const float* diag, float* z, const SolverScalars* sc,
                           int64_t nseg);
+// The calling thread's last error message (ariel_last_error(NULL)).
+void set_last_error(const std::string& msg);
+
 // net.wd from the cell classes and the diagonal of the current matrix.
 void launch_diag_weight(hipStream_t s, int N, const uint8_t* A, const float* diag, float* wd);
 

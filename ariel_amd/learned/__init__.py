@@ -2,8 +2,10 @@
 
 The network (a 4-level convolutional U-Net, BF16 weights) runs inside the
 library's flexible PCG; this package holds its packed weights
-(``weights.bin``), the blob reader/writer, and, in ``net`` and ``train``, the
-PyTorch restatement used to train it (torch is needed only there).
+(``weights.bin``) and the blob reader/writer. ``trainer`` drives the library's
+own training kernels, ``train`` is the training script, and ``net`` is the
+PyTorch restatement of the network that tests check the kernels against
+(torch is needed only for those three).
 
 Blob layout: ``b"ARIELNN1"``, int32 layer count (11), int32 zero, int64 value
 count, then that many BF16 values (little endian) in the layer order and
@@ -65,9 +67,9 @@ def bf16_bits_to_f32(b):
     return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
-def pack(layers):
-    """layers: list of 11 float32 arrays in torch conv layout [cout][cin][3][3][3]
-    (or [cout][cin][1][1][1]) in LAYERS order -> blob bytes."""
+def layers_to_flat(layers):
+    """11 float32 arrays in torch conv layout ([cout][cin][k][k][k]) -> the packed
+    FP32 vector (the blob's layout, no rounding)."""
     if len(layers) != len(LAYERS):
         raise ValueError("expected %d layers" % len(LAYERS))
     parts = []
@@ -88,25 +90,21 @@ def pack(layers):
             m = w.reshape(cout, cin, 27).transpose(2, 0, 1)
         else:
             m = w.reshape(cout, cin)
-        parts.append(f32_to_bf16_bits(m.reshape(-1)))
-    vals = np.concatenate(parts)
-    _off, total = layout()
-    assert vals.size == total
-    return MAGIC + struct.pack("<iiq", len(LAYERS), 0, total) + vals.astype("<u2").tobytes()
+        parts.append(np.ascontiguousarray(m, np.float32).reshape(-1))
+    flat = np.concatenate(parts)
+    assert flat.size == layout()[1]
+    return flat
 
 
-def unpack(blob):
-    """Blob bytes -> list of float32 arrays in torch conv layout (BF16 values)."""
-    if blob[:8] != MAGIC:
-        raise ValueError("bad magic")
-    nl, _z, total = struct.unpack("<iiq", blob[8:24])
+def flat_to_layers(flat):
+    """The packed vector -> 11 float32 arrays in torch conv layout."""
+    flat = np.asarray(flat, np.float32)
     off, want = layout()
-    if nl != len(LAYERS) or total != want or len(blob) != 24 + 2 * total:
-        raise ValueError("layout mismatch")
-    vals = bf16_bits_to_f32(np.frombuffer(blob, "<u2", total, 24))
+    if flat.size != want:
+        raise ValueError("packed weights: %d values, want %d" % (flat.size, want))
     out = []
     for (name, taps, _s, cin, cout, _r), o in zip(LAYERS, off):
-        m = vals[o:o + packed_size(name, taps, cin, cout)]
+        m = flat[o:o + packed_size(name, taps, cin, cout)]
         if name == "in":
             w = m.reshape(16, 64)[:, :54].reshape(cout, 27, cin).transpose(0, 2, 1)
             w = w.reshape(cout, cin, 3, 3, 3)
@@ -118,3 +116,35 @@ def unpack(blob):
             w = m.reshape(cout, cin, 1, 1, 1)
         out.append(np.ascontiguousarray(w, np.float32))
     return out
+
+
+def pack_flat(flat):
+    """Packed FP32 vector -> blob bytes (BF16, round to nearest even)."""
+    flat = np.asarray(flat, np.float32)
+    _off, total = layout()
+    if flat.size != total:
+        raise ValueError("packed weights: %d values, want %d" % (flat.size, total))
+    vals = f32_to_bf16_bits(flat)
+    return MAGIC + struct.pack("<iiq", len(LAYERS), 0, total) + vals.astype("<u2").tobytes()
+
+
+def pack(layers):
+    """layers: list of 11 float32 arrays in torch conv layout [cout][cin][3][3][3]
+    (or [cout][cin][1][1][1]) in LAYERS order -> blob bytes."""
+    return pack_flat(layers_to_flat(layers))
+
+
+def unpack_flat(blob):
+    """Blob bytes -> the packed vector of its BF16 values as float32."""
+    if blob[:8] != MAGIC:
+        raise ValueError("bad magic")
+    nl, _z, total = struct.unpack("<iiq", blob[8:24])
+    _off, want = layout()
+    if nl != len(LAYERS) or total != want or len(blob) != 24 + 2 * total:
+        raise ValueError("layout mismatch")
+    return bf16_bits_to_f32(np.frombuffer(blob, "<u2", total, 24))
+
+
+def unpack(blob):
+    """Blob bytes -> list of float32 arrays in torch conv layout (BF16 values)."""
+    return flat_to_layers(unpack_flat(blob))

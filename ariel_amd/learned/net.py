@@ -32,6 +32,12 @@ def _bf16(x):
     return x.to(torch.bfloat16).to(torch.float32)
 
 
+def _bf16_ste(x):
+    """BF16 rounding in the forward pass, identity in the backward pass (the
+    library's training kernels propagate gradients through the roundings)."""
+    return x + (_bf16(x) - x).detach()
+
+
 class PressureNet(nn.Module):
     def __init__(self, seed=0):
         super().__init__()
@@ -54,8 +60,10 @@ class PressureNet(nn.Module):
         return torch.sqrt(r2 / (8.0 * nseg)).to(torch.float32)
 
     def forward(self, r, fluid, dw, emulate_bf16=False):
-        """r, fluid, dw = diag_weight(...): (B, 1, N, N, N) float32; returns z."""
-        q = _bf16 if emulate_bf16 else (lambda t: t)
+        """r, fluid, dw = diag_weight(...): (B, 1, N, N, N) float32; returns z.
+        emulate_bf16: False (FP32), True (BF16 roundings as the kernels store) or
+        "ste" (the same roundings, straight-through in the backward pass)."""
+        q = (_bf16_ste if emulate_bf16 == "ste" else _bf16) if emulate_bf16 else (lambda t: t)
         w = [q(p) for p in self.w]
         s = self.scale(r, fluid).view(-1, 1, 1, 1, 1)
         inv = torch.where(s > 0, 1.0 / s, torch.zeros_like(s))
@@ -132,13 +140,14 @@ def apply_laplacian(x, diag, fluid):
 
 
 def a_angle_loss(z, r, x, diag, fluid):
-    """sin^2 of the A-angle between z and the exact solution x of A x = r:
-    1 - (r.z)^2 / ((r.x)(z.Az)), invariant to the scale of z."""
+    """sin^2 of the A-angle between z and the exact solution x of A x = r,
+    signed: 1 - (r.z)|r.z| / ((r.x)(z.Az)), invariant to positive scalings of z
+    and above 1 when z points away from x (flexible PCG needs z.r > 0)."""
     b = z.shape[0]
     rz = (r * z).reshape(b, -1).sum(1)
     rx = (r * x).reshape(b, -1).sum(1).clamp(min=1e-30)
     zaz = (z * apply_laplacian(z, diag, fluid)).reshape(b, -1).sum(1).clamp(min=1e-30)
-    return (1.0 - rz * rz / (rx * zaz)).mean()
+    return (1.0 - rz * rz.abs() / (rx * zaz)).mean()
 
 
 def numpy_scale(r, flags):

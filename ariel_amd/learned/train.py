@@ -12,11 +12,12 @@
    (r_k, x* - x_k) at several iterations k: A (x* - x_k) = r_k, and r_k has the
    roughness and surface concentration of the residuals a preconditioner
    actually meets inside PCG.
-2. Fit the network (``net.PressureNet``) by minimising the A-angle loss
-   between its output and the solution, A built from the cell classes and
-   level set (``net.laplacian``); Adam, cosine learning rate, BF16 autocast.
-   Between rounds, add the residuals of flexible PCG driven by the current
-   network (the distribution the network sees when it is used).
+2. Fit the network with the library's own training kernels (``Trainer``:
+   BF16 MFMA forward, FP32 backward, Adam on FP32 master weights) by
+   minimising the A-angle loss between its output and the solution, A built
+   from the cell classes and level set (``net.laplacian``); cosine learning
+   rate. Between rounds, add the residuals of flexible PCG driven by the
+   current network (the distribution the network sees when it is used).
 3. Check: held-out loss, flexible PCG iterations in torch, then through the
    library with the new weights against MIC(0)-PCG; write ``weights.bin`` and
    ``report.json`` into DIR.
@@ -34,7 +35,9 @@ import torch
 import torch.nn.functional as F
 
 from .. import FlipSim, _abi, scenes
+from . import layers_to_flat
 from . import net as lnet
+from .trainer import Trainer
 
 SCENES = {
     "dam": scenes.dam_break,
@@ -132,10 +135,11 @@ def jacobi(diag, fluid):
     return lambda r: r * inv
 
 
-def net_precond(model, fluid, w):
+def net_precond(trainer, diag, fluid):
+    """z = network(r) through the library's kernels (the trainer's weights)."""
     def apply(r):
-        with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
-            return model(r, fluid, w).float()
+        z = torch.empty_like(r)
+        return trainer.forward(r.contiguous(), diag, fluid, z)
     return apply
 
 
@@ -197,7 +201,6 @@ def harvest(pairs, spec, rhs_per_step, gen, dev):
 def collect_fpcg(pairs, rhs, model, limit, gen):
     """Residuals of flexible PCG driven by the current network, with their exact
     errors x* - x_k, for up to `limit` of the given right-hand sides."""
-    model.eval()
     added = 0
     for n, lst in rhs.items():
         if not lst:
@@ -205,12 +208,11 @@ def collect_fpcg(pairs, rhs, model, limit, gen):
         perm = torch.randperm(len(lst), generator=gen).tolist()[:max(1, limit // len(rhs))]
         for i in perm:
             gi, b, xs = lst[i]
-            diag, fluid, w = pairs.geo[n][gi]
+            diag, fluid, _w = pairs.geo[n][gi]
             before = pairs.count()
-            for k, rk, xk in pcg(b, diag, fluid, net_precond(model, fluid, w), max(KSET), KSET):
+            for k, rk, xk in pcg(b, diag, fluid, net_precond(model, diag, fluid), max(KSET), KSET):
                 pairs.add_residual(n, gi, rk, xs - xk, b)
             added += pairs.count() - before
-    model.train()
     return added
 
 
@@ -249,23 +251,17 @@ def stack(pairs, n, ids):
 
 
 def evaluate(model, pairs, held, bs):
-    """Mean loss over the held-out pairs (batched like training: the BF16
-    convolutions are compiled per shape)."""
-    model.eval()
+    """Mean loss over the held-out pairs (no update)."""
     tot, cnt = 0.0, 0
     by = {}
     for n, i in sorted(held):
         by.setdefault(n, []).append(i)
-    with torch.no_grad():
-        for n, ids in by.items():
-            b = bs.get(n, 1)
-            for k in range(0, len(ids) - b + 1, b):
-                r, e, diag, fluid, w = stack(pairs, n, ids[k:k + b])
-                with torch.autocast("cuda", dtype=torch.bfloat16):
-                    z = model(r, fluid, w)
-                tot += float(lnet.a_angle_loss(z.float(), r, e, diag, fluid)) * b
-                cnt += b
-    model.train()
+    for n, ids in by.items():
+        b = bs.get(n, 1)
+        for k in range(0, len(ids) - b + 1, b):
+            r, e, diag, fluid, _w = stack(pairs, n, ids[k:k + b])
+            tot += model.step(r, e, diag, fluid, update=False) * b
+            cnt += b
     return tot / max(cnt, 1)
 
 
@@ -277,8 +273,8 @@ def torch_fpcg_check(model, pairs, held_geo):
         if not lst:
             continue
         _gi, b, _xs = lst[0]
-        diag, fluid, w = pairs.geo[n][gi]
-        it_net = run_pcg(b, diag, fluid, net_precond(model, fluid, w), 600, tol=1e-6)
+        diag, fluid, _w = pairs.geo[n][gi]
+        it_net = run_pcg(b, diag, fluid, net_precond(model, diag, fluid), 600, tol=1e-6)
         it_jac = run_pcg(b, diag, fluid, jacobi(diag, fluid), 3000, tol=1e-6)
         out.append({"n": n, "geo": gi, "net_fpcg": it_net, "jacobi_pcg": it_jac})
         log("torch check n=%d geo %d: net FPCG %d it, Jacobi PCG %d it" % (n, gi, it_net, it_jac))
@@ -355,11 +351,11 @@ def main(argv=None):
                 if (n, t[0]) in hg}
 
     train_rhs = {n: [t for t in lst if (n, t[0]) not in hg] for n, lst in pairs.rhs.items()}
-    model = lnet.PressureNet(seed=a.seed)
     if a.init:
-        model = lnet.PressureNet.from_blob(open(a.init, "rb").read())
-    model = model.to(dev)
-    opt = torch.optim.Adam(model.parameters(), lr=a.lr)
+        model = Trainer(open(a.init, "rb").read())
+    else:
+        init = lnet.PressureNet(seed=a.seed)
+        model = Trainer(layers_to_flat([p.detach().numpy() for p in init.w]))
     bs = {32: 8, 48: 4, 64: 4, 96: 1, 128: 1}
     budget = a.minutes * 60.0
     held = held_set()
@@ -378,27 +374,20 @@ def main(argv=None):
             for n, ids in batches(pairs, held, bs, cgen):
                 t_b = time.time()
                 frac = min(trained / budget, 1.0)
-                for g in opt.param_groups:
-                    g["lr"] = a.lr * 0.5 * (1.0 + math.cos(math.pi * frac))
-                r, e, diag, fluid, w = stack(pairs, n, ids)
-                with torch.autocast("cuda", dtype=torch.bfloat16):
-                    z = model(r, fluid, w)
-                loss = lnet.a_angle_loss(z.float(), r, e, diag, fluid)
-                opt.zero_grad(set_to_none=True)
-                if not bool(torch.isfinite(loss)):
+                lr = a.lr * 0.5 * (1.0 + math.cos(math.pi * frac))
+                r, e, diag, fluid, _w = stack(pairs, n, ids)
+                loss = model.step(r, e, diag, fluid, lr=lr, clip=1.0)
+                if not math.isfinite(loss):
                     skipped += 1
                     continue
-                loss.backward()
-                torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
-                opt.step()
-                run += float(loss.detach())
+                run += loss
                 cnt += 1
                 step += 1
                 trained += time.time() - t_b
                 if time.time() - last > 20.0:
                     last = time.time()
                     log("round %d epoch %d step %d loss %.4f lr %.2e skipped %d" % (
-                        rnd, epoch, step, run / max(cnt, 1), opt.param_groups[0]["lr"], skipped))
+                        rnd, epoch, step, run / max(cnt, 1), lr, skipped))
                 if trained >= r_end:
                     break
             ho = evaluate(model, pairs, held, bs)
@@ -412,7 +401,7 @@ def main(argv=None):
             torch_fpcg_check(model, pairs, held_geo[:2])
 
     fp = torch_fpcg_check(model, pairs, held_geo)
-    blob = model.cpu().blob()
+    blob = model.blob()
     with open(os.path.join(a.out, "weights.bin"), "wb") as f:
         f.write(blob)
     check = []

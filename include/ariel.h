@@ -323,6 +323,34 @@ int ariel_batch_step(ariel_batch* b, int32_t nsteps);
 /* nsteps steps as a loop of ariel_step over the tanks: the baseline. */
 int ariel_batch_step_sequential(ariel_batch* b, int32_t nsteps);
 
+/* ---------------------------------------------------------------- training
+ * Training of the learned preconditioner's network on the GPU with this
+ * library's own forward (BF16 MFMA) and backward (FP32) kernels and Adam on
+ * FP32 master weights (no counterpart in the reference). Weights are the 11
+ * layers in the packed layout of the ARIELNN1 blob, as FP32. Data pointers are
+ * DEVICE pointers to `batch` consecutive grids of N^3 floats each (N a
+ * multiple of 16): residuals r, exact errors e with A e = r, the matrix
+ * diagonal diag (ADiag, unscaled) and the FLUID mask fluid (1/0). The loss of a
+ * pair is the signed A-angle 1 - (r.z)|r.z|/((r.e)(z.Az)) of the network
+ * output z. */
+typedef struct ariel_trainer ariel_trainer;
+
+int ariel_trainer_create(ariel_trainer** out);
+int ariel_trainer_destroy(ariel_trainer* t);
+const char* ariel_trainer_last_error(const ariel_trainer* t);
+/* n = the packed weight count (248064); set resets the Adam state. */
+int ariel_trainer_set_weights(ariel_trainer* t, const float* w, int64_t n);
+int ariel_trainer_get_weights(ariel_trainer* t, float* w, int64_t n);
+/* Mean loss over the batch into *loss; with update != 0 one Adam step (learning
+ * rate lr, gradient clipped to norm clip if clip > 0); grad (host, n floats,
+ * may be NULL) receives the batch-mean gradient. */
+int ariel_trainer_step(ariel_trainer* t, int32_t N, int32_t batch, const float* r, const float* e,
+                       const float* diag, const float* fluid, float lr, float clip, int32_t update,
+                       double* loss, float* grad);
+/* z = network(r) for each grid of the batch (device pointers). */
+int ariel_trainer_forward(ariel_trainer* t, int32_t N, int32_t batch, const float* r,
+                          const float* diag, const float* fluid, float* z);
+
 /* Self-check of the device division routines against IEEE division on the
  * current device (no counterpart in the reference: it guards the bit-exactness
  * of Smooth and Sharpen inside the kernels). For every float x with

@@ -107,4 +107,6 @@ def test_a_angle_loss_is_zero_at_the_solution():
     sh = (1, 1, n, n, n)
     args = (r.float().view(sh), x.float().view(sh), diag.view(sh), fluid.view(sh))
     assert float(lnet.a_angle_loss(x.float().view(sh) * 3.0, *args)) == pytest.approx(0.0, abs=1e-5)
+    # signed: the reversed solution is the worst direction, not a second minimum
+    assert float(lnet.a_angle_loss(x.float().view(sh) * -3.0, *args)) == pytest.approx(2.0, abs=1e-5)
     assert float(lnet.a_angle_loss(r.float().view(sh), *args)) > 1e-3
