@@ -56,6 +56,10 @@ def parse():
                          "at 128^3 and one at 256^3 solved from zero to 1e-6 with MIC(0)-PCG "
                          "and with flexible PCG on the network; iterations and solve time")
     ap.add_argument("--weights", default=None, help="learned weights (default: shipped)")
+    ap.add_argument("--table", action="store_true",
+                    help="the per-configuration table instead: the reference's algorithm on the "
+                         "host cores (all, and one), the GPU step with MIC(0) and with the learned "
+                         "preconditioner, PCG ms, P2G/SpMV algorithmic HBM fractions")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (a plain copy "
                          "kernel of known traffic, to scale FETCH_SIZE/WRITE_SIZE)")
@@ -234,8 +238,118 @@ def learned(args):
     print(json.dumps(out), flush=True)
 
 
+def _gpu_steps(desc, warm, steps):
+    """ms/step of `steps` GPU steps after `warm`, PCG ms and iterations, then two
+    profiled steps for the P2G and SpMV per-launch algorithmic HBM fractions."""
+    from ariel_amd import FlipSim, _abi
+    sim = FlipSim(desc)
+    sim.Init()
+    sim.Step(warm) if warm else None
+    sim.sync()
+    solve, its, ok = [], [], True
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.Step(1)
+        st = sim.stats()
+        solve.append(st.solve_ms)
+        its.append(st.iterations)
+        ok = ok and st.solve_status == 0
+    sim.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    sim.set_profiling(True)
+    sim.reset_kernel_stats()
+    sim.Step(2)
+    fr = {}
+    for k, name in ((_abi.KERNEL_P2G, "p2g"), (_abi.KERNEL_SPMV, "spmv")):
+        ks = sim.kernel_stats(k)
+        avg = ks.total_ms / max(ks.launches, 1)
+        fr[name] = ks.bytes_per_launch / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS if ks.launches else None
+    n_particles = sim.num_particles
+    sim.close()
+    return {"ms_per_step": ms, "pcg_ms": float(np.mean(solve)), "pcg_iterations": float(np.mean(its)),
+            "all_converged": ok, "p2g_hbm_frac": fr["p2g"], "spmv_hbm_frac": fr["spmv"],
+            "particles": int(n_particles), "steps": steps, "after_warmup": warm}
+
+
+def _cpu_reference(desc, threads, steps):
+    """The oracle in reference mode (the reference's preconditioner, FP32 serial
+    dots and absolute stop rule r.r <= 1e-2 N^3): ms/step from initialisation."""
+    from oracle import OracleSim
+    from oracle.oracle import FAITHFUL
+    o = OracleSim(desc, threads=threads)
+    o.init()
+    ms, its = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        o.step(mode=FAITHFUL)
+        ms.append((time.perf_counter() - t0) * 1e3)
+        its.append(o.stats().iterations)
+    o.close()
+    return {"ms_per_step": float(np.mean(ms)), "pcg_iterations": float(np.mean(its)),
+            "threads": threads, "steps": steps}
+
+
+def table(args):
+    """Configurations 1-5 side by side (see _gpu_steps and _cpu_reference)."""
+    from ariel_amd import FlipBatch, _abi, scenes
+    cores = max(1, min(16, os.cpu_count() or 1))
+    learned_kw = {"precond": _abi.PRECOND_LEARNED}
+    cfgs = [
+        ("1", "dam break 32^3, 50 steps", lambda **k: scenes.dam_break(32, **k),
+         (0, 50), (0, 50), (50, 50)),
+        ("2", "dam break + sphere 128^3", lambda **k: scenes.dam_break_sphere(128, **k),
+         (5, 30), (5, 10), (3, 1)),
+        ("3", "pool + falling cube 256^3", lambda **k: scenes.pool_and_cube(256, **k),
+         (3, 10), (2, 3), (1, 0)),
+    ]
+    rows = []
+    for cid, name, fn, plain, lrn, cpu in cfgs:
+        row = {"config": cid, "workload": name}
+        log(f"config {cid}: GPU MIC(0)")
+        row["gpu_mic0"] = _gpu_steps(fn(), *plain)
+        log(f"config {cid}: GPU learned")
+        row["gpu_learned"] = _gpu_steps(fn(**learned_kw), *lrn)
+        log(f"config {cid}: CPU reference algorithm, {cores} threads, {cpu[0]} steps")
+        row["cpu_all_cores"] = _cpu_reference(fn(), cores, cpu[0])
+        if cpu[1]:
+            log(f"config {cid}: CPU reference algorithm, 1 thread, {cpu[1]} steps")
+            row["cpu_one_core"] = _cpu_reference(fn(), 1, cpu[1])
+        log(f"config {cid}: {json.dumps(row)}")
+        rows.append(row)
+    # configuration 4: 64 tanks, batched vs a loop of single-tank steps; the CPU
+    # reference steps each tank once
+    descs = scenes.batched_tanks(64, seed=2)
+    b = FlipBatch(descs)
+    b.Init()
+    b.autotune()
+    b.set_graphs(True)
+    b.Step(3)
+    t0 = time.perf_counter()
+    b.Step(10)
+    t_b = (time.perf_counter() - t0) * 1e3 / 10
+    b.step_sequential(2)
+    t0 = time.perf_counter()
+    b.step_sequential(5)
+    t_s = (time.perf_counter() - t0) * 1e3 / 5
+    b.close()
+    log("config 4: CPU reference algorithm, one step per tank")
+    t_cpu = 0.0
+    for i, d in enumerate(descs):
+        t_cpu += _cpu_reference(d, cores, 1)["ms_per_step"]
+        if i % 8 == 7:
+            log(f"config 4: CPU, {i + 1} tanks done")
+    rows.append({"config": "4", "workload": "64 dam-break tanks, edges 48-96",
+                 "gpu_batched_ms_per_step": t_b, "gpu_sequential_ms_per_step": t_s,
+                 "cpu_all_cores": {"ms_per_step": t_cpu, "threads": cores, "steps": 1}})
+    log(f"config 4: batched {t_b:.1f} ms, sequential {t_s:.1f} ms, CPU {t_cpu:.0f} ms per step")
+    print(json.dumps({"table": rows, "host_threads": cores}), flush=True)
+
+
 def main():
     args = parse()
+    if args.table:
+        table(args)
+        return
     if args.learned:
         learned(args)
         return
