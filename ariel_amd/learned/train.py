@@ -11,7 +11,9 @@
    (b, x*) and, from a Jacobi-PCG run on the same system, the pairs
    (r_k, x* - x_k) at several iterations k: A (x* - x_k) = r_k, and r_k has the
    roughness and surface concentration of the residuals a preconditioner
-   actually meets inside PCG.
+   actually meets inside PCG. The step's own solve adds its warm-started
+   system: r0 = b - A p_prev with the error P - p_prev (the library's solve
+   from the previous pressure p_prev), and Jacobi-PCG pairs on it.
 2. Fit the network with the library's own training kernels (``Trainer``:
    BF16 MFMA forward, FP32 backward, Adam on FP32 master weights) by
    minimising the A-angle loss between its output and the solution, A built
@@ -163,8 +165,11 @@ def harvest(pairs, spec, rhs_per_step, gen, dev):
         sim.Init()
         t = time.time()
         for st in range(steps):
+            harvest_step = (st + 1) % every == 0
+            # the warm start of this step's solve: the previous step's pressure
+            P_prev = sim.get_field(_abi.FIELD_P) if harvest_step else None
             sim.Step(1)
-            if (st + 1) % every:
+            if not harvest_step:
                 continue
             flags = sim.get_flags()
             if not (flags == lnet.FLUID).any():
@@ -179,6 +184,16 @@ def harvest(pairs, spec, rhs_per_step, gen, dev):
             fluid = fluid.view(1, 1, n, n, n)
             gi = pairs.add_geo(n, diag, fluid)
             b0 = torch.from_numpy(D).to(dev).view(1, 1, n, n, n) * fluid
+            # the step's own warm-started system: r0 = b - A p_prev, error P - p_prev
+            # (P solves it to 1e-6 of |r0|)
+            pp = torch.from_numpy(P_prev).to(dev).view(1, 1, n, n, n) * fluid
+            r0 = b0 - lnet.apply_laplacian(pp, diag, fluid)
+            e0 = torch.from_numpy(P).to(dev).view(1, 1, n, n, n) * fluid - pp
+            if float(dot(r0, r0)) > 0.0:
+                pairs.add(n, gi, r0, e0)
+                pairs.rhs.setdefault(n, []).append((gi, r0, e0))
+                for k, rk, xk in pcg(r0, diag, fluid, jacobi(diag, fluid), max(KSET), KSET):
+                    pairs.add_residual(n, gi, rk, e0 - xk, r0)
             for j, b in enumerate(rhs_set(b0, fluid, rhs_per_step, gen)):
                 x, sst = sim.solve_rhs(b.reshape(-1).cpu().numpy())
                 if sst.solve_status != 0:
@@ -304,6 +319,22 @@ def library_check(blob, spec):
                "mic_true_rel": sm.true_residual_norm / max(sm.rhs_norm, 1e-30)}
         sim.close()
         mic.close()
+        # inside a simulation: warm-started solves of the next steps
+        its = {}
+        for tag, kw in (("learned", {"precond": _abi.PRECOND_LEARNED}), ("mic", {})):
+            s2 = FlipSim(SCENES[name](n, **kw))
+            if kw:
+                s2.set_learned_weights(blob)
+            s2.Init()
+            s2.Step(steps)
+            acc = []
+            for _ in range(3):
+                s2.Step(1)
+                acc.append(s2.stats().iterations)
+            its[tag] = float(sum(acc) / len(acc))
+            s2.close()
+        res["steps_learned_iters"] = its["learned"]
+        res["steps_mic_iters"] = its["mic"]
         log("library check %s" % json.dumps(res))
         out.append(res)
     return out
