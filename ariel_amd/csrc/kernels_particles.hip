@@ -538,97 +538,7 @@ struct KMark {
 // face normal, 4 along the others. Fused: StorePreviousGrid (flip.cpp:324-363,
 // copy before the boundary pass) and EnforceBoundaryVelocity
 // (particlegridoperations.inl:45-104).
-struct KP2g {
-    static constexpr int kThreads = kBlock;
-    struct Args {
-        Params p;
-        const float4* pos;
-        const float4* vel;
-        const int32_t* offsets;
-        const uint8_t* A;
-        float* ux;
-        float* uy;
-        float* uz;
-        float* dux;
-        float* duy;
-        float* duz;
-    };
-    static __device__ void run(const Args& kargs) {
-        const Params& p = kargs.p;
-        const float4* __restrict__ pos = kargs.pos;
-        const float4* __restrict__ vel = kargs.vel;
-        const int32_t* __restrict__ offsets = kargs.offsets;
-        const uint8_t* __restrict__ A = kargs.A;
-        float* __restrict__ ux = kargs.ux;
-        float* __restrict__ uy = kargs.uy;
-        float* __restrict__ uz = kargs.uz;
-        float* __restrict__ dux = kargs.dux;
-        float* __restrict__ duy = kargs.duy;
-        float* __restrict__ duz = kargs.duz;
-
-        const int N = p.N;
-        const int64_t nf = (int64_t)(N + 1) * N * N;
-        int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (g >= 3 * nf) return;
-        const int comp = (int)(g / nf);
-        const int64_t f = g - comp * nf;
-        int i, j, k;
-        if (comp == 0) { k = (int)(f % N); j = (int)((f / N) % N); i = (int)(f / ((int64_t)N * N)); }
-        else if (comp == 1) { k = (int)(f % N); j = (int)((f / N) % (N + 1)); i = (int)(f / ((int64_t)N * (N + 1))); }
-        else { k = (int)(f % (N + 1)); j = (int)((f / (N + 1)) % N); i = (int)(f / ((int64_t)N * (N + 1))); }
-        const float maxd = (float)N;
-        const float RE = p.sharpen_re;
-        const bool tri = p.splat == 1;
-        float fx = (float)i, fy = (float)j, fz = (float)k;
-        int xlo, xhi, ylo, yhi, zlo, zhi;
-        if (comp == 0) { fy += 0.5f; fz += 0.5f; xlo = i - 1; xhi = i; ylo = j - 2; yhi = j + 1; zlo = k - 2; zhi = k + 1; }
-        else if (comp == 1) { fx += 0.5f; fz += 0.5f; xlo = i - 2; xhi = i + 1; ylo = j - 1; yhi = j; zlo = k - 2; zhi = k + 1; }
-        else { fx += 0.5f; fy += 0.5f; xlo = i - 2; xhi = i + 1; ylo = j - 2; yhi = j + 1; zlo = k - 1; zhi = k; }
-        zlo = max(zlo, 0);
-        zhi = min(zhi, N - 1);
-        float sumw = 0.0f, sumc = 0.0f;
-        if (zlo <= zhi) {
-            for (int sx = max(xlo, 0); sx <= min(xhi, N - 1); ++sx) {
-                for (int sy = max(ylo, 0); sy <= min(yhi, N - 1); ++sy) {
-                    const int64_t col = cidx(N, sx, sy, 0);
-                    const int32_t q0 = offsets[col + zlo], q1 = offsets[col + zhi + 1];
-                    for (int32_t q = q0; q < q1; ++q) {
-                        const float4 o = pos[q];
-                        float px = gmax(0.0f, gmin(maxd, maxd * o.x));
-                        float py = gmax(0.0f, gmin(maxd, maxd * o.y));
-                        float pz = gmax(0.0f, gmin(maxd, maxd * o.z));
-                        float w;
-                        if (tri) {
-                            w = (gmax(1.0f - fabsf(px - fx), 0.0f) * gmax(1.0f - fabsf(py - fy), 0.0f)) *
-                                gmax(1.0f - fabsf(pz - fz), 0.0f);
-                        } else {
-                            w = 1.0f * sharpen(sqrlength(px, py, pz, fx, fy, fz), RE);
-                        }
-                        const float4 v = vel[q];
-                        const float vc = comp == 0 ? v.x : (comp == 1 ? v.y : v.z);
-                        sumc += w * vc;
-                        sumw += w;
-                    }
-                }
-            }
-        }
-        float val = 0.0f;
-        if (sumw > 0) val = sumc / sumw;
-        float* du = comp == 0 ? dux : (comp == 1 ? duy : duz);
-        float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
-        du[f] = val;
-        // boundary: domain faces, and faces between a SOLID and a non-SOLID cell
-        int a = comp == 0 ? i : (comp == 1 ? j : k);
-        bool zero = (a == 0 || a == N);
-        if (!zero) {
-            int bi = i - (comp == 0), bj = j - (comp == 1), bk = k - (comp == 2);
-            bool s0 = A[cidx(N, i, j, k)] == kSolid, s1 = A[cidx(N, bi, bj, bk)] == kSolid;
-            zero = s0 != s1;
-        }
-        u[f] = zero ? 0.0f : val;
-    }
-};
-
+//
 // P2G through LDS. A 64-lane block owns the cells (i, j, k0..k0+63) and computes
 // the x-, y- and z-faces of each (faces at i = N, j = N, k = N included). The
 // union of the three reference windows is the 4x4 block of particle columns
@@ -637,7 +547,7 @@ struct KP2g {
 // staged in LDS with their clamped grid positions, and every face adds the
 // particles of its own window in ascending order: per face the additions happen
 // in exactly the reference's order (sx -> sy -> sz -> list), so the sums are the
-// same as the per-face gather's. Weights beyond the kernel radius are exactly
+// same as a per-face gather's. Weights beyond the kernel radius are exactly
 // zero and skipped (adding +0 to a sum that starts at +0 never changes it).
 constexpr int kP2GCap = 256;   // particles staged per pass (small: occupancy hides the staging)
 
@@ -1269,14 +1179,6 @@ void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4*
         launch<KP2gCols<true>>(s, dim3((unsigned)xcd_grid(nb)), dim3(64), {p, pos, vel, offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]});
     else
         launch<KP2gCols<false>>(s, dim3((unsigned)xcd_grid(nb)), dim3(64), {p, pos, vel, offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]});
-}
-
-// Reference-shaped per-face gather (one thread per face), kept for cross-checks.
-void launch_p2g_faces(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
-                      const int32_t* offsets, const uint8_t* A, float* const u[3],
-                      float* const du[3]) {
-    int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
-    launch<KP2g>(s, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), {p, pos, vel, offsets, A, u[0], u[1], u[2], du[0], du[1], du[2]});
 }
 
 void launch_g2p_advect(hipStream_t s, const Params& p, float4* pos, float4* vel, int64_t np,
