@@ -156,3 +156,14 @@ def test_batch_rejects_mixed_solver_settings():
     arr[0].precond = _abi.PRECOND_LEARNED
     assert lib.ariel_batch_create(arr, 1, ctypes.byref(h)) == _abi.ERR_UNSUPPORTED
     assert b"single tanks" in lib.ariel_batch_last_error(None)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_trainer_without_gpu_fails_loudly():
+    import ctypes
+    from ariel_amd import _abi
+    lib = _abi.load()
+    h = ctypes.c_void_p()
+    assert lib.ariel_trainer_create(ctypes.byref(h)) == _abi.ERR_HIP
+    assert not h.value
+    assert b"trainer_create" in lib.ariel_last_error(None)
