@@ -114,6 +114,7 @@ PROTOTYPES = {
     "ariel_batch_create": (_I32, [ctypes.POINTER(SceneDesc), _I32, ctypes.POINTER(_P)]),
     "ariel_batch_destroy": (_I32, [_P]),
     "ariel_batch_last_error": (ctypes.c_char_p, [_P]),
+    "ariel_desc_from_json": (_I32, [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]),
     "ariel_trainer_create": (_I32, [ctypes.POINTER(_P)]),
     "ariel_trainer_destroy": (_I32, [_P]),
     "ariel_trainer_last_error": (ctypes.c_char_p, [_P]),

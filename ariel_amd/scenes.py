@@ -159,6 +159,24 @@ def batched_tanks(ntanks=64, seed=2, sizes=(48, 64, 80, 96), lo=0.2, hi=0.6, **o
     return out
 
 
+def from_json(text_or_path, **overrides):
+    """A descriptor from the reference's JSON scene format (a path or the text),
+    through ariel_desc_from_json; keyword overrides set descriptor fields after."""
+    import os
+    text = text_or_path
+    if isinstance(text_or_path, str) and os.path.exists(text_or_path):
+        text = open(text_or_path).read()
+    lib = _abi.load()
+    d = SceneDesc()
+    rc = lib.ariel_desc_from_json(text.encode(), d)
+    if rc != 0:
+        from .flipsim import ArielError
+        raise ArielError(rc, lib.ariel_last_error(None).decode())
+    for k, v in overrides.items():
+        setattr(d, k, v)
+    return d
+
+
 def copy(d):
     out = SceneDesc()
     out_bytes = bytes(d)

@@ -294,6 +294,14 @@ int ariel_set_learned_weights(ariel_sim* h, const void* blob, int64_t bytes);
  * are N^3 host floats; overwrites the handle's divergence and pressure. */
 int ariel_solve_rhs(ariel_sim* h, const float* rhs, float* x, ariel_step_stats* stats);
 
+/* The reference's JSON scene (src/scene/sceneloader.cpp) as a descriptor:
+ * defaults, then settings (density, step_size, dim), globalforces, box and
+ * sphere mesh generators with translation/scale transforms, one-frame mesh
+ * geoms and liquid sim entries with velocities (geometry in grid cells).
+ * Meshes from files, rotations, animation, persistent emitters and solids give
+ * ARIEL_ERR_UNSUPPORTED; malformed JSON gives ARIEL_ERR_INVALID. */
+int ariel_desc_from_json(const char* json_text, ariel_scene_desc* out);
+
 /* ---------------------------------------------------------------- batches
  * Independent tanks stepped together (no counterpart in the reference, which
  * runs one FlipSim per process). Every stage of a step is one kernel launch
