@@ -34,7 +34,24 @@ namespace ariel {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x32_bf16: D(16x16) += A(16x32) B(32x16). Lane l holds A row
+// l%16 and B column l%16 at 8 k-slots; D as for the 16x16x16 form (column l%16,
+// rows 4*(l/16)+j). The instruction sums over the slots, so any labelling of
+// the slots that A and B share is valid: here lane group g = l/16 takes the 8
+// consecutive k = 8g..8g+7 of each 32-wide chunk, which with CIN a multiple of
+// 8 are 8 consecutive channels of one tap (one 16-byte load per operand).
+__device__ __forceinline__ f32x4 mfma32(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8 zero8() {
+    return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
+}
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -96,24 +113,28 @@ struct KConv {
         f32x4 acc[NB];
 #pragma unroll
         for (int n = 0; n < NB; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int t = 0; t < TAPS; ++t) {
-            const int di = TAPS == 27 ? t / 9 - 1 : 0, dj = TAPS == 27 ? (t / 3) % 3 - 1 : 0,
-                      dk = TAPS == 27 ? t % 3 - 1 : 0;
-            const int ii = oi * STRIDE + di, jj = oj * STRIDE + dj, kk = ok * STRIDE + dk;
-            const bool inb = vin && ii >= 0 && ii < a.Din && jj >= 0 && jj < a.Din && kk >= 0 &&
-                             kk < a.Din;
-            const int64_t src = ((int64_t)ii * a.Din + jj) * a.Din + kk;
+        // K = taps x input channels, in chunks of 32 (see mfma32)
+        constexpr int K = TAPS * CIN, NCH = (K + 31) / 32;
+#pragma unroll 2
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int k0 = 32 * ch + 8 * kq;
+            bf16x8 av = zero8();
+            bf16x8 bv[NB];
 #pragma unroll
-            for (int kc = 0; kc < CIN / 16; ++kc) {
-                bf16x4 av = {0, 0, 0, 0};
-                if (inb) av = *reinterpret_cast<const bf16x4*>(a.in + src * CIN + kc * 16 + kq * 4);
+            for (int n = 0; n < NB; ++n) bv[n] = zero8();
+            if (k0 < K) {
+                const int t = k0 / CIN, c0 = k0 - t * CIN;
+                const int di = TAPS == 27 ? t / 9 - 1 : 0, dj = TAPS == 27 ? (t / 3) % 3 - 1 : 0,
+                          dk = TAPS == 27 ? t % 3 - 1 : 0;
+                const int ii = oi * STRIDE + di, jj = oj * STRIDE + dj, kk = ok * STRIDE + dk;
+                if (vin && ii >= 0 && ii < a.Din && jj >= 0 && jj < a.Din && kk >= 0 && kk < a.Din)
+                    av = load8(a.in + (((int64_t)ii * a.Din + jj) * a.Din + kk) * CIN + c0);
 #pragma unroll
-                for (int n = 0; n < NB; ++n) {
-                    const bf16x4 bv = *reinterpret_cast<const bf16x4*>(
-                        a.w + ((int64_t)t * COUT + (nb0 + n) * 16 + row) * CIN + kc * 16 + kq * 4);
-                    acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[n], 0, 0, 0);
-                }
+                for (int n = 0; n < NB; ++n)
+                    bv[n] = load8(a.w + ((int64_t)t * COUT + (nb0 + n) * 16 + row) * CIN + c0);
             }
+#pragma unroll
+            for (int n = 0; n < NB; ++n) acc[n] = mfma32(av, bv[n], acc[n]);
         }
         // D: column (output channel) = lane%16, rows (voxels) 4*(lane/16)+j
 #pragma unroll
@@ -229,7 +250,11 @@ __device__ __forceinline__ void stage(uint16_t* s_in, const uint16_t* in, const 
 template <int CIN, int COUT, int MODE>
 struct KConvTile {
     static constexpr int kThreads = 256;
-    static constexpr int CP = MODE == tile::kInput ? 2 : CIN + 4;   // LDS channel stride
+    // LDS channel stride: 16-channel layers use 16x16x16 MFMAs over one tap each
+    // (8-byte rows; measured faster than pairing taps into 16x16x32), wider ones
+    // 16x16x32 over 8-channel slices (16-byte rows)
+    static constexpr bool K16 = MODE != tile::kInput && CIN == 16;
+    static constexpr int CP = MODE == tile::kInput ? 2 : (K16 ? CIN + 4 : CIN + 8);
     static constexpr int NB = COUT / 16;
     // the input layer's weights (2 KB) in LDS; the others' from global memory
     // (LDS-resident 16x16 weights lowered occupancy and ran slower)
@@ -290,53 +315,66 @@ struct KConvTile {
 #pragma unroll
             for (int n = 0; n < NB; ++n) acc[j][n] = f32x4{0.f, 0.f, 0.f, 0.f};
         if constexpr (MODE == kInput) {
+            // K = 2 channels x 27 taps packed as k = 2*tap + c (64 with padding):
+            // a lane's 8 slots are 4 consecutive taps, both channels
             const uint32_t* s32 = reinterpret_cast<const uint32_t*>(s_in);
 #pragma unroll
-            for (int kc = 0; kc < 4; ++kc) {
-                const bf16x4 bv = *reinterpret_cast<const bf16x4*>(W + row * 64 + kc * 16 + kq * 4);
-                const int t0 = kc * 8 + kq * 2;
+            for (int ch = 0; ch < 2; ++ch) {
+                const int k0 = 32 * ch + 8 * kq;
+                const bf16x8 bv = load8(W + row * 64 + k0);
+                const int t0 = k0 >> 1;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int col = wave * 4 + j, cx = col / TY, cy = col % TY;
-                    uint32_t p0 = 0u, p1 = 0u;
-                    if (t0 < 27) {
-                        const int hv = ((cx + 1 + t0 / 9 - 1) * HY + (cy + 1 + (t0 / 3) % 3 - 1)) * HZ +
-                                       row + 1 + t0 % 3 - 1;
-                        p0 = s32[hv];
+                    uint32_t p[4];
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const int t = t0 + m;
+                        p[m] = 0u;
+                        if (t < 27)
+                            p[m] = s32[((cx + t / 9) * HY + (cy + (t / 3) % 3)) * HZ + row + t % 3];
                     }
-                    if (t0 + 1 < 27) {
-                        const int t1 = t0 + 1;
-                        const int hv = ((cx + 1 + t1 / 9 - 1) * HY + (cy + 1 + (t1 / 3) % 3 - 1)) * HZ +
-                                       row + 1 + t1 % 3 - 1;
-                        p1 = s32[hv];
-                    }
-                    bf16x4 av;
-                    av[0] = (short)(p0 & 0xFFFFu);
-                    av[1] = (short)(p0 >> 16);
-                    av[2] = (short)(p1 & 0xFFFFu);
-                    av[3] = (short)(p1 >> 16);
-                    acc[j][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[j][0], 0, 0, 0);
+                    const bf16x8 av = __builtin_bit_cast(bf16x8, make_uint4(p[0], p[1], p[2], p[3]));
+                    acc[j][0] = mfma32(av, bv, acc[j][0]);
+                }
+            }
+        } else if constexpr (K16) {
+            for (int t = 0; t < 27; ++t) {
+                const int di = t / 9, dj = (t / 3) % 3, dk = t % 3;
+                bf16x4 bv[NB];
+#pragma unroll
+                for (int n = 0; n < NB; ++n)
+                    bv[n] = *reinterpret_cast<const bf16x4*>(W + (t * COUT + n * 16 + row) * CIN + kq * 4);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int col = wave * 4 + j, cx = col / TY, cy = col % TY;
+                    const int hv = ((cx + di) * HY + (cy + dj)) * HZ + row + dk;
+                    const bf16x4 av = *reinterpret_cast<const bf16x4*>(s_in + hv * CP + kq * 4);
+#pragma unroll
+                    for (int n = 0; n < NB; ++n)
+                        acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[n], acc[j][n], 0, 0, 0);
                 }
             }
         } else {
-            for (int t = 0; t < 27; ++t) {
+            // K = 27 taps x CIN in chunks of 32 (see mfma32)
+            constexpr int K = 27 * CIN, NCH = (K + 31) / 32;
+#pragma unroll 2
+            for (int ch = 0; ch < NCH; ++ch) {
+                const int k0 = 32 * ch + 8 * kq;
+                const bool live = k0 < K;
+                const int t = live ? k0 / CIN : 0, c0 = live ? k0 - t * CIN : 0;
                 const int di = t / 9, dj = (t / 3) % 3, dk = t % 3;
+                bf16x8 bv[NB];
 #pragma unroll
-                for (int kc = 0; kc < CIN / 16; ++kc) {
-                    bf16x4 bv[NB];
+                for (int n = 0; n < NB; ++n)
+                    bv[n] = live ? load8(W + (t * COUT + n * 16 + row) * CIN + c0) : zero8();
 #pragma unroll
-                    for (int n = 0; n < NB; ++n)
-                        bv[n] = *reinterpret_cast<const bf16x4*>(
-                            W + (t * COUT + n * 16 + row) * CIN + kc * 16 + kq * 4);
+                for (int j = 0; j < 4; ++j) {
+                    const int col = wave * 4 + j, cx = col / TY, cy = col % TY;
+                    const int hv = ((cx + di) * HY + (cy + dj)) * HZ + row + dk;
+                    const bf16x8 av = live ? load8(s_in + hv * CP + c0) : zero8();
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int col = wave * 4 + j, cx = col / TY, cy = col % TY;
-                        const int hv = ((cx + di) * HY + (cy + dj)) * HZ + row + dk;
-                        const bf16x4 av = *reinterpret_cast<const bf16x4*>(s_in + hv * CP + kc * 16 + kq * 4);
-#pragma unroll
-                        for (int n = 0; n < NB; ++n)
-                            acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[n], acc[j][n], 0, 0, 0);
-                    }
+                    for (int n = 0; n < NB; ++n) acc[j][n] = mfma32(av, bv[n], acc[j][n]);
                 }
             }
         }
