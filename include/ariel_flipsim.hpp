@@ -5,7 +5,9 @@
 //
 // Differences that a caller sees:
 //  * the scene is an ariel_scene_desc (liquid primitives, forces, seed) instead
-//    of sceneCore::Scene; JSON loading and mesh geometry stay with the caller;
+//    of sceneCore::Scene; ariel_desc_from_json reads the reference's JSON scene
+//    format (box and sphere generators), other mesh geometry stays with the
+//    caller;
 //  * GetParticles() returns a snapshot by value (device memory stays on the GPU)
 //    in particle-vector order, not a pointer to live particles;
 //  * failures throw ariel::Error carrying the ABI status and message.
