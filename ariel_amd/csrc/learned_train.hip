@@ -26,6 +26,7 @@ namespace {
 
 constexpr int kTB = 256;
 constexpr int kMaxRedBlocks = 1024;
+constexpr int kWChunks = 16384;   // weight-gradient partials per layer: chunks x taps
 
 __device__ __forceinline__ void block_sum3(double v0, double v1, double v2, double* out) {
     __shared__ double sm[3][kTB / 64];
@@ -50,6 +51,21 @@ __device__ __forceinline__ void block_sum3(double v0, double v1, double v2, doub
         out[1] = b;
         out[2] = c;
     }
+}
+
+// Sums of the first nparts triples of part, in a fixed order, by one whole
+// wave (threads 0-63 all call it); lane 0 receives the sums.
+__device__ __forceinline__ void wave_sum_parts(const double* part, int nparts, double& s0, double& s1,
+                                               double& s2) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = threadIdx.x & 63; i < nparts; i += 64) {
+        a += part[3 * i];
+        b += part[3 * i + 1];
+        c += part[3 * i + 2];
+    }
+    s0 = wave_sum(a);
+    s1 = wave_sum(b);
+    s2 = wave_sum(c);
 }
 
 // Per-sample set-up: diagonal weights wd = sqrt(6/diag) on FLUID cells, and
@@ -92,12 +108,9 @@ struct KTrainPrepFinal {
         SolverScalars* sc;
     };
     static __device__ void run(const Args& a) {
+        double r2, segs, unused;
+        wave_sum_parts(a.part, a.nparts, r2, segs, unused);
         if (threadIdx.x != 0) return;
-        double r2 = 0.0, segs = 0.0;
-        for (int b = 0; b < a.nparts; ++b) {
-            r2 += a.part[3 * b];
-            segs += a.part[3 * b + 1];
-        }
         a.sc->rnorm2 = r2;
         a.sc->nseg = (int64_t)segs;
         a.sc->done = 0;
@@ -203,13 +216,9 @@ struct KLossFinal {
         float inv_batch;
     };
     static __device__ void run(const Args& a) {
+        double rz, re, zaz;
+        wave_sum_parts(a.part, a.nparts, rz, re, zaz);
         if (threadIdx.x != 0) return;
-        double rz = 0, re = 0, zaz = 0;
-        for (int b = 0; b < a.nparts; ++b) {
-            rz += a.part[3 * b];
-            re += a.part[3 * b + 1];
-            zaz += a.part[3 * b + 2];
-        }
         re = re > 1e-30 ? re : 1e-30;
         zaz = zaz > 1e-30 ? zaz : 1e-30;
         const double arz = rz < 0.0 ? -rz : rz;
@@ -292,14 +301,17 @@ struct KBwdData {
         int Din, Dout;
         int acc;
     };
+    static constexpr int NCI = CIN / 4;   // each thread: one voxel, 4 input channels
     static __device__ void run(const Args& a) {
-        const int64_t total = (int64_t)a.Din * a.Din * a.Din * CIN;
+        static_assert(CIN % 4 == 0, "input channels in fours");
+        const int64_t total = (int64_t)a.Din * a.Din * a.Din * NCI;
         const int64_t gid = (int64_t)blockIdx.x * kTB + threadIdx.x;
         if (gid >= total) return;
-        const int ci = (int)(gid % CIN);
+        const int cb = (int)(gid % NCI) * 4;
+        const int64_t u = gid / NCI;
         int ui, uj, uk;
-        decode3(gid / CIN, a.Din, ui, uj, uk);
-        float sum = 0.0f;
+        decode3(u, a.Din, ui, uj, uk);
+        float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int t = 0; t < TAPS; ++t) {
             const int di = TAPS == 27 ? t / 9 - 1 : 0, dj = TAPS == 27 ? (t / 3) % 3 - 1 : 0,
                       dk = TAPS == 27 ? t % 3 - 1 : 0;
@@ -308,70 +320,126 @@ struct KBwdData {
             const int vi = xi / S, vj = xj / S, vk = xk / S;
             if (xi < 0 || xj < 0 || xk < 0 || vi >= a.Dout || vj >= a.Dout || vk >= a.Dout) continue;
             const float* gv = a.g + (((int64_t)vi * a.Dout + vj) * a.Dout + vk) * COUT;
-            const uint16_t* wt = a.w + (int64_t)t * WCO * CIN + ci;
+            const uint16_t* wt = a.w + (int64_t)t * WCO * CIN + cb;
 #pragma unroll 4
-            for (int co = 0; co < COUT; ++co) sum += bf2f(wt[co * CIN]) * gv[co];
+            for (int co = 0; co < COUT; ++co) {
+                const float gc = gv[co];
+                const uint2 w2 = *reinterpret_cast<const uint2*>(wt + co * CIN);
+                sum[0] += __uint_as_float(w2.x << 16) * gc;
+                sum[1] += __uint_as_float(w2.x & 0xFFFF0000u) * gc;
+                sum[2] += __uint_as_float(w2.y << 16) * gc;
+                sum[3] += __uint_as_float(w2.y & 0xFFFF0000u) * gc;
+            }
         }
-        a.dx[gid] = a.acc ? a.dx[gid] + sum : sum;
+        float4* d = reinterpret_cast<float4*>(a.dx + u * CIN + cb);
+        if (a.acc) {
+            const float4 o = *d;
+            sum[0] = o.x + sum[0];
+            sum[1] = o.y + sum[1];
+            sum[2] = o.z + sum[2];
+            sum[3] = o.w + sum[3];
+        }
+        *d = make_float4(sum[0], sum[1], sum[2], sum[3]);
     }
 };
 
 // part[chunk][t][co][ci] = sum over the chunk's output voxels v of
-// X[S v + off(t)][ci] G[v][co]; blockIdx.y = tap. PPT pairs (co, ci) per thread.
+// X[S v + off(t)][ci] G[v][co]; blockIdx.y = tap. A chunk is a run of output
+// rows (vi, vj); a row whose input row falls outside the grid is skipped whole
+// and the k range is clipped once per row, so the voxel loop carries no index
+// decode. Each thread owns an RCO x RCI block of (co, ci) pairs (one vector load
+// of X and one of G feed RCO*RCI products); the TPV threads covering all pairs
+// form a voxel lane, VL lanes split each row's voxels, and the lanes' sums are
+// combined through LDS in a fixed order.
 template <int TAPS, int S, int CIN, int COUT, bool XF32>
 struct KBwdWeight {
     static constexpr int kThreads = kTB;
     static constexpr int PAIRS = CIN * COUT;
-    static constexpr int PPT = (PAIRS + kTB - 1) / kTB;
+    static constexpr int RCI = CIN % 4 == 0 ? 4 : CIN;
+    static constexpr int RCO = COUT % 4 == 0 ? 4 : COUT;
+    static constexpr int NCI = CIN / RCI;
+    static constexpr int TPV = (COUT / RCO) * NCI;
+    static constexpr int VL = kTB / TPV;
     struct Args {
         const void* x;
         const float* g;
         float* part;
         int Din, Dout;
-        int64_t per_chunk;
+        int rows_per_chunk;
     };
     static __device__ void run(const Args& a) {
+        static_assert(TPV <= kTB && CIN % RCI == 0 && COUT % RCO == 0, "pair tiling");
+        static_assert(RCI == 4 || RCI == 2 || RCI == 1, "x vector width");
+        static_assert(RCO == 4 || RCO == 1, "g vector width");
+        static_assert(!XF32 || RCI == 2, "the f32 input is read in pairs");
+        __shared__ float s_red[VL * PAIRS];
         const int t = blockIdx.y;
         const int di = TAPS == 27 ? t / 9 - 1 : 0, dj = TAPS == 27 ? (t / 3) % 3 - 1 : 0,
                   dk = TAPS == 27 ? t % 3 - 1 : 0;
-        const int64_t nv = (int64_t)a.Dout * a.Dout * a.Dout;
-        const int64_t v0 = (int64_t)blockIdx.x * a.per_chunk;
-        const int64_t v1 = v0 + a.per_chunk < nv ? v0 + a.per_chunk : nv;
-        float acc[PPT];
-        int co[PPT], ci[PPT];
+        const int rows = a.Dout * a.Dout;
+        const int r0 = blockIdx.x * a.rows_per_chunk;
+        const int r1 = r0 + a.rows_per_chunk < rows ? r0 + a.rows_per_chunk : rows;
+        // output k range whose input k = S k + dk lies in [0, Din)
+        const int k0 = dk < 0 ? 1 : 0;
+        const int kmax = (a.Din - 1 - dk) / S + 1;
+        const int k1 = kmax < a.Dout ? kmax : a.Dout;
+        const int vl = threadIdx.x / TPV, q = threadIdx.x % TPV;
+        const bool live = vl < VL;
+        const int cb = (q % NCI) * RCI, ob = (q / NCI) * RCO;
+        float acc[RCO][RCI];
 #pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            acc[k] = 0.0f;
-            const int p = threadIdx.x + k * kTB;
-            co[k] = p < PAIRS ? p / CIN : 0;
-            ci[k] = p < PAIRS ? p % CIN : 0;
-        }
-        int vi = 0, vj = 0, vk = 0;
-        if (v0 < v1) decode3(v0, a.Dout, vi, vj, vk);
-        for (int64_t v = v0; v < v1; ++v) {
-            const int xi = S * vi + di, xj = S * vj + dj, xk = S * vk + dk;
-            // next voxel in z-fastest order
-            if (++vk == a.Dout) {
-                vk = 0;
-                if (++vj == a.Dout) {
-                    vj = 0;
-                    ++vi;
+        for (int m = 0; m < RCO; ++m)
+#pragma unroll
+            for (int n = 0; n < RCI; ++n) acc[m][n] = 0.0f;
+        for (int r = r0; r < r1 && live; ++r) {
+            const int vi = r / a.Dout, vj = r - vi * a.Dout;
+            const int xi = S * vi + di, xj = S * vj + dj;
+            if (xi < 0 || xi >= a.Din || xj < 0 || xj >= a.Din) continue;
+            const int64_t xrow = ((int64_t)xi * a.Din + xj) * a.Din + dk;
+            const float* grow = a.g + (int64_t)r * a.Dout * COUT + ob;
+            for (int vk = k0 + vl; vk < k1; vk += VL) {
+                const int64_t xe = (xrow + (int64_t)S * vk) * CIN + cb;
+                float xv[RCI], gv[RCO];
+                if constexpr (XF32) {
+                    const float2 w = *reinterpret_cast<const float2*>(static_cast<const float*>(a.x) + xe);
+                    xv[0] = w.x;
+                    xv[1] = w.y;
+                } else if constexpr (RCI == 4) {
+                    const uint2 w = *reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(a.x) + xe);
+                    xv[0] = __uint_as_float(w.x << 16);
+                    xv[1] = __uint_as_float(w.x & 0xFFFF0000u);
+                    xv[2] = __uint_as_float(w.y << 16);
+                    xv[3] = __uint_as_float(w.y & 0xFFFF0000u);
+                } else {
+#pragma unroll
+                    for (int n = 0; n < RCI; ++n) xv[n] = bf2f(static_cast<const uint16_t*>(a.x)[xe + n]);
                 }
-            }
-            if (xi < 0 || xi >= a.Din || xj < 0 || xj >= a.Din || xk < 0 || xk >= a.Din) continue;
-            const int64_t xs = ((int64_t)xi * a.Din + xj) * a.Din + xk;
-            const float* gv = a.g + v * COUT;
+                if constexpr (RCO == 4) {
+                    const float4 w = *reinterpret_cast<const float4*>(grow + (int64_t)vk * COUT);
+                    gv[0] = w.x;
+                    gv[1] = w.y;
+                    gv[2] = w.z;
+                    gv[3] = w.w;
+                } else {
+                    gv[0] = grow[(int64_t)vk * COUT];
+                }
 #pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                const float xv = XF32 ? static_cast<const float*>(a.x)[xs * CIN + ci[k]]
-                                      : bf2f(static_cast<const uint16_t*>(a.x)[xs * CIN + ci[k]]);
-                acc[k] += xv * gv[co[k]];
+                for (int m = 0; m < RCO; ++m)
+#pragma unroll
+                    for (int n = 0; n < RCI; ++n) acc[m][n] += xv[n] * gv[m];
             }
         }
+        if (live) {
 #pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            const int p = threadIdx.x + k * kTB;
-            if (p < PAIRS) a.part[((int64_t)blockIdx.x * TAPS + t) * PAIRS + p] = acc[k];
+            for (int m = 0; m < RCO; ++m)
+#pragma unroll
+                for (int n = 0; n < RCI; ++n) s_red[vl * PAIRS + (ob + m) * CIN + cb + n] = acc[m][n];
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < PAIRS; p += kTB) {
+            float sum = 0.0f;
+            for (int m = 0; m < VL; ++m) sum += s_red[m * PAIRS + p];
+            a.part[((int64_t)blockIdx.x * TAPS + t) * PAIRS + p] = sum;
         }
     }
 };
@@ -381,17 +449,26 @@ struct KBwdWeight {
 // layer's [t][16][16], column 0.
 struct KReduceW {
     static constexpr int kThreads = kTB;
+    static constexpr int kLanes = 16;             // chunk lanes per element
+    static constexpr int kPerBlock = kTB / kLanes;
     struct Args {
         const float* part;
         float* grad;
         int nchunks, taps, cout, cin, layout;
     };
     static __device__ void run(const Args& a) {
+        __shared__ float s_sum[kTB];
         const int n = a.taps * a.cout * a.cin;
-        const int e = blockIdx.x * kTB + threadIdx.x;
-        if (e >= n) return;
+        const int lane = threadIdx.x / kPerBlock;
+        const int e = blockIdx.x * kPerBlock + threadIdx.x % kPerBlock;
         float s = 0.0f;
-        for (int c = 0; c < a.nchunks; ++c) s += a.part[(int64_t)c * n + e];
+        if (e < n)
+            for (int c = lane; c < a.nchunks; c += kLanes) s += a.part[(int64_t)c * n + e];
+        s_sum[threadIdx.x] = s;
+        __syncthreads();
+        if (lane != 0 || e >= n) return;
+        s = 0.0f;
+        for (int m = 0; m < kLanes; ++m) s += s_sum[m * kPerBlock + threadIdx.x];
         const int ci = e % a.cin, co = (e / a.cin) % a.cout, t = e / (a.cin * a.cout);
         int idx = e;
         if (a.layout == 1) idx = co * 64 + 2 * t + ci;
@@ -436,10 +513,16 @@ struct KAdam {
         int64_t n;
     };
     static __device__ void run(const Args& a) {
+        __shared__ double s_nrm2;
+        if (threadIdx.x < 64) {
+            double n2, u1, u2;
+            wave_sum_parts(a.part, a.nparts, n2, u1, u2);
+            if (threadIdx.x == 0) s_nrm2 = n2;
+        }
+        __syncthreads();
         const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
         if (i >= a.n) return;
-        double nrm2 = 0.0;
-        for (int b = 0; b < a.nparts; ++b) nrm2 += a.part[3 * b];
+        const double nrm2 = s_nrm2;
         if (!(nrm2 < 1e300)) return;   // non-finite gradient: the batch is skipped
         const float nrm = (float)sqrt(nrm2);
         const float sc = a.clip > 0.0f && nrm > a.clip ? a.clip / (nrm + 1e-6f) : 1.0f;
@@ -526,8 +609,8 @@ int space_for(ariel_trainer* T, int N, TrainSpace** out) {
     TrainSpace S;
     S.N = N;
     const int64_t v0 = (int64_t)N * N * N, v1 = v0 / 8, v2 = v0 / 64, v3 = v0 / 512;
-    // largest weight-gradient partial: chunks x taps x pairs over the layers
-    const int64_t part = (int64_t)2048 * 27 * 64 * 48 / 8 + (int64_t)v0;
+    // largest weight-gradient partial: kWChunks x the most (co, ci) pairs (64 x 48)
+    const int64_t part = (int64_t)kWChunks * 64 * 48;
     bool ok = true;
     auto A = [&](auto** p, int64_t n) { ok = ok && talloc(S.mem, p, n) == hipSuccess; };
     A(&S.e0, 16 * v0); A(&S.e1, 32 * v1); A(&S.e2, 48 * v2); A(&S.e3, 64 * v3);
@@ -554,23 +637,26 @@ int space_for(ariel_trainer* T, int N, TrainSpace** out) {
 template <int TAPS, int S, int CIN, int COUT, bool XF32>
 void weight_grad(ariel_trainer* T, TrainSpace& W, const void* x, const float* g, int Din, int Dout,
                  int layer, int layout) {
-    const int64_t nv = (int64_t)Dout * Dout * Dout;
-    int64_t chunks = nv / 64;
-    const int64_t cap = 2048 / TAPS > 0 ? 2048 / TAPS : 1;
-    if (chunks > cap) chunks = cap;
-    if (chunks < 1) chunks = 1;
-    const int64_t per = (nv + chunks - 1) / chunks;
-    chunks = (nv + per - 1) / per;
+    // chunks of whole output rows, about 256 voxels each, fewer where that
+    // leaves under 2048 blocks (enough to hide the loads' latency), and
+    // kWChunks x TAPS partials at most
+    const int rows = Dout * Dout;
+    int per = (256 + Dout - 1) / Dout;
+    const int fill = rows * TAPS / 2048;
+    if (fill < per) per = fill > 1 ? fill : 1;
+    const int cap = kWChunks / TAPS > 0 ? kWChunks / TAPS : 1;
+    if ((rows + per - 1) / per > cap) per = (rows + cap - 1) / cap;
+    const int64_t chunks = (rows + per - 1) / per;
     launch<KBwdWeight<TAPS, S, CIN, COUT, XF32>>(T->stream, dim3((unsigned)chunks, TAPS), dim3(kTB),
                                                 {x, g, W.part, Din, Dout, per});
     const int n = TAPS * COUT * CIN;
-    launch<KReduceW>(T->stream, dim3(blocks(n)), dim3(kTB),
+    launch<KReduceW>(T->stream, dim3((unsigned)((n + KReduceW::kPerBlock - 1) / KReduceW::kPerBlock)), dim3(kTB),
                      {W.part, T->grad + T->off[layer], (int)chunks, TAPS, COUT, CIN, layout});
 }
 
 template <int TAPS, int S, int CIN, int COUT, int WCO>
 void data_grad(ariel_trainer* T, const float* g, int layer, float* dx, int Din, int Dout, int acc) {
-    const int64_t n = (int64_t)Din * Din * Din * CIN;
+    const int64_t n = (int64_t)Din * Din * Din * (CIN / 4);
     launch<KBwdData<TAPS, S, CIN, COUT, WCO>>(T->stream, dim3(blocks(n)), dim3(kTB),
                                              {g, T->wbf + T->off[layer], dx, Din, Dout, acc});
 }
