@@ -67,7 +67,7 @@ class StepStats(ctypes.Structure):
 
 class KernelStats(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_int64), ("total_ms", ctypes.c_double),
-                ("bytes_per_launch", ctypes.c_double)]
+                ("bytes_per_launch", ctypes.c_double), ("calls", ctypes.c_int64)]
 
 
 _P = ctypes.c_void_p

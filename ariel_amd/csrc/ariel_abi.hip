@@ -210,12 +210,19 @@ double kernel_bytes(const ariel_sim* h, int k) {
     }
 }
 
+// Each event record leaves the GPU idle for a few microseconds between the
+// kernels it separates, so inside the PCG loop only one iteration in
+// kProfEvery is timed (the launches do not depend on the iteration index).
+constexpr int kProfEvery = 8;
+inline bool prof_sampled(int it) { return (it - 1) % kProfEvery == 0; }
+
 struct Prof {
     ariel_sim* h;
     int k;
     hipEvent_t e0 = nullptr;
-    Prof(ariel_sim* hh, int kk) : h(hh), k(kk) {
-        if (h->profile) {
+    Prof(ariel_sim* hh, int kk, bool on = true) : h(hh), k(kk) {
+        if (h->profile) h->kstats[k].calls += 1;
+        if (h->profile && on) {
             e0 = take_event(h);
             if (e0) (void)hipEventRecord(e0, h->stream);
         }
@@ -430,6 +437,39 @@ int read_scalars(ariel_sim* h) {
     return ARIEL_OK;
 }
 
+// PCG iterations are queued in chunks with one readback of the solver scalars
+// after each. A readback idles the GPU for a few tens of microseconds; an
+// iteration queued past convergence still costs the dispatch of each of its
+// kernels (they return at once), about 80 us. The first chunk is two thirds of
+// the previous solve's count; later ones extrapolate the residual's decay
+// since the previous readback (or since the initial residual) to the stop
+// threshold and queue half of that estimate: CG's decay speeds up as it
+// proceeds, so the estimate runs high.
+class ChunkPlan {
+  public:
+    explicit ChunkPlan(int last_iters) : first_(std::max(4, 2 * last_iters / 3)) {}
+    int first() const { return first_; }
+    int next(const SolverScalars& sc) {
+        const double r0 = have_ ? r2_ : sc.rnorm2_0;
+        const int dn = sc.it - (have_ ? it_ : 0);
+        have_ = true;
+        it_ = sc.it;
+        r2_ = sc.rnorm2;
+        if (dn <= 0 || !(sc.rnorm2 > 0.0) || !(r0 > sc.rnorm2) || !(sc.thr > 0.0)) return 4;
+        const double need = std::log(sc.thr / sc.rnorm2);   // < 0 while not converged
+        if (!(need < 0.0)) return 2;
+        const double per_it = std::log(sc.rnorm2 / r0) / dn;  // < 0
+        const double est = 0.5 * need / per_it;
+        return est < 2.0 ? 2 : est > 64.0 ? 64 : (int)est;
+    }
+
+  private:
+    int first_;
+    bool have_ = false;
+    int it_ = 0;
+    double r2_ = 0.0;
+};
+
 // Active-set descriptor of a handle: counts from the last readback, or -1
 // (read on the device) for batched launches; caps are the largest possible.
 void set_active(ariel_sim* h, bool device_counts) {
@@ -492,23 +532,24 @@ int stage_solve(ariel_sim* h) {
     TRY(check_launch(h, "solve setup"));
     int queued = 0;
     const int k = h->d.refine_every;
-    int chunk = std::max(4, h->last_iters - 3);
+    ChunkPlan plan(h->last_iters);
+    int chunk = plan.first();
     for (;;) {
         const int n = std::min(chunk, h->d.max_iters - queued);
         for (int j = 0; j < n; ++j) {
             const int it = queued + j + 1;
             {
-                Prof prof(h, ARIEL_KERNEL_SPMV);
+                Prof prof(h, ARIEL_KERNEL_SPMV, prof_sampled(it));
                 launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, act);
             }
             {
-                Prof prof(h, ARIEL_KERNEL_AXPY);
+                Prof prof(h, ARIEL_KERNEL_AXPY, prof_sampled(it));
                 launch_axpy2(s, h->P, h->S, h->R, h->Z, red, act);
             }
             if (k > 0 && it % k == 0)
                 launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, act);
             {
-                Prof prof(h, ARIEL_KERNEL_PRECOND);
+                Prof prof(h, ARIEL_KERNEL_PRECOND, prof_sampled(it));
                 launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, act);
             }
             launch_update_s(s, h->Z, h->S, red, 0, act);
@@ -517,7 +558,7 @@ int stage_solve(ariel_sim* h) {
         queued += n;
         TRY(read_scalars(h));
         if (h->sc_host->done || queued >= h->d.max_iters) break;
-        chunk = 4;
+        chunk = plan.next(*h->sc_host);
     }
     if (refine) launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
     HIPCHK(h, hipEventRecord(h->solve_t1, s));
@@ -574,24 +615,25 @@ int stage_solve_learned(ariel_sim* h) {
     bool on_mic = false;
     int queued = 0;
     const int k = h->d.refine_every;
-    int chunk = std::max(4, h->last_iters - 3);
+    ChunkPlan plan(h->last_iters);
+    int chunk = plan.first();
     for (;;) {
         const int n = std::min(chunk, h->d.max_iters - queued);
         for (int j = 0; j < n; ++j) {
             const int it = queued + j + 1;
             {
-                Prof prof(h, ARIEL_KERNEL_SPMV);
+                Prof prof(h, ARIEL_KERNEL_SPMV, prof_sampled(it));
                 launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, act);
             }
             {
-                Prof prof(h, ARIEL_KERNEL_AXPY);
+                Prof prof(h, ARIEL_KERNEL_AXPY, prof_sampled(it));
                 launch_copy_list(s, h->R, h->R_old, act, h->sc);
                 launch_axpy2(s, h->P, h->S, h->R, h->Z, red, act);
             }
             if (k > 0 && it % k == 0)
                 launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, act);
             {
-                Prof prof(h, ARIEL_KERNEL_PRECOND);
+                Prof prof(h, ARIEL_KERNEL_PRECOND, prof_sampled(it));
                 learned_precond(h, p, red, on_mic);
             }
             launch_update_s(s, h->Z, h->S, red, 0, act);
@@ -601,7 +643,7 @@ int stage_solve_learned(ariel_sim* h) {
         TRY(read_scalars(h));
         on_mic = on_mic || h->sc_host->use_mic;
         if (h->sc_host->done || queued >= h->d.max_iters) break;
-        chunk = 4;
+        chunk = plan.next(*h->sc_host);
     }
     if (refine) launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
     HIPCHK(h, hipEventRecord(h->solve_t1, s));

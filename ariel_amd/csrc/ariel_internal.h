@@ -52,6 +52,7 @@ struct SolverScalars {
     double rnorm2_prev;    // ||r||^2 at the start of the fallback test's window
     int32_t fallback_it;   // iteration at which flexible PCG switched to MIC(0)
     int32_t fallback_reason;  // 1: slow progress, 2: not a descent direction
+    double rnorm2_0;       // ||r||^2 of the solve's initial residual
 };
 
 // Cells the pressure solve touches. A z-segment is 8 cells (cells 8s..8s+7) and

@@ -494,15 +494,12 @@ struct KFlush {
         nseg = seg_count(nseg, sc);
         const int64_t nb = list_grid(nseg);
         if ((int64_t)blockIdx.x >= nb) return;
-        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+        // one cell per thread and iteration: neighbouring lanes take neighbouring cells
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < 8 * nseg;
              u += nb * blockDim.x) {
-            const int64_t c0 = 8 * (int64_t)segs[u];
-    #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                const int64_t c = c0 + m;
-                if (A[c] == kFluid) P64[c] += (double)P[c];
-                P[c] = 0.0f;
-            }
+            const int64_t c = 8 * (int64_t)segs[u >> 3] + (u & 7);
+            if (A[c] == kFluid) P64[c] += (double)P[c];
+            P[c] = 0.0f;
         }
     }
 };
@@ -543,23 +540,21 @@ struct KTrueResidual {
         if ((int64_t)blockIdx.x >= nb) return;
         const int N = p.N;
         double v[1] = {0.0};
-        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+        // one cell per thread and iteration: neighbouring lanes take neighbouring cells
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < 8 * nseg;
              u += nb * blockDim.x) {
-            const int64_t c0 = 8 * (int64_t)segs[u];
-            for (int m = 0; m < 8; ++m) {
-                const int64_t c = c0 + m;
-                if (A[c] != kFluid) {
-                    if (mode == 0) R[c] = 0.0f;
-                    continue;
-                }
-                int i, j, k;
-                decode_cell(N, c, i, j, k);
-                const double ax = mode == 0 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
-                                            : spmv_cell_f64(A, L, P, N, i, j, k, c);
-                const double r = (double)D[c] - ax;
-                v[0] += r * r;
-                if (mode == 0) R[c] = (float)r;
+            const int64_t c = 8 * (int64_t)segs[u >> 3] + (u & 7);
+            if (A[c] != kFluid) {
+                if (mode == 0) R[c] = 0.0f;
+                continue;
             }
+            int i, j, k;
+            decode_cell(N, c, i, j, k);
+            const double ax = mode == 0 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
+                                        : spmv_cell_f64(A, L, P, N, i, j, k, c);
+            const double r = (double)D[c] - ax;
+            v[0] += r * r;
+            if (mode == 0) R[c] = (float)r;
         }
         block_partials<1>(v, red.partials, blockIdx.x);
     }
@@ -585,14 +580,10 @@ struct KRefineFinish {
 
         const int64_t nb = list_grid(nseg);
         if ((int64_t)blockIdx.x >= nb) return;
-        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
+        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < 8 * nseg;
              u += nb * blockDim.x) {
-            const int64_t c0 = 8 * (int64_t)segs[u];
-    #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                const int64_t c = c0 + m;
-                P[c] = A[c] == kFluid ? (float)(P64[c] + (double)P[c]) : 0.0f;
-            }
+            const int64_t c = 8 * (int64_t)segs[u >> 3] + (u & 7);
+            P[c] = A[c] == kFluid ? (float)(P64[c] + (double)P[c]) : 0.0f;
         }
     }
 };
@@ -764,6 +755,7 @@ struct KFinalize {
                 sc->bnorm2 = s0;
                 sc->rnorm2 = s1;
                 sc->rnorm2_prev = s1;
+                sc->rnorm2_0 = s1;
                 if (stop_rule == 1) sc->thr = (double)(1.0e-2f * (float)(N * N * N));
                 else sc->thr = tol * tol * s0;
                 if (s1 <= sc->thr) {

@@ -173,8 +173,10 @@ typedef struct ariel_step_stats {
 
 typedef struct ariel_kernel_stats {
     int64_t launches;          /* timed launches (groups) since the last reset */
-    double  total_ms;          /* summed device time (HIP events on the stream) */
+    double  total_ms;          /* summed device time of the timed launches     */
     double  bytes_per_launch;  /* algorithmic HBM bytes of one launch          */
+    int64_t calls;             /* all launches (groups) while profiling, timed
+                                  or not: total time ~ total_ms / launches * calls */
 } ariel_kernel_stats;
 
 typedef struct ariel_sim ariel_sim;
@@ -264,7 +266,9 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z);
 
 /* Per-kernel device timing with HIP events recorded on the handle's stream
  * around each launch group (no reference counterpart: measurement only).
- * Enabling adds two event records per launch; stats accumulate until reset. */
+ * Enabling adds two event records per launch group, each of which idles the
+ * GPU for a few microseconds; inside the PCG loop only one iteration in 8 is
+ * timed for that reason. Stats accumulate until reset. */
 int ariel_set_profiling(ariel_sim* h, int32_t enable);
 int ariel_reset_kernel_stats(ariel_sim* h);
 int ariel_get_kernel_stats(ariel_sim* h, int32_t kernel, ariel_kernel_stats* out);
