@@ -1286,36 +1286,90 @@ int ariel_load_state(ariel_sim* h, const char* path) {
 // units (levelset.cpp:101-111, ParticlesToLevelSet with r = 0.5 at voxel size 1,
 // positions scaled by maxd) sampled on the (N+1)^3 lattice of integer points.
 // phi = min over particles of |x - p| - 0.5, clamped to the band of 3 voxels.
+// The particle level set of the current particles on the host ((N+1)^3 nodes,
+// x-major): rasterised on the GPU (export.hip), then copied out.
+constexpr float kSdfBand = 3.0f, kSdfRadius = 0.5f;
+int particle_sdf(ariel_sim* h, std::vector<float>& phi) {
+    const int64_t M = h->N + 1, nm = M * M * M;
+    SdfScratch w;
+    float* dphi = nullptr;
+    int rc = ARIEL_OK;
+    if ((rc = dalloc(h, &w.counts, h->nc)) || (rc = dalloc(h, &w.offsets, h->nc + 1)) ||
+        (rc = dalloc(h, &w.scan_sums, h->nc / 256 + 64)) || (rc = dalloc(h, &w.cell_of, h->np)) ||
+        (rc = dalloc(h, &w.rank, h->np)) || (rc = dalloc(h, &w.sorted, h->np)) || (rc = dalloc(h, &dphi, nm))) {
+    } else {
+        launch_particle_sdf(h->stream, h->pos[h->cur], h->np, h->N, h->maxd, kSdfBand, kSdfRadius, w, dphi);
+        rc = check_launch(h, "particle level set");
+        phi.resize((size_t)nm);
+        hipError_t e = hipSuccess;
+        if (rc == ARIEL_OK) e = hipMemcpyAsync(phi.data(), dphi, nm * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (rc == ARIEL_OK && e != hipSuccess) rc = fail(h, ARIEL_ERR_HIP, std::string("level set copy: ") + hipGetErrorString(e));
+    }
+    for (void* q : {(void*)w.counts, (void*)w.offsets, (void*)w.scan_sums, (void*)w.cell_of, (void*)w.rank,
+                    (void*)w.sorted, (void*)dphi})
+        if (q) (void)hipFree(q);
+    return rc;
+}
+
 int ariel_export_levelset(ariel_sim* h, const char* path) {
     TRY(check_handle(h));
     if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
-    std::vector<float> pos(3 * h->np);
-    int64_t n = 0;
-    TRY(ariel_get_particles(h, pos.data(), nullptr, nullptr, h->np, &n));
+    std::vector<float> phi;
+    TRY(particle_sdf(h, phi));
     const int M = h->N + 1;
-    const float band = 3.0f, rad = 0.5f;
-    std::vector<float> phi((size_t)M * M * M, band);
-    for (int64_t p = 0; p < n; ++p) {
-        const float x = pos[3 * p] * h->maxd, y = pos[3 * p + 1] * h->maxd, z = pos[3 * p + 2] * h->maxd;
-        const int r = (int)std::ceil(band + rad);
-        for (int i = std::max(0, (int)x - r); i <= std::min(M - 1, (int)x + r); ++i)
-            for (int j = std::max(0, (int)y - r); j <= std::min(M - 1, (int)y + r); ++j)
-                for (int k = std::max(0, (int)z - r); k <= std::min(M - 1, (int)z + r); ++k) {
-                    const float dx = i - x, dy = j - y, dz = k - z;
-                    const float d = std::sqrt(dx * dx + dy * dy + dz * dz) - rad;
-                    float& v = phi[((size_t)i * M + j) * M + k];
-                    if (d < v) v = d;
-                }
-    }
     FILE* f = std::fopen(path, "wb");
     if (!f) return fail(h, ARIEL_ERR_IO, std::string("cannot open ") + path);
     char header[256];
     int len = std::snprintf(header, sizeof(header),
-                            "ARIEL_LEVELSET 1\ndims %d %d %d\nvoxel 1.0\nbackground %.6f\nfloat32 le\n", M, M, M, band);
+                            "ARIEL_LEVELSET 1\ndims %d %d %d\nvoxel 1.0\nbackground %.6f\nfloat32 le\n", M, M, M,
+                            kSdfBand);
     bool ok = std::fwrite(header, 1, (size_t)len, f) == (size_t)len;
     ok = ok && std::fwrite(phi.data(), sizeof(float), phi.size(), f) == phi.size();
-    std::fclose(f);
+    ok = std::fclose(f) == 0 && ok;
     if (!ok) return fail(h, ARIEL_ERR_IO, std::string("write failed: ") + path);
+    return ARIEL_OK;
+}
+
+int ariel_export_mesh(ariel_sim* h, const char* path) {
+    TRY(check_handle(h));
+    if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
+    std::vector<float> phi, verts;
+    std::vector<int32_t> quads;
+    TRY(particle_sdf(h, phi));
+    surface_nets(phi.data(), h->N + 1, verts, quads);
+    if (!write_obj(path, verts, quads)) return fail(h, ARIEL_ERR_IO, std::string("cannot write ") + path);
+    return ARIEL_OK;
+}
+
+int ariel_export_particles(ariel_sim* h, const char* path) {
+    TRY(check_handle(h));
+    if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
+    const std::string p(path);
+    if (p.size() < 4 || p.compare(p.size() - 4, 4, ".geo") != 0)
+        return fail(h, ARIEL_ERR_UNSUPPORTED, "particle export: only .geo (Houdini classic ASCII) is built");
+    std::vector<float> pos(3 * h->np), vel(3 * h->np);
+    int64_t n = 0;
+    TRY(ariel_get_particles(h, pos.data(), vel.data(), nullptr, h->np, &n));
+    for (float& x : pos) x = x * h->maxd;
+    if (!write_geo(path, n, pos.data(), vel.data())) return fail(h, ARIEL_ERR_IO, std::string("cannot write ") + path);
+    return ARIEL_OK;
+}
+
+int ariel_mesh_sdf(const float* phi, int32_t M, const char* path, int64_t* nverts, int64_t* nquads) {
+    if (!phi || !path || M < 2 || M > 4097) {
+        set_last_error("ariel_mesh_sdf: bad arguments");
+        return ARIEL_ERR_INVALID;
+    }
+    std::vector<float> verts;
+    std::vector<int32_t> quads;
+    surface_nets(phi, M, verts, quads);
+    if (nverts) *nverts = (int64_t)(verts.size() / 3);
+    if (nquads) *nquads = (int64_t)(quads.size() / 4);
+    if (!write_obj(path, verts, quads)) {
+        set_last_error(std::string("cannot write ") + path);
+        return ARIEL_ERR_IO;
+    }
     return ARIEL_OK;
 }
 

@@ -198,6 +198,24 @@ void launch_unsort(hipStream_t s, const float4* pos, const float4* vel, const ui
                    int64_t np, float4* pos_out, float4* vel_out, uint8_t* accept_out,
                    int32_t* ids_out);
 void launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+
+// frame export (export.hip)
+struct SdfScratch {
+    int32_t* counts = nullptr;     // N^3
+    int32_t* offsets = nullptr;    // N^3 + 1
+    int32_t* scan_sums = nullptr;  // block sums of launch_scan over N^3
+    int32_t* cell_of = nullptr;    // np
+    int32_t* rank = nullptr;       // np
+    float4* sorted = nullptr;      // np
+};
+// phi of the particle level set on the (N+1)^3 nodes (grid units).
+void launch_particle_sdf(hipStream_t s, const float4* pos, int64_t np, int N, float maxd, float band,
+                         float rad, SdfScratch& w, float* phi);
+// Dual mesh of phi's zero surface on an M^3 node grid: vertices (x, y, z in node
+// units) and quads (4 vertex indices each, normals out of the negative side).
+void surface_nets(const float* phi, int M, std::vector<float>& verts, std::vector<int32_t>& quads);
+bool write_obj(const char* path, const std::vector<float>& verts, const std::vector<int32_t>& quads);
+bool write_geo(const char* path, int64_t n, const float* pos, const float* vel);
 void launch_fill_f32(hipStream_t s, float* p, int64_t n, float v);
 
 // grid (kernels_grid.hip)

@@ -218,5 +218,46 @@ class FlipSim:
     def export_levelset(self, path):
         self._check(self._lib.ariel_export_levelset(self._h, str(path).encode()))
 
+    def export_mesh(self, path):
+        """OBJ of the particle level set's zero surface (Scene::ExportParticles' OBJ path)."""
+        self._check(self._lib.ariel_export_mesh(self._h, str(path).encode()))
 
-__all__ = ["FlipSim", "ArielError", "SceneDesc"]
+    def export_particles(self, path):
+        """Partio-style particle file (Scene::ExportParticles' Partio path); ".geo" only."""
+        self._check(self._lib.ariel_export_particles(self._h, str(path).encode()))
+
+
+def read_levelset(path):
+    """The volume ariel_export_levelset writes: (phi as an (M, M, M) float32 array
+    indexed [i, j, k], background value)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    lines, pos = [], 0
+    while len(lines) < 5:
+        end = raw.index(b"\n", pos)
+        lines.append(raw[pos:end].decode())
+        pos = end + 1
+    if lines[0] != "ARIEL_LEVELSET 1" or lines[4] != "float32 le":
+        raise ValueError("not an ARIEL_LEVELSET 1 file")
+    dims = [int(v) for v in lines[1].split()[1:4]]
+    background = float(lines[3].split()[1])
+    phi = np.frombuffer(raw, "<f4", dims[0] * dims[1] * dims[2], pos).reshape(dims)
+    return phi, background
+
+
+def mesh_sdf(phi, path):
+    """The library's mesher on a caller's level set (M^3 node values, indexed
+    [i, j, k]), host only: writes an OBJ and returns (vertices, quads)."""
+    lib = _abi.load()
+    phi = np.ascontiguousarray(phi, np.float32)
+    if phi.ndim != 3 or len(set(phi.shape)) != 1:
+        raise ValueError("phi must be an M x M x M array")
+    nv, nq = ctypes.c_int64(), ctypes.c_int64()
+    rc = lib.ariel_mesh_sdf(phi.ctypes.data_as(_abi._PF), phi.shape[0], str(path).encode(),
+                            ctypes.byref(nv), ctypes.byref(nq))
+    if rc != 0:
+        raise ArielError(rc, lib.ariel_last_error(None).decode())
+    return nv.value, nq.value
+
+
+__all__ = ["FlipSim", "ArielError", "SceneDesc", "read_levelset", "mesh_sdf"]

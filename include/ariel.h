@@ -278,11 +278,33 @@ int ariel_get_kernel_stats(ariel_sim* h, int32_t kernel, ariel_kernel_stats* out
 int ariel_save_state(ariel_sim* h, const char* path);
 int ariel_load_state(ariel_sim* h, const char* path);
 
-/* Scene::ExportParticles VDB/OBJ path (src/scene/scene.cpp:84-101): writes the
- * fluid particles as a narrow-band level set sampled on a dense grid of
- * (N+1)^3 voxels of size 1 in grid units, sphere radius 0.5 (levelset.cpp:
- * 101-111), as a raw little-endian float32 volume with a text header. */
+/* Scene::ExportParticles VDB path (src/scene/scene.cpp:84-101, LevelSet from
+ * particles, src/grid/levelset.cpp:101-111): the fluid particles' level set,
+ * rasterised on the GPU on the (N+1)^3 nodes of a grid of voxel size 1 in grid
+ * units (a particle sits at p * maxd): min over particles of |x - p| - 0.5,
+ * limited to the narrow band value 3. Written as a raw little-endian float32
+ * volume (x-major) after a text header; no OpenVDB writer in this build. */
 int ariel_export_levelset(ariel_sim* h, const char* path);
+
+/* Scene::ExportParticles OBJ path (scene.cpp:93-99, LevelSet::WriteObjToFile,
+ * levelset.cpp:113-173): the zero surface of the same level set as a dual
+ * quad mesh (one vertex per voxel the surface crosses, one quad per crossed
+ * voxel edge, normals out of the liquid), in grid units, written like
+ * Obj::WriteObj ("v x y z" lines, then "f a b c d" with 1-based indices). */
+int ariel_export_mesh(ariel_sim* h, const char* path);
+
+/* Scene::ExportParticles Partio path (scene.cpp:50-82): every fluid particle
+ * with its position in grid units, velocity "v" and integer "id" (its index in
+ * the file), in GetParticles order. The format follows the file extension as
+ * Partio's does; only ".geo" (Houdini classic ASCII geometry) is built, other
+ * extensions give ARIEL_ERR_UNSUPPORTED. */
+int ariel_export_particles(ariel_sim* h, const char* path);
+
+/* The mesher of ariel_export_mesh on a caller's level set (host only, no GPU):
+ * phi holds M^3 node values, x-major (the volume ariel_export_levelset writes);
+ * writes the OBJ to path and returns the vertex and quad counts. No reference
+ * counterpart (the reference meshes inside OpenVDB). */
+int ariel_mesh_sdf(const float* phi, int32_t M, const char* path, int64_t* nverts, int64_t* nquads);
 
 /* Weights of the learned preconditioner (ARIEL_PRECOND_LEARNED; no reference
  * counterpart): the packed BF16 blob written by ariel_amd/learned/train.py --

@@ -15,6 +15,7 @@
 #define ARIEL_FLIPSIM_HPP
 
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -60,15 +61,22 @@ public:
         sync_frame();
     }
 
-    // Step(saveVDB, saveOBJ, savePARTIO) (flip.cpp:92-128). Export writes the
-    // particle level set to `export_path` when saveVDB is set.
+    // Step(saveVDB, saveOBJ, savePARTIO) (flip.cpp:92-128) with the export of
+    // Scene::ExportParticles (scene.cpp:34-101) after the step: the level set,
+    // its mesh and the particles go to <export_path>.<frame>.levelset, .obj and
+    // .geo, the frame padded to 4 digits and placed before the extension as the
+    // reference names its files (scene.cpp:48-58, 85-90).
     void Step(bool saveVDB = false, bool saveOBJ = false, bool savePARTIO = false,
               const char* export_path = nullptr) {
-        (void)saveOBJ;
-        (void)savePARTIO;
         check(ariel_step(h_, 1), h_);
         sync_frame();
-        if (saveVDB && export_path) check(ariel_export_levelset(h_, export_path), h_);
+        if (!export_path || !(saveVDB || saveOBJ || savePARTIO)) return;
+        char frame[16];
+        std::snprintf(frame, sizeof(frame), ".%04d", m_frame);
+        const std::string base = std::string(export_path) + frame;
+        if (saveVDB) check(ariel_export_levelset(h_, (base + ".levelset").c_str()), h_);
+        if (saveOBJ) check(ariel_export_mesh(h_, (base + ".obj").c_str()), h_);
+        if (savePARTIO) check(ariel_export_particles(h_, (base + ".geo").c_str()), h_);
     }
 
     std::vector<Particle> GetParticles() {
