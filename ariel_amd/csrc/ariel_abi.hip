@@ -69,6 +69,8 @@ struct ariel_sim {
     float* du[3] = {nullptr, nullptr, nullptr};
     float *P = nullptr, *D = nullptr, *L = nullptr;
     uint8_t* A = nullptr;
+    // pc: 2 N^3 floats, the factor in the natural layout and then the tile-major
+    // copy the MIC(0) apply reads; Q is the apply's intermediate, tile-major
     float *R = nullptr, *Z = nullptr, *S = nullptr, *Q = nullptr, *pc = nullptr, *diag = nullptr;
     uint8_t* nup = nullptr;
     uint8_t* fl8 = nullptr;       // fluid bits per 8-cell z-segment
@@ -184,7 +186,6 @@ hipEvent_t take_event(ariel_sim* h) {
     return e;
 }
 
-// Algorithmic HBM bytes of one launch group: per-unit figures x units.
 // Algorithmic HBM bytes of one launch group: each input read once and each
 // output written once. The solver groups count the cells they touch: the
 // active segments (8 cells each) and, for the preconditioner, the active tiles
@@ -809,7 +810,7 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
     }
     if ((rc = dalloc(h, &h->P, h->nc)) || (rc = dalloc(h, &h->D, h->nc)) || (rc = dalloc(h, &h->L, h->nc)) ||
         (rc = dalloc(h, &h->A, h->nc)) || (rc = dalloc(h, &h->R, h->nc)) || (rc = dalloc(h, &h->Z, h->nc)) ||
-        (rc = dalloc(h, &h->S, h->nc)) || (rc = dalloc(h, &h->Q, h->nc)) || (rc = dalloc(h, &h->pc, h->nc)) ||
+        (rc = dalloc(h, &h->S, h->nc)) || (rc = dalloc(h, &h->Q, h->nc)) || (rc = dalloc(h, &h->pc, 2 * h->nc)) ||
         (rc = dalloc(h, &h->diag, h->nc)) || (rc = dalloc(h, &h->nup, h->nc)) ||
         (rc = dalloc(h, &h->fl8, h->nc / 8)) ||
         (rc = dalloc(h, &h->counts, h->nc)) || (rc = dalloc(h, &h->offsets, h->nc + 1)) ||
@@ -839,7 +840,8 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
     launch_fill_f32(s, h->P, h->nc, 0.0f);
     launch_fill_f32(s, h->D, h->nc, 0.0f);
     launch_fill_f32(s, h->L, h->nc, 1.6f);   // macgrid.inl:82
-    for (float* v : {h->R, h->Z, h->S, h->Q, h->pc, h->diag}) launch_fill_f32(s, v, h->nc, 0.0f);
+    for (float* v : {h->R, h->Z, h->S, h->Q, h->diag}) launch_fill_f32(s, v, h->nc, 0.0f);
+    launch_fill_f32(s, h->pc, 2 * h->nc, 0.0f);   // natural layout, then the MIC(0) apply's tile-major copy
     launch_fill_i32(s, h->counts, h->nc, 0);
     launch_fill_i32(s, h->offsets, h->nc + 1, 0);
     if (hipMemsetAsync(h->A, 0, (size_t)h->nc, s) != hipSuccess ||

@@ -1090,6 +1090,41 @@ struct KMicBuild {
     }
 };
 
+// The MIC(0) apply reads pc and its own intermediate q tile by tile. In the
+// natural layout a tile's z-line is 32 bytes between lines of the other colour's
+// tiles, so every read of one colour fetches whole cache lines half of which it
+// does not use. The apply therefore reads a tile-major copy of pc (tile t at
+// [512 t, 512 t + 512), cell (il, jl, kl) at 64 il + 8 jl + kl), made once per
+// solve over the active tiles, and keeps q in the same layout.
+__host__ __device__ inline int64_t tile_major(int64_t t, int il, int jl, int kl) {
+    return t * 512 + (il * 8 + jl) * 8 + kl;
+}
+struct KPcTileMajor {
+    static constexpr int kThreads = 64;
+    struct Args {
+        int N;
+        const float* pc;
+        float* pct;
+        const int32_t* tiles;
+        int64_t ntiles;
+        const SolverScalars* sc;
+    };
+    static __device__ void run(const Args& a) {
+        const int TN = a.N / kTile;
+        const int64_t nt = tile_count(a.ntiles, a.sc, 2);
+        const int il = threadIdx.x >> 3, jl = threadIdx.x & 7;
+        for (int64_t u = blockIdx.x; u < nt; u += gridDim.x) {
+            const int t = a.tiles[u];
+            const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
+            const int64_t ln = cidx(a.N, ti * kTile + il, tj * kTile + jl, tk * kTile);
+            const float4* src = reinterpret_cast<const float4*>(a.pc + ln);
+            float4* dst = reinterpret_cast<float4*>(a.pct + tile_major(t, il, jl, 0));
+            dst[0] = src[0];
+            dst[1] = src[1];
+        }
+    }
+};
+
 // Fluid bits of every z-segment of 8 cells: fl8[(i*N + j)*(N/8) + k/8], bit k%8.
 struct KFluidSegments {
     static constexpr int kThreads = kBlock;
@@ -1225,16 +1260,19 @@ struct KMicApply {
     #pragma unroll
             for (int m = 0; m < kTile; ++m) {
                 const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
+                const int64_t gt = tile_major(t, m, cj, ce);
                 const int li = (m * 8 + cj) * kPad + ce;
-                lds[o_pc + li] = pc[g];
+                lds[o_pc + li] = pc[gt];
                 rreg[m] = r[g];
-                lds[o_sa + li] = do_fwd ? rreg[m] : Q[g];
+                lds[o_sa + li] = do_fwd ? rreg[m] : Q[gt];
             }
             if constexpr (halo) {
                 // neighbour lines across the tile faces: forward halos hold pc*Q of
                 // the red neighbours, backward halos their Z; -0.0 off-fluid
+                // (pc and Q tile-major, Z natural)
                 const float* va = halo_fwd ? pc : Z;
                 const float* vb = halo_fwd ? Q : nullptr;
+                const int TT = TN * TN;
                 // lanes on a tile face own one halo line; at the domain boundary it
                 // holds -0.0
                 if (il == 0 || il == 7) {
@@ -1251,7 +1289,8 @@ struct KMicApply {
                 if (il == 0 && gi > 0) xi = gi - 1;
                 if (il == 7 && gi + 1 < N) xi = gi + 1;
                 if (xi >= 0) {
-                    const int64_t ln = cidx(N, xi, gj, k0);
+                    const int64_t ln = halo_fwd ? tile_major(t + (xi < gi ? -TT : TT), xi & 7, jl, 0)
+                                                : cidx(N, xi, gj, k0);
                     const uint32_t hf = fl8[((int64_t)xi * N + gj) * NS + tk];
                     float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
                     const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
@@ -1271,7 +1310,8 @@ struct KMicApply {
                 if (jl == 0 && gj > 0) yj = gj - 1;
                 if (jl == 7 && gj + 1 < N) yj = gj + 1;
                 if (yj >= 0) {
-                    const int64_t ln = cidx(N, gi, yj, k0);
+                    const int64_t ln = halo_fwd ? tile_major(t + (yj < gj ? -TN : TN), il, yj & 7, 0)
+                                                : cidx(N, gi, yj, k0);
                     const uint32_t hf = fl8[((int64_t)gi * N + yj) * NS + tk];
                     float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
                     const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
@@ -1291,7 +1331,11 @@ struct KMicApply {
                 const int64_t line = cidx(N, gi, gj, k0);
                 const int64_t sgl = ((int64_t)gi * N + gj) * NS + tk;
                 const bool has_m = k0 > 0, has_p = k0 + kTile < N;
-                const int64_t lm = has_m ? line - 1 : line, lp = has_p ? line + kTile : line;
+                int64_t lm = has_m ? line - 1 : line, lp = has_p ? line + kTile : line;
+                if (halo_fwd) {
+                    lm = has_m ? tile_major(t - 1, il, jl, kTile - 1) : tile_major(t, il, jl, 0);
+                    lp = has_p ? tile_major(t + 1, il, jl, 0) : tile_major(t, il, jl, 0);
+                }
                 const uint32_t fm = fl8[has_m ? sgl - 1 : sgl], fp = fl8[has_p ? sgl + 1 : sgl];
                 const float vm = vb ? va[lm] * vb[lm] : va[lm];
                 const float vp = vb ? va[lp] * vb[lp] : va[lp];
@@ -1379,7 +1423,7 @@ struct KMicApply {
                     Z[g] = v;
                     dot[0] += (double)v * (double)rreg[m];
                 } else {
-                    Q[g] = v;
+                    Q[tile_major(t, m, cj, ce)] = v;
                 }
             }
         }
@@ -1648,6 +1692,8 @@ void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, cons
     } else {
         launch<KMicBuild>(s, dim3(nb(2)), dim3(64), {p, A, diag, nup, pc, -1, act.tiles[2], act.ntiles[2], sc});
     }
+    const int64_t nc = (int64_t)p.N * p.N * p.N;
+    launch<KPcTileMajor>(s, dim3(nb(2)), dim3(64), {p.N, pc, pc + nc, act.tiles[2], act.ntiles[2], sc});
 }
 
 void launch_fluid_segments(hipStream_t s, int N, const uint8_t* A, uint8_t* fl8) {
@@ -1683,15 +1729,16 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, co
         launch_finalize(s, red, FIN_RHO, dev ? PARTS_LIST : nb);
         return;
     }
+    const float* pct = pc + (int64_t)p.N * p.N * p.N;   // the tile-major copy (MIC(0) only)
     if (p.precond == 0) {
         const int nr = (int)mic_grid(tiles_or_cap(act, 0)), nbk = (int)mic_grid(tiles_or_cap(act, 1));
-        launch<KMicApply<0>>(s, dim3(nr), dim3(256), {p, fl8, pc, r, q, z, act.tiles[0], act.ntiles[0], red, 0});
-        launch<KMicApply<1>>(s, dim3(nbk), dim3(256), {p, fl8, pc, r, q, z, act.tiles[1], act.ntiles[1], red, 0});
-        launch<KMicApply<2>>(s, dim3(nr), dim3(256), {p, fl8, pc, r, q, z, act.tiles[0], act.ntiles[0], red, dev ? -1 : nbk});
+        launch<KMicApply<0>>(s, dim3(nr), dim3(256), {p, fl8, pct, r, q, z, act.tiles[0], act.ntiles[0], red, 0});
+        launch<KMicApply<1>>(s, dim3(nbk), dim3(256), {p, fl8, pct, r, q, z, act.tiles[1], act.ntiles[1], red, 0});
+        launch<KMicApply<2>>(s, dim3(nr), dim3(256), {p, fl8, pct, r, q, z, act.tiles[0], act.ntiles[0], red, dev ? -1 : nbk});
         launch_finalize(s, red, FIN_RHO, dev ? PARTS_MIC_RB : nbk + nr);
     } else {
         const int nb = (int)mic_grid(tiles_or_cap(act, 2));
-        launch<KMicApply<3>>(s, dim3(nb), dim3(256), {p, fl8, pc, r, q, z, act.tiles[2], act.ntiles[2], red, 0});
+        launch<KMicApply<3>>(s, dim3(nb), dim3(256), {p, fl8, pct, r, q, z, act.tiles[2], act.ntiles[2], red, 0});
         launch_finalize(s, red, FIN_RHO, dev ? PARTS_MIC_ALL : nb);
     }
 }
