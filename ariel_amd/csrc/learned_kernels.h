@@ -398,7 +398,9 @@ struct KConvTile {
 };
 
 // Output layer on the VALU: z = s * w * sum_{tap, c} W[tap][c] d0[v + tap][c],
-// one thread per voxel of a 4x4x16 brick, d0's halo staged in LDS, FP32.
+// one thread per voxel of a 4x4x16 brick, d0's halo staged in LDS, FP32. The
+// weights are the same for every lane: they are read with scalar loads (kept
+// out of LDS, whose bandwidth the halo reads need).
 struct KConvOutTile {
     static constexpr int kThreads = 256;
     static constexpr int CP = 20;
@@ -415,15 +417,13 @@ struct KConvOutTile {
         using namespace tile;
         if (gated(a.sc)) return;
         __shared__ __align__(16) uint16_t s_in[HV * CP];
-        __shared__ uint32_t s_w[27 * 8];    // column 0 of each tap, as BF16 pairs
+        const uint32_t* __restrict__ w32 = reinterpret_cast<const uint32_t*>(a.w);   // row 0 of tap t at 128 t
         const int N = a.N;
         const int64_t nb = bricks(N);
         const int64_t b = xcd_block(nb);
         if (b >= nb) return;
         int x0, y0, z0;
         brick_origin(b, N, x0, y0, z0);
-        for (int i = threadIdx.x; i < 27 * 8; i += blockDim.x)
-            s_w[i] = reinterpret_cast<const uint32_t*>(a.w)[(i >> 3) * 128 + (i & 7)];
         stage<16, CP, kPlain>(s_in, a.in, nullptr, N, x0, y0, z0);
         __syncthreads();
         const int cz = threadIdx.x & 15, cy = (threadIdx.x >> 4) & 3, cx = threadIdx.x >> 6;
@@ -437,8 +437,8 @@ struct KConvOutTile {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint2 v = src[q];
-                acc = __builtin_amdgcn_fdot2_f32_bf16(as_bf2(v.x), as_bf2(s_w[t * 8 + 2 * q]), acc, false);
-                acc = __builtin_amdgcn_fdot2_f32_bf16(as_bf2(v.y), as_bf2(s_w[t * 8 + 2 * q + 1]), acc, false);
+                acc = __builtin_amdgcn_fdot2_f32_bf16(as_bf2(v.x), as_bf2(w32[t * 128 + 2 * q]), acc, false);
+                acc = __builtin_amdgcn_fdot2_f32_bf16(as_bf2(v.y), as_bf2(w32[t * 128 + 2 * q + 1]), acc, false);
             }
         }
         const int64_t vo = ((int64_t)gx * N + gy) * N + gz;
