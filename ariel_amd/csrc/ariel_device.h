@@ -13,6 +13,11 @@ namespace ariel {
 
 __host__ __device__ __forceinline__ float gmax(float x, float y) { return (x < y) ? y : x; }
 __host__ __device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
+// gmax(x, c) as one v_max_f32, for x that is never NaN and never -0.0 (where
+// gmax would keep -0.0 and the hardware max may give +0.0): differences
+// 1 - q and q - 1 of finite q (an exact zero difference is +0.0 under
+// round-to-nearest) and sums of squares.
+__device__ __forceinline__ float gmax_num(float x, float c) { return fmaxf(x, c); }
 
 // Sqrlength (src/math/kernels.inl:25-30)
 __device__ __forceinline__ float sqrlength(float ax, float ay, float az, float bx, float by,
@@ -49,7 +54,7 @@ __device__ __forceinline__ float div_fixed(float x, float d, float yd) {
 }
 // smooth(r2, h) with hh = h*h and yhh = 1.0f / hh: same value, bit for bit.
 __device__ __forceinline__ float smooth_fixed(float r2, float hh, float yhh) {
-    return gmax(1.0f - div_fixed(r2, hh, yhh), 0.0f);
+    return gmax_num(1.0f - div_fixed(r2, hh, yhh), 0.0f);
 }
 
 // Correctly rounded n / d: Newton-Raphson on the hardware reciprocal (one
@@ -70,7 +75,7 @@ __device__ __forceinline__ float div_nr(float n, float d) {
 // sharpen(r2, h) with hh = h*h and the division by div_nr: same value for
 // 2^-60 <= hh <= 2^60 (max(r2, 1e-5) keeps the divisor in range).
 __device__ __forceinline__ float sharpen_nr(float r2, float hh) {
-    return gmax(div_nr(hh, gmax(r2, 1.0e-5f)) - 1.0f, 0.0f);
+    return gmax_num(div_nr(hh, gmax_num(r2, 1.0e-5f)) - 1.0f, 0.0f);
 }
 
 // Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD b % 8).

@@ -455,6 +455,7 @@ struct KDensity {
                     const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
                     const int l0 = staged_index(st, sx, sy, q0);
                     const int l1 = l0 + (q1 - q0);
+#pragma unroll 4
                     for (int l = l0; l < l1; ++l) {
                         float sqd = sqrlength(st.x[l], st.y[l], st.z[l], me.x, me.y, me.z);
                         float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
@@ -558,13 +559,14 @@ constexpr int kP2GCap = 256;   // particles staged per pass (small: occupancy hi
 template <bool TRI>
 __device__ __forceinline__ float p2g_weight(float dx, float dy, float dz, float hh) {
     if constexpr (TRI) {
+        // a factor is +0 once its |d| >= 1, so the product is too
         const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
-        const float w = (gmax(1.0f - ax, 0.0f) * gmax(1.0f - ay, 0.0f)) * gmax(1.0f - az, 0.0f);
-        return (ax >= 1.0f || ay >= 1.0f || az >= 1.0f) ? 0.0f : w;
+        return (gmax_num(1.0f - ax, 0.0f) * gmax_num(1.0f - ay, 0.0f)) * gmax_num(1.0f - az, 0.0f);
     } else {
+        // r2 >= hh gives hh / max(r2, 1e-5) <= 1 (a correctly rounded quotient
+        // of a smaller by a larger number), so the weight is +0 there already
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const float w = 1.0f * sharpen_nr(r2, hh);
-        return r2 >= hh ? 0.0f : w;
+        return 1.0f * sharpen_nr(r2, hh);
     }
 }
 
