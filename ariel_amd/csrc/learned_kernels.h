@@ -57,6 +57,29 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ bf16x2 as_bf2(uint32_t u) { return __builtin_bit_cast(bf16x2, u); }
 // round to nearest even (finite inputs)
+// Transpose of the 4x4 block held by the 4 lanes of a quad (lane a of the quad
+// holds v[0..3] = M[a][0..3]; afterwards it holds M[0..3][a]): two exchanges
+// across the quad with DPP quad permutations, first of lane bit 0 with
+// register bit 0, then of bit 1 with bit 1. b0, b1: bits 0 and 1 of the lane
+// index within the quad.
+__device__ __forceinline__ float quad_swap(float x, int ctrl_is_pairs) {
+    const int v = __float_as_int(x);
+    return __int_as_float(ctrl_is_pairs ? __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false)    // [2,3,0,1]
+                                        : __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));  // [1,0,3,2]
+}
+__device__ __forceinline__ void quad_transpose4(float (&v)[4], bool b0, bool b1) {
+    const float x0 = quad_swap(v[0], 0), x1 = quad_swap(v[1], 0);
+    const float x2 = quad_swap(v[2], 0), x3 = quad_swap(v[3], 0);
+    const float n0 = b0 ? x1 : v[0], n1 = b0 ? v[1] : x0;
+    const float n2 = b0 ? x3 : v[2], n3 = b0 ? v[3] : x2;
+    const float y0 = quad_swap(n0, 1), y1 = quad_swap(n1, 1);
+    const float y2 = quad_swap(n2, 1), y3 = quad_swap(n3, 1);
+    v[0] = b1 ? y2 : n0;
+    v[1] = b1 ? y3 : n1;
+    v[2] = b1 ? n2 : y0;
+    v[3] = b1 ? n3 : y1;
+}
+
 __device__ __forceinline__ uint16_t f2bf(float f) {
     const uint32_t u = __float_as_uint(f);
     const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
@@ -378,20 +401,30 @@ struct KConvTile {
                 }
             }
         }
-        // D: column (output channel) = lane%16, rows (z offsets) 4*(lane/16)+i
+        // D: column (output channel) = lane%16, rows (z offsets) 4*(lane/16)+i.
+        // Each 4x4 block of (lanes 4m..4m+3) x (registers 0..3) is transposed
+        // within its quad, so lane 4m+a then holds channels 4m..4m+3 of voxel
+        // 4*kq+a and writes them as one 8-byte store (a wave writes 16 voxels x
+        // 16 channels as one contiguous 512-byte run per block of channels).
+        const bool b0 = (row & 1) != 0, b1 = (row & 2) != 0;
+        const int gz = z0 + kq * 4 + (row & 3);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = wave * 4 + j;
             const int gx = x0 + col / TY, gy = y0 + col % TY;
-            if (gx >= D || gy >= D) continue;
+            if (gx >= D || gy >= D) continue;   // uniform over the wave
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int gz = z0 + kq * 4 + i;
-                if (gz >= D) continue;
-                const int64_t vo = ((int64_t)gx * D + gy) * D + gz;
+            for (int n = 0; n < NB; ++n) {
+                float v[4];
 #pragma unroll
-                for (int n = 0; n < NB; ++n)
-                    a.out[vo * COUT + n * 16 + row] = f2bf(fmaxf(acc[j][n][i], 0.0f));
+                for (int i = 0; i < 4; ++i) v[i] = fmaxf(acc[j][n][i], 0.0f);
+                quad_transpose4(v, b0, b1);
+                if (gz < D) {
+                    const int64_t vo = ((int64_t)gx * D + gy) * D + gz;
+                    const uint2 w = {(uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                     (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16)};
+                    *reinterpret_cast<uint2*>(a.out + vo * COUT + n * 16 + (row & ~3)) = w;
+                }
             }
         }
     }

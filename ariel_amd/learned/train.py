@@ -387,7 +387,7 @@ def main(argv=None):
     else:
         init = lnet.PressureNet(seed=a.seed)
         model = Trainer(layers_to_flat([p.detach().numpy() for p in init.w]))
-    bs = {32: 8, 48: 4, 64: 4, 96: 1, 128: 1}
+    bs = {32: 8, 48: 4, 64: 4, 96: 1, 128: 1, 256: 1}
     budget = a.minutes * 60.0
     held = held_set()
     log("pairs %d (held out %d); held-out loss before training: %.4f"
