@@ -229,35 +229,60 @@ struct KScatter {
 // Per-cell insertion sort of (id, slot) pairs by id: restores the reference's
 // per-cell order (push_back in particle-vector order). Particles that stayed in
 // their cell arrive in order already, so most segments need no move.
+// One wave owns 64 consecutive cells, whose particles are contiguous: it stages
+// their (id, slot) pairs in LDS with coalesced loads, each lane sorts its cell's
+// segment there, and the wave writes the pairs back only if a lane moved one.
+// A wave whose cells hold more than kCap particles sorts in global memory.
 struct KCellsort {
-    static constexpr int kThreads = kBlock;
+    static constexpr int kThreads = 64;
+    static constexpr int kCap = 1024;
     struct Args {
         const int32_t* offsets;
         int64_t nc;
         int32_t* sid;
         int32_t* slots;
     };
-    static __device__ void run(const Args& kargs) {
-        const int32_t* __restrict__ offsets = kargs.offsets;
-        int64_t nc = kargs.nc;
-        int32_t* __restrict__ sid = kargs.sid;
-        int32_t* __restrict__ slots = kargs.slots;
-
-        int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (c >= nc) return;
-        const int32_t o0 = offsets[c], o1 = offsets[c + 1];
+    static __device__ void sort_segment(int32_t* __restrict__ id, int32_t* __restrict__ sl, int32_t o0,
+                                        int32_t o1, bool& moved) {
         for (int32_t a = o0 + 1; a < o1; ++a) {
-            const int32_t key = sid[a];
-            if (sid[a - 1] <= key) continue;
-            const int32_t sv = slots[a];
+            const int32_t key = id[a];
+            if (id[a - 1] <= key) continue;
+            moved = true;
+            const int32_t sv = sl[a];
             int32_t b = a - 1;
-            while (b >= o0 && sid[b] > key) {
-                sid[b + 1] = sid[b];
-                slots[b + 1] = slots[b];
+            while (b >= o0 && id[b] > key) {
+                id[b + 1] = id[b];
+                sl[b + 1] = sl[b];
                 --b;
             }
-            sid[b + 1] = key;
-            slots[b + 1] = sv;
+            id[b + 1] = key;
+            sl[b + 1] = sv;
+        }
+    }
+    static __device__ void run(const Args& a) {
+        __shared__ int32_t s_id[kCap], s_sl[kCap];
+        const int lane = threadIdx.x;
+        const int64_t c_first = (int64_t)blockIdx.x * kThreads;
+        if (c_first >= a.nc) return;   // beyond this grid (a launch sized for a larger tank)
+        const int64_t c_end = c_first + kThreads < a.nc ? c_first + kThreads : a.nc;
+        const int64_t c = c_first + lane;
+        const int32_t base = a.offsets[c_first], n = a.offsets[c_end] - base;
+        bool moved = false;
+        if (n > kCap) {
+            if (c < a.nc) sort_segment(a.sid, a.slots, a.offsets[c], a.offsets[c + 1], moved);
+            return;
+        }
+        for (int32_t u = lane; u < n; u += kThreads) {
+            s_id[u] = a.sid[base + u];
+            s_sl[u] = a.slots[base + u];
+        }
+        __syncthreads();
+        if (c < a.nc) sort_segment(s_id, s_sl, a.offsets[c] - base, a.offsets[c + 1] - base, moved);
+        __syncthreads();
+        if (!__any(moved)) return;
+        for (int32_t u = lane; u < n; u += kThreads) {
+            a.sid[base + u] = s_id[u];
+            a.slots[base + u] = s_sl[u];
         }
     }
 };
@@ -1151,7 +1176,7 @@ void launch_scatter(hipStream_t s, const int32_t* cell_of, int64_t np, const int
 
 void launch_cellsort(hipStream_t s, const int32_t* offsets, int64_t nc, int32_t* sid,
                      int32_t* slots) {
-    launch<KCellsort>(s, dim3(grid_for(nc, kBlock)), dim3(kBlock), {offsets, nc, sid, slots});
+    launch<KCellsort>(s, dim3(grid_for(nc, KCellsort::kThreads)), dim3(KCellsort::kThreads), {offsets, nc, sid, slots});
 }
 
 void launch_gather(hipStream_t s, const int32_t* slots, int64_t np, const float4* pos_in,
