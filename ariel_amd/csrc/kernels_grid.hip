@@ -12,14 +12,31 @@ namespace {
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-// Face and cell indices stay below 2^31 for N <= 1024: decode in 32-bit.
-__device__ __forceinline__ void decode_face(int N, int comp, int64_t f64, int& i, int& j, int& k) {
-    const uint32_t f = (uint32_t)f64, n = (uint32_t)N, n1 = n + 1;
-    if (comp == 0) { k = (int)(f % n); const uint32_t q = f / n; j = (int)(q % n); i = (int)(q / n); }
-    else if (comp == 1) { k = (int)(f % n); const uint32_t q = f / n; j = (int)(q % n1); i = (int)(q / n1); }
-    else { k = (int)(f % n1); const uint32_t q = f / n1; j = (int)(q % n); i = (int)(q / n); }
+// Faces in rows along k: row r (one wave each) of component comp holds the
+// faces (i, j, 0..kext-1), with rows of component 0, then 1, then 2, each in
+// (i, j) order, so the row's first face is fidx(N, comp, i, j, 0) = rl * kext
+// for the row's index rl within its component. The row index is the same for
+// the whole wave, so its decode is scalar code instead of a division per lane.
+inline int64_t face_rows(int N) { return 2 * (int64_t)(N + 1) * N + (int64_t)N * N; }
+__device__ __forceinline__ bool face_row(int N, int64_t r, int& comp, int& i, int& j, int& kext,
+                                         int64_t& f0) {
+    const int64_t r01 = (int64_t)(N + 1) * N;   // rows of component 0, and of 1
+    if (r < r01) {
+        comp = 0; i = (int)(r / N); j = (int)(r % N); kext = N; f0 = r * kext;
+    } else if (r < 2 * r01) {
+        r -= r01;
+        comp = 1; i = (int)(r / (N + 1)); j = (int)(r % (N + 1)); kext = N; f0 = r * kext;
+    } else if (r < 2 * r01 + (int64_t)N * N) {
+        r -= 2 * r01;
+        comp = 2; i = (int)(r / N); j = (int)(r % N); kext = N + 1; f0 = r * kext;
+    } else {
+        return false;
+    }
+    return true;
 }
+constexpr int kRowThreads = 64;
 
+// Cell indices stay below 2^31 for N <= 1024: decode in 32-bit.
 __device__ __forceinline__ void decode_cell32(int N, int64_t c64, int& i, int& j, int& k) {
     const uint32_t c = (uint32_t)c64, n = (uint32_t)N;
     k = (int)(c % n);
@@ -62,7 +79,7 @@ struct KDivergence {
 // SubtractPressureGradient (flip.cpp:568-680, subcell = 1) + EnforceBoundaryVelocity
 // (particlegridoperations.inl:45-104) per face.
 struct KGradient {
-    static constexpr int kThreads = kBlock;
+    static constexpr int kThreads = kRowThreads;
     struct Args {
         Params p;
         const float* P;
@@ -82,14 +99,16 @@ struct KGradient {
         float* __restrict__ uz = kargs.uz;
 
         const int N = p.N;
-        const int64_t nf = (int64_t)(N + 1) * N * N;
-        int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (g >= 3 * nf) return;
-        const int comp = (int)(g / nf);
-        const int64_t f = g - comp * nf;
-        int i, j, k;
-        decode_face(N, comp, f, i, j, k);
+        int comp, i, j, kext;
+        int64_t f0;
+        if (!face_row(N, blockIdx.x, comp, i, j, kext, f0)) return;
         float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
+        for (int k = threadIdx.x; k < kext; k += kRowThreads) face(p, P, L, A, u, comp, i, j, k, f0 + k);
+    }
+    static __device__ void face(const Params& p, const float* __restrict__ P, const float* __restrict__ L,
+                                const uint8_t* __restrict__ A, float* __restrict__ u, int comp, int i,
+                                int j, int k, int64_t f) {
+        const int N = p.N;
         const int a = comp == 0 ? i : (comp == 1 ? j : k);
         if (a == 0 || a == N) {
             u[f] = 0.0f;
@@ -130,7 +149,7 @@ __device__ __forceinline__ bool face_wallmark(const uint8_t* A, int N, int comp,
 }
 
 struct KExtrapolate {
-    static constexpr int kThreads = kBlock;
+    static constexpr int kThreads = kRowThreads;
     struct Args {
         Params p;
         const uint8_t* A;
@@ -152,15 +171,15 @@ struct KExtrapolate {
         float* duz = kargs.duz;
 
         const int N = p.N;
-        const int64_t nf = (int64_t)(N + 1) * N * N;
-        int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (g >= 3 * nf) return;
-        const int comp = (int)(g / nf);
-        const int64_t f = g - comp * nf;
-        int i, j, k;
-        decode_face(N, comp, f, i, j, k);
+        int comp, i, j, kext;
+        int64_t f0;
+        if (!face_row(N, blockIdx.x, comp, i, j, kext, f0)) return;
         float* u = comp == 0 ? ux : (comp == 1 ? uy : uz);
         float* du = comp == 0 ? dux : (comp == 1 ? duy : duz);
+        for (int k = threadIdx.x; k < kext; k += kRowThreads) face(A, N, u, du, comp, i, j, k, f0 + k);
+    }
+    static __device__ void face(const uint8_t* __restrict__ A, int N, float* u, float* du, int comp, int i,
+                                int j, int k, int64_t f) {
         float val = u[f];
         if (!face_mark(A, N, comp, i, j, k) && face_wallmark(A, N, comp, i, j, k)) {
             unsigned wsum = 0;
@@ -271,14 +290,13 @@ void launch_divergence(hipStream_t s, const Params& p, float* const u[3], float*
 
 void launch_gradient(hipStream_t s, const Params& p, const float* P, const float* L,
                      const uint8_t* A, float* const u[3]) {
-    int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
-    launch<KGradient>(s, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), {p, P, L, A, u[0], u[1], u[2]});
+    launch<KGradient>(s, dim3((unsigned)face_rows(p.N)), dim3(kRowThreads), {p, P, L, A, u[0], u[1], u[2]});
 }
 
 void launch_extrapolate(hipStream_t s, const Params& p, const uint8_t* A, float* const u[3],
                         float* const du[3]) {
-    int64_t nf = (int64_t)(p.N + 1) * p.N * p.N;
-    launch<KExtrapolate>(s, dim3(grid_for(3 * nf, kBlock)), dim3(kBlock), {p, A, u[0], u[1], u[2], du[0], du[1], du[2]});
+    launch<KExtrapolate>(s, dim3((unsigned)face_rows(p.N)), dim3(kRowThreads),
+                         {p, A, u[0], u[1], u[2], du[0], du[1], du[2]});
 }
 
 void launch_grid_stats(hipStream_t s, const Params& p, float* const u[3], const uint8_t* A,
