@@ -240,32 +240,55 @@ __device__ __forceinline__ float input_scale(const SolverScalars* sc, int64_t ns
 template <int CIN, int CP, int MODE>
 __device__ __forceinline__ void stage(uint16_t* s_in, const uint16_t* in, const uint16_t* up,
                                       int D, int x0, int y0, int z0) {
+    // Each thread first issues the loads of all its entries (a fixed count for
+    // the callers' 256 threads) and only then waits, rounds and stores: a loop
+    // that waited on each entry's load before the next kept one load in flight.
     constexpr int C8 = CIN / 8;
+    constexpr int kTotal = HV * C8;
+    constexpr int kNT = 256;
+    constexpr int kPer = (kTotal + kNT - 1) / kNT;
     const int H = D >> 1;
-    for (int u = threadIdx.x; u < HV * C8; u += blockDim.x) {
-        const int hv = u / C8, c8 = u - hv * C8;
-        const int hz = hv % HZ, hy = (hv / HZ) % HY, hx = hv / (HZ * HY);
-        const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-        uint4 val = {0u, 0u, 0u, 0u};
-        if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
-            const int64_t v = ((int64_t)gx * D + gy) * D + gz;
-            val = *reinterpret_cast<const uint4*>(in + v * CIN + c8 * 8);
-            if constexpr (MODE == kUpAdd) {
-                const int64_t pv = ((int64_t)(gx >> 1) * H + (gy >> 1)) * H + (gz >> 1);
-                const uint4 pp = *reinterpret_cast<const uint4*>(up + pv * CIN + c8 * 8);
-                uint32_t* e = reinterpret_cast<uint32_t*>(&val);
-                const uint32_t* q = reinterpret_cast<const uint32_t*>(&pp);
+    uint4 val[kPer];
+    uint4 upv[MODE == kUpAdd ? kPer : 1];
+    int dst[kPer];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const uint32_t lo = f2bf(bf2f((uint16_t)(e[m] & 0xFFFFu)) + bf2f((uint16_t)(q[m] & 0xFFFFu)));
-                    const uint32_t hi = f2bf(bf2f((uint16_t)(e[m] >> 16)) + bf2f((uint16_t)(q[m] >> 16)));
-                    e[m] = lo | (hi << 16);
+    for (int m = 0; m < kPer; ++m) {
+        const int u = threadIdx.x + m * kNT;
+        val[m] = make_uint4(0u, 0u, 0u, 0u);
+        if constexpr (MODE == kUpAdd) upv[m] = make_uint4(0u, 0u, 0u, 0u);
+        dst[m] = -1;
+        if (u < kTotal) {
+            const int hv = u / C8, c8 = u - hv * C8;
+            dst[m] = hv * CP + c8 * 8;
+            const int hz = hv % HZ, hy = (hv / HZ) % HY, hx = hv / (HZ * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
+                const int64_t v = ((int64_t)gx * D + gy) * D + gz;
+                val[m] = *reinterpret_cast<const uint4*>(in + v * CIN + c8 * 8);
+                if constexpr (MODE == kUpAdd) {
+                    const int64_t pv = ((int64_t)(gx >> 1) * H + (gy >> 1)) * H + (gz >> 1);
+                    upv[m] = *reinterpret_cast<const uint4*>(up + pv * CIN + c8 * 8);
                 }
             }
         }
-        uint2* dst = reinterpret_cast<uint2*>(s_in + hv * CP + c8 * 8);
-        dst[0] = make_uint2(val.x, val.y);
-        dst[1] = make_uint2(val.z, val.w);
+    }
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+        if (dst[m] < 0) continue;
+        if constexpr (MODE == kUpAdd) {
+            // e + up(p) rounded to BF16 (outside the grid both are +0 and so is the sum)
+            uint32_t* e = reinterpret_cast<uint32_t*>(&val[m]);
+            const uint32_t* q = reinterpret_cast<const uint32_t*>(&upv[m]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t lo = f2bf(bf2f((uint16_t)(e[c] & 0xFFFFu)) + bf2f((uint16_t)(q[c] & 0xFFFFu)));
+                const uint32_t hi = f2bf(bf2f((uint16_t)(e[c] >> 16)) + bf2f((uint16_t)(q[c] >> 16)));
+                e[c] = lo | (hi << 16);
+            }
+        }
+        uint2* d2 = reinterpret_cast<uint2*>(s_in + dst[m]);
+        d2[0] = make_uint2(val[m].x, val[m].y);
+        d2[1] = make_uint2(val[m].z, val[m].w);
     }
 }
 }  // namespace tile
@@ -314,16 +337,30 @@ struct KConvTile {
             const float s = input_scale(a.sc, a.nseg);
             const float inv = s > 0.0f ? 1.0f / s : 0.0f;
             uint32_t* s32 = reinterpret_cast<uint32_t*>(s_in);
-            for (int hv = threadIdx.x; hv < HV; hv += blockDim.x) {
-                const int hz = hv % HZ, hy = (hv / HZ) % HY, hx = hv / (HZ * HY);
-                const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-                uint32_t pair = 0u;
-                if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
-                    const int64_t q = ((int64_t)gx * D + gy) * D + gz;
-                    const float w = a.wd[q];
-                    if (w != 0.0f) pair = (uint32_t)f2bf(a.r[q] * inv * w) | ((uint32_t)f2bf(w) << 16);
+            // all loads first (see stage()), then the pairs
+            constexpr int kPer = (HV + kThreads - 1) / kThreads;
+            float wv[kPer], rv[kPer];
+#pragma unroll
+            for (int m = 0; m < kPer; ++m) {
+                const int hv = threadIdx.x + m * kThreads;
+                wv[m] = 0.0f;
+                rv[m] = 0.0f;
+                if (hv < HV) {
+                    const int hz = hv % HZ, hy = (hv / HZ) % HY, hx = hv / (HZ * HY);
+                    const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+                    if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
+                        const int64_t q = ((int64_t)gx * D + gy) * D + gz;
+                        wv[m] = a.wd[q];
+                        rv[m] = a.r[q];
+                    }
                 }
-                s32[hv] = pair;
+            }
+#pragma unroll
+            for (int m = 0; m < kPer; ++m) {
+                const int hv = threadIdx.x + m * kThreads;
+                if (hv >= HV) continue;
+                const float w = wv[m];
+                s32[hv] = w != 0.0f ? ((uint32_t)f2bf(rv[m] * inv * w) | ((uint32_t)f2bf(w) << 16)) : 0u;
             }
         } else {
             stage<CIN, CP, MODE>(s_in, a.in, a.up, D, x0, y0, z0);
