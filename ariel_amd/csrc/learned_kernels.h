@@ -136,9 +136,10 @@ struct KConv {
         f32x4 acc[NB];
 #pragma unroll
         for (int n = 0; n < NB; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // K = taps x input channels, in chunks of 32 (see mfma32)
+        // K = taps x input channels, in chunks of 32 (see mfma32); unrolled
+        // so that the loads of several chunks are in flight together
         constexpr int K = TAPS * CIN, NCH = (K + 31) / 32;
-#pragma unroll 2
+#pragma unroll 7
         for (int ch = 0; ch < NCH; ++ch) {
             const int k0 = 32 * ch + 8 * kq;
             bf16x8 av = zero8();
