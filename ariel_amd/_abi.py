@@ -111,6 +111,7 @@ PROTOTYPES = {
     "ariel_export_particles": (_I32, [_P, ctypes.c_char_p]),
     "ariel_mesh_sdf": (_I32, [_PF, _I32, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                               ctypes.POINTER(ctypes.c_int64)]),
+    "ariel_selftest_bf16": (_I32, [ctypes.POINTER(ctypes.c_int64)]),
     "ariel_selftest_division": (_I32, [ctypes.c_int32, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, ctypes.POINTER(ctypes.c_int64)]),
     "ariel_set_learned_weights": (_I32, [_P, ctypes.c_void_p, ctypes.c_int64]),

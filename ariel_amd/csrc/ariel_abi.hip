@@ -1437,6 +1437,15 @@ int ariel_set_learned_weights(ariel_sim* h, const void* blob, int64_t bytes) {
     return ARIEL_OK;
 }
 
+int ariel_selftest_bf16(int64_t* mismatches) {
+    if (!mismatches) return fail(nullptr, ARIEL_ERR_INVALID, "selftest_bf16: null pointer");
+    *mismatches = -1;
+    hipError_t e = run_bf16_selftest(mismatches);
+    if (e != hipSuccess)
+        return fail(nullptr, ARIEL_ERR_HIP, std::string("selftest_bf16: ") + hipGetErrorString(e));
+    return ARIEL_OK;
+}
+
 int ariel_selftest_division(int32_t which, float a, float lo, float hi, int64_t* mismatches) {
     if (!mismatches || (which != 0 && which != 1) || !(lo >= 0.0f) || !(hi >= lo) ||
         !std::isfinite(hi) || !(a > 0.0f) || !std::isfinite(a))

@@ -100,6 +100,7 @@ struct Reduce {
 };
 
 // Exhaustive division self-check (selftest.hip); lo, hi non-negative.
+hipError_t run_bf16_selftest(int64_t* mismatches);
 hipError_t run_division_selftest(int which, float a, float lo, float hi, int64_t* mismatches);
 
 // Learned preconditioner (learned.hip): packed BF16 weights and activation

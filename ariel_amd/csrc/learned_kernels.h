@@ -56,7 +56,6 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ bf16x2 as_bf2(uint32_t u) { return __builtin_bit_cast(bf16x2, u); }
-// round to nearest even (finite inputs)
 // Transpose of the 4x4 block held by the 4 lanes of a quad (lane a of the quad
 // holds v[0..3] = M[a][0..3]; afterwards it holds M[0..3][a]): two exchanges
 // across the quad with DPP quad permutations, first of lane bit 0 with
@@ -80,10 +79,18 @@ __device__ __forceinline__ void quad_transpose4(float (&v)[4], bool b0, bool b1)
     v[3] = b1 ? n3 : y1;
 }
 
+// round to nearest even (finite inputs)
 __device__ __forceinline__ uint16_t f2bf(float f) {
     const uint32_t u = __float_as_uint(f);
     const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(r >> 16);
+}
+// Two floats rounded to BF16 (nearest even) and packed, lo in the low half:
+// one v_cvt_pk_bf16_f32, equal to f2bf of each for finite inputs.
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+    const f32x2v f = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
 }
 
 // symmetric diagonal scaling of a FLUID cell: sqrt(6 / diag), IEEE divide and root
@@ -282,9 +289,8 @@ __device__ __forceinline__ void stage(uint16_t* s_in, const uint16_t* in, const 
             const uint32_t* q = reinterpret_cast<const uint32_t*>(&upv[m]);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const uint32_t lo = f2bf(bf2f((uint16_t)(e[c] & 0xFFFFu)) + bf2f((uint16_t)(q[c] & 0xFFFFu)));
-                const uint32_t hi = f2bf(bf2f((uint16_t)(e[c] >> 16)) + bf2f((uint16_t)(q[c] >> 16)));
-                e[c] = lo | (hi << 16);
+                e[c] = f2bf2(bf2f((uint16_t)(e[c] & 0xFFFFu)) + bf2f((uint16_t)(q[c] & 0xFFFFu)),
+                             bf2f((uint16_t)(e[c] >> 16)) + bf2f((uint16_t)(q[c] >> 16)));
             }
         }
         uint2* d2 = reinterpret_cast<uint2*>(s_in + dst[m]);
@@ -361,7 +367,7 @@ struct KConvTile {
                 const int hv = threadIdx.x + m * kThreads;
                 if (hv >= HV) continue;
                 const float w = wv[m];
-                s32[hv] = w != 0.0f ? ((uint32_t)f2bf(rv[m] * inv * w) | ((uint32_t)f2bf(w) << 16)) : 0u;
+                s32[hv] = w != 0.0f ? f2bf2(rv[m] * inv * w, w) : 0u;
             }
         } else {
             stage<CIN, CP, MODE>(s_in, a.in, a.up, D, x0, y0, z0);
@@ -459,8 +465,7 @@ struct KConvTile {
                 quad_transpose4(v, b0, b1);
                 if (gz < D) {
                     const int64_t vo = ((int64_t)gx * D + gy) * D + gz;
-                    const uint2 w = {(uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
-                                     (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16)};
+                    const uint2 w = {f2bf2(v[0], v[1]), f2bf2(v[2], v[3])};
                     *reinterpret_cast<uint2*>(a.out + vo * COUT + n * 16 + (row & ~3)) = w;
                 }
             }

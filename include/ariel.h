@@ -395,6 +395,11 @@ int ariel_trainer_forward(ariel_trainer* t, int32_t N, int32_t batch, const floa
  * *mismatches receives the number of x whose result differs in any bit. */
 int ariel_selftest_division(int32_t which, float a, float lo, float hi, int64_t* mismatches);
 
+/* Self-check of the network kernels' BF16 rounding (no counterpart in the
+ * reference): the hardware conversion of two floats to packed BF16 against
+ * round-to-nearest-even on the bit pattern, for every finite float. */
+int ariel_selftest_bf16(int64_t* mismatches);
+
 #ifdef __cplusplus
 }
 #endif

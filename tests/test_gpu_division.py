@@ -68,3 +68,14 @@ def test_division_check_detects_a_wrong_divisor_range():
     # sanity of the checker: outside the routine's stated domain (divisors far
     # below 2^-60 with a large numerator overflow the quotient) mismatches appear
     assert _check(1, 1.0e30, 1.0e-30, 1.0e-28) > 0
+
+
+def test_bf16_rounding_matches_round_to_nearest_even():
+    """The network kernels round FP32 to BF16 with the hardware's packed
+    conversion; it must equal round-to-nearest-even on the bit pattern (the
+    rounding the PyTorch restatement and the trainer use) for every finite float."""
+    lib = _abi.load()
+    bad = ctypes.c_int64(-1)
+    rc = lib.ariel_selftest_bf16(ctypes.byref(bad))
+    assert rc == 0, lib.ariel_last_error(None).decode()
+    assert bad.value == 0
