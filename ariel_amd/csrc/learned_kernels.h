@@ -167,18 +167,20 @@ struct KConv {
 #pragma unroll
             for (int n = 0; n < NB; ++n) acc[n] = mfma32(av, bv[n], acc[n]);
         }
-        // D: column (output channel) = lane%16, rows (voxels) 4*(lane/16)+j
+        // D: column (output channel) = lane%16, rows (voxels) 4*(lane/16)+j;
+        // transposed within quads of lanes (see KConvTile) for 8-byte stores
+        const bool b0 = (row & 1) != 0, b1 = (row & 2) != 0;
+        const int64_t vo = v0 + kq * 4 + (row & 3);
 #pragma unroll
-        for (int n = 0; n < NB; ++n)
+        for (int n = 0; n < NB; ++n) {
+            float x[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t vo = v0 + kq * 4 + j;
-                if (vo < nvox) {
-                    float x = acc[n][j];
-                    if (RELU) x = fmaxf(x, 0.0f);
-                    a.out[vo * COUT + (nb0 + n) * 16 + row] = f2bf(x);
-                }
-            }
+            for (int j = 0; j < 4; ++j) x[j] = RELU ? fmaxf(acc[n][j], 0.0f) : acc[n][j];
+            quad_transpose4(x, b0, b1);
+            if (vo < nvox)
+                *reinterpret_cast<uint2*>(a.out + vo * COUT + (nb0 + n) * 16 + (row & ~3)) =
+                    make_uint2(f2bf2(x[0], x[1]), f2bf2(x[2], x[3]));
+        }
     }
 };
 
