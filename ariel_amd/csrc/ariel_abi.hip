@@ -214,7 +214,7 @@ double kernel_bytes(const ariel_sim* h, int k) {
 // Each event record leaves the GPU idle for a few microseconds between the
 // kernels it separates, so inside the PCG loop only one iteration in
 // kProfEvery is timed (the launches do not depend on the iteration index).
-constexpr int kProfEvery = 8;
+constexpr int kProfEvery = 16;
 inline bool prof_sampled(int it) { return (it - 1) % kProfEvery == 0; }
 
 struct Prof {

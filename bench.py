@@ -406,7 +406,7 @@ def main():
         avg_ms = ks.total_ms / ks.launches
         gbs = ks.bytes_per_launch / (avg_ms * 1e-3) / 1e9
         kernels[_abi.KERNEL_NAMES[k]] = {
-            # PCG-loop groups are timed on one iteration in 8: total_ms estimates all calls
+            # PCG-loop groups are timed on one iteration in 16: total_ms estimates all calls
             "launches": int(ks.calls), "timed_launches": int(ks.launches), "avg_ms": avg_ms,
             "total_ms": avg_ms * ks.calls,
             "bytes_per_launch": ks.bytes_per_launch, "achieved_GBs": gbs,

@@ -267,7 +267,7 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z);
 /* Per-kernel device timing with HIP events recorded on the handle's stream
  * around each launch group (no reference counterpart: measurement only).
  * Enabling adds two event records per launch group, each of which idles the
- * GPU for a few microseconds; inside the PCG loop only one iteration in 8 is
+ * GPU for a few microseconds; inside the PCG loop only one iteration in 16 is
  * timed for that reason. Stats accumulate until reset. */
 int ariel_set_profiling(ariel_sim* h, int32_t enable);
 int ariel_reset_kernel_stats(ariel_sim* h);
