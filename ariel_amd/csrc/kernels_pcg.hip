@@ -348,9 +348,11 @@ __device__ __forceinline__ void spmv_quad(const Params& p, const uint8_t* __rest
 
 // z = A s, sigma = z.s, alpha = rho / sigma over the active segments (two quads
 // each); y elsewhere keeps the zeros of the solve's set-up.
-// One active segment per thread; blocks take contiguous runs of the list and
-// each XCD a contiguous eighth of the blocks, so the +-x neighbour planes of a
-// segment are read by the same XCD that reads them as its own (shared L2).
+// One quad per thread (neighbouring lanes take the two quads of a segment, so a
+// wave's loads of the list's segments are contiguous); blocks take contiguous
+// runs of the list and each XCD a contiguous eighth of the blocks, so the +-x
+// neighbour planes of a segment are read by the same XCD that reads them as its
+// own (shared L2).
 struct KSpmvDot {
     static constexpr int kThreads = 256;
     struct Args {
@@ -377,22 +379,23 @@ struct KSpmvDot {
 
         if (red.sc->done) return;
         nseg = seg_count(nseg, red.sc);
-        per_thread = spmv_per_thread(nseg);
+        const int64_t nq = 2 * nseg;   // quads
+        per_thread = spmv_per_thread(nq);
         const int64_t per_block = (int64_t)blockDim.x * per_thread;
-        const int64_t nb = spmv_grid(nseg);
+        const int64_t nb = spmv_grid(nq);
         const int64_t b = xcd_block(nb);
         if (b >= nb) return;
         double v[1] = {0.0};
         for (int m = 0; m < per_thread; ++m) {
             const int64_t u = b * per_block + (int64_t)m * blockDim.x + threadIdx.x;
-            if (u >= nseg) break;
-            const uint32_t sg = (uint32_t)segs[u];
+            if (u >= nq) break;
+            const uint32_t sg = (uint32_t)segs[u >> 1];
+            const int half = (int)(u & 1);
             const uint32_t NS = (uint32_t)p.N / kTile;
             const uint32_t line = sg / NS;   // i*N + j
-            const int k0 = (int)(sg - line * NS) * kTile;
+            const int k0 = (int)(sg - line * NS) * kTile + 4 * half;
             const int j = (int)(line % (uint32_t)p.N), i = (int)(line / (uint32_t)p.N);
-            spmv_quad(p, A, L, x, y, 2 * (int64_t)sg, i, j, k0, v[0]);
-            spmv_quad(p, A, L, x, y, 2 * (int64_t)sg + 1, i, j, k0 + 4, v[0]);
+            spmv_quad(p, A, L, x, y, 2 * (int64_t)sg + half, i, j, k0, v[0]);
         }
         block_partials<1>(v, red.partials, b);
     }
@@ -680,7 +683,7 @@ enum { PARTS_LIST = -1, PARTS_SPMV = -2, PARTS_MIC_RB = -3, PARTS_MIC_ALL = -4 }
 __device__ __forceinline__ int64_t parts_from_counts(int code, const SolverScalars* sc) {
     switch (code) {
         case PARTS_LIST: return list_grid(sc->nseg);
-        case PARTS_SPMV: return spmv_grid(sc->nseg);
+        case PARTS_SPMV: return spmv_grid(2 * sc->nseg);
         case PARTS_MIC_RB: return mic_grid(sc->ntile[1]) + mic_grid(sc->ntile[0]);
         default: return mic_grid(sc->ntile[2]);
     }
@@ -1754,9 +1757,9 @@ void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const fl
 
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                      const float* x, float* y, const Reduce& red, const Active& act) {
-    const int64_t n = seg_or_cap(act);
-    const int64_t nb = spmv_grid(n);
-    launch<KSpmvDot>(s, dim3((unsigned)xcd_grid(nb)), dim3(kBlock), {p, A, L, x, y, act.segs, act.nseg, spmv_per_thread(n), red});
+    const int64_t nq = 2 * seg_or_cap(act);   // quads
+    const int64_t nb = spmv_grid(nq);
+    launch<KSpmvDot>(s, dim3((unsigned)xcd_grid(nb)), dim3(kBlock), {p, A, L, x, y, act.segs, act.nseg, spmv_per_thread(nq), red});
     launch_finalize(s, red, FIN_SIGMA, act.nseg < 0 ? PARTS_SPMV : (int)nb);
 }
 
