@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(_HERE, "_build", "liboracle.so")
+# ARIEL_ORACLE_LIB selects another build of the same source (the planted-mistake
+# test loads deliberately broken copies through it).
+LIB = os.environ.get("ARIEL_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 FAITHFUL = 1

@@ -1,0 +1,59 @@
+"""Planted mistakes: each copy of the oracle source below carries one plausible
+error (a wrong window, radius, constant, face offset, ghost ratio, sign or
+averaging rule) and must fail the brute-force and closed-form pins of
+test_oracle_pins.py. The copies are compiled on the CPU into a temporary
+folder; the committed oracle is left untouched."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "oracle", "flip_oracle.cpp")
+
+MISTAKES = {
+    "density window of two cells": ("for_cell_neighbors(s, position, 1,", "for_cell_neighbors(s, position, 2,"),
+    "density radius 3 instead of 4 lattice spacings": (
+        "const float hsm = 4.0f * s.d.density / maxd;", "const float hsm = 3.0f * s.d.density / maxd;"),
+    "cell SDF constant": ("return 0.2f * n0 - accm;", "return 0.25f * n0 - accm;"),
+    "splat x window symmetric": ("for_wall_neighbors(s, i, j, k, 1, 2, 2,", "for_wall_neighbors(s, i, j, k, 2, 2, 2,"),
+    "splat y face at the cell centre": (
+        "V3 py = {(float)i + 0.5f, (float)j, (float)k + 0.5f};",
+        "V3 py = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};"),
+    "ghost pressure ratio inverted": (
+        "else pb = Lc(bi, bj, bk) / gmin(1.0e-6f, Lc(i, j, k)) * Pc(i, j, k);",
+        "else pb = Lc(i, j, k) / gmin(1.0e-6f, Lc(bi, bj, bk)) * Pc(i, j, k);"),
+    "pressure gradient sign": ("return (pf - pb) / h;", "return (pb - pf) / h;"),
+    "extrapolation into every unmarked face": (
+        "if (!(!mark[n][f] && wallmark[n][f])) continue;", "if (!(!mark[n][f])) continue;"),
+    "extrapolation mean over one extra face": (
+        "if (wsum) s.uf[n][f] = sum / (float)wsum;", "if (wsum) s.uf[n][f] = sum / (float)(wsum + 1);"),
+}
+
+
+def _run_pins(tmp_path, text):
+    src = tmp_path / "flip_oracle.cpp"
+    src.write_text(text)
+    lib = tmp_path / "liboracle.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-shared",
+                    "-I", os.path.join(ROOT, "oracle"), str(src), "-o", str(lib)], check=True)
+    env = dict(os.environ, ARIEL_ORACLE_LIB=str(lib))
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
+                        os.path.join(ROOT, "tests", "test_oracle_pins.py")],
+                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    return r.returncode, r.stdout[-2000:]
+
+
+def test_unplanted_copy_passes_the_pins(tmp_path):
+    rc, out = _run_pins(tmp_path, open(SRC).read())
+    assert rc == 0, out
+
+
+@pytest.mark.parametrize("name", list(MISTAKES))
+def test_planted_mistake_fails_the_pins(tmp_path, name):
+    old, new = MISTAKES[name]
+    text = open(SRC).read()
+    assert text.count(old) == 1, "the planted site must exist exactly once"
+    rc, out = _run_pins(tmp_path, text.replace(old, new))
+    assert rc == 1, f"{name}: the pins passed on a broken oracle\n{out}"
