@@ -17,6 +17,10 @@ oracle bit for bit in test_gpu_parity.py).
 - ExtrapolateVelocity (flip.cpp:452-566): a constant field is a fixed point; faces
   next to a FLUID cell never change; every changed face lies between two
   SOLID-or-outside cells and takes a value inside the range of its neighbours.
+- ResampleParticles (particleresampler.inl:35-148): an all-pairs float64 spring
+  displacement and Sharpen-weighted velocity (both kernels vanish beyond re = half
+  a cell, inside the one-cell search window) and the wall accept rule, on
+  particles with no pair close enough to take the coincident-pair branch.
 """
 import numpy as np
 import pytest
@@ -217,4 +221,51 @@ def test_extrapolation_fixed_point_and_range():
     o.stage(_abi.STAGE_EXTRAPOLATE)
     for comp in range(3):
         assert np.all(o.get_field(_abi.FIELD_UX + comp) == np.float32(1.25))
+    o.close()
+
+
+def test_resample_is_the_all_pairs_spring_and_sharpen_mean():
+    n, m = N, 1500
+    d = scenes.dam_break(n)
+    o = OracleSim(d, threads=1)
+    o.init()
+    rng = np.random.default_rng(5)
+    re = float(np.float32(d.density) / np.float32(n))
+    pos = rng.uniform(0.04, 0.62, (4 * m, 3)).astype(np.float32)
+    keep = np.ones(len(pos), bool)
+    d2 = ((pos[:, None, :].astype(np.float64) - pos[None, :, :]) ** 2).sum(axis=2)
+    np.fill_diagonal(d2, np.inf)
+    for a in range(len(pos)):   # no pair within 0.2 re: the coincident branch stays off
+        if keep[a] and (d2[a, keep] < (0.2 * re) ** 2).any():
+            keep[a] = False
+    pos = pos[keep][:m]
+    vel = rng.normal(0.0, 1.0, (len(pos), 3)).astype(np.float32)
+    o.set_particles(pos, vel, np.zeros(len(pos), np.float32))
+    o.stage(_abi.STAGE_RESAMPLE)
+    p_got, u_got, _ = o.get_particles()
+    acc = o.get_accept().astype(bool)
+
+    p64, u64 = pos.astype(np.float64), vel.astype(np.float64)
+    diff = p64[:, None, :] - p64[None, :, :]
+    dist = np.sqrt((diff ** 2).sum(axis=2))
+    np.fill_diagonal(dist, np.inf)
+    w = float(np.float32(d.spring)) * np.maximum(1.0 - dist ** 2 / re ** 2, 0.0)
+    spring = (w[:, :, None] * diff / dist[:, :, None] * re).sum(axis=1)
+    t = p64 + float(np.float32(d.dt)) * spring
+    r2 = ((t[:, None, :] - p64[None, :, :]) ** 2).sum(axis=2)
+    ws = np.maximum(re ** 2 / np.maximum(r2, 1e-5) - 1.0, 0.0)
+    sw = ws.sum(axis=1)
+    u_new = np.where(sw[:, None] > 0, (ws @ u64) / np.where(sw > 0, sw, 1.0)[:, None], u64)
+    x = t * n
+    inside = ((x < d.wall) | (x > n - d.wall)).any(axis=1)
+    margin = np.minimum(np.abs(x - d.wall), np.abs(x - (n - d.wall))).min(axis=1)
+    sure = margin > 1e-4
+    assert np.array_equal(acc[sure], ~inside[sure])
+    assert acc.sum() > 100 and (~acc).sum() > 20
+    assert (w > 0).sum() > 1000   # springs act
+    np.testing.assert_allclose(p_got[acc & sure], t[acc & sure], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(p_got[~acc & sure], pos[~acc & sure], rtol=0, atol=0)
+    np.testing.assert_allclose(u_got[acc & sure], u_new[acc & sure], rtol=1e-4, atol=1e-4)
+    moved = np.abs(t - p64).max(axis=1) > 1e-4
+    assert moved.sum() > 100
     o.close()

@@ -1,6 +1,6 @@
 """Planted mistakes: each copy of the oracle source below carries one plausible
-error (a wrong window, radius, constant, face offset, ghost ratio, sign or
-averaging rule) and must fail the brute-force and closed-form pins of
+error (a wrong window, radius, constant, face offset, ghost ratio, sign,
+averaging rule, weight or accept rule) and must fail the brute-force and closed-form pins of
 test_oracle_pins.py. The copies are compiled on the CPU into a temporary
 folder; the committed oracle is left untouched."""
 import os
@@ -29,6 +29,13 @@ MISTAKES = {
         "if (!(!mark[n][f] && wallmark[n][f])) continue;", "if (!(!mark[n][f])) continue;"),
     "extrapolation mean over one extra face": (
         "if (wsum) s.uf[n][f] = sum / (float)wsum;", "if (wsum) s.uf[n][f] = sum / (float)(wsum + 1);"),
+    "resample spring pulls instead of pushing": (
+        "spring.x += w * (pp.x - np_.x) / dist * re;", "spring.x += w * (np_.x - pp.x) / dist * re;"),
+    "resample radius of a whole cell": (
+        "const float re = s.d.density / maxd;", "const float re = 2.0f * s.d.density / maxd;"),
+    "resample velocity with Smooth weights": (
+        "float w = 1.0f * sharpen(dist2, re);", "float w = 1.0f * smooth(dist2, re);"),
+    "resample accept rule inverted": ("if (!inside_wall(s, s.t[n])) {", "if (inside_wall(s, s.t[n])) {"),
 }
 
 
