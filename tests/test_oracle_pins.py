@@ -17,6 +17,9 @@ oracle bit for bit in test_gpu_parity.py).
 - ExtrapolateVelocity (flip.cpp:452-566): a constant field is a fixed point; faces
   next to a FLUID cell never change; every changed face lies between two
   SOLID-or-outside cells and takes a value inside the range of its neighbours.
+- Divergence (flip.cpp:417-437): -(a + b + c) for the linear field (a x, b y, c z)
+  in every cell, and the discrete divergence theorem on a random field (the
+  per-cell sums telescope to the boundary fluxes).
 - ResampleParticles (particleresampler.inl:35-148): an all-pairs float64 spring
   displacement and Sharpen-weighted velocity (both kernels vanish beyond re = half
   a cell, inside the one-cell search window) and the wall accept rule, on
@@ -268,4 +271,40 @@ def test_resample_is_the_all_pairs_spring_and_sharpen_mean():
     np.testing.assert_allclose(u_got[acc & sure], u_new[acc & sure], rtol=1e-4, atol=1e-4)
     moved = np.abs(t - p64).max(axis=1) > 1e-4
     assert moved.sum() > 100
+    o.close()
+
+
+def test_divergence_linear_field_and_divergence_theorem():
+    n = 10
+    o = OracleSim(scenes.dam_break(n), threads=1)
+    o.init()
+    coef = [0.75, -1.5, 0.25]
+    for comp in range(3):
+        shape = [n, n, n]
+        shape[comp] += 1
+        f = np.meshgrid(*[np.arange(s_) for s_ in shape], indexing="ij")[comp]
+        o.set_field(_abi.FIELD_UX + comp, (np.float32(coef[comp]) * f / np.float32(n)).astype(np.float32).ravel())
+    o.stage(_abi.STAGE_DIVERGENCE)
+    D = o.get_field(_abi.FIELD_D)
+    np.testing.assert_allclose(D, -sum(coef), rtol=1e-5)
+
+    rng = np.random.default_rng(9)
+    u = []
+    for comp in range(3):
+        shape = [n, n, n]
+        shape[comp] += 1
+        u.append(rng.uniform(-1, 1, shape).astype(np.float32))
+        o.set_field(_abi.FIELD_UX + comp, u[-1].ravel())
+    o.stage(_abi.STAGE_DIVERGENCE)
+    D = o.get_field(_abi.FIELD_D).reshape(n, n, n).astype(np.float64)
+    flux = sum(np.take(u[c], n, axis=c).astype(np.float64).sum() - np.take(u[c], 0, axis=c).astype(np.float64).sum()
+               for c in range(3))
+    assert D.sum() == pytest.approx(-n * flux, rel=1e-4, abs=1e-3)
+    # and per plane: the sum over cells i < m telescopes to the flux through x = m
+    for m in (3, 7):
+        part = D[:m].sum()
+        fl = (u[0][m].astype(np.float64).sum() - u[0][0].astype(np.float64).sum()
+              + (u[1][:m, n] - u[1][:m, 0]).astype(np.float64).sum()
+              + (u[2][:m, :, n] - u[2][:m, :, 0]).astype(np.float64).sum())
+        assert part == pytest.approx(-n * fl, rel=1e-4, abs=1e-3)
     o.close()

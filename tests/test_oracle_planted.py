@@ -36,6 +36,9 @@ MISTAKES = {
     "resample velocity with Smooth weights": (
         "float w = 1.0f * sharpen(dist2, re);", "float w = 1.0f * smooth(dist2, re);"),
     "resample accept rule inverted": ("if (!inside_wall(s, s.t[n])) {", "if (inside_wall(s, s.t[n])) {"),
+    "divergence y faces swapped": (
+        "s.uf[1][s.fidx(1, i, j + 1, k)] - s.uf[1][s.fidx(1, i, j, k)] +\n                            s.uf[2][s.fidx(2, i, j, k + 1)] - s.uf[2][s.fidx(2, i, j, k)]) / h;\n                s.D",
+        "s.uf[1][s.fidx(1, i, j, k)] - s.uf[1][s.fidx(1, i, j + 1, k)] +\n                            s.uf[2][s.fidx(2, i, j, k + 1)] - s.uf[2][s.fidx(2, i, j, k)]) / h;\n                s.D"),
 }
 
 
