@@ -20,6 +20,9 @@ oracle bit for bit in test_gpu_parity.py).
 - Divergence (flip.cpp:417-437): -(a + b + c) for the linear field (a x, b y, c z)
   in every cell, and the discrete divergence theorem on a random field (the
   per-cell sums telescope to the boundary fluxes).
+- SolvePicFlip + AdvectParticles (flip.cpp:229-322): with uniform grid and
+  grid-change fields (Interpolate reproduces constants) each particle ends at
+  (1 - r) U + r (u + dU) and moves by dt U, clamped to [1/n, 1 - 1/n].
 - ResampleParticles (particleresampler.inl:35-148): an all-pairs float64 spring
   displacement and Sharpen-weighted velocity (both kernels vanish beyond re = half
   a cell, inside the one-cell search window) and the wall accept rule, on
@@ -307,4 +310,32 @@ def test_divergence_linear_field_and_divergence_theorem():
               + (u[1][:m, n] - u[1][:m, 0]).astype(np.float64).sum()
               + (u[2][:m, :, n] - u[2][:m, :, 0]).astype(np.float64).sum())
         assert part == pytest.approx(-n * fl, rel=1e-4, abs=1e-3)
+    o.close()
+
+
+def test_pic_flip_blend_and_advection_on_uniform_fields():
+    n = 10
+    d = scenes.dam_break(n)
+    o = OracleSim(d, threads=1)
+    o.init()
+    rng = np.random.default_rng(11)
+    m = 400
+    pos = rng.uniform(0.0, 1.0, (m, 3)).astype(np.float32)
+    vel = rng.normal(0.0, 1.0, (m, 3)).astype(np.float32)
+    o.set_particles(pos, vel, np.zeros(m, np.float32))
+    U = np.array([2.0, -3.5, 1.25], np.float32)
+    dU = np.array([0.5, 0.25, -0.75], np.float32)
+    for comp in range(3):
+        size = o.get_field(_abi.FIELD_UX + comp).size
+        o.set_field(_abi.FIELD_UX + comp, np.full(size, U[comp], np.float32))
+        o.set_field(_abi.FIELD_DUX + comp, np.full(size, dU[comp], np.float32))
+    o.stage(_abi.STAGE_ADVECT)
+    p_got, u_got, _ = o.get_particles()
+    r = float(np.float32(d.picflip))
+    u_ref = (1.0 - r) * U.astype(np.float64) + r * (vel.astype(np.float64) + dU)
+    np.testing.assert_allclose(u_got, u_ref, rtol=1e-5, atol=1e-5)
+    lo, hi = 1.0 / n, 1.0 - 1.0 / n
+    p_ref = np.clip(pos.astype(np.float64) + float(np.float32(d.dt)) * U, lo, hi)
+    np.testing.assert_allclose(p_got, p_ref, rtol=0, atol=2e-6)
+    assert (p_ref == lo).any() and (p_ref == hi).any()
     o.close()
