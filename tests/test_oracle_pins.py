@@ -20,6 +20,9 @@ oracle bit for bit in test_gpu_parity.py).
 - Divergence (flip.cpp:417-437): -(a + b + c) for the linear field (a x, b y, c z)
   in every cell, and the discrete divergence theorem on a random field (the
   per-cell sums telescope to the boundary fluxes).
+- EnforceBoundaryVelocity (particlegridoperations.inl:45-104): with zero pressure
+  the gradient stage changes exactly the domain-boundary faces and the faces
+  between a SOLID and a non-SOLID cell, which it sets to zero.
 - SolvePicFlip + AdvectParticles (flip.cpp:229-322): with uniform grid and
   grid-change fields (Interpolate reproduces constants) each particle ends at
   (1 - r) U + r (u + dU) and moves by dt U, clamped to [1/n, 1 - 1/n].
@@ -338,4 +341,40 @@ def test_pic_flip_blend_and_advection_on_uniform_fields():
     p_ref = np.clip(pos.astype(np.float64) + float(np.float32(d.dt)) * U, lo, hi)
     np.testing.assert_allclose(p_got, p_ref, rtol=0, atol=2e-6)
     assert (p_ref == lo).any() and (p_ref == hi).any()
+    o.close()
+
+
+def test_boundary_pass_zeroes_exactly_the_solid_interfaces():
+    n = 8
+    o = OracleSim(scenes.dam_break(n), threads=1)
+    o.init()
+    rng = np.random.default_rng(13)
+    flags = rng.choice([_abi.AIR, _abi.FLUID, _abi.SOLID], size=(n, n, n), p=[0.3, 0.4, 0.3]).astype(np.uint8)
+    o.set_flags(flags.ravel())
+    o.set_field(_abi.FIELD_P, np.zeros(n ** 3, np.float32))
+    o.set_field(_abi.FIELD_L, rng.uniform(-1, 1, n ** 3).astype(np.float32))
+    base = []
+    for comp in range(3):
+        size = o.get_field(_abi.FIELD_UX + comp).size
+        base.append(rng.uniform(0.5, 1.5, size).astype(np.float32))
+        o.set_field(_abi.FIELD_UX + comp, base[-1])
+    o.stage(_abi.STAGE_GRADIENT)
+    solid = flags == _abi.SOLID
+    for comp in range(3):
+        shape = [n, n, n]
+        shape[comp] += 1
+        got = o.get_field(_abi.FIELD_UX + comp).reshape(shape)
+        f = np.stack(np.meshgrid(*[np.arange(s_) for s_ in shape], indexing="ij"), -1)
+        a = f[..., comp]
+        boundary = (a == 0) | (a == n)
+        lo = f.copy()
+        lo[..., comp] -= 1
+        hi = np.clip(f, 0, n - 1)
+        loc = np.clip(lo, 0, n - 1)
+        s_hi = solid[hi[..., 0], hi[..., 1], hi[..., 2]]
+        s_lo = solid[loc[..., 0], loc[..., 1], loc[..., 2]]
+        zero = boundary | (s_hi != s_lo)
+        assert np.all(got[zero] == 0.0)
+        assert np.array_equal(got[~zero], base[comp].reshape(shape)[~zero])
+        assert (~boundary & zero).sum() > 50 and (~zero).sum() > 50
     o.close()

@@ -39,6 +39,9 @@ MISTAKES = {
     "PIC/FLIP ratio on the PIC part": (
         "s.u[i].y = (1.0f - ratio) * pic.y + ratio * t.y;", "s.u[i].y = ratio * pic.y + (1.0f - ratio) * t.y;"),
     "advection clamp of two cells": ("const float r = 1.0f / s.maxd;", "const float r = 2.0f / s.maxd;"),
+    "boundary pass between like cells": (
+        "if (i < N && i > 0 && check_wall(s, i, j, k) * check_wall(s, i - 1, j, k) < 0)",
+        "if (i < N && i > 0 && check_wall(s, i, j, k) * check_wall(s, i - 1, j, k) > 0)"),
     "divergence y faces swapped": (
         "s.uf[1][s.fidx(1, i, j + 1, k)] - s.uf[1][s.fidx(1, i, j, k)] +\n                            s.uf[2][s.fidx(2, i, j, k + 1)] - s.uf[2][s.fidx(2, i, j, k)]) / h;\n                s.D",
         "s.uf[1][s.fidx(1, i, j, k)] - s.uf[1][s.fidx(1, i, j + 1, k)] +\n                            s.uf[2][s.fidx(2, i, j, k + 1)] - s.uf[2][s.fidx(2, i, j, k)]) / h;\n                s.D"),
