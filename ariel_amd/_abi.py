@@ -88,7 +88,7 @@ PROTOTYPES = {
     "ariel_get_frame": (_I32, [_P, ctypes.POINTER(_I32)]),
     "ariel_get_dimensions": (_I32, [_P, _I32 * 3]),
     "ariel_particle_count": (_I32, [_P, ctypes.POINTER(_I64)]),
-    "ariel_get_particles": (_I32, [_P, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),
+    "ariel_get_particles": (_I32, [_P, _P, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),
     "ariel_set_particles": (_I32, [_P, _P, _P, _P, _I64]),
     "ariel_get_macgrid": (_I32, [_P, _I32, _P, _I64]),
     "ariel_set_macgrid": (_I32, [_P, _I32, _P, _I64]),

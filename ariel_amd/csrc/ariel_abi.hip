@@ -756,6 +756,37 @@ float* field_ptr(ariel_sim* h, int which) {
     }
 }
 
+// Max-density calibration (flip.cpp:42-71): a 10^3 lattice at spacing
+// density/maxd, sorted and run through ComputeDensity with m_max_density = 1;
+// the largest density becomes m_max_density.
+int calibrate_max_density(ariel_sim* h) {
+    const float hsp = h->d.density / h->maxd;
+    std::vector<float4> tp(1000), tv(1000, make_float4(0, 0, 0, 0));
+    for (int i = 0, n = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j)
+            for (int k = 0; k < 10; ++k, ++n) {
+                int32_t id = n;
+                float f;
+                std::memcpy(&f, &id, 4);
+                tp[n] = make_float4(((float)i + 0.5f) * hsp, ((float)j + 0.5f) * hsp, ((float)k + 0.5f) * hsp, f);
+            }
+    TRY(ensure_particles(h, 1000));
+    h->cur = 0;
+    h->np = 1000;
+    HIPCHK(h, hipMemcpyAsync(h->pos[0], tp.data(), 1000 * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->vel[0], tv.data(), 1000 * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    TRY(stage_sort(h));
+    h->max_density = 1.0f;
+    TRY(stage_density(h, 0));
+    HIPCHK(h, hipMemcpyAsync(tv.data(), h->vel[h->cur], 1000 * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    float md = 0.0f;
+    for (const float4& v : tv) md = gmax(md, v.w);
+    h->max_density = md;
+    return check_launch(h, "max-density calibration");
+}
+
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -894,31 +925,7 @@ const char* ariel_last_error(const ariel_sim* h) {
 
 int ariel_init(ariel_sim* h) {
     TRY(check_handle(h));
-    // Max-density calibration (flip.cpp:42-71): a 10^3 lattice at spacing
-    // density/maxd, sorted and run through ComputeDensity with m_max_density = 1.
-    const float hsp = h->d.density / h->maxd;
-    std::vector<float4> tp(1000), tv(1000, make_float4(0, 0, 0, 0));
-    for (int i = 0, n = 0; i < 10; ++i)
-        for (int j = 0; j < 10; ++j)
-            for (int k = 0; k < 10; ++k, ++n) {
-                int32_t id = n;
-                float f;
-                std::memcpy(&f, &id, 4);
-                tp[n] = make_float4(((float)i + 0.5f) * hsp, ((float)j + 0.5f) * hsp, ((float)k + 0.5f) * hsp, f);
-            }
-    TRY(ensure_particles(h, 1000));
-    h->cur = 0;
-    h->np = 1000;
-    HIPCHK(h, hipMemcpyAsync(h->pos[0], tp.data(), 1000 * sizeof(float4), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->vel[0], tv.data(), 1000 * sizeof(float4), hipMemcpyHostToDevice, h->stream));
-    TRY(stage_sort(h));
-    h->max_density = 1.0f;
-    TRY(stage_density(h, 0));
-    HIPCHK(h, hipMemcpyAsync(tv.data(), h->vel[h->cur], 1000 * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    float md = 0.0f;
-    for (const float4& v : tv) md = gmax(md, v.w);
-    h->max_density = md;
+    TRY(calibrate_max_density(h));
 
     // Scene::GenerateParticles at frame 0, Sort, MarkCellTypes (flip.cpp:74-76).
     std::vector<float4> pos, vel;
@@ -946,6 +953,7 @@ int ariel_init(ariel_sim* h) {
 int ariel_step(ariel_sim* h, int32_t nsteps) {
     TRY(check_handle(h));
     if (!h->initialized) return fail(h, ARIEL_ERR_STATE, "ariel_step before ariel_init");
+    if (!(h->max_density > 0.0f)) return fail(h, ARIEL_ERR_STATE, "max density not calibrated");
     if (nsteps < 0) return fail(h, ARIEL_ERR_INVALID, "negative step count");
     for (int n = 0; n < nsteps; ++n) {
         h->frame++;
@@ -963,6 +971,8 @@ int ariel_run_stage(ariel_sim* h, int32_t stage) {
     if (!h->initialized && h->np == 0 && stage != ARIEL_STAGE_SOLVE)
         return fail(h, ARIEL_ERR_STATE, "no particles: call ariel_init or ariel_set_particles");
     if (stage < 0 || stage >= ARIEL_NUM_STAGES) return fail(h, ARIEL_ERR_INVALID, "unknown stage");
+    if (stage == ARIEL_STAGE_DENSITY && !(h->max_density > 0.0f))
+        return fail(h, ARIEL_ERR_STATE, "max density not calibrated");
     TRY(run_stage(h, stage));
     TRY(collect_stats(h));
     return ARIEL_OK;
@@ -995,11 +1005,14 @@ int ariel_particle_count(ariel_sim* h, int64_t* n) {
     return ARIEL_OK;
 }
 
-int ariel_get_particles(ariel_sim* h, float* pos, float* vel, float* density, int64_t cap, int64_t* n) {
+int ariel_get_particles(ariel_sim* h, float* pos, float* vel, float* density, uint8_t* type,
+                        int64_t cap, int64_t* n) {
     TRY(check_handle(h));
     if (n) *n = h->np;
-    if (!pos && !vel && !density) return ARIEL_OK;
+    if (!pos && !vel && !density && !type) return ARIEL_OK;
     if (cap < h->np) return fail(h, ARIEL_ERR_CAPACITY, "particle buffer too small");
+    if (type) std::memset(type, ARIEL_FLUID, (size_t)h->np);
+    if (!pos && !vel && !density) return ARIEL_OK;
     if (h->np == 0) return ARIEL_OK;
     const int b = h->cur, o = 1 - b;
     // scatter to id order into the idle buffers
@@ -1021,6 +1034,7 @@ int ariel_set_particles(ariel_sim* h, const float* pos, const float* vel, const 
     TRY(check_handle(h));
     if (n < 0 || (n > 0 && !pos)) return fail(h, ARIEL_ERR_INVALID, "bad particle arrays");
     if (n >= INT32_MAX) return fail(h, ARIEL_ERR_UNSUPPORTED, "too many particles");
+    if (!(h->max_density > 0.0f)) TRY(calibrate_max_density(h));
     TRY(ensure_particles(h, std::max<int64_t>(n, 1)));
     std::vector<float4> hp(n), hv(n);
     for (int64_t i = 0; i < n; ++i) {
@@ -1214,7 +1228,7 @@ int ariel_save_state(ariel_sim* h, const char* path) {
     if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
     std::vector<float> pos(3 * h->np), vel(3 * h->np), rho(h->np);
     int64_t n = 0;
-    TRY(ariel_get_particles(h, pos.data(), vel.data(), rho.data(), h->np, &n));
+    TRY(ariel_get_particles(h, pos.data(), vel.data(), rho.data(), nullptr, h->np, &n));
     FILE* f = std::fopen(path, "wb");
     if (!f) return fail(h, ARIEL_ERR_IO, std::string("cannot open ") + path);
     bool ok = std::fwrite(kMagic, 1, 8, f) == 8;
@@ -1352,7 +1366,7 @@ int ariel_export_particles(ariel_sim* h, const char* path) {
         return fail(h, ARIEL_ERR_UNSUPPORTED, "particle export: only .geo (Houdini classic ASCII) is built");
     std::vector<float> pos(3 * h->np), vel(3 * h->np);
     int64_t n = 0;
-    TRY(ariel_get_particles(h, pos.data(), vel.data(), nullptr, h->np, &n));
+    TRY(ariel_get_particles(h, pos.data(), vel.data(), nullptr, nullptr, h->np, &n));
     for (float& x : pos) x = x * h->maxd;
     if (!write_geo(path, n, pos.data(), vel.data())) return fail(h, ARIEL_ERR_IO, std::string("cannot write ") + path);
     return ARIEL_OK;
@@ -1415,9 +1429,20 @@ int ariel_set_learned_weights(ariel_sim* h, const void* blob, int64_t bytes) {
             pos[i] = total;
             total += (sizes[i] + 127) / 128 * 128;
         }
-        TRY(dalloc(h, &h->net_mem, total));
-        TRY(dalloc(h, &h->R_old, h->nc));
-        TRY(dalloc(h, &h->net.wd, h->nc));
+        // all three buffers or none: a failure leaves the handle without them
+        uint16_t* mem = nullptr;
+        float *r_old = nullptr, *wd = nullptr;
+        int rc = dalloc(h, &mem, total);
+        if (rc == ARIEL_OK) rc = dalloc(h, &r_old, h->nc);
+        if (rc == ARIEL_OK) rc = dalloc(h, &wd, h->nc);
+        if (rc != ARIEL_OK) {
+            for (void* q : {(void*)mem, (void*)r_old, (void*)wd})
+                if (q) (void)hipFree(q);
+            return rc;
+        }
+        h->net_mem = mem;
+        h->R_old = r_old;
+        h->net.wd = wd;
         uint16_t* m = h->net_mem;
         h->net.weights = m + pos[0];
         h->net.e0 = m + pos[1];

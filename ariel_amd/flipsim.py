@@ -74,16 +74,24 @@ class FlipSim:
     def Step(self, nsteps=1):
         self._check(self._lib.ariel_step(self._h, nsteps))
 
-    def GetParticles(self):
-        """(pos[n,3], vel[n,3], density[n]) in particle-vector (id) order."""
+    def GetParticles(self, types=False):
+        """(pos[n,3], vel[n,3], density[n]) in particle-vector (id) order; with
+        types=True also Particle::m_type (macgrid.inl:36-42) as uint8[n]."""
         n = self.num_particles
         pos = np.empty((n, 3), np.float32)
         vel = np.empty((n, 3), np.float32)
         rho = np.empty(n, np.float32)
+        typ = np.empty(n, np.uint8) if types else None
         cnt = ctypes.c_int64()
-        self._check(self._lib.ariel_get_particles(self._h, _p(pos), _p(vel), _p(rho), n,
+        self._check(self._lib.ariel_get_particles(self._h, _p(pos), _p(vel), _p(rho),
+                                                  _p(typ) if types else None, n,
                                                   ctypes.byref(cnt)))
-        return pos, vel, rho
+        return (pos, vel, rho, typ) if types else (pos, vel, rho)
+
+    def GetScene(self):
+        """FlipSim::GetScene (flip.hpp:31): the scene description this handle was
+        created from (the descriptor replaces sceneCore::Scene)."""
+        return self.desc
 
     def GetDimensions(self):
         d = (ctypes.c_int32 * 3)()

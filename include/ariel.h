@@ -222,12 +222,18 @@ int ariel_particle_count(ariel_sim* h, int64_t* n);
 
 /* FlipSim::GetParticles (src/sim/flip.cpp:743-745): copies particles out in
  * particle-vector order (particle id order). pos/vel are n*3 floats, density
- * n floats; any pointer may be NULL. cap is the capacity in particles. */
+ * n floats, type n bytes (Particle::m_type, src/grid/macgrid.inl:36-42: every
+ * particle of this build is ARIEL_FLUID, the reference's solid particles belong
+ * to the solid geometry it does not run); any pointer may be NULL. cap is the
+ * capacity in particles. */
 int ariel_get_particles(ariel_sim* h, float* pos, float* vel, float* density,
-                        int64_t cap, int64_t* n);
+                        uint8_t* type, int64_t cap, int64_t* n);
 
 /* Replace all particles (id = index). Used to load a state; the particles are
- * unsorted afterwards, so the next stage to run must be ARIEL_STAGE_SORT. */
+ * unsorted afterwards, so the next stage to run must be ARIEL_STAGE_SORT. On a
+ * handle that was never initialised, the max-density calibration of
+ * FlipSim::Init (src/sim/flip.cpp:42-71) runs first, so that ComputeDensity
+ * divides by the reference's m_max_density. */
 int ariel_set_particles(ariel_sim* h, const float* pos, const float* vel,
                         const float* density, int64_t n);
 

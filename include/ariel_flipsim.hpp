@@ -34,11 +34,12 @@ private:
 };
 
 // One particle as the reference's Particle exposes it (macgrid.inl:36-50):
-// position, velocity, density; type is always FLUID here.
+// position, velocity, density and type (always ARIEL_FLUID here).
 struct Particle {
     float p[3];
     float u[3];
     float density;
+    int type;
 };
 
 struct Dimensions {
@@ -49,7 +50,7 @@ class FlipSim {
 public:
     // FlipSim(maxres, density, stepsize, scene, verbose) (flip.cpp:15-30):
     // dims, density and dt live in the descriptor.
-    explicit FlipSim(const ariel_scene_desc& desc) : m_frame(0) {
+    explicit FlipSim(const ariel_scene_desc& desc) : m_frame(0), desc_(desc) {
         check(ariel_create(&desc, &h_), nullptr);
     }
     ~FlipSim() { ariel_destroy(h_); }
@@ -83,7 +84,8 @@ public:
         int64_t n = 0;
         check(ariel_particle_count(h_, &n), h_);
         std::vector<float> pos(3 * n), vel(3 * n), rho(n);
-        check(ariel_get_particles(h_, pos.data(), vel.data(), rho.data(), n, &n), h_);
+        std::vector<uint8_t> type(n);
+        check(ariel_get_particles(h_, pos.data(), vel.data(), rho.data(), type.data(), n, &n), h_);
         std::vector<Particle> out((size_t)n);
         for (int64_t i = 0; i < n; ++i) {
             for (int c = 0; c < 3; ++c) {
@@ -91,9 +93,14 @@ public:
                 out[i].u[c] = vel[3 * i + c];
             }
             out[i].density = rho[i];
+            out[i].type = type[i];
         }
         return out;
     }
+
+    // FlipSim::GetScene (flip.hpp:31): the scene the simulation was made from,
+    // here its descriptor.
+    const ariel_scene_desc& GetScene() const { return desc_; }
 
     Dimensions GetDimensions() {
         int32_t d[3];
@@ -130,6 +137,7 @@ private:
     static void check(int rc, const ariel_sim* h) {
         if (rc != ARIEL_OK) throw Error(rc, ariel_last_error(h));
     }
+    ariel_scene_desc desc_;
     ariel_sim* h_ = nullptr;
 };
 

@@ -62,8 +62,10 @@ inline float smooth(float r2, float h) { return gmax(1.0f - r2 / (h * h), 0.0f);
 inline float sharpen(float r2, float h) { return gmax(h * h / gmax(r2, 1.0e-5f) - 1.0f, 0.0f); }
 
 // ---------------------------------------------------------------- hash (E2/E6)
-// 32-bit integer finaliser ("lowbias32", C. Wellons' hash-prospector). Parity
-// of the product's copy is pinned by the fixed vectors in tests/golden.
+// 32-bit integer finaliser ("lowbias32", C. Wellons' hash-prospector), chained
+// over four words. Pinned by tests/golden/hash4_vectors.json (written by an
+// independent Python restatement) and by the published inverse of lowbias32
+// (tests/test_oracle_pins.py).
 inline uint32_t mix32(uint32_t x) {
     x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
     return x;
@@ -1096,7 +1098,9 @@ bool inside_wall(const oracle_sim& s, V3 t) {
 
 // ResampleParticles + Resample (particleresampler.inl:35-148). The rand() of the
 // coincident-neighbour branch is replaced by hash4(seed, 3*frame+axis, id_p, id_q)
-// (E6). parity unpinned for that branch (the reference's is nondeterministic).
+// (E6). The reference's draws are unseeded and nondeterministic, so no run of it
+// can be matched there; the replacement is pinned by isolated coincident groups
+// whose displacement is recomputed from an independent hash.
 void resample(oracle_sim& s) {
     const float maxd = s.maxd;
     const float re = s.d.density / maxd;   // flip.cpp:120
@@ -1423,6 +1427,7 @@ void oracle_get_stats(const oracle_sim* s, oracle_stats* st) {
     st->max_density = s->max_density;
 }
 int oracle_get_frame(const oracle_sim* s) { return s->frame; }
+void oracle_set_frame(oracle_sim* s, int frame) { s->frame = frame; }
 
 void oracle_apply_A(const oracle_sim* s, const float* x, float* y) {
     compute_ax(*s, x, y, *s->pool);

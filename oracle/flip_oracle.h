@@ -101,6 +101,8 @@ float  oracle_get_max_density(const oracle_sim* s);
 void   oracle_set_max_density(oracle_sim* s, float v);
 void   oracle_get_stats(const oracle_sim* s, oracle_stats* st);
 int    oracle_get_frame(const oracle_sim* s);
+/* FlipSim::m_frame for stage-wise runs (Step increments it before its stages). */
+void   oracle_set_frame(oracle_sim* s, int frame);
 
 /* Reference operator y = A x on the current A/L (solver.inl:196-230). */
 void   oracle_apply_A(const oracle_sim* s, const float* x, float* y);

@@ -54,6 +54,7 @@ def load():
         "oracle_get_sort": (None, [V, V, V]), "oracle_get_accept": (None, [V, V]),
         "oracle_get_max_density": (F, [V]), "oracle_set_max_density": (None, [V, F]),
         "oracle_get_stats": (None, [V, V]), "oracle_get_frame": (I, [V]),
+        "oracle_set_frame": (None, [V, I]),
         "oracle_apply_A": (None, [V, V, V]), "oracle_apply_A_f64": (None, [V, V, V]), "oracle_apply_precond": (None, [V, V, V]),
         "oracle_get_precond": (None, [V, V]),
         "oracle_solve_fp64": (I, [V, D, I, V, V]),
@@ -159,6 +160,14 @@ class OracleSim:
     @max_density.setter
     def max_density(self, v):
         self.lib.oracle_set_max_density(self.h, v)
+
+    @property
+    def frame(self):
+        return int(self.lib.oracle_get_frame(self.h))
+
+    @frame.setter
+    def frame(self, v):
+        self.lib.oracle_set_frame(self.h, int(v))
 
     def stats(self):
         st = OracleStats()

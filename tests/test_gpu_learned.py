@@ -11,6 +11,8 @@ import torch
 
 from ariel_amd import FlipSim, _abi, learned, scenes
 from ariel_amd.learned import net as lnet
+from oracle import OracleSim
+from tests import precision
 
 pytestmark = pytest.mark.gpu
 
@@ -42,7 +44,7 @@ def _shipped_or_random():
     return lnet.PressureNet(seed=11).blob()
 
 
-@pytest.mark.parametrize("n", [32, 48, 64])
+@pytest.mark.parametrize("n", [32, 48, 64, 128])
 def test_network_matches_torch(n):
     blob = lnet.PressureNet(seed=7).blob()
     _say(f"learned forward n={n}")
@@ -64,8 +66,11 @@ def test_network_matches_torch(n):
     rel = np.linalg.norm(z - ref) / np.linalg.norm(ref)
     mx = np.abs(z - ref).max() / np.abs(ref).max()
     _say(f"  rel L2 {rel:.3e} max {mx:.3e}")
-    # BF16 storage of every activation; accumulation order differs from torch's
-    assert rel < 1e-2 and mx < 5e-2
+    # BF16 storage of every activation; accumulation order differs from torch's,
+    # so an activation can round to the neighbouring BF16 value (2^-8 relative).
+    # Measured on MI355X: rel L2 4.5e-5 to 2.9e-3, max 4.6e-4 to 6.8e-3 (32^3 to
+    # 128^3); the bounds leave a factor of about two.
+    assert rel < 5e-3 and mx < 1.5e-2
 
 
 def _solve_pair(n, blob, seed):
@@ -158,3 +163,32 @@ def test_learned_needs_weights_and_divisible_grid():
     with pytest.raises(Exception):
         sim.Step(1)
     sim.close()
+
+
+@pytest.mark.parametrize("n,steps", [(128, 2), (256, 1)])
+def test_learned_steps_against_fp64_oracle(n, steps):
+    """The learned-preconditioner workload at its sizes: steps of the 128^3 dam
+    break + sphere and the 256^3 pool + cube with flexible PCG on the shipped
+    network; each step's pressure against the FP64 oracle's solve of the same
+    right-hand side (cell classes, level set, divergence of the step)."""
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        pytest.skip("no shipped weights")
+    d = (scenes.dam_break_sphere(n, precond=_abi.PRECOND_LEARNED) if n == 128 else
+         scenes.pool_and_cube(n, precond=_abi.PRECOND_LEARNED))
+    g = FlipSim(d)
+    g.Init()
+    chk = OracleSim(d, threads=16)
+    for step in range(steps):
+        g.Step(1)
+        st = g.stats()
+        assert st.solve_status == 0
+        assert st.residual_norm <= 1e-6 * st.rhs_norm * (1 + 1e-6)
+        e = precision.errors(chk, g.get_flags(), g.get_field(_abi.FIELD_L),
+                             g.get_field(_abi.FIELD_D), g.get_field(_abi.FIELD_P))
+        _say(f"learned n={n} step {step + 1}: {st.iterations} it (fallback {st.precond_fallback}); "
+             f"pressure rel error {e['pressure_rel_error']:.2e}, velocity {e['velocity_rel_error']:.2e}")
+        assert e["fp64_rel_residual"] <= 1e-10
+        assert e["pressure_rel_error"] < 1e-5
+        assert e["velocity_rel_error"] < 1e-4
+    g.close()
+    chk.close()

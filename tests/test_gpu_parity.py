@@ -149,14 +149,15 @@ def test_free_running_steps_track_oracle():
 
 
 def test_dam_break_sphere_128_first_steps():
-    """The 128^3 dam break + sphere, free running: step 1's cell classes and
-    per-cell particle counts are bit-exact (every stage before them is); later
-    steps see pressures that differ in the last bits (FP64 reduction order), so
-    a few particles near cell faces may land differently. Particle totals are
-    conserved exactly."""
+    """The 128^3 dam break + sphere, free running for steps 1-5: cell classes and
+    per-cell particle counts identical to the oracle's on every step, and the
+    particle totals equal. Every stage up to the solve is bit-exact; the pressure
+    agrees to its last bits only (FP64 reduction order), so a particle that sat
+    within an ulp of a cell face after advection could land in another cell. For
+    this seeded scene none does on steps 1-5 (the measured differences are zero),
+    and the test holds it to that."""
     d = scenes.dam_break_sphere(128)
     o, g = _pair(d)
-    n3 = 128 ** 3
     for step in range(5):
         o.step()
         g.Step(1)
@@ -165,6 +166,5 @@ def test_dam_break_sphere_128_first_steps():
         flag_mism = int((g.get_flags() != o.get_flags()).sum())
         count_mism = int((np.diff(g.get_sort()[1]) != np.diff(o.get_sort()[1])).sum())
         print(f"step {step + 1}: {flag_mism} cell classes, {count_mism} cell counts differ", flush=True)
-        if step == 0:
-            assert flag_mism == 0
-        assert flag_mism <= 1e-4 * n3
+        assert flag_mism == 0, f"step {step + 1}: {flag_mism} cell classes differ"
+        assert count_mism == 0, f"step {step + 1}: {count_mism} per-cell counts differ"

@@ -97,3 +97,31 @@ def test_boundary_pass_on_random_cell_classes():
     _gradient_case(g, o, n, L, np.zeros(n ** 3, np.float32), flags, 7)
     g.close()
     o.close()
+
+
+def test_coincident_particles_through_the_resampler():
+    """The resampler's coincident-pair branch (the hash in place of rand()) on the
+    isolated coincident groups of the oracle's pin, plus a random cloud around
+    them: positions, velocities and the accept mask bit for bit."""
+    from tests.test_oracle_pins import coincident_groups
+    n = 16
+    d = scenes.dam_break(n, seed=9)
+    g, o = FlipSim(d), OracleSim(d)
+    g.Init()
+    o.init()
+    pos, _ = coincident_groups(n)
+    rng = np.random.default_rng(21)
+    cloud = rng.uniform(0.1, 0.9, (600, 3)).astype(np.float32)
+    pos = np.concatenate([pos, cloud, cloud[:40]])   # 40 more exact duplicates
+    vel = rng.normal(0.0, 1.0, pos.shape).astype(np.float32)
+    for sim in (g, o):
+        sim.set_particles(pos, vel, np.zeros(len(pos), np.float32))
+    print("coincident resample", len(pos), flush=True)
+    g.run_stage(_abi.STAGE_RESAMPLE)
+    o.stage(_abi.STAGE_RESAMPLE)
+    for a, b, what in zip(g.GetParticles(), o.get_particles(), ("pos", "vel", "rho")):
+        assert_bitexact(a, b, f"resample {what}")
+    assert_bitexact(g.get_accept(), o.get_accept(), "accept mask")
+    assert np.abs(g.GetParticles()[0][:60] - pos[:60]).max() > 1e-5   # the groups moved
+    g.close()
+    o.close()

@@ -30,7 +30,20 @@ oracle bit for bit in test_gpu_parity.py).
   displacement and Sharpen-weighted velocity (both kernels vanish beyond re = half
   a cell, inside the one-cell search window) and the wall accept rule, on
   particles with no pair close enough to take the coincident-pair branch.
+- The counter-based hash that replaces rand() and jitters the seeding lattice:
+  the fixed vectors of tests/golden/hash4_vectors.json (written by an independent
+  Python lowbias32, tests/golden/make_hash_vectors.py), and the published inverse
+  of lowbias32, which must return the seed from every hash4 output.
+- Seeding jitter (E2): every seeded position equals the lattice point plus the
+  jitter drawn from the independent hash, in float32, in the E10 order.
+- The resampler's coincident-pair branch (particleresampler.inl:63-77, rand()
+  replaced by the hash): isolated groups of exactly coincident particles move by
+  dt * sum over partners of 0.01 re / dt * (h % 101) / 100 per axis, with h from
+  the independent hash keyed (seed, 3 frame + axis, id_p, id_q).
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -38,6 +51,32 @@ from ariel_amd import _abi, scenes
 from oracle import OracleSim
 
 N = 12
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+M32 = 0xFFFFFFFF
+
+
+def _lowbias32(x):
+    """C. Wellons' lowbias32 finaliser (hash prospector), independent of the oracle."""
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & M32
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & M32
+    x ^= x >> 16
+    return x
+
+
+def _lowbias32_inverse(x):
+    """The published inverse of lowbias32."""
+    x ^= x >> 16
+    x = (x * 0x43021123) & M32
+    x ^= (x >> 15) ^ (x >> 30)
+    x = (x * 0x1D69E2A5) & M32
+    x ^= x >> 16
+    return x
+
+
+def _hash4(seed, a, b, c):
+    return _lowbias32(_lowbias32(_lowbias32(_lowbias32(seed) ^ a) ^ b) ^ c)
 
 
 def _cells(pos, n):
@@ -377,4 +416,104 @@ def test_boundary_pass_zeroes_exactly_the_solid_interfaces():
         assert np.all(got[zero] == 0.0)
         assert np.array_equal(got[~zero], base[comp].reshape(shape)[~zero])
         assert (~boundary & zero).sum() > 50 and (~zero).sum() > 50
+    o.close()
+
+
+def test_hash_fixed_vectors_and_inverse():
+    """hash4 against the committed vectors and against the inverse of lowbias32:
+    peeling c, b, a off an output with the published inverse must return the seed,
+    which fails for a wrong constant, a dropped round or a swapped key order."""
+    from oracle import oracle as ora
+    lib = ora.load()
+    with open(os.path.join(GOLDEN, "hash4_vectors.json")) as f:
+        vec = json.load(f)["vectors"]
+    assert len(vec) > 150
+    for v in vec:
+        s_, a, b, c = v["in"]
+        assert lib.oracle_hash4(s_, a, b, c) == v["out"], v
+        assert _hash4(s_, a, b, c) == v["out"]
+    rng = np.random.default_rng(11)
+    for s_, a, b, c in rng.integers(0, 2 ** 32, (2000, 4), dtype=np.uint64).tolist():
+        h = lib.oracle_hash4(s_, a, b, c)
+        x = _lowbias32_inverse(h) ^ c
+        x = _lowbias32_inverse(x) ^ b
+        x = _lowbias32_inverse(x) ^ a
+        assert _lowbias32_inverse(x) == s_
+
+
+def test_seeding_jitter_is_the_hash_draw():
+    """E2: lattice point a*w + u*w with u = (hash4(seed, 3l + axis, a*N*m + b, c) >> 8)
+    * 2^-24 per axis, every interior lattice point inside the liquid kept, in the
+    E10 order (cells i, j, k ascending, octants lexicographic)."""
+    n = 8
+    d = scenes.dam_break(n, seed=5)
+    o = OracleSim(d, threads=1)
+    o.init()
+    pos, _, _ = o.get_particles()
+    m = int(round(1.0 / d.density))
+    w = np.float32(d.density) * np.float32(np.float32(1.0) / np.float32(n))
+    lq = d.liquids[0]
+    lo, hi = np.array(lq.lo[:], np.float32), np.array(lq.hi[:], np.float32)
+    want = []
+    for ci in range(d.wall, n - d.wall):
+        for cj in range(d.wall, n - d.wall):
+            for ck in range(d.wall, n - d.wall):
+                for oa in range(m):
+                    for ob in range(m):
+                        for oc in range(m):
+                            a, b, c = ci * m + oa, cj * m + ob, ck * m + oc
+                            key = (a * n * m + b) & M32
+                            u = [np.float32(_hash4(d.seed, ax, key, c) >> 8) * np.float32(2.0 ** -24)
+                                 for ax in range(3)]
+                            x = np.array([np.float32(a) * w + u[0] * w, np.float32(b) * w + u[1] * w,
+                                          np.float32(c) * w + u[2] * w], np.float32)
+                            if np.all(x >= lo) and np.all(x < hi):
+                                want.append(x)
+    want = np.array(want, np.float32)
+    assert want.shape == pos.shape and len(pos) > 100
+    assert np.array_equal(pos, want)
+    o.close()
+
+
+def coincident_groups(n, seed=4):
+    """Isolated groups of 2 or 3 exactly coincident particles, 3 cells apart, in a
+    shuffled id order; returns positions and the group of each particle."""
+    rng = np.random.default_rng(seed)
+    sites = [(i, j, k) for i in range(2, n - 3, 3) for j in range(2, n - 3, 3) for k in range(2, n - 3, 3)]
+    pos, group = [], []
+    for g, (i, j, k) in enumerate(sites):
+        x = (np.array([i, j, k]) + rng.uniform(0.2, 0.8, 3)) / n
+        for _ in range(2 + (g % 2)):
+            pos.append(x.astype(np.float32))
+            group.append(g)
+    perm = rng.permutation(len(pos))
+    return np.array(pos, np.float32)[perm], np.array(group)[perm]
+
+
+def test_resample_coincident_pairs_move_by_the_hash_jitter():
+    n = 16
+    d = scenes.dam_break(n, seed=9)
+    o = OracleSim(d, threads=1)
+    o.init()
+    pos, group = coincident_groups(n)
+    vel = np.zeros_like(pos)
+    o.set_particles(pos, vel, np.zeros(len(pos), np.float32))
+    o.stage(_abi.STAGE_RESAMPLE)
+    got, _, _ = o.get_particles()
+    frame = o.stats().frame
+    re = float(np.float32(d.density) / np.float32(n))
+    dt = float(np.float32(d.dt))
+    disp = np.zeros((len(pos), 3))
+    for p in range(len(pos)):
+        for q in np.flatnonzero(group == group[p]):
+            if q == p:
+                continue
+            for ax in range(3):
+                h = _hash4(d.seed, 3 * frame + ax, p, int(q))
+                disp[p, ax] += dt * (0.01 * re / dt * (h % 101) / 100.0)
+    assert np.all(o.get_accept() == 1)
+    moved = got.astype(np.float64) - pos.astype(np.float64)
+    # positions near 0.5 carry float32 steps of 6e-8; one step of h % 101 moves 3e-6
+    np.testing.assert_allclose(moved, disp, rtol=0, atol=2.5e-7)
+    assert (np.abs(disp) > 1e-5).sum() > 0.9 * disp.size
     o.close()

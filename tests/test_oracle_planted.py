@@ -1,7 +1,7 @@
 """Planted mistakes: each copy of the oracle source below carries one plausible
 error (a wrong window, radius, constant, face offset, ghost ratio, sign,
-averaging rule, weight or accept rule) and must fail the brute-force and closed-form pins of
-test_oracle_pins.py. The copies are compiled on the CPU into a temporary
+averaging rule, weight, accept rule, hash key or jitter scale) and must fail the brute-force and closed-form pins of
+test_oracle_pins.py (and the reference solver's direct-solve pin). The copies are compiled on the CPU into a temporary
 folder; the committed oracle is left untouched."""
 import os
 import subprocess
@@ -45,6 +45,17 @@ MISTAKES = {
     "divergence y faces swapped": (
         "s.uf[1][s.fidx(1, i, j + 1, k)] - s.uf[1][s.fidx(1, i, j, k)] +\n                            s.uf[2][s.fidx(2, i, j, k + 1)] - s.uf[2][s.fidx(2, i, j, k)]) / h;\n                s.D",
         "s.uf[1][s.fidx(1, i, j, k)] - s.uf[1][s.fidx(1, i, j + 1, k)] +\n                            s.uf[2][s.fidx(2, i, j, k + 1)] - s.uf[2][s.fidx(2, i, j, k)]) / h;\n                s.D"),
+    "coincident-pair jitter scale 0.05": ("spring.x += 0.01f * re / dt", "spring.x += 0.05f * re / dt"),
+    "coincident-pair hash keys swapped": (
+        "hash4(seed, 3u * fr + 0u, (uint32_t)n0, (uint32_t)q)", "hash4(seed, 3u * fr + 0u, (uint32_t)q, (uint32_t)n0)"),
+    "hash key words mixed in the wrong order": (
+        "return mix32(mix32(mix32(mix32(seed) ^ a) ^ b) ^ c);", "return mix32(mix32(mix32(mix32(seed) ^ b) ^ a) ^ c);"),
+    "hash multiplier off by two": ("x *= 0x7feb352dU;", "x *= 0x7feb352bU;"),
+    "seeding jitter axes swapped": (
+        "jx = hash_unit(hash4(s.d.seed, 3u * l + 0u, key, c));", "jx = hash_unit(hash4(s.d.seed, 3u * l + 1u, key, c));"),
+    "reference solver stop ten times looser": (
+        "float eps = 1.0e-2f * (float)(N * N * N);", "float eps = 1.0e-1f * (float)(N * N * N);"),
+    "reference solver beta inverted": ("float beta = a2 / a;", "float beta = a / a2;"),
 }
 
 
@@ -56,7 +67,9 @@ def _run_pins(tmp_path, text):
                     "-I", os.path.join(ROOT, "oracle"), str(src), "-o", str(lib)], check=True)
     env = dict(os.environ, ARIEL_ORACLE_LIB=str(lib))
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
-                        os.path.join(ROOT, "tests", "test_oracle_pins.py")],
+                        os.path.join(ROOT, "tests", "test_oracle_pins.py"),
+                        os.path.join(ROOT, "tests", "test_oracle_solver.py") +
+                        "::test_faithful_reference_solver_matches_direct_solve"],
                        cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
     return r.returncode, r.stdout[-2000:]
 

@@ -201,6 +201,39 @@ def test_faithful_reference_solver_terminates():
     assert st.solve_status == 0 and 0 < st.iterations < 16 ** 3
 
 
+def test_faithful_reference_solver_matches_direct_solve():
+    """SolveConjugateGradient verbatim (solver.inl:289-354): it stops at the first
+    iteration whose FP32 r.r falls to 1e-2 N^3 (absolute). With a right-hand side
+    large enough that this is a tight relative bound, its pressure must match
+    scipy's direct solve of the assembled operator, and the true residual of the
+    result must meet the same absolute bound up to FP32 drift."""
+    o, d = stepped_oracle(n=16, precond=_abi.PRECOND_REFERENCE)
+    n = o.n
+    flags, L = o.get_flags(), o.get_field(_abi.FIELD_L)
+    m = np.flatnonzero(flags == FLUID)
+    rng = np.random.default_rng(8)
+    b = np.zeros(n ** 3, np.float32)
+    b[m] = (rng.standard_normal(m.size) * 1e4).astype(np.float32)
+    o.set_field(_abi.FIELD_D, b)
+    o.set_field(_abi.FIELD_P, np.zeros(n ** 3, np.float32))
+    o.stage(_abi.STAGE_SOLVE, mode=1)
+    st = o.stats()
+    eps = 1e-2 * n ** 3
+    assert st.solve_status == 0 and 1 < st.iterations < n ** 3
+    assert st.residual_norm ** 2 <= eps
+    A = assemble(flags, L, n)[m][:, m].tocsc()
+    x = spla.spsolve(A, b[m].astype(np.float64))
+    p = o.get_field(_abi.FIELD_P).astype(np.float64)
+    r_true = b[m] - A @ p[m]
+    assert (r_true ** 2).sum() <= 1.1 * eps
+    bound = np.sqrt(1.1 * eps) / float(np.linalg.norm(b[m]))   # relative residual reached
+    assert bound < 1e-4
+    # error <= cond * residual; measured 1.4e-4 against a residual bound of 2.5e-5
+    assert np.linalg.norm(p[m] - x) <= 20.0 * bound * np.linalg.norm(x)
+    assert np.all(p[np.flatnonzero(flags != FLUID)] == 0.0)
+    o.close()
+
+
 def test_precision_measure_on_an_oracle_step():
     """The FP64 error measurement pinned on the oracle's own FP32 step: the FP32
     pressure is within the solver tolerance's reach of the FP64 solution, the
