@@ -104,6 +104,7 @@ PROTOTYPES = {
     "ariel_set_profiling": (_I32, [_P, _I32]),
     "ariel_reset_kernel_stats": (_I32, [_P]),
     "ariel_get_kernel_stats": (_I32, [_P, _I32, ctypes.POINTER(KernelStats)]),
+    "ariel_export_levelset_noalloc": (_I32, [_P, _P, _I64]),
     "ariel_save_state": (_I32, [_P, ctypes.c_char_p]),
     "ariel_load_state": (_I32, [_P, ctypes.c_char_p]),
     "ariel_export_levelset": (_I32, [_P, ctypes.c_char_p]),

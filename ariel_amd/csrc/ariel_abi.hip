@@ -1347,6 +1347,20 @@ int ariel_export_levelset(ariel_sim* h, const char* path) {
     return ARIEL_OK;
 }
 
+int ariel_export_levelset_noalloc(ariel_sim* h, float* phi, int64_t cap) {
+    TRY(check_handle(h));
+    if (!phi) return fail(h, ARIEL_ERR_INVALID, "null level-set buffer");
+    const int64_t M = h->N + 1;
+    if (cap < M * M * M) return fail(h, ARIEL_ERR_CAPACITY, "level-set buffer too small");
+    // the grid and particle limits are checked before anything is queued
+    std::string err;
+    const hipError_t e = noalloc_levelset(h->stream, h->device, h->pos[h->cur], h->np, h->N, h->maxd,
+                                          kSdfBand, kSdfRadius, phi, err);
+    if (e == hipErrorInvalidValue && !err.empty()) return fail(h, ARIEL_ERR_CAPACITY, err);
+    if (e != hipSuccess) return fail(h, ARIEL_ERR_HIP, err.empty() ? hipGetErrorString(e) : err);
+    return ARIEL_OK;
+}
+
 int ariel_export_mesh(ariel_sim* h, const char* path) {
     TRY(check_handle(h));
     if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");

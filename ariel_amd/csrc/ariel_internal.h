@@ -216,6 +216,11 @@ void launch_particle_sdf(hipStream_t s, const float4* pos, int64_t np, int N, fl
 // units) and quads (4 vertex indices each, normals out of the negative side).
 void surface_nets(const float* phi, int M, std::vector<float>& verts, std::vector<int32_t>& quads);
 bool write_obj(const char* path, const std::vector<float>& verts, const std::vector<int32_t>& quads);
+// The same level set with its workspace in a fixed device array (noalloc.hip):
+// no device allocation. Fails (err set, hipErrorInvalidValue) beyond the
+// array's limits; device = the current device.
+hipError_t noalloc_levelset(hipStream_t s, int device, const float4* pos, int64_t np, int N,
+                            float maxd, float band, float rad, float* phi, std::string& err);
 bool write_geo(const char* path, int64_t n, const float* pos, const float* vel);
 void launch_fill_f32(hipStream_t s, float* p, int64_t n, float v);
 

@@ -292,6 +292,22 @@ int ariel_load_state(ariel_sim* h, const char* path);
  * volume (x-major) after a text header; no OpenVDB writer in this build. */
 int ariel_export_levelset(ariel_sim* h, const char* path);
 
+/* Allocation-free variant of ariel_export_levelset (no counterpart in the
+ * reference; it is for callers that cannot allocate device memory: code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). The same
+ * level set, bit for bit, written into the caller's DEVICE buffer phi of cap
+ * floats (at least (N+1)^3, x-major) instead of a file, queued on the handle's
+ * stream without synchronising (ariel_sync(h) waits for it). Its workspace is a
+ * fixed device array of about 26 MiB that the library holds on every device it
+ * runs on until the process ends; it calls no hipMalloc/hipFree. Limits: grid
+ * edge <= ARIEL_NOALLOC_MAX_N and particles <= ARIEL_NOALLOC_MAX_PARTICLES,
+ * beyond which it returns ARIEL_ERR_CAPACITY. Every call on one device shares
+ * that workspace: calls that overlap on one device (other streams, other host
+ * threads) race and must be serialised by the caller. */
+#define ARIEL_NOALLOC_MAX_N         64
+#define ARIEL_NOALLOC_MAX_PARTICLES 1048576
+int ariel_export_levelset_noalloc(ariel_sim* h, float* phi, int64_t cap);
+
 /* Scene::ExportParticles OBJ path (scene.cpp:93-99, LevelSet::WriteObjToFile,
  * levelset.cpp:113-173): the zero surface of the same level set as a dual
  * quad mesh (one vertex per voxel the surface crosses, one quad per crossed
