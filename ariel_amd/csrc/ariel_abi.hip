@@ -3,126 +3,14 @@
 // (src/sim/flip.cpp:42-128) as a sequence of HIP kernel launches on the
 // handle's stream. Host work per step is launch-only apart from one small
 // device->host read of the solver scalars per chunk of PCG iterations.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <thread>
-#include <vector>
-
-#include "../../include/ariel.h"
-#include "ariel_device.h"
-#include "ariel_internal.h"
+#include "ariel_handle.h"
 
 using namespace ariel;
+using namespace ariel::host;
 
-namespace {
-
-thread_local std::string g_last_error = "";
-
-struct Stage {
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    bool recorded = false;
-};
-
-struct KTimer {
-    int kernel;
-    hipEvent_t e0, e1;
-};
-
-}  // namespace
+thread_local std::string ariel::host::g_last_error = "";
 
 void ariel::set_last_error(const std::string& msg) { g_last_error = msg; }
-
-struct ariel_sim {
-    ariel_scene_desc d{};
-    int N = 0;
-    int64_t nc = 0, nf = 0, np = 0;
-    float maxd = 0.f;
-    float max_density = 0.f;
-    int frame = 0;
-    bool initialized = false;
-    int last_iters = 0;
-    int p2g_tile = 8;             // P2G face-column tile edge (autotuned in batches)
-    std::string err;
-    hipStream_t stream = nullptr;
-    int device = 0;
-
-    // particles (sorted by (cell, id)); b = current buffer
-    float4* pos[2] = {nullptr, nullptr};
-    float4* vel[2] = {nullptr, nullptr};
-    int cur = 0;
-    int32_t* cell_of = nullptr;
-    int32_t* slots = nullptr;
-    int32_t* sid = nullptr;       // particle ids next to slots (per-cell ordering)
-    int32_t* counts = nullptr;
-    int32_t* offsets = nullptr;
-    int32_t* scan_sums = nullptr;
-    uint8_t* accept = nullptr;
-    int64_t np_cap = 0;
-
-    // grids
-    float* u[3] = {nullptr, nullptr, nullptr};
-    float* du[3] = {nullptr, nullptr, nullptr};
-    float *P = nullptr, *D = nullptr, *L = nullptr;
-    uint8_t* A = nullptr;
-    // pc: 2 N^3 floats, the factor in the natural layout and then the tile-major
-    // copy the MIC(0) apply reads; Q is the apply's intermediate, tile-major
-    float *R = nullptr, *Z = nullptr, *S = nullptr, *Q = nullptr, *pc = nullptr, *diag = nullptr;
-    uint8_t* nup = nullptr;
-    uint8_t* fl8 = nullptr;       // fluid bits per 8-cell z-segment
-    double* P64 = nullptr;        // FP64 solution of the refined solve
-    // active sets of the pressure solve (segments, tiles), rebuilt per solve
-    int32_t *seg_flag = nullptr, *seg_scan = nullptr, *seg_list = nullptr;
-    int32_t *tile_any = nullptr, *tile_lists = nullptr;
-    Active act;
-    // learned preconditioner: weights, activations, r_old (allocated on load)
-    LearnedNet net;
-    uint16_t* net_mem = nullptr;
-    float* R_old = nullptr;
-    bool net_ready = false;
-    double* partials = nullptr;
-    uint32_t* counter = nullptr;
-    SolverScalars* sc = nullptr;
-    SolverScalars* sc_host = nullptr;   // pinned
-
-    Stage stages[ARIEL_NUM_STAGES];
-    bool profile = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<KTimer> pending;
-    ariel_kernel_stats kstats[ARIEL_NUM_KERNELS]{};
-    hipEvent_t step_t0 = nullptr, step_t1 = nullptr, solve_t0 = nullptr, solve_t1 = nullptr;
-    ariel_step_stats stats{};
-
-    Params params() const {
-        Params p;
-        p.N = N;
-        p.maxd = maxd;
-        p.density = d.density;
-        p.dt = d.dt;
-        p.picflip = d.picflip;
-        p.sharpen_re = d.sharpen_re;
-        p.spring = d.spring;
-        p.splat = d.splat;
-        p.wall = d.wall;
-        p.seed = d.seed;
-        p.n_forces = d.n_forces;
-        for (int f = 0; f < 4; ++f)
-            for (int c = 0; c < 3; ++c) p.forces[f][c] = d.forces[f][c];
-        p.max_density = max_density;
-        p.frame = frame;
-        p.mic_tau = d.mic_tau;
-        p.mic_sigma = d.mic_sigma;
-        // the learned preconditioner's fallback and set-up are the tiled MIC(0)
-        p.precond = d.precond == ARIEL_PRECOND_LEARNED ? ARIEL_PRECOND_MIC0_TILED : d.precond;
-        p.p2g_tile = p2g_tile;
-        return p;
-    }
-    Reduce reduce() const { return Reduce{partials, counter, sc}; }
-};
 
 namespace ariel {
 Recorder*& recorder() {
@@ -135,39 +23,8 @@ bool& batch_sizing() {
 }
 }  // namespace ariel
 
-namespace {
-
-int fail(ariel_sim* h, int code, const std::string& msg) {
-    if (h) h->err = msg;
-    g_last_error = msg;
-    return code;
-}
-
-#define HIPCHK(h, expr)                                                                   \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail((h), ARIEL_ERR_HIP,                                               \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));               \
-    } while (0)
-
-template <class T>
-int dalloc(ariel_sim* h, T** p, int64_t n) {
-    *p = nullptr;
-    if (n <= 0) n = 1;
-    hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return fail(h, ARIEL_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
-    return ARIEL_OK;
-}
-
-#define TRY(expr)                       \
-    do {                                \
-        int rc_ = (expr);               \
-        if (rc_ != ARIEL_OK) return rc_; \
-    } while (0)
+namespace ariel {
+namespace host {
 
 int check_launch(ariel_sim* h, const char* what) {
     hipError_t e = hipGetLastError();
@@ -210,32 +67,6 @@ double kernel_bytes(const ariel_sim* h, int k) {
         default: return 0.0;
     }
 }
-
-// Each event record leaves the GPU idle for a few microseconds between the
-// kernels it separates, so inside the PCG loop only one iteration in
-// kProfEvery is timed (the launches do not depend on the iteration index).
-constexpr int kProfEvery = 16;
-inline bool prof_sampled(int it) { return (it - 1) % kProfEvery == 0; }
-
-struct Prof {
-    ariel_sim* h;
-    int k;
-    hipEvent_t e0 = nullptr;
-    Prof(ariel_sim* hh, int kk, bool on = true) : h(hh), k(kk) {
-        if (h->profile) h->kstats[k].calls += 1;
-        if (h->profile && on) {
-            e0 = take_event(h);
-            if (e0) (void)hipEventRecord(e0, h->stream);
-        }
-    }
-    ~Prof() {
-        if (!e0) return;
-        hipEvent_t e1 = take_event(h);
-        if (!e1) return;
-        (void)hipEventRecord(e1, h->stream);
-        h->pending.push_back(KTimer{k, e0, e1});
-    }
-};
 
 // Called after a stream synchronisation: fold finished timers into kstats.
 void drain_timers(ariel_sim* h) {
@@ -432,236 +263,6 @@ int stage_divergence(ariel_sim* h) {
     return check_launch(h, "divergence");
 }
 
-int read_scalars(ariel_sim* h) {
-    HIPCHK(h, hipMemcpyAsync(h->sc_host, h->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    return ARIEL_OK;
-}
-
-// PCG iterations are queued in chunks with one readback of the solver scalars
-// after each. A readback idles the GPU for a few tens of microseconds; an
-// iteration queued past convergence still costs the dispatch of each of its
-// kernels (they return at once), about 80 us. The first chunk is two thirds of
-// the previous solve's count; later ones extrapolate the residual's decay
-// since the previous readback (or since the initial residual) to the stop
-// threshold and queue half of that estimate: CG's decay speeds up as it
-// proceeds, so the estimate runs high.
-class ChunkPlan {
-  public:
-    explicit ChunkPlan(int last_iters) : first_(std::max(4, 2 * last_iters / 3)) {}
-    int first() const { return first_; }
-    int next(const SolverScalars& sc) {
-        const double r0 = have_ ? r2_ : sc.rnorm2_0;
-        const int dn = sc.it - (have_ ? it_ : 0);
-        have_ = true;
-        it_ = sc.it;
-        r2_ = sc.rnorm2;
-        if (dn <= 0 || !(sc.rnorm2 > 0.0) || !(r0 > sc.rnorm2) || !(sc.thr > 0.0)) return 4;
-        const double need = std::log(sc.thr / sc.rnorm2);   // < 0 while not converged
-        if (!(need < 0.0)) return 2;
-        const double per_it = std::log(sc.rnorm2 / r0) / dn;  // < 0
-        const double est = 0.5 * need / per_it;
-        return est < 2.0 ? 2 : est > 64.0 ? 64 : (int)est;
-    }
-
-  private:
-    int first_;
-    bool have_ = false;
-    int it_ = 0;
-    double r2_ = 0.0;
-};
-
-// Active-set descriptor of a handle: counts from the last readback, or -1
-// (read on the device) for batched launches; caps are the largest possible.
-void set_active(ariel_sim* h, bool device_counts) {
-    const int64_t TN = h->N / 8;
-    const int64_t nt = TN * TN * TN;
-    const int64_t per_colour = (TN & 1) == 0 ? nt / 2 : nt;
-    h->act.segs = h->seg_list;
-    h->act.nseg = device_counts ? -1 : h->sc_host->nseg;
-    h->act.seg_cap = h->nc / 8;
-    for (int c = 0; c < 3; ++c) {
-        h->act.tiles[c] = h->tile_lists + c * nt;
-        h->act.ntiles[c] = device_counts ? -1 : (int64_t)h->sc_host->ntile[c];
-        h->act.tile_cap[c] = c == 2 ? nt : per_colour;
-    }
-}
-
-// Preconditioner set-up on the current cell classes: diagonal, fluid segment
-// bits, the active segment and tile lists (one host round trip for their
-// counts), then the factorisation over the active tiles. Leaves the solver
-// scalars initialised.
-int prepare_solve(ariel_sim* h) {
-    const Params p = h->params();
-    hipStream_t s = h->stream;
-    launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
-    launch_fluid_segments(s, h->N, h->A, h->fl8);
-    launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every,
-                       h->d.precond == ARIEL_PRECOND_LEARNED ? 1 : 0);
-    launch_active_sets(s, h->N, h->fl8, h->seg_flag, h->seg_scan, h->seg_list, h->tile_any,
-                       h->tile_lists, h->scan_sums, h->sc);
-    TRY(check_launch(h, "active sets"));
-    TRY(read_scalars(h));
-    set_active(h, false);
-    launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc, h->act, h->sc);
-    return check_launch(h, "preconditioner build");
-}
-
-// BuildPreconditioner + SolveConjugateGradient (solver.inl:104-135, 289-354) with
-// the relative stop rule, FP64 reductions and residual replacement. The
-// iterations touch the active segments and tiles only. The full-grid set-up
-// below leaves P and R zero off-FLUID and Z zero everywhere; S and Z are then
-// written on every active segment before they are read, and values off the
-// active sets are never read as FLUID values.
-int stage_solve_learned(ariel_sim* h);
-
-int stage_solve(ariel_sim* h) {
-    if (h->d.precond == ARIEL_PRECOND_LEARNED) return stage_solve_learned(h);
-    const Params p = h->params();
-    const Reduce red = h->reduce();
-    hipStream_t s = h->stream;
-    HIPCHK(h, hipEventRecord(h->solve_t0, s));
-    TRY(prepare_solve(h));
-    const Active& act = h->act;
-    const int refine = h->d.refine_every > 0;
-    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
-                     h->P64, refine);
-    if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
-    launch_fill_f32(s, h->Z, h->nc, 0.0f);
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, act);
-    launch_update_s(s, h->Z, h->S, red, 1, act);
-    TRY(check_launch(h, "solve setup"));
-    int queued = 0;
-    const int k = h->d.refine_every;
-    ChunkPlan plan(h->last_iters);
-    int chunk = plan.first();
-    for (;;) {
-        const int n = std::min(chunk, h->d.max_iters - queued);
-        for (int j = 0; j < n; ++j) {
-            const int it = queued + j + 1;
-            {
-                Prof prof(h, ARIEL_KERNEL_SPMV, prof_sampled(it));
-                launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, act);
-            }
-            {
-                Prof prof(h, ARIEL_KERNEL_AXPY, prof_sampled(it));
-                launch_axpy2(s, h->P, h->S, h->R, h->Z, red, act);
-            }
-            if (k > 0 && it % k == 0)
-                launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, act);
-            {
-                Prof prof(h, ARIEL_KERNEL_PRECOND, prof_sampled(it));
-                launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, act);
-            }
-            launch_update_s(s, h->Z, h->S, red, 0, act);
-        }
-        TRY(check_launch(h, "pcg"));
-        queued += n;
-        TRY(read_scalars(h));
-        if (h->sc_host->done || queued >= h->d.max_iters) break;
-        chunk = plan.next(*h->sc_host);
-    }
-    if (refine) launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
-    HIPCHK(h, hipEventRecord(h->solve_t1, s));
-    h->last_iters = h->sc_host->it;
-    h->stats.iterations = h->sc_host->it;
-    h->stats.solve_status = h->sc_host->status;
-    h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
-    h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
-    // true residual of the final pressure, for the report only
-    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, act);
-    TRY(check_launch(h, "true residual"));
-    TRY(read_scalars(h));
-    h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
-    return ARIEL_OK;
-}
-
-// Flexible PCG (Notay 2000; Polak-Ribiere beta) with the learned network as
-// preconditioner and the tiled MIC(0) as fallback: over every window of
-// kFlexWindow iterations the residual must fall by 5% per iteration on average,
-// and every z of the network must be a descent direction (z.r > 0); otherwise
-// the solve switches to MIC(0) for good and restarts its search direction. The
-// network and MIC(0) kernels are both queued each iteration; the one not in use
-// returns at once. Once the host has seen the switch (at a readback), it
-// queues the MIC(0) kernels only.
-void learned_precond(ariel_sim* h, const Params& p, const Reduce& red, bool mic_only) {
-    hipStream_t s = h->stream;
-    if (!mic_only) {
-        launch_learned_apply(s, h->net, h->N, h->R, h->A, h->diag, h->Z, h->sc, h->act.nseg);
-        launch_pr_dot(s, h->Z, h->R, h->R_old, red, h->act, true);
-    }
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, h->act);
-    launch_pr_dot(s, h->Z, h->R, h->R_old, red, h->act, false);
-}
-
-int stage_solve_learned(ariel_sim* h) {
-    if (!h->net_ready)
-        return fail(h, ARIEL_ERR_STATE, "learned preconditioner: no weights (ariel_set_learned_weights)");
-    const Params p = h->params();
-    const Reduce red = h->reduce();
-    hipStream_t s = h->stream;
-    HIPCHK(h, hipEventRecord(h->solve_t0, s));
-    TRY(prepare_solve(h));
-    const Active& act = h->act;
-    const int refine = h->d.refine_every > 0;
-    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
-                     h->P64, refine);
-    if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
-    launch_fill_f32(s, h->Z, h->nc, 0.0f);
-    launch_fill_f32(s, h->R_old, h->nc, 0.0f);
-    launch_diag_weight(s, h->N, h->A, h->diag, h->net.wd);
-    learned_precond(h, p, red, false);
-    launch_update_s(s, h->Z, h->S, red, 1, act);
-    TRY(check_launch(h, "learned solve setup"));
-    bool on_mic = false;
-    int queued = 0;
-    const int k = h->d.refine_every;
-    ChunkPlan plan(h->last_iters);
-    int chunk = plan.first();
-    for (;;) {
-        const int n = std::min(chunk, h->d.max_iters - queued);
-        for (int j = 0; j < n; ++j) {
-            const int it = queued + j + 1;
-            {
-                Prof prof(h, ARIEL_KERNEL_SPMV, prof_sampled(it));
-                launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, act);
-            }
-            {
-                Prof prof(h, ARIEL_KERNEL_AXPY, prof_sampled(it));
-                launch_copy_list(s, h->R, h->R_old, act, h->sc);
-                launch_axpy2(s, h->P, h->S, h->R, h->Z, red, act);
-            }
-            if (k > 0 && it % k == 0)
-                launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, act);
-            {
-                Prof prof(h, ARIEL_KERNEL_PRECOND, prof_sampled(it));
-                learned_precond(h, p, red, on_mic);
-            }
-            launch_update_s(s, h->Z, h->S, red, 0, act);
-        }
-        TRY(check_launch(h, "learned pcg"));
-        queued += n;
-        TRY(read_scalars(h));
-        on_mic = on_mic || h->sc_host->use_mic;
-        if (h->sc_host->done || queued >= h->d.max_iters) break;
-        chunk = plan.next(*h->sc_host);
-    }
-    if (refine) launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
-    HIPCHK(h, hipEventRecord(h->solve_t1, s));
-    h->last_iters = h->sc_host->it;
-    h->stats.iterations = h->sc_host->it;
-    h->stats.solve_status = h->sc_host->status;
-    h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
-    h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
-    h->stats.precond_fallback = h->sc_host->use_mic ? h->sc_host->fallback_reason : 0;
-    h->stats.fallback_iter = h->sc_host->use_mic ? h->sc_host->fallback_it : 0;
-    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, act);
-    TRY(check_launch(h, "true residual"));
-    TRY(read_scalars(h));
-    h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
-    return ARIEL_OK;
-}
-
 int stage_gradient(ariel_sim* h) {
     Prof prof(h, ARIEL_KERNEL_GRADIENT);
     launch_gradient(h->stream, h->params(), h->P, h->L, h->A, h->u);
@@ -787,7 +388,8 @@ int calibrate_max_density(ariel_sim* h) {
 }
 
 
-}  // namespace
+}  // namespace host
+}  // namespace ariel
 
 // =================================================================== C ABI
 extern "C" {
@@ -1493,458 +1095,6 @@ int ariel_selftest_division(int32_t which, float a, float lo, float hi, int64_t*
     hipError_t e = run_division_selftest(which, a, lo, hi, mismatches);
     if (e != hipSuccess)
         return fail(nullptr, ARIEL_ERR_HIP, std::string("selftest_division: ") + hipGetErrorString(e));
-    return ARIEL_OK;
-}
-
-}  // extern "C"
-
-// ================================================================ batches
-// Independent tanks stepped together: every stage is one launch over all tanks
-// (blockIdx.y = tank). The launch sequence of a step is recorded per tank with
-// batch sizing (solver counts read on the device), the tanks' Args are packed
-// into device tables, and each launch position is issued once with the largest
-// tank's grid. Kernels size their work by their own tank, so each tank's
-// results equal those of stepping it alone. The pre-solve part, one PCG
-// iteration (with and without reliable update) and the post-solve part can be
-// captured as HIP graphs and relaunched every step.
-struct BatchPos {
-    const void* many = nullptr;
-    dim3 grid, block;
-    size_t arg_size = 0;
-    size_t offset = 0;   // of the tank table inside the arena
-};
-
-struct BatchPhase {
-    std::vector<BatchPos> pos;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    std::vector<uint64_t> shape;   // layout the graph was captured for
-};
-
-struct ariel_batch {
-    std::vector<ariel_sim*> tanks;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    BatchPhase pre, it_plain, it_refine, post;
-    void* arena = nullptr;
-    size_t arena_bytes = 0;
-    unsigned char* host_arena = nullptr;   // pinned staging of the tables
-    size_t host_bytes = 0;
-    SolverScalars* sc_host = nullptr;      // pinned, one per tank
-    bool graphs = false;
-    int last_iters = 0;
-    std::string err;
-};
-
-namespace {
-
-int bfail(ariel_batch* b, int code, const std::string& msg) {
-    if (b) b->err = msg;
-    g_last_error = msg;
-    return code;
-}
-
-#define BHIP(b, expr)                                                                     \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return bfail((b), ARIEL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-// The launches of one tank's step, in four phases.
-struct TankRec {
-    Recorder pre, it_plain, it_refine, post;
-};
-
-void record_pre(ariel_sim* h, Recorder& r) {
-    recorder() = &r;
-    stage_sort(h);
-    stage_density(h, 1);
-    stage_mark(h);
-    stage_p2g(h);
-    stage_divergence(h);
-    const Params p = h->params();
-    const Reduce red = h->reduce();
-    hipStream_t s = h->stream;
-    launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
-    launch_fluid_segments(s, h->N, h->A, h->fl8);
-    launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every);
-    launch_active_sets(s, h->N, h->fl8, h->seg_flag, h->seg_scan, h->seg_list, h->tile_any,
-                       h->tile_lists, h->scan_sums, h->sc);
-    set_active(h, true);
-    launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc, h->act, h->sc);
-    const int refine = h->d.refine_every > 0;
-    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
-                     h->P64, refine);
-    if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
-    launch_fill_f32(s, h->Z, h->nc, 0.0f);
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, h->act);
-    launch_update_s(s, h->Z, h->S, red, 1, h->act);
-    recorder() = nullptr;
-}
-
-void record_iteration(ariel_sim* h, Recorder& r, bool refine) {
-    recorder() = &r;
-    const Params p = h->params();
-    const Reduce red = h->reduce();
-    hipStream_t s = h->stream;
-    launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, h->act);
-    launch_axpy2(s, h->P, h->S, h->R, h->Z, red, h->act);
-    if (refine) launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, h->act);
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, h->act);
-    launch_update_s(s, h->Z, h->S, red, 0, h->act);
-    recorder() = nullptr;
-}
-
-void record_post(ariel_sim* h, Recorder& r) {
-    recorder() = &r;
-    const Params p = h->params();
-    const Reduce red = h->reduce();
-    hipStream_t s = h->stream;
-    if (h->d.refine_every > 0) launch_refine_finish(s, h->A, h->P, h->P64, h->act, h->sc);
-    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, h->act);
-    stage_gradient(h);
-    stage_extrapolate(h);
-    stage_advect(h);
-    stage_resample(h);
-    recorder() = nullptr;
-}
-
-// Packs phase `which` of every tank into positions (tables appended to `host`).
-int build_phase(ariel_batch* B, const std::vector<TankRec>& recs,
-                Recorder TankRec::*which, BatchPhase& ph, std::vector<unsigned char>& host) {
-    const size_t T = recs.size();
-    const auto& first = (recs[0].*which).seq;
-    ph.pos.clear();
-    for (size_t i = 0; i < first.size(); ++i) {
-        BatchPos bp;
-        bp.many = first[i].many;
-        bp.block = first[i].block;
-        bp.arg_size = first[i].args.size();
-        unsigned gx = 1;
-        for (size_t t = 0; t < T; ++t) {
-            const auto& seq = (recs[t].*which).seq;
-            if (seq.size() != first.size() || seq[i].many != bp.many ||
-                seq[i].block.x != bp.block.x || seq[i].args.size() != bp.arg_size)
-                return bfail(B, ARIEL_ERR_INVALID, "batch: tanks record different launch sequences");
-            gx = std::max(gx, seq[i].grid.x);
-        }
-        bp.grid = dim3(gx, (unsigned)T, 1);
-        bp.offset = (host.size() + 255) / 256 * 256;
-        host.resize(bp.offset + T * bp.arg_size);
-        for (size_t t = 0; t < T; ++t)
-            std::memcpy(host.data() + bp.offset + t * bp.arg_size, (recs[t].*which).seq[i].args.data(),
-                        bp.arg_size);
-        ph.pos.push_back(bp);
-    }
-    return ARIEL_OK;
-}
-
-std::vector<uint64_t> phase_shape(const BatchPhase& ph) {
-    std::vector<uint64_t> v;
-    for (const auto& p : ph.pos) {
-        v.push_back((uint64_t)(uintptr_t)p.many);
-        v.push_back(((uint64_t)p.grid.x << 32) | p.block.x);
-        v.push_back(p.offset);
-    }
-    return v;
-}
-
-int issue_phase(ariel_batch* B, BatchPhase& ph) {
-    for (const auto& p : ph.pos) {
-        void* table = (unsigned char*)B->arena + p.offset;
-        void* args[] = {&table};
-        BHIP(B, hipLaunchKernel(p.many, p.grid, p.block, args, 0, B->stream));
-    }
-    return ARIEL_OK;
-}
-
-int run_phase(ariel_batch* B, BatchPhase& ph) {
-    if (!B->graphs) return issue_phase(B, ph);
-    const auto shape = phase_shape(ph);
-    if (!ph.exec || shape != ph.shape) {
-        if (ph.exec) hipGraphExecDestroy(ph.exec);
-        if (ph.graph) hipGraphDestroy(ph.graph);
-        ph.exec = nullptr;
-        ph.graph = nullptr;
-        BHIP(B, hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-        int rc = issue_phase(B, ph);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(B->stream, &g);
-        if (rc != ARIEL_OK) return rc;
-        if (e != hipSuccess) return bfail(B, ARIEL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        ph.graph = g;
-        BHIP(B, hipGraphInstantiate(&ph.exec, g, nullptr, nullptr, 0));
-        ph.shape = shape;
-    }
-    BHIP(B, hipGraphLaunch(ph.exec, B->stream));
-    return ARIEL_OK;
-}
-
-int read_batch_scalars(ariel_batch* B) {
-    for (size_t t = 0; t < B->tanks.size(); ++t)
-        BHIP(B, hipMemcpyAsync(&B->sc_host[t], B->tanks[t]->sc, sizeof(SolverScalars),
-                               hipMemcpyDeviceToHost, B->stream));
-    BHIP(B, hipStreamSynchronize(B->stream));
-    return ARIEL_OK;
-}
-
-int batch_step_once(ariel_batch* B) {
-    const size_t T = B->tanks.size();
-    std::vector<TankRec> recs(T);
-    for (size_t t = 0; t < T; ++t) {
-        ariel_sim* h = B->tanks[t];
-        h->frame++;
-        record_pre(h, recs[t].pre);
-        record_iteration(h, recs[t].it_plain, false);
-        record_iteration(h, recs[t].it_refine, true);
-        record_post(h, recs[t].post);
-    }
-    std::vector<unsigned char> host;
-    TRY(build_phase(B, recs, &TankRec::pre, B->pre, host));
-    TRY(build_phase(B, recs, &TankRec::it_plain, B->it_plain, host));
-    TRY(build_phase(B, recs, &TankRec::it_refine, B->it_refine, host));
-    TRY(build_phase(B, recs, &TankRec::post, B->post, host));
-    if (host.size() > B->arena_bytes) {
-        if (B->arena) hipFree(B->arena);
-        if (B->host_arena) hipHostFree(B->host_arena);
-        B->arena = nullptr;
-        B->host_arena = nullptr;
-        const size_t bytes = host.size() + host.size() / 4 + 4096;
-        BHIP(B, hipMalloc(&B->arena, bytes));
-        BHIP(B, hipHostMalloc((void**)&B->host_arena, bytes, hipHostMallocDefault));
-        B->arena_bytes = B->host_bytes = bytes;
-        // tables moved: every graph is captured again
-        for (BatchPhase* ph : {&B->pre, &B->it_plain, &B->it_refine, &B->post}) ph->shape.clear();
-    }
-    // the previous step's graphs may still read the staging buffer's copy
-    BHIP(B, hipStreamSynchronize(B->stream));
-    std::memcpy(B->host_arena, host.data(), host.size());
-    BHIP(B, hipMemcpyAsync(B->arena, B->host_arena, host.size(), hipMemcpyHostToDevice, B->stream));
-
-    TRY(run_phase(B, B->pre));
-    // PCG: iterations in chunks, one readback of every tank's scalars per chunk
-    int max_iters = 0, k = 0;
-    for (ariel_sim* h : B->tanks) {
-        max_iters = std::max(max_iters, h->d.max_iters);
-        k = h->d.refine_every;
-    }
-    int queued = 0;
-    int chunk = std::max(4, B->last_iters - 3);
-    for (;;) {
-        const int n = std::min(chunk, max_iters - queued);
-        for (int j = 0; j < n; ++j) {
-            const int it = queued + j + 1;
-            TRY(run_phase(B, (k > 0 && it % k == 0) ? B->it_refine : B->it_plain));
-        }
-        queued += n;
-        TRY(read_batch_scalars(B));
-        bool all_done = true;
-        for (size_t t = 0; t < T; ++t) all_done = all_done && B->sc_host[t].done;
-        if (all_done || queued >= max_iters) break;
-        chunk = 4;
-    }
-    TRY(run_phase(B, B->post));
-    TRY(read_batch_scalars(B));
-    int most = 0;
-    for (size_t t = 0; t < T; ++t) {
-        ariel_sim* h = B->tanks[t];
-        const SolverScalars& sc = B->sc_host[t];
-        h->last_iters = sc.it;
-        most = std::max(most, sc.it);
-        h->stats.iterations = sc.it;
-        h->stats.solve_status = sc.status;
-        h->stats.rhs_norm = std::sqrt(sc.bnorm2);
-        h->stats.residual_norm = std::sqrt(sc.rnorm2);
-        h->stats.true_residual_norm = std::sqrt(sc.tmp);
-        h->stats.fluid_cells = sc.fluid_cells;
-        float md;
-        uint32_t bits = sc.maxdiv_bits;
-        std::memcpy(&md, &bits, 4);
-        h->stats.max_divergence = md;
-        h->stats.frame = h->frame;
-    }
-    B->last_iters = most;
-    return ARIEL_OK;
-}
-
-void destroy_phase(BatchPhase& ph) {
-    if (ph.exec) hipGraphExecDestroy(ph.exec);
-    if (ph.graph) hipGraphDestroy(ph.graph);
-    ph.exec = nullptr;
-    ph.graph = nullptr;
-}
-
-// Autotuner: for each grid size, time P2G with each face-column tile edge on the
-// first tank of that size and keep the fastest for all its tanks.
-int autotune(ariel_batch* B) {
-    std::vector<int> sizes;
-    for (ariel_sim* h : B->tanks)
-        if (std::find(sizes.begin(), sizes.end(), h->N) == sizes.end()) sizes.push_back(h->N);
-    hipEvent_t e0, e1;
-    BHIP(B, hipEventCreate(&e0));
-    BHIP(B, hipEventCreate(&e1));
-    for (int n : sizes) {
-        ariel_sim* h = nullptr;
-        for (ariel_sim* t : B->tanks)
-            if (t->N == n) { h = t; break; }
-        int best = h->p2g_tile;
-        float best_ms = 1e30f;
-        for (int tile : {4, 8, 16, 32}) {
-            h->p2g_tile = tile;
-            const Params p = h->params();
-            for (int rep = 0; rep < 4; ++rep) {
-                if (rep == 1) BHIP(B, hipEventRecord(e0, h->stream));
-                launch_p2g(h->stream, p, h->pos[h->cur], h->vel[h->cur], h->offsets, h->A, h->u, h->du);
-            }
-            BHIP(B, hipEventRecord(e1, h->stream));
-            BHIP(B, hipEventSynchronize(e1));
-            float ms = 0.f;
-            BHIP(B, hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = tile;
-            }
-        }
-        for (ariel_sim* t : B->tanks)
-            if (t->N == n) t->p2g_tile = best;
-    }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    // P2G left the tanks' grids as a fresh splat would; the next step redoes it
-    return ARIEL_OK;
-}
-
-}  // namespace
-
-extern "C" {
-
-int ariel_batch_create(const ariel_scene_desc* descs, int32_t ntanks, ariel_batch** out) {
-    if (!descs || !out || ntanks <= 0) return bfail(nullptr, ARIEL_ERR_INVALID, "batch_create: bad arguments");
-    *out = nullptr;
-    if (descs[0].precond == ARIEL_PRECOND_LEARNED)
-        return bfail(nullptr, ARIEL_ERR_UNSUPPORTED,
-                     "batch_create: the learned preconditioner runs on single tanks only");
-    for (int32_t t = 1; t < ntanks; ++t) {
-        const ariel_scene_desc& a = descs[0];
-        const ariel_scene_desc& b = descs[t];
-        if (a.precond != b.precond || a.splat != b.splat || a.refine_every != b.refine_every ||
-            a.resample != b.resample || a.max_iters != b.max_iters)
-            return bfail(nullptr, ARIEL_ERR_INVALID,
-                         "batch_create: tanks must share preconditioner, splat, refinement, "
-                         "resampling and iteration limit");
-    }
-    ariel_batch* B = new ariel_batch();
-    if (hipGetDevice(&B->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete B;
-        return bfail(nullptr, ARIEL_ERR_HIP, "batch_create: no device");
-    }
-    for (int32_t t = 0; t < ntanks; ++t) {
-        ariel_sim* h = nullptr;
-        const int rc = ariel_create(&descs[t], &h);
-        if (rc != ARIEL_OK) {
-            const std::string msg = g_last_error;
-            ariel_batch_destroy(B);
-            return bfail(nullptr, rc, "batch_create: tank " + std::to_string(t) + ": " + msg);
-        }
-        B->tanks.push_back(h);
-    }
-    if (hipHostMalloc((void**)&B->sc_host, sizeof(SolverScalars) * ntanks, hipHostMallocDefault) != hipSuccess) {
-        ariel_batch_destroy(B);
-        return bfail(nullptr, ARIEL_ERR_NOMEM, "batch_create: hipHostMalloc failed");
-    }
-    *out = B;
-    return ARIEL_OK;
-}
-
-int ariel_batch_destroy(ariel_batch* B) {
-    if (!B) return ARIEL_OK;
-    if (B->stream) hipStreamSynchronize(B->stream);
-    for (BatchPhase* ph : {&B->pre, &B->it_plain, &B->it_refine, &B->post}) destroy_phase(*ph);
-    for (ariel_sim* h : B->tanks) ariel_destroy(h);
-    if (B->arena) hipFree(B->arena);
-    if (B->host_arena) hipHostFree(B->host_arena);
-    if (B->sc_host) hipHostFree(B->sc_host);
-    if (B->stream) hipStreamDestroy(B->stream);
-    delete B;
-    return ARIEL_OK;
-}
-
-const char* ariel_batch_last_error(const ariel_batch* B) {
-    if (B) return B->err.c_str();
-    return g_last_error.c_str();
-}
-
-int ariel_batch_size(ariel_batch* B, int32_t* n) {
-    if (!B || !n) return bfail(B, ARIEL_ERR_INVALID, "null pointer");
-    *n = (int32_t)B->tanks.size();
-    return ARIEL_OK;
-}
-
-int ariel_batch_tank(ariel_batch* B, int32_t i, ariel_sim** out) {
-    if (!B || !out) return bfail(B, ARIEL_ERR_INVALID, "null pointer");
-    if (i < 0 || i >= (int32_t)B->tanks.size()) return bfail(B, ARIEL_ERR_INVALID, "tank index out of range");
-    *out = B->tanks[i];
-    return ARIEL_OK;
-}
-
-int ariel_batch_init(ariel_batch* B) {
-    if (!B) return bfail(nullptr, ARIEL_ERR_INVALID, "null batch");
-    for (size_t t = 0; t < B->tanks.size(); ++t) {
-        const int rc = ariel_init(B->tanks[t]);
-        if (rc != ARIEL_OK)
-            return bfail(B, rc, "tank " + std::to_string(t) + ": " + B->tanks[t]->err);
-    }
-    return ARIEL_OK;
-}
-
-int ariel_batch_autotune(ariel_batch* B) {
-    if (!B) return bfail(nullptr, ARIEL_ERR_INVALID, "null batch");
-    for (ariel_sim* h : B->tanks)
-        if (!h->initialized) return bfail(B, ARIEL_ERR_STATE, "autotune before ariel_batch_init");
-    TRY(autotune(B));
-    for (ariel_sim* h : B->tanks) BHIP(B, hipStreamSynchronize(h->stream));
-    return ARIEL_OK;
-}
-
-int ariel_batch_get_tuning(ariel_batch* B, int32_t i, int32_t* p2g_tile) {
-    if (!B || !p2g_tile) return bfail(B, ARIEL_ERR_INVALID, "null pointer");
-    if (i < 0 || i >= (int32_t)B->tanks.size()) return bfail(B, ARIEL_ERR_INVALID, "tank index out of range");
-    *p2g_tile = B->tanks[i]->p2g_tile;
-    return ARIEL_OK;
-}
-
-int ariel_batch_set_graphs(ariel_batch* B, int32_t enable) {
-    if (!B) return bfail(nullptr, ARIEL_ERR_INVALID, "null batch");
-    B->graphs = enable != 0;
-    return ARIEL_OK;
-}
-
-int ariel_batch_step(ariel_batch* B, int32_t nsteps) {
-    if (!B) return bfail(nullptr, ARIEL_ERR_INVALID, "null batch");
-    if (nsteps < 0) return bfail(B, ARIEL_ERR_INVALID, "negative step count");
-    for (ariel_sim* h : B->tanks) {
-        if (!h->initialized) return bfail(B, ARIEL_ERR_STATE, "ariel_batch_step before ariel_batch_init");
-        if (h->profile) return bfail(B, ARIEL_ERR_STATE, "profiling is per handle; disable it on batch tanks");
-        // earlier single-tank work on the tanks' own streams comes first
-        BHIP(B, hipStreamSynchronize(h->stream));
-    }
-    if (hipSetDevice(B->device) != hipSuccess) return bfail(B, ARIEL_ERR_HIP, "hipSetDevice failed");
-    for (int32_t n = 0; n < nsteps; ++n) TRY(batch_step_once(B));
-    BHIP(B, hipStreamSynchronize(B->stream));
-    return ARIEL_OK;
-}
-
-int ariel_batch_step_sequential(ariel_batch* B, int32_t nsteps) {
-    if (!B) return bfail(nullptr, ARIEL_ERR_INVALID, "null batch");
-    BHIP(B, hipStreamSynchronize(B->stream));
-    for (int32_t n = 0; n < nsteps; ++n)
-        for (ariel_sim* h : B->tanks) {
-            const int rc = ariel_step(h, 1);
-            if (rc != ARIEL_OK) return bfail(B, rc, h->err);
-        }
-    for (ariel_sim* h : B->tanks) BHIP(B, hipStreamSynchronize(h->stream));
     return ARIEL_OK;
 }
 
