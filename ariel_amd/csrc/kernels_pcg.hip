@@ -88,6 +88,9 @@ __device__ void block_partials(double (&v)[NV], double* part, int slot) {
             part[slot * NV + i] = t;
         }
     }
+    // sm is shared by every call: a later call in the same kernel may not
+    // overwrite it before thread 0 has read it
+    __syncthreads();
 }
 
 __device__ __forceinline__ bool finite(double x) { return isfinite(x); }
