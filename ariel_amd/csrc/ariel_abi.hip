@@ -1088,7 +1088,7 @@ int ariel_selftest_bf16(int64_t* mismatches) {
 }
 
 int ariel_selftest_division(int32_t which, float a, float lo, float hi, int64_t* mismatches) {
-    if (!mismatches || (which != 0 && which != 1) || !(lo >= 0.0f) || !(hi >= lo) ||
+    if (!mismatches || (which < 0 || which > 2) || !(lo >= 0.0f) || !(hi >= lo) ||
         !std::isfinite(hi) || !(a > 0.0f) || !std::isfinite(a))
         return fail(nullptr, ARIEL_ERR_INVALID, "selftest_division: bad arguments");
     *mismatches = -1;

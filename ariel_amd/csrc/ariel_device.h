@@ -56,6 +56,20 @@ __device__ __forceinline__ float div_fixed(float x, float d, float yd) {
 __device__ __forceinline__ float smooth_fixed(float r2, float hh, float yhh) {
     return gmax_num(1.0f - div_fixed(r2, hh, yhh), 0.0f);
 }
+// smooth(r2, h) when hh = h*h is a power of two and yhh = 1.0f / hh (then exact):
+// r2 / hh and r2 * yhh are both the exact value r2 * 2^-e correctly rounded, so
+// the product is the quotient, bit for bit, for every r2 (subnormal results
+// included). Power-of-two grids at the reference's lattice densities give such
+// radii (4 * 0.5 / 256 = 2^-7).
+__device__ __forceinline__ float smooth_pow2(float r2, float yhh) {
+    return gmax_num(1.0f - r2 * yhh, 0.0f);
+}
+__host__ __device__ inline bool is_pow2_float(float x) {
+    uint32_t b;
+    __builtin_memcpy(&b, &x, 4);
+    const uint32_t e = (b >> 23) & 0xFFu;
+    return (b >> 31) == 0 && (b & 0x7FFFFFu) == 0 && e > 0 && e < 0xFFu;
+}
 
 // Correctly rounded n / d: Newton-Raphson on the hardware reciprocal (one
 // reciprocal refinement, a quotient, two residual corrections), the arithmetic
@@ -76,6 +90,25 @@ __device__ __forceinline__ float div_nr(float n, float d) {
 // 2^-60 <= hh <= 2^60 (max(r2, 1e-5) keeps the divisor in range).
 __device__ __forceinline__ float sharpen_nr(float r2, float hh) {
     return gmax_num(div_nr(hh, gmax_num(r2, 1.0e-5f)) - 1.0f, 0.0f);
+}
+
+// The reference's P2G Sharpen numerator, RE^2 with RE = 1.4 cells
+// (particlegridoperations.inl:149), as a float.
+constexpr float kSharpenHH = 1.4f * 1.4f;
+// n / d for n = kSharpenHH by the hardware reciprocal and one residual
+// correction. Compared with IEEE division for every float d in [1e-5, 4]
+// (ariel_selftest_division, which = 2): equal bit for bit. max(r2, 1e-5) puts the
+// divisor at or above 1e-5, and beyond 4 the quotient is below 1/2, where the
+// Sharpen weight is 0 whatever the last bit.
+__device__ __forceinline__ float div_sharpen_hh(float d) {
+    const float y = __builtin_amdgcn_rcpf(d);
+    const float q = kSharpenHH * y;
+    const float r = fmaf(-d, q, kSharpenHH);
+    return fmaf(r, y, q);
+}
+// sharpen(r2, 1.4) with the division by div_sharpen_hh: same value for every r2.
+__device__ __forceinline__ float sharpen_re14(float r2) {
+    return gmax_num(div_sharpen_hh(gmax_num(r2, 1.0e-5f)) - 1.0f, 0.0f);
 }
 
 // Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD b % 8).

@@ -440,6 +440,60 @@ __device__ __forceinline__ int staged_index(const StageLds& st, int cx, int cy, 
 // flip.cpp:697-733 + ApplyExternalForces (flip.cpp:682-695) fused: each particle
 // gathers Smooth weights (h = 4*density/maxd) over the 3x3x3 cells around its
 // cell; the 3 cells along z of each (sx,sy) column are one contiguous range.
+// ComputeDensity's sum for one particle over its staged (or, when the
+// neighbourhood did not fit, global) 3x3x3 cells, in the reference's order.
+// POW2: the smoothing radius squared is a power of two (smooth_pow2).
+template <bool POW2>
+__device__ __forceinline__ float density_sum(const StageLds& st, const float4& me, int x0, int x1,
+                                             int y0, int y1, int z0, int z1, int N,
+                                             const int32_t* __restrict__ offsets,
+                                             const float4* __restrict__ pos, float hh, float yhh) {
+    float weightsum = 0.0f;
+    if (st.ok) {
+        // the nine columns' ranges are fetched together before any is used
+        int32_t qa[9], qb[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int sx = x0 + c / 3, sy = y0 + c % 3;
+            qa[c] = qb[c] = 0;
+            if (sx <= x1 && sy <= y1) {
+                const int64_t col = cidx(N, sx, sy, 0);
+                qa[c] = offsets[col + z0];
+                qb[c] = offsets[col + z1 + 1];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int sx = x0 + c / 3, sy = y0 + c % 3;
+            if (sx > x1 || sy > y1) continue;
+            {
+                const int l0 = staged_index(st, sx, sy, qa[c]);
+                const int l1 = l0 + (qb[c] - qa[c]);
+#pragma unroll 4
+                for (int l = l0; l < l1; ++l) {
+                    float sqd = sqrlength(st.x[l], st.y[l], st.z[l], me.x, me.y, me.z);
+                    float weight = 1.0f * (POW2 ? smooth_pow2(sqd, yhh) : smooth_fixed(sqd, hh, yhh));
+                    weightsum = weightsum + weight;
+                }
+            }
+        }
+    } else {
+        for (int sx = x0; sx <= x1; ++sx) {
+            for (int sy = y0; sy <= y1; ++sy) {
+                const int64_t col = cidx(N, sx, sy, 0);
+                const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
+                for (int32_t q = q0; q < q1; ++q) {
+                    const float4 o = pos[q];
+                    float sqd = sqrlength(o.x, o.y, o.z, me.x, me.y, me.z);
+                    float weight = 1.0f * (POW2 ? smooth_pow2(sqd, yhh) : smooth_fixed(sqd, hh, yhh));
+                    weightsum = weightsum + weight;
+                }
+            }
+        }
+    }
+    return weightsum;
+}
+
 struct KDensity {
     static constexpr int kThreads = kStageBlock;
     struct Args {
@@ -472,36 +526,10 @@ struct KDensity {
         const int z0 = max(cz - 1, 0), z1 = min(cz + 1, N - 1);
         stage_neighbourhood(st, valid, x0, x1, y0, y1, z0, z1, N, offsets, pos);
         if (!valid) return;
-        float weightsum = 0.0f;
-        if (st.ok) {
-            for (int sx = x0; sx <= x1; ++sx) {
-                for (int sy = y0; sy <= y1; ++sy) {
-                    const int64_t col = cidx(N, sx, sy, 0);
-                    const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
-                    const int l0 = staged_index(st, sx, sy, q0);
-                    const int l1 = l0 + (q1 - q0);
-#pragma unroll 4
-                    for (int l = l0; l < l1; ++l) {
-                        float sqd = sqrlength(st.x[l], st.y[l], st.z[l], me.x, me.y, me.z);
-                        float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
-                        weightsum = weightsum + weight;
-                    }
-                }
-            }
-        } else {
-            for (int sx = x0; sx <= x1; ++sx) {
-                for (int sy = y0; sy <= y1; ++sy) {
-                    const int64_t col = cidx(N, sx, sy, 0);
-                    const int32_t q0 = offsets[col + z0], q1 = offsets[col + z1 + 1];
-                    for (int32_t q = q0; q < q1; ++q) {
-                        const float4 o = pos[q];
-                        float sqd = sqrlength(o.x, o.y, o.z, me.x, me.y, me.z);
-                        float weight = 1.0f * smooth_fixed(sqd, hh, yhh);
-                        weightsum = weightsum + weight;
-                    }
-                }
-            }
-        }
+        // the radius is the same for every particle: a uniform branch
+        const float weightsum =
+            is_pow2_float(hh) ? density_sum<true>(st, me, x0, x1, y0, y1, z0, z1, N, offsets, pos, hh, yhh)
+                              : density_sum<false>(st, me, x0, x1, y0, y1, z0, z1, N, offsets, pos, hh, yhh);
         float4 v = vel[s];
         v.w = weightsum / p.max_density;
         if (apply_forces) {
@@ -595,6 +623,45 @@ __device__ __forceinline__ float p2g_weight(float dx, float dy, float dz, float 
     }
 }
 
+// One staged span [q0, q1) of a particle column added to the block's faces, in
+// list order. DX / DY: the column lies in the x / y faces' window (uniform over
+// the block); the z faces take every column, the cells of their window being
+// [rb0, rb1). Every operand of an iteration is read from LDS before any weight
+// is computed and no branch separates the components, so the LDS latency of one
+// iteration is paid once. Lanes whose face does not exist accumulate too and
+// never store. RE14: the Sharpen radius is the reference's 1.4 (sharpen_re14).
+template <bool TRI, bool RE14, bool DX, bool DY>
+__device__ __forceinline__ void p2g_span(const float (*__restrict__ sp)[kP2GCap], int32_t q0, int32_t q1,
+                                         int32_t c0, int32_t rb0, int32_t rb1, float fi, float fih,
+                                         float fj, float fjh, float fk, float fkh, float hh, float& sux,
+                                         float& swx, float& suy, float& swy, float& suz, float& swz) {
+    auto sharpen_w = [&](float r2) { return RE14 ? sharpen_re14(r2) : sharpen_nr(r2, hh); };
+    for (int32_t q = q0; q < q1; ++q) {
+        const int t = q - c0;
+        const float pz = sp[2][t];
+        const float s0 = sp[0][t], s1 = sp[1][t], s6 = TRI ? 0.0f : sp[6][t];
+        const float ux = DX ? sp[3][t] : 0.0f, uy = DY ? sp[4][t] : 0.0f, uz = sp[5][t];
+        // offsets to the faces (x face at (i, j+.5, k+.5), ...); for Sharpen r^2
+        // in the reference's order (dx^2 + dy^2) + dz^2, the first two terms staged
+        const float az = pz - fk, bz = pz - fkh;
+        const float bz2 = bz * bz;
+        if constexpr (DX) {
+            const float w = TRI ? p2g_weight<true>(s0 - fi, s1 - fjh, bz, hh) : 1.0f * sharpen_w(s0 + bz2);
+            sux += w * ux;
+            swx += w;
+        }
+        if constexpr (DY) {
+            const float w = TRI ? p2g_weight<true>(s0 - fih, s1 - fj, bz, hh) : 1.0f * sharpen_w(s1 + bz2);
+            suy += w * uy;
+            swy += w;
+        }
+        const float wz0 = TRI ? p2g_weight<true>(s0 - fih, s1 - fjh, az, hh) : 1.0f * sharpen_w(s6 + az * az);
+        const float wz = (q >= rb0 && q < rb1) ? wz0 : 0.0f;
+        suz += wz * uz;
+        swz += wz;
+    }
+}
+
 template <bool TRI>
 struct KP2gCols {
     static constexpr int kThreads = 64;
@@ -624,7 +691,11 @@ struct KP2gCols {
         float* __restrict__ duy = kargs.duy;
         float* __restrict__ duz = kargs.duz;
 
-        __shared__ float sp[6][kP2GCap];
+        // staged particles: the Sharpen kernel keeps, per particle, the first two
+        // terms of r^2 = (dx^2 + dy^2) + dz^2 for each face component (dx and dy
+        // are the same for all 64 faces of the block's column) and pz; the
+        // trilinear kernel keeps px, py, pz; then the velocity
+        __shared__ float sp[7][kP2GCap];
         const int N = p.N;
         const int nkc = (N + 1 + 63) / 64;
         // Blocks run over 8x8 tiles of face columns (i, j), k-chunks innermost, so
@@ -656,6 +727,7 @@ struct KP2gCols {
         const float RE = p.sharpen_re;
         const float hh = RE * RE;
         const bool tri = TRI;
+        const bool re14 = !TRI && hh == kSharpenHH;
         const float fi = (float)i, fj = (float)j, fk = (float)k;
         const float fih = fi + 0.5f, fjh = fj + 0.5f, fkh = fk + 0.5f;
         float swx = 0.0f, sux = 0.0f, swy = 0.0f, suy = 0.0f, swz = 0.0f, suz = 0.0f;
@@ -679,7 +751,7 @@ struct KP2gCols {
                 int32_t ra0 = 0, ra1 = 0, rb0 = 0, rb1 = 0;
                 if (kval && a0 <= a1) { ra0 = offsets[col + a0]; ra1 = offsets[col + a1 + 1]; }
                 if (kval && b0 <= b1) { rb0 = offsets[col + b0]; rb1 = offsets[col + b1 + 1]; }
-                const bool dox = vx && colx, doy = vy && coly, doz = vz && colz;
+                const bool inx = sx >= i - 1 && sx <= i, iny = sy >= j - 1 && sy <= j;
                 for (int32_t c0 = c_begin; c0 < c_end; c0 += kP2GCap) {
                     const int32_t c1 = min(c_end, c0 + kP2GCap);
                     __syncthreads();
@@ -687,8 +759,18 @@ struct KP2gCols {
                         const float4 o = pos[q];
                         const float4 v = vel[q];
                         const int t = q - c0;
-                        sp[0][t] = gmax(0.0f, gmin(maxd, maxd * o.x));
-                        sp[1][t] = gmax(0.0f, gmin(maxd, maxd * o.y));
+                        const float px = gmax(0.0f, gmin(maxd, maxd * o.x));
+                        const float py = gmax(0.0f, gmin(maxd, maxd * o.y));
+                        if constexpr (TRI) {
+                            sp[0][t] = px;
+                            sp[1][t] = py;
+                        } else {
+                            // x face (i, j+.5, k+.5), y face (i+.5, j, k+.5), z face (i+.5, j+.5, k)
+                            const float ax = px - fi, bx = px - fih, ay = py - fj, by = py - fjh;
+                            sp[0][t] = ax * ax + by * by;
+                            sp[1][t] = bx * bx + ay * ay;
+                            sp[6][t] = bx * bx + by * by;
+                        }
                         sp[2][t] = gmax(0.0f, gmin(maxd, maxd * o.z));
                         sp[3][t] = v.x;
                         sp[4][t] = v.y;
@@ -696,28 +778,17 @@ struct KP2gCols {
                     }
                     __syncthreads();
                     const int32_t q0 = max(ra0, c0), q1 = min(ra1, c1);
-                    for (int32_t q = q0; q < q1; ++q) {
-                        const int t = q - c0;
-                        const float px = sp[0][t], py = sp[1][t], pz = sp[2][t];
-                        // offsets to the three faces (x face at (i, j+.5, k+.5), ...)
-                        const float ax = px - fi, bx = px - fih;
-                        const float ay = py - fj, by = py - fjh;
-                        const float az = pz - fk, bz = pz - fkh;
-                        if (dox) {
-                            const float w = p2g_weight<TRI>(ax, by, bz, hh);
-                            sux += w * sp[3][t];
-                            swx += w;
-                        }
-                        if (doy) {
-                            const float w = p2g_weight<TRI>(bx, ay, bz, hh);
-                            suy += w * sp[4][t];
-                            swy += w;
-                        }
-                        if (doz) {
-                            const float w = (q >= rb0 && q < rb1) ? p2g_weight<TRI>(bx, by, az, hh) : 0.0f;
-                            suz += w * sp[5][t];
-                            swz += w;
-                        }
+                    const int32_t b0c = rb0, b1c = rb1;
+                    if (re14) {
+                        if (inx && iny) p2g_span<TRI, true, true, true>(sp, q0, q1, c0, b0c, b1c, fi, fih, fj, fjh, fk, fkh, hh, sux, swx, suy, swy, suz, swz);
+                        else if (inx) p2g_span<TRI, true, true, false>(sp, q0, q1, c0, b0c, b1c, fi, fih, fj, fjh, fk, fkh, hh, sux, swx, suy, swy, suz, swz);
+                        else if (iny) p2g_span<TRI, true, false, true>(sp, q0, q1, c0, b0c, b1c, fi, fih, fj, fjh, fk, fkh, hh, sux, swx, suy, swy, suz, swz);
+                        else p2g_span<TRI, true, false, false>(sp, q0, q1, c0, b0c, b1c, fi, fih, fj, fjh, fk, fkh, hh, sux, swx, suy, swy, suz, swz);
+                    } else {
+                        if (inx && iny) p2g_span<TRI, false, true, true>(sp, q0, q1, c0, b0c, b1c, fi, fih, fj, fjh, fk, fkh, hh, sux, swx, suy, swy, suz, swz);
+                        else if (inx) p2g_span<TRI, false, true, false>(sp, q0, q1, c0, b0c, b1c, fi, fih, fj, fjh, fk, fkh, hh, sux, swx, suy, swy, suz, swz);
+                        else if (iny) p2g_span<TRI, false, false, true>(sp, q0, q1, c0, b0c, b1c, fi, fih, fj, fjh, fk, fkh, hh, sux, swx, suy, swy, suz, swz);
+                        else p2g_span<TRI, false, false, false>(sp, q0, q1, c0, b0c, b1c, fi, fih, fj, fjh, fk, fkh, hh, sux, swx, suy, swy, suz, swz);
                     }
                 }
             }

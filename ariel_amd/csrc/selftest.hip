@@ -26,9 +26,14 @@ __global__ void k_div_check(int which, float a, uint32_t lo, uint32_t hi,
                 same = __float_as_uint(gmax(1.0f - ref, 0.0f)) ==
                        __float_as_uint(gmax(1.0f - got, 0.0f));
             }
-        } else {
+        } else if (which == 1) {
             const float ref = a / x;
             const float got = div_nr(a, x);
+            same = __float_as_uint(ref) == __float_as_uint(got);
+        } else {
+            // the numerator is fixed (kSharpenHH); a is unused
+            const float ref = kSharpenHH / x;
+            const float got = div_sharpen_hh(x);
             same = __float_as_uint(ref) == __float_as_uint(got);
         }
         bad += same ? 0 : 1;

@@ -413,7 +413,9 @@ int ariel_trainer_forward(ariel_trainer* t, int32_t N, int32_t batch, const floa
  * lo <= x <= hi (lo >= 0):
  *   which = 0: x / a by the fixed-divisor routine (ComputeDensity's Smooth);
  *              for x < 2^-100 the compared value is max(1 - x/a, 0);
- *   which = 1: a / x by the reciprocal-based routine (Sharpen).
+ *   which = 1: a / x by the reciprocal-based routine (Sharpen);
+ *   which = 2: 1.96f / x (the Sharpen numerator RE^2 = 1.4^2 of P2G) by the
+ *              one-correction routine P2G uses for that radius; a is unused.
  * *mismatches receives the number of x whose result differs in any bit. */
 int ariel_selftest_division(int32_t which, float a, float lo, float hi, int64_t* mismatches);
 

@@ -3,7 +3,8 @@
 The density kernel divides by a fixed h*h (Smooth, kernels.inl:32-34) with a
 Markstein-corrected product by 1/(h*h); P2G and the resampler divide a fixed h*h
 by max(r2, 1e-5) (Sharpen, kernels.inl:36-38) with a reciprocal-based
-Newton-Raphson sequence. Both must return the IEEE quotient bit for bit, since
+Newton-Raphson sequence (P2G at the reference radius 1.4 with the hardware
+reciprocal and one correction). All must return the IEEE quotient bit for bit, since
 the stage outputs are compared bit-exactly with the oracle. Every float in each
 range is tried on the device; the divisors are those of the dam-break and pool
 workloads (density 0.5 at the listed grid sizes) plus random ones.
@@ -62,6 +63,14 @@ def _sharpen_numerators():
 def test_sharpen_divide_matches_ieee(hh):
     # every float divisor max(r2, 1e-5) in [1e-5, 4]
     assert _check(1, hh, _f32(1e-5), 4.0) == 0
+
+
+def test_p2g_sharpen_divide_matches_ieee():
+    """P2G's Sharpen at the reference radius (RE = 1.4, particlegridoperations.inl:149)
+    divides 1.96f by max(r2, 1e-5) with the hardware reciprocal and one residual
+    correction: equal to IEEE division for every float divisor in [1e-5, 4]
+    (beyond 4 the quotient is below 1/2, where the weight is 0 either way)."""
+    assert _check(2, 1.0, _f32(1e-5), 4.0) == 0
 
 
 def test_division_check_detects_a_wrong_divisor_range():
