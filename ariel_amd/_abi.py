@@ -62,7 +62,7 @@ class StepStats(ctypes.Structure):
                 ("max_divergence", ctypes.c_double), ("max_density", ctypes.c_double),
                 ("stage_ms", ctypes.c_double * NUM_STAGES), ("step_ms", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("precond_fallback", ctypes.c_int32),
-                ("fallback_iter", ctypes.c_int32)]
+                ("fallback_iter", ctypes.c_int32), ("true_residual_norm64", ctypes.c_double)]
 
 
 class KernelStats(ctypes.Structure):

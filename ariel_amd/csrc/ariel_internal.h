@@ -53,6 +53,7 @@ struct SolverScalars {
     int32_t fallback_it;   // iteration at which flexible PCG switched to MIC(0)
     int32_t fallback_reason;  // 1: slow progress, 2: not a descent direction
     double rnorm2_0;       // ||r||^2 of the solve's initial residual
+    double tmp64;          // ||b - A x64||^2 of the FP64 solution (report)
 };
 
 // Cells the pressure solve touches. A z-segment is 8 cells (cells 8s..8s+7) and
@@ -279,6 +280,12 @@ void launch_pr_dot(hipStream_t s, const float* z, const float* r, const float* r
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                                const float* P, const float* D, const Reduce& red,
                                const Active& act);
+// End of a solve with reliable updates: x64 += p, p = 0, then ||b - A x64||^2 of
+// the FP64 solution in FP64 into sc->tmp64 (R untouched). launch_refine_finish
+// follows and rounds x64 to the pressure.
+void launch_true_residual_norm64(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                                 float* P, double* P64, const float* D, const Reduce& red,
+                                 const Active& act);
 
 // reduction helpers
 int reduce_blocks(int64_t n, int per_thread);

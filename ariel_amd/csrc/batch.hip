@@ -109,7 +109,10 @@ void record_post(ariel_sim* h, Recorder& r) {
     const Params p = h->params();
     const Reduce red = h->reduce();
     hipStream_t s = h->stream;
-    if (h->d.refine_every > 0) launch_refine_finish(s, h->A, h->P, h->P64, h->act, h->sc);
+    if (h->d.refine_every > 0) {
+        launch_true_residual_norm64(s, p, h->A, h->L, h->P, h->P64, h->D, red, h->act);
+        launch_refine_finish(s, h->A, h->P, h->P64, h->act, h->sc);
+    }
     launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, h->act);
     stage_gradient(h);
     stage_extrapolate(h);
@@ -265,6 +268,8 @@ int batch_step_once(ariel_batch* B) {
         h->stats.rhs_norm = std::sqrt(sc.bnorm2);
         h->stats.residual_norm = std::sqrt(sc.rnorm2);
         h->stats.true_residual_norm = std::sqrt(sc.tmp);
+        h->stats.true_residual_norm64 =
+            h->d.refine_every > 0 ? std::sqrt(sc.tmp64) : h->stats.true_residual_norm;
         h->stats.fluid_cells = sc.fluid_cells;
         float md;
         uint32_t bits = sc.maxdiv_bits;

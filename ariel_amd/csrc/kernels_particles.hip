@@ -402,11 +402,13 @@ __device__ void stage_neighbourhood(StageLds& st, bool valid, int x0, int x1, in
     if (!st.ok) return;
     const int total = st.total;
     float4 t[kStagePer];
+    // a thread's slots u increase with m, so its column search resumes where the
+    // previous one stopped
+    int c = 0;
 #pragma unroll
     for (int m = 0; m < kStagePer; ++m) {
         const int u = (int)threadIdx.x + m * kStageBlock;
         if (u < total) {
-            int c = 0;
             while (c + 1 < ncols && st.base[c + 1] <= u) ++c;
             t[m] = pos[st.q0[c] + (u - st.base[c])];
         }
@@ -1022,8 +1024,9 @@ struct KResample {
             // the reference's order (cx, then cy, ascending) from these lists, so a
             // wave iterates about 2 x 2 columns instead of 3 x 3 of which each lane
             // needs some.
-            int cxs[3], cys[3], nx = 0, ny = 0, za0 = 0, zb0 = -1;
+            int cxs[3], cys[3], nx = 0, ny = 0;
             float gxs[3], gys[3], gzs[3];
+            bool zin[3];
     #pragma unroll
             for (int m = 0; m < 3; ++m) {
                 const int c = xl + m;
@@ -1038,55 +1041,65 @@ struct KResample {
             }
     #pragma unroll
             for (int m = 0; m < 3; ++m) {
-                const int c = zl + m;
-                const float g = cell_gap(zg, c, margin);
-                gzs[m] = g;
-                if (c <= zh && g * g <= gbound) {
-                    if (zb0 < za0) za0 = c;
-                    zb0 = c;
-                }
+                gzs[m] = cell_gap(zg, zl + m, margin);
+                zin[m] = zl + m <= zh;
             }
+            // The candidate ranges of the columns, (cx, cy) in the reference's order,
+            // each narrowed to the z cells within the bound (a contiguous run: the
+            // bound grows away from the point), are fetched together before the scan.
+            int32_t cq0[9], cq1[9];
     #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                if (a >= nx) break;
     #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    if (b >= ny) break;
-                    const int cx = cxs[a], cy = cys[b];
+                    const int slot = 3 * a + b;
+                    cq0[slot] = cq1[slot] = 0;
+                    if (a >= nx || b >= ny) continue;
                     const float gxy2 = gxs[a] * gxs[a] + gys[b] * gys[b];
-                    if (gxy2 > gbound) continue;
-                    // z cells of this column within the bound (ends of [za0, zb0])
-                    int za = za0, zb = zb0;
-                    while (za <= zb && gxy2 + gzs[za - zl] * gzs[za - zl] > gbound) ++za;
-                    while (zb >= za && gxy2 + gzs[zb - zl] * gzs[zb - zl] > gbound) --zb;
+                    int za = zh + 1, zb = zl - 1;
+    #pragma unroll
+                    for (int m = 2; m >= 0; --m)
+                        if (zin[m] && gxy2 + gzs[m] * gzs[m] <= gbound) za = zl + m;
+    #pragma unroll
+                    for (int m = 0; m < 3; ++m)
+                        if (zin[m] && gxy2 + gzs[m] * gzs[m] <= gbound) zb = zl + m;
                     if (za > zb) continue;
-                    const int64_t col = cidx(N, cx, cy, 0);
-                    const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
-                    if (staged) {
-                        const int l0 = staged_index(st, cx, cy, q0);
-                        for (int32_t q = q0, l = l0; q < q1; ++q, ++l) {
-                            if (q == (int32_t)s) continue;
-                            const float dx = me.x - st.x[l], dy = me.y - st.y[l], dz = me.z - st.z[l];
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and
-                            // then dist > 0.1 re: the term adds +-0 to the spring. d2
-                            // above re^2 by a margin far beyond the rounding of sqrt and
-                            // the square guarantees that, so the pair is skipped without
-                            // changing a bit.
-                            if (d2 > re2_skip) continue;
-                            s_queue[queued][threadIdx.x] = l;
-                            if (++queued == kSpringQueue) flush();
-                        }
-                    } else {
-                        for (int32_t q = q0; q < q1; ++q) {
-                            if (q == (int32_t)s) continue;
-                            const float4 o = pos[q];
-                            const float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            if (d2 > re2_skip) continue;
-                            s_queue[queued][threadIdx.x] = q;
-                            if (++queued == kSpringQueue) flush();
-                        }
+                    const int64_t col = cidx(N, cxs[a], cys[b], 0);
+                    cq0[slot] = offsets[col + za];
+                    cq1[slot] = offsets[col + zb + 1];
+                }
+            }
+            // Candidates within re (bit-exact skip, below) are queued by LDS slot
+            // (staged) or particle index; the queue write is unconditional and only
+            // a taken candidate advances it, so the scan has one rare branch.
+    #pragma unroll
+            for (int slot = 0; slot < 9; ++slot) {
+                const int32_t q0 = cq0[slot], q1 = cq1[slot];
+                if (q0 >= q1) continue;
+                if (staged) {
+                    const int l0 = staged_index(st, cxs[slot / 3], cys[slot % 3], q0);
+                    for (int32_t q = q0, l = l0; q < q1; ++q, ++l) {
+                        const float dx = me.x - st.x[l], dy = me.y - st.y[l], dz = me.z - st.z[l];
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        // Smooth(dist^2, re) is exactly 0 once dist^2 >= re^2, and
+                        // then dist > 0.1 re: the term adds +-0 to the spring. d2
+                        // above re^2 by a margin far beyond the rounding of sqrt and
+                        // the square guarantees that, so the pair is skipped without
+                        // changing a bit.
+                        const bool take = q != (int32_t)s && d2 <= re2_skip;
+                        s_queue[queued][threadIdx.x] = l;
+                        queued += take ? 1 : 0;
+                        if (queued == kSpringQueue) flush();
+                    }
+                } else {
+                    for (int32_t q = q0; q < q1; ++q) {
+                        const float4 o = pos[q];
+                        const float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const bool take = q != (int32_t)s && d2 <= re2_skip;
+                        s_queue[queued][threadIdx.x] = q;
+                        queued += take ? 1 : 0;
+                        if (queued == kSpringQueue) flush();
                     }
                 }
             }

@@ -487,6 +487,7 @@ struct KFlush {
         const int32_t* segs;
         int64_t nseg;
         const SolverScalars* sc;
+        int final_flush;   // 1: the end of the solve (every cell, done or not)
     };
     static __device__ void run(const Args& kargs) {
         const uint8_t* __restrict__ A = kargs.A;
@@ -496,7 +497,7 @@ struct KFlush {
         int64_t nseg = kargs.nseg;
         const SolverScalars* sc = kargs.sc;
 
-        if (sc->done || !refine_due(sc)) return;
+        if (!kargs.final_flush && (sc->done || !refine_due(sc))) return;
         nseg = seg_count(nseg, sc);
         const int64_t nb = list_grid(nseg);
         if ((int64_t)blockIdx.x >= nb) return;
@@ -512,6 +513,7 @@ struct KFlush {
 
 // mode 0 (reliable update, part 2): r = b - A x64 in FP64, stored in FP32, then the
 // stop test. mode 1 (report): ||b - A p||^2 of the FP32 pressure into sc->tmp.
+// mode 3 (report): ||b - A x64||^2 of the FP64 solution into sc->tmp64.
 struct KTrueResidual {
     static constexpr int kThreads = kBlock;
     struct Args {
@@ -556,7 +558,7 @@ struct KTrueResidual {
             }
             int i, j, k;
             decode_cell(N, c, i, j, k);
-            const double ax = mode == 0 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
+            const double ax = mode != 1 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
                                         : spmv_cell_f64(A, L, P, N, i, j, k, c);
             const double r = (double)D[c] - ax;
             v[0] += r * r;
@@ -693,7 +695,7 @@ __device__ __forceinline__ int64_t parts_from_counts(int code, const SolverScala
 }
 
 enum { FIN_RESID0 = 0, FIN_SIGMA = 1, FIN_RNORM = 2, FIN_TRUE = 3, FIN_REPORT = 4, FIN_RHO = 5,
-       FIN_PR = 6, FIN_PR_CHECK = 7 };
+       FIN_PR = 6, FIN_PR_CHECK = 7, FIN_REPORT64 = 8 };
 // Flexible PCG falls back to MIC(0) when, over a window of this many
 // iterations, ||r|| fell by less than 5% per iteration on average.
 // Flexible PCG's progress test: over every window of kFlexWindow iterations the
@@ -803,6 +805,9 @@ struct KFinalize {
                 break;
             case FIN_REPORT:
                 sc->tmp = s0;
+                break;
+            case FIN_REPORT64:
+                sc->tmp64 = s0;
                 break;
             case FIN_RHO:
                 finish_rho(sc, s0);
@@ -1793,6 +1798,15 @@ void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A,
     const int nb = (int)list_grid(seg_or_cap(act));
     launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, P, (const double*)nullptr, D, (float*)nullptr, act.segs, act.nseg, red, 1});
     launch_finalize(s, red, FIN_REPORT, act.nseg < 0 ? PARTS_LIST : nb);
+}
+
+void launch_true_residual_norm64(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+                                 float* P, double* P64, const float* D, const Reduce& red,
+                                 const Active& act) {
+    const int nb = (int)list_grid(seg_or_cap(act));
+    launch<KFlush>(s, dim3(nb), dim3(kBlock), {A, P, P64, act.segs, act.nseg, (const SolverScalars*)red.sc, 1});
+    launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, (const float*)P, (const double*)P64, D, (float*)nullptr, act.segs, act.nseg, red, 3});
+    launch_finalize(s, red, FIN_REPORT64, act.nseg < 0 ? PARTS_LIST : nb);
 }
 
 void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,

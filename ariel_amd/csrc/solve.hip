@@ -139,7 +139,10 @@ int stage_solve(ariel_sim* h) {
         if (h->sc_host->done || queued >= h->d.max_iters) break;
         chunk = plan.next(*h->sc_host);
     }
-    if (refine) launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
+    if (refine) {
+        launch_true_residual_norm64(s, p, h->A, h->L, h->P, h->P64, h->D, red, act);
+        launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
+    }
     HIPCHK(h, hipEventRecord(h->solve_t1, s));
     h->last_iters = h->sc_host->it;
     h->stats.iterations = h->sc_host->it;
@@ -151,6 +154,7 @@ int stage_solve(ariel_sim* h) {
     TRY(check_launch(h, "true residual"));
     TRY(read_scalars(h));
     h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
+    h->stats.true_residual_norm64 = refine ? std::sqrt(h->sc_host->tmp64) : h->stats.true_residual_norm;
     return ARIEL_OK;
 }
 
@@ -224,7 +228,10 @@ int stage_solve_learned(ariel_sim* h) {
         if (h->sc_host->done || queued >= h->d.max_iters) break;
         chunk = plan.next(*h->sc_host);
     }
-    if (refine) launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
+    if (refine) {
+        launch_true_residual_norm64(s, p, h->A, h->L, h->P, h->P64, h->D, red, act);
+        launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
+    }
     HIPCHK(h, hipEventRecord(h->solve_t1, s));
     h->last_iters = h->sc_host->it;
     h->stats.iterations = h->sc_host->it;
@@ -237,6 +244,7 @@ int stage_solve_learned(ariel_sim* h) {
     TRY(check_launch(h, "true residual"));
     TRY(read_scalars(h));
     h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
+    h->stats.true_residual_norm64 = refine ? std::sqrt(h->sc_host->tmp64) : h->stats.true_residual_norm;
     return ARIEL_OK;
 }
 

@@ -11,12 +11,19 @@ reductions and a true-residual recompute every 25 iterations. A step is one full
 FlipSim::Step (sort, density, P2G, classification, projection, G2P/advection,
 resampling) with every input resident in HBM.
 
-Prints one JSON line (rank 0). `roofline` is the dominant kernel's algorithmic
-HBM bytes per launch over its mean launch time, timed with HIP events on the
-simulation stream during the timed steps. `cpu_baseline` times the CPU oracle
-(the serial-order C++ restatement, thread pool over the host cores) on the same
-state: one full step with its PCG capped at a few iterations, the per-iteration
-time scaled to the GPU's mean iteration count.
+Prints one JSON line (rank 0). `roofline` is the MIC(0) preconditioner apply
+(the solver's dominant HBM-bound kernel group): its algorithmic HBM bytes per
+launch over its mean launch time, timed with HIP events on the simulation stream
+during the timed steps; `traffic` is its HBM traffic per launch from the
+calibrated counter profile committed in profiles/ (FETCH_SIZE and WRITE_SIZE).
+`cpu_baseline` is one full step of the reference's algorithm (the oracle in
+faithful mode: the reference's preconditioner, FP32 serial dots and absolute stop
+rule; serial Sort/Product, parallel loops on a thread pool) on the GPU's state,
+timed on the host threads this process may use; `cpu_port_extrapolated` is the
+oracle's restatement of the GPU algorithm with its PCG capped and scaled.
+The 1e-6 of the metric is the PCG stop rule on the recursive residual; the true
+residuals of the FP64 solution and of the FP32 pressure are reported beside it.
+`impact_phase` times steps 56-100, where the cube meets the pool.
 """
 import argparse
 import json
@@ -47,6 +54,8 @@ def parse():
     ap.add_argument("--precond", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--no-impact", dest="impact", action="store_false",
+                    help="skip the impact-phase timing (steps 56-100)")
     ap.add_argument("--batched", action="store_true",
                     help="the batched workload instead: 64 tanks (edges from {48,64,80,96}, "
                          "seed 2), aggregate tank-steps/s of batched steps vs a per-tank loop")
@@ -64,6 +73,30 @@ def parse():
                     help="only run 5 device-to-device copies of MIB MiB (a plain copy "
                          "kernel of known traffic, to scale FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def profile_traffic(group):
+    """HBM traffic per launch (bytes) of a kernel group from the calibrated counter
+    summary profiles/flip_step_256_traffic.md (median FETCH_SIZE x 2 + WRITE_SIZE
+    per dispatch, MB = 1e6 bytes), and the rows it sums; (None, None) if absent."""
+    kernels = {"precond": ["KMicApply<0>", "KMicApply<1>", "KMicApply<2>"], "spmv": ["KSpmvDot"],
+               "p2g": ["KP2gCols<false>"], "axpy": ["KAxpy2"], "density": ["KDensity"],
+               "resample": ["KResample"], "advect": ["KG2pAdvect"]}.get(group)
+    path = os.path.join(HERE, "profiles", "flip_step_256_traffic.md")
+    if not kernels or not os.path.exists(path):
+        return None, None
+    rows = {}
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+            if len(cells) == 5 and cells[0] in kernels:
+                try:
+                    rows[cells[0]] = float(cells[4]) * 1e6
+                except ValueError:
+                    pass
+    if set(rows) != set(kernels):
+        return None, None
+    return sum(rows.values()), "profiles/flip_step_256_traffic.md: " + " + ".join(kernels)
 
 
 def copy_calibration(mib):
@@ -84,23 +117,82 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_per_launch": n}), flush=True)
 
 
-def cpu_baseline(sim, desc, gpu_iters, cpu_iters):
-    """Time the oracle (test infrastructure, used here only as the CPU baseline)
-    on the GPU's current state."""
-    from ariel_amd import _abi, scenes
-    from oracle import OracleSim
+def host_info():
+    """Host cores visible, cores this process may run on, and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "affinity": affinity, "cpu_model": model}
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    d = scenes.copy(desc)
-    d.max_iters = cpu_iters
-    o = OracleSim(d, threads=threads)
+
+def cpu_threads():
+    """Threads of the CPU baseline: the cores this process may use, at most 16
+    (a GPU box's CPU share per GPU)."""
+    return max(1, min(16, host_info()["affinity"]))
+
+
+def _load_state(o, sim):
     pos, vel, rho = sim.GetParticles()
     o.set_particles(pos, vel, rho)
     for f in range(9):
         o.set_field(f, sim.get_field(f))
     o.set_flags(sim.get_flags())
     o.max_density = sim.max_density
-    del pos, vel, rho
+    o.frame = sim.m_frame
+
+
+def cpu_reference_step(sim, desc):
+    """One full step of the reference's algorithm (the oracle in faithful mode,
+    test infrastructure used here only as the CPU baseline) on the GPU's state."""
+    from oracle import OracleSim
+    from oracle.oracle import FAITHFUL
+
+    threads = cpu_threads()
+    o = OracleSim(desc, threads=threads)
+    _load_state(o, sim)
+    t0 = time.perf_counter()
+    o.step(mode=FAITHFUL)
+    wall = time.perf_counter() - t0
+    st = o.stats()
+    o.close()
+    h = host_info()
+    return {
+        "value": 1.0 / wall,
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"one full step of the reference's algorithm (oracle, faithful mode: the "
+                   f"reference's preconditioner, FP32 serial dots, stop r.r <= 1e-2 N^3) on the "
+                   f"GPU state at {desc.dims[0]}^3 after step {sim.m_frame}: {wall:.1f} s wall, PCG "
+                   f"{st.solve_ms / 1e3:.1f} s for {st.iterations} iterations, on {threads} threads"),
+        "nproc": h["nproc"],
+        "affinity": h["affinity"],
+        "cpu_model": h["cpu_model"],
+    }
+
+
+def cpu_baseline(sim, desc, gpu_iters, cpu_iters):
+    """The oracle's restatement of the GPU algorithm (relative stop, MIC(0), FP64
+    dots) on the GPU's current state: one step with its PCG capped at cpu_iters
+    iterations, the per-iteration time scaled to the GPU's mean iteration count."""
+    from ariel_amd import scenes
+    from oracle import OracleSim
+
+    threads = cpu_threads()
+    d = scenes.copy(desc)
+    d.max_iters = cpu_iters
+    o = OracleSim(d, threads=threads)
+    _load_state(o, sim)
     t0 = time.perf_counter()
     o.step()
     wall = time.perf_counter() - t0
@@ -292,7 +384,7 @@ def _cpu_reference(desc, threads, steps):
 def table(args):
     """Configurations 1-5 side by side (see _gpu_steps and _cpu_reference)."""
     from ariel_amd import FlipBatch, _abi, scenes
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = cpu_threads()
     learned_kw = {"precond": _abi.PRECOND_LEARNED}
     cfgs = [
         ("1", "dam break 32^3, 50 steps", lambda **k: scenes.dam_break(32, **k),
@@ -383,7 +475,7 @@ def main():
     log(f"warmup done ({args.warmup} steps)")
     sim.set_profiling(True)
     sim.reset_kernel_stats()
-    iters, solve_ms, status, true_rel = [], [], [], []
+    iters, solve_ms, status, true_rel, true_rel64 = [], [], [], [], []
     sim.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -393,6 +485,7 @@ def main():
         solve_ms.append(st.solve_ms)
         status.append(st.solve_status)
         true_rel.append(st.true_residual_norm / max(st.rhs_norm, 1e-300))
+        true_rel64.append(st.true_residual_norm64 / max(st.rhs_norm, 1e-300))
     sim.sync()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
@@ -412,12 +505,37 @@ def main():
             "bytes_per_launch": ks.bytes_per_launch, "achieved_GBs": gbs,
             "frac": gbs / HBM_PEAK_GBS,
         }
-    dom_name = max(kernels, key=lambda n: kernels[n]["total_ms"])
+    dom_name = "precond" if "precond" in kernels else max(kernels, key=lambda n: kernels[n]["total_ms"])
     dom = kernels[dom_name]
+    traffic, traffic_src = profile_traffic(dom_name)
     roofline = {
         "bound": "hbm", "kernel": dom_name, "achieved": dom["achieved_GBs"],
-        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["frac"], "traffic": None,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["frac"], "traffic": traffic,
     }
+    if traffic_src:
+        roofline["traffic_source"] = traffic_src
+
+    # the impact of the cube on the pool (steps 56-100 of the 100-step workload),
+    # timed without profiling events
+    sim.set_profiling(False)
+    impact = None
+    if args.impact and args.n == 256:
+        done = args.warmup + args.steps
+        while done < 55:
+            sim.Step(1)
+            done += 1
+        sim.sync()
+        t1 = time.perf_counter()
+        it_imp = []
+        for _ in range(100 - done):
+            sim.Step(1)
+            it_imp.append(sim.stats().iterations)
+        sim.sync()
+        n_imp = len(it_imp)
+        if n_imp:
+            impact = {"steps": f"{done + 1}-100", "ms_per_step": (time.perf_counter() - t1) * 1e3 / n_imp,
+                      "pcg_iterations": float(np.mean(it_imp))}
+            log(f"impact phase: steps {done + 1}-100 at {impact['ms_per_step']:.2f} ms/step")
     out = {
         "metric": METRIC,
         "value": 1e3 / ms_per_step,
@@ -442,7 +560,10 @@ def main():
         "pcg_solve_ms": float(np.mean(solve_ms)),
         "pcg_iterations": float(np.mean(iters)),
         "pcg_all_converged": all(s == 0 for s in status),
+        "pcg_stop_rule": "recursive residual <= 1e-6 * ||b|| (FP64 norms)",
         "pcg_true_rel_residual_max": float(np.max(true_rel)),
+        "pcg_true_rel_residual_fp64_solution_max": float(np.max(true_rel64)),
+        "impact_phase": impact,
         "p2g_hbm_frac": kernels.get("p2g", {}).get("frac"),
         "spmv_hbm_frac": kernels.get("spmv", {}).get("frac"),
         "roofline": roofline,
@@ -450,8 +571,10 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        log("cpu baseline: one oracle step on the same state")
-        out["cpu_baseline"] = cpu_baseline(sim, desc, float(np.mean(iters)), args.cpu_iters)
+        log("cpu baseline: one step of the reference's algorithm on the same state")
+        out["cpu_baseline"] = cpu_reference_step(sim, desc)
+        log("cpu baseline: the GPU algorithm's restatement, PCG capped and scaled")
+        out["cpu_port_extrapolated"] = cpu_baseline(sim, desc, float(np.mean(iters)), args.cpu_iters)
         log("cpu baseline done")
     sim.close()
     print(json.dumps(out), flush=True)

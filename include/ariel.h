@@ -169,6 +169,9 @@ typedef struct ariel_step_stats {
                                    1 switched to MIC(0) for slow progress, 2 for a network
                                    output that was no descent direction */
     int32_t fallback_iter;      /* iteration of that switch (0 if none) */
+    double  true_residual_norm64; /* ||b - A x||_2 of the FP64 solution x of a solve with
+                                     reliable updates, before its rounding to the FP32
+                                     pressure (equal to true_residual_norm otherwise) */
 } ariel_step_stats;
 
 typedef struct ariel_kernel_stats {

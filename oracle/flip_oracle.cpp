@@ -923,13 +923,17 @@ void solve_extended(oracle_sim& s) {
     } else {
         status = 0;
     }
-    if (kref > 0)
+    if (kref > 0) {
         for (int64_t c = 0; c < n; ++c)
-            s.P[c] = (s.A[c] == FLUID) ? (float)(X64[c] + (double)s.P[c]) : 0.0f;
+            if (s.A[c] == FLUID) X64[c] += (double)s.P[c];
+        s.st.true_residual_norm64 = std::sqrt(true_residual_d(s, X64.data(), R.data(), *s.pool));
+        for (int64_t c = 0; c < n; ++c) s.P[c] = (s.A[c] == FLUID) ? (float)X64[c] : 0.0f;
+    }
     s.st.iterations = it;
     s.st.solve_status = status;
     s.st.residual_norm = std::sqrt(rnorm2);
     s.st.true_residual_norm = std::sqrt(true_residual(s, s.P.data(), R.data(), *s.pool));
+    if (kref <= 0) s.st.true_residual_norm64 = s.st.true_residual_norm;
 }
 
 // SubtractPressureGradient (flip.cpp:568-680), subcell = 1.

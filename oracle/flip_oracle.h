@@ -72,6 +72,7 @@ typedef struct oracle_stats {
     double  solve_ms;
     int32_t precond_fallback;   /* always 0: the oracle has no learned preconditioner */
     int32_t pad_;
+    double  true_residual_norm64; /* FP64 solution of a refined solve, before rounding */
 } oracle_stats;
 
 typedef struct oracle_sim oracle_sim;

@@ -28,7 +28,7 @@ class OracleStats(ctypes.Structure):
                 ("max_divergence", ctypes.c_double), ("max_density", ctypes.c_double),
                 ("stage_ms", ctypes.c_double * 10), ("step_ms", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("precond_fallback", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("pad_", ctypes.c_int32), ("true_residual_norm64", ctypes.c_double)]
 
 
 def build():
