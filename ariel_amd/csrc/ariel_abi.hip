@@ -44,22 +44,26 @@ hipEvent_t take_event(ariel_sim* h) {
 }
 
 // Algorithmic HBM bytes of one launch group: each input read once and each
-// output written once. The solver groups count the cells they touch: the
-// active segments (8 cells each) and, for the preconditioner, the active tiles
-// (512 cells each; r and pc read, z written, and q written and read again on
-// the red half).
+// output written once. The solver groups count the cells of the active tiles
+// (512 each) that they sweep: the operator reads z and s_prev and writes s and
+// y (16 B) and reads the FLUID bits (1 B per 8 cells, counted as 1 B per cell
+// with the class reads at the free surface); the preconditioner reads r and
+// pc, writes z, and writes and reads again q on the red half (20 B); the AXPY
+// reads p, s, r, y and writes p, r (24 B). With the tiled red-black MIC(0) the
+// AXPY rides on the preconditioner's first pass over each colour, where r is
+// read once for both: that group then moves 20 + 24 - 4 = 40 B per cell.
 double kernel_bytes(const ariel_sim* h, int k) {
     const double np = (double)h->np, nc = (double)h->nc;
-    const double seg_cells = 8.0 * (double)h->act.nseg;
     const double tile_cells = 512.0 * (double)h->act.ntiles[2];
     switch (k) {
         case ARIEL_KERNEL_SORT: return 60.0 * np + 8.0 * nc;
         case ARIEL_KERNEL_DENSITY: return 40.0 * np;
         case ARIEL_KERNEL_MARK: return 4.0 * np + 9.0 * nc;
         case ARIEL_KERNEL_P2G: return 24.0 * np + 12.0 * nc;
-        case ARIEL_KERNEL_SPMV: return 9.0 * seg_cells;
-        case ARIEL_KERNEL_PRECOND: return 20.0 * tile_cells;
-        case ARIEL_KERNEL_AXPY: return 24.0 * seg_cells;
+        case ARIEL_KERNEL_SPMV: return 17.0 * tile_cells;
+        case ARIEL_KERNEL_PRECOND:
+            return (h->d.precond == ARIEL_PRECOND_MIC0_TILED ? 40.0 : 20.0) * tile_cells;
+        case ARIEL_KERNEL_AXPY: return 24.0 * tile_cells;
         case ARIEL_KERNEL_GRADIENT: return 33.0 * nc;
         case ARIEL_KERNEL_EXTRAPOLATE: return 36.0 * nc;
         case ARIEL_KERNEL_ADVECT: return 48.0 * np + 24.0 * nc;
@@ -444,6 +448,8 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
     if ((rc = dalloc(h, &h->P, h->nc)) || (rc = dalloc(h, &h->D, h->nc)) || (rc = dalloc(h, &h->L, h->nc)) ||
         (rc = dalloc(h, &h->A, h->nc)) || (rc = dalloc(h, &h->R, h->nc)) || (rc = dalloc(h, &h->Z, h->nc)) ||
         (rc = dalloc(h, &h->S, h->nc)) || (rc = dalloc(h, &h->Q, h->nc)) || (rc = dalloc(h, &h->pc, 2 * h->nc)) ||
+        (rc = dalloc(h, &h->Rt, h->nc)) || (rc = dalloc(h, &h->Zt, h->nc)) || (rc = dalloc(h, &h->St, 2 * h->nc)) ||
+        (rc = dalloc(h, &h->Yt, h->nc)) || (rc = dalloc(h, &h->Pt, h->nc)) ||
         (rc = dalloc(h, &h->diag, h->nc)) || (rc = dalloc(h, &h->nup, h->nc)) ||
         (rc = dalloc(h, &h->fl8, h->nc / 8)) ||
         (rc = dalloc(h, &h->counts, h->nc)) || (rc = dalloc(h, &h->offsets, h->nc + 1)) ||
@@ -473,7 +479,8 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
     launch_fill_f32(s, h->P, h->nc, 0.0f);
     launch_fill_f32(s, h->D, h->nc, 0.0f);
     launch_fill_f32(s, h->L, h->nc, 1.6f);   // macgrid.inl:82
-    for (float* v : {h->R, h->Z, h->S, h->Q, h->diag}) launch_fill_f32(s, v, h->nc, 0.0f);
+    for (float* v : {h->R, h->Z, h->S, h->Q, h->diag, h->Rt, h->Zt, h->Yt, h->Pt}) launch_fill_f32(s, v, h->nc, 0.0f);
+    launch_fill_f32(s, h->St, 2 * h->nc, 0.0f);
     launch_fill_f32(s, h->pc, 2 * h->nc, 0.0f);   // natural layout, then the MIC(0) apply's tile-major copy
     launch_fill_i32(s, h->counts, h->nc, 0);
     launch_fill_i32(s, h->offsets, h->nc + 1, 0);
@@ -498,6 +505,7 @@ int ariel_destroy(ariel_sim* h) {
     for (void* p : {(void*)h->cell_of, (void*)h->slots, (void*)h->sid, (void*)h->counts, (void*)h->offsets,
                     (void*)h->scan_sums, (void*)h->accept, (void*)h->P, (void*)h->D, (void*)h->L,
                     (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
+                    (void*)h->Rt, (void*)h->Zt, (void*)h->St, (void*)h->Yt, (void*)h->Pt,
                     (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc,
                     (void*)h->seg_flag, (void*)h->seg_scan, (void*)h->seg_list, (void*)h->tile_any,
                     (void*)h->tile_lists, (void*)h->net_mem, (void*)h->R_old, (void*)h->net.wd})
@@ -790,7 +798,11 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
         launch_diag_weight(h->stream, h->N, h->A, h->diag, h->net.wd);
         launch_learned_apply(h->stream, h->net, h->N, h->R, h->A, h->diag, h->Z, h->sc, h->act.nseg);
     } else {
-        launch_precond_apply(h->stream, p, h->fl8, h->pc, h->R, h->Q, h->Z, h->reduce(), h->act);
+        // the apply works on the tile-major vectors over the active tiles; cells
+        // outside them are not FLUID and keep z = 0
+        launch_tile_convert(h->stream, h->N, h->R, h->Rt, h->act, h->sc, true, false);
+        launch_precond_apply(h->stream, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, h->reduce(), h->act);
+        launch_tile_convert(h->stream, h->N, h->Zt, h->Z, h->act, h->sc, false, false);
     }
     TRY(check_launch(h, "preconditioner"));
     HIPCHK(h, hipMemcpyAsync(z, h->Z, h->nc * sizeof(float), hipMemcpyDeviceToHost, h->stream));

@@ -75,8 +75,12 @@ struct ariel_sim {
     float *P = nullptr, *D = nullptr, *L = nullptr;
     uint8_t* A = nullptr;
     // pc: 2 N^3 floats, the factor in the natural layout and then the tile-major
-    // copy the MIC(0) apply reads; Q is the apply's intermediate, tile-major
+    // copy the preconditioner applies read. R, Z, S: natural-layout staging of
+    // the accessors and of the learned network's input and output. The solve's
+    // iteration vectors are tile-major: Rt, Zt, St (two buffers), Yt = A s, the
+    // iterate Pt, and Q, the MIC(0) apply's intermediate.
     float *R = nullptr, *Z = nullptr, *S = nullptr, *Q = nullptr, *pc = nullptr, *diag = nullptr;
+    float *Rt = nullptr, *Zt = nullptr, *St = nullptr, *Yt = nullptr, *Pt = nullptr;
     uint8_t* nup = nullptr;
     uint8_t* fl8 = nullptr;       // fluid bits per 8-cell z-segment
     double* P64 = nullptr;        // FP64 solution of the refined solve
@@ -87,7 +91,7 @@ struct ariel_sim {
     // learned preconditioner: weights, activations, r_old (allocated on load)
     LearnedNet net;
     uint16_t* net_mem = nullptr;
-    float* R_old = nullptr;
+    float* R_old = nullptr;   // tile-major
     bool net_ready = false;
     double* partials = nullptr;
     uint32_t* counter = nullptr;
@@ -214,6 +218,13 @@ int stage_resample(ariel_sim* h);
 // pressure solve helpers (solve.hip)
 void set_active(ariel_sim* h, bool device_counts);
 int prepare_solve(ariel_sim* h);
+// The solve's launches after prepare_solve (solve.hip): set-up, one iteration
+// (refine: with the reliable update; learned: flexible PCG, mic_only once the
+// host has seen the fallback; prof: sampled kernel timing), and the pressure
+// from the iterate.
+void enqueue_solve_setup(ariel_sim* h, bool learned);
+void enqueue_iteration(ariel_sim* h, int it, bool refine, bool learned, bool mic_only, bool prof);
+void enqueue_solve_finish(ariel_sim* h);
 
 }  // namespace host
 }  // namespace ariel

@@ -7,7 +7,11 @@
 //   sort: cell_of[Np], slots[Np] (int32), counts[N^3] (int32, zero between
 //         sorts), offsets[N^3+1] (int32)
 //   MAC grid: u[3], du[3] (faces, float), P, D, L (cells, float), A (cells, u8)
-//   solver: R, Z, S, Q, pc, diag (float), nup (u8), reduction partials (double)
+//   solver: diag, pc (natural, then its tile-major copy), nup (u8), fl8 (FLUID
+//         bits per 8-cell z-segment); the iteration vectors r, z, s (two
+//         buffers), y = A s, the iterate p and q, tile-major (tile t's 512 cells
+//         at [512 t, 512 t + 512), cell (il, jl, kl) at 64 il + 8 jl + kl);
+//         x64 (double, natural) for reliable updates; reduction partials (double)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -247,31 +251,53 @@ void launch_fluid_segments(hipStream_t s, int N, const uint8_t* A, uint8_t* fl8)
 void launch_active_sets(hipStream_t s, int N, const uint8_t* fl8, int32_t* seg_flag,
                         int32_t* seg_scan, int32_t* seg_list, int32_t* tile_any,
                         int32_t* tile_lists, int32_t* scan_sums, SolverScalars* sc);
-// z = M^-1 r; the last pass finalises rho_new = z.r and beta in the scalars.
+// The iteration's p += alpha s, r -= alpha y, carried by the preconditioner
+// apply (tiled red-black MIC(0) only): the tile-major iterate, both
+// search-direction buffers, y, and N^3.
+struct IterAxpy {
+    float* P;
+    const float* S;
+    const float* Y;
+    int64_t nc;
+};
+// z = M^-1 r (r, q, z tile-major); the last pass finalises rho_new = z.r and
+// beta in the scalars. With ax (tiled red-black MIC(0)), the first pass over
+// each colour first applies the iteration's AXPY to its tiles and the finalize
+// also takes ||r||^2 and the stop test.
 void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, const float* pc,
-                          const float* r, float* q, float* z, const Reduce& red,
-                          const Active& act);
+                          float* r, float* q, float* z, const Reduce& red,
+                          const Active& act, const IterAxpy* ax = nullptr);
+// r = b - A p from the warm start P (natural; cleared off FLUID) into the
+// tile-major Rt and Pt (refine: x64 = P, Pt untouched).
 void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                      float* P, const float* D, float* R, const Reduce& red, double tol,
-                      int stop_rule, double* P64, int refine);
+                      float* P, const float* D, float* Rt, const Reduce& red, double tol,
+                      int stop_rule, double* P64, int refine, float* Pt);
+// s = z (first iteration) or z + beta s_prev, y = A s, alpha = rho / s.y. St
+// holds two tile-major buffers selected by the iteration's parity.
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                     const float* x, float* y, const Reduce& red, const Active& act);
-void launch_axpy2(hipStream_t s, float* P, const float* S, float* R, const float* Z,
+                     const uint8_t* fl8, const float* Zt, float* St, float* Yt, const Reduce& red,
+                     const Active& act);
+// p += alpha s, r -= alpha y, ||r||^2 and the stop test (tile-major).
+void launch_axpy2(hipStream_t s, float* Pt, const float* St, float* Rt, const float* Yt, int64_t nc,
                   const Reduce& red, const Active& act);
 // Reliable update every refine_every iterations: x64 += p, p = 0, r = b - A x64 (FP64).
 void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                            float* P, double* P64, const float* D, float* R, const Reduce& red,
+                            float* Pt, double* P64, const float* D, float* Rt, const Reduce& red,
                             const Active& act);
-void launch_refine_finish(hipStream_t s, const uint8_t* A, float* P, const double* P64,
-                          const Active& act, const SolverScalars* sc);
-void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,
-                     const Active& act);
+// The pressure P (natural) from the iterate Pt: (float)(x64 + p) when P64 is
+// given, else p; 0 off FLUID.
+void launch_refine_finish(hipStream_t s, int N, const uint8_t* A, float* P, const float* Pt,
+                          const double* P64, const Active& act, const SolverScalars* sc);
+// Natural <-> tile-major copy over the active tiles (gate: only while flexible
+// PCG is on the network).
+void launch_tile_convert(hipStream_t s, int N, const float* src, float* dst, const Active& act,
+                         const SolverScalars* sc, bool to_tiles, bool gate);
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                        const float* x, float* y);
 void launch_solver_init(hipStream_t s, SolverScalars* sc, int max_iters, int refine_every,
                         int flex = 0);
-// Flexible PCG helpers: dst = src on the active segments; z.r and z.r_old with
-// the Polak-Ribiere finalize (check: only the descent test of the network's z).
+// Flexible PCG helpers: dst = src on the active tiles; z.r and z.r_old with the
+// Polak-Ribiere finalize (check: only the descent test of the network's z).
 void launch_copy_list(hipStream_t s, const float* src, float* dst, const Active& act,
                       const SolverScalars* sc);
 void launch_pr_dot(hipStream_t s, const float* z, const float* r, const float* r_old,
@@ -284,7 +310,7 @@ void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A,
 // the FP64 solution in FP64 into sc->tmp64 (R untouched). launch_refine_finish
 // follows and rounds x64 to the pressure.
 void launch_true_residual_norm64(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                                 float* P, double* P64, const float* D, const Reduce& red,
+                                 float* Pt, double* P64, const float* D, const Reduce& red,
                                  const Active& act);
 
 // reduction helpers

@@ -72,7 +72,6 @@ void record_pre(ariel_sim* h, Recorder& r) {
     stage_p2g(h);
     stage_divergence(h);
     const Params p = h->params();
-    const Reduce red = h->reduce();
     hipStream_t s = h->stream;
     launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
     launch_fluid_segments(s, h->N, h->A, h->fl8);
@@ -81,26 +80,13 @@ void record_pre(ariel_sim* h, Recorder& r) {
                        h->tile_lists, h->scan_sums, h->sc);
     set_active(h, true);
     launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc, h->act, h->sc);
-    const int refine = h->d.refine_every > 0;
-    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
-                     h->P64, refine);
-    if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
-    launch_fill_f32(s, h->Z, h->nc, 0.0f);
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, h->act);
-    launch_update_s(s, h->Z, h->S, red, 1, h->act);
+    enqueue_solve_setup(h, false);
     recorder() = nullptr;
 }
 
 void record_iteration(ariel_sim* h, Recorder& r, bool refine) {
     recorder() = &r;
-    const Params p = h->params();
-    const Reduce red = h->reduce();
-    hipStream_t s = h->stream;
-    launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, h->act);
-    launch_axpy2(s, h->P, h->S, h->R, h->Z, red, h->act);
-    if (refine) launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, h->act);
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, h->act);
-    launch_update_s(s, h->Z, h->S, red, 0, h->act);
+    enqueue_iteration(h, 0, refine, false, false, false);
     recorder() = nullptr;
 }
 
@@ -109,10 +95,7 @@ void record_post(ariel_sim* h, Recorder& r) {
     const Params p = h->params();
     const Reduce red = h->reduce();
     hipStream_t s = h->stream;
-    if (h->d.refine_every > 0) {
-        launch_true_residual_norm64(s, p, h->A, h->L, h->P, h->P64, h->D, red, h->act);
-        launch_refine_finish(s, h->A, h->P, h->P64, h->act, h->sc);
-    }
+    enqueue_solve_finish(h);
     launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, h->act);
     stage_gradient(h);
     stage_extrapolate(h);

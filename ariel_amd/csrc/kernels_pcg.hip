@@ -7,7 +7,12 @@
 // SolverScalars. Later kernels read those scalars and return early once `done`
 // is set, so the host can queue iterations in chunks without synchronising.
 // The iterations run over ascending lists of the active z-segments and 8^3 tiles
-// (those holding a FLUID cell), built once per solve.
+// (those holding a FLUID cell), built once per solve. The iteration vectors (r,
+// z, s, y = A s, the iterate p and the preconditioner's intermediate q) are kept
+// tile-major: tile t = (ti*TN + tj)*TN + tk holds its 512 cells contiguously,
+// cell (il, jl, kl) at 512 t + 64 il + 8 jl + kl, so every pass over the tiles of
+// one colour reads and writes whole 2 KB runs. The right-hand side, the cell
+// classes, the level set and the pressure handed back stay in the natural layout.
 //
 // MIC(0) (Bridson, "Fluid Simulation for Computer Graphics", ch. 5; the
 // reference's BuildPreconditioner/ApplyPreconditioner, solver.inl:104-286) is
@@ -17,6 +22,8 @@
 // tile's all black (upper). One 64-lane wave owns a tile, lane = (il,jl) owns the
 // z-line of 8 cells, and the in-tile triangular solves run as a 22-step
 // wavefront over i+j+k through LDS.
+#include <type_traits>
+
 #include "ariel_device.h"
 #include "ariel_internal.h"
 
@@ -39,21 +46,6 @@ __host__ __device__ inline int64_t list_grid(int64_t n) {
     const int64_t b = (n + kBlock - 1) / kBlock;
     return b < 1 ? 1 : (b < kRedBlocks ? b : kRedBlocks);
 }
-__host__ __device__ inline int spmv_per_thread(int64_t nseg) {
-    const int64_t cap = (int64_t)kBlock * kMaxPartials;
-    const int64_t per = (nseg + cap - 1) / cap;
-    return per < 1 ? 1 : (int)per;
-}
-__host__ __device__ inline int64_t spmv_grid(int64_t nseg) {
-    const int64_t per_block = (int64_t)kBlock * spmv_per_thread(nseg);
-    const int64_t nb = (nseg + per_block - 1) / per_block;
-    return nb < 1 ? 1 : nb;
-}
-constexpr int kTilesPerBlock = 4;
-__host__ __device__ inline int64_t mic_grid(int64_t ntiles) {
-    const int64_t nb = (ntiles + kTilesPerBlock - 1) / kTilesPerBlock;
-    return nb < 1 ? 1 : nb;
-}
 // Counts passed as -1 are read from the tank's device scalars (batched launches,
 // which are recorded before the counts exist).
 __device__ __forceinline__ int64_t seg_count(int64_t n, const SolverScalars* sc) {
@@ -61,6 +53,16 @@ __device__ __forceinline__ int64_t seg_count(int64_t n, const SolverScalars* sc)
 }
 __device__ __forceinline__ int64_t tile_count(int64_t n, const SolverScalars* sc, int which) {
     return n >= 0 ? n : (int64_t)sc->ntile[which];
+}
+
+// Tile-major index of tile t's cell (il, jl, kl), and of the natural cell (i, j, k).
+__host__ __device__ inline int64_t tile_major(int64_t t, int il, int jl, int kl) {
+    return t * 512 + (il * 8 + jl) * 8 + kl;
+}
+__device__ __forceinline__ int64_t tm_cell(int N, int i, int j, int k) {
+    const int TN = N / kTile;
+    const int64_t t = ((int64_t)(i / kTile) * TN + j / kTile) * TN + k / kTile;
+    return tile_major(t, i % kTile, j % kTile, k % kTile);
 }
 
 // ------------------------------------------------------------ reductions
@@ -90,6 +92,23 @@ __device__ void block_partials(double (&v)[NV], double* part, int slot) {
     }
     // sm is shared by every call: a later call in the same kernel may not
     // overwrite it before thread 0 has read it
+    __syncthreads();
+}
+
+// One value per block into component `comp` of the two-value layout
+// (part[2 slot + comp]); the other component is left to another kernel.
+__device__ void block_partial_comp(double v, double* part, int slot, int comp) {
+    double t[1] = {v};
+    __shared__ double sm1[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    const double ws = wave_sum(t[0]);
+    if (lane == 0) sm1[w] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int ww = 0; ww < nw; ++ww) a += sm1[ww];
+        part[2 * slot + comp] = a;
+    }
     __syncthreads();
 }
 
@@ -206,8 +225,9 @@ struct KSolverInit {
 
 // Warm start: P := 0 off FLUID (what the reference's first Op does), r = b - A p,
 // ||b||^2 and ||r||^2 over FLUID cells; stop threshold; early exit if converged.
-// With reliable updates (refine): x64 = P, r = b - A x64 in FP64; P is cleared
-// by a separate fill once every thread has read it.
+// The residual goes to the tile-major Rt, the iterate to the tile-major Pt (0 off
+// FLUID). With reliable updates (refine): x64 = P, r = b - A x64 in FP64, and Pt
+// (the correction to x64) is cleared by a separate fill.
 struct KResidual0 {
     static constexpr int kThreads = kBlock;
     struct Args {
@@ -216,12 +236,11 @@ struct KResidual0 {
         const float* L;
         float* P;
         const float* D;
-        float* R;
+        float* R;      // tile-major
         Reduce red;
-        double tol;
-        int stop_rule;
         double* P64;
         int refine;
+        float* Pt;     // tile-major
     };
     static __device__ void run(const Args& kargs) {
         const Params& p = kargs.p;
@@ -231,10 +250,9 @@ struct KResidual0 {
         const float* __restrict__ D = kargs.D;
         float* __restrict__ R = kargs.R;
         const Reduce& red = kargs.red;
-        double tol = kargs.tol;
-        int stop_rule = kargs.stop_rule;
         double* __restrict__ P64 = kargs.P64;
         int refine = kargs.refine;
+        float* __restrict__ Pt = kargs.Pt;
 
         const int N = p.N;
         const int64_t nc = (int64_t)N * N * N;
@@ -245,25 +263,30 @@ struct KResidual0 {
         double v[2] = {0.0, 0.0};
         for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
              c += nb * blockDim.x) {
-            if (A[c] != kFluid) {
-                if (!refine) P[c] = 0.0f;
-                else P64[c] = 0.0;
-                R[c] = 0.0f;
-                continue;
-            }
             int i, j, k;
             decode_cell(N, c, i, j, k);
+            const int64_t ct = tm_cell(N, i, j, k);
+            // P is read below only at FLUID cells, so clearing it off FLUID here
+            // races with no reader
+            if (A[c] != kFluid) {
+                P[c] = 0.0f;
+                if (refine) P64[c] = 0.0;
+                else Pt[ct] = 0.0f;
+                R[ct] = 0.0f;
+                continue;
+            }
             const float d = D[c];
             v[0] += (double)d * (double)d;
             if (refine) {
                 const double r = (double)d - spmv_cell_f64(A, L, P, N, i, j, k, c);
                 P64[c] = (double)P[c];
-                R[c] = (float)r;
+                R[ct] = (float)r;
                 v[1] += r * r;
             } else {
                 const float ax = spmv_cell(A, L, P, N, i, j, k, c, h);
                 const float r = d + (-1.0f) * ax;
-                R[c] = r;
+                R[ct] = r;
+                Pt[ct] = P[c];
                 v[1] += (double)r * (double)r;
             }
         }
@@ -271,136 +294,262 @@ struct KResidual0 {
     }
 };
 
-// One XRef term for a neighbour with class t, value xq, level set lq; cell value
-// xc with level set lc (solver.inl:138-154).
-__device__ __forceinline__ float xref_term(uint8_t t, float xq, const float* __restrict__ L,
-                                           int64_t q, float lc, float xc) {
-    if (t == kFluid) return xq;
-    if (t == kSolid) return xc;
-    return L[q] / gmin(1.0e-6f, lc) * xc;
+// Waits for the wave's own LDS writes and makes them visible to its other lanes
+// (each wave owns its LDS region): a wavefront-scope release/acquire pair around
+// a wave barrier, instead of a barrier over the whole block.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// y = A x on the quad of 4 consecutive cells t (cells 4t..4t+3) with vector loads
-// (x and the classes of the quad and of its +-y and +-x neighbour quads) and the
-// reference expression per cell; adds y.x over its FLUID cells to acc.
-__device__ __forceinline__ void spmv_quad(const Params& p, const uint8_t* __restrict__ A,
-                                          const float* __restrict__ L,
-                                          const float* __restrict__ x, float* __restrict__ y,
-                                          int64_t t, int i, int j, int k0, double& acc) {
-    const int N = p.N;
-    const int64_t NN = (int64_t)N * N;
-    const float n = (float)N;
-    const float h = 1.0f / (n * n);
-    const float4* __restrict__ X4 = reinterpret_cast<const float4*>(x);
-    const uchar4* __restrict__ A4 = reinterpret_cast<const uchar4*>(A);
-    float4* __restrict__ Y4 = reinterpret_cast<float4*>(y);
-    const int64_t c = 4 * t;
-    const uchar4 ac = A4[t];
-    if (ac.x != kFluid && ac.y != kFluid && ac.z != kFluid && ac.w != kFluid) {
-        Y4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return;
-    }
-    const float4 xc4 = X4[t];
-    const float xs[4] = {xc4.x, xc4.y, xc4.z, xc4.w};
-    const uint8_t as[4] = {ac.x, ac.y, ac.z, ac.w};
-    // neighbour quads (clamped to self at the domain boundary)
-    const bool hxp = i + 1 < N, hxm = i > 0, hyp = j + 1 < N, hym = j > 0;
-    float4 xxp = xc4, xxm = xc4, xyp = xc4, xym = xc4;
-    uchar4 axp = ac, axm = ac, ayp = ac, aym = ac;
-    if (hxp) { xxp = X4[t + NN / 4]; axp = A4[t + NN / 4]; }
-    if (hxm) { xxm = X4[t - NN / 4]; axm = A4[t - NN / 4]; }
-    if (hyp) { xyp = X4[t + N / 4]; ayp = A4[t + N / 4]; }
-    if (hym) { xym = X4[t - N / 4]; aym = A4[t - N / 4]; }
-    float xzm = 0.f, xzp = 0.f;
-    uint8_t azm = kSolid, azp = kSolid;
-    if (k0 > 0) { xzm = x[c - 1]; azm = A[c - 1]; }
-    if (k0 + 4 < N) { xzp = x[c + 4]; azp = A[c + 4]; }
-    const float vxp[4] = {xxp.x, xxp.y, xxp.z, xxp.w}, vxm[4] = {xxm.x, xxm.y, xxm.z, xxm.w};
-    const float vyp[4] = {xyp.x, xyp.y, xyp.z, xyp.w}, vym[4] = {xym.x, xym.y, xym.z, xym.w};
-    const uint8_t txp[4] = {axp.x, axp.y, axp.z, axp.w}, txm[4] = {axm.x, axm.y, axm.z, axm.w};
-    const uint8_t typ[4] = {ayp.x, ayp.y, ayp.z, ayp.w}, tym[4] = {aym.x, aym.y, aym.z, aym.w};
-    float out[4];
+constexpr int kTilesPerBlock = 4;
+__host__ __device__ inline int64_t mic_grid(int64_t ntiles) {
+    const int64_t nb = (ntiles + kTilesPerBlock - 1) / kTilesPerBlock;
+    return nb < 1 ? 1 : nb;
+}
+
+// One XRef term (solver.inl:138-154) of FLUID cell c with value xc: out of range
+// -> xc (the clamped index is c itself); FLUID neighbour -> its value xq; SOLID
+// -> xc; AIR -> L_q / min(1e-6, L_c) * xc. q = c + dq in the natural layout.
+// The same term from preloaded values: the neighbour's class aq and level set
+// lq, the cell's level set lc.
+__device__ __forceinline__ float xref_term(bool in_range, bool nb_fluid, float xq, uint32_t aq,
+                                           float lq, float lc, float xc) {
+    if (!in_range) return xc;
+    if (nb_fluid) return xq;
+    if (aq == (uint32_t)kSolid) return xc;
+    return lq / gmin(1.0e-6f, lc) * xc;
+}
+
+// 8 floats of a tile-major line: s = z + b * s_prev, b = 0 for the first
+// direction (s = z up to the sign of a zero z: -0 + 0 gives +0, which changes
+// no value the solve computes) and beta after. s_prev is finite (zero-filled
+// at creation, then written by earlier solves).
+__device__ __forceinline__ void search_line(const float* __restrict__ Z, const float* __restrict__ Sp,
+                                            int64_t off, float b, float (&x)[8]) {
+    const float4 z0 = *reinterpret_cast<const float4*>(Z + off);
+    const float4 z1 = *reinterpret_cast<const float4*>(Z + off + 4);
+    const float4 s0 = *reinterpret_cast<const float4*>(Sp + off);
+    const float4 s1 = *reinterpret_cast<const float4*>(Sp + off + 4);
+    const float z[8] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
+    const float sp[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        if (as[m] != kFluid) {
-            out[m] = 0.0f;
-            continue;
-        }
-        const int64_t cm = c + m;
-        const float xcm = xs[m];
-        const bool need_l = txp[m] == kAir || txm[m] == kAir || typ[m] == kAir || tym[m] == kAir ||
-                            (m < 3 ? as[m + 1] == kAir : azp == kAir) ||
-                            (m > 0 ? as[m - 1] == kAir : azm == kAir);
-        const float lc = need_l ? L[cm] : 0.0f;
-        // +x, -x, +y, -y: a missing neighbour clamps to the cell itself (FLUID)
-        const float t1 = hxp ? xref_term(txp[m], vxp[m], L, cm + NN, lc, xcm) : xcm;
-        const float t2 = hxm ? xref_term(txm[m], vxm[m], L, cm - NN, lc, xcm) : xcm;
-        const float t3 = hyp ? xref_term(typ[m], vyp[m], L, cm + N, lc, xcm) : xcm;
-        const float t4 = hym ? xref_term(tym[m], vym[m], L, cm - N, lc, xcm) : xcm;
-        float t5, t6;
-        if (m < 3) t5 = xref_term(as[m + 1], xs[m + 1], L, cm + 1, lc, xcm);
-        else t5 = (k0 + 4 < N) ? xref_term(azp, xzp, L, cm + 1, lc, xcm) : xcm;
-        if (m > 0) t6 = xref_term(as[m - 1], xs[m - 1], L, cm - 1, lc, xcm);
-        else t6 = (k0 > 0) ? xref_term(azm, xzm, L, cm - 1, lc, xcm) : xcm;
-        const float yc = (6.0f * xcm - t1 - t2 - t3 - t4 - t5 - t6) / h;
-        out[m] = yc;
-        acc += (double)yc * (double)xcm;
-    }
-    Y4[t] = make_float4(out[0], out[1], out[2], out[3]);
+    for (int kl = 0; kl < 8; ++kl) x[kl] = z[kl] + b * sp[kl];
+}
+__device__ __forceinline__ float search_cell(const float* __restrict__ Z, const float* __restrict__ Sp,
+                                             int64_t off, float b) {
+    return Z[off] + b * Sp[off];
+}
+__device__ __forceinline__ uint32_t byte_of(uint2 v, int k) {
+    return ((k < 4 ? v.x : v.y) >> (8 * (k & 3))) & 0xffu;
+}
+__device__ __forceinline__ void load_line(const float* __restrict__ p, float (&v)[8]) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    const float4 b = *reinterpret_cast<const float4*>(p + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 
-// z = A s, sigma = z.s, alpha = rho / sigma over the active segments (two quads
-// each); y elsewhere keeps the zeros of the solve's set-up.
-// One quad per thread (neighbouring lanes take the two quads of a segment, so a
-// wave's loads of the list's segments are contiguous); blocks take contiguous
-// runs of the list and each XCD a contiguous eighth of the blocks, so the +-x
-// neighbour planes of a segment are read by the same XCD that reads them as its
-// own (shared L2).
-struct KSpmvDot {
+// The search direction and the operator over the active tiles (one wave per
+// tile, lane (il, jl) owning the z-line of 8 cells): s = z on the first
+// iteration, else s = z + beta s_prev (the update of s, fused here); y = A s
+// (ComputeAx, solver.inl:196-230: y_c = (6 s_c - sum XRef_q) / h, h = 1/(n n),
+// terms in the order +x, -x, +y, -y, +z, -z; y = 0 off FLUID); sigma = y.s over
+// FLUID cells into the block partials.
+// S holds two tile-major buffers: the iteration after `it` completed ones reads
+// s_prev from buffer it&1 and writes s to buffer (it+1)&1, so the neighbour
+// values that a wave recomputes from z and s_prev never meet another wave's
+// stores. Neighbour lines inside the tile come from the wave's LDS; lines across
+// tile faces and the z-neighbours at the line ends from the neighbouring tiles.
+// A neighbour's value is used only when its FLUID bit is set, and FLUID cells
+// lie in active tiles, so values of inactive tiles are never used.
+struct KTileSpmv {
     static constexpr int kThreads = 256;
     struct Args {
         Params p;
         const uint8_t* A;
         const float* L;
-        const float* x;
-        float* y;
-        const int32_t* segs;
-        int64_t nseg;
-        int per_thread;
+        const uint8_t* fl8;
+        const float* Z;
+        float* S;
+        float* Y;
+        const int32_t* tiles;
+        int64_t ntiles;
         Reduce red;
     };
-    static __device__ void run(const Args& kargs) {
-        const Params& p = kargs.p;
-        const uint8_t* __restrict__ A = kargs.A;
-        const float* __restrict__ L = kargs.L;
-        const float* __restrict__ x = kargs.x;
-        float* __restrict__ y = kargs.y;
-        const int32_t* __restrict__ segs = kargs.segs;
-        int64_t nseg = kargs.nseg;
-        int per_thread = kargs.per_thread;
-        const Reduce& red = kargs.red;
-
-        if (red.sc->done) return;
-        nseg = seg_count(nseg, red.sc);
-        const int64_t nq = 2 * nseg;   // quads
-        per_thread = spmv_per_thread(nq);
-        const int64_t per_block = (int64_t)blockDim.x * per_thread;
-        const int64_t nb = spmv_grid(nq);
-        const int64_t b = xcd_block(nb);
-        if (b >= nb) return;
-        double v[1] = {0.0};
-        for (int m = 0; m < per_thread; ++m) {
-            const int64_t u = b * per_block + (int64_t)m * blockDim.x + threadIdx.x;
-            if (u >= nq) break;
-            const uint32_t sg = (uint32_t)segs[u >> 1];
-            const int half = (int)(u & 1);
-            const uint32_t NS = (uint32_t)p.N / kTile;
-            const uint32_t line = sg / NS;   // i*N + j
-            const int k0 = (int)(sg - line * NS) * kTile + 4 * half;
-            const int j = (int)(line % (uint32_t)p.N), i = (int)(line / (uint32_t)p.N);
-            spmv_quad(p, A, L, x, y, 2 * (int64_t)sg + half, i, j, k0, v[0]);
+    static __device__ void run(const Args& a) {
+        const SolverScalars* sc = a.red.sc;
+        if (sc->done) return;
+        const int64_t ntiles = tile_count(a.ntiles, sc, 2);
+        // each XCD takes a contiguous run of the list, so the neighbouring tiles
+        // whose faces a tile reads were mostly read through the same L2
+        const int64_t nbk = mic_grid(ntiles);
+        const int64_t bl = xcd_block(nbk);
+        if (bl >= nbk) return;
+        const int N = a.p.N;
+        const int TN = N / kTile, NS = N / kTile;
+        const int64_t nc = (int64_t)N * N * N, NN = (int64_t)N * N;
+        const int it = sc->it;
+        const float b = it == 0 ? 0.0f : (float)sc->beta;
+        const float* __restrict__ Z = a.Z;
+        const float* __restrict__ Sp = a.S + (int64_t)(it & 1) * nc;
+        float* __restrict__ Sn = a.S + (int64_t)((it + 1) & 1) * nc;
+        const uint8_t* __restrict__ A = a.A;
+        const float* __restrict__ L = a.L;
+        const uint8_t* __restrict__ fl8 = a.fl8;
+        __shared__ float lds[kTilesPerBlock][64 * 8];
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int64_t idx = bl * kTilesPerBlock + wave;
+        double dot[1] = {0.0};
+        if (idx < ntiles) {
+            const int64_t t = a.tiles[idx];
+            const int tk = (int)(t % TN), tj = (int)((t / TN) % TN), ti = (int)(t / ((int64_t)TN * TN));
+            const int il = lane >> 3, jl = lane & 7;
+            const int gi = ti * kTile + il, gj = tj * kTile + jl, k0 = tk * kTile;
+            const int64_t own = tile_major(t, il, jl, 0);
+            const int64_t sg = ((int64_t)gi * N + gj) * NS + tk;
+            const int64_t TT = (int64_t)TN * TN;
+            const bool r_xp = gi + 1 < N, r_xm = gi > 0, r_yp = gj + 1 < N, r_ym = gj > 0;
+            const bool r_zp = k0 + kTile < N, r_zm = k0 > 0;
+            // every global load is issued before the first use: the own line, one
+            // x-halo line (il == 7: +x, il == 0: -x), one y-halo line (jl == 7: +y,
+            // jl == 0: -y) and the two z neighbours at the line's ends, with their
+            // FLUID bits; lanes without a halo line on an axis read their own line
+            // instead and discard it
+            // branch-free offsets: the +x line (0, jl) of tile t + TN^2 lies
+            // 512 TN^2 - 7*64 floats past the own line (7, jl), the -x line the
+            // same distance before (0, jl); likewise +-y with 512 TN - 7*8
+            const int hx = (il == 7 && r_xp) ? 1 : ((il == 0 && r_xm) ? -1 : 0);
+            const int hy = (jl == 7 && r_yp) ? 1 : ((jl == 0 && r_ym) ? -1 : 0);
+            const int dzp = r_zp ? 1 : 0, dzm = r_zm ? 1 : 0;
+            const int64_t lnx = own + (int64_t)hx * (512 * TT - 7 * 64);
+            const int64_t lny = own + (int64_t)hy * (512 * (int64_t)TN - 7 * 8);
+            const int64_t lzp = own + (int64_t)dzp * 512;
+            const int64_t lzm = own - (int64_t)dzm * (512 - 7);
+            const uint32_t fl = fl8[sg];
+            const uint32_t fhx = fl8[sg + (int64_t)hx * (NN / kTile)];
+            const uint32_t fhy = fl8[sg + (int64_t)hy * NS];
+            const uint32_t f_zp = (fl8[sg + dzp] & 1u) & (uint32_t)dzp;
+            const uint32_t f_zm = ((fl8[sg - dzm] >> 7) & 1u) & (uint32_t)dzm;
+            float x[8], hxl[8], hyl[8];
+            search_line(Z, Sp, own, b, x);
+            search_line(Z, Sp, lnx, b, hxl);
+            search_line(Z, Sp, lny, b, hyl);
+            const float zp = search_cell(Z, Sp, lzp, b);
+            const float zm = search_cell(Z, Sp, lzm, b);
+            *reinterpret_cast<float4*>(Sn + own) = make_float4(x[0], x[1], x[2], x[3]);
+            *reinterpret_cast<float4*>(Sn + own + 4) = make_float4(x[4], x[5], x[6], x[7]);
+            float* w = &lds[wave][0];
+            *reinterpret_cast<float4*>(w + lane * 8) = make_float4(x[0], x[1], x[2], x[3]);
+            *reinterpret_cast<float4*>(w + lane * 8 + 4) = make_float4(x[4], x[5], x[6], x[7]);
+            // FLUID bits of the in-tile neighbour lines
+            const uint32_t f_xp_in = __shfl(fl, (lane + 8) & 63, 64), f_xm_in = __shfl(fl, (lane - 8) & 63, 64);
+            const uint32_t f_yp_in = __shfl(fl, (lane + 1) & 63, 64), f_ym_in = __shfl(fl, (lane - 1) & 63, 64);
+            wave_sync();
+            float xp[8], xm[8], yp[8], ym[8];
+#pragma unroll
+            for (int kl = 0; kl < 8; ++kl) {
+                xp[kl] = il < 7 ? w[((lane + 8) & 63) * 8 + kl] : hxl[kl];
+                xm[kl] = il > 0 ? w[((lane - 8) & 63) * 8 + kl] : hxl[kl];
+                yp[kl] = jl < 7 ? w[((lane + 1) & 63) * 8 + kl] : hyl[kl];
+                ym[kl] = jl > 0 ? w[((lane - 1) & 63) * 8 + kl] : hyl[kl];
+            }
+            const uint32_t f_xp = il < 7 ? f_xp_in : (hx == 1 ? fhx : 0u);
+            const uint32_t f_xm = il > 0 ? f_xm_in : (hx == -1 ? fhx : 0u);
+            const uint32_t f_yp = jl < 7 ? f_yp_in : (hy == 1 ? fhy : 0u);
+            const uint32_t f_ym = jl > 0 ? f_ym_in : (hy == -1 ? fhy : 0u);
+            const float n = (float)N;
+            const float h = 1.0f / (n * n);
+            const int64_t c0 = cidx(N, gi, gj, k0);
+            const uint32_t nzp = ((fl >> 1) | (f_zp << 7)) & 0xffu;
+            const uint32_t nzm = ((fl << 1) | f_zm) & 0xffu;
+            // FLUID cells with an in-range neighbour that is not FLUID need the
+            // ghost terms (SOLID: s_c; AIR: L_q / min(1e-6, L_c) s_c)
+            const uint32_t need =
+                fl & ((r_xp ? ~f_xp : 0u) | (r_xm ? ~f_xm : 0u) | (r_yp ? ~f_yp : 0u) |
+                      (r_ym ? ~f_ym : 0u) | (~nzp & (r_zp ? 0xffu : 0x7fu)) |
+                      (~nzm & (r_zm ? 0xffu : 0xfeu))) & 0xffu;
+            const bool ghosts = __ballot(need != 0u) != 0ull;
+            // y_c = (6 s_c - t1 - ... - t6) / h; dividing by h = 2^-2k (N a power
+            // of two) is the exact product with 2^2k
+            const bool p2 = is_pow2_float(h);
+            const float inv_h = 1.0f / h;
+            float y[8];
+            auto cells = [&](auto p2_tag, auto ghost_tag) {
+                constexpr bool P2 = decltype(p2_tag)::value;
+                constexpr bool GHOST = decltype(ghost_tag)::value;
+                // the classes and level sets of the own and neighbour lines
+                // (natural layout), loaded together, only in waves with ghost terms
+                uint2 acl = make_uint2(0u, 0u), axp = acl, axm = acl, ayp = acl, aym = acl;
+                float lcl[8], lxp[8], lxm[8], lyp[8], lym[8];
+                uint32_t azp = 0u, azm = 0u;
+                float lzpv = 0.0f, lzmv = 0.0f;
+                if constexpr (GHOST) {
+                    const int64_t cxp = r_xp ? c0 + NN : c0, cxm = r_xm ? c0 - NN : c0;
+                    const int64_t cyp = r_yp ? c0 + N : c0, cym = r_ym ? c0 - N : c0;
+                    const int64_t czp = r_zp ? c0 + kTile : c0, czm = r_zm ? c0 - 1 : c0;
+                    acl = *reinterpret_cast<const uint2*>(A + c0);
+                    axp = *reinterpret_cast<const uint2*>(A + cxp);
+                    axm = *reinterpret_cast<const uint2*>(A + cxm);
+                    ayp = *reinterpret_cast<const uint2*>(A + cyp);
+                    aym = *reinterpret_cast<const uint2*>(A + cym);
+                    azp = A[czp];
+                    azm = A[czm];
+                    load_line(L + c0, lcl);
+                    load_line(L + cxp, lxp);
+                    load_line(L + cxm, lxm);
+                    load_line(L + cyp, lyp);
+                    load_line(L + cym, lym);
+                    lzpv = L[czp];
+                    lzmv = L[czm];
+                }
+#pragma unroll
+                for (int kl = 0; kl < 8; ++kl) {
+                    const bool fc = (fl >> kl) & 1u;
+                    const float xc = x[kl];
+                    const int ku = kl < 7 ? kl + 1 : 7, kd = kl > 0 ? kl - 1 : 0;
+                    float t1, t2, t3, t4, t5, t6;
+                    if constexpr (GHOST) {
+                        if (!fc) {
+                            y[kl] = 0.0f;
+                            continue;
+                        }
+                        const float lc = lcl[kl];
+                        t1 = xref_term(r_xp, (f_xp >> kl) & 1u, xp[kl], byte_of(axp, kl), lxp[kl], lc, xc);
+                        t2 = xref_term(r_xm, (f_xm >> kl) & 1u, xm[kl], byte_of(axm, kl), lxm[kl], lc, xc);
+                        t3 = xref_term(r_yp, (f_yp >> kl) & 1u, yp[kl], byte_of(ayp, kl), lyp[kl], lc, xc);
+                        t4 = xref_term(r_ym, (f_ym >> kl) & 1u, ym[kl], byte_of(aym, kl), lym[kl], lc, xc);
+                        t5 = kl < 7 ? xref_term(true, (fl >> ku) & 1u, x[ku], byte_of(acl, ku), lcl[ku], lc, xc)
+                                    : xref_term(r_zp, f_zp, zp, azp, lzpv, lc, xc);
+                        t6 = kl > 0 ? xref_term(true, (fl >> kd) & 1u, x[kd], byte_of(acl, kd), lcl[kd], lc, xc)
+                                    : xref_term(r_zm, f_zm, zm, azm, lzmv, lc, xc);
+                    } else {
+                        // every in-range neighbour of a FLUID cell is FLUID
+                        t1 = r_xp ? xp[kl] : xc;
+                        t2 = r_xm ? xm[kl] : xc;
+                        t3 = r_yp ? yp[kl] : xc;
+                        t4 = r_ym ? ym[kl] : xc;
+                        t5 = kl < 7 ? x[ku] : (r_zp ? zp : xc);
+                        t6 = kl > 0 ? x[kd] : (r_zm ? zm : xc);
+                    }
+                    const float acc = 6.0f * xc - t1 - t2 - t3 - t4 - t5 - t6;
+                    const float yc = P2 ? acc * inv_h : acc / h;
+                    y[kl] = fc ? yc : 0.0f;
+                    dot[0] += fc ? (double)yc * (double)xc : 0.0;
+                }
+            };
+            if (ghosts) {
+                if (p2) cells(std::true_type{}, std::true_type{});
+                else cells(std::false_type{}, std::true_type{});
+            } else {
+                if (p2) cells(std::true_type{}, std::false_type{});
+                else cells(std::false_type{}, std::false_type{});
+            }
+            *reinterpret_cast<float4*>(a.Y + own) = make_float4(y[0], y[1], y[2], y[3]);
+            *reinterpret_cast<float4*>(a.Y + own + 4) = make_float4(y[4], y[5], y[6], y[7]);
         }
-        block_partials<1>(v, red.partials, b);
+        block_partials<1>(dot, a.red.partials, (int)bl);
     }
 };
 
@@ -422,65 +571,72 @@ __device__ __forceinline__ bool refine_due(const SolverScalars* sc) {
     return sc->refine_every > 0 && (sc->it % sc->refine_every) == 0;
 }
 
-// p += alpha s; r += (-alpha) z (Op, solver.inl:157-177); ||r||^2; stop test.
-// One active segment (two float4 of each operand) per thread and iteration.
+// Elementwise passes over the active tiles: work item e is line e%64 of the
+// list's tile e/64 (8 tile-major floats, two float4), so consecutive threads
+// touch consecutive 32-byte lines.
+__host__ __device__ inline int64_t tile_lines_grid(int64_t ntiles) { return list_grid(64 * ntiles); }
+__device__ __forceinline__ int64_t line_offset(const int32_t* __restrict__ tiles, int64_t e) {
+    return (int64_t)tiles[e >> 6] * 512 + (e & 63) * 8;
+}
+// The search direction of the iteration after `it` completed ones (KTileSpmv).
+__device__ __forceinline__ const float* search_dir(const float* S, const SolverScalars* sc, int64_t nc) {
+    return S + (int64_t)((sc->it + 1) & 1) * nc;
+}
+
+// p += alpha s; r += (-alpha) y (Op, solver.inl:157-177); ||r||^2; stop test.
+// Cells of active tiles off FLUID hold zero in s, y, p and r and stay zero.
 struct KAxpy2 {
     static constexpr int kThreads = kBlock;
     struct Args {
         float* P;
-        const float* S;
+        const float* S;   // both search-direction buffers
         float* R;
-        const float* Z;
-        const int32_t* segs;
-        int64_t nseg;
+        const float* Y;
+        const int32_t* tiles;
+        int64_t ntiles;
+        int64_t nc;
         Reduce red;
     };
     static __device__ void run(const Args& kargs) {
-        float* __restrict__ P = kargs.P;
-        const float* __restrict__ S = kargs.S;
-        float* __restrict__ R = kargs.R;
-        const float* __restrict__ Z = kargs.Z;
-        const int32_t* __restrict__ segs = kargs.segs;
-        int64_t nseg = kargs.nseg;
         const Reduce& red = kargs.red;
-
         if (red.sc->done) return;
-        nseg = seg_count(nseg, red.sc);
-        const int64_t nb = list_grid(nseg);
+        const int64_t ntl = tile_count(kargs.ntiles, red.sc, 2);
+        const int64_t nb = tile_lines_grid(ntl);
         if ((int64_t)blockIdx.x >= nb) return;
         const float af = (float)red.sc->alpha;
         const float naf = -af;
-        float4* __restrict__ P4 = reinterpret_cast<float4*>(P);
-        float4* __restrict__ R4 = reinterpret_cast<float4*>(R);
-        const float4* __restrict__ S4 = reinterpret_cast<const float4*>(S);
-        const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
+        float* __restrict__ P = kargs.P;
+        float* __restrict__ R = kargs.R;
+        const float* __restrict__ S = search_dir(kargs.S, red.sc, kargs.nc);
+        const float* __restrict__ Y = kargs.Y;
         double v[1] = {0.0};
-        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += nb * blockDim.x) {
-            const int64_t c = 2 * (int64_t)segs[u];
-            float4 p0 = P4[c], s0 = S4[c], r0 = R4[c], z0 = Z4[c];
-            float4 p1 = P4[c + 1], s1 = S4[c + 1], r1 = R4[c + 1], z1 = Z4[c + 1];
-            p0.x = p0.x + af * s0.x; p0.y = p0.y + af * s0.y; p0.z = p0.z + af * s0.z; p0.w = p0.w + af * s0.w;
-            r0.x = r0.x + naf * z0.x; r0.y = r0.y + naf * z0.y; r0.z = r0.z + naf * z0.z; r0.w = r0.w + naf * z0.w;
-            p1.x = p1.x + af * s1.x; p1.y = p1.y + af * s1.y; p1.z = p1.z + af * s1.z; p1.w = p1.w + af * s1.w;
-            r1.x = r1.x + naf * z1.x; r1.y = r1.y + naf * z1.y; r1.z = r1.z + naf * z1.z; r1.w = r1.w + naf * z1.w;
-            P4[c] = p0;
-            R4[c] = r0;
-            P4[c + 1] = p1;
-            R4[c + 1] = r1;
-            v[0] += (double)r0.x * (double)r0.x + (double)r0.y * (double)r0.y +
-                    (double)r0.z * (double)r0.z + (double)r0.w * (double)r0.w;
-            v[0] += (double)r1.x * (double)r1.x + (double)r1.y * (double)r1.y +
-                    (double)r1.z * (double)r1.z + (double)r1.w * (double)r1.w;
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 64 * ntl;
+             e += nb * blockDim.x) {
+            const int64_t o = line_offset(kargs.tiles, e);
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                float4 p = *reinterpret_cast<float4*>(P + o + 4 * hh);
+                float4 r = *reinterpret_cast<float4*>(R + o + 4 * hh);
+                const float4 sv = *reinterpret_cast<const float4*>(S + o + 4 * hh);
+                const float4 y = *reinterpret_cast<const float4*>(Y + o + 4 * hh);
+                p.x = p.x + af * sv.x; p.y = p.y + af * sv.y; p.z = p.z + af * sv.z; p.w = p.w + af * sv.w;
+                r.x = r.x + naf * y.x; r.y = r.y + naf * y.y; r.z = r.z + naf * y.z; r.w = r.w + naf * y.w;
+                *reinterpret_cast<float4*>(P + o + 4 * hh) = p;
+                *reinterpret_cast<float4*>(R + o + 4 * hh) = r;
+                v[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y +
+                        (double)r.z * (double)r.z + (double)r.w * (double)r.w;
+            }
         }
         block_partials<1>(v, red.partials, blockIdx.x);
     }
 };
 
-// Reliable update, part 1: x64 += p, p = 0 (every refine_every iterations).
+// Reliable update, part 1: x64 += p, p = 0 (every refine_every iterations);
+// x64 natural, p tile-major.
 struct KFlush {
     static constexpr int kThreads = kBlock;
     struct Args {
+        int N;
         const uint8_t* A;
         float* P;
         double* P64;
@@ -505,15 +661,19 @@ struct KFlush {
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < 8 * nseg;
              u += nb * blockDim.x) {
             const int64_t c = 8 * (int64_t)segs[u >> 3] + (u & 7);
-            if (A[c] == kFluid) P64[c] += (double)P[c];
-            P[c] = 0.0f;
+            int i, j, k;
+            decode_cell(kargs.N, c, i, j, k);
+            const int64_t ct = tm_cell(kargs.N, i, j, k);
+            if (A[c] == kFluid) P64[c] += (double)P[ct];
+            P[ct] = 0.0f;
         }
     }
 };
 
-// mode 0 (reliable update, part 2): r = b - A x64 in FP64, stored in FP32, then the
-// stop test. mode 1 (report): ||b - A p||^2 of the FP32 pressure into sc->tmp.
-// mode 3 (report): ||b - A x64||^2 of the FP64 solution into sc->tmp64.
+// mode 0 (reliable update, part 2): r = b - A x64 in FP64, stored in FP32 into the
+// tile-major R, then the stop test. mode 1 (report): ||b - A p||^2 of the FP32
+// pressure (natural) into sc->tmp. mode 3 (report): ||b - A x64||^2 of the FP64
+// solution into sc->tmp64.
 struct KTrueResidual {
     static constexpr int kThreads = kBlock;
     struct Args {
@@ -552,29 +712,33 @@ struct KTrueResidual {
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < 8 * nseg;
              u += nb * blockDim.x) {
             const int64_t c = 8 * (int64_t)segs[u >> 3] + (u & 7);
-            if (A[c] != kFluid) {
-                if (mode == 0) R[c] = 0.0f;
-                continue;
-            }
             int i, j, k;
             decode_cell(N, c, i, j, k);
+            if (A[c] != kFluid) {
+                if (mode == 0) R[tm_cell(N, i, j, k)] = 0.0f;
+                continue;
+            }
             const double ax = mode != 1 ? spmv_cell_f64d(A, L, P64, N, i, j, k, c)
                                         : spmv_cell_f64(A, L, P, N, i, j, k, c);
             const double r = (double)D[c] - ax;
             v[0] += r * r;
-            if (mode == 0) R[c] = (float)r;
+            if (mode == 0) R[tm_cell(N, i, j, k)] = (float)r;
         }
         block_partials<1>(v, red.partials, blockIdx.x);
     }
 };
 
-// End of a refined solve: p = (float)(x64 + p) on FLUID cells, 0 elsewhere.
+// End of a solve: the pressure (natural) from the iterate (tile-major) on the
+// active segments: P = (float)(x64 + p) with reliable updates, else P = p; 0 off
+// FLUID.
 struct KRefineFinish {
     static constexpr int kThreads = kBlock;
     struct Args {
+        int N;
         const uint8_t* A;
         float* P;
-        const double* P64;
+        const float* Pt;
+        const double* P64;   // null: no reliable updates
         const int32_t* segs;
         int64_t nseg;
         const SolverScalars* sc;
@@ -591,50 +755,11 @@ struct KRefineFinish {
         for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < 8 * nseg;
              u += nb * blockDim.x) {
             const int64_t c = 8 * (int64_t)segs[u >> 3] + (u & 7);
-            P[c] = A[c] == kFluid ? (float)(P64[c] + (double)P[c]) : 0.0f;
-        }
-    }
-};
-
-struct KUpdateS {
-    static constexpr int kThreads = kBlock;
-    struct Args {
-        const float* Z;
-        float* S;
-        const int32_t* segs;
-        int64_t nseg;
-        const SolverScalars* sc;
-        int first;
-    };
-    static __device__ void run(const Args& kargs) {
-        const float* __restrict__ Z = kargs.Z;
-        float* __restrict__ S = kargs.S;
-        const int32_t* __restrict__ segs = kargs.segs;
-        int64_t nseg = kargs.nseg;
-        const SolverScalars* sc = kargs.sc;
-        int first = kargs.first;
-
-        if (sc->done) return;
-        nseg = seg_count(nseg, sc);
-        const int64_t nb = list_grid(nseg);
-        if ((int64_t)blockIdx.x >= nb) return;
-        const float bf = (float)sc->beta;
-        const float4* __restrict__ Z4 = reinterpret_cast<const float4*>(Z);
-        float4* __restrict__ S4 = reinterpret_cast<float4*>(S);
-        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += nb * blockDim.x) {
-            const int64_t c = 2 * (int64_t)segs[u];
-    #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const float4 z = Z4[c + h];
-                if (first) {
-                    S4[c + h] = z;
-                } else {
-                    float4 sv = S4[c + h];
-                    sv.x = z.x + bf * sv.x; sv.y = z.y + bf * sv.y; sv.z = z.z + bf * sv.z; sv.w = z.w + bf * sv.w;
-                    S4[c + h] = sv;
-                }
-            }
+            int i, j, k;
+            decode_cell(kargs.N, c, i, j, k);
+            const float pt = kargs.Pt[tm_cell(kargs.N, i, j, k)];
+            if (A[c] != kFluid) P[c] = 0.0f;
+            else P[c] = P64 ? (float)(P64[c] + (double)pt) : pt;
         }
     }
 };
@@ -684,18 +809,18 @@ __device__ __forceinline__ void finish_rho(SolverScalars* sc, double rho_new) {
 // ------------------------------------------------------------ finalize
 // Partial counts that depend on the active sets, for finalize launches recorded
 // before the counts exist (nparts < 0 selects the producer's grid rule).
-enum { PARTS_LIST = -1, PARTS_SPMV = -2, PARTS_MIC_RB = -3, PARTS_MIC_ALL = -4 };
+enum { PARTS_LIST = -1, PARTS_TILE_LINES = -2, PARTS_MIC_RB = -3, PARTS_MIC_ALL = -4 };
 __device__ __forceinline__ int64_t parts_from_counts(int code, const SolverScalars* sc) {
     switch (code) {
         case PARTS_LIST: return list_grid(sc->nseg);
-        case PARTS_SPMV: return spmv_grid(2 * sc->nseg);
+        case PARTS_TILE_LINES: return tile_lines_grid(sc->ntile[2]);
         case PARTS_MIC_RB: return mic_grid(sc->ntile[1]) + mic_grid(sc->ntile[0]);
         default: return mic_grid(sc->ntile[2]);
     }
 }
 
 enum { FIN_RESID0 = 0, FIN_SIGMA = 1, FIN_RNORM = 2, FIN_TRUE = 3, FIN_REPORT = 4, FIN_RHO = 5,
-       FIN_PR = 6, FIN_PR_CHECK = 7, FIN_REPORT64 = 8 };
+       FIN_PR = 6, FIN_PR_CHECK = 7, FIN_REPORT64 = 8, FIN_RHO_RNORM = 9 };
 // Flexible PCG falls back to MIC(0) when, over a window of this many
 // iterations, ||r|| fell by less than 5% per iteration on average.
 // Flexible PCG's progress test: over every window of kFlexWindow iterations the
@@ -732,12 +857,13 @@ struct KFinalize {
 
         SolverScalars* sc = red.sc;
         if ((kind == FIN_SIGMA || kind == FIN_RNORM || kind == FIN_RHO || kind == FIN_PR ||
-             kind == FIN_PR_CHECK) && sc->done) return;
+             kind == FIN_PR_CHECK || kind == FIN_RHO_RNORM) && sc->done) return;
         if (kind == FIN_TRUE && (sc->done || !refine_due(sc))) return;
         if (kind == FIN_RHO && sc->flex) return;          // flexible PCG sets beta itself
         if (kind == FIN_PR_CHECK && (!sc->flex || sc->use_mic)) return;
         if (nparts < 0) nparts = (int)parts_from_counts(nparts, sc);
-        const int NV = (kind == FIN_RESID0 || kind == FIN_PR || kind == FIN_PR_CHECK) ? 2 : 1;
+        const int NV = (kind == FIN_RESID0 || kind == FIN_PR || kind == FIN_PR_CHECK ||
+                        kind == FIN_RHO || kind == FIN_RHO_RNORM) ? 2 : 1;
         double a0 = 0.0, a1 = 0.0;
         for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
             a0 += red.partials[b * NV];
@@ -810,7 +936,16 @@ struct KFinalize {
                 sc->tmp64 = s0;
                 break;
             case FIN_RHO:
-                finish_rho(sc, s0);
+                finish_rho(sc, s1);
+                break;
+            case FIN_RHO_RNORM:
+                // the AXPY rode on the apply: ||r||^2 and the stop test of the
+                // iteration, then rho_new = z.r and beta (a converged iteration's
+                // z is never used)
+                sc->rnorm2 = s0;
+                sc->it += 1;
+                decide(sc);
+                finish_rho(sc, s1);
                 break;
             case FIN_PR_CHECK:
                 // the network's z must be a descent direction (z.r > 0); if not,
@@ -929,21 +1064,6 @@ __device__ __forceinline__ uint32_t fluid_bits(const uint8_t* __restrict__ A, in
     uint32_t m = 0;
 #pragma unroll
     for (int kl = 0; kl < kTile; ++kl) m |= (A[line + kl] == kFluid ? 1u : 0u) << kl;
-    return m;
-}
-
-// Load a neighbouring line (8 cells) into LDS slot `dst` (stride kPad) as v = a*b
-// (b may be null -> v = a) together with its fluid bits.
-__device__ __forceinline__ uint32_t load_halo_line(const uint8_t* __restrict__ A,
-                                                   const float* __restrict__ a,
-                                                   const float* __restrict__ b, int64_t line,
-                                                   float* dst) {
-    uint32_t m = fluid_bits(A, line);
-#pragma unroll
-    for (int kl = 0; kl < kTile; ++kl) {
-        const float x = a[line + kl];
-        dst[kl] = b ? x * b[line + kl] : x;
-    }
     return m;
 }
 
@@ -1101,15 +1221,10 @@ struct KMicBuild {
     }
 };
 
-// The MIC(0) apply reads pc and its own intermediate q tile by tile. In the
-// natural layout a tile's z-line is 32 bytes between lines of the other colour's
-// tiles, so every read of one colour fetches whole cache lines half of which it
-// does not use. The apply therefore reads a tile-major copy of pc (tile t at
-// [512 t, 512 t + 512), cell (il, jl, kl) at 64 il + 8 jl + kl), made once per
-// solve over the active tiles, and keeps q in the same layout.
-__host__ __device__ inline int64_t tile_major(int64_t t, int il, int jl, int kl) {
-    return t * 512 + (il * 8 + jl) * 8 + kl;
-}
+// The preconditioner applies read a tile-major copy of pc, made once per solve
+// over the active tiles (in the natural layout a tile's z-line is 32 bytes
+// between lines of the other colour's tiles, so a pass over one colour would
+// fetch whole cache lines half of which it does not use).
 struct KPcTileMajor {
     static constexpr int kThreads = 64;
     struct Args {
@@ -1166,18 +1281,16 @@ struct KFluidSegments {
 constexpr int kLine = 64 * kPad;          // floats of one per-line LDS array
 constexpr int kHalo = 2 * kTile * kPad;   // two halo faces (minus / plus side) of 8 lines
 
-// Each wave owns its tile and its LDS region, so the wavefront steps only need
-// the wave's own LDS writes to be visible to its other lanes: a wavefront-scope
-// release/acquire pair (waits for the wave's LDS operations) around a wave
-// barrier, instead of a barrier over the whole block.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // MIC(0) apply, one pass over the tiles of one colour; 4 tiles per 256-thread
-// block, one wave per tile, lane (il, jl) owning the z-line of 8 cells.
+// block, one wave per tile, lane (il, jl) owning the z-line of 8 cells. r, q, z
+// and the factor are tile-major.
+// AXPY (passes 0 and 1, the first pass over each colour): the iteration's
+// p += alpha s and r += (-alpha) y (Op, solver.inl:157-177) for the tile's cells
+// as they are loaded, r written back, and ||r||^2 into component 0 of the block
+// partials; the stop test then follows the apply (FIN_RHO_RNORM).
+// Partials use the two-value layout: component 0 ||r||^2 (AXPY passes),
+// component 1 z.r (passes 1, 2, 3); black blocks take slots [0, nb), red
+// blocks [nb, nb + nr).
 //   PH 0 (red, forward):       q = L^-1 r inside the tile          -> Q
 //   PH 1 (black, fwd + bwd):   forward with red halos, backward     -> Z, z.r partials
 //   PH 2 (red, backward):      backward with black halos            -> Z, z.r partials
@@ -1199,26 +1312,31 @@ constexpr int kGuardLo = 16;                          // >= 14 = max il+jl
 constexpr int kWaveFloats = 3 * kLine + 2 * kHalo + kTile;
 constexpr int kGuardHi = 32;                          // >= 21 past the last region
 
-template <int PH>
+template <int PH, bool AXPY = false>
 struct KMicApply {
+    static_assert(!AXPY || PH == 0 || PH == 1, "the AXPY rides on the first pass over a colour");
     static constexpr int kThreads = 256;
     struct Args {
         Params p;
         const uint8_t* fl8;
-        const float* pc;
-        const float* r;
+        const float* pc;   // tile-major factor
+        float* r;
         float* Q;
         float* Z;
         const int32_t* tiles;
         int64_t ntiles;
         Reduce red;
         int slot_base;
+        float* P;          // AXPY: iterate, both search-direction buffers, y
+        const float* S;
+        const float* Y;
+        int64_t nc;
     };
     static __device__ void run(const Args& kargs) {
         const Params& p = kargs.p;
         const uint8_t* __restrict__ fl8 = kargs.fl8;
         const float* __restrict__ pc = kargs.pc;
-        const float* __restrict__ r = kargs.r;
+        float* __restrict__ r = kargs.r;
         float* __restrict__ Q = kargs.Q;
         float* __restrict__ Z = kargs.Z;
         const int32_t* __restrict__ tiles = kargs.tiles;
@@ -1244,14 +1362,22 @@ struct KMicApply {
         const int o_hy = o_hx + kHalo;                    // y halo [(jl==7)*72 + il*9 + kl]
         const int o_zero = o_hy + kHalo;                  // 8 x -0.0
         int ti = 0, tj = 0, tk = 0;
+        // logical block: each XCD takes a contiguous run of the list, so the
+        // neighbouring tiles whose faces a tile reads were mostly handled by the
+        // same XCD (its L2); the grid is xcd_grid of the pass's block count
+        const int64_t nb_arg = mic_grid(tile_count(ntiles, red.sc, PH == 1 ? 1 : (PH == 3 ? 2 : 0)));
+        const int64_t bl = xcd_block(nb_arg);
+        const int64_t idx = bl * kTilesPerBlock + wave;
+        // the tile index is fetched together with the scalars that gate the pass
+        // (every list has room for all TN^3 tiles, so the clamped read is in bounds)
+        const int64_t nt_all = (int64_t)TN * TN * TN;
+        const int t_pre = tiles[idx < nt_all ? idx : nt_all - 1];
         ntiles = tile_count(ntiles, red.sc, PH == 1 ? 1 : (PH == 3 ? 2 : 0));
-        if ((int64_t)blockIdx.x >= mic_grid(ntiles)) return;
+        if (bl >= nb_arg) return;
         if (red.sc->flex && !red.sc->use_mic) return;   // flexible PCG still on the network
         if (slot_base < 0) slot_base = (int)mic_grid(tile_count(-1, red.sc, 1));
-        const int64_t idx = (int64_t)blockIdx.x * kTilesPerBlock + wave;
         const bool active = idx < ntiles;
-        // the tile index and the stop flag are fetched together
-        const int t = active ? tiles[idx] : 0;
+        const int t = active ? t_pre : 0;
         if (red.sc->done) return;
         if (active) {
             tk = t % TN;
@@ -1263,93 +1389,121 @@ struct KMicApply {
         uint32_t fl = 0;
         float hzm = -0.0f, hzp = -0.0f;   // z-halo terms (-0.0 when absent)
         float rreg[kTile];
+        double rr = 0.0;                  // ||r||^2 of the tile after the AXPY
         // coalesced mapping: round m covers line (m, lane>>3), element lane&7
         const int ce = lane & 7, cj = lane >> 3;
         if (lane < kTile) lds[o_zero + lane] = -0.0f;
         if (active) {
-            fl = fl8[((int64_t)gi * N + gj) * NS + tk];
+            // every global load of the pass is issued before the first LDS store:
+            // own data, then one x-halo line, one y-halo line and the two z
+            // neighbours per lane (lanes without a halo line on an axis read their
+            // own line instead and discard it)
+            const int64_t sgl = ((int64_t)gi * N + gj) * NS + tk;
+            fl = fl8[sgl];
+            float pcv[kTile], sav[kTile], pv[kTile], sv[kTile], yv[kTile];
+            const float* __restrict__ Sd = AXPY ? search_dir(kargs.S, red.sc, kargs.nc) : nullptr;
     #pragma unroll
             for (int m = 0; m < kTile; ++m) {
-                const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
                 const int64_t gt = tile_major(t, m, cj, ce);
-                const int li = (m * 8 + cj) * kPad + ce;
-                lds[o_pc + li] = pc[gt];
-                rreg[m] = r[g];
-                lds[o_sa + li] = do_fwd ? rreg[m] : Q[gt];
+                pcv[m] = pc[gt];
+                rreg[m] = r[gt];
+                sav[m] = do_fwd ? rreg[m] : Q[gt];
+                if constexpr (AXPY) {
+                    pv[m] = kargs.P[gt];
+                    sv[m] = Sd[gt];
+                    yv[m] = kargs.Y[gt];
+                }
             }
+            float hxv[kTile], hyv[kTile], vm = -0.0f, vp = -0.0f;
+            uint32_t hfx = 0, hfy = 0, fm = 0, fp = 0;
+            int xi = -1, yj = -1;
+            bool has_m = false, has_p = false;
             if constexpr (halo) {
                 // neighbour lines across the tile faces: forward halos hold pc*Q of
                 // the red neighbours, backward halos their Z; -0.0 off-fluid
-                // (pc and Q tile-major, Z natural)
                 const float* va = halo_fwd ? pc : Z;
                 const float* vb = halo_fwd ? Q : nullptr;
                 const int TT = TN * TN;
+                const int64_t own = tile_major(t, il, jl, 0);
+                // branch-free offsets (dx, dy in {-1, 0, 1}): the +x line (0, jl) of
+                // tile t + TN^2 lies 512 TN^2 - 7*64 floats past the own line (7, jl),
+                // the -x line the same distance before (0, jl); likewise +-y
+                const int dx = (il == 7 && gi + 1 < N) ? 1 : ((il == 0 && gi > 0) ? -1 : 0);
+                const int dy = (jl == 7 && gj + 1 < N) ? 1 : ((jl == 0 && gj > 0) ? -1 : 0);
+                xi = dx != 0 ? gi + dx : -1;
+                yj = dy != 0 ? gj + dy : -1;
+                has_m = k0 > 0;
+                has_p = k0 + kTile < N;
+                const int dzm = has_m ? 1 : 0, dzp = has_p ? 1 : 0;
+                const int64_t lnx = own + (int64_t)dx * (512 * (int64_t)TT - 7 * 64);
+                const int64_t lny = own + (int64_t)dy * (512 * (int64_t)TN - 7 * 8);
+                const int64_t lm = own - (int64_t)dzm * (512 - (kTile - 1));
+                const int64_t lp = own + (int64_t)dzp * 512;
+                hfx = fl8[sgl + (int64_t)dx * ((int64_t)N * NS)];
+                hfy = fl8[sgl + (int64_t)dy * NS];
+                fm = fl8[sgl - dzm];
+                fp = fl8[sgl + dzp];
+                const float4 x0 = *reinterpret_cast<const float4*>(va + lnx);
+                const float4 x1 = *reinterpret_cast<const float4*>(va + lnx + 4);
+                const float4 y0 = *reinterpret_cast<const float4*>(va + lny);
+                const float4 y1 = *reinterpret_cast<const float4*>(va + lny + 4);
+                const float zmv = va[lm], zpv = va[lp];
+                hxv[0] = x0.x; hxv[1] = x0.y; hxv[2] = x0.z; hxv[3] = x0.w;
+                hxv[4] = x1.x; hxv[5] = x1.y; hxv[6] = x1.z; hxv[7] = x1.w;
+                hyv[0] = y0.x; hyv[1] = y0.y; hyv[2] = y0.z; hyv[3] = y0.w;
+                hyv[4] = y1.x; hyv[5] = y1.y; hyv[6] = y1.z; hyv[7] = y1.w;
+                vm = zmv;
+                vp = zpv;
+                if (vb) {
+                    const float4 bx0 = *reinterpret_cast<const float4*>(vb + lnx);
+                    const float4 bx1 = *reinterpret_cast<const float4*>(vb + lnx + 4);
+                    const float4 by0 = *reinterpret_cast<const float4*>(vb + lny);
+                    const float4 by1 = *reinterpret_cast<const float4*>(vb + lny + 4);
+                    const float bzm = vb[lm], bzp = vb[lp];
+                    const float bxv[8] = {bx0.x, bx0.y, bx0.z, bx0.w, bx1.x, bx1.y, bx1.z, bx1.w};
+                    const float byv[8] = {by0.x, by0.y, by0.z, by0.w, by1.x, by1.y, by1.z, by1.w};
+    #pragma unroll
+                    for (int kl = 0; kl < kTile; ++kl) {
+                        hxv[kl] = hxv[kl] * bxv[kl];
+                        hyv[kl] = hyv[kl] * byv[kl];
+                    }
+                    vm = vm * bzm;
+                    vp = vp * bzp;
+                }
+            }
+            if constexpr (AXPY) {
+                const float af = (float)red.sc->alpha;
+                const float naf = -af;
+    #pragma unroll
+                for (int m = 0; m < kTile; ++m) {
+                    const int64_t gt = tile_major(t, m, cj, ce);
+                    pv[m] = pv[m] + af * sv[m];
+                    rreg[m] = rreg[m] + naf * yv[m];
+                    sav[m] = rreg[m];
+                    kargs.P[gt] = pv[m];
+                    r[gt] = rreg[m];
+                    rr += (double)rreg[m] * (double)rreg[m];
+                }
+            }
+    #pragma unroll
+            for (int m = 0; m < kTile; ++m) {
+                const int li = (m * 8 + cj) * kPad + ce;
+                lds[o_pc + li] = pcv[m];
+                lds[o_sa + li] = sav[m];
+            }
+            if constexpr (halo) {
                 // lanes on a tile face own one halo line; at the domain boundary it
                 // holds -0.0
                 if (il == 0 || il == 7) {
                     float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
     #pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) d[kl] = -0.0f;
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = (xi >= 0 && ((hfx >> kl) & 1u)) ? hxv[kl] : -0.0f;
                 }
                 if (jl == 0 || jl == 7) {
                     float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
     #pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) d[kl] = -0.0f;
+                    for (int kl = 0; kl < kTile; ++kl) d[kl] = (yj >= 0 && ((hfy >> kl) & 1u)) ? hyv[kl] : -0.0f;
                 }
-                int xi = -1;
-                if (il == 0 && gi > 0) xi = gi - 1;
-                if (il == 7 && gi + 1 < N) xi = gi + 1;
-                if (xi >= 0) {
-                    const int64_t ln = halo_fwd ? tile_major(t + (xi < gi ? -TT : TT), xi & 7, jl, 0)
-                                                : cidx(N, xi, gj, k0);
-                    const uint32_t hf = fl8[((int64_t)xi * N + gj) * NS + tk];
-                    float* d = lds + o_hx + (il == 7) * kTile * kPad + jl * kPad;
-                    const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
-                    const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
-                    float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-                    if (vb) {
-                        const float4 b0 = *reinterpret_cast<const float4*>(vb + ln);
-                        const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
-                        float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-    #pragma unroll
-                        for (int kl = 0; kl < kTile; ++kl) av[kl] = av[kl] * bv[kl];
-                    }
-    #pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
-                }
-                int yj = -1;
-                if (jl == 0 && gj > 0) yj = gj - 1;
-                if (jl == 7 && gj + 1 < N) yj = gj + 1;
-                if (yj >= 0) {
-                    const int64_t ln = halo_fwd ? tile_major(t + (yj < gj ? -TN : TN), il, yj & 7, 0)
-                                                : cidx(N, gi, yj, k0);
-                    const uint32_t hf = fl8[((int64_t)gi * N + yj) * NS + tk];
-                    float* d = lds + o_hy + (jl == 7) * kTile * kPad + il * kPad;
-                    const float4 a0 = *reinterpret_cast<const float4*>(va + ln);
-                    const float4 a1 = *reinterpret_cast<const float4*>(va + ln + 4);
-                    float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-                    if (vb) {
-                        const float4 b0 = *reinterpret_cast<const float4*>(vb + ln);
-                        const float4 b1 = *reinterpret_cast<const float4*>(vb + ln + 4);
-                        float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-    #pragma unroll
-                        for (int kl = 0; kl < kTile; ++kl) av[kl] = av[kl] * bv[kl];
-                    }
-    #pragma unroll
-                    for (int kl = 0; kl < kTile; ++kl) d[kl] = ((hf >> kl) & 1u) ? av[kl] : -0.0f;
-                }
-                // z neighbours: values and fluid bits loaded together, masked after
-                const int64_t line = cidx(N, gi, gj, k0);
-                const int64_t sgl = ((int64_t)gi * N + gj) * NS + tk;
-                const bool has_m = k0 > 0, has_p = k0 + kTile < N;
-                int64_t lm = has_m ? line - 1 : line, lp = has_p ? line + kTile : line;
-                if (halo_fwd) {
-                    lm = has_m ? tile_major(t - 1, il, jl, kTile - 1) : tile_major(t, il, jl, 0);
-                    lp = has_p ? tile_major(t + 1, il, jl, 0) : tile_major(t, il, jl, 0);
-                }
-                const uint32_t fm = fl8[has_m ? sgl - 1 : sgl], fp = fl8[has_p ? sgl + 1 : sgl];
-                const float vm = vb ? va[lm] * vb[lm] : va[lm];
-                const float vp = vb ? va[lp] * vb[lp] : va[lp];
                 if (has_m && ((fm >> 7) & 1u)) hzm = vm;
                 if (has_p && (fp & 1u)) hzp = vp;
             }
@@ -1427,117 +1581,154 @@ struct KMicApply {
         if (active) {
     #pragma unroll
             for (int m = 0; m < kTile; ++m) {
-                const int64_t g = cidx(N, ti * kTile + m, tj * kTile + cj, k0 + ce);
+                const int64_t gt = tile_major(t, m, cj, ce);
                 const int li = (m * 8 + cj) * kPad + ce;
                 const float v = lds[o_sa + li];
                 if constexpr (do_bwd) {
-                    Z[g] = v;
+                    Z[gt] = v;
                     dot[0] += (double)v * (double)rreg[m];
                 } else {
-                    Q[tile_major(t, m, cj, ce)] = v;
+                    Q[gt] = v;
                 }
             }
         }
-        if constexpr (PH == 0) return;
-        block_partials<1>(dot, red.partials, slot_base + blockIdx.x);
+        if constexpr (PH == 0) {
+            if constexpr (AXPY) block_partial_comp(rr, red.partials, slot_base + (int)bl, 0);
+            return;
+        }
+        if constexpr (PH == 2) {
+            block_partial_comp(dot[0], red.partials, slot_base + (int)bl, 1);
+        } else {
+            double both[2] = {rr, dot[0]};
+            block_partials<2>(both, red.partials, slot_base + (int)bl);
+        }
     }
 };
 
-// r_old = r on the active segments (flexible PCG).
+// r_old = r on the active tiles (flexible PCG).
 struct KCopyList {
     static constexpr int kThreads = kBlock;
     struct Args {
         const float* src;
         float* dst;
-        const int32_t* segs;
-        int64_t nseg;
+        const int32_t* tiles;
+        int64_t ntiles;
         const SolverScalars* sc;
     };
     static __device__ void run(const Args& a) {
         if (a.sc->done) return;
-        const int64_t nseg = seg_count(a.nseg, a.sc);
-        const int64_t nb = list_grid(nseg);
+        const int64_t ntl = tile_count(a.ntiles, a.sc, 2);
+        const int64_t nb = tile_lines_grid(ntl);
         if ((int64_t)blockIdx.x >= nb) return;
-        const float4* __restrict__ S4 = reinterpret_cast<const float4*>(a.src);
-        float4* __restrict__ D4 = reinterpret_cast<float4*>(a.dst);
-        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += nb * blockDim.x) {
-            const int64_t c = 2 * (int64_t)a.segs[u];
-            D4[c] = S4[c];
-            D4[c + 1] = S4[c + 1];
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 64 * ntl;
+             e += nb * blockDim.x) {
+            const int64_t o = line_offset(a.tiles, e);
+            *reinterpret_cast<float4*>(a.dst + o) = *reinterpret_cast<const float4*>(a.src + o);
+            *reinterpret_cast<float4*>(a.dst + o + 4) = *reinterpret_cast<const float4*>(a.src + o + 4);
         }
     }
 };
 
-// z.r and z.r_old on the active segments (flexible PCG's Polak-Ribiere beta).
+// z.r and z.r_old on the active tiles (flexible PCG's Polak-Ribiere beta).
 struct KPRDot {
     static constexpr int kThreads = kBlock;
     struct Args {
         const float* z;
         const float* r;
         const float* r_old;
-        const int32_t* segs;
-        int64_t nseg;
+        const int32_t* tiles;
+        int64_t ntiles;
         Reduce red;
         int check;   // 1: only while the network is in use
     };
     static __device__ void run(const Args& a) {
         const SolverScalars* sc = a.red.sc;
         if (sc->done || (a.check && sc->use_mic)) return;
-        const int64_t nseg = seg_count(a.nseg, sc);
-        const int64_t nb = list_grid(nseg);
+        const int64_t ntl = tile_count(a.ntiles, sc, 2);
+        const int64_t nb = tile_lines_grid(ntl);
         if ((int64_t)blockIdx.x >= nb) return;
         double v[2] = {0.0, 0.0};
-        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += nb * blockDim.x) {
-            const int64_t c0 = 8 * (int64_t)a.segs[u];
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 64 * ntl;
+             e += nb * blockDim.x) {
+            const int64_t o = line_offset(a.tiles, e);
 #pragma unroll
             for (int m = 0; m < 8; ++m) {
-                const double zz = a.z[c0 + m];
-                v[0] += zz * (double)a.r[c0 + m];
-                v[1] += zz * (double)a.r_old[c0 + m];
+                const double zz = a.z[o + m];
+                v[0] += zz * (double)a.r[o + m];
+                v[1] += zz * (double)a.r_old[o + m];
             }
         }
         block_partials<2>(v, a.red.partials, blockIdx.x);
     }
 };
 
-// Jacobi: z = r * (1/diag), rho = z.r over the active segments
+// Jacobi: z = r * (1/diag), rho = z.r over the active tiles (pc tile-major).
 struct KJacobiApply {
     static constexpr int kThreads = kBlock;
     struct Args {
         const float* pc;
         const float* r;
         float* Z;
-        const int32_t* segs;
-        int64_t nseg;
+        const int32_t* tiles;
+        int64_t ntiles;
         Reduce red;
     };
     static __device__ void run(const Args& kargs) {
-        const float* __restrict__ pc = kargs.pc;
-        const float* __restrict__ r = kargs.r;
-        float* __restrict__ Z = kargs.Z;
-        const int32_t* __restrict__ segs = kargs.segs;
-        int64_t nseg = kargs.nseg;
         const Reduce& red = kargs.red;
-
         if (red.sc->done) return;
-        nseg = seg_count(nseg, red.sc);
-        const int64_t nb = list_grid(nseg);
+        const int64_t ntl = tile_count(kargs.ntiles, red.sc, 2);
+        const int64_t nb = tile_lines_grid(ntl);
         if ((int64_t)blockIdx.x >= nb) return;
         double v[1] = {0.0};
-        for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nseg;
-             u += nb * blockDim.x) {
-            const int64_t c0 = 8 * (int64_t)segs[u];
-    #pragma unroll
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 64 * ntl;
+             e += nb * blockDim.x) {
+            const int64_t o = line_offset(kargs.tiles, e);
+#pragma unroll
             for (int m = 0; m < 8; ++m) {
-                const int64_t c = c0 + m;
-                const float z = r[c] * pc[c];
-                Z[c] = z;
-                v[0] += (double)z * (double)r[c];
+                const float r = kargs.r[o + m];
+                const float z = r * kargs.pc[o + m];
+                kargs.Z[o + m] = z;
+                v[0] += (double)z * (double)r;
             }
         }
-        block_partials<1>(v, red.partials, blockIdx.x);
+        double both[2] = {0.0, v[0]};
+        block_partials<2>(both, red.partials, blockIdx.x);
+    }
+};
+
+// Layout conversion over the active tiles: to_tiles 1: dst (tile-major) = src
+// (natural); 0: dst (natural) = src (tile-major). gate 1: only while flexible
+// PCG is on the network and not done (the network's input and output).
+struct KTileConvert {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int N;
+        const float* src;
+        float* dst;
+        const int32_t* tiles;
+        int64_t ntiles;
+        const SolverScalars* sc;
+        int to_tiles;
+        int gate;
+    };
+    static __device__ void run(const Args& a) {
+        if (a.gate && (a.sc->done || !a.sc->flex || a.sc->use_mic)) return;
+        const int64_t ntl = tile_count(a.ntiles, a.sc, 2);
+        const int64_t nb = tile_lines_grid(ntl);
+        if ((int64_t)blockIdx.x >= nb) return;
+        const int TN = a.N / kTile;
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 64 * ntl;
+             e += nb * blockDim.x) {
+            const int64_t t = a.tiles[e >> 6];
+            const int line = (int)(e & 63), il = line >> 3, jl = line & 7;
+            const int tk = (int)(t % TN), tj = (int)((t / TN) % TN), ti = (int)(t / ((int64_t)TN * TN));
+            const int64_t nat = cidx(a.N, ti * kTile + il, tj * kTile + jl, tk * kTile);
+            const int64_t tmo = t * 512 + line * 8;
+            const float* src = a.src + (a.to_tiles ? nat : tmo);
+            float* dst = a.dst + (a.to_tiles ? tmo : nat);
+            *reinterpret_cast<float4*>(dst) = *reinterpret_cast<const float4*>(src);
+            *reinterpret_cast<float4*>(dst + 4) = *reinterpret_cast<const float4*>(src + 4);
+        }
     }
 };
 
@@ -1692,7 +1883,6 @@ void launch_precond_setup(hipStream_t s, const Params& p, const uint8_t* A, cons
 
 void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, const float* diag,
                           const uint8_t* nup, float* pc, const Active& act, SolverScalars* sc) {
-    if (p.precond == 2) return;   // Jacobi: built by setup
     auto nb = [&](int w) {
         const int64_t n = act.ntiles[w] >= 0 ? act.ntiles[w] : act.tile_cap[w];
         return (unsigned)std::max<int64_t>(1, std::min<int64_t>(n, 4096));
@@ -1700,7 +1890,7 @@ void launch_precond_build(hipStream_t s, const Params& p, const uint8_t* A, cons
     if (p.precond == 0) {
         launch<KMicBuild>(s, dim3(nb(0)), dim3(64), {p, A, diag, nup, pc, 0, act.tiles[0], act.ntiles[0], sc});
         launch<KMicBuild>(s, dim3(nb(1)), dim3(64), {p, A, diag, nup, pc, 1, act.tiles[1], act.ntiles[1], sc});
-    } else {
+    } else if (p.precond != 2) {   // Jacobi's 1/diag comes from the set-up
         launch<KMicBuild>(s, dim3(nb(2)), dim3(64), {p, A, diag, nup, pc, -1, act.tiles[2], act.ntiles[2], sc});
     }
     const int64_t nc = (int64_t)p.N * p.N * p.N;
@@ -1731,65 +1921,71 @@ static int64_t seg_or_cap(const Active& a) { return a.nseg >= 0 ? a.nseg : a.seg
 static int64_t tiles_or_cap(const Active& a, int w) { return a.ntiles[w] >= 0 ? a.ntiles[w] : a.tile_cap[w]; }
 
 void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, const float* pc,
-                          const float* r, float* q, float* z, const Reduce& red,
-                          const Active& act) {
+                          float* r, float* q, float* z, const Reduce& red,
+                          const Active& act, const IterAxpy* ax) {
     const bool dev = act.nseg < 0;
+    const float* pct = pc + (int64_t)p.N * p.N * p.N;   // the tile-major copy
     if (p.precond == 2) {
-        const int nb = (int)list_grid(seg_or_cap(act));
-        launch<KJacobiApply>(s, dim3(nb), dim3(kBlock), {pc, r, z, act.segs, act.nseg, red});
-        launch_finalize(s, red, FIN_RHO, dev ? PARTS_LIST : nb);
+        const int nb = (int)tile_lines_grid(tiles_or_cap(act, 2));
+        launch<KJacobiApply>(s, dim3(nb), dim3(kBlock), {pct, r, z, act.tiles[2], act.ntiles[2], red});
+        launch_finalize(s, red, FIN_RHO, dev ? PARTS_TILE_LINES : nb);
         return;
     }
-    const float* pct = pc + (int64_t)p.N * p.N * p.N;   // the tile-major copy (MIC(0) only)
     if (p.precond == 0) {
         const int nr = (int)mic_grid(tiles_or_cap(act, 0)), nbk = (int)mic_grid(tiles_or_cap(act, 1));
-        launch<KMicApply<0>>(s, dim3(nr), dim3(256), {p, fl8, pct, r, q, z, act.tiles[0], act.ntiles[0], red, 0});
-        launch<KMicApply<1>>(s, dim3(nbk), dim3(256), {p, fl8, pct, r, q, z, act.tiles[1], act.ntiles[1], red, 0});
-        launch<KMicApply<2>>(s, dim3(nr), dim3(256), {p, fl8, pct, r, q, z, act.tiles[0], act.ntiles[0], red, dev ? -1 : nbk});
-        launch_finalize(s, red, FIN_RHO, dev ? PARTS_MIC_RB : nbk + nr);
+        const int red_base = dev ? -1 : nbk;
+        if (ax) {
+            launch<KMicApply<0, true>>(s, dim3((unsigned)xcd_grid(nr)), dim3(256), {p, fl8, pct, r, q, z, act.tiles[0], act.ntiles[0], red, red_base, ax->P, ax->S, ax->Y, ax->nc});
+            launch<KMicApply<1, true>>(s, dim3((unsigned)xcd_grid(nbk)), dim3(256), {p, fl8, pct, r, q, z, act.tiles[1], act.ntiles[1], red, 0, ax->P, ax->S, ax->Y, ax->nc});
+        } else {
+            launch<KMicApply<0>>(s, dim3((unsigned)xcd_grid(nr)), dim3(256), {p, fl8, pct, r, q, z, act.tiles[0], act.ntiles[0], red, red_base, nullptr, nullptr, nullptr, 0});
+            launch<KMicApply<1>>(s, dim3((unsigned)xcd_grid(nbk)), dim3(256), {p, fl8, pct, r, q, z, act.tiles[1], act.ntiles[1], red, 0, nullptr, nullptr, nullptr, 0});
+        }
+        launch<KMicApply<2>>(s, dim3((unsigned)xcd_grid(nr)), dim3(256), {p, fl8, pct, r, q, z, act.tiles[0], act.ntiles[0], red, red_base, nullptr, nullptr, nullptr, 0});
+        launch_finalize(s, red, ax ? FIN_RHO_RNORM : FIN_RHO, dev ? PARTS_MIC_RB : nbk + nr);
     } else {
         const int nb = (int)mic_grid(tiles_or_cap(act, 2));
-        launch<KMicApply<3>>(s, dim3(nb), dim3(256), {p, fl8, pct, r, q, z, act.tiles[2], act.ntiles[2], red, 0});
+        launch<KMicApply<3>>(s, dim3((unsigned)xcd_grid(nb)), dim3(256), {p, fl8, pct, r, q, z, act.tiles[2], act.ntiles[2], red, 0, nullptr, nullptr, nullptr, 0});
         launch_finalize(s, red, FIN_RHO, dev ? PARTS_MIC_ALL : nb);
     }
 }
 
 void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L, float* P,
-                      const float* D, float* R, const Reduce& red, double tol, int stop_rule,
-                      double* P64, int refine) {
+                      const float* D, float* Rt, const Reduce& red, double tol, int stop_rule,
+                      double* P64, int refine, float* Pt) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
     const int nb = (int)list_grid(nc);
-    launch<KResidual0>(s, dim3(nb), dim3(kBlock), {p, A, L, P, D, R, red, tol, stop_rule, P64, refine});
+    launch<KResidual0>(s, dim3(nb), dim3(kBlock), {p, A, L, P, D, Rt, red, P64, refine, Pt});
     launch_finalize(s, red, FIN_RESID0, nb, tol, stop_rule, p.N);
 }
 
 void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                     const float* x, float* y, const Reduce& red, const Active& act) {
-    const int64_t nq = 2 * seg_or_cap(act);   // quads
-    const int64_t nb = spmv_grid(nq);
-    launch<KSpmvDot>(s, dim3((unsigned)xcd_grid(nb)), dim3(kBlock), {p, A, L, x, y, act.segs, act.nseg, spmv_per_thread(nq), red});
-    launch_finalize(s, red, FIN_SIGMA, act.nseg < 0 ? PARTS_SPMV : (int)nb);
+                     const uint8_t* fl8, const float* Zt, float* St, float* Yt, const Reduce& red,
+                     const Active& act) {
+    const int nb = (int)mic_grid(tiles_or_cap(act, 2));
+    launch<KTileSpmv>(s, dim3((unsigned)xcd_grid(nb)), dim3(256), {p, A, L, fl8, Zt, St, Yt, act.tiles[2], act.ntiles[2], red});
+    launch_finalize(s, red, FIN_SIGMA, act.ntiles[2] < 0 ? PARTS_MIC_ALL : nb);
 }
 
-void launch_axpy2(hipStream_t s, float* P, const float* S, float* R, const float* Z,
+void launch_axpy2(hipStream_t s, float* Pt, const float* St, float* Rt, const float* Yt, int64_t nc,
                   const Reduce& red, const Active& act) {
-    const int nb = (int)list_grid(seg_or_cap(act));
-    launch<KAxpy2>(s, dim3(nb), dim3(kBlock), {P, S, R, Z, act.segs, act.nseg, red});
-    launch_finalize(s, red, FIN_RNORM, act.nseg < 0 ? PARTS_LIST : nb);
+    const int nb = (int)tile_lines_grid(tiles_or_cap(act, 2));
+    launch<KAxpy2>(s, dim3(nb), dim3(kBlock), {Pt, St, Rt, Yt, act.tiles[2], act.ntiles[2], nc, red});
+    launch_finalize(s, red, FIN_RNORM, act.ntiles[2] < 0 ? PARTS_TILE_LINES : nb);
 }
 
 void launch_reliable_update(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                            float* P, double* P64, const float* D, float* R, const Reduce& red,
+                            float* Pt, double* P64, const float* D, float* Rt, const Reduce& red,
                             const Active& act) {
     const int nb = (int)list_grid(seg_or_cap(act));
-    launch<KFlush>(s, dim3(nb), dim3(kBlock), {A, P, P64, act.segs, act.nseg, (const SolverScalars*)red.sc});
-    launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, (const float*)P, (const double*)P64, D, R, act.segs, act.nseg, red, 0});
+    launch<KFlush>(s, dim3(nb), dim3(kBlock), {p.N, A, Pt, P64, act.segs, act.nseg, (const SolverScalars*)red.sc, 0});
+    launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, (const float*)nullptr, (const double*)P64, D, Rt, act.segs, act.nseg, red, 0});
     launch_finalize(s, red, FIN_TRUE, act.nseg < 0 ? PARTS_LIST : nb);
 }
 
-void launch_refine_finish(hipStream_t s, const uint8_t* A, float* P, const double* P64,
-                          const Active& act, const SolverScalars* sc) {
-    launch<KRefineFinish>(s, dim3((unsigned)list_grid(seg_or_cap(act))), dim3(kBlock), {A, P, P64, act.segs, act.nseg, sc});
+void launch_refine_finish(hipStream_t s, int N, const uint8_t* A, float* P, const float* Pt,
+                          const double* P64, const Active& act, const SolverScalars* sc) {
+    launch<KRefineFinish>(s, dim3((unsigned)list_grid(seg_or_cap(act))), dim3(kBlock), {N, A, P, Pt, P64, act.segs, act.nseg, sc});
 }
 
 void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
@@ -1801,29 +1997,30 @@ void launch_true_residual_norm(hipStream_t s, const Params& p, const uint8_t* A,
 }
 
 void launch_true_residual_norm64(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
-                                 float* P, double* P64, const float* D, const Reduce& red,
+                                 float* Pt, double* P64, const float* D, const Reduce& red,
                                  const Active& act) {
     const int nb = (int)list_grid(seg_or_cap(act));
-    launch<KFlush>(s, dim3(nb), dim3(kBlock), {A, P, P64, act.segs, act.nseg, (const SolverScalars*)red.sc, 1});
-    launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, (const float*)P, (const double*)P64, D, (float*)nullptr, act.segs, act.nseg, red, 3});
+    launch<KFlush>(s, dim3(nb), dim3(kBlock), {p.N, A, Pt, P64, act.segs, act.nseg, (const SolverScalars*)red.sc, 1});
+    launch<KTrueResidual>(s, dim3(nb), dim3(kBlock), {p, A, L, (const float*)nullptr, (const double*)P64, D, (float*)nullptr, act.segs, act.nseg, red, 3});
     launch_finalize(s, red, FIN_REPORT64, act.nseg < 0 ? PARTS_LIST : nb);
-}
-
-void launch_update_s(hipStream_t s, const float* Z, float* S, const Reduce& red, int first,
-                     const Active& act) {
-    launch<KUpdateS>(s, dim3((unsigned)list_grid(seg_or_cap(act))), dim3(kBlock), {Z, S, act.segs, act.nseg, (const SolverScalars*)red.sc, first});
 }
 
 void launch_copy_list(hipStream_t s, const float* src, float* dst, const Active& act,
                       const SolverScalars* sc) {
-    launch<KCopyList>(s, dim3((unsigned)list_grid(seg_or_cap(act))), dim3(kBlock), {src, dst, act.segs, act.nseg, sc});
+    launch<KCopyList>(s, dim3((unsigned)tile_lines_grid(tiles_or_cap(act, 2))), dim3(kBlock), {src, dst, act.tiles[2], act.ntiles[2], sc});
 }
 
 void launch_pr_dot(hipStream_t s, const float* z, const float* r, const float* r_old,
                    const Reduce& red, const Active& act, bool check) {
-    const int nb = (int)list_grid(seg_or_cap(act));
-    launch<KPRDot>(s, dim3(nb), dim3(kBlock), {z, r, r_old, act.segs, act.nseg, red, check ? 1 : 0});
-    launch_finalize(s, red, check ? FIN_PR_CHECK : FIN_PR, act.nseg < 0 ? PARTS_LIST : nb);
+    const int nb = (int)tile_lines_grid(tiles_or_cap(act, 2));
+    launch<KPRDot>(s, dim3(nb), dim3(kBlock), {z, r, r_old, act.tiles[2], act.ntiles[2], red, check ? 1 : 0});
+    launch_finalize(s, red, check ? FIN_PR_CHECK : FIN_PR, act.ntiles[2] < 0 ? PARTS_TILE_LINES : nb);
+}
+
+void launch_tile_convert(hipStream_t s, int N, const float* src, float* dst, const Active& act,
+                         const SolverScalars* sc, bool to_tiles, bool gate) {
+    launch<KTileConvert>(s, dim3((unsigned)tile_lines_grid(tiles_or_cap(act, 2))), dim3(kBlock),
+                         {N, src, dst, act.tiles[2], act.ntiles[2], sc, to_tiles ? 1 : 0, gate ? 1 : 0});
 }
 
 void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const float* L,

@@ -85,30 +85,97 @@ int prepare_solve(ariel_sim* h) {
     return check_launch(h, "preconditioner build");
 }
 
-// BuildPreconditioner + SolveConjugateGradient (solver.inl:104-135, 289-354) with
-// the relative stop rule, FP64 reductions and residual replacement. The
-// iterations touch the active segments and tiles only. The full-grid set-up
-// below leaves P and R zero off-FLUID and Z zero everywhere; S and Z are then
-// written on every active segment before they are read, and values off the
-// active sets are never read as FLUID values.
-int stage_solve_learned(ariel_sim* h);
+void learned_precond(ariel_sim* h, const Params& p, const Reduce& red, bool mic_only);
 
+// The launches of a solve after the preconditioner build, shared by the single
+// tank path below and the batched tanks' recorded phases.
+//
+// Set-up: r = b - A p from the warm start into the tile-major Rt, the iterate
+// into Pt (with reliable updates x64 = p and Pt = 0), then z = M^-1 r. The
+// first iteration's operator pass takes s = z.
+void enqueue_solve_setup(ariel_sim* h, bool learned) {
+    const Params p = h->params();
+    const Reduce red = h->reduce();
+    hipStream_t s = h->stream;
+    const Active& act = h->act;
+    const int refine = h->d.refine_every > 0;
+    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->Rt, red, (double)h->d.tol, h->d.stop_rule,
+                     h->P64, refine, h->Pt);
+    if (refine) launch_fill_f32(s, h->Pt, h->nc, 0.0f);
+    if (!learned) {
+        launch_precond_apply(s, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, red, act);
+        return;
+    }
+    // the network reads r and writes z in the natural layout; both are zero
+    // outside the active tiles
+    launch_fill_f32(s, h->R, h->nc, 0.0f);
+    launch_fill_f32(s, h->Z, h->nc, 0.0f);
+    launch_fill_f32(s, h->R_old, h->nc, 0.0f);
+    launch_diag_weight(s, h->N, h->A, h->diag, h->net.wd);
+    learned_precond(h, p, red, false);
+}
+
+// One PCG iteration: s and y = A s (alpha), p += alpha s and r -= alpha y (the
+// stop test), the reliable update when due, z = M^-1 r (beta). `it` is the
+// iteration's number (1-based) for sampled profiling; prof = 0 disables it.
+void enqueue_iteration(ariel_sim* h, int it, bool refine, bool learned, bool mic_only, bool prof) {
+    const Params p = h->params();
+    const Reduce red = h->reduce();
+    hipStream_t s = h->stream;
+    const Active& act = h->act;
+    const bool sampled = prof && prof_sampled(it);
+    {
+        Prof pr(h, ARIEL_KERNEL_SPMV, sampled);
+        launch_spmv_dot(s, p, h->A, h->L, h->fl8, h->Zt, h->St, h->Yt, red, act);
+    }
+    // the tiled red-black MIC(0) carries the AXPY in its first pass over each
+    // colour, except where the reliable update must come between the two
+    if (!learned && !refine && p.precond == ARIEL_PRECOND_MIC0_TILED) {
+        const IterAxpy ax{h->Pt, h->St, h->Yt, h->nc};
+        Prof pr(h, ARIEL_KERNEL_PRECOND, sampled);
+        launch_precond_apply(s, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, red, act, &ax);
+        return;
+    }
+    {
+        Prof pr(h, ARIEL_KERNEL_AXPY, sampled);
+        if (learned) launch_copy_list(s, h->Rt, h->R_old, act, h->sc);
+        launch_axpy2(s, h->Pt, h->St, h->Rt, h->Yt, h->nc, red, act);
+    }
+    if (refine) launch_reliable_update(s, p, h->A, h->L, h->Pt, h->P64, h->D, h->Rt, red, act);
+    {
+        Prof pr(h, ARIEL_KERNEL_PRECOND, sampled);
+        if (learned) learned_precond(h, p, red, mic_only);
+        else launch_precond_apply(s, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, red, act);
+    }
+}
+
+// After the iterations: the pressure (natural) from the iterate; with reliable
+// updates first x64 += p and the FP64 solution's true residual into sc->tmp64.
+void enqueue_solve_finish(ariel_sim* h) {
+    const Params p = h->params();
+    const Reduce red = h->reduce();
+    hipStream_t s = h->stream;
+    const Active& act = h->act;
+    const bool refine = h->d.refine_every > 0;
+    if (refine) launch_true_residual_norm64(s, p, h->A, h->L, h->Pt, h->P64, h->D, red, act);
+    launch_refine_finish(s, h->N, h->A, h->P, h->Pt, refine ? h->P64 : nullptr, act, h->sc);
+}
+
+// BuildPreconditioner + SolveConjugateGradient (solver.inl:104-135, 289-354) with
+// the relative stop rule, FP64 reductions and residual replacement, or flexible
+// PCG on the learned network. The iterations touch the active tiles only.
 int stage_solve(ariel_sim* h) {
-    if (h->d.precond == ARIEL_PRECOND_LEARNED) return stage_solve_learned(h);
+    const bool learned = h->d.precond == ARIEL_PRECOND_LEARNED;
+    if (learned && !h->net_ready)
+        return fail(h, ARIEL_ERR_STATE, "learned preconditioner: no weights (ariel_set_learned_weights)");
     const Params p = h->params();
     const Reduce red = h->reduce();
     hipStream_t s = h->stream;
     HIPCHK(h, hipEventRecord(h->solve_t0, s));
     TRY(prepare_solve(h));
-    const Active& act = h->act;
-    const int refine = h->d.refine_every > 0;
-    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
-                     h->P64, refine);
-    if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
-    launch_fill_f32(s, h->Z, h->nc, 0.0f);
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, act);
-    launch_update_s(s, h->Z, h->S, red, 1, act);
+    enqueue_solve_setup(h, learned);
     TRY(check_launch(h, "solve setup"));
+    bool on_mic = false;
     int queued = 0;
     const int k = h->d.refine_every;
     ChunkPlan plan(h->last_iters);
@@ -117,42 +184,29 @@ int stage_solve(ariel_sim* h) {
         const int n = std::min(chunk, h->d.max_iters - queued);
         for (int j = 0; j < n; ++j) {
             const int it = queued + j + 1;
-            {
-                Prof prof(h, ARIEL_KERNEL_SPMV, prof_sampled(it));
-                launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, act);
-            }
-            {
-                Prof prof(h, ARIEL_KERNEL_AXPY, prof_sampled(it));
-                launch_axpy2(s, h->P, h->S, h->R, h->Z, red, act);
-            }
-            if (k > 0 && it % k == 0)
-                launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, act);
-            {
-                Prof prof(h, ARIEL_KERNEL_PRECOND, prof_sampled(it));
-                launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, act);
-            }
-            launch_update_s(s, h->Z, h->S, red, 0, act);
+            enqueue_iteration(h, it, k > 0 && it % k == 0, learned, on_mic, true);
         }
         TRY(check_launch(h, "pcg"));
         queued += n;
         TRY(read_scalars(h));
+        on_mic = on_mic || h->sc_host->use_mic;
         if (h->sc_host->done || queued >= h->d.max_iters) break;
         chunk = plan.next(*h->sc_host);
     }
-    if (refine) {
-        launch_true_residual_norm64(s, p, h->A, h->L, h->P, h->P64, h->D, red, act);
-        launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
-    }
+    enqueue_solve_finish(h);
     HIPCHK(h, hipEventRecord(h->solve_t1, s));
     h->last_iters = h->sc_host->it;
     h->stats.iterations = h->sc_host->it;
     h->stats.solve_status = h->sc_host->status;
     h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
     h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
+    h->stats.precond_fallback = learned && h->sc_host->use_mic ? h->sc_host->fallback_reason : 0;
+    h->stats.fallback_iter = learned && h->sc_host->use_mic ? h->sc_host->fallback_it : 0;
     // true residual of the final pressure, for the report only
-    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, act);
+    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, h->act);
     TRY(check_launch(h, "true residual"));
     TRY(read_scalars(h));
+    const bool refine = h->d.refine_every > 0;
     h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
     h->stats.true_residual_norm64 = refine ? std::sqrt(h->sc_host->tmp64) : h->stats.true_residual_norm;
     return ARIEL_OK;
@@ -165,87 +219,18 @@ int stage_solve(ariel_sim* h) {
 // the solve switches to MIC(0) for good and restarts its search direction. The
 // network and MIC(0) kernels are both queued each iteration; the one not in use
 // returns at once. Once the host has seen the switch (at a readback), it
-// queues the MIC(0) kernels only.
+// queues the MIC(0) kernels only. The network reads r and writes z in the
+// natural layout, converted from and to the tile-major vectors around it.
 void learned_precond(ariel_sim* h, const Params& p, const Reduce& red, bool mic_only) {
     hipStream_t s = h->stream;
     if (!mic_only) {
+        launch_tile_convert(s, h->N, h->Rt, h->R, h->act, h->sc, false, true);
         launch_learned_apply(s, h->net, h->N, h->R, h->A, h->diag, h->Z, h->sc, h->act.nseg);
-        launch_pr_dot(s, h->Z, h->R, h->R_old, red, h->act, true);
+        launch_tile_convert(s, h->N, h->Z, h->Zt, h->act, h->sc, true, true);
+        launch_pr_dot(s, h->Zt, h->Rt, h->R_old, red, h->act, true);
     }
-    launch_precond_apply(s, p, h->fl8, h->pc, h->R, h->Q, h->Z, red, h->act);
-    launch_pr_dot(s, h->Z, h->R, h->R_old, red, h->act, false);
-}
-
-int stage_solve_learned(ariel_sim* h) {
-    if (!h->net_ready)
-        return fail(h, ARIEL_ERR_STATE, "learned preconditioner: no weights (ariel_set_learned_weights)");
-    const Params p = h->params();
-    const Reduce red = h->reduce();
-    hipStream_t s = h->stream;
-    HIPCHK(h, hipEventRecord(h->solve_t0, s));
-    TRY(prepare_solve(h));
-    const Active& act = h->act;
-    const int refine = h->d.refine_every > 0;
-    launch_residual0(s, p, h->A, h->L, h->P, h->D, h->R, red, (double)h->d.tol, h->d.stop_rule,
-                     h->P64, refine);
-    if (refine) launch_fill_f32(s, h->P, h->nc, 0.0f);
-    launch_fill_f32(s, h->Z, h->nc, 0.0f);
-    launch_fill_f32(s, h->R_old, h->nc, 0.0f);
-    launch_diag_weight(s, h->N, h->A, h->diag, h->net.wd);
-    learned_precond(h, p, red, false);
-    launch_update_s(s, h->Z, h->S, red, 1, act);
-    TRY(check_launch(h, "learned solve setup"));
-    bool on_mic = false;
-    int queued = 0;
-    const int k = h->d.refine_every;
-    ChunkPlan plan(h->last_iters);
-    int chunk = plan.first();
-    for (;;) {
-        const int n = std::min(chunk, h->d.max_iters - queued);
-        for (int j = 0; j < n; ++j) {
-            const int it = queued + j + 1;
-            {
-                Prof prof(h, ARIEL_KERNEL_SPMV, prof_sampled(it));
-                launch_spmv_dot(s, p, h->A, h->L, h->S, h->Z, red, act);
-            }
-            {
-                Prof prof(h, ARIEL_KERNEL_AXPY, prof_sampled(it));
-                launch_copy_list(s, h->R, h->R_old, act, h->sc);
-                launch_axpy2(s, h->P, h->S, h->R, h->Z, red, act);
-            }
-            if (k > 0 && it % k == 0)
-                launch_reliable_update(s, p, h->A, h->L, h->P, h->P64, h->D, h->R, red, act);
-            {
-                Prof prof(h, ARIEL_KERNEL_PRECOND, prof_sampled(it));
-                learned_precond(h, p, red, on_mic);
-            }
-            launch_update_s(s, h->Z, h->S, red, 0, act);
-        }
-        TRY(check_launch(h, "learned pcg"));
-        queued += n;
-        TRY(read_scalars(h));
-        on_mic = on_mic || h->sc_host->use_mic;
-        if (h->sc_host->done || queued >= h->d.max_iters) break;
-        chunk = plan.next(*h->sc_host);
-    }
-    if (refine) {
-        launch_true_residual_norm64(s, p, h->A, h->L, h->P, h->P64, h->D, red, act);
-        launch_refine_finish(s, h->A, h->P, h->P64, act, h->sc);
-    }
-    HIPCHK(h, hipEventRecord(h->solve_t1, s));
-    h->last_iters = h->sc_host->it;
-    h->stats.iterations = h->sc_host->it;
-    h->stats.solve_status = h->sc_host->status;
-    h->stats.rhs_norm = std::sqrt(h->sc_host->bnorm2);
-    h->stats.residual_norm = std::sqrt(h->sc_host->rnorm2);
-    h->stats.precond_fallback = h->sc_host->use_mic ? h->sc_host->fallback_reason : 0;
-    h->stats.fallback_iter = h->sc_host->use_mic ? h->sc_host->fallback_it : 0;
-    launch_true_residual_norm(s, p, h->A, h->L, h->P, h->D, red, act);
-    TRY(check_launch(h, "true residual"));
-    TRY(read_scalars(h));
-    h->stats.true_residual_norm = std::sqrt(h->sc_host->tmp);
-    h->stats.true_residual_norm64 = refine ? std::sqrt(h->sc_host->tmp64) : h->stats.true_residual_norm;
-    return ARIEL_OK;
+    launch_precond_apply(s, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, red, h->act);
+    launch_pr_dot(s, h->Zt, h->Rt, h->R_old, red, h->act, false);
 }
 
 }  // namespace host
