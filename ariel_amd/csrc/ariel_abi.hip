@@ -449,7 +449,8 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->A, h->nc)) || (rc = dalloc(h, &h->R, h->nc)) || (rc = dalloc(h, &h->Z, h->nc)) ||
         (rc = dalloc(h, &h->S, h->nc)) || (rc = dalloc(h, &h->Q, h->nc)) || (rc = dalloc(h, &h->pc, 2 * h->nc)) ||
         (rc = dalloc(h, &h->Rt, h->nc)) || (rc = dalloc(h, &h->Zt, h->nc)) || (rc = dalloc(h, &h->St, 2 * h->nc)) ||
-        (rc = dalloc(h, &h->Yt, h->nc)) || (rc = dalloc(h, &h->Pt, h->nc)) ||
+        (rc = dalloc(h, &h->Yt, h->nc)) || (rc = dalloc(h, &h->Pt, h->nc)) || (rc = dalloc(h, &h->Gt, h->nc)) ||
+        (rc = dalloc(h, &h->sb, h->nc / 8)) ||
         (rc = dalloc(h, &h->diag, h->nc)) || (rc = dalloc(h, &h->nup, h->nc)) ||
         (rc = dalloc(h, &h->fl8, h->nc / 8)) ||
         (rc = dalloc(h, &h->counts, h->nc)) || (rc = dalloc(h, &h->offsets, h->nc + 1)) ||
@@ -505,7 +506,7 @@ int ariel_destroy(ariel_sim* h) {
     for (void* p : {(void*)h->cell_of, (void*)h->slots, (void*)h->sid, (void*)h->counts, (void*)h->offsets,
                     (void*)h->scan_sums, (void*)h->accept, (void*)h->P, (void*)h->D, (void*)h->L,
                     (void*)h->A, (void*)h->R, (void*)h->Z, (void*)h->S, (void*)h->Q, (void*)h->pc,
-                    (void*)h->Rt, (void*)h->Zt, (void*)h->St, (void*)h->Yt, (void*)h->Pt,
+                    (void*)h->Rt, (void*)h->Zt, (void*)h->St, (void*)h->Yt, (void*)h->Pt, (void*)h->Gt, (void*)h->sb,
                     (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc,
                     (void*)h->seg_flag, (void*)h->seg_scan, (void*)h->seg_list, (void*)h->tile_any,
                     (void*)h->tile_lists, (void*)h->net_mem, (void*)h->R_old, (void*)h->net.wd})

@@ -81,6 +81,8 @@ struct ariel_sim {
     // iterate Pt, and Q, the MIC(0) apply's intermediate.
     float *R = nullptr, *Z = nullptr, *S = nullptr, *Q = nullptr, *pc = nullptr, *diag = nullptr;
     float *Rt = nullptr, *Zt = nullptr, *St = nullptr, *Yt = nullptr, *Pt = nullptr;
+    float* Gt = nullptr;   // ghost level set of the solve (L, NaN at SOLID), tile-major
+    uint8_t* sb = nullptr;  // SOLID bits per tile-major 8-cell line
     uint8_t* nup = nullptr;
     uint8_t* fl8 = nullptr;       // fluid bits per 8-cell z-segment
     double* P64 = nullptr;        // FP64 solution of the refined solve

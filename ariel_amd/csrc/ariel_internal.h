@@ -272,9 +272,14 @@ void launch_precond_apply(hipStream_t s, const Params& p, const uint8_t* fl8, co
 void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
                       float* P, const float* D, float* Rt, const Reduce& red, double tol,
                       int stop_rule, double* P64, int refine, float* Pt);
+// The ghost level set G of a solve (L with NaN at SOLID cells) and the SOLID
+// bits sb of every 8-cell line, both tile-major over every tile (the
+// operator's ghost terms in the tile-major solve).
+void launch_ghost_levelset(hipStream_t s, int N, const uint8_t* A, const float* L, float* G,
+                           uint8_t* sb);
 // s = z (first iteration) or z + beta s_prev, y = A s, alpha = rho / s.y. St
 // holds two tile-major buffers selected by the iteration's parity.
-void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+void launch_spmv_dot(hipStream_t s, const Params& p, const float* G, const uint8_t* sb,
                      const uint8_t* fl8, const float* Zt, float* St, float* Yt, const Reduce& red,
                      const Active& act);
 // p += alpha s, r -= alpha y, ||r||^2 and the stop test (tile-major).

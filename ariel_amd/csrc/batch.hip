@@ -75,6 +75,7 @@ void record_pre(ariel_sim* h, Recorder& r) {
     hipStream_t s = h->stream;
     launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
     launch_fluid_segments(s, h->N, h->A, h->fl8);
+    launch_ghost_levelset(s, h->N, h->A, h->L, h->Gt, h->sb);
     launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every);
     launch_active_sets(s, h->N, h->fl8, h->seg_flag, h->seg_scan, h->seg_list, h->tile_any,
                        h->tile_lists, h->scan_sums, h->sc);

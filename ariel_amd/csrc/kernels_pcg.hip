@@ -312,14 +312,14 @@ __host__ __device__ inline int64_t mic_grid(int64_t ntiles) {
 // One XRef term (solver.inl:138-154) of FLUID cell c with value xc: out of range
 // -> xc (the clamped index is c itself); FLUID neighbour -> its value xq; SOLID
 // -> xc; AIR -> L_q / min(1e-6, L_c) * xc. q = c + dq in the natural layout.
-// The same term from preloaded values: the neighbour's class aq and level set
-// lq, the cell's level set lc.
-__device__ __forceinline__ float xref_term(bool in_range, bool nb_fluid, float xq, uint32_t aq,
-                                           float lq, float lc, float xc) {
+// The same term from the ghost level set G (L, with NaN at SOLID cells): the
+// neighbour's gq and the cell's gc = L_c (a FLUID cell is never SOLID).
+__device__ __forceinline__ float xref_term(bool in_range, bool nb_fluid, float xq, float gq,
+                                           float gc, float xc) {
     if (!in_range) return xc;
     if (nb_fluid) return xq;
-    if (aq == (uint32_t)kSolid) return xc;
-    return lq / gmin(1.0e-6f, lc) * xc;
+    if (gq != gq) return xc;   // SOLID
+    return gq / gmin(1.0e-6f, gc) * xc;
 }
 
 // 8 floats of a tile-major line: s = z + b * s_prev, b = 0 for the first
@@ -341,15 +341,56 @@ __device__ __forceinline__ float search_cell(const float* __restrict__ Z, const 
                                              int64_t off, float b) {
     return Z[off] + b * Sp[off];
 }
-__device__ __forceinline__ uint32_t byte_of(uint2 v, int k) {
-    return ((k < 4 ? v.x : v.y) >> (8 * (k & 3))) & 0xffu;
-}
 __device__ __forceinline__ void load_line(const float* __restrict__ p, float (&v)[8]) {
     const float4 a = *reinterpret_cast<const float4*>(p);
     const float4 b = *reinterpret_cast<const float4*>(p + 4);
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
     v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
+
+// The ghost level set of a solve, tile-major over every tile: G = L, except NaN
+// at SOLID cells, so that one float tells the operator's ghost term of a
+// non-FLUID neighbour (SOLID: s_c; AIR: L_q / min(1e-6, L_c) s_c) and gives a
+// FLUID cell its own L_c; and the SOLID bits of every tile-major line (byte
+// off/8 for the line at tile-major offset off). One z-line of 8 cells per thread.
+struct KGhostLevelSet {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int N;
+        const uint8_t* A;
+        const float* L;
+        float* G;
+        uint8_t* sb;
+    };
+    static __device__ void run(const Args& a) {
+        const int N = a.N, TN = N / kTile;
+        const int64_t nlines = (int64_t)N * N * TN;
+        const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= nlines) return;
+        // e enumerates tile-major lines: tile e/64, line e%64 = (il, jl)
+        const int64_t t = e >> 6;
+        const int line = (int)(e & 63), il = line >> 3, jl = line & 7;
+        const int tk = (int)(t % TN), tj = (int)((t / TN) % TN), ti = (int)(t / ((int64_t)TN * TN));
+        const int64_t nat = cidx(N, ti * kTile + il, tj * kTile + jl, tk * kTile);
+        const uint2 cls = *reinterpret_cast<const uint2*>(a.A + nat);
+        float v[8];
+        load_line(a.L + nat, v);
+        const float nan = __builtin_nanf("");
+        uint32_t solid = 0;
+#pragma unroll
+        for (int kl = 0; kl < 8; ++kl) {
+            const uint32_t c = ((kl < 4 ? cls.x : cls.y) >> (8 * (kl & 3))) & 0xffu;
+            if (c == (uint32_t)kSolid) {
+                v[kl] = nan;
+                solid |= 1u << kl;
+            }
+        }
+        a.sb[e] = (uint8_t)solid;
+        float* g = a.G + e * 8;
+        *reinterpret_cast<float4*>(g) = make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4*>(g + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    }
+};
 
 // The search direction and the operator over the active tiles (one wave per
 // tile, lane (il, jl) owning the z-line of 8 cells): s = z on the first
@@ -368,8 +409,8 @@ struct KTileSpmv {
     static constexpr int kThreads = 256;
     struct Args {
         Params p;
-        const uint8_t* A;
-        const float* L;
+        const float* G;      // ghost level set (KGhostLevelSet), tile-major
+        const uint8_t* sb;   // SOLID bits per tile-major line
         const uint8_t* fl8;
         const float* Z;
         float* S;
@@ -395,8 +436,7 @@ struct KTileSpmv {
         const float* __restrict__ Z = a.Z;
         const float* __restrict__ Sp = a.S + (int64_t)(it & 1) * nc;
         float* __restrict__ Sn = a.S + (int64_t)((it + 1) & 1) * nc;
-        const uint8_t* __restrict__ A = a.A;
-        const float* __restrict__ L = a.L;
+        const float* __restrict__ G = a.G;
         const uint8_t* __restrict__ fl8 = a.fl8;
         __shared__ float lds[kTilesPerBlock][64 * 8];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -432,6 +472,10 @@ struct KTileSpmv {
             const uint32_t fhy = fl8[sg + (int64_t)hy * NS];
             const uint32_t f_zp = (fl8[sg + dzp] & 1u) & (uint32_t)dzp;
             const uint32_t f_zm = ((fl8[sg - dzm] >> 7) & 1u) & (uint32_t)dzm;
+            const uint32_t sl = a.sb[own >> 3];
+            const uint32_t shx = a.sb[lnx >> 3], shy = a.sb[lny >> 3];
+            const uint32_t s_zp = (a.sb[lzp >> 3] & 1u) & (uint32_t)dzp;
+            const uint32_t s_zm = ((a.sb[lzm >> 3] >> 7) & 1u) & (uint32_t)dzm;
             float x[8], hxl[8], hyl[8];
             search_line(Z, Sp, own, b, x);
             search_line(Z, Sp, lnx, b, hxl);
@@ -443,9 +487,11 @@ struct KTileSpmv {
             float* w = &lds[wave][0];
             *reinterpret_cast<float4*>(w + lane * 8) = make_float4(x[0], x[1], x[2], x[3]);
             *reinterpret_cast<float4*>(w + lane * 8 + 4) = make_float4(x[4], x[5], x[6], x[7]);
-            // FLUID bits of the in-tile neighbour lines
+            // FLUID and SOLID bits of the in-tile neighbour lines
             const uint32_t f_xp_in = __shfl(fl, (lane + 8) & 63, 64), f_xm_in = __shfl(fl, (lane - 8) & 63, 64);
             const uint32_t f_yp_in = __shfl(fl, (lane + 1) & 63, 64), f_ym_in = __shfl(fl, (lane - 1) & 63, 64);
+            const uint32_t s_xp_in = __shfl(sl, (lane + 8) & 63, 64), s_xm_in = __shfl(sl, (lane - 8) & 63, 64);
+            const uint32_t s_yp_in = __shfl(sl, (lane + 1) & 63, 64), s_ym_in = __shfl(sl, (lane - 1) & 63, 64);
             wave_sync();
             float xp[8], xm[8], yp[8], ym[8];
 #pragma unroll
@@ -459,17 +505,22 @@ struct KTileSpmv {
             const uint32_t f_xm = il > 0 ? f_xm_in : (hx == -1 ? fhx : 0u);
             const uint32_t f_yp = jl < 7 ? f_yp_in : (hy == 1 ? fhy : 0u);
             const uint32_t f_ym = jl > 0 ? f_ym_in : (hy == -1 ? fhy : 0u);
+            const uint32_t s_xp = il < 7 ? s_xp_in : (hx == 1 ? shx : 0u);
+            const uint32_t s_xm = il > 0 ? s_xm_in : (hx == -1 ? shx : 0u);
+            const uint32_t s_yp = jl < 7 ? s_yp_in : (hy == 1 ? shy : 0u);
+            const uint32_t s_ym = jl > 0 ? s_ym_in : (hy == -1 ? shy : 0u);
             const float n = (float)N;
             const float h = 1.0f / (n * n);
-            const int64_t c0 = cidx(N, gi, gj, k0);
-            const uint32_t nzp = ((fl >> 1) | (f_zp << 7)) & 0xffu;
-            const uint32_t nzm = ((fl << 1) | f_zm) & 0xffu;
-            // FLUID cells with an in-range neighbour that is not FLUID need the
-            // ghost terms (SOLID: s_c; AIR: L_q / min(1e-6, L_c) s_c)
+            // FLUID or SOLID bits of the z-neighbours along the line
+            const uint32_t nzp = ((fl | sl) >> 1 | (f_zp | s_zp) << 7) & 0xffu;
+            const uint32_t nzm = ((fl | sl) << 1 | (f_zm | s_zm)) & 0xffu;
+            // FLUID cells with an in-range AIR neighbour need the level-set ghost
+            // term L_q / min(1e-6, L_c) s_c; every other term is s_q (FLUID
+            // neighbour) or s_c (SOLID or out of range)
             const uint32_t need =
-                fl & ((r_xp ? ~f_xp : 0u) | (r_xm ? ~f_xm : 0u) | (r_yp ? ~f_yp : 0u) |
-                      (r_ym ? ~f_ym : 0u) | (~nzp & (r_zp ? 0xffu : 0x7fu)) |
-                      (~nzm & (r_zm ? 0xffu : 0xfeu))) & 0xffu;
+                fl & ((r_xp ? ~(f_xp | s_xp) : 0u) | (r_xm ? ~(f_xm | s_xm) : 0u) |
+                      (r_yp ? ~(f_yp | s_yp) : 0u) | (r_ym ? ~(f_ym | s_ym) : 0u) |
+                      (~nzp & (r_zp ? 0xffu : 0x7fu)) | (~nzm & (r_zm ? 0xffu : 0xfeu))) & 0xffu;
             const bool ghosts = __ballot(need != 0u) != 0ull;
             // y_c = (6 s_c - t1 - ... - t6) / h; dividing by h = 2^-2k (N a power
             // of two) is the exact product with 2^2k
@@ -479,30 +530,28 @@ struct KTileSpmv {
             auto cells = [&](auto p2_tag, auto ghost_tag) {
                 constexpr bool P2 = decltype(p2_tag)::value;
                 constexpr bool GHOST = decltype(ghost_tag)::value;
-                // the classes and level sets of the own and neighbour lines
-                // (natural layout), loaded together, only in waves with ghost terms
-                uint2 acl = make_uint2(0u, 0u), axp = acl, axm = acl, ayp = acl, aym = acl;
-                float lcl[8], lxp[8], lxm[8], lyp[8], lym[8];
-                uint32_t azp = 0u, azm = 0u;
-                float lzpv = 0.0f, lzmv = 0.0f;
+                // the ghost level set of the own line, the in-tile neighbour lines
+                // (through the wave's LDS, whose s values are already in
+                // registers), the halo lines and the z-ends: only in waves with
+                // ghost terms, at the same tile-major offsets as s
+                float gcl[8], gxp[8], gxm[8], gyp[8], gym[8], gzp = 0.0f, gzm = 0.0f;
                 if constexpr (GHOST) {
-                    const int64_t cxp = r_xp ? c0 + NN : c0, cxm = r_xm ? c0 - NN : c0;
-                    const int64_t cyp = r_yp ? c0 + N : c0, cym = r_ym ? c0 - N : c0;
-                    const int64_t czp = r_zp ? c0 + kTile : c0, czm = r_zm ? c0 - 1 : c0;
-                    acl = *reinterpret_cast<const uint2*>(A + c0);
-                    axp = *reinterpret_cast<const uint2*>(A + cxp);
-                    axm = *reinterpret_cast<const uint2*>(A + cxm);
-                    ayp = *reinterpret_cast<const uint2*>(A + cyp);
-                    aym = *reinterpret_cast<const uint2*>(A + cym);
-                    azp = A[czp];
-                    azm = A[czm];
-                    load_line(L + c0, lcl);
-                    load_line(L + cxp, lxp);
-                    load_line(L + cxm, lxm);
-                    load_line(L + cyp, lyp);
-                    load_line(L + cym, lym);
-                    lzpv = L[czp];
-                    lzmv = L[czm];
+                    float ghx[8], ghy[8];
+                    load_line(G + own, gcl);
+                    load_line(G + lnx, ghx);
+                    load_line(G + lny, ghy);
+                    gzp = G[lzp];
+                    gzm = G[lzm];
+                    *reinterpret_cast<float4*>(w + lane * 8) = make_float4(gcl[0], gcl[1], gcl[2], gcl[3]);
+                    *reinterpret_cast<float4*>(w + lane * 8 + 4) = make_float4(gcl[4], gcl[5], gcl[6], gcl[7]);
+                    wave_sync();
+#pragma unroll
+                    for (int kl = 0; kl < 8; ++kl) {
+                        gxp[kl] = il < 7 ? w[((lane + 8) & 63) * 8 + kl] : ghx[kl];
+                        gxm[kl] = il > 0 ? w[((lane - 8) & 63) * 8 + kl] : ghx[kl];
+                        gyp[kl] = jl < 7 ? w[((lane + 1) & 63) * 8 + kl] : ghy[kl];
+                        gym[kl] = jl > 0 ? w[((lane - 1) & 63) * 8 + kl] : ghy[kl];
+                    }
                 }
 #pragma unroll
                 for (int kl = 0; kl < 8; ++kl) {
@@ -515,23 +564,23 @@ struct KTileSpmv {
                             y[kl] = 0.0f;
                             continue;
                         }
-                        const float lc = lcl[kl];
-                        t1 = xref_term(r_xp, (f_xp >> kl) & 1u, xp[kl], byte_of(axp, kl), lxp[kl], lc, xc);
-                        t2 = xref_term(r_xm, (f_xm >> kl) & 1u, xm[kl], byte_of(axm, kl), lxm[kl], lc, xc);
-                        t3 = xref_term(r_yp, (f_yp >> kl) & 1u, yp[kl], byte_of(ayp, kl), lyp[kl], lc, xc);
-                        t4 = xref_term(r_ym, (f_ym >> kl) & 1u, ym[kl], byte_of(aym, kl), lym[kl], lc, xc);
-                        t5 = kl < 7 ? xref_term(true, (fl >> ku) & 1u, x[ku], byte_of(acl, ku), lcl[ku], lc, xc)
-                                    : xref_term(r_zp, f_zp, zp, azp, lzpv, lc, xc);
-                        t6 = kl > 0 ? xref_term(true, (fl >> kd) & 1u, x[kd], byte_of(acl, kd), lcl[kd], lc, xc)
-                                    : xref_term(r_zm, f_zm, zm, azm, lzmv, lc, xc);
+                        const float gc = gcl[kl];
+                        t1 = xref_term(r_xp, (f_xp >> kl) & 1u, xp[kl], gxp[kl], gc, xc);
+                        t2 = xref_term(r_xm, (f_xm >> kl) & 1u, xm[kl], gxm[kl], gc, xc);
+                        t3 = xref_term(r_yp, (f_yp >> kl) & 1u, yp[kl], gyp[kl], gc, xc);
+                        t4 = xref_term(r_ym, (f_ym >> kl) & 1u, ym[kl], gym[kl], gc, xc);
+                        t5 = kl < 7 ? xref_term(true, (fl >> ku) & 1u, x[ku], gcl[ku], gc, xc)
+                                    : xref_term(r_zp, f_zp, zp, gzp, gc, xc);
+                        t6 = kl > 0 ? xref_term(true, (fl >> kd) & 1u, x[kd], gcl[kd], gc, xc)
+                                    : xref_term(r_zm, f_zm, zm, gzm, gc, xc);
                     } else {
-                        // every in-range neighbour of a FLUID cell is FLUID
-                        t1 = r_xp ? xp[kl] : xc;
-                        t2 = r_xm ? xm[kl] : xc;
-                        t3 = r_yp ? yp[kl] : xc;
-                        t4 = r_ym ? ym[kl] : xc;
-                        t5 = kl < 7 ? x[ku] : (r_zp ? zp : xc);
-                        t6 = kl > 0 ? x[kd] : (r_zm ? zm : xc);
+                        // no AIR neighbour: s_q for a FLUID neighbour, else s_c
+                        t1 = (f_xp >> kl) & 1u ? xp[kl] : xc;
+                        t2 = (f_xm >> kl) & 1u ? xm[kl] : xc;
+                        t3 = (f_yp >> kl) & 1u ? yp[kl] : xc;
+                        t4 = (f_ym >> kl) & 1u ? ym[kl] : xc;
+                        t5 = kl < 7 ? ((fl >> ku) & 1u ? x[ku] : xc) : (f_zp ? zp : xc);
+                        t6 = kl > 0 ? ((fl >> kd) & 1u ? x[kd] : xc) : (f_zm ? zm : xc);
                     }
                     const float acc = 6.0f * xc - t1 - t2 - t3 - t4 - t5 - t6;
                     const float yc = P2 ? acc * inv_h : acc / h;
@@ -1959,11 +2008,17 @@ void launch_residual0(hipStream_t s, const Params& p, const uint8_t* A, const fl
     launch_finalize(s, red, FIN_RESID0, nb, tol, stop_rule, p.N);
 }
 
-void launch_spmv_dot(hipStream_t s, const Params& p, const uint8_t* A, const float* L,
+void launch_ghost_levelset(hipStream_t s, int N, const uint8_t* A, const float* L, float* G,
+                           uint8_t* sb) {
+    const int64_t nlines = (int64_t)N * N * (N / kTile);
+    launch<KGhostLevelSet>(s, dim3(grid_for(nlines, kBlock)), dim3(kBlock), {N, A, L, G, sb});
+}
+
+void launch_spmv_dot(hipStream_t s, const Params& p, const float* G, const uint8_t* sb,
                      const uint8_t* fl8, const float* Zt, float* St, float* Yt, const Reduce& red,
                      const Active& act) {
     const int nb = (int)mic_grid(tiles_or_cap(act, 2));
-    launch<KTileSpmv>(s, dim3((unsigned)xcd_grid(nb)), dim3(256), {p, A, L, fl8, Zt, St, Yt, act.tiles[2], act.ntiles[2], red});
+    launch<KTileSpmv>(s, dim3((unsigned)xcd_grid(nb)), dim3(256), {p, G, sb, fl8, Zt, St, Yt, act.tiles[2], act.ntiles[2], red});
     launch_finalize(s, red, FIN_SIGMA, act.ntiles[2] < 0 ? PARTS_MIC_ALL : nb);
 }
 

@@ -74,6 +74,7 @@ int prepare_solve(ariel_sim* h) {
     hipStream_t s = h->stream;
     launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
     launch_fluid_segments(s, h->N, h->A, h->fl8);
+    launch_ghost_levelset(s, h->N, h->A, h->L, h->Gt, h->sb);
     launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every,
                        h->d.precond == ARIEL_PRECOND_LEARNED ? 1 : 0);
     launch_active_sets(s, h->N, h->fl8, h->seg_flag, h->seg_scan, h->seg_list, h->tile_any,
@@ -126,7 +127,7 @@ void enqueue_iteration(ariel_sim* h, int it, bool refine, bool learned, bool mic
     const bool sampled = prof && prof_sampled(it);
     {
         Prof pr(h, ARIEL_KERNEL_SPMV, sampled);
-        launch_spmv_dot(s, p, h->A, h->L, h->fl8, h->Zt, h->St, h->Yt, red, act);
+        launch_spmv_dot(s, p, h->Gt, h->sb, h->fl8, h->Zt, h->St, h->Yt, red, act);
     }
     // the tiled red-black MIC(0) carries the AXPY in its first pass over each
     // colour, except where the reliable update must come between the two
