@@ -61,8 +61,11 @@ __device__ __forceinline__ float smooth_fixed(float r2, float hh, float yhh) {
 // the product is the quotient, bit for bit, for every r2 (subnormal results
 // included). Power-of-two grids at the reference's lattice densities give such
 // radii (4 * 0.5 / 256 = 2^-7).
+// One fma, 1 - r2*yhh rounded once, gives the bits of the multiply followed by
+// the rounded subtraction: the product is exact, or below 2^-126 where both
+// forms round to 1.
 __device__ __forceinline__ float smooth_pow2(float r2, float yhh) {
-    return gmax_num(1.0f - r2 * yhh, 0.0f);
+    return gmax_num(fmaf(-r2, yhh, 1.0f), 0.0f);
 }
 __host__ __device__ inline bool is_pow2_float(float x) {
     uint32_t b;

@@ -14,6 +14,8 @@
 //         x64 (double, natural) for reliable updates; reduction partials (double)
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -154,12 +156,24 @@ Recorder*& recorder();        // thread-local (ariel_abi.hip)
 // tank and every step.
 bool& batch_sizing();
 
+// A kernel may ask for a minimum number of resident waves per SIMD
+// (K::kMinWavesPerEU), which caps the registers the compiler gives each lane.
+template <class K, class = void>
+struct min_waves_per_eu {
+    static constexpr int value = 1;
+};
 template <class K>
-__global__ __launch_bounds__(K::kThreads) void k_one(typename K::Args a) {
+struct min_waves_per_eu<K, std::void_t<decltype(K::kMinWavesPerEU)>> {
+    static constexpr int value = K::kMinWavesPerEU;
+};
+
+template <class K>
+__global__ __launch_bounds__(K::kThreads, min_waves_per_eu<K>::value) void k_one(typename K::Args a) {
     K::run(a);
 }
 template <class K>
-__global__ __launch_bounds__(K::kThreads) void k_many(const typename K::Args* __restrict__ t) {
+__global__ __launch_bounds__(K::kThreads, min_waves_per_eu<K>::value) void k_many(
+    const typename K::Args* __restrict__ t) {
     K::run(t[blockIdx.y]);
 }
 

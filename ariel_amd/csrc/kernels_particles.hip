@@ -315,22 +315,29 @@ struct KGather {
 
 // ------------------------------------------------ neighbourhood staging
 // A block of 256 consecutive sorted particles sits in a few cells of one or two
-// grid columns, so the cells its particles query form a small box. When the box
-// spans at most kStageCols columns and its particles fit kStageCap, each
-// column's z-range of the box is copied (x, y, z) into LDS with coalesced loads,
-// all in flight at once, and the neighbour loops read LDS instead of issuing a
-// dependent global load per candidate. Candidate order and values are unchanged.
+// grid columns. Its particles are split into groups, the runs of equal query
+// columns (lanes whose windows start at the same (x, y) cell); each group's box
+// is the union of its lanes' windows. When the groups' columns number at most
+// kStageCols and their particles fit kStageCap, each column's z-range of its
+// group's box is copied (x, y, z) into LDS with coalesced loads, all in flight
+// at once, and the neighbour loops read LDS instead of issuing a dependent
+// global load per candidate. A block whose particles end one grid column and
+// begin the next thus stages two short boxes instead of one box spanning the
+// whole column height. Candidate order and values are unchanged.
 constexpr int kStageBlock = 256;   // threads (particles) per staging block
 constexpr int kStageCap = 2560;
-constexpr int kStageCols = 16;
+constexpr int kStageGroups = 4;
+constexpr int kStageCols = 36;
 constexpr int kStagePer = kStageCap / kStageBlock;   // staged particles per thread
 
 struct StageLds {
     float x[kStageCap], y[kStageCap], z[kStageCap];
-    int q0[kStageCols];     // first particle of each column's staged range
-    int base[kStageCols];   // its LDS index
-    int box[6];             // x0, x1, y0, y1, z0, z1 (cells, inclusive)
-    int ncols, ny, ok, total;
+    int q0[kStageCols];          // first particle of each column's staged range
+    int base[kStageCols + 1];    // its LDS index; base[ncols] = total
+    int gbox[kStageGroups][6];   // per group: x0, x1, y0, y1, z0, z1 (cells, inclusive)
+    int gcol[kStageGroups];      // index of the group's first column
+    int wstart[kStageBlock / 64];
+    int ncols, ngroups, ok, total;
 };
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -345,61 +352,95 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // Every thread of the block calls this; valid threads pass their query box.
-// Returns with st.ok set (block-uniform) and, if set, the columns staged.
-__device__ void stage_neighbourhood(StageLds& st, bool valid, int x0, int x1, int y0, int y1,
-                                    int z0, int z1, int N, const int32_t* __restrict__ offsets,
-                                    const float4* __restrict__ pos) {
-    if (threadIdx.x == 0) {
-        st.box[0] = st.box[2] = st.box[4] = 1 << 30;
-        st.box[1] = st.box[3] = st.box[5] = -1;
+// Returns the thread's group (-1 if not valid) with st.ok set (block-uniform)
+// and, if set, the columns staged.
+__device__ int stage_neighbourhood(StageLds& st, bool valid, int x0, int x1, int y0, int y1,
+                                   int z0, int z1, int N, const int32_t* __restrict__ offsets,
+                                   const float4* __restrict__ pos) {
+    const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid < kStageGroups) {
+        st.gbox[tid][0] = st.gbox[tid][2] = st.gbox[tid][4] = 1 << 30;
+        st.gbox[tid][1] = st.gbox[tid][3] = st.gbox[tid][5] = -1;
     }
+    // group starts: a valid lane whose window start differs from the previous
+    // thread's (or which follows an invalid thread or begins the block)
+    const int key = valid ? x0 * 65536 + y0 : -1;
+    const int kprev = __shfl_up(key, 1, 64);
+    __shared__ int s_lastkey[kStageBlock / 64];
+    if (lane == 63) s_lastkey[w] = key;
     __syncthreads();
-    const int big = 1 << 30;
-    const int a0 = wave_min(valid ? x0 : big), a1 = wave_max(valid ? x1 : -1);
-    const int b0 = wave_min(valid ? y0 : big), b1 = wave_max(valid ? y1 : -1);
-    const int c0 = wave_min(valid ? z0 : big), c1 = wave_max(valid ? z1 : -1);
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(&st.box[0], a0);
-        atomicMax(&st.box[1], a1);
-        atomicMin(&st.box[2], b0);
-        atomicMax(&st.box[3], b1);
-        atomicMin(&st.box[4], c0);
-        atomicMax(&st.box[5], c1);
+    const int before = lane > 0 ? kprev : (w > 0 ? s_lastkey[w - 1] : -1);
+    const bool start = valid && key != before;
+    const unsigned long long ball = __ballot(start);
+    if (lane == 0) st.wstart[w] = __popcll(ball);
+    __syncthreads();
+    int g = __popcll(ball & ((2ull << lane) - 1ull)) - 1;   // starts up to this lane
+    int ng = 0;
+#pragma unroll
+    for (int ww = 0; ww < kStageBlock / 64; ++ww) {
+        if (ww < w) g += st.wstart[ww];
+        ng += st.wstart[ww];
     }
-    __syncthreads();
-    const int bx0 = st.box[0], bx1 = st.box[1], by0 = st.box[2], by1 = st.box[3];
-    const int bz0 = st.box[4], bz1 = st.box[5];
-    const int ny = by1 - by0 + 1;
-    const int ncols = bx1 < bx0 ? 0 : (bx1 - bx0 + 1) * ny;
-    if (ncols <= 0 || ncols > kStageCols) {
-        if (threadIdx.x == 0) st.ok = 0;
+    if (!valid) g = -1;
+    if (ng > kStageGroups) {
+        if (tid == 0) {
+            st.ok = 0;
+            st.ngroups = ng;
+        }
         __syncthreads();
-        return;
+        return g;
     }
-    int len = 0;
-    if ((int)threadIdx.x < ncols) {
-        const int cx = bx0 + (int)threadIdx.x / ny, cy = by0 + (int)threadIdx.x % ny;
-        const int64_t col = cidx(N, cx, cy, 0);
-        const int q0 = offsets[col + bz0];
-        len = offsets[col + bz1 + 1] - q0;
-        st.q0[threadIdx.x] = q0;
-        st.base[threadIdx.x] = len;   // lengths, turned into bases below
+    if (g >= 0) {
+        atomicMin(&st.gbox[g][0], x0);
+        atomicMax(&st.gbox[g][1], x1);
+        atomicMin(&st.gbox[g][2], y0);
+        atomicMax(&st.gbox[g][3], y1);
+        atomicMin(&st.gbox[g][4], z0);
+        atomicMax(&st.gbox[g][5], z1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
+        int nc = 0;
+        for (int gg = 0; gg < ng; ++gg) {
+            st.gcol[gg] = nc;
+            nc += (st.gbox[gg][1] - st.gbox[gg][0] + 1) * (st.gbox[gg][3] - st.gbox[gg][2] + 1);
+        }
+        st.ncols = nc;
+        st.ngroups = ng;
+    }
+    __syncthreads();
+    const int ncols = st.ncols;
+    if (ncols <= 0 || ncols > kStageCols) {
+        __syncthreads();
+        if (tid == 0) st.ok = 0;
+        __syncthreads();
+        return g;
+    }
+    if (tid < ncols) {
+        int gg = 0;
+        while (gg + 1 < ng && st.gcol[gg + 1] <= tid) ++gg;
+        const int* bx = st.gbox[gg];
+        const int c = tid - st.gcol[gg], ny = bx[3] - bx[2] + 1;
+        const int cx = bx[0] + c / ny, cy = bx[2] + c % ny;
+        const int64_t col = cidx(N, cx, cy, 0);
+        const int q0 = offsets[col + bx[4]];
+        st.q0[tid] = q0;
+        st.base[tid] = offsets[col + bx[5] + 1] - q0;   // lengths, turned into bases below
+    }
+    __syncthreads();
+    if (tid == 0) {
         int acc = 0;
         for (int c = 0; c < ncols; ++c) {
             const int l = st.base[c];
             st.base[c] = acc;
             acc += l;
         }
+        st.base[ncols] = acc;
         st.ok = acc <= kStageCap;
-        st.ncols = ncols;
-        st.ny = ny;
         st.total = acc;
     }
     __syncthreads();
-    if (!st.ok) return;
+    if (!st.ok) return g;
     const int total = st.total;
     float4 t[kStagePer];
     // a thread's slots u increase with m, so its column search resumes where the
@@ -407,7 +448,7 @@ __device__ void stage_neighbourhood(StageLds& st, bool valid, int x0, int x1, in
     int c = 0;
 #pragma unroll
     for (int m = 0; m < kStagePer; ++m) {
-        const int u = (int)threadIdx.x + m * kStageBlock;
+        const int u = tid + m * kStageBlock;
         if (u < total) {
             while (c + 1 < ncols && st.base[c + 1] <= u) ++c;
             t[m] = pos[st.q0[c] + (u - st.base[c])];
@@ -415,7 +456,7 @@ __device__ void stage_neighbourhood(StageLds& st, bool valid, int x0, int x1, in
     }
 #pragma unroll
     for (int m = 0; m < kStagePer; ++m) {
-        const int u = (int)threadIdx.x + m * kStageBlock;
+        const int u = tid + m * kStageBlock;
         if (u < total) {
             st.x[u] = t[m].x;
             st.y[u] = t[m].y;
@@ -423,18 +464,26 @@ __device__ void stage_neighbourhood(StageLds& st, bool valid, int x0, int x1, in
         }
     }
     __syncthreads();
+    return g;
 }
 
-// Particle index of LDS slot l of a staged box.
+// Particle index of LDS slot l of a staged block.
 __device__ __forceinline__ int staged_particle(const StageLds& st, int l) {
     int c = 0;
     while (c + 1 < st.ncols && st.base[c + 1] <= l) ++c;
     return st.q0[c] + (l - st.base[c]);
 }
 
-// LDS index of particle q of column (cx, cy) inside a staged box.
-__device__ __forceinline__ int staged_index(const StageLds& st, int cx, int cy, int q) {
-    const int c = (cx - st.box[0]) * st.ny + (cy - st.box[2]);
+// Whether column (cx, cy), cells [za, zb], lies in group g's staged box.
+__device__ __forceinline__ bool staged_has(const StageLds& st, int g, int cx, int cy, int za, int zb) {
+    const int* bx = st.gbox[g];
+    return cx >= bx[0] && cx <= bx[1] && cy >= bx[2] && cy <= bx[3] && za >= bx[4] && zb <= bx[5];
+}
+
+// LDS index of particle q of column (cx, cy) inside group g's staged box.
+__device__ __forceinline__ int staged_index(const StageLds& st, int g, int cx, int cy, int q) {
+    const int* bx = st.gbox[g];
+    const int c = st.gcol[g] + (cx - bx[0]) * (bx[3] - bx[2] + 1) + (cy - bx[2]);
     return st.base[c] + (q - st.q0[c]);
 }
 
@@ -446,7 +495,7 @@ __device__ __forceinline__ int staged_index(const StageLds& st, int cx, int cy, 
 // neighbourhood did not fit, global) 3x3x3 cells, in the reference's order.
 // POW2: the smoothing radius squared is a power of two (smooth_pow2).
 template <bool POW2>
-__device__ __forceinline__ float density_sum(const StageLds& st, const float4& me, int x0, int x1,
+__device__ __forceinline__ float density_sum(const StageLds& st, int g, const float4& me, int x0, int x1,
                                              int y0, int y1, int z0, int z1, int N,
                                              const int32_t* __restrict__ offsets,
                                              const float4* __restrict__ pos, float hh, float yhh) {
@@ -469,7 +518,7 @@ __device__ __forceinline__ float density_sum(const StageLds& st, const float4& m
             const int sx = x0 + c / 3, sy = y0 + c % 3;
             if (sx > x1 || sy > y1) continue;
             {
-                const int l0 = staged_index(st, sx, sy, qa[c]);
+                const int l0 = staged_index(st, g, sx, sy, qa[c]);
                 const int l1 = l0 + (qb[c] - qa[c]);
 #pragma unroll 4
                 for (int l = l0; l < l1; ++l) {
@@ -526,12 +575,12 @@ struct KDensity {
         const int x0 = max(cx - 1, 0), x1 = min(cx + 1, N - 1);
         const int y0 = max(cy - 1, 0), y1 = min(cy + 1, N - 1);
         const int z0 = max(cz - 1, 0), z1 = min(cz + 1, N - 1);
-        stage_neighbourhood(st, valid, x0, x1, y0, y1, z0, z1, N, offsets, pos);
+        const int g = stage_neighbourhood(st, valid, x0, x1, y0, y1, z0, z1, N, offsets, pos);
         if (!valid) return;
         // the radius is the same for every particle: a uniform branch
         const float weightsum =
-            is_pow2_float(hh) ? density_sum<true>(st, me, x0, x1, y0, y1, z0, z1, N, offsets, pos, hh, yhh)
-                              : density_sum<false>(st, me, x0, x1, y0, y1, z0, z1, N, offsets, pos, hh, yhh);
+            is_pow2_float(hh) ? density_sum<true>(st, g, me, x0, x1, y0, y1, z0, z1, N, offsets, pos, hh, yhh)
+                              : density_sum<false>(st, g, me, x0, x1, y0, y1, z0, z1, N, offsets, pos, hh, yhh);
         float4 v = vel[s];
         v.w = weightsum / p.max_density;
         if (apply_forces) {
@@ -963,6 +1012,7 @@ struct KResample {
         const float margin = 1.0e-3f + 4.0e-7f * maxd;
         const uint32_t fr = (uint32_t)p.frame;
         float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
+        int grp = -1;   // the particle's staging group
         // spring term of one neighbour within re (first loop of Resample); dx = me - o
         // q is the neighbour's particle index, or its LDS slot when `slot` is set
         // (resolved only on the rare coincident-pair path)
@@ -993,7 +1043,7 @@ struct KResample {
             cell_range(x, N, xl, xh);
             cell_range(y, N, yl, yh);
             cell_range(z, N, zl, zh);
-            stage_neighbourhood(st, valid && zl <= zh, xl, xh, yl, yh, zl, zh, N, offsets, pos);
+            grp = stage_neighbourhood(st, valid && zl <= zh, xl, xh, yl, yh, zl, zh, N, offsets, pos);
             if (!valid) return;
             // With the neighbourhood staged, in-radius neighbours are queued (LDS
             // index, in visiting order) and their spring terms evaluated in batches:
@@ -1002,7 +1052,7 @@ struct KResample {
             int queued = 0;
             // An entry is the neighbour's LDS slot when the block is staged and its
             // particle index otherwise.
-            const bool staged = st.ok;
+            const bool staged = st.ok && grp >= 0;
             auto flush = [&]() {
                 for (int t = 0; t < queued; ++t) {
                     const int e = s_queue[t][threadIdx.x];
@@ -1077,7 +1127,7 @@ struct KResample {
                 const int32_t q0 = cq0[slot], q1 = cq1[slot];
                 if (q0 >= q1) continue;
                 if (staged) {
-                    const int l0 = staged_index(st, cxs[slot / 3], cys[slot % 3], q0);
+                    const int l0 = staged_index(st, grp, cxs[slot / 3], cys[slot % 3], q0);
                     for (int32_t q = q0, l = l0; q < q1; ++q, ++l) {
                         const float dx = me.x - st.x[l], dy = me.y - st.y[l], dz = me.z - st.z[l];
                         const float d2 = dx * dx + dy * dy + dz * dz;
@@ -1130,18 +1180,29 @@ struct KResample {
                     if (za > zb) continue;
                     const int64_t col = cidx(N, cx, cy, 0);
                     const int32_t q0 = offsets[col + za], q1 = offsets[col + zb + 1];
-                    for (int32_t q = q0; q < q1; ++q) {
-                        const float4 o = pos[q];
-                        float dist2 = sqrlength(tx, ty, tz, o.x, o.y, o.z);
-                        // Sharpen(dist2, re) = max(re^2/max(dist2,1e-5) - 1, 0) is exactly 0
-                        // when max(dist2, 1e-5) >= re^2; the +0 term is skipped.
-                        if (gmax(dist2, 1.0e-5f) >= re2) continue;
+                    // Sharpen(dist2, re) = max(re^2/max(dist2,1e-5) - 1, 0) is exactly 0
+                    // when max(dist2, 1e-5) >= re^2; the +0 term is skipped.
+                    auto term = [&](int32_t q, float ox, float oy, float oz) {
+                        float dist2 = sqrlength(tx, ty, tz, ox, oy, oz);
+                        if (gmax(dist2, 1.0e-5f) >= re2) return;
                         float w = 1.0f * sharpen_nr(dist2, re2);
                         const float4 ov = vel[q];
                         rx = rx + w * ov.x;
                         ry = ry + w * ov.y;
                         rz = rz + w * ov.z;
                         wsum += w;
+                    };
+                    // t lies near the particle, so its cells are normally in its
+                    // group's staged box: their positions are read from LDS (the
+                    // same bits)
+                    if (st.ok && grp >= 0 && staged_has(st, grp, cx, cy, za, zb)) {
+                        const int l0 = staged_index(st, grp, cx, cy, q0);
+                        for (int32_t q = q0, l = l0; q < q1; ++q, ++l) term(q, st.x[l], st.y[l], st.z[l]);
+                    } else {
+                        for (int32_t q = q0; q < q1; ++q) {
+                            const float4 o = pos[q];
+                            term(q, o.x, o.y, o.z);
+                        }
                     }
                 }
             }
