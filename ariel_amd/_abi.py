@@ -19,6 +19,7 @@ PRECOND_MIC0_TILED, PRECOND_MIC0, PRECOND_JACOBI, PRECOND_REFERENCE, PRECOND_MIC
     PRECOND_LEARNED = range(6)
 STOP_RELATIVE, STOP_REFERENCE_ABS = 0, 1
 SPLAT_SHARPEN, SPLAT_TRILINEAR = 0, 1
+LEVELSET_SPHERES, LEVELSET_TRAILS = 0, 1
 FIELD_UX, FIELD_UY, FIELD_UZ, FIELD_DUX, FIELD_DUY, FIELD_DUZ, FIELD_P, FIELD_D, FIELD_L = range(9)
 (STAGE_SORT, STAGE_DENSITY, STAGE_MARK, STAGE_P2G, STAGE_DIVERGENCE, STAGE_SOLVE,
  STAGE_GRADIENT, STAGE_EXTRAPOLATE, STAGE_ADVECT, STAGE_RESAMPLE) = range(10)
@@ -110,6 +111,9 @@ PROTOTYPES = {
     "ariel_export_levelset": (_I32, [_P, ctypes.c_char_p]),
     "ariel_export_mesh": (_I32, [_P, ctypes.c_char_p]),
     "ariel_export_particles": (_I32, [_P, ctypes.c_char_p]),
+    "ariel_set_levelset_mode": (_I32, [_P, _I32]),
+    "ariel_write_vdb": (_I32, [_PF, _I32, ctypes.c_float, ctypes.c_char_p]),
+    "ariel_write_particles": (_I32, [ctypes.c_char_p, _I64, _PF, _PF]),
     "ariel_mesh_sdf": (_I32, [_PF, _I32, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                               ctypes.POINTER(ctypes.c_int64)]),
     "ariel_selftest_bf16": (_I32, [ctypes.POINTER(ctypes.c_int64)]),

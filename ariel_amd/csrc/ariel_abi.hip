@@ -920,6 +920,7 @@ int ariel_load_state(ariel_sim* h, const char* path) {
 // The particle level set of the current particles on the host ((N+1)^3 nodes,
 // x-major): rasterised on the GPU (export.hip), then copied out.
 constexpr float kSdfBand = 3.0f, kSdfRadius = 0.5f;
+constexpr float kTrailRmin = 0.01f;   // raster.setRmin(.01f), levelset.cpp:104
 int particle_sdf(ariel_sim* h, std::vector<float>& phi) {
     const int64_t M = h->N + 1, nm = M * M * M;
     SdfScratch w;
@@ -929,7 +930,11 @@ int particle_sdf(ariel_sim* h, std::vector<float>& phi) {
         (rc = dalloc(h, &w.scan_sums, h->nc / 256 + 64)) || (rc = dalloc(h, &w.cell_of, h->np)) ||
         (rc = dalloc(h, &w.rank, h->np)) || (rc = dalloc(h, &w.sorted, h->np)) || (rc = dalloc(h, &dphi, nm))) {
     } else {
-        launch_particle_sdf(h->stream, h->pos[h->cur], h->np, h->N, h->maxd, kSdfBand, kSdfRadius, w, dphi);
+        if (h->levelset_mode == ARIEL_LEVELSET_TRAILS)
+            launch_trail_sdf(h->stream, h->pos[h->cur], h->vel[h->cur], h->np, h->N, h->maxd, kSdfBand,
+                             kSdfRadius, kTrailRmin, dphi);
+        else
+            launch_particle_sdf(h->stream, h->pos[h->cur], h->np, h->N, h->maxd, kSdfBand, kSdfRadius, w, dphi);
         rc = check_launch(h, "particle level set");
         phi.resize((size_t)nm);
         hipError_t e = hipSuccess;
@@ -943,12 +948,26 @@ int particle_sdf(ariel_sim* h, std::vector<float>& phi) {
     return rc;
 }
 
+int ariel_set_levelset_mode(ariel_sim* h, int32_t mode) {
+    TRY(check_handle(h));
+    if (mode != ARIEL_LEVELSET_SPHERES && mode != ARIEL_LEVELSET_TRAILS)
+        return fail(h, ARIEL_ERR_INVALID, "unknown level-set mode");
+    h->levelset_mode = mode;
+    return ARIEL_OK;
+}
+
 int ariel_export_levelset(ariel_sim* h, const char* path) {
     TRY(check_handle(h));
     if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
     std::vector<float> phi;
     TRY(particle_sdf(h, phi));
     const int M = h->N + 1;
+    const std::string ps(path);
+    if (ps.size() >= 4 && ps.compare(ps.size() - 4, 4, ".vdb") == 0) {
+        std::string err;
+        if (!write_vdb(path, phi.data(), M, kSdfBand, "surface", err)) return fail(h, ARIEL_ERR_IO, err);
+        return ARIEL_OK;
+    }
     FILE* f = std::fopen(path, "wb");
     if (!f) return fail(h, ARIEL_ERR_IO, std::string("cannot open ") + path);
     char header[256];
@@ -990,15 +1009,40 @@ int ariel_export_mesh(ariel_sim* h, const char* path) {
 int ariel_export_particles(ariel_sim* h, const char* path) {
     TRY(check_handle(h));
     if (!path) return fail(h, ARIEL_ERR_INVALID, "null path");
-    const std::string p(path);
-    if (p.size() < 4 || p.compare(p.size() - 4, 4, ".geo") != 0)
-        return fail(h, ARIEL_ERR_UNSUPPORTED, "particle export: only .geo (Houdini classic ASCII) is built");
+    if (partio_format(path) == 0)
+        return fail(h, ARIEL_ERR_UNSUPPORTED, "particle export: the extension must be .geo or .bgeo (optionally .gz)");
+    std::string err;
     std::vector<float> pos(3 * h->np), vel(3 * h->np);
     int64_t n = 0;
     TRY(ariel_get_particles(h, pos.data(), vel.data(), nullptr, nullptr, h->np, &n));
     for (float& x : pos) x = x * h->maxd;
-    if (!write_geo(path, n, pos.data(), vel.data())) return fail(h, ARIEL_ERR_IO, std::string("cannot write ") + path);
+    const int rc = write_partio(path, n, pos.data(), vel.data(), err);
+    if (rc != ARIEL_OK) return fail(h, rc, err);
     return ARIEL_OK;
+}
+
+int ariel_write_vdb(const float* phi, int32_t M, float bg, const char* path) {
+    if (!phi || !path || M < 1 || M > 4096 || !(bg > 0.0f)) {
+        set_last_error("ariel_write_vdb: bad arguments");
+        return ARIEL_ERR_INVALID;
+    }
+    std::string err;
+    if (!write_vdb(path, phi, M, bg, "surface", err)) {
+        set_last_error(err);
+        return ARIEL_ERR_IO;
+    }
+    return ARIEL_OK;
+}
+
+int ariel_write_particles(const char* path, int64_t n, const float* pos, const float* vel) {
+    if (!path || n < 0 || (n > 0 && (!pos || !vel))) {
+        set_last_error("ariel_write_particles: bad arguments");
+        return ARIEL_ERR_INVALID;
+    }
+    std::string err;
+    const int rc = write_partio(path, n, pos, vel, err);
+    if (rc != ARIEL_OK) set_last_error(err);
+    return rc;
 }
 
 int ariel_mesh_sdf(const float* phi, int32_t M, const char* path, int64_t* nverts, int64_t* nquads) {

@@ -52,6 +52,7 @@ struct ariel_sim {
     bool initialized = false;
     int last_iters = 0;
     int p2g_tile = 8;             // P2G face-column tile edge (autotuned in batches)
+    int levelset_mode = ARIEL_LEVELSET_SPHERES;   // export level set: spheres or trails
     std::string err;
     hipStream_t stream = nullptr;
     int device = 0;

@@ -231,6 +231,9 @@ struct SdfScratch {
 // phi of the particle level set on the (N+1)^3 nodes (grid units).
 void launch_particle_sdf(hipStream_t s, const float4* pos, int64_t np, int N, float maxd, float band,
                          float rad, SdfScratch& w, float* phi);
+// phi of the particles' velocity trails on the (N+1)^3 nodes (grid units).
+void launch_trail_sdf(hipStream_t s, const float4* pos, const float4* vel, int64_t np, int N, float maxd,
+                      float band, float rad, float rmin, float* phi);
 // Dual mesh of phi's zero surface on an M^3 node grid: vertices (x, y, z in node
 // units) and quads (4 vertex indices each, normals out of the negative side).
 void surface_nets(const float* phi, int M, std::vector<float>& verts, std::vector<int32_t>& quads);
@@ -240,7 +243,10 @@ bool write_obj(const char* path, const std::vector<float>& verts, const std::vec
 // array's limits; device = the current device.
 hipError_t noalloc_levelset(hipStream_t s, int device, const float4* pos, int64_t np, int N,
                             float maxd, float band, float rad, float* phi, std::string& err);
-bool write_geo(const char* path, int64_t n, const float* pos, const float* vel);
+// files.hip: the OpenVDB level-set file and the Partio particle files
+bool write_vdb(const char* path, const float* phi, int M, float bg, const char* grid_name, std::string& err);
+int write_partio(const char* path, int64_t n, const float* pos, const float* vel, std::string& err);
+int partio_format(const char* path);   // 1 ".geo", 2 ".bgeo" (either + ".gz"), 0 other
 void launch_fill_f32(hipStream_t s, float* p, int64_t n, float v);
 
 // grid (kernels_grid.hip)

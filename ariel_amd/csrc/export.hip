@@ -1,7 +1,7 @@
 // export.hip — frame export of the fluid (Scene::ExportParticles,
 // src/scene/scene.cpp:34-101): the particle level set rasterised on the GPU,
-// its zero surface as a polygon mesh (OBJ), and the particles as a Partio-style
-// point file (Houdini classic ASCII geometry).
+// its zero surface as a polygon mesh (OBJ); the particle and VDB files are
+// written by files.hip.
 //
 // Level set (LevelSet(particles, maxd), src/grid/levelset.cpp:101-111): nodes
 // of a grid of voxel size 1 in grid units (a particle sits at p * maxd), node
@@ -11,6 +11,8 @@
 // velocity trails with OpenVDB's ParticlesToLevelSet (not in this image); the
 // union of spheres here is its rasterizeSpheres variant (commented out at
 // levelset.cpp:108).
+//
+// File formats (OpenVDB, Partio) are in files.hip.
 //
 // Mesh (LevelSet::WriteObjToFile, levelset.cpp:113-173, which calls OpenVDB's
 // VolumeToMesh): a dual surface like VolumeToMesh's (one vertex per voxel
@@ -140,6 +142,110 @@ struct KParticleSdf {
     }
 };
 
+// Velocity trails (LevelSet(particles, maxd) calls rasterizeTrails,
+// levelset.cpp:109; OpenVDB's ParticlesToLevelSet, not in this image, restated
+// from its published source). Per particle, radius R0 = 0.5 (getPosRadVel,
+// levelset.hpp:55-67) at P0 = p * maxd, trail direction n = -v/|v| with v the
+// particle velocity as given (unit-box units per second, unscaled, as the
+// reference passes it): spheres at P, radius R, starting at (P0, R0), each
+// followed by P += 0.5 R n, d = |P - P0|, R = R0 - (R0 - Rmin) d / |v|, while
+// d <= |v| (Rmin = 0.01, raster.setRmin at levelset.cpp:104; a particle at rest
+// gives the one sphere at P0). A sphere stamps every node c with |c - P|^2 <
+// (R + w)^2 (w = 3, the narrow band) with float(|c - P| - R) when smaller (CSG
+// union); a node at |c - P|^2 <= max(0, R - w)^2 becomes -w. Arithmetic in double
+// like OpenVDB's (Vec3R, Real); nodes outside [0, N]^3 are not stored. The
+// union is a minimum, taken with integer atomics on an order-preserving key of
+// the float, so the result does not depend on the order of the particles.
+__device__ __forceinline__ int32_t float_key(float f) {
+    const int32_t b = __float_as_int(f);
+    return b >= 0 ? b : b ^ 0x7fffffff;
+}
+__device__ __forceinline__ float key_float(int32_t k) { return __int_as_float(k >= 0 ? k : k ^ 0x7fffffff); }
+
+struct KTrailSdf {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const float4* pos;
+        const float4* vel;
+        int64_t np;
+        float maxd;
+        int M;             // nodes per axis (N + 1)
+        double band, rmin, r0;
+        int32_t* key;      // M^3 keys of phi, initialised to the key of band
+    };
+    static __device__ void sphere(const Args& a, double px, double py, double pz, double R) {
+        const double mx = R + a.band, max2 = mx * mx;
+        const double inner = R - a.band > 0.0 ? R - a.band : 0.0, min2 = inner * inner;
+        const int x0 = max((int)floor(px - mx), 0), x1 = min((int)ceil(px + mx), a.M - 1);
+        const int y0 = max((int)floor(py - mx), 0), y1 = min((int)ceil(py + mx), a.M - 1);
+        const int z0 = max((int)floor(pz - mx), 0), z1 = min((int)ceil(pz + mx), a.M - 1);
+        for (int x = x0; x <= x1; ++x) {
+            const double x2 = (x - px) * (x - px);
+            for (int y = y0; y <= y1; ++y) {
+                const double x2y2 = x2 + (y - py) * (y - py);
+                for (int z = z0; z <= z1; ++z) {
+                    const double d2 = x2y2 + (z - pz) * (z - pz);
+                    if (d2 >= max2) continue;
+                    const float v = d2 <= min2 ? (float)-a.band : (float)(sqrt(d2) - R);
+                    const int32_t k = float_key(v);
+                    int32_t* slot = a.key + ((int64_t)x * a.M + y) * a.M + z;
+                    if (k < *slot) atomicMin(slot, k);
+                }
+            }
+        }
+    }
+    static __device__ void run(const Args& a) {
+        const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+        if (s >= a.np) return;
+        const float4 p = a.pos[s];
+        const float4 u = a.vel[s];
+        // positions scaled in float (the reference's Vec3f), then widened
+        const double P0x = (double)(p.x * a.maxd), P0y = (double)(p.y * a.maxd), P0z = (double)(p.z * a.maxd);
+        const double vx = (double)u.x, vy = (double)u.y, vz = (double)u.z;
+        const double speed = sqrt(vx * vx + vy * vy + vz * vz), inv = 1.0 / speed;
+        const double nx = -vx * inv, ny = -vy * inv, nz = -vz * inv;
+        const double R0 = a.r0;
+        if (R0 < a.rmin) return;
+        double Px = P0x, Py = P0y, Pz = P0z, R = R0, d = 0.0;
+        // a sphere per iteration; the trail ends once d exceeds the speed (at
+        // rest the direction is NaN and the first step ends it)
+        for (int m = 0; d <= speed && m < 1 << 20; ++m) {
+            sphere(a, Px, Py, Pz, R);
+            Px += 0.5 * R * nx;
+            Py += 0.5 * R * ny;
+            Pz += 0.5 * R * nz;
+            const double ex = Px - P0x, ey = Py - P0y, ez = Pz - P0z;
+            d = sqrt(ex * ex + ey * ey + ez * ez);
+            R = R0 - (R0 - a.rmin) * d * inv;
+        }
+    }
+};
+
+struct KKeyToFloat {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int32_t* key;
+        int64_t n;
+    };
+    static __device__ void run(const Args& a) {
+        const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+        if (i < a.n) a.key[i] = __float_as_int(key_float(a.key[i]));
+    }
+};
+
+struct KFillKey {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int32_t* key;
+        int64_t n;
+        float v;
+    };
+    static __device__ void run(const Args& a) {
+        const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+        if (i < a.n) a.key[i] = float_key(a.v);
+    }
+};
+
 }  // namespace
 
 void launch_particle_sdf(hipStream_t s, const float4* pos, int64_t np, int N, float maxd, float band,
@@ -154,6 +260,17 @@ void launch_particle_sdf(hipStream_t s, const float4* pos, int64_t np, int N, fl
         launch<KSdfScatter>(s, dim3(grid_for(np, kBlock)), dim3(kBlock),
                             {pos, np, maxd, w.cell_of, w.rank, w.offsets, w.sorted});
     launch<KParticleSdf>(s, dim3(grid_for(M * M * M, kBlock)), dim3(kBlock), {w.sorted, w.offsets, N, band, rad, phi});
+}
+
+void launch_trail_sdf(hipStream_t s, const float4* pos, const float4* vel, int64_t np, int N, float maxd,
+                      float band, float rad, float rmin, float* phi) {
+    const int64_t M = N + 1, nm = M * M * M;
+    int32_t* key = reinterpret_cast<int32_t*>(phi);
+    launch<KFillKey>(s, dim3(grid_for(nm, kBlock)), dim3(kBlock), {key, nm, band});
+    if (np > 0)
+        launch<KTrailSdf>(s, dim3(grid_for(np, kBlock)), dim3(kBlock),
+                          {pos, vel, np, maxd, (int)M, (double)band, (double)rmin, (double)rad, key});
+    launch<KKeyToFloat>(s, dim3(grid_for(nm, kBlock)), dim3(kBlock), {key, nm});
 }
 
 // ---------------------------------------------------------------- host side
@@ -226,47 +343,6 @@ bool write_obj(const char* path, const std::vector<float>& verts, const std::vec
     for (size_t q = 0; ok && q < quads.size(); q += 4)
         ok = std::fprintf(f, "f %d %d %d %d\n", quads[q] + 1, quads[q + 1] + 1, quads[q + 2] + 1,
                           quads[q + 3] + 1) > 0;
-    return std::fclose(f) == 0 && ok;
-}
-
-bool write_geo(const char* path, int64_t n, const float* pos, const float* vel) {
-    FILE* f = std::fopen(path, "w");
-    if (!f) return false;
-    // the attributes of Scene::ExportParticles (scene.cpp:60-77): position (grid
-    // units), velocity "v", integer "id" = index in the file
-    bool ok = std::fprintf(f,
-                           "PGEOMETRY V5\n"
-                           "NPoints %lld NPrims 0\n"
-                           "NPointGroups 0 NPrimGroups 0\n"
-                           "NPointAttrib 2 NVertexAttrib 0 NPrimAttrib 0 NAttrib 0\n"
-                           "PointAttrib\n"
-                           "v 3 vector 0 0 0\n"
-                           "id 1 int 0\n",
-                           (long long)n) > 0;
-    // the point lines are formatted by several threads, a block of points each,
-    // and written in order
-    const int64_t kBlockPoints = 1 << 16;
-    const int64_t nblocks = (n + kBlockPoints - 1) / kBlockPoints;
-    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    for (int64_t b0 = 0; ok && b0 < nblocks; b0 += nthreads) {
-        const int64_t b1 = std::min<int64_t>(nblocks, b0 + nthreads);
-        std::vector<std::string> text((size_t)(b1 - b0));
-        std::vector<std::thread> pool;
-        for (int64_t b = b0; b < b1; ++b)
-            pool.emplace_back([&, b] {
-                std::string& t = text[(size_t)(b - b0)];
-                char line[256];
-                for (int64_t p = b * kBlockPoints; p < std::min(n, (b + 1) * kBlockPoints); ++p) {
-                    const int len = std::snprintf(line, sizeof(line), "%.9g %.9g %.9g 1 (%.9g %.9g %.9g %lld)\n",
-                                                  pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], vel[3 * p],
-                                                  vel[3 * p + 1], vel[3 * p + 2], (long long)p);
-                    t.append(line, (size_t)len);
-                }
-            });
-        for (auto& th : pool) th.join();
-        for (const std::string& t : text) ok = ok && std::fwrite(t.data(), 1, t.size(), f) == t.size();
-    }
-    ok = ok && std::fprintf(f, "beginExtra\nendExtra\n") > 0;
     return std::fclose(f) == 0 && ok;
 }
 

@@ -223,7 +223,14 @@ class FlipSim:
             self._h, rhs.ctypes.data_as(_abi._PF), x.ctypes.data_as(_abi._PF), ctypes.byref(st)))
         return x, st
 
+    def set_levelset_mode(self, mode):
+        """_abi.LEVELSET_SPHERES (default) or _abi.LEVELSET_TRAILS (the reference's
+        rasterizeTrails) for later level-set and mesh exports."""
+        self._check(self._lib.ariel_set_levelset_mode(self._h, mode))
+
     def export_levelset(self, path):
+        """The particle level set; an OpenVDB file if path ends in ".vdb", else the
+        raw volume that read_levelset reads."""
         self._check(self._lib.ariel_export_levelset(self._h, str(path).encode()))
 
     def export_mesh(self, path):
@@ -231,7 +238,8 @@ class FlipSim:
         self._check(self._lib.ariel_export_mesh(self._h, str(path).encode()))
 
     def export_particles(self, path):
-        """Partio-style particle file (Scene::ExportParticles' Partio path); ".geo" only."""
+        """Partio particle file (Scene::ExportParticles' Partio path): ".geo" or
+        ".bgeo", optionally gzipped (".gz")."""
         self._check(self._lib.ariel_export_particles(self._h, str(path).encode()))
 
 
@@ -268,4 +276,29 @@ def mesh_sdf(phi, path):
     return nv.value, nq.value
 
 
-__all__ = ["FlipSim", "ArielError", "SceneDesc", "read_levelset", "mesh_sdf"]
+def write_vdb(phi, path, background=3.0):
+    """The library's OpenVDB writer on a caller's M^3 node volume (host only)."""
+    lib = _abi.load()
+    phi = np.ascontiguousarray(phi, np.float32)
+    if phi.ndim != 3 or len(set(phi.shape)) != 1:
+        raise ValueError("phi must be an M x M x M array")
+    rc = lib.ariel_write_vdb(phi.ctypes.data_as(_abi._PF), phi.shape[0], background, str(path).encode())
+    if rc != 0:
+        raise ArielError(rc, lib.ariel_last_error(None).decode())
+
+
+def write_particles(path, pos, vel):
+    """The library's Partio writer on caller data (positions in grid units), host only."""
+    lib = _abi.load()
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    if len(pos) != len(vel):
+        raise ValueError("pos and vel differ in length")
+    rc = lib.ariel_write_particles(str(path).encode(), len(pos), pos.ctypes.data_as(_abi._PF),
+                                   vel.ctypes.data_as(_abi._PF))
+    if rc != 0:
+        raise ArielError(rc, lib.ariel_last_error(None).decode())
+
+
+__all__ = ["FlipSim", "ArielError", "SceneDesc", "read_levelset", "mesh_sdf", "write_vdb",
+           "write_particles"]

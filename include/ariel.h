@@ -290,15 +290,29 @@ int ariel_load_state(ariel_sim* h, const char* path);
 /* Scene::ExportParticles VDB path (src/scene/scene.cpp:84-101, LevelSet from
  * particles, src/grid/levelset.cpp:101-111): the fluid particles' level set,
  * rasterised on the GPU on the (N+1)^3 nodes of a grid of voxel size 1 in grid
- * units (a particle sits at p * maxd): min over particles of |x - p| - 0.5,
- * limited to the narrow band value 3. Written as a raw little-endian float32
- * volume (x-major) after a text header; no OpenVDB writer in this build. */
+ * units (a particle sits at p * maxd), limited to the narrow band value 3:
+ *  - ARIEL_LEVELSET_SPHERES (the default; rasterizeSpheres, levelset.cpp:108):
+ *    min over particles of |x - p| - 0.5;
+ *  - ARIEL_LEVELSET_TRAILS (what the reference runs: rasterizeTrails,
+ *    levelset.cpp:109; the C++ facade's choice): each particle a trail of
+ *    spheres from radius 0.5 at p * maxd backwards along its velocity, the
+ *    radius falling to 0.01 over the trail's length |v| (OpenVDB
+ *    ParticlesToLevelSet's rule, restated).
+ * ariel_set_levelset_mode picks one for the handle's later exports (the level
+ * set, and the mesh of ariel_export_mesh).
+ * A path ending in ".vdb" is written as an OpenVDB file (LevelSet::
+ * WriteVDBGridToFile, levelset.cpp:195-201; file format 222, one level-set
+ * FloatGrid "surface", voxel size 1, active nodes |phi| < 3); any other path as
+ * a raw little-endian float32 volume (x-major) after a text header. */
+#define ARIEL_LEVELSET_SPHERES 0
+#define ARIEL_LEVELSET_TRAILS  1
+int ariel_set_levelset_mode(ariel_sim* h, int32_t mode);
 int ariel_export_levelset(ariel_sim* h, const char* path);
 
 /* Allocation-free variant of ariel_export_levelset (no counterpart in the
  * reference; it is for callers that cannot allocate device memory: code under
  * HIP graph capture, serving loops, harnesses that forbid hipMalloc). The same
- * level set, bit for bit, written into the caller's DEVICE buffer phi of cap
+ * level set (ARIEL_LEVELSET_SPHERES), bit for bit, written into the caller's DEVICE buffer phi of cap
  * floats (at least (N+1)^3, x-major) instead of a file, queued on the handle's
  * stream without synchronising (ariel_sync(h) waits for it). Its workspace is a
  * fixed device array of about 26 MiB that the library holds on every device it
@@ -321,9 +335,18 @@ int ariel_export_mesh(ariel_sim* h, const char* path);
 /* Scene::ExportParticles Partio path (scene.cpp:50-82): every fluid particle
  * with its position in grid units, velocity "v" and integer "id" (its index in
  * the file), in GetParticles order. The format follows the file extension as
- * Partio's does; only ".geo" (Houdini classic ASCII geometry) is built, other
+ * Partio::write's does: ".geo" (Houdini classic ASCII geometry) or ".bgeo"
+ * (its binary form), each optionally gzipped with ".gz" appended; other
  * extensions give ARIEL_ERR_UNSUPPORTED. */
 int ariel_export_particles(ariel_sim* h, const char* path);
+
+/* The writers of ariel_export_levelset and ariel_export_particles on a caller's
+ * data (host only, no GPU): the M^3 node volume phi (x-major) with background
+ * bg as an OpenVDB file, and n particles (positions already in grid units,
+ * velocities) as the Partio file of path's extension. No reference counterpart
+ * (the reference writes through OpenVDB and Partio). */
+int ariel_write_vdb(const float* phi, int32_t M, float bg, const char* path);
+int ariel_write_particles(const char* path, int64_t n, const float* pos, const float* vel);
 
 /* The mesher of ariel_export_mesh on a caller's level set (host only, no GPU):
  * phi holds M^3 node values, x-major (the volume ariel_export_levelset writes);

@@ -63,9 +63,11 @@ public:
     }
 
     // Step(saveVDB, saveOBJ, savePARTIO) (flip.cpp:92-128) with the export of
-    // Scene::ExportParticles (scene.cpp:34-101) after the step: the level set,
-    // its mesh and the particles go to <export_path>.<frame>.levelset, .obj and
-    // .geo, the frame padded to 4 digits and placed before the extension as the
+    // Scene::ExportParticles (scene.cpp:34-101) after the step: the particles'
+    // velocity-trail level set (LevelSet(particles, maxd), levelset.cpp:101-111)
+    // as an OpenVDB file, its mesh and the particles go to
+    // <export_path>.<frame>.vdb, .obj and <partio_extension> (".geo" unless set),
+    // the frame padded to 4 digits and placed before the extension as the
     // reference names its files (scene.cpp:48-58, 85-90).
     void Step(bool saveVDB = false, bool saveOBJ = false, bool savePARTIO = false,
               const char* export_path = nullptr) {
@@ -75,10 +77,15 @@ public:
         char frame[16];
         std::snprintf(frame, sizeof(frame), ".%04d", m_frame);
         const std::string base = std::string(export_path) + frame;
-        if (saveVDB) check(ariel_export_levelset(h_, (base + ".levelset").c_str()), h_);
+        check(ariel_set_levelset_mode(h_, ARIEL_LEVELSET_TRAILS), h_);
+        if (saveVDB) check(ariel_export_levelset(h_, (base + ".vdb").c_str()), h_);
         if (saveOBJ) check(ariel_export_mesh(h_, (base + ".obj").c_str()), h_);
-        if (savePARTIO) check(ariel_export_particles(h_, (base + ".geo").c_str()), h_);
+        if (savePARTIO) check(ariel_export_particles(h_, (base + partio_extension).c_str()), h_);
     }
+
+    // The Partio format of Step's particle files, by extension (".geo", ".bgeo",
+    // either with ".gz"), as the reference's scene path selects it.
+    std::string partio_extension = ".geo";
 
     std::vector<Particle> GetParticles() {
         int64_t n = 0;

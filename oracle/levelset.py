@@ -57,3 +57,68 @@ def brute_force_levelset(pos, maxd, N):
         dx, dy, dz = I - p[0], J - p[1], K - p[2]
         best = np.minimum(best, np.sqrt((dx * dx + dy * dy) + dz * dz))
     return np.minimum(best - RADIUS, BAND)
+
+
+# ---------------------------------------------------------------------------
+# Velocity trails (rasterizeTrails, levelset.cpp:109), restating OpenVDB's
+# ParticlesToLevelSet (a third-party library, unvendored in the reference and
+# absent here; version unpinned: Nuparu bundles it, README.md:6), whose
+# published rule is: per particle, radius R0 (0.5, getPosRadVel,
+# levelset.hpp:55-67), position P0 = p * maxd (float32 product, widened),
+# velocity v as given; spheres at P with radius R starting at (P0, R0); after
+# each, P += 0.5 R n with n = -v / |v|, d = |P - P0|, R = R0 - (R0 - Rmin) d / |v|,
+# while d <= |v|; Rmin = 0.01f (raster.setRmin, levelset.cpp:104). A sphere
+# writes, at every node c with |c - P|^2 < (R + 3)^2, float(|c - P| - R) when
+# smaller, and -3 where |c - P|^2 <= max(0, R - 3)^2. Arithmetic in float64
+# (OpenVDB's Real), in this order.
+TRAIL_RMIN = np.float64(np.float32(0.01))
+
+
+def trail_levelset(pos, vel, maxd, N, max_spheres=1 << 20):
+    """pos, vel: (n, 3) float32; returns phi on the (N+1)^3 nodes, [i, j, k]."""
+    M = N + 1
+    band = np.float64(BAND)
+    r0 = np.float64(RADIUS)
+    phi = np.full((M, M, M), BAND, np.float32)
+    p0 = (np.asarray(pos, np.float32) * np.float32(maxd)).astype(np.float64)
+    v = np.asarray(vel, np.float32).astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        speed = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+        inv = 1.0 / speed
+        nrm = -v * inv[:, None]
+    P = p0.copy()
+    R = np.full(len(p0), r0)
+    d = np.zeros(len(p0))
+    offs = np.arange(-4, 5)
+    live = np.ones(len(p0), bool) & (r0 >= TRAIL_RMIN)
+    steps = 0
+    while live.any() and steps < max_spheres:
+        idx = np.nonzero(live)[0]
+        Pl, Rl = P[idx], R[idx]
+        mx = Rl + band
+        max2 = mx * mx
+        inner = np.maximum(Rl - band, 0.0)
+        min2 = inner * inner
+        base = np.floor(Pl).astype(np.int64)
+        for a in offs:
+            x = base[:, 0] + a
+            x2 = (x - Pl[:, 0]) * (x - Pl[:, 0])
+            for b in offs:
+                y = base[:, 1] + b
+                x2y2 = x2 + (y - Pl[:, 1]) * (y - Pl[:, 1])
+                for c in offs:
+                    z = base[:, 2] + c
+                    d2 = x2y2 + (z - Pl[:, 2]) * (z - Pl[:, 2])
+                    ok = (d2 < max2) & (x >= 0) & (x < M) & (y >= 0) & (y < M) & (z >= 0) & (z < M)
+                    if not ok.any():
+                        continue
+                    val = np.where(d2[ok] <= min2[ok], -band, np.sqrt(d2[ok]) - Rl[ok]).astype(np.float32)
+                    np.minimum.at(phi, (x[ok], y[ok], z[ok]), val)
+        with np.errstate(invalid="ignore"):
+            P[idx] = Pl + (0.5 * Rl)[:, None] * nrm[idx]
+            e = P[idx] - p0[idx]
+            d[idx] = np.sqrt(e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1] + e[:, 2] * e[:, 2])
+            R[idx] = r0 - (r0 - TRAIL_RMIN) * d[idx] * inv[idx]
+            live[idx] = d[idx] <= speed[idx]
+        steps += 1
+    return phi

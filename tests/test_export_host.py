@@ -139,3 +139,112 @@ def test_levelset_oracle_matches_brute_force_bit_for_bit():
     a = particle_levelset(p, N, N)
     b = brute_force_levelset(p, N, N)
     assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------------ files
+# The OpenVDB and Partio writers on caller data (host only): written, then read
+# back by the package's own readers (ariel_amd.fileio), which parse the format's
+# layout independently of the writer.
+def _band_volume(M, seed):
+    rng = np.random.default_rng(seed)
+    I, J, K = grid(M)
+    c = rng.uniform(0.3 * M, 0.7 * M, 3)
+    phi = np.sqrt((I - c[0]) ** 2 + (J - c[1]) ** 2 + (K - c[2]) ** 2) - 0.3 * M
+    phi = np.minimum(phi + rng.normal(0, 0.05, phi.shape), BAND).astype(np.float32)
+    return phi
+
+
+@pytest.mark.parametrize("M,seed", [(9, 0), (33, 1), (47, 2)])
+def test_vdb_round_trip(tmp_path, M, seed):
+    from ariel_amd import read_vdb, write_vdb
+    phi = _band_volume(M, seed)
+    phi[phi < -BAND] = -BAND          # inactive nodes off the background are kept too
+    path = tmp_path / "a.vdb"
+    write_vdb(phi, path, float(BAND))
+    back, bg, active, meta = read_vdb(path, M)
+    assert bg == BAND
+    assert np.array_equal(back, phi)
+    assert np.array_equal(active, np.abs(phi) < BAND)
+    assert meta["class"] == "level set" and meta["name"] == "surface" and meta["version"] == 222
+    assert meta["file_voxel_count"] == int(active.sum())
+    lo, hi = np.argwhere(active).min(axis=0), np.argwhere(active).max(axis=0)
+    assert tuple(lo) == meta["file_bbox_min"] and tuple(hi) == meta["file_bbox_max"]
+    # leaves: the 8^3 blocks holding a node off the background
+    blocks = set(map(tuple, (np.argwhere(phi != BAND) // 8).tolist()))
+    assert meta["leaves"] == len(blocks)
+    raw = open(path, "rb").read()
+    assert raw[:8] == (0x56444220).to_bytes(8, "little")
+
+
+def test_vdb_of_an_empty_band(tmp_path):
+    from ariel_amd import read_vdb, write_vdb
+    phi = np.full((17, 17, 17), BAND, np.float32)
+    write_vdb(phi, tmp_path / "e.vdb", float(BAND))
+    back, bg, active, meta = read_vdb(tmp_path / "e.vdb", 17)
+    assert meta["leaves"] == 0 and not active.any() and np.all(back == BAND)
+
+
+@pytest.mark.parametrize("ext", [".geo", ".bgeo", ".geo.gz", ".bgeo.gz"])
+def test_partio_round_trip(tmp_path, ext):
+    from ariel_amd import read_partio, write_particles
+    rng = np.random.default_rng(3)
+    pos = (rng.random((777, 3)) * 64).astype(np.float32)
+    vel = rng.standard_normal((777, 3)).astype(np.float32)
+    path = tmp_path / ("p" + ext)
+    write_particles(path, pos, vel)
+    p, v, i = read_partio(path)
+    assert np.array_equal(p, pos) and np.array_equal(v, vel) and np.array_equal(i, np.arange(777))
+    if ext.endswith(".gz"):
+        assert open(path, "rb").read(2) == b"\x1f\x8b"
+
+
+def test_partio_refuses_other_formats(tmp_path):
+    from ariel_amd import write_particles
+    with pytest.raises(ArielError) as e:
+        write_particles(tmp_path / "p.pdb", np.zeros((1, 3), np.float32), np.zeros((1, 3), np.float32))
+    assert e.value.code == -6
+
+
+# ------------------------------------------------------------------ trails
+# Pins of the oracle's velocity-trail rasterisation (OpenVDB's rule restated;
+# the library is not in this image): what the geometry of a trail fixes.
+def test_trails_at_rest_are_the_spheres():
+    from oracle.levelset import trail_levelset
+    rng = np.random.default_rng(4)
+    pos = (rng.random((60, 3)) * 0.6 + 0.2).astype(np.float32)
+    a = trail_levelset(pos, np.zeros_like(pos), 16.0, 16)
+    b = particle_levelset(pos, 16.0, 16)
+    # float64 against float32 distances: within one float ulp of |phi| <= 3.5
+    assert np.abs(a - b).max() <= 2.5e-7
+
+
+def test_trail_of_one_particle_runs_back_along_its_velocity():
+    from oracle.levelset import trail_levelset
+    p = np.array([[0.5, 0.5, 0.5]], np.float32)   # node (12, 12, 12) of a 24^3 grid
+    v = np.array([[3.0, 0.0, 0.0]], np.float32)   # moving +x at 3 voxels per unit time
+    phi = trail_levelset(p, v, 24.0, 24)
+    c = 12
+    # the head sits exactly on a node: OpenVDB's inner rule (|c - P|^2 <=
+    # max(0, R - 3)^2) marks such a node -3, deep inside
+    assert phi[c, c, c] == -BAND
+    # behind the head the radius falls linearly to Rmin over |v| = 3 voxels
+    for t in (1, 2):
+        r = 0.5 - (0.5 - 0.01) * t / 3.0
+        assert -r - 0.02 <= phi[c - t, c, c] <= -r + 0.13, (t, phi[c - t, c, c])
+    assert phi[c - 3, c, c] <= 0.02                  # the tail reaches 3 voxels back
+    assert phi[c - 4, c, c] > 0.9                    # and no further
+    assert phi[c + 1, c, c] == np.float32(0.5)       # ahead: the head sphere only
+    # symmetric about the trail's axis
+    assert phi[c - 1, c + 1, c] == phi[c - 1, c - 1, c] == phi[c - 1, c, c + 1] == phi[c - 1, c, c - 1]
+
+
+def test_trails_are_order_free_and_contain_the_heads():
+    from oracle.levelset import trail_levelset
+    rng = np.random.default_rng(5)
+    pos = (rng.random((80, 3)) * 0.6 + 0.2).astype(np.float32)
+    vel = (rng.standard_normal((80, 3)) * 1.5).astype(np.float32)
+    a = trail_levelset(pos, vel, 20.0, 20)
+    perm = rng.permutation(80)
+    assert np.array_equal(a, trail_levelset(pos[perm], vel[perm], 20.0, 20))
+    assert np.all(a <= particle_levelset(pos, 20.0, 20) + 2.5e-7)
+    assert (a < particle_levelset(pos, 20.0, 20) - 0.1).any()
