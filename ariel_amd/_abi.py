@@ -112,6 +112,11 @@ PROTOTYPES = {
     "ariel_export_mesh": (_I32, [_P, ctypes.c_char_p]),
     "ariel_export_particles": (_I32, [_P, ctypes.c_char_p]),
     "ariel_set_levelset_mode": (_I32, [_P, _I32]),
+    "ariel_create_from_json": (_I32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(SceneDesc),
+                                      ctypes.POINTER(_P)]),
+    "ariel_scene_emit": (_I32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(SceneDesc), _I32,
+                                _PF, _PF, _I64, ctypes.POINTER(ctypes.c_int64)]),
+    "ariel_get_desc": (_I32, [_P, ctypes.POINTER(SceneDesc)]),
     "ariel_write_vdb": (_I32, [_PF, _I32, ctypes.c_float, ctypes.c_char_p]),
     "ariel_write_particles": (_I32, [ctypes.c_char_p, _I64, _PF, _PF]),
     "ariel_mesh_sdf": (_I32, [_PF, _I32, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),

@@ -231,6 +231,42 @@ int ensure_particles(ariel_sim* h, int64_t np) {
     return ARIEL_OK;
 }
 
+// Scene::GenerateParticles at a step's frame (flip.cpp:98): the particles the
+// scene's liquid geoms emit on it join the simulation, appended in id order
+// after the existing ones (the particle arrays grow, keeping their contents).
+int emit_particles(ariel_sim* h) {
+    std::vector<float4> pos, vel;
+    emit_geom_liquids(*h->geom, h->d, h->N, h->frame, pos, vel);
+    if (pos.empty()) return ARIEL_OK;
+    const int64_t add = (int64_t)pos.size(), n = h->np + add;
+    if (n >= INT32_MAX) return fail(h, ARIEL_ERR_UNSUPPORTED, "too many particles");
+    for (int64_t i = 0; i < add; ++i) {
+        const int32_t id = (int32_t)(h->np + i);
+        std::memcpy(&pos[i].w, &id, 4);
+    }
+    if (n > h->np_cap) {
+        // keep the current particles across the reallocation
+        std::vector<float4> hp(h->np), hv(h->np);
+        if (h->np) {
+            HIPCHK(h, hipMemcpyAsync(hp.data(), h->pos[h->cur], h->np * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipMemcpyAsync(hv.data(), h->vel[h->cur], h->np * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+        }
+        TRY(ensure_particles(h, std::max<int64_t>(n, h->np_cap + h->np_cap / 2)));
+        h->cur = 0;
+        if (h->np) {
+            HIPCHK(h, hipMemcpyAsync(h->pos[0], hp.data(), h->np * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(h, hipMemcpyAsync(h->vel[0], hv.data(), h->np * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        }
+    }
+    HIPCHK(h, hipMemcpyAsync(h->pos[h->cur] + h->np, pos.data(), add * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->vel[h->cur] + h->np, vel.data(), add * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));   // the host vectors go out of scope
+    h->np = n;
+    h->stats.particles = n;
+    return ARIEL_OK;
+}
+
 // ----------------------------------------------------------------- stages
 int stage_sort(ariel_sim* h) {
     const int b = h->cur, o = 1 - b;
@@ -529,6 +565,73 @@ int ariel_destroy(ariel_sim* h) {
     return ARIEL_OK;
 }
 
+// The solver fields of `solver` over the scene's settings and forces.
+void take_solver(ariel_scene_desc& d, const ariel_scene_desc* solver) {
+    if (!solver) return;
+    d.picflip = solver->picflip;
+    d.wall = solver->wall;
+    d.seed = solver->seed;
+    d.jitter = solver->jitter;
+    d.precond = solver->precond;
+    d.mic_tau = solver->mic_tau;
+    d.mic_sigma = solver->mic_sigma;
+    d.tile = solver->tile;
+    d.tol = solver->tol;
+    d.max_iters = solver->max_iters;
+    d.refine_every = solver->refine_every;
+    d.stop_rule = solver->stop_rule;
+    d.splat = solver->splat;
+    d.resample = solver->resample;
+    d.sharpen_re = solver->sharpen_re;
+    d.spring = solver->spring;
+}
+
+int ariel_create_from_json(const char* json, const char* base_dir, const ariel_scene_desc* solver,
+                           ariel_sim** out) {
+    if (!out || !json) return fail(nullptr, ARIEL_ERR_INVALID, "null argument");
+    *out = nullptr;
+    ariel_scene_desc d;
+    auto geom = std::make_unique<SceneGeom>();
+    std::string err;
+    const int rc = scene_from_json(json, base_dir, &d, geom.get(), err);
+    if (rc != ARIEL_OK) return fail(nullptr, rc, "create_from_json: " + err);
+    take_solver(d, solver);
+    TRY(ariel_create(&d, out));
+    (*out)->geom = std::move(geom);
+    return ARIEL_OK;
+}
+
+int ariel_get_desc(ariel_sim* h, ariel_scene_desc* out) {
+    TRY(check_handle(h));
+    if (!out) return fail(h, ARIEL_ERR_INVALID, "null pointer");
+    *out = h->d;
+    return ARIEL_OK;
+}
+
+int ariel_scene_emit(const char* json, const char* base_dir, const ariel_scene_desc* solver,
+                     int32_t frame, float* pos, float* vel, int64_t cap, int64_t* n) {
+    if (!json || !n) return fail(nullptr, ARIEL_ERR_INVALID, "null argument");
+    ariel_scene_desc d;
+    SceneGeom geom;
+    std::string err;
+    const int rc = scene_from_json(json, base_dir, &d, &geom, err);
+    if (rc != ARIEL_OK) return fail(nullptr, rc, "scene_emit: " + err);
+    take_solver(d, solver);
+    std::string msg;
+    const int vr = validate(&d, msg);
+    if (vr != ARIEL_OK) return fail(nullptr, vr, msg);
+    std::vector<float4> p, v;
+    emit_geom_liquids(geom, d, d.dims[0], frame, p, v);
+    *n = (int64_t)p.size();
+    if (!pos && !vel) return ARIEL_OK;
+    if (cap < *n) return fail(nullptr, ARIEL_ERR_CAPACITY, "particle buffer too small");
+    for (size_t i = 0; i < p.size(); ++i) {
+        if (pos) { pos[3 * i] = p[i].x; pos[3 * i + 1] = p[i].y; pos[3 * i + 2] = p[i].z; }
+        if (vel) { vel[3 * i] = v[i].x; vel[3 * i + 1] = v[i].y; vel[3 * i + 2] = v[i].z; }
+    }
+    return ARIEL_OK;
+}
+
 const char* ariel_last_error(const ariel_sim* h) {
     if (h) return h->err.c_str();
     return g_last_error.c_str();
@@ -541,6 +644,14 @@ int ariel_init(ariel_sim* h) {
     // Scene::GenerateParticles at frame 0, Sort, MarkCellTypes (flip.cpp:74-76).
     std::vector<float4> pos, vel;
     seed_particles(h->d, h->N, pos, vel);
+    if (h->geom) {
+        const size_t n0 = pos.size();
+        emit_geom_liquids(*h->geom, h->d, h->N, 0, pos, vel);
+        for (size_t i = n0; i < pos.size(); ++i) {
+            const int32_t id = (int32_t)i;
+            std::memcpy(&pos[i].w, &id, 4);
+        }
+    }
     if (pos.size() >= (size_t)INT32_MAX) return fail(h, ARIEL_ERR_UNSUPPORTED, "too many particles");
     h->np = (int64_t)pos.size();
     h->np_cap = 0;
@@ -568,6 +679,7 @@ int ariel_step(ariel_sim* h, int32_t nsteps) {
     if (nsteps < 0) return fail(h, ARIEL_ERR_INVALID, "negative step count");
     for (int n = 0; n < nsteps; ++n) {
         h->frame++;
+        if (h->geom) TRY(emit_particles(h));
         HIPCHK(h, hipEventRecord(h->step_t0, h->stream));
         for (int st = 0; st < ARIEL_NUM_STAGES; ++st) TRY(run_stage(h, st));
         HIPCHK(h, hipEventRecord(h->step_t1, h->stream));

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -44,6 +45,9 @@ using namespace ariel;
 
 struct ariel_sim {
     ariel_scene_desc d{};
+    // the reference scene's geometry (ariel_create_from_json): liquid geoms
+    // that emit particles on the frames they are in
+    std::unique_ptr<ariel::SceneGeom> geom;
     int N = 0;
     int64_t nc = 0, nf = 0, np = 0;
     float maxd = 0.f;

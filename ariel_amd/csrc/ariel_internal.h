@@ -14,9 +14,13 @@
 //         x64 (double, natural) for reliable updates; reduction partials (double)
 #pragma once
 
+#include <array>
 #include <type_traits>
+#include <algorithm>
 
 #include <hip/hip_runtime.h>
+
+#include "../../include/ariel.h"
 #include <stdint.h>
 
 #include <cstring>
@@ -243,6 +247,45 @@ bool write_obj(const char* path, const std::vector<float>& verts, const std::vec
 // array's limits; device = the current device.
 hipError_t noalloc_levelset(hipStream_t s, int device, const float4* pos, int64_t np, int N,
                             float maxd, float band, float rad, float* phi, std::string& err);
+// scene_geom.hip: the reference's scene geometry (meshes, per-frame transforms,
+// emitters) on the host
+struct GeomMesh {
+    std::vector<std::array<float, 3>> v;
+    std::vector<std::array<int, 4>> f;   // 1-based vertex indices; f[3] == 0: a triangle
+    std::array<float, 3> lo{}, hi{};
+};
+struct GeomXform {
+    float t[3] = {0, 0, 0}, r[3] = {0, 0, 0}, s[3] = {1, 1, 1};   // rotation in degrees
+};
+struct GeomNode {   // MeshContainer (mesh.hpp): per-frame meshes and transforms
+    std::vector<int> mesh, xform;
+    int offset = 0, interval = 1;
+    bool pre_persist = false, post_persist = false;
+    int frames() const { return (int)std::min(mesh.size(), xform.size()); }
+};
+struct GeomLiquid {
+    int geom;
+    float velocity[3];
+};
+struct SceneGeom {
+    std::vector<GeomMesh> meshes;
+    std::vector<GeomXform> xforms;
+    std::vector<GeomNode> geoms;
+    std::vector<GeomLiquid> liquids;
+};
+bool read_obj_mesh(const std::string& path, GeomMesh& g, std::string& err);
+void cube_mesh(GeomMesh& g, const float p0[3], const float p1[3]);
+void sphere_mesh(GeomMesh& g, const float center[3], float radius, int subdivs);
+bool geom_in_frame(const GeomNode& n, float frame);
+bool geom_contains(const SceneGeom& s, const GeomNode& n, float frame, const float p[3]);
+bool geom_bounds(const SceneGeom& s, const GeomNode& n, float frame, float lo[3], float hi[3]);
+void emit_geom_liquids(const SceneGeom& s, const ariel_scene_desc& d, int N, int frame,
+                       std::vector<float4>& pos, std::vector<float4>& vel);
+// scene_json.hip: the reference's JSON scene (settings, forces and, with geom,
+// every geometry node and liquid; base_dir resolves mesh files)
+int scene_from_json(const char* text, const char* base_dir, ariel_scene_desc* d, SceneGeom* geom,
+                    std::string& err);
+
 // files.hip: the OpenVDB level-set file and the Partio particle files
 bool write_vdb(const char* path, const float* phi, int M, float bg, const char* grid_name, std::string& err);
 int write_partio(const char* path, int64_t n, const float* pos, const float* vel, std::string& err);

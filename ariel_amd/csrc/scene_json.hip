@@ -389,3 +389,202 @@ extern "C" int ariel_desc_from_json(const char* text, ariel_scene_desc* out) {
     *out = d;
     return ARIEL_OK;
 }
+
+namespace ariel {
+
+// The whole reference scene (SceneLoader, sceneloader.cpp:11-474): settings and
+// forces into d, and every geometry node into geom: mesh files (OBJ, read from
+// base_dir, or the box and sphere generators tessellated as the reference
+// does), transforms with rotation, animated mesh sequences (a mesh per frame;
+// the reference interpolates between consecutive meshes by the fraction of the
+// frame number, which is 0 on the integer frames a simulation steps through),
+// geoms with their frame ranges and persistence, and the liquids of "sim".
+// Solid geometry is refused (no solid constraints in this build).
+int scene_from_json(const char* text, const char* base_dir, ariel_scene_desc* d, SceneGeom* geom,
+                    std::string& err) {
+    Parser P{text, std::strlen(text)};
+    JValue root;
+    if (!P.value(root, 0)) {
+        err = P.err;
+        return ARIEL_ERR_INVALID;
+    }
+    P.ws();
+    if (P.i != P.n || root.kind != JValue::Object) {
+        err = "the document is not one JSON object";
+        return ARIEL_ERR_INVALID;
+    }
+    int rc = ariel_desc_defaults(d);
+    if (rc != ARIEL_OK) return rc;
+    float density = 0.5f, step = 0.005f;
+    int dims[3] = {32, 32, 32};
+    if (const JValue* st = at(root.get("settings"), 0)) {
+        if (st->get("density")) density = as_float(st->get("density"));
+        if (st->get("step_size")) step = as_float(st->get("step_size"));
+        if (const JValue* dim = st->get("dim")) {
+            dims[0] = as_int(dim->get("x"));
+            dims[1] = as_int(dim->get("y"));
+            dims[2] = as_int(dim->get("z"));
+        }
+    }
+    for (int a = 0; a < 3; ++a) d->dims[a] = dims[a];
+    d->density = density;
+    d->dt = step;
+    d->n_forces = 0;
+    if (const JValue* gf = root.get("globalforces")) {
+        for (size_t k = 0; k < size_of(gf); ++k) {
+            if (d->n_forces >= ARIEL_MAX_FORCES) {
+                err = "too many global forces";
+                return ARIEL_ERR_UNSUPPORTED;
+            }
+            vec3(at(gf, k), d->forces[d->n_forces]);
+            ++d->n_forces;
+        }
+    }
+    d->n_liquids = 0;
+    SceneGeom& G = *geom;
+    G = SceneGeom();
+    std::map<std::string, int> xform_id, mesh_id, anim_id, geom_id;
+    if (const JValue* tr = root.get("transforms")) {
+        for (size_t k = 0; k < size_of(tr); ++k) {
+            const JValue* t = at(tr, k);
+            if (!t || !t->get("id")) continue;
+            const std::string id = as_string(t->get("id"));
+            if (xform_id.count(id)) continue;
+            GeomXform x;
+            if (t->get("translation")) vec3(t->get("translation"), x.t);
+            if (t->get("rotation")) vec3(t->get("rotation"), x.r);
+            if (t->get("scale")) vec3(t->get("scale"), x.s);
+            xform_id[id] = (int)G.xforms.size();
+            G.xforms.push_back(x);
+        }
+    }
+    const std::string base = base_dir && *base_dir ? std::string(base_dir) + "/" : std::string();
+    if (const JValue* mf = root.get("meshfiles")) {
+        for (size_t k = 0; k < size_of(mf); ++k) {
+            const JValue* m = at(mf, k);
+            if (!m || !m->get("id")) continue;
+            const std::string id = as_string(m->get("id"));
+            if (mesh_id.count(id)) continue;
+            GeomMesh g;
+            if (m->get("file")) {
+                if (!read_obj_mesh(base + as_string(m->get("file")), g, err)) return ARIEL_ERR_IO;
+            } else {
+                const std::string gen = as_string(m->get("mesh_gen"));
+                if (gen == "box") {
+                    float p0[3], p1[3];
+                    vec3(m->get("point0"), p0);
+                    vec3(m->get("point1"), p1);
+                    cube_mesh(g, p0, p1);
+                } else if (gen == "sphere") {
+                    float c[3];
+                    vec3(m->get("center"), c);
+                    sphere_mesh(g, c, as_float(m->get("radius")), 20);
+                }
+            }
+            mesh_id[id] = (int)G.meshes.size();
+            G.meshes.push_back(std::move(g));
+        }
+    }
+    std::vector<std::vector<int>> anims;
+    if (const JValue* am = root.get("animatedmeshes")) {
+        for (size_t k = 0; k < size_of(am); ++k) {
+            const JValue* a = at(am, k);
+            if (!a || !a->get("id")) continue;
+            const std::string id = as_string(a->get("id"));
+            if (anim_id.count(id)) continue;
+            std::vector<int> seq;
+            for (size_t f = 0; f < size_of(a->get("frames")); ++f) {
+                auto it = mesh_id.find(as_string(at(a->get("frames"), f)));
+                if (it == mesh_id.end()) {
+                    err = "animated mesh \"" + id + "\" names an unknown mesh file";
+                    return ARIEL_ERR_INVALID;
+                }
+                seq.push_back(it->second);
+            }
+            anim_id[id] = (int)anims.size();
+            anims.push_back(seq);
+        }
+    }
+    if (const JValue* gs = root.get("geoms")) {
+        for (size_t k = 0; k < size_of(gs); ++k) {
+            const JValue* g = at(gs, k);
+            if (!g || !g->get("id")) continue;
+            const std::string id = as_string(g->get("id"));
+            if (geom_id.count(id)) continue;
+            GeomNode n;
+            n.pre_persist = as_bool(g->get("pre_persist"));
+            n.post_persist = as_bool(g->get("post_persist"));
+            n.interval = as_int(g->get("frame_interval"));
+            n.offset = as_int(g->get("frame_offset"));
+            if (n.interval <= 0) {
+                err = "geom \"" + id + "\": frame_interval must be positive";
+                return ARIEL_ERR_INVALID;
+            }
+            const std::string type = as_string(g->get("type"));
+            std::vector<int> meshes;
+            if (type == "mesh") {
+                for (size_t f = 0; f < size_of(g->get("geom_frames")); ++f) {
+                    auto it = mesh_id.find(as_string(at(g->get("geom_frames"), f)));
+                    if (it == mesh_id.end()) {
+                        err = "geom \"" + id + "\" names an unknown mesh file";
+                        return ARIEL_ERR_INVALID;
+                    }
+                    meshes.push_back(it->second);
+                }
+            } else if (type == "animated_mesh") {
+                auto it = anim_id.find(as_string(g->get("anim_sequence")));
+                if (it == anim_id.end()) {
+                    err = "geom \"" + id + "\" names an unknown animated mesh";
+                    return ARIEL_ERR_INVALID;
+                }
+                meshes = anims[it->second];
+            } else {
+                continue;
+            }
+            std::vector<int> xf;
+            for (size_t f = 0; f < size_of(g->get("transform_frames")); ++f) {
+                auto it = xform_id.find(as_string(at(g->get("transform_frames"), f)));
+                if (it == xform_id.end()) {
+                    err = "geom \"" + id + "\" names an unknown transform";
+                    return ARIEL_ERR_INVALID;
+                }
+                xf.push_back(it->second);
+            }
+            const size_t frames = std::min(meshes.size(), xf.size());
+            if (frames == 0) {
+                err = "geom \"" + id + "\" has no frames";
+                return ARIEL_ERR_INVALID;
+            }
+            n.mesh.assign(meshes.begin(), meshes.begin() + frames);
+            n.xform.assign(xf.begin(), xf.begin() + frames);
+            geom_id[id] = (int)G.geoms.size();
+            G.geoms.push_back(n);
+        }
+    }
+    if (const JValue* sim = root.get("sim")) {
+        for (size_t k = 0; k < size_of(sim); ++k) {
+            const JValue* s = at(sim, k);
+            if (!s) continue;
+            const std::string type = as_string(s->get("type"));
+            const std::string gid = as_string(s->get("geom"));
+            if (type == "solid") {
+                err = "solid geometry (geom \"" + gid + "\"): no solid constraints in this build";
+                return ARIEL_ERR_UNSUPPORTED;
+            }
+            if (type != "liquid") continue;
+            auto git = geom_id.find(gid);
+            if (git == geom_id.end()) {
+                err = "sim entry names unknown geom \"" + gid + "\"";
+                return ARIEL_ERR_INVALID;
+            }
+            GeomLiquid lq;
+            lq.geom = git->second;
+            lq.velocity[0] = lq.velocity[1] = lq.velocity[2] = 0.0f;
+            if (s->get("velocity")) vec3(s->get("velocity"), lq.velocity);
+            G.liquids.push_back(lq);
+        }
+    }
+    return ARIEL_OK;
+}
+
+}  // namespace ariel

@@ -45,6 +45,29 @@ class FlipSim:
             if os.path.exists(DEFAULT_WEIGHTS):
                 self.set_learned_weights(DEFAULT_WEIGHTS)
 
+    @classmethod
+    def from_json(cls, text_or_path, base_dir=None, solver=None):
+        """FlipSim with the reference scene's full geometry (ariel_create_from_json):
+        settings, forces, mesh files (relative to base_dir, default the JSON file's
+        folder), transforms, animated meshes, emitters; solver fields from `solver`."""
+        lib = _abi.load()
+        text = text_or_path
+        if os.path.exists(str(text_or_path)):
+            text = open(text_or_path).read()
+            if base_dir is None:
+                base_dir = os.path.dirname(os.path.abspath(str(text_or_path)))
+        h = ctypes.c_void_p()
+        rc = lib.ariel_create_from_json(text.encode(), (base_dir or "").encode(),
+                                        ctypes.byref(solver) if solver is not None else None,
+                                        ctypes.byref(h))
+        if rc != 0:
+            raise ArielError(rc, lib.ariel_last_error(None).decode())
+        d = SceneDesc()
+        lib.ariel_get_desc(h, ctypes.byref(d))
+        sim = cls(d, _handle=h.value)
+        sim._owned = True
+        return sim
+
     # ------------------------------------------------------------ lifecycle
     def _check(self, rc):
         if rc != 0:

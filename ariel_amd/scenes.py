@@ -177,6 +177,28 @@ def from_json(text_or_path, **overrides):
     return d
 
 
+def emit(text, frame, base_dir=None, solver=None):
+    """The particles a JSON scene's liquid geoms emit on `frame` (ariel_scene_emit,
+    host only): (pos, vel), float32 arrays of shape (n, 3)."""
+    import ctypes
+    import numpy as np
+    lib = _abi.load()
+    n = ctypes.c_int64()
+    args = (text.encode(), (base_dir or "").encode(),
+            ctypes.byref(solver) if solver is not None else None, frame)
+    rc = lib.ariel_scene_emit(*args, None, None, 0, ctypes.byref(n))
+    if rc == 0:
+        pos = np.zeros((n.value, 3), np.float32)
+        vel = np.zeros((n.value, 3), np.float32)
+        PF = ctypes.POINTER(ctypes.c_float)
+        rc = lib.ariel_scene_emit(*args, pos.ctypes.data_as(PF), vel.ctypes.data_as(PF), n.value,
+                                  ctypes.byref(n))
+    if rc != 0:
+        from .flipsim import ArielError
+        raise ArielError(rc, lib.ariel_last_error(None).decode())
+    return pos, vel
+
+
 def copy(d):
     out = SceneDesc()
     out_bytes = bytes(d)

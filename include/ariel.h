@@ -287,6 +287,34 @@ int ariel_get_kernel_stats(ariel_sim* h, int32_t kernel, ariel_kernel_stats* out
 int ariel_save_state(ariel_sim* h, const char* path);
 int ariel_load_state(ariel_sim* h, const char* path);
 
+/* FlipSim(maxres, density, stepsize, Scene*, ...) with the Scene of
+ * SceneLoader (src/scene/sceneloader.cpp:11-474): the reference's JSON scene,
+ * text in json, mesh files read relative to base_dir. Settings (dim, density,
+ * step_size) and global forces come from the scene, the solver fields of
+ * *solver (every other field; defaults if solver is NULL). Geometry: OBJ mesh
+ * files (faces of 3 or 4 vertices) and the box and sphere generators,
+ * tessellated as the reference does; transforms with translation, rotation
+ * (degrees, R = Rz Ry Rx) and scale, interpolated between frames; animated
+ * mesh sequences; geoms with frame offset, interval and pre/post persistence.
+ * Every liquid geom emits particles (lattice points it contains, by ray
+ * parity along +z, scene.cpp:362-378, 436-455) at Init and on every step whose
+ * frame it is in (Scene::GenerateParticles, scene.cpp:257-300), so a
+ * persistent emitter keeps adding particles. Solid geometry returns
+ * ARIEL_ERR_UNSUPPORTED (no solid constraints in this build). */
+int ariel_create_from_json(const char* json, const char* base_dir, const ariel_scene_desc* solver,
+                           ariel_sim** out);
+
+/* FlipSim::GetScene (src/sim/flip.hpp:31): the descriptor the handle runs with
+ * (for ariel_create_from_json, the scene's settings and forces with the solver
+ * fields). */
+int ariel_get_desc(ariel_sim* h, ariel_scene_desc* out);
+
+/* The particles the scene's liquid geoms emit on `frame` (host only, no GPU):
+ * positions and velocities in the order they join a simulation; *n is set to
+ * their count (pos and vel may be NULL to query it). No reference counterpart. */
+int ariel_scene_emit(const char* json, const char* base_dir, const ariel_scene_desc* solver,
+                     int32_t frame, float* pos, float* vel, int64_t cap, int64_t* n);
+
 /* Scene::ExportParticles VDB path (src/scene/scene.cpp:84-101, LevelSet from
  * particles, src/grid/levelset.cpp:101-111): the fluid particles' level set,
  * rasterised on the GPU on the (N+1)^3 nodes of a grid of voxel size 1 in grid

@@ -4,10 +4,10 @@
 // Step() in a loop, reading GetParticles() after each step) keep their calls.
 //
 // Differences that a caller sees:
-//  * the scene is an ariel_scene_desc (liquid primitives, forces, seed) instead
-//    of sceneCore::Scene; ariel_desc_from_json reads the reference's JSON scene
-//    format (box and sphere generators), other mesh geometry stays with the
-//    caller;
+//  * the scene is an ariel_scene_desc (liquid primitives, forces, seed) or the
+//    reference's JSON scene with its mesh files (the second constructor:
+//    meshes, transforms, animated meshes, emitters; no solid geometry) instead
+//    of sceneCore::Scene;
 //  * GetParticles() returns a snapshot by value (device memory stays on the GPU)
 //    in particle-vector order, not a pointer to live particles;
 //  * failures throw ariel::Error carrying the ABI status and message.
@@ -52,6 +52,15 @@ public:
     // dims, density and dt live in the descriptor.
     explicit FlipSim(const ariel_scene_desc& desc) : m_frame(0), desc_(desc) {
         check(ariel_create(&desc, &h_), nullptr);
+    }
+    // FlipSim(..., Scene*) with the Scene that SceneLoader reads from a JSON scene
+    // (sceneloader.cpp:11-474): the scene's text, the folder its mesh files are
+    // relative to, and the solver fields (defaults when null).
+    FlipSim(const std::string& scene_json, const std::string& base_dir,
+            const ariel_scene_desc* solver = nullptr)
+        : m_frame(0) {
+        check(ariel_create_from_json(scene_json.c_str(), base_dir.c_str(), solver, &h_), nullptr);
+        check(ariel_get_desc(h_, &desc_), h_);
     }
     ~FlipSim() { ariel_destroy(h_); }
     FlipSim(const FlipSim&) = delete;

@@ -118,6 +118,83 @@ void mesher_paths(const char* dir) {
           "unwritable path");
 }
 
+ariel_scene_desc dam_break(int n);
+
+// The file writers (VDB, Partio) and the scene geometry's emission: valid
+// inputs, truncated OBJ files and refusals.
+void files_and_scene_paths(const char* dir) {
+    const int M = 21;
+    std::vector<float> phi((size_t)M * M * M);
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j)
+            for (int k = 0; k < M; ++k) {
+                const float fx = i - 10.f, fy = j - 9.f, fz = k - 11.f;
+                const float v = std::sqrt(fx * fx + fy * fy + fz * fz) - 6.1f;
+                phi[((size_t)i * M + j) * M + k] = v < -3.f ? -3.f : (v > 3.f ? 3.f : v);
+            }
+    const std::string vdb = std::string(dir) + "/s.vdb";
+    check(ariel_write_vdb(phi.data(), M, 3.0f, vdb.c_str()) == ARIEL_OK, "vdb write");
+    check(ariel_write_vdb(phi.data(), M, 3.0f, "/nonexistent-dir/s.vdb") == ARIEL_ERR_IO, "vdb unwritable");
+    check(ariel_write_vdb(nullptr, M, 3.0f, vdb.c_str()) == ARIEL_ERR_INVALID, "vdb null");
+    std::vector<float> pos(300), vel(300);
+    for (int i = 0; i < 300; ++i) {
+        pos[i] = 0.1f * i;
+        vel[i] = -0.05f * i;
+    }
+    for (const char* ext : {".geo", ".bgeo", ".geo.gz", ".bgeo.gz"}) {
+        const std::string p = std::string(dir) + "/p" + ext;
+        check(ariel_write_particles(p.c_str(), 100, pos.data(), vel.data()) == ARIEL_OK, "partio write");
+    }
+    check(ariel_write_particles((std::string(dir) + "/p.pdb").c_str(), 100, pos.data(), vel.data()) ==
+              ARIEL_ERR_UNSUPPORTED, "partio format refused");
+    // scenes: generators, a rotation, an OBJ file (and a truncated one), frames
+    const std::string obj = std::string(dir) + "/t.obj";
+    if (FILE* f = std::fopen(obj.c_str(), "w")) {
+        std::fputs("v 4 4 4\nv 20 4 4\nv 4 20 4\nv 4 4 20\nf 1 3 2\nf 1 2 4\nf 1 4 3\nf 2 3 4\n", f);
+        std::fclose(f);
+    }
+    const std::string bad = std::string(dir) + "/bad.obj";
+    if (FILE* f = std::fopen(bad.c_str(), "w")) {
+        std::fputs("v 1 2\nf 1 9 3\n", f);
+        std::fclose(f);
+    }
+    const char* scene =
+        "{\"settings\":[{\"density\":0.5,\"step_size\":0.004,\"dim\":{\"x\":24,\"y\":24,\"z\":24}}],"
+        "\"meshfiles\":[{\"id\":\"b\",\"mesh_gen\":\"box\",\"point0\":{\"x\":2,\"y\":2,\"z\":2},"
+        "\"point1\":{\"x\":9,\"y\":12,\"z\":20}},{\"id\":\"s\",\"mesh_gen\":\"sphere\","
+        "\"center\":{\"x\":15,\"y\":15,\"z\":12},\"radius\":4},{\"id\":\"o\",\"file\":\"t.obj\"}],"
+        "\"transforms\":[{\"id\":\"t\"},{\"id\":\"r\",\"rotation\":{\"x\":10,\"y\":20,\"z\":30}}],"
+        "\"animatedmeshes\":[{\"id\":\"a\",\"frames\":[\"b\",\"s\"]}],"
+        "\"geoms\":[{\"id\":\"g0\",\"type\":\"mesh\",\"geom_frames\":[\"b\"],\"transform_frames\":[\"r\"],"
+        "\"frame_interval\":1,\"frame_offset\":0},{\"id\":\"g1\",\"type\":\"animated_mesh\","
+        "\"anim_sequence\":\"a\",\"transform_frames\":[\"t\",\"t\"],\"frame_interval\":2,"
+        "\"frame_offset\":0,\"post_persist\":true},{\"id\":\"g2\",\"type\":\"mesh\","
+        "\"geom_frames\":[\"o\"],\"transform_frames\":[\"t\"],\"frame_interval\":1,\"frame_offset\":0}],"
+        "\"sim\":[{\"geom\":\"g0\",\"type\":\"liquid\"},{\"geom\":\"g1\",\"type\":\"liquid\"},"
+        "{\"geom\":\"g2\",\"type\":\"liquid\",\"velocity\":{\"x\":1,\"y\":0,\"z\":0}}]}";
+    ariel_scene_desc solver = dam_break(24);
+    for (int frame = 0; frame < 4; ++frame) {
+        int64_t n = -1;
+        check(ariel_scene_emit(scene, dir, &solver, frame, nullptr, nullptr, 0, &n) == ARIEL_OK && n > 0,
+              "scene emission count");
+        std::vector<float> p(3 * (size_t)n), v(3 * (size_t)n);
+        check(ariel_scene_emit(scene, dir, &solver, frame, p.data(), v.data(), n, &n) == ARIEL_OK,
+              "scene emission");
+    }
+    std::string broken(scene);
+    const size_t at = broken.find("t.obj");
+    broken.replace(at, 5, "bad.obj");
+    int64_t n = 0;
+    check(ariel_scene_emit(broken.c_str(), dir, &solver, 0, nullptr, nullptr, 0, &n) != ARIEL_OK,
+          "truncated OBJ refused");
+    check(ariel_scene_emit(scene, "/nonexistent-dir", &solver, 0, nullptr, nullptr, 0, &n) == ARIEL_ERR_IO,
+          "missing mesh file");
+    for (size_t cut = 1; cut < std::strlen(scene); cut += 37) {
+        const std::string part(scene, cut);
+        (void)ariel_scene_emit(part.c_str(), dir, &solver, 0, nullptr, nullptr, 0, &n);
+    }
+}
+
 ariel_scene_desc dam_break(int n) {
     ariel_scene_desc d;
     ariel_desc_defaults(&d);
@@ -237,6 +314,7 @@ int main(int argc, char** argv) {
     const char* dir = argc > 1 ? argv[1] : "/tmp";
     json_paths();
     mesher_paths(dir);
+    files_and_scene_paths(dir);
     boundary_paths();
     oracle_paths();
     if (g_fail) {

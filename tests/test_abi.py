@@ -122,6 +122,11 @@ int main() {
   } catch (const ariel::Error& e) {
     std::printf("error %d\n", e.code());
   }
+  try {
+    ariel::FlipSim js(std::string("{\"sim\": []}"), std::string("."));
+  } catch (const ariel::Error& e) {
+    std::printf("json error %d\n", e.code());
+  }
   return 0;
 }
 '''

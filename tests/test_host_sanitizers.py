@@ -53,7 +53,7 @@ def test_host_code_is_clean_under_asan_and_ubsan(tmp_path):
             assert r.returncode == 0, f"{' '.join(cmd)}\n{r.stderr[-4000:]}"
     exe = os.path.join(out, "host_sanitize")
     cmd, r = _run([CLANG, f"-fsanitize={SAN}", "-pthread", "-o", exe, *objs, "-L/opt/rocm/lib",
-                   "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"], out)
+                   "-lamdhip64", "-lz", "-Wl,-rpath,/opt/rocm/lib"], out)
     assert r.returncode == 0, r.stderr[-4000:]
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
                UBSAN_OPTIONS="print_stacktrace=1")
