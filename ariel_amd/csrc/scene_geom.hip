@@ -16,10 +16,19 @@
 //    (Ray::Transform: origin and origin + direction transformed, their
 //    difference, normalised, the direction), counting every triangle or quad it meets
 //    (Obj::RayTriangleTest, obj.cpp:354-400, quads as two triangles counted
-//    once, obj.cpp:344-351) whose hit lies in front of the origin by the
-//    reference's test acos(dot(normalize(hit - o), d)) < pi/2 or NaN
-//    (bvh.inl:80-86); an odd count is inside. Every primitive is tested (no
-//    BVH): a BVH only skips primitives the ray misses.
+//    once, obj.cpp:344-351) whose hit lies in front of the origin or at it
+//    (bvh.inl:80-86); an odd count is inside. The reference decides "in front"
+//    by acos(dot(normalize(hit - o), d)) < pi/2 or NaN. Every hit lies on the
+//    ray, so that cosine is +-1 up to rounding, and a hit behind the origin
+//    whose rounded cosine falls below -1 gives NaN and is counted: for rays
+//    that are not axis-aligned in object space (rotated geoms) about half of
+//    the hits behind the origin count, and points outside the mesh are
+//    emitted. Here the test is what the reference's own comment there asks
+//    for ("hit is actually in front of origin"): dot(hit - o, d) > 0 or
+//    hit == o, the reference's outcome whenever its cosine rounds to +-1
+//    exactly. Instead of a BVH the primitives are binned by their projection
+//    along the ray direction (RayCull); both only skip primitives the ray
+//    misses.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -295,38 +304,207 @@ bool geom_at(const SceneGeom& s, const GeomNode& n, float frame, int& mesh, floa
 }
 
 // Point p (grid units) inside geom n at `frame` (CheckPointInsideGeomByID).
-bool geom_contains(const SceneGeom& s, const GeomNode& n, float frame, const float p[3]) {
+namespace {
+
+// The inside test of one geom at one frame, prepared for many points: the
+// inverse transform once, and the primitives binned by their projection on the
+// plane normal to the object-space ray direction d0 (every point's ray is +z in
+// world space, so in object space it runs along d0 up to rounding). A primitive
+// can report a hit only for origins whose projection lies in its projected
+// bounds widened by a margin that covers the rounding of the hit test
+// (barycentrics are quotients by the projected area, so thin primitives get
+// wide margins); primitives whose margin would cover much of the grid, or whose
+// projection is nearly degenerate, are tested for every point. The count of
+// hits, hence the result, is the one a test of every primitive gives.
+struct RayCull {
+    const GeomMesh* g = nullptr;
+    M4 inv;
+    double u[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    double x0 = 0, y0 = 0, inv_cs = 0;
+    int nx = 0, ny = 0;
+    std::vector<int> start, cand, always;
+};
+
+double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+bool prepare_cull(const SceneGeom& s, const GeomNode& n, float frame, const float qlo[3],
+                  const float qhi[3], RayCull& rc) {
     int mi;
     float Mf[16], Mi[16];
     if (!geom_at(s, n, frame, mi, Mf, Mi)) return false;
-    M4 inv;
-    std::memcpy(inv.m, Mi, sizeof(inv.m));
+    std::memcpy(rc.inv.m, Mi, sizeof(rc.inv.m));
+    const GeomMesh& g = s.meshes[mi];
+    rc.g = &g;
+    // object-space direction of world +z, and a basis (u, v) of its normal plane
+    const float z0[4] = {0, 0, 0, 1}, z1[4] = {0, 0, 1, 1};
+    float a0[4], a1[4];
+    apply(rc.inv, z0, a0);
+    apply(rc.inv, z1, a1);
+    double d0[3] = {(double)a1[0] - a0[0], (double)a1[1] - a0[1], (double)a1[2] - a0[2]};
+    const double dl = std::sqrt(dot3(d0, d0));
+    for (double& c : d0) c /= dl;
+    int ax = 0;
+    for (int k = 1; k < 3; ++k)
+        if (std::fabs(d0[k]) < std::fabs(d0[ax])) ax = k;
+    double e[3] = {0, 0, 0};
+    e[ax] = 1.0;
+    double* u = rc.u;
+    double* v = rc.v;
+    u[0] = d0[1] * e[2] - d0[2] * e[1];
+    u[1] = d0[2] * e[0] - d0[0] * e[2];
+    u[2] = d0[0] * e[1] - d0[1] * e[0];
+    const double ul = std::sqrt(dot3(u, u));
+    for (int k = 0; k < 3; ++k) u[k] /= ul;
+    v[0] = d0[1] * u[2] - d0[2] * u[1];
+    v[1] = d0[2] * u[0] - d0[0] * u[2];
+    v[2] = d0[0] * u[1] - d0[1] * u[0];
+    // D bounds the object-space coordinates of every origin and vertex
+    double D = 0.0;
+    for (int c = 0; c < 8; ++c) {
+        const float w4[4] = {(c & 4) ? qhi[0] : qlo[0], (c & 2) ? qhi[1] : qlo[1], (c & 1) ? qhi[2] : qlo[2], 1.0f};
+        float o4[4];
+        apply(rc.inv, w4, o4);
+        D = std::max(D, std::sqrt((double)o4[0] * o4[0] + (double)o4[1] * o4[1] + (double)o4[2] * o4[2]));
+    }
+    for (const auto& p : g.v) D = std::max(D, std::sqrt((double)p[0] * p[0] + (double)p[1] * p[1] + (double)p[2] * p[2]));
+    D = 2.0 * D + 1e-30;
+    const size_t F = g.f.size();
+    std::vector<double> box(4 * F);
+    std::vector<char> wide(F, 0);
+    double gx0 = 1e300, gy0 = 1e300, gx1 = -1e300, gy1 = -1e300;
+    for (size_t fi = 0; fi < F; ++fi) {
+        const auto& f = g.f[fi];
+        const int nv = f[3] != 0 ? 4 : 3;
+        double px[4], py[4];
+        for (int k = 0; k < nv; ++k) {
+            const double q[3] = {g.v[f[k] - 1][0], g.v[f[k] - 1][1], g.v[f[k] - 1][2]};
+            px[k] = dot3(q, u);
+            py[k] = dot3(q, v);
+        }
+        // the smaller projected area of the primitive's triangles (a quad is
+        // (0, 1, 2) and (0, 3, 2)) and its longest projected edge
+        auto area2 = [&](int i, int j, int k) {
+            return std::fabs((px[j] - px[i]) * (py[k] - py[i]) - (px[k] - px[i]) * (py[j] - py[i]));
+        };
+        double A = area2(0, 1, 2);
+        if (nv == 4) A = std::min(A, area2(0, 3, 2));
+        double L2 = 0.0;
+        for (int i = 0; i < nv; ++i)
+            for (int j = i + 1; j < nv; ++j)
+                L2 = std::max(L2, (px[j] - px[i]) * (px[j] - px[i]) + (py[j] - py[i]) * (py[j] - py[i]));
+        double lo0 = px[0], hi0 = px[0], lo1 = py[0], hi1 = py[0];
+        for (int k = 1; k < nv; ++k) {
+            lo0 = std::min(lo0, px[k]);
+            hi0 = std::max(hi0, px[k]);
+            lo1 = std::min(lo1, py[k]);
+            hi1 = std::max(hi1, py[k]);
+        }
+        if (!(A >= 1e-3 * L2) || !(L2 > 0.0)) {
+            wide[fi] = 1;
+            continue;
+        }
+        const double m = 1e-4 * D * (1.0 + L2 / A);
+        box[4 * fi + 0] = lo0 - m;
+        box[4 * fi + 1] = hi0 + m;
+        box[4 * fi + 2] = lo1 - m;
+        box[4 * fi + 3] = hi1 + m;
+        gx0 = std::min(gx0, box[4 * fi + 0]);
+        gx1 = std::max(gx1, box[4 * fi + 1]);
+        gy0 = std::min(gy0, box[4 * fi + 2]);
+        gy1 = std::max(gy1, box[4 * fi + 3]);
+    }
+    const int side = std::max(1, std::min(512, (int)std::sqrt((double)F * 2.0)));
+    const double span = std::max(gx1 - gx0, gy1 - gy0);
+    if (!(span > 0.0) || !std::isfinite(span)) {
+        for (size_t fi = 0; fi < F; ++fi) rc.always.push_back((int)fi);
+        return true;
+    }
+    rc.x0 = gx0;
+    rc.y0 = gy0;
+    rc.inv_cs = side / span;
+    rc.nx = std::max(1, (int)std::ceil((gx1 - gx0) * rc.inv_cs));
+    rc.ny = std::max(1, (int)std::ceil((gy1 - gy0) * rc.inv_cs));
+    const int64_t ncell = (int64_t)rc.nx * rc.ny;
+    auto cell_range = [&](size_t fi, int& i0, int& i1, int& j0, int& j1) {
+        i0 = std::max(0, (int)std::floor((box[4 * fi + 0] - rc.x0) * rc.inv_cs));
+        i1 = std::min(rc.nx - 1, (int)std::floor((box[4 * fi + 1] - rc.x0) * rc.inv_cs));
+        j0 = std::max(0, (int)std::floor((box[4 * fi + 2] - rc.y0) * rc.inv_cs));
+        j1 = std::min(rc.ny - 1, (int)std::floor((box[4 * fi + 3] - rc.y0) * rc.inv_cs));
+    };
+    std::vector<int> count(ncell + 1, 0);
+    for (size_t fi = 0; fi < F; ++fi) {
+        if (wide[fi]) continue;
+        int i0, i1, j0, j1;
+        cell_range(fi, i0, i1, j0, j1);
+        if ((int64_t)(i1 - i0 + 1) * (j1 - j0 + 1) * 4 > ncell) {
+            wide[fi] = 1;
+            continue;
+        }
+        for (int i = i0; i <= i1; ++i)
+            for (int j = j0; j <= j1; ++j) ++count[(int64_t)i * rc.ny + j];
+    }
+    rc.start.assign(ncell + 1, 0);
+    for (int64_t c = 0; c < ncell; ++c) rc.start[c + 1] = rc.start[c] + count[c];
+    rc.cand.resize(rc.start[ncell]);
+    std::vector<int> fill(rc.start.begin(), rc.start.end() - 1);
+    for (size_t fi = 0; fi < F; ++fi) {
+        if (wide[fi]) {
+            rc.always.push_back((int)fi);
+            continue;
+        }
+        int i0, i1, j0, j1;
+        cell_range(fi, i0, i1, j0, j1);
+        for (int i = i0; i <= i1; ++i)
+            for (int j = j0; j <= j1; ++j) rc.cand[fill[(int64_t)i * rc.ny + j]++] = (int)fi;
+    }
+    return true;
+}
+
+bool cull_contains(const RayCull& rc, const float p[3]) {
     const float o4[4] = {p[0], p[1], p[2], 1.0f}, e4[4] = {p[0], p[1], p[2] + 1.0f, 1.0f};
     float o[4], e[4];
-    apply(inv, o4, o);
-    apply(inv, e4, e);
+    apply(rc.inv, o4, o);
+    apply(rc.inv, e4, e);
     // the Ray constructor normalises its direction (ray.cpp:24-30)
     float d[3] = {e[0] - o[0], e[1] - o[1], e[2] - o[2]};
     const float dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     for (float& c : d) c = c / dl;
-    const GeomMesh& g = s.meshes[mi];
-    unsigned hits = 0;
+    const GeomMesh& g = *rc.g;
+    // in front of the origin, or at it: n = hit - o points along d, or is zero
     auto front = [&](const float hit[3]) {
-        float n3[3] = {hit[0] - o[0], hit[1] - o[1], hit[2] - o[2]};
-        const float len = std::sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
-        for (float& c : n3) c = c / len;
-        const float deg = std::acos(n3[0] * d[0] + n3[1] * d[1] + n3[2] * d[2]);
-        return (double)deg < kPi / 2.0f || deg != deg;
+        const float n3[3] = {hit[0] - o[0], hit[1] - o[1], hit[2] - o[2]};
+        if (n3[0] == 0.0f && n3[1] == 0.0f && n3[2] == 0.0f) return true;
+        return n3[0] * d[0] + n3[1] * d[1] + n3[2] * d[2] > 0.0f;
     };
-    for (const auto& f : g.f) {
+    auto hits_face = [&](int fi) {
+        const auto& f = g.f[fi];
         float hit[3];
         const auto& v0 = g.v[f[0] - 1];
         const auto& v2 = g.v[f[2] - 1];
         bool h = ray_triangle(v0, g.v[f[1] - 1], v2, o, d, hit);
         if (!h && f[3] != 0) h = ray_triangle(v0, g.v[f[3] - 1], v2, o, d, hit);
-        if (h && front(hit)) ++hits;
+        return h && front(hit);
+    };
+    unsigned hits = 0;
+    for (int fi : rc.always) hits += hits_face(fi) ? 1u : 0u;
+    if (rc.nx > 0) {
+        const double od[3] = {o[0], o[1], o[2]};
+        const double pu = dot3(od, rc.u), pv = dot3(od, rc.v);
+        const double fx = std::floor((pu - rc.x0) * rc.inv_cs), fy = std::floor((pv - rc.y0) * rc.inv_cs);
+        if (fx >= 0.0 && fy >= 0.0 && fx < rc.nx && fy < rc.ny) {
+            const int64_t c = (int64_t)fx * rc.ny + (int64_t)fy;
+            for (int k = rc.start[c]; k < rc.start[c + 1]; ++k) hits += hits_face(rc.cand[k]) ? 1u : 0u;
+        }
     }
     return hits % 2 == 1;
+}
+
+}  // namespace
+
+bool geom_contains(const SceneGeom& s, const GeomNode& n, float frame, const float p[3]) {
+    RayCull rc;
+    if (!prepare_cull(s, n, frame, p, p, rc)) return false;
+    return cull_contains(rc, p);
 }
 
 // World-space bounds of geom n at `frame` (MeshContainer::GetAabb, Aabb::Transform).
@@ -379,6 +557,10 @@ void emit_geom_liquids(const SceneGeom& s, const ariel_scene_desc& d, int N, int
         }
         const int span = std::max(0, c1[0] - c0[0]);
         if (span == 0 || c1[1] <= c0[1] || c1[2] <= c0[2]) continue;
+        const float qlo[3] = {(float)c0[0], (float)c0[1], (float)c0[2]};
+        const float qhi[3] = {(float)c1[0], (float)c1[1], (float)c1[2]};
+        RayCull rc;
+        if (!prepare_cull(s, gn, (float)frame, qlo, qhi, rc)) continue;
         const unsigned hw = std::thread::hardware_concurrency();
         const int nth = std::max(1, std::min<int>(std::min(hw, 16u), span));
         std::vector<std::vector<float4>> P(nth), V(nth);
@@ -405,7 +587,7 @@ void emit_geom_liquids(const SceneGeom& s, const ariel_scene_desc& d, int N, int
                                         const float x = (float)a * w + jx * w, y = (float)b * w + jy * w,
                                                     z = (float)c * w + jz * w;
                                         const float world[3] = {x * maxd, y * maxd, z * maxd};
-                                        if (!geom_contains(s, gn, (float)frame, world)) continue;
+                                        if (!cull_contains(rc, world)) continue;
                                         P[t].push_back(make_float4(x, y, z, 0.0f));
                                         V[t].push_back(make_float4(lq.velocity[0], lq.velocity[1], lq.velocity[2], 10.0f));
                                     }

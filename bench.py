@@ -79,8 +79,8 @@ def profile_traffic(group):
     """HBM traffic per launch (bytes) of a kernel group from the calibrated counter
     summary profiles/flip_step_256_traffic.md (median FETCH_SIZE x 2 + WRITE_SIZE
     per dispatch, MB = 1e6 bytes), and the rows it sums; (None, None) if absent."""
-    kernels = {"precond": ["KMicApply<0>", "KMicApply<1>", "KMicApply<2>"], "spmv": ["KSpmvDot"],
-               "p2g": ["KP2gCols<false>"], "axpy": ["KAxpy2"], "density": ["KDensity"],
+    kernels = {"precond": ["KMicApply<0,true>", "KMicApply<1,true>", "KMicApply<2,false>"],
+               "spmv": ["KTileSpmv"], "p2g": ["KP2gCols<false>"], "density": ["KDensity"],
                "resample": ["KResample"], "advect": ["KG2pAdvect"]}.get(group)
     path = os.path.join(HERE, "profiles", "flip_step_256_traffic.md")
     if not kernels or not os.path.exists(path):

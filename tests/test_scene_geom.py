@@ -200,3 +200,78 @@ def test_liquid_velocity_and_refusals(tmp_path):
     with pytest.raises(ArielError) as e:
         scenes.emit(missing, 0, base_dir=str(tmp_path), solver=_solver())
     assert e.value.code == -7
+
+
+def _torus(path, R=9.0, r=3.5, nu=32, nv=16):
+    """A torus of nu x nv quads, written as an OBJ file; the reference tests a
+    quad as the triangles (0, 1, 2) and (0, 3, 2), listed here with consistent
+    orientation for the winding number."""
+    a = 2 * np.pi * np.arange(nu)[:, None] / nu
+    b = 2 * np.pi * np.arange(nv)[None, :] / nv
+    verts = np.stack([(R + r * np.cos(b)) * np.cos(a), np.broadcast_to(r * np.sin(b), (nu, nv)),
+                      (R + r * np.cos(b)) * np.sin(a)], -1).reshape(-1, 3)
+    quads = []
+    for i in range(nu):
+        for j in range(nv):
+            quads.append((i * nv + j, ((i + 1) % nu) * nv + j, ((i + 1) % nu) * nv + (j + 1) % nv, i * nv + (j + 1) % nv))
+    with open(path, "w") as f:
+        for v in verts:
+            f.write("v %.6f %.6f %.6f\n" % tuple(v))
+        for q in quads:
+            f.write("f " + " ".join(str(i + 1) for i in q) + "\n")
+    verts = np.loadtxt(path, usecols=(1, 2, 3), max_rows=len(verts)).astype(np.float32).astype(np.float64)
+    tris = [(q[0], q[1], q[2]) for q in quads] + [(q[0], q[2], q[3]) for q in quads]
+    return verts, tris
+
+
+def test_rotated_scaled_torus_against_the_winding_number(tmp_path):
+    # a 1024-triangle torus turned about all three axes and scaled unevenly:
+    # the binned ray test must keep exactly the lattice points of winding number 1
+    verts, tris = _torus(tmp_path / "t.obj")
+    t, r, s = np.array([16.3, 15.1, 16.7]), np.array([23.0, -41.0, 67.0]), np.array([1.2, 0.8, 1.05])
+    text = _one({"id": "m", "file": "t.obj"}, {"translation": dict(zip("xyz", t)),
+                                               "rotation": dict(zip("xyz", r)), "scale": dict(zip("xyz", s))})
+    pos, _ = scenes.emit(text, 0, base_dir=str(tmp_path), solver=_solver())
+    def rot(axis, deg):
+        c, sn = np.cos(np.radians(deg)), np.sin(np.radians(deg))
+        m = np.eye(3)
+        i, j = [(1, 2), (2, 0), (0, 1)][axis]
+        m[i, i], m[i, j], m[j, i], m[j, j] = c, -sn, sn, c
+        return m
+    R = rot(2, r[2]) @ rot(1, r[1]) @ rot(0, r[0])          # T Rz Ry Rx S
+    world = verts * s @ R.T + t
+    pall, _ = scenes.emit(_one(_box("m", (1, 1, 1), (31, 31, 31))), 0, solver=_solver())
+    g = pall.astype(np.float64) * N
+    near = np.all((g > world.min(0) - 1) & (g < world.max(0) + 1), axis=1)   # outside: winding 0
+    w = np.zeros(len(g))
+    w[near] = np.abs(_winding(g[near], world, tris))      # the quads face inwards: -1 inside
+    inside = np.abs(w - 1.0) < 1e-3
+    assert np.all((np.abs(w) < 1e-3) | inside)
+    assert inside.sum() > 1000
+    assert np.array_equal(np.sort(pos.view("V12"), axis=0), np.sort(pall[inside].view("V12"), axis=0))
+
+
+@pytest.mark.parametrize("r", [(23.0, 0.0, 0.0), (0.0, -41.0, 0.0), (23.0, -41.0, 67.0), (90.0, 45.0, 0.0)])
+def test_box_turned_about_every_axis(r):
+    # the ray from a lattice point is tilted in the box's frame: only hits in
+    # front of the point count, so the emitted set is exactly the lattice points
+    # inside the turned box (R = Rz Ry Rx, degrees)
+    h, c = np.array([5.0, 3.0, 7.0]), np.array([16.0, 16.0, 16.0])
+    text = _one(_box("m", tuple(-h), tuple(h)), {"translation": dict(zip("xyz", c)), "rotation": dict(zip("xyz", r))})
+    pos, _ = scenes.emit(text, 0, solver=_solver())
+    def rot(axis, deg):
+        cs, sn = np.cos(np.radians(deg)), np.sin(np.radians(deg))
+        m = np.eye(3)
+        i, j = [(1, 2), (2, 0), (0, 1)][axis]
+        m[i, i], m[i, j], m[j, i], m[j, j] = cs, -sn, sn, cs
+        return m
+    R = rot(2, r[2]) @ rot(1, r[1]) @ rot(0, r[0])
+    pall, _ = scenes.emit(_one(_box("m", (1, 1, 1), (31, 31, 31))), 0, solver=_solver())
+    local = (pall.astype(np.float64) * N - c) @ R
+    margin = np.min(h - np.abs(local), axis=1)
+    clear = np.abs(margin) > 1e-4                          # leave out points on a face
+    ref = pall[clear & (margin > 0)]
+    assert len(ref) > 1000 and (~clear).sum() < 10
+    near = {tuple(p) for p in pall[~clear].view(np.uint32).tolist()}
+    got = np.array([p for p in pos if tuple(p.view(np.uint32).tolist()) not in near], np.float32).reshape(-1, 3)
+    assert np.array_equal(np.sort(got.view("V12"), axis=0), np.sort(ref.view("V12"), axis=0))
