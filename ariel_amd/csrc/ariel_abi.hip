@@ -469,6 +469,7 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
     h->d = *d;
     h->N = d->dims[0];
     h->nc = (int64_t)h->N * h->N * h->N;
+    h->part_cap = partial_capacity(h->N);
     h->nf = (int64_t)(h->N + 1) * h->N * h->N;
     h->maxd = std::max(std::max((float)d->dims[0], (float)d->dims[2]), (float)d->dims[1]);
     hipGetDevice(&h->device);
@@ -495,7 +496,7 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->seg_flag, h->nc / 8)) || (rc = dalloc(h, &h->seg_scan, h->nc / 8 + 1)) ||
         (rc = dalloc(h, &h->seg_list, h->nc / 8)) || (rc = dalloc(h, &h->tile_any, h->nc / 512)) ||
         (rc = dalloc(h, &h->tile_lists, 3 * (h->nc / 512))) ||
-        (rc = dalloc(h, &h->partials, 2 * (int64_t)kMaxPartials)) || (rc = dalloc(h, &h->counter, kCounterWords)) ||
+        (rc = dalloc(h, &h->partials, 3 * h->part_cap)) || (rc = dalloc(h, &h->counter, kCounterWords)) ||
         (rc = dalloc(h, &h->sc, 1)))
         return bail(rc);
     if (hipHostMalloc((void**)&h->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)

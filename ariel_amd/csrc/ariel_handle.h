@@ -100,7 +100,8 @@ struct ariel_sim {
     uint16_t* net_mem = nullptr;
     float* R_old = nullptr;   // tile-major
     bool net_ready = false;
-    double* partials = nullptr;
+    double* partials = nullptr;   // 3 * part_cap: two-value partials, then the SpMV's
+    int64_t part_cap = 0;
     uint32_t* counter = nullptr;
     SolverScalars* sc = nullptr;
     SolverScalars* sc_host = nullptr;   // pinned
@@ -137,7 +138,7 @@ struct ariel_sim {
         p.p2g_tile = p2g_tile;
         return p;
     }
-    Reduce reduce() const { return Reduce{partials, counter, sc}; }
+    Reduce reduce() const { return Reduce{partials, counter, sc, partials + 2 * part_cap}; }
 };
 
 namespace ariel {

@@ -105,10 +105,15 @@ struct Params {
 
 // Reduction workspace: per-block double partials + an arrival counter.
 struct Reduce {
-    double* partials;      // kMaxPartials * 2
+    double* partials;      // 2 * partial_capacity(N): one or two values per block
     uint32_t* counter;     // 33 arrival counters, 128 bytes apart (zero between reductions)
     SolverScalars* sc;     // device scalars
+    double* sig;           // partial_capacity(N): the SpMV's s.As per block, apart from
+                           // `partials` so the first MIC(0) pass can sum them while it
+                           // writes its own
 };
+// Blocks a reducing solver kernel may launch on an N^3 grid (one partial each).
+int64_t partial_capacity(int N);
 
 // Exhaustive division self-check (selftest.hip); lo, hi non-negative.
 hipError_t run_bf16_selftest(int64_t* mismatches);
@@ -342,9 +347,11 @@ void launch_ghost_levelset(hipStream_t s, int N, const uint8_t* A, const float* 
                            uint8_t* sb);
 // s = z (first iteration) or z + beta s_prev, y = A s, alpha = rho / s.y. St
 // holds two tile-major buffers selected by the iteration's parity.
+// fold_sigma: no finalize; the first MIC(0) pass with the AXPY sums the
+// partials itself (alpha = rho / s.As)
 void launch_spmv_dot(hipStream_t s, const Params& p, const float* G, const uint8_t* sb,
                      const uint8_t* fl8, const float* Zt, float* St, float* Yt, const Reduce& red,
-                     const Active& act);
+                     const Active& act, bool fold_sigma = false);
 // p += alpha s, r -= alpha y, ||r||^2 and the stop test (tile-major).
 void launch_axpy2(hipStream_t s, float* Pt, const float* St, float* Rt, const float* Yt, int64_t nc,
                   const Reduce& red, const Active& act);

@@ -114,6 +114,34 @@ __device__ void block_partial_comp(double v, double* part, int slot, int comp) {
 
 __device__ __forceinline__ bool finite(double x) { return isfinite(x); }
 
+// Sum of n one-value block partials by a block of 256 threads, in KFinalize's
+// order (thread t adds part[t], part[t + 256], ... in turn, then the waves' trees,
+// then the four waves in order); every thread gets it. Sixteen loads per thread
+// are issued before their adds (adding 0.0 past the end changes no sum: the
+// running sum starts at +0.0 and never becomes -0.0).
+__device__ double block_partial_sum(const double* __restrict__ part, int64_t n) {
+    __shared__ double sm[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double a = 0.0;
+    for (int64_t b0 = 0; b0 < n; b0 += 16 * 256) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int64_t b = b0 + threadIdx.x + k * 256;
+            v[k] = b < n ? part[b] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a += v[k];
+    }
+    a = wave_sum(a);
+    if (lane == 0) sm[w] = a;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < 4; ++i) t += sm[i];
+    __syncthreads();
+    return t;
+}
+
 // ------------------------------------------------------------ operator
 // ComputeAx + XRef (solver.inl:138-154, 196-230): y_c = (6 x_c - sum XRef_q) / h,
 // h = 1/(n*n); XRef = x_q (FLUID), x_c (SOLID or clamped out of range),
@@ -598,7 +626,7 @@ struct KTileSpmv {
             *reinterpret_cast<float4*>(a.Y + own) = make_float4(y[0], y[1], y[2], y[3]);
             *reinterpret_cast<float4*>(a.Y + own + 4) = make_float4(y[4], y[5], y[6], y[7]);
         }
-        block_partials<1>(dot, a.red.partials, (int)bl);
+        block_partials<1>(dot, a.red.sig, (int)bl);
     }
 };
 
@@ -883,6 +911,24 @@ constexpr double flex_need() {   // 0.95^(2 kFlexWindow), on ||r||^2
     return v;
 }
 
+// sigma = s.As: alpha = rho / sigma, or the breakdown statuses (3: not finite,
+// 2: sigma <= 0) that end the solve.
+__device__ __forceinline__ bool sigma_breakdown(const SolverScalars* sc, double sigma) {
+    return !finite(sigma) || !finite(sc->rho) || !(sigma > 0.0);
+}
+__device__ void set_alpha(SolverScalars* sc, double sigma) {
+    sc->sigma = sigma;
+    if (!finite(sigma) || !finite(sc->rho)) {
+        sc->status = 3;
+        sc->done = 1;
+    } else if (!(sigma > 0.0)) {
+        sc->status = 2;
+        sc->done = 1;
+    } else {
+        sc->alpha = sc->rho / sigma;
+    }
+}
+
 // One block: sum the partials of the preceding reducing kernel in a fixed order
 // and update the scalars. Gated exactly like its producer, so a skipped producer
 // (solve finished) is matched by a skipped finalize.
@@ -911,6 +957,12 @@ struct KFinalize {
         if (kind == FIN_RHO && sc->flex) return;          // flexible PCG sets beta itself
         if (kind == FIN_PR_CHECK && (!sc->flex || sc->use_mic)) return;
         if (nparts < 0) nparts = (int)parts_from_counts(nparts, sc);
+        if (kind == FIN_SIGMA) {
+            // shared with the first MIC(0) pass, which folds this finalize in
+            const double sg = block_partial_sum(red.partials, nparts);
+            if (threadIdx.x == 0) set_alpha(sc, sg);
+            return;
+        }
         const int NV = (kind == FIN_RESID0 || kind == FIN_PR || kind == FIN_PR_CHECK ||
                         kind == FIN_RHO || kind == FIN_RHO_RNORM) ? 2 : 1;
         double a0 = 0.0, a1 = 0.0;
@@ -944,18 +996,6 @@ struct KFinalize {
                 if (s1 <= sc->thr) {
                     sc->done = 1;
                     sc->status = 0;
-                }
-                break;
-            case FIN_SIGMA:
-                sc->sigma = s0;
-                if (!finite(s0) || !finite(sc->rho)) {
-                    sc->status = 3;
-                    sc->done = 1;
-                } else if (!(s0 > 0.0)) {
-                    sc->status = 2;
-                    sc->done = 1;
-                } else {
-                    sc->alpha = sc->rho / s0;
                 }
                 break;
             case FIN_RNORM:
@@ -1439,6 +1479,17 @@ struct KMicApply {
         float hzm = -0.0f, hzp = -0.0f;   // z-halo terms (-0.0 when absent)
         float rreg[kTile];
         double rr = 0.0;                  // ||r||^2 of the tile after the AXPY
+        // the first pass with the AXPY takes alpha from the SpMV's partials itself
+        // (no finalize launch between them): every block sums them in KFinalize's
+        // order, so all get the same sigma; logical block 0 records it, and on a
+        // breakdown every block returns
+        constexpr bool kFoldSigma = AXPY && PH == 0;
+        float af = 0.0f;
+        float pcv[kTile], sav[kTile], pv[kTile], sv[kTile], yv[kTile];
+        float hxv[kTile], hyv[kTile], vm = -0.0f, vp = -0.0f;
+        uint32_t hfx = 0, hfy = 0, fm = 0, fp = 0;
+        int xi = -1, yj = -1;
+        bool has_m = false, has_p = false;
         // coalesced mapping: round m covers line (m, lane>>3), element lane&7
         const int ce = lane & 7, cj = lane >> 3;
         if (lane < kTile) lds[o_zero + lane] = -0.0f;
@@ -1449,7 +1500,6 @@ struct KMicApply {
             // own line instead and discard it)
             const int64_t sgl = ((int64_t)gi * N + gj) * NS + tk;
             fl = fl8[sgl];
-            float pcv[kTile], sav[kTile], pv[kTile], sv[kTile], yv[kTile];
             const float* __restrict__ Sd = AXPY ? search_dir(kargs.S, red.sc, kargs.nc) : nullptr;
     #pragma unroll
             for (int m = 0; m < kTile; ++m) {
@@ -1463,10 +1513,6 @@ struct KMicApply {
                     yv[m] = kargs.Y[gt];
                 }
             }
-            float hxv[kTile], hyv[kTile], vm = -0.0f, vp = -0.0f;
-            uint32_t hfx = 0, hfy = 0, fm = 0, fp = 0;
-            int xi = -1, yj = -1;
-            bool has_m = false, has_p = false;
             if constexpr (halo) {
                 // neighbour lines across the tile faces: forward halos hold pc*Q of
                 // the red neighbours, backward halos their Z; -0.0 off-fluid
@@ -1520,8 +1566,18 @@ struct KMicApply {
                     vp = vp * bzp;
                 }
             }
+        }
+        if constexpr (kFoldSigma) {
+            // after the pass's loads are issued, by every thread of the block
+            const double sg = block_partial_sum(red.sig, mic_grid(tile_count(-1, red.sc, 2)));
+            if (bl == 0 && threadIdx.x == 0) set_alpha(red.sc, sg);
+            if (sigma_breakdown(red.sc, sg)) return;
+            af = (float)(red.sc->rho / sg);
+        } else if constexpr (AXPY) {
+            af = (float)red.sc->alpha;
+        }
+        if (active) {
             if constexpr (AXPY) {
-                const float af = (float)red.sc->alpha;
                 const float naf = -af;
     #pragma unroll
                 for (int m = 0; m < kTile; ++m) {
@@ -2016,10 +2072,19 @@ void launch_ghost_levelset(hipStream_t s, int N, const uint8_t* A, const float* 
 
 void launch_spmv_dot(hipStream_t s, const Params& p, const float* G, const uint8_t* sb,
                      const uint8_t* fl8, const float* Zt, float* St, float* Yt, const Reduce& red,
-                     const Active& act) {
+                     const Active& act, bool fold_sigma) {
     const int nb = (int)mic_grid(tiles_or_cap(act, 2));
     launch<KTileSpmv>(s, dim3((unsigned)xcd_grid(nb)), dim3(256), {p, G, sb, fl8, Zt, St, Yt, act.tiles[2], act.ntiles[2], red});
-    launch_finalize(s, red, FIN_SIGMA, act.ntiles[2] < 0 ? PARTS_MIC_ALL : nb);
+    if (fold_sigma) return;
+    Reduce rs = red;
+    rs.partials = red.sig;
+    launch_finalize(s, rs, FIN_SIGMA, act.ntiles[2] < 0 ? PARTS_MIC_ALL : nb);
+}
+
+int64_t partial_capacity(int N) {
+    const int64_t tn = N / kTile;
+    const int64_t nb = mic_grid(tn * tn * tn) + 1;   // red and black blocks round up apart
+    return nb > kMaxPartials ? nb : kMaxPartials;
 }
 
 void launch_axpy2(hipStream_t s, float* Pt, const float* St, float* Rt, const float* Yt, int64_t nc,

@@ -125,13 +125,15 @@ void enqueue_iteration(ariel_sim* h, int it, bool refine, bool learned, bool mic
     hipStream_t s = h->stream;
     const Active& act = h->act;
     const bool sampled = prof && prof_sampled(it);
+    // the tiled red-black MIC(0) carries the AXPY in its first pass over each
+    // colour, except where the reliable update must come between the two; that
+    // first pass also sums the SpMV's partials for alpha (no finalize launch)
+    const bool fused = !learned && !refine && p.precond == ARIEL_PRECOND_MIC0_TILED;
     {
         Prof pr(h, ARIEL_KERNEL_SPMV, sampled);
-        launch_spmv_dot(s, p, h->Gt, h->sb, h->fl8, h->Zt, h->St, h->Yt, red, act);
+        launch_spmv_dot(s, p, h->Gt, h->sb, h->fl8, h->Zt, h->St, h->Yt, red, act, fused);
     }
-    // the tiled red-black MIC(0) carries the AXPY in its first pass over each
-    // colour, except where the reliable update must come between the two
-    if (!learned && !refine && p.precond == ARIEL_PRECOND_MIC0_TILED) {
+    if (fused) {
         const IterAxpy ax{h->Pt, h->St, h->Yt, h->nc};
         Prof pr(h, ARIEL_KERNEL_PRECOND, sampled);
         launch_precond_apply(s, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, red, act, &ax);
