@@ -16,7 +16,7 @@ ARIEL_OK = 0
 AIR, FLUID, SOLID = 0, 1, 2
 LIQUID_BOX, LIQUID_SPHERE = 0, 1
 PRECOND_MIC0_TILED, PRECOND_MIC0, PRECOND_JACOBI, PRECOND_REFERENCE, PRECOND_MIC0_BLOCK, \
-    PRECOND_LEARNED = range(6)
+    PRECOND_LEARNED, PRECOND_MG = range(7)
 STOP_RELATIVE, STOP_REFERENCE_ABS = 0, 1
 SPLAT_SHARPEN, SPLAT_TRILINEAR = 0, 1
 LEVELSET_SPHERES, LEVELSET_TRAILS = 0, 1

@@ -62,6 +62,9 @@ double kernel_bytes(const ariel_sim* h, int k) {
         case ARIEL_KERNEL_P2G: return 24.0 * np + 12.0 * nc;
         case ARIEL_KERNEL_SPMV: return 17.0 * tile_cells;
         case ARIEL_KERNEL_PRECOND:
+            // multigrid: the two level-0 passes read r, the diagonal and the
+            // classes (9 B per cell) and the second writes z
+            if (h->d.precond == ARIEL_PRECOND_MG) return 22.0 * tile_cells;
             return (h->d.precond == ARIEL_PRECOND_MIC0_TILED ? 40.0 : 20.0) * tile_cells;
         case ARIEL_KERNEL_AXPY: return 24.0 * tile_cells;
         case ARIEL_KERNEL_GRADIENT: return 33.0 * nc;
@@ -111,10 +114,12 @@ int validate(const ariel_scene_desc* d, std::string& msg) {
         return ARIEL_ERR_UNSUPPORTED;
     }
     if (d->precond != ARIEL_PRECOND_MIC0_TILED && d->precond != ARIEL_PRECOND_MIC0_BLOCK &&
-        d->precond != ARIEL_PRECOND_JACOBI && d->precond != ARIEL_PRECOND_LEARNED) {
+        d->precond != ARIEL_PRECOND_JACOBI && d->precond != ARIEL_PRECOND_LEARNED &&
+        d->precond != ARIEL_PRECOND_MG) {
         msg = "unknown preconditioner"; return ARIEL_ERR_INVALID;
     }
-    if ((d->precond == ARIEL_PRECOND_MIC0_TILED || d->precond == ARIEL_PRECOND_MIC0_BLOCK) &&
+    if ((d->precond == ARIEL_PRECOND_MIC0_TILED || d->precond == ARIEL_PRECOND_MIC0_BLOCK ||
+         d->precond == ARIEL_PRECOND_MG) &&
         d->tile != 8) {
         msg = "tiled MIC(0) uses 8^3 tiles (tile must be 8)"; return ARIEL_ERR_UNSUPPORTED;
     }
@@ -499,6 +504,16 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->partials, 3 * h->part_cap)) || (rc = dalloc(h, &h->counter, kCounterWords)) ||
         (rc = dalloc(h, &h->sc, 1)))
         return bail(rc);
+    if (h->d.precond == ARIEL_PRECOND_MG) {
+        mg_plan(h->N, h->mg);
+        for (int l = 1; l <= h->mg.L; ++l) {
+            const int64_t n = (int64_t)h->mg.M[l] * h->mg.M[l] * h->mg.M[l];
+            if ((rc = dalloc(h, &h->mg.type[l], n)) || (rc = dalloc(h, &h->mg.d[l], n)) ||
+                (rc = dalloc(h, &h->mg.r[l], n)) || (rc = dalloc(h, &h->mg.z[l], n)) ||
+                (rc = dalloc(h, &h->mg.zo[l], n)))
+                return bail(rc);
+        }
+    }
     if (hipHostMalloc((void**)&h->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, ARIEL_ERR_NOMEM, "hipHostMalloc failed"));
     for (int i = 0; i < ARIEL_NUM_STAGES; ++i) {
@@ -552,6 +567,10 @@ int ariel_destroy(ariel_sim* h) {
         hipFree(h->u[c]);
         hipFree(h->du[c]);
     }
+    for (int l = 1; l < kMgMaxLevels; ++l)
+        for (void* p : {(void*)h->mg.type[l], (void*)h->mg.d[l], (void*)h->mg.r[l], (void*)h->mg.z[l],
+                        (void*)h->mg.zo[l]})
+            hipFree(p);
     if (h->sc_host) hipHostFree(h->sc_host);
     for (int i = 0; i < ARIEL_NUM_STAGES; ++i) {
         if (h->stages[i].t0) hipEventDestroy(h->stages[i].t0);
@@ -896,8 +915,8 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
     TRY(check_handle(h));
     if (!r || !z) return fail(h, ARIEL_ERR_INVALID, "null pointer");
     const Params p = h->params();
-    HIPCHK(h, hipMemcpyAsync(h->R, r, h->nc * sizeof(float), hipMemcpyHostToDevice, h->stream));
     TRY(prepare_solve(h));
+    HIPCHK(h, hipMemcpyAsync(h->R, r, h->nc * sizeof(float), hipMemcpyHostToDevice, h->stream));
     launch_fill_f32(h->stream, h->Z, h->nc, 0.0f);
     if (h->d.precond == ARIEL_PRECOND_LEARNED) {
         if (!h->net_ready) return fail(h, ARIEL_ERR_STATE, "learned preconditioner: no weights");
@@ -915,7 +934,7 @@ int ariel_apply_preconditioner(ariel_sim* h, const float* r, float* z) {
         // the apply works on the tile-major vectors over the active tiles; cells
         // outside them are not FLUID and keep z = 0
         launch_tile_convert(h->stream, h->N, h->R, h->Rt, h->act, h->sc, true, false);
-        launch_precond_apply(h->stream, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, h->reduce(), h->act);
+        precond_apply(h, h->reduce());
         launch_tile_convert(h->stream, h->N, h->Zt, h->Z, h->act, h->sc, false, false);
     }
     TRY(check_launch(h, "preconditioner"));

@@ -100,6 +100,7 @@ struct ariel_sim {
     uint16_t* net_mem = nullptr;
     float* R_old = nullptr;   // tile-major
     bool net_ready = false;
+    MgLevels mg;                  // coarse levels of ARIEL_PRECOND_MG
     double* partials = nullptr;   // 3 * part_cap: two-value partials, then the SpMV's
     int64_t part_cap = 0;
     uint32_t* counter = nullptr;
@@ -233,6 +234,9 @@ int prepare_solve(ariel_sim* h);
 void enqueue_solve_setup(ariel_sim* h, bool learned);
 void enqueue_iteration(ariel_sim* h, int it, bool refine, bool learned, bool mic_only, bool prof);
 void enqueue_solve_finish(ariel_sim* h);
+// z = M^-1 r on the tile-major vectors with z.r and beta (every preconditioner
+// but the learned one)
+void precond_apply(ariel_sim* h, const Reduce& red);
 
 }  // namespace host
 }  // namespace ariel

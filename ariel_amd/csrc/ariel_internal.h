@@ -115,6 +115,28 @@ struct Reduce {
 // Blocks a reducing solver kernel may launch on an N^3 grid (one partial each).
 int64_t partial_capacity(int N);
 
+// Coarse levels of the multigrid preconditioner (ARIEL_PRECOND_MG): level l >= 1
+// is a dense grid of edge M[l] = ceil(M[l-1] / 2), M[0] = N, down to the
+// coarsest, L, of edge <= 8 (L = 0: no coarse level).
+constexpr int kMgMaxLevels = 12;
+struct MgLevels {
+    int L = 0;
+    int M[kMgMaxLevels] = {};
+    uint8_t* type[kMgMaxLevels] = {};
+    float* d[kMgMaxLevels] = {};
+    float* r[kMgMaxLevels] = {};
+    float* z[kMgMaxLevels] = {};
+    float* zo[kMgMaxLevels] = {};
+};
+// edges of the levels for grid edge N (fills L and M)
+void mg_plan(int N, MgLevels& mg);
+// coarse cell classes and diagonals from the solve's classes A (once per solve)
+void launch_mg_build(hipStream_t s, int N, const uint8_t* A, const MgLevels& mg);
+// z = B r (one V-cycle) on the active tiles, tile-major r and z; z.r into the
+// partials and FIN_RHO (beta)
+void launch_mg_apply(hipStream_t s, int N, const uint8_t* A, const float* diag, const MgLevels& mg,
+                     const float* Rt, float* Zt, const Reduce& red, const Active& act);
+
 // Exhaustive division self-check (selftest.hip); lo, hi non-negative.
 hipError_t run_bf16_selftest(int64_t* mismatches);
 hipError_t run_division_selftest(int which, float a, float lo, float hi, int64_t* mismatches);

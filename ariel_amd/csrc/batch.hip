@@ -319,9 +319,9 @@ extern "C" {
 int ariel_batch_create(const ariel_scene_desc* descs, int32_t ntanks, ariel_batch** out) {
     if (!descs || !out || ntanks <= 0) return bfail(nullptr, ARIEL_ERR_INVALID, "batch_create: bad arguments");
     *out = nullptr;
-    if (descs[0].precond == ARIEL_PRECOND_LEARNED)
+    if (descs[0].precond == ARIEL_PRECOND_LEARNED || descs[0].precond == ARIEL_PRECOND_MG)
         return bfail(nullptr, ARIEL_ERR_UNSUPPORTED,
-                     "batch_create: the learned preconditioner runs on single tanks only");
+                     "batch_create: the learned and multigrid preconditioners run on single tanks only");
     for (int32_t t = 1; t < ntanks; ++t) {
         const ariel_scene_desc& a = descs[0];
         const ariel_scene_desc& b = descs[t];

@@ -886,11 +886,12 @@ __device__ __forceinline__ void finish_rho(SolverScalars* sc, double rho_new) {
 // ------------------------------------------------------------ finalize
 // Partial counts that depend on the active sets, for finalize launches recorded
 // before the counts exist (nparts < 0 selects the producer's grid rule).
-enum { PARTS_LIST = -1, PARTS_TILE_LINES = -2, PARTS_MIC_RB = -3, PARTS_MIC_ALL = -4 };
+enum { PARTS_LIST = -1, PARTS_TILE_LINES = -2, PARTS_MIC_RB = -3, PARTS_MIC_ALL = -4, PARTS_TILE_CELLS = -5 };
 __device__ __forceinline__ int64_t parts_from_counts(int code, const SolverScalars* sc) {
     switch (code) {
         case PARTS_LIST: return list_grid(sc->nseg);
         case PARTS_TILE_LINES: return tile_lines_grid(sc->ntile[2]);
+        case PARTS_TILE_CELLS: return list_grid(512 * (int64_t)sc->ntile[2]);
         case PARTS_MIC_RB: return mic_grid(sc->ntile[1]) + mic_grid(sc->ntile[0]);
         default: return mic_grid(sc->ntile[2]);
     }
@@ -1967,6 +1968,384 @@ int stream_blocks(int64_t n) {
     return (int)(b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks);
 }
 
+
+// ------------------------------------------------------------ multigrid
+// A multigrid V(1,1) cycle as the preconditioner of PCG (ARIEL_PRECOND_MG; no
+// reference counterpart: the reference's preconditioner is an incomplete
+// factorisation, solver.inl:104-135). Level 0 is the solve's grid on its active
+// tiles (tile-major r and z, the exact operator with its ghost-fluid diagonal);
+// coarser levels are dense grids of edge M_l = ceil(M_{l-1} / 2) down to an
+// edge of at most 8. A coarse cell is FLUID when any of its 2x2x2 children is,
+// else AIR when any child is AIR, else SOLID; its operator is the 7-point
+// Laplacian on spacing 2^l / N with Dirichlet AIR and Neumann SOLID (and
+// out-of-range) neighbours. Transfers are piecewise constant: a fine cell takes
+// its parent's value, and a coarse cell's right-hand side is the mean of its
+// children's residuals (restriction = prolongation transposed / 8). Each level
+// smooths once before and once after its coarse correction by damped Jacobi;
+// the coarsest level runs symmetric red-black Gauss-Seidel sweeps in one
+// workgroup. Jacobi is symmetric and the restriction is the prolongation's
+// transpose, so the cycle is a symmetric positive definite operator and plain
+// PCG applies. (Trilinear transfers took as many PCG iterations at 8 times the
+// memory traffic.)
+//
+// With z1 = w r / d (Jacobi from zero, d the diagonal scaled by 1/h2), the
+// residual r - A z1 = (1 - w_c) r_c + sum_{FLUID q} w_q r_q / d_q needs no
+// stored z1; pre-smoothing and restriction are one pass, one work item per
+// coarse cell over its 8 children. On level 0 the Jacobi weight is 1 where the
+// ghost-fluid terms raise the diagonal above 6 (free-surface cells, diagonals
+// up to ~10^3): there a full local solve keeps the coarse correction out of the
+// stiff cell, which otherwise makes A z large there and the FP32 recursive
+// residual drift from the true one. Both weights keep 2 M - A diagonally
+// dominant, so the cycle stays positive definite.
+constexpr float kMgOmega = 6.0f / 7.0f;
+constexpr int kMgCoarsestSweeps = 6;
+__device__ __forceinline__ float mg_omega0(float dg) { return dg > 6.0f ? 1.0f : kMgOmega; }
+
+struct MgLevelArgs {
+    int M;                  // edge of the level's grid
+    float h2;               // squared spacing, 4^l / N^2
+    const uint8_t* type;    // cell classes
+    const float* d;         // diagonal count (6 - Neumann neighbours) on FLUID cells
+    float* r;               // right-hand side (restricted residual)
+    float* z;               // pre-smoothed z (FLUID cells)
+    float* zo;              // z after the coarse correction and post-smoothing
+};
+
+__device__ __forceinline__ void mg_decode(int M, int64_t c, int& i, int& j, int& k) {
+    k = (int)(c % M);
+    const int64_t q = c / M;
+    j = (int)(q % M);
+    i = (int)(q / M);
+}
+
+// cell classes of level l from its finer level (A of the solve for l = 1)
+struct KMgTypes {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const uint8_t* fine;
+        int Mf;
+        uint8_t* coarse;
+        int Mc;
+    };
+    static __device__ void run(const Args& a) {
+        const int64_t n = (int64_t)a.Mc * a.Mc * a.Mc;
+        const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= n) return;
+        int I, J, K;
+        mg_decode(a.Mc, c, I, J, K);
+        bool fluid = false, air = false;
+        for (int u = 0; u < 8; ++u) {
+            const int x = 2 * I + (u >> 2), y = 2 * J + ((u >> 1) & 1), z = 2 * K + (u & 1);
+            if (x >= a.Mf || y >= a.Mf || z >= a.Mf) continue;
+            const uint8_t t = a.fine[((int64_t)x * a.Mf + y) * a.Mf + z];
+            fluid |= t == kFluid;
+            air |= t == kAir;
+        }
+        a.coarse[c] = fluid ? (uint8_t)kFluid : (air ? (uint8_t)kAir : (uint8_t)kSolid);
+    }
+};
+
+// diagonal counts of a coarse level: 6 less the SOLID or out-of-range neighbours
+struct KMgDiag {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const uint8_t* type;
+        int M;
+        float* d;
+    };
+    static __device__ void run(const Args& a) {
+        const int M = a.M;
+        const int64_t n = (int64_t)M * M * M;
+        const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= n) return;
+        int i, j, k;
+        mg_decode(M, c, i, j, k);
+        float d = 6.0f;
+        const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+        for (int m = 0; m < 6; ++m) {
+            const int x = q[m][0], y = q[m][1], z = q[m][2];
+            const bool in = x >= 0 && x < M && y >= 0 && y < M && z >= 0 && z < M;
+            if (!in || a.type[((int64_t)x * M + y) * M + z] == kSolid) d -= 1.0f;
+        }
+        a.d[c] = a.type[c] == kFluid ? d : 1.0f;
+    }
+};
+
+// Sum over the 8 consecutive lanes that hold a coarse cell's children (lanes
+// 8m .. 8m+7), in a fixed order (pairs, then quads, then the halves); every lane
+// of the group gets it.
+__device__ __forceinline__ float mg_group8_sum(float v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    return v;
+}
+
+// Level 0, pre-smoothing and restriction: work item e is child e % 8 of coarse
+// cell (e / 8) % 64 of the list's tile e / 512 (a 4^3 block of level 1 over
+// the 8^3 fine tile), so a coarse cell's 8 children sit in 8 consecutive lanes;
+// the mean over its FLUID children of res = r - A (w_c r / d) is its right-hand
+// side. Coarse cells off the active tiles have no FLUID child and are not
+// written (every reader of r checks the class).
+struct KMgSmoothRestrict0 {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int N;
+        const uint8_t* A;
+        const float* diag;
+        const float* R;     // tile-major r
+        float* rc;          // level 1 right-hand side, dense
+        int Mc;
+        const int32_t* tiles;
+        int64_t ntiles;
+        const SolverScalars* sc;
+    };
+    static __device__ void run(const Args& a) {
+        if (a.sc->done) return;
+        const int64_t ntl = tile_count(a.ntiles, a.sc, 2);
+        const int64_t nb = list_grid(512 * ntl);
+        if ((int64_t)blockIdx.x >= nb) return;
+        const int N = a.N, TN = N / kTile, Mc = a.Mc;
+        // every lane runs the same number of rounds (the shuffles need the full
+        // wave); work items past the end contribute nothing
+        const int64_t stride = nb * blockDim.x;
+        const int64_t rounds = (512 * ntl + stride - 1) / stride;
+        for (int64_t rd = 0; rd < rounds; ++rd) {
+            const int64_t e = rd * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            const bool live = e < 512 * ntl;
+            const int t = a.tiles[live ? (e >> 9) : 0];
+            const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
+            const int lc = (int)((e >> 3) & 63), u = (int)(e & 7);
+            const int I = ti * 4 + (lc >> 4), J = tj * 4 + ((lc >> 2) & 3), K = tk * 4 + (lc & 3);
+            const int i = 2 * I + (u >> 2), j = 2 * J + ((u >> 1) & 1), k = 2 * K + (u & 1);
+            float res = 0.0f;
+            const int64_t c = cidx(N, i, j, k);
+            if (live && a.A[c] == kFluid) {
+                float sum = 0.0f;
+                const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    const int x = q[m][0], y = q[m][1], z = q[m][2];
+                    if (x < 0 || x >= N || y < 0 || y >= N || z < 0 || z >= N) continue;
+                    const int64_t cq = cidx(N, x, y, z);
+                    if (a.A[cq] != kFluid) continue;
+                    const float dq = a.diag[cq];
+                    sum += mg_omega0(dq) * a.R[tm_cell(N, x, y, z)] / dq;
+                }
+                res = (1.0f - mg_omega0(a.diag[c])) * a.R[tm_cell(N, i, j, k)] + sum;
+            }
+            const float acc = mg_group8_sum(res);
+            if (live && u == 0) a.rc[((int64_t)I * Mc + J) * Mc + K] = acc * 0.125f;
+        }
+    }
+};
+
+// Levels >= 1, pre-smoothing and restriction to level l+1: work item e is child
+// e % 8 of coarse cell e / 8 (8 consecutive lanes); FLUID children get
+// z = w h2 r / d, and the mean of their residuals r - A z is the coarse cell's
+// right-hand side (0 on non-FLUID coarse cells).
+struct KMgSmoothRestrict {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        MgLevelArgs f;
+        MgLevelArgs c;
+        const SolverScalars* sc;
+    };
+    static __device__ void run(const Args& a) {
+        if (a.sc->done) return;
+        const int Mc = a.c.M, M = a.f.M;
+        const int64_t n = 8 * (int64_t)Mc * Mc * Mc;
+        const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const bool live = e < n;
+        const int64_t cc = live ? e >> 3 : 0;
+        const int u = (int)(e & 7);
+        int I, J, K;
+        mg_decode(Mc, cc, I, J, K);
+        const int i = 2 * I + (u >> 2), j = 2 * J + ((u >> 1) & 1), k = 2 * K + (u & 1);
+        float res = 0.0f;
+        if (live && i < M && j < M && k < M) {
+            const int64_t c = ((int64_t)i * M + j) * M + k;
+            if (a.f.type[c] == kFluid) {
+                const float r = a.f.r[c];
+                float sum = 0.0f;
+                const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    const int x = q[m][0], y = q[m][1], z = q[m][2];
+                    if (x < 0 || x >= M || y < 0 || y >= M || z < 0 || z >= M) continue;
+                    const int64_t cq = ((int64_t)x * M + y) * M + z;
+                    if (a.f.type[cq] != kFluid) continue;
+                    sum += a.f.r[cq] / a.f.d[cq];
+                }
+                a.f.z[c] = kMgOmega * a.f.h2 * r / a.f.d[c];
+                res = (1.0f - kMgOmega) * r + kMgOmega * sum;
+            }
+        }
+        const float acc = mg_group8_sum(res);
+        if (live && u == 0) a.c.r[cc] = a.c.type[cc] == kFluid ? acc * 0.125f : 0.0f;
+    }
+};
+
+// The coarse correction P e at fine cell (x, y, z): its parent's value in the
+// coarse level's final z (dense, edge Mc; 0 where the parent is not FLUID).
+__device__ __forceinline__ float mg_prolong(const float* __restrict__ e, int Mc, int x, int y, int z) {
+    return Mc > 0 ? e[((int64_t)(x >> 1) * Mc + (y >> 1)) * Mc + (z >> 1)] : 0.0f;
+}
+
+// Levels >= 1, coarse correction and post-smoothing in one pass: zt = z + P e
+// (recomputed for the FLUID neighbours), zo = zt + w (h2 r + sum zt_q - d zt) / d,
+// as (1 - w) zt + w (h2 r + sum zt_q) / d.
+struct KMgProlongPost {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        MgLevelArgs l;
+        const float* e;     // coarse level's final z
+        int Mc;
+        const SolverScalars* sc;
+    };
+    static __device__ void run(const Args& a) {
+        if (a.sc->done) return;
+        const int M = a.l.M;
+        const int64_t n = (int64_t)M * M * M;
+        const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= n) return;
+        if (a.l.type[c] != kFluid) {
+            a.l.zo[c] = 0.0f;
+            return;
+        }
+        int i, j, k;
+        mg_decode(M, c, i, j, k);
+        const float ztc = a.l.z[c] + mg_prolong(a.e, a.Mc, i, j, k);
+        float sum = 0.0f;
+        const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const int x = q[m][0], y = q[m][1], z = q[m][2];
+            if (x < 0 || x >= M || y < 0 || y >= M || z < 0 || z >= M) continue;
+            const int64_t cq = ((int64_t)x * M + y) * M + z;
+            if (a.l.type[cq] != kFluid) continue;
+            sum += a.l.z[cq] + mg_prolong(a.e, a.Mc, x, y, z);
+        }
+        const float d = a.l.d[c];
+        a.l.zo[c] = (1.0f - kMgOmega) * ztc + kMgOmega * ((a.l.h2 * a.l.r[c] + sum) / d);
+    }
+};
+
+// Coarsest level (M^3 <= 512 cells): z from r by kMgCoarsestSweeps symmetric
+// red-black Gauss-Seidel sweeps (red, then black and red again per sweep, from
+// z = 0), in LDS, one thread per cell.
+struct KMgCoarsest {
+    static constexpr int kThreads = 512;
+    struct Args {
+        MgLevelArgs l;
+        const SolverScalars* sc;
+    };
+    static __device__ void run(const Args& a) {
+        if (a.sc->done) return;
+        __shared__ float z[512];
+        __shared__ uint8_t ty[512];
+        const int M = a.l.M;
+        const int n = M * M * M;
+        const int c = threadIdx.x;
+        int i = 0, j = 0, k = 0;
+        float rc = 0.0f, d = 1.0f;
+        if (c < n) {
+            mg_decode(M, c, i, j, k);
+            ty[c] = a.l.type[c];
+            rc = a.l.r[c];
+            d = a.l.d[c];
+            z[c] = 0.0f;
+        }
+        __syncthreads();
+        const bool mine = c < n && ty[c] == kFluid;
+        const int colour = (i + j + k) & 1;
+        auto half = [&](int col) {
+            float v = 0.0f;
+            if (mine && colour == col) {
+                float sum = 0.0f;
+                const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+                for (int m = 0; m < 6; ++m) {
+                    const int x = q[m][0], y = q[m][1], zz = q[m][2];
+                    if (x < 0 || x >= M || y < 0 || y >= M || zz < 0 || zz >= M) continue;
+                    const int cq = (x * M + y) * M + zz;
+                    if (ty[cq] == kFluid) sum += z[cq];
+                }
+                v = (a.l.h2 * rc + sum) / d;
+            }
+            __syncthreads();
+            if (mine && colour == col) z[c] = v;
+            __syncthreads();
+        };
+        half(0);
+        for (int s = 0; s < kMgCoarsestSweeps; ++s) {
+            half(1);
+            half(0);
+        }
+        if (c < n) a.l.zo[c] = mine ? z[c] : 0.0f;
+    }
+};
+
+// Level 0, coarse correction and post-smoothing into the tile-major z of PCG (0
+// off FLUID): zt = w_c h2 r / d + P e (recomputed for the FLUID neighbours),
+// z = zt + w_c (h2 r + sum_{FLUID q} zt_q - d zt) / d, as (1 - w_c) zt +
+// w_c (h2 r + sum zt_q) / d; z.r into component 1 of the block partials
+// (two-value layout; component 0 is 0) for FIN_RHO. One cell per work item
+// (cell e % 512 of the list's tile e / 512).
+struct KMgPost0 {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int N;
+        float h2;
+        const uint8_t* A;
+        const float* diag;
+        const float* R;
+        const float* e;     // level 1's final z (null with no coarse level)
+        int Mc;
+        float* Z;
+        const int32_t* tiles;
+        int64_t ntiles;
+        Reduce red;
+    };
+    static __device__ void run(const Args& a) {
+        if (a.red.sc->done) return;
+        const int64_t ntl = tile_count(a.ntiles, a.red.sc, 2);
+        const int64_t nb = list_grid(512 * ntl);
+        if ((int64_t)blockIdx.x >= nb) return;
+        const int N = a.N, TN = N / kTile;
+        const float h2 = a.h2;
+        double v[2] = {0.0, 0.0};
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 512 * ntl; e += nb * blockDim.x) {
+            const int t = a.tiles[e >> 9];
+            const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
+            const int li = (int)(e & 511);
+            const int i = ti * kTile + (li >> 6), j = tj * kTile + ((li >> 3) & 7), k = tk * kTile + (li & 7);
+            const int64_t o = (int64_t)t * 512 + li;
+            const int64_t c = cidx(N, i, j, k);
+            float out = 0.0f;
+            if (a.A[c] == kFluid) {
+                const float d = a.diag[c];
+                const float w = mg_omega0(d);
+                const float rc = a.R[o];
+                const float ztc = w * h2 * rc / d + mg_prolong(a.e, a.Mc, i, j, k);
+                float sum = 0.0f;
+                const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    const int x = q[m][0], y = q[m][1], z = q[m][2];
+                    if (x < 0 || x >= N || y < 0 || y >= N || z < 0 || z >= N) continue;
+                    const int64_t cq = cidx(N, x, y, z);
+                    if (a.A[cq] != kFluid) continue;
+                    const float dq = a.diag[cq];
+                    sum += mg_omega0(dq) * h2 * a.R[tm_cell(N, x, y, z)] / dq + mg_prolong(a.e, a.Mc, x, y, z);
+                }
+                out = (1.0f - w) * ztc + w * ((h2 * rc + sum) / d);
+                v[1] += (double)out * (double)rc;
+            }
+            a.Z[o] = out;
+        }
+        block_partials<2>(v, a.red.partials, blockIdx.x);
+    }
+};
 }  // namespace
 
 int reduce_blocks(int64_t n, int) { return stream_blocks(n); }
@@ -2147,6 +2526,62 @@ void launch_spmv_plain(hipStream_t s, const Params& p, const uint8_t* A, const f
                        const float* x, float* y) {
     const int64_t nc = (int64_t)p.N * p.N * p.N;
     launch<KSpmvPlain>(s, dim3(grid_for(nc, kBlock)), dim3(kBlock), {p, A, L, x, y});
+}
+
+
+void mg_plan(int N, MgLevels& mg) {
+    mg.M[0] = N;
+    int l = 0;
+    while (mg.M[l] > 8 && l + 1 < kMgMaxLevels) {
+        mg.M[l + 1] = (mg.M[l] + 1) / 2;
+        ++l;
+    }
+    mg.L = l;
+}
+
+static MgLevelArgs mg_level(const MgLevels& mg, int N, int l) {
+    MgLevelArgs a{};
+    a.M = mg.M[l];
+    a.h2 = (float)((double)(1u << (2 * l)) / ((double)N * (double)N));
+    a.type = mg.type[l];
+    a.d = mg.d[l];
+    a.r = mg.r[l];
+    a.z = mg.z[l];
+    a.zo = mg.zo[l];
+    return a;
+}
+
+static unsigned mg_cells_grid(int M) { return grid_for((int64_t)M * M * M, kBlock); }
+
+void launch_mg_build(hipStream_t s, int N, const uint8_t* A, const MgLevels& mg) {
+    for (int l = 1; l <= mg.L; ++l) {
+        launch<KMgTypes>(s, dim3(mg_cells_grid(mg.M[l])), dim3(kBlock),
+                         {l == 1 ? A : mg.type[l - 1], mg.M[l - 1], mg.type[l], mg.M[l]});
+        launch<KMgDiag>(s, dim3(mg_cells_grid(mg.M[l])), dim3(kBlock), {mg.type[l], mg.M[l], mg.d[l]});
+    }
+}
+
+void launch_mg_apply(hipStream_t s, int N, const uint8_t* A, const float* diag, const MgLevels& mg,
+                     const float* Rt, float* Zt, const Reduce& red, const Active& act) {
+    const SolverScalars* sc = red.sc;
+    const int64_t ntl = tiles_or_cap(act, 2);
+    const float h2 = (float)(1.0 / ((double)N * (double)N));
+    if (mg.L > 0) {
+        launch<KMgSmoothRestrict0>(s, dim3((unsigned)list_grid(512 * ntl)), dim3(kBlock),
+                                   {N, A, diag, Rt, mg.r[1], mg.M[1], act.tiles[2], act.ntiles[2], sc});
+        for (int l = 1; l < mg.L; ++l)
+            launch<KMgSmoothRestrict>(s, dim3(grid_for(8 * (int64_t)mg.M[l + 1] * mg.M[l + 1] * mg.M[l + 1], kBlock)),
+                                      dim3(kBlock), {mg_level(mg, N, l), mg_level(mg, N, l + 1), sc});
+        launch<KMgCoarsest>(s, dim3(1), dim3(512), {mg_level(mg, N, mg.L), sc});
+        for (int l = mg.L - 1; l >= 1; --l)
+            launch<KMgProlongPost>(s, dim3(mg_cells_grid(mg.M[l])), dim3(kBlock),
+                                   {mg_level(mg, N, l), mg.zo[l + 1], mg.M[l + 1], sc});
+    }
+    const unsigned nb0 = (unsigned)list_grid(512 * ntl);
+    launch<KMgPost0>(s, dim3(nb0), dim3(kBlock),
+                     {N, h2, A, diag, Rt, mg.L > 0 ? mg.zo[1] : nullptr, mg.L > 0 ? mg.M[1] : 0, Zt,
+                      act.tiles[2], act.ntiles[2], red});
+    launch_finalize(s, red, FIN_RHO, act.ntiles[2] < 0 ? PARTS_TILE_CELLS : (int)nb0);
 }
 
 }  // namespace ariel

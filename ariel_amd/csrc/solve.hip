@@ -69,21 +69,39 @@ void set_active(ariel_sim* h, bool device_counts) {
 // bits, the active segment and tile lists (one host round trip for their
 // counts), then the factorisation over the active tiles. Leaves the solver
 // scalars initialised.
+// Iterations between reliable updates during a solve. Multigrid solves take
+// none: they converge in a few tens of iterations, and a residual replaced near
+// convergence (by the true one, which has drifted from the recursive one by the
+// FP32 rounding of A s at free-surface cells) stalls them; their iterate still
+// accumulates in FP64 when refinement is on, and the end of the solve flushes it.
+int refine_interval(const ariel_sim* h) {
+    return h->d.precond == ARIEL_PRECOND_MG ? 0 : h->d.refine_every;
+}
+
 int prepare_solve(ariel_sim* h) {
     const Params p = h->params();
     hipStream_t s = h->stream;
     launch_precond_setup(s, p, h->A, h->L, h->diag, h->nup, h->pc);
     launch_fluid_segments(s, h->N, h->A, h->fl8);
     launch_ghost_levelset(s, h->N, h->A, h->L, h->Gt, h->sb);
-    launch_solver_init(s, h->sc, h->d.max_iters, h->d.refine_every,
+    launch_solver_init(s, h->sc, h->d.max_iters, refine_interval(h),
                        h->d.precond == ARIEL_PRECOND_LEARNED ? 1 : 0);
     launch_active_sets(s, h->N, h->fl8, h->seg_flag, h->seg_scan, h->seg_list, h->tile_any,
                        h->tile_lists, h->scan_sums, h->sc);
     TRY(check_launch(h, "active sets"));
     TRY(read_scalars(h));
     set_active(h, false);
-    launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc, h->act, h->sc);
+    if (h->d.precond == ARIEL_PRECOND_MG) launch_mg_build(s, h->N, h->A, h->mg);
+    else launch_precond_build(s, p, h->A, h->diag, h->nup, h->pc, h->act, h->sc);
     return check_launch(h, "preconditioner build");
+}
+
+void precond_apply(ariel_sim* h, const Reduce& red) {
+    if (h->d.precond == ARIEL_PRECOND_MG) {
+        launch_mg_apply(h->stream, h->N, h->A, h->diag, h->mg, h->Rt, h->Zt, red, h->act);
+        return;
+    }
+    launch_precond_apply(h->stream, h->params(), h->fl8, h->pc, h->Rt, h->Q, h->Zt, red, h->act);
 }
 
 void learned_precond(ariel_sim* h, const Params& p, const Reduce& red, bool mic_only);
@@ -104,7 +122,7 @@ void enqueue_solve_setup(ariel_sim* h, bool learned) {
                      h->P64, refine, h->Pt);
     if (refine) launch_fill_f32(s, h->Pt, h->nc, 0.0f);
     if (!learned) {
-        launch_precond_apply(s, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, red, act);
+        precond_apply(h, red);
         return;
     }
     // the network reads r and writes z in the natural layout; both are zero
@@ -148,7 +166,7 @@ void enqueue_iteration(ariel_sim* h, int it, bool refine, bool learned, bool mic
     {
         Prof pr(h, ARIEL_KERNEL_PRECOND, sampled);
         if (learned) learned_precond(h, p, red, mic_only);
-        else launch_precond_apply(s, p, h->fl8, h->pc, h->Rt, h->Q, h->Zt, red, act);
+        else precond_apply(h, red);
     }
 }
 
@@ -180,7 +198,7 @@ int stage_solve(ariel_sim* h) {
     TRY(check_launch(h, "solve setup"));
     bool on_mic = false;
     int queued = 0;
-    const int k = h->d.refine_every;
+    const int k = refine_interval(h);
     ChunkPlan plan(h->last_iters);
     int chunk = plan.first();
     for (;;) {

@@ -553,7 +553,7 @@ def main():
             "workload": f"pool depth 0.3 + falling cube side 0.25, {args.n}^3, 8 ppc jittered, seed 1",
             "grid": args.n, "particles": int(np_), "dt": float(desc.dt),
             "preconditioner": ["mic0_tiled_rb", "mic0", "jacobi", "reference", "mic0_block",
-                               "learned"][desc.precond],
+                               "learned", "multigrid"][desc.precond],
             "tol": float(desc.tol), "refine_every": int(desc.refine_every),
             "parallelism": "single GPU",
         },

@@ -57,6 +57,8 @@ extern "C" {
 #define ARIEL_PRECOND_REFERENCE    3  /* solver.inl:104-135,232-286 verbatim   */
 #define ARIEL_PRECOND_MIC0_BLOCK   4  /* MIC(0) inside tiles, no coupling      */
 #define ARIEL_PRECOND_LEARNED      5  /* learned conv net inside flexible PCG  */
+#define ARIEL_PRECOND_MG           6  /* multigrid V(1,1) cycle (no reference
+                                         counterpart; single tanks)            */
 
 /* ---- PCG stop rules ----------------------------------------------------- */
 #define ARIEL_STOP_RELATIVE      0   /* ||r||_2 <= tol * ||b||_2 (FP64 norms) */
