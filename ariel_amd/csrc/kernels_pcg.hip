@@ -886,12 +886,12 @@ __device__ __forceinline__ void finish_rho(SolverScalars* sc, double rho_new) {
 // ------------------------------------------------------------ finalize
 // Partial counts that depend on the active sets, for finalize launches recorded
 // before the counts exist (nparts < 0 selects the producer's grid rule).
-enum { PARTS_LIST = -1, PARTS_TILE_LINES = -2, PARTS_MIC_RB = -3, PARTS_MIC_ALL = -4, PARTS_TILE_CELLS = -5 };
+enum { PARTS_LIST = -1, PARTS_TILE_LINES = -2, PARTS_MIC_RB = -3, PARTS_MIC_ALL = -4, PARTS_TILES = -5 };
 __device__ __forceinline__ int64_t parts_from_counts(int code, const SolverScalars* sc) {
     switch (code) {
         case PARTS_LIST: return list_grid(sc->nseg);
         case PARTS_TILE_LINES: return tile_lines_grid(sc->ntile[2]);
-        case PARTS_TILE_CELLS: return list_grid(512 * (int64_t)sc->ntile[2]);
+        case PARTS_TILES: return sc->ntile[2] > 0 ? sc->ntile[2] : 1;
         case PARTS_MIC_RB: return mic_grid(sc->ntile[1]) + mic_grid(sc->ntile[0]);
         default: return mic_grid(sc->ntile[2]);
     }
@@ -2081,14 +2081,45 @@ __device__ __forceinline__ float mg_group8_sum(float v) {
     return v;
 }
 
-// Level 0, pre-smoothing and restriction: work item e is child e % 8 of coarse
-// cell (e / 8) % 64 of the list's tile e / 512 (a 4^3 block of level 1 over
-// the 8^3 fine tile), so a coarse cell's 8 children sit in 8 consecutive lanes;
-// the mean over its FLUID children of res = r - A (w_c r / d) is its right-hand
-// side. Coarse cells off the active tiles have no FLUID child and are not
-// written (every reader of r checks the class).
+// Level 0 works one 8^3 tile per block of 512 threads: each thread computes a
+// value of its own cell, threads 0..383 the same value of the cells across the
+// tile's 6 faces (64 each: +x, -x, +y, -y, +z, -z, in-plane coordinates in
+// (y, z), (y, z), (x, z), (x, z), (x, y), (x, y) order), all into LDS, and then
+// each cell reads its 6 neighbours' values from LDS.
+constexpr int kMgTileThreads = 512;
+__device__ __forceinline__ void mg_halo_cell(int h, int ti, int tj, int tk, int& i, int& j, int& k) {
+    const int f = h >> 6, a = (h >> 3) & 7, b = h & 7;
+    const int x0 = ti * kTile, y0 = tj * kTile, z0 = tk * kTile;
+    switch (f) {
+        case 0: i = x0 + kTile; j = y0 + a; k = z0 + b; break;
+        case 1: i = x0 - 1; j = y0 + a; k = z0 + b; break;
+        case 2: i = x0 + a; j = y0 + kTile; k = z0 + b; break;
+        case 3: i = x0 + a; j = y0 - 1; k = z0 + b; break;
+        case 4: i = x0 + a; j = y0 + b; k = z0 + kTile; break;
+        default: i = x0 + a; j = y0 + b; k = z0 - 1; break;
+    }
+}
+// sum of the 6 neighbours' values of in-tile cell (il, jl, kl): own-tile values
+// v[il*64 + jl*8 + kl], face values hv[face*64 + a*8 + b] (0 outside the grid)
+__device__ __forceinline__ float mg_nbr_sum(const float* v, const float* hv, int il, int jl, int kl) {
+    const int li = il * 64 + jl * 8 + kl;
+    float s = il < 7 ? v[li + 64] : hv[0 * 64 + jl * 8 + kl];
+    s += il > 0 ? v[li - 64] : hv[1 * 64 + jl * 8 + kl];
+    s += jl < 7 ? v[li + 8] : hv[2 * 64 + il * 8 + kl];
+    s += jl > 0 ? v[li - 8] : hv[3 * 64 + il * 8 + kl];
+    s += kl < 7 ? v[li + 1] : hv[4 * 64 + il * 8 + jl];
+    s += kl > 0 ? v[li - 1] : hv[5 * 64 + il * 8 + jl];
+    return s;
+}
+
+// Level 0, pre-smoothing and restriction, one tile per block: u = w_c r / d on
+// FLUID cells (0 elsewhere), res = (1 - w_c) r + sum of the neighbours' u, and
+// each coarse cell of the tile's 4^3 block of level 1 takes the mean of its 8
+// children's res (its children in 8 consecutive lanes: thread t is child t % 8
+// of coarse cell t / 8). Coarse cells off the active tiles have no FLUID child
+// and are not written (every reader of r checks the class).
 struct KMgSmoothRestrict0 {
-    static constexpr int kThreads = kBlock;
+    static constexpr int kThreads = kMgTileThreads;
     struct Args {
         int N;
         const uint8_t* A;
@@ -2103,40 +2134,39 @@ struct KMgSmoothRestrict0 {
     static __device__ void run(const Args& a) {
         if (a.sc->done) return;
         const int64_t ntl = tile_count(a.ntiles, a.sc, 2);
-        const int64_t nb = list_grid(512 * ntl);
-        if ((int64_t)blockIdx.x >= nb) return;
+        if ((int64_t)blockIdx.x >= ntl) return;
+        __shared__ float uv[512], hu[384];
         const int N = a.N, TN = N / kTile, Mc = a.Mc;
-        // every lane runs the same number of rounds (the shuffles need the full
-        // wave); work items past the end contribute nothing
-        const int64_t stride = nb * blockDim.x;
-        const int64_t rounds = (512 * ntl + stride - 1) / stride;
-        for (int64_t rd = 0; rd < rounds; ++rd) {
-            const int64_t e = rd * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-            const bool live = e < 512 * ntl;
-            const int t = a.tiles[live ? (e >> 9) : 0];
-            const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
-            const int lc = (int)((e >> 3) & 63), u = (int)(e & 7);
-            const int I = ti * 4 + (lc >> 4), J = tj * 4 + ((lc >> 2) & 3), K = tk * 4 + (lc & 3);
-            const int i = 2 * I + (u >> 2), j = 2 * J + ((u >> 1) & 1), k = 2 * K + (u & 1);
-            float res = 0.0f;
-            const int64_t c = cidx(N, i, j, k);
-            if (live && a.A[c] == kFluid) {
-                float sum = 0.0f;
-                const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
-#pragma unroll
-                for (int m = 0; m < 6; ++m) {
-                    const int x = q[m][0], y = q[m][1], z = q[m][2];
-                    if (x < 0 || x >= N || y < 0 || y >= N || z < 0 || z >= N) continue;
-                    const int64_t cq = cidx(N, x, y, z);
-                    if (a.A[cq] != kFluid) continue;
+        const int t = a.tiles[blockIdx.x];
+        const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
+        const int tid = threadIdx.x;
+        const int lc = tid >> 3, u = tid & 7;
+        const int ci = lc >> 4, cj = (lc >> 2) & 3, ck = lc & 3;
+        const int il = 2 * ci + (u >> 2), jl = 2 * cj + ((u >> 1) & 1), kl = 2 * ck + (u & 1);
+        const int i = ti * kTile + il, j = tj * kTile + jl, k = tk * kTile + kl;
+        const int64_t c = cidx(N, i, j, k);
+        const bool fl = a.A[c] == kFluid;
+        const float r = fl ? a.R[(int64_t)t * 512 + il * 64 + jl * 8 + kl] : 0.0f;
+        const float d = a.diag[c];
+        const float w = mg_omega0(d);
+        uv[il * 64 + jl * 8 + kl] = fl ? w * r / d : 0.0f;
+        if (tid < 384) {
+            int x, y, z;
+            mg_halo_cell(tid, ti, tj, tk, x, y, z);
+            float hv = 0.0f;
+            if (x >= 0 && x < N && y >= 0 && y < N && z >= 0 && z < N) {
+                const int64_t cq = cidx(N, x, y, z);
+                if (a.A[cq] == kFluid) {
                     const float dq = a.diag[cq];
-                    sum += mg_omega0(dq) * a.R[tm_cell(N, x, y, z)] / dq;
+                    hv = mg_omega0(dq) * a.R[tm_cell(N, x, y, z)] / dq;
                 }
-                res = (1.0f - mg_omega0(a.diag[c])) * a.R[tm_cell(N, i, j, k)] + sum;
             }
-            const float acc = mg_group8_sum(res);
-            if (live && u == 0) a.rc[((int64_t)I * Mc + J) * Mc + K] = acc * 0.125f;
+            hu[tid] = hv;
         }
+        __syncthreads();
+        const float res = fl ? (1.0f - w) * r + mg_nbr_sum(uv, hu, il, jl, kl) : 0.0f;
+        const float acc = mg_group8_sum(res);
+        if (u == 0) a.rc[((int64_t)(ti * 4 + ci) * Mc + (tj * 4 + cj)) * Mc + (tk * 4 + ck)] = acc * 0.125f;
     }
 };
 
@@ -2286,13 +2316,13 @@ struct KMgCoarsest {
 };
 
 // Level 0, coarse correction and post-smoothing into the tile-major z of PCG (0
-// off FLUID): zt = w_c h2 r / d + P e (recomputed for the FLUID neighbours),
+// off FLUID), one tile per block: zt = w_c h2 r / d + P e on FLUID cells (0
+// elsewhere) for the tile and its faces into LDS, then
 // z = zt + w_c (h2 r + sum_{FLUID q} zt_q - d zt) / d, as (1 - w_c) zt +
-// w_c (h2 r + sum zt_q) / d; z.r into component 1 of the block partials
-// (two-value layout; component 0 is 0) for FIN_RHO. One cell per work item
-// (cell e % 512 of the list's tile e / 512).
+// w_c (h2 r + sum zt_q) / d; z.r into component 1 of the block's partial
+// (two-value layout; component 0 is 0) for FIN_RHO.
 struct KMgPost0 {
-    static constexpr int kThreads = kBlock;
+    static constexpr int kThreads = kMgTileThreads;
     struct Args {
         int N;
         float h2;
@@ -2309,40 +2339,44 @@ struct KMgPost0 {
     static __device__ void run(const Args& a) {
         if (a.red.sc->done) return;
         const int64_t ntl = tile_count(a.ntiles, a.red.sc, 2);
-        const int64_t nb = list_grid(512 * ntl);
-        if ((int64_t)blockIdx.x >= nb) return;
+        if ((int64_t)blockIdx.x >= ntl) return;
+        __shared__ float zv[512], hz[384];
         const int N = a.N, TN = N / kTile;
         const float h2 = a.h2;
-        double v[2] = {0.0, 0.0};
-        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 512 * ntl; e += nb * blockDim.x) {
-            const int t = a.tiles[e >> 9];
-            const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
-            const int li = (int)(e & 511);
-            const int i = ti * kTile + (li >> 6), j = tj * kTile + ((li >> 3) & 7), k = tk * kTile + (li & 7);
-            const int64_t o = (int64_t)t * 512 + li;
-            const int64_t c = cidx(N, i, j, k);
-            float out = 0.0f;
-            if (a.A[c] == kFluid) {
-                const float d = a.diag[c];
-                const float w = mg_omega0(d);
-                const float rc = a.R[o];
-                const float ztc = w * h2 * rc / d + mg_prolong(a.e, a.Mc, i, j, k);
-                float sum = 0.0f;
-                const int q[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
-#pragma unroll
-                for (int m = 0; m < 6; ++m) {
-                    const int x = q[m][0], y = q[m][1], z = q[m][2];
-                    if (x < 0 || x >= N || y < 0 || y >= N || z < 0 || z >= N) continue;
-                    const int64_t cq = cidx(N, x, y, z);
-                    if (a.A[cq] != kFluid) continue;
+        const int t = a.tiles[blockIdx.x];
+        const int tk = t % TN, tj = (t / TN) % TN, ti = t / (TN * TN);
+        const int li = threadIdx.x;
+        const int il = li >> 6, jl = (li >> 3) & 7, kl = li & 7;
+        const int i = ti * kTile + il, j = tj * kTile + jl, k = tk * kTile + kl;
+        const int64_t c = cidx(N, i, j, k);
+        const int64_t o = (int64_t)t * 512 + li;
+        const bool fl = a.A[c] == kFluid;
+        const float rc = fl ? a.R[o] : 0.0f;
+        const float d = a.diag[c];
+        const float w = mg_omega0(d);
+        const float ztc = fl ? w * h2 * rc / d + mg_prolong(a.e, a.Mc, i, j, k) : 0.0f;
+        zv[li] = ztc;
+        if (li < 384) {
+            int x, y, z;
+            mg_halo_cell(li, ti, tj, tk, x, y, z);
+            float hv = 0.0f;
+            if (x >= 0 && x < N && y >= 0 && y < N && z >= 0 && z < N) {
+                const int64_t cq = cidx(N, x, y, z);
+                if (a.A[cq] == kFluid) {
                     const float dq = a.diag[cq];
-                    sum += mg_omega0(dq) * h2 * a.R[tm_cell(N, x, y, z)] / dq + mg_prolong(a.e, a.Mc, x, y, z);
+                    hv = mg_omega0(dq) * h2 * a.R[tm_cell(N, x, y, z)] / dq + mg_prolong(a.e, a.Mc, x, y, z);
                 }
-                out = (1.0f - w) * ztc + w * ((h2 * rc + sum) / d);
-                v[1] += (double)out * (double)rc;
             }
-            a.Z[o] = out;
+            hz[li] = hv;
         }
+        __syncthreads();
+        double v[2] = {0.0, 0.0};
+        float out = 0.0f;
+        if (fl) {
+            out = (1.0f - w) * ztc + w * ((h2 * rc + mg_nbr_sum(zv, hz, il, jl, kl)) / d);
+            v[1] = (double)out * (double)rc;
+        }
+        a.Z[o] = out;
         block_partials<2>(v, a.red.partials, blockIdx.x);
     }
 };
@@ -2462,7 +2496,7 @@ void launch_spmv_dot(hipStream_t s, const Params& p, const float* G, const uint8
 
 int64_t partial_capacity(int N) {
     const int64_t tn = N / kTile;
-    const int64_t nb = mic_grid(tn * tn * tn) + 1;   // red and black blocks round up apart
+    const int64_t nb = tn * tn * tn + 1;   // one block per tile (multigrid level 0) at most
     return nb > kMaxPartials ? nb : kMaxPartials;
 }
 
@@ -2567,7 +2601,7 @@ void launch_mg_apply(hipStream_t s, int N, const uint8_t* A, const float* diag, 
     const int64_t ntl = tiles_or_cap(act, 2);
     const float h2 = (float)(1.0 / ((double)N * (double)N));
     if (mg.L > 0) {
-        launch<KMgSmoothRestrict0>(s, dim3((unsigned)list_grid(512 * ntl)), dim3(kBlock),
+        launch<KMgSmoothRestrict0>(s, dim3((unsigned)std::max<int64_t>(1, ntl)), dim3(kMgTileThreads),
                                    {N, A, diag, Rt, mg.r[1], mg.M[1], act.tiles[2], act.ntiles[2], sc});
         for (int l = 1; l < mg.L; ++l)
             launch<KMgSmoothRestrict>(s, dim3(grid_for(8 * (int64_t)mg.M[l + 1] * mg.M[l + 1] * mg.M[l + 1], kBlock)),
@@ -2577,11 +2611,11 @@ void launch_mg_apply(hipStream_t s, int N, const uint8_t* A, const float* diag, 
             launch<KMgProlongPost>(s, dim3(mg_cells_grid(mg.M[l])), dim3(kBlock),
                                    {mg_level(mg, N, l), mg.zo[l + 1], mg.M[l + 1], sc});
     }
-    const unsigned nb0 = (unsigned)list_grid(512 * ntl);
-    launch<KMgPost0>(s, dim3(nb0), dim3(kBlock),
+    const unsigned nb0 = (unsigned)std::max<int64_t>(1, ntl);
+    launch<KMgPost0>(s, dim3(nb0), dim3(kMgTileThreads),
                      {N, h2, A, diag, Rt, mg.L > 0 ? mg.zo[1] : nullptr, mg.L > 0 ? mg.M[1] : 0, Zt,
                       act.tiles[2], act.ntiles[2], red});
-    launch_finalize(s, red, FIN_RHO, act.ntiles[2] < 0 ? PARTS_TILE_CELLS : (int)nb0);
+    launch_finalize(s, red, FIN_RHO, act.ntiles[2] < 0 ? PARTS_TILES : (int)nb0);
 }
 
 }  // namespace ariel
