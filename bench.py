@@ -24,6 +24,8 @@ oracle's restatement of the GPU algorithm with its PCG capped and scaled.
 The 1e-6 of the metric is the PCG stop rule on the recursive residual; the true
 residuals of the FP64 solution and of the FP32 pressure are reported beside it.
 `impact_phase` times steps 56-100, where the cube meets the pool.
+`multigrid` repeats the workload with the multigrid preconditioner (not the
+workload's MIC(0); an extension) on a fresh handle after the main run.
 """
 import argparse
 import json
@@ -54,6 +56,8 @@ def parse():
     ap.add_argument("--precond", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--no-multigrid", action="store_true",
+                    help="skip the multigrid-preconditioner leg of the default run")
     ap.add_argument("--no-impact", dest="impact", action="store_false",
                     help="skip the impact-phase timing (steps 56-100)")
     ap.add_argument("--batched", action="store_true",
@@ -507,7 +511,9 @@ def main():
         }
     dom_name = "precond" if "precond" in kernels else max(kernels, key=lambda n: kernels[n]["total_ms"])
     dom = kernels[dom_name]
-    traffic, traffic_src = profile_traffic(dom_name)
+    # the counter summary covers the default workload's kernels (tiled MIC(0))
+    traffic, traffic_src = (profile_traffic(dom_name) if desc.precond == _abi.PRECOND_MIC0_TILED
+                            else (None, None))
     roofline = {
         "bound": "hbm", "kernel": dom_name, "achieved": dom["achieved_GBs"],
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["frac"], "traffic": traffic,
@@ -577,7 +583,55 @@ def main():
         out["cpu_port_extrapolated"] = cpu_baseline(sim, desc, float(np.mean(iters)), args.cpu_iters)
         log("cpu baseline done")
     sim.close()
+    if args.precond is None and args.impact and args.n == 256 and not args.no_multigrid:
+        out["multigrid"] = multigrid_leg(args, ms_per_step, impact)
     print(json.dumps(out), flush=True)
+
+
+def multigrid_leg(args, mic_ms, mic_impact):
+    """The same workload with the multigrid preconditioner (ARIEL_PRECOND_MG, not
+    the workload's MIC(0)): the same warm-up and timed steps, then steps 56-100,
+    on a fresh handle after the main run; the main line keeps MIC(0)."""
+    from ariel_amd import FlipSim, _abi, scenes
+    desc = scenes.pool_and_cube(args.n, seed=1)
+    desc.precond = _abi.PRECOND_MG
+    sim = FlipSim(desc)
+    sim.Init()
+    for _ in range(args.warmup):
+        sim.Step(1)
+    sim.sync()
+    iters, solve_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.Step(1)
+        st = sim.stats()
+        iters.append(st.iterations)
+        solve_ms.append(st.solve_ms)
+    sim.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    done = args.warmup + args.steps
+    while done < 55:
+        sim.Step(1)
+        done += 1
+    sim.sync()
+    t1 = time.perf_counter()
+    it_imp = []
+    for _ in range(100 - done):
+        sim.Step(1)
+        it_imp.append(sim.stats().iterations)
+    sim.sync()
+    imp = None
+    if it_imp:
+        imp = {"steps": f"{done + 1}-100", "ms_per_step": (time.perf_counter() - t1) * 1e3 / len(it_imp),
+               "pcg_iterations": float(np.mean(it_imp))}
+    sim.close()
+    log(f"multigrid: {ms:.2f} ms/step, {np.mean(iters):.1f} PCG iterations")
+    return {"preconditioner": "multigrid V(1,1), Jacobi smoothing, piecewise-constant transfers",
+            "steps_per_s": 1e3 / ms, "ms_per_step": ms, "pcg_solve_ms": float(np.mean(solve_ms)),
+            "pcg_iterations": float(np.mean(iters)), "impact_phase": imp,
+            "step_speedup_vs_mic0": mic_ms / ms,
+            "impact_speedup_vs_mic0": (mic_impact["ms_per_step"] / imp["ms_per_step"]
+                                       if (mic_impact and imp) else None)}
 
 
 if __name__ == "__main__":
