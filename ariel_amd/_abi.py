@@ -55,6 +55,11 @@ class SceneDesc(ctypes.Structure):
                 ("spring", ctypes.c_float), ("reserved", ctypes.c_int32 * 8)]
 
 
+class SolverTrial(ctypes.Structure):
+    _fields_ = [("precond", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("solve_ms", ctypes.c_float)]
+
+
 class StepStats(ctypes.Structure):
     _fields_ = [("frame", ctypes.c_int32), ("iterations", ctypes.c_int32),
                 ("solve_status", ctypes.c_int32), ("fluid_cells", ctypes.c_int32),
@@ -84,6 +89,7 @@ PROTOTYPES = {
     "ariel_last_error": (ctypes.c_char_p, [_P]),
     "ariel_init": (_I32, [_P]),
     "ariel_step": (_I32, [_P, _I32]),
+    "ariel_step_autotune": (_I32, [_P, _P, _I32, _P, _P]),
     "ariel_run_stage": (_I32, [_P, _I32]),
     "ariel_sync": (_I32, [_P]),
     "ariel_get_frame": (_I32, [_P, ctypes.POINTER(_I32)]),

@@ -338,6 +338,24 @@ int stage_resample(ariel_sim* h) {
     return check_launch(h, "resample");
 }
 
+// Coarse levels of the multigrid preconditioner, allocated on first use.
+int ensure_mg(ariel_sim* h) {
+    if (h->mg.type[1] || h->N <= 8) {
+        if (h->mg.L == 0) mg_plan(h->N, h->mg);
+        return ARIEL_OK;
+    }
+    mg_plan(h->N, h->mg);
+    for (int l = 1; l <= h->mg.L; ++l) {
+        const int64_t n = (int64_t)h->mg.M[l] * h->mg.M[l] * h->mg.M[l];
+        TRY(dalloc(h, &h->mg.type[l], n));
+        TRY(dalloc(h, &h->mg.d[l], n));
+        TRY(dalloc(h, &h->mg.r[l], n));
+        TRY(dalloc(h, &h->mg.z[l], n));
+        TRY(dalloc(h, &h->mg.zo[l], n));
+    }
+    return ARIEL_OK;
+}
+
 int run_stage(ariel_sim* h, int stage) {
     Stage& st = h->stages[stage];
     HIPCHK(h, hipEventRecord(st.t0, h->stream));
@@ -504,16 +522,7 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->partials, 3 * h->part_cap)) || (rc = dalloc(h, &h->counter, kCounterWords)) ||
         (rc = dalloc(h, &h->sc, 1)))
         return bail(rc);
-    if (h->d.precond == ARIEL_PRECOND_MG) {
-        mg_plan(h->N, h->mg);
-        for (int l = 1; l <= h->mg.L; ++l) {
-            const int64_t n = (int64_t)h->mg.M[l] * h->mg.M[l] * h->mg.M[l];
-            if ((rc = dalloc(h, &h->mg.type[l], n)) || (rc = dalloc(h, &h->mg.d[l], n)) ||
-                (rc = dalloc(h, &h->mg.r[l], n)) || (rc = dalloc(h, &h->mg.z[l], n)) ||
-                (rc = dalloc(h, &h->mg.zo[l], n)))
-                return bail(rc);
-        }
-    }
+    if (h->d.precond == ARIEL_PRECOND_MG && (rc = ensure_mg(h)) != ARIEL_OK) return bail(rc);
     if (hipHostMalloc((void**)&h->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, ARIEL_ERR_NOMEM, "hipHostMalloc failed"));
     for (int i = 0; i < ARIEL_NUM_STAGES; ++i) {
@@ -705,6 +714,62 @@ int ariel_step(ariel_sim* h, int32_t nsteps) {
         HIPCHK(h, hipEventRecord(h->step_t1, h->stream));
     }
     if (nsteps > 0) TRY(collect_stats(h));
+    h->stats.frame = h->frame;
+    return ARIEL_OK;
+}
+
+int ariel_step_autotune(ariel_sim* h, const int32_t* candidates, int32_t n, ariel_solver_trial* trials,
+                        int32_t* chosen) {
+    TRY(check_handle(h));
+    if (!h->initialized) return fail(h, ARIEL_ERR_STATE, "ariel_step_autotune before ariel_init");
+    if (!(h->max_density > 0.0f)) return fail(h, ARIEL_ERR_STATE, "max density not calibrated");
+    if (!candidates || n < 1 || n > 8) return fail(h, ARIEL_ERR_INVALID, "autotune: 1 to 8 candidates");
+    for (int i = 0; i < n; ++i) {
+        const int c = candidates[i];
+        if (c != ARIEL_PRECOND_MIC0_TILED && c != ARIEL_PRECOND_MIC0_BLOCK && c != ARIEL_PRECOND_JACOBI &&
+            c != ARIEL_PRECOND_MG)
+            return fail(h, ARIEL_ERR_INVALID, "autotune: candidates are MIC0_TILED, MIC0_BLOCK, JACOBI or MG");
+        if (c != ARIEL_PRECOND_JACOBI && h->d.tile != 8)
+            return fail(h, ARIEL_ERR_UNSUPPORTED, "autotune: tiled preconditioners need tile 8");
+        if (c == ARIEL_PRECOND_MG) TRY(ensure_mg(h));
+    }
+    h->frame++;
+    if (h->geom) TRY(emit_particles(h));
+    HIPCHK(h, hipEventRecord(h->step_t0, h->stream));
+    for (int st = 0; st < ARIEL_STAGE_SOLVE; ++st) TRY(run_stage(h, st));
+    // every trial starts from the step's warm start, kept in the staging buffer S
+    // (no stage of a step uses it)
+    const size_t bytes = (size_t)h->nc * sizeof(float);
+    HIPCHK(h, hipMemcpyAsync(h->S, h->P, bytes, hipMemcpyDeviceToDevice, h->stream));
+    const int saved_iters = h->last_iters;
+    int best = -1;
+    float best_ms = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        h->d.precond = candidates[i];
+        ariel_solver_trial tr{candidates[i], 0, 0, 0.0f};
+        for (int rep = 0; rep < 2; ++rep) {
+            HIPCHK(h, hipMemcpyAsync(h->P, h->S, bytes, hipMemcpyDeviceToDevice, h->stream));
+            h->last_iters = saved_iters;
+            TRY(stage_solve(h));
+            TRY(collect_stats(h));
+            tr.iterations = h->stats.iterations;
+            tr.status = h->stats.solve_status;
+            tr.solve_ms = (float)h->stats.solve_ms;
+        }
+        if (trials) trials[i] = tr;
+        if (tr.status == ARIEL_SOLVE_CONVERGED && (best < 0 || tr.solve_ms < best_ms)) {
+            best = i;
+            best_ms = tr.solve_ms;
+        }
+    }
+    if (best < 0) best = 0;
+    if (chosen) *chosen = best;
+    h->d.precond = candidates[best];
+    HIPCHK(h, hipMemcpyAsync(h->P, h->S, bytes, hipMemcpyDeviceToDevice, h->stream));
+    h->last_iters = saved_iters;
+    for (int st = ARIEL_STAGE_SOLVE; st < ARIEL_NUM_STAGES; ++st) TRY(run_stage(h, st));
+    HIPCHK(h, hipEventRecord(h->step_t1, h->stream));
+    TRY(collect_stats(h));
     h->stats.frame = h->frame;
     return ARIEL_OK;
 }

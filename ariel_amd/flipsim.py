@@ -97,6 +97,21 @@ class FlipSim:
     def Step(self, nsteps=1):
         self._check(self._lib.ariel_step(self._h, nsteps))
 
+    def StepAutotune(self, candidates=None):
+        """One Step whose pressure solve is first timed with each candidate
+        preconditioner (ariel_step_autotune); the fastest converged one becomes
+        this handle's. Returns (trials, chosen): trials as dicts of precond,
+        iterations, status, solve_ms."""
+        if candidates is None:
+            candidates = [_abi.PRECOND_MIC0_TILED, _abi.PRECOND_MG, _abi.PRECOND_MIC0_BLOCK,
+                          _abi.PRECOND_JACOBI]
+        c = (ctypes.c_int32 * len(candidates))(*candidates)
+        tr = (_abi.SolverTrial * len(candidates))()
+        chosen = ctypes.c_int32()
+        self._check(self._lib.ariel_step_autotune(self._h, c, len(candidates), tr, ctypes.byref(chosen)))
+        return ([{"precond": t.precond, "iterations": t.iterations, "status": t.status,
+                  "solve_ms": t.solve_ms} for t in tr], chosen.value)
+
     def GetParticles(self, types=False):
         """(pos[n,3], vel[n,3], density[n]) in particle-vector (id) order; with
         types=True also Particle::m_type (macgrid.inl:36-42) as uint8[n]."""
@@ -112,9 +127,12 @@ class FlipSim:
         return (pos, vel, rho, typ) if types else (pos, vel, rho)
 
     def GetScene(self):
-        """FlipSim::GetScene (flip.hpp:31): the scene description this handle was
-        created from (the descriptor replaces sceneCore::Scene)."""
-        return self.desc
+        """FlipSim::GetScene (flip.hpp:31): the scene description the handle runs
+        with (ariel_get_desc; the descriptor replaces sceneCore::Scene): the one it
+        was created from, with the preconditioner an autotuned step picked."""
+        d = SceneDesc()
+        self._check(self._lib.ariel_get_desc(self._h, ctypes.byref(d)))
+        return d
 
     def GetDimensions(self):
         d = (ctypes.c_int32 * 3)()

@@ -209,6 +209,25 @@ int ariel_init(ariel_sim* h);
 /* nsteps x FlipSim::Step (src/sim/flip.cpp:92-128), without export. */
 int ariel_step(ariel_sim* h, int32_t nsteps);
 
+/* One solver trial of ariel_step_autotune. */
+typedef struct ariel_solver_trial {
+    int32_t precond;        /* ARIEL_PRECOND_* tried                       */
+    int32_t iterations;     /* PCG iterations of its solve                 */
+    int32_t status;         /* ARIEL_SOLVE_*                               */
+    float   solve_ms;       /* device time of its second solve             */
+} ariel_solver_trial;
+
+/* Solver autotuner (no reference counterpart): one FlipSim::Step whose pressure
+ * solve is first run with each of the n candidate preconditioners (ARIEL_PRECOND_
+ * MIC0_TILED, _MIC0_BLOCK, _JACOBI, _MG; 1 <= n <= 8), twice each from the same
+ * warm start on the same right-hand side, and timed; the fastest candidate that
+ * converged becomes the handle's preconditioner, and the step finishes with its
+ * solve, so the step equals an ariel_step with that preconditioner. trials (n
+ * entries, may be NULL) receives the measurements, *chosen (may be NULL) the
+ * index of the pick. */
+int ariel_step_autotune(ariel_sim* h, const int32_t* candidates, int32_t n, ariel_solver_trial* trials,
+                        int32_t* chosen);
+
 /* One stage of Step on the current device state (ARIEL_STAGE_*). Stages after
  * ARIEL_STAGE_SORT assume the particles are sorted. Used by parity tests. */
 int ariel_run_stage(ariel_sim* h, int32_t stage);
