@@ -19,6 +19,7 @@ PRECOND_MIC0_TILED, PRECOND_MIC0, PRECOND_JACOBI, PRECOND_REFERENCE, PRECOND_MIC
     PRECOND_LEARNED, PRECOND_MG = range(7)
 STOP_RELATIVE, STOP_REFERENCE_ABS = 0, 1
 SPLAT_SHARPEN, SPLAT_TRILINEAR = 0, 1
+P2G_ORDERED, P2G_TILED = 0, 1
 LEVELSET_SPHERES, LEVELSET_TRAILS = 0, 1
 FIELD_UX, FIELD_UY, FIELD_UZ, FIELD_DUX, FIELD_DUY, FIELD_DUZ, FIELD_P, FIELD_D, FIELD_L = range(9)
 (STAGE_SORT, STAGE_DENSITY, STAGE_MARK, STAGE_P2G, STAGE_DIVERGENCE, STAGE_SOLVE,
@@ -52,7 +53,8 @@ class SceneDesc(ctypes.Structure):
                 ("max_iters", ctypes.c_int32), ("refine_every", ctypes.c_int32),
                 ("stop_rule", ctypes.c_int32), ("splat", ctypes.c_int32),
                 ("resample", ctypes.c_int32), ("sharpen_re", ctypes.c_float),
-                ("spring", ctypes.c_float), ("reserved", ctypes.c_int32 * 8)]
+                ("spring", ctypes.c_float), ("p2g", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 7)]
 
 
 class SolverTrial(ctypes.Structure):

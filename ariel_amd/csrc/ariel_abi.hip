@@ -128,6 +128,12 @@ int validate(const ariel_scene_desc* d, std::string& msg) {
     if (d->refine_every < 0) { msg = "refine_every must be >= 0"; return ARIEL_ERR_INVALID; }
     if (d->stop_rule != ARIEL_STOP_RELATIVE && d->stop_rule != ARIEL_STOP_REFERENCE_ABS) { msg = "unknown stop rule"; return ARIEL_ERR_INVALID; }
     if (d->splat != ARIEL_SPLAT_SHARPEN && d->splat != ARIEL_SPLAT_TRILINEAR) { msg = "unknown splat kernel"; return ARIEL_ERR_INVALID; }
+    if (d->p2g != ARIEL_P2G_ORDERED && d->p2g != ARIEL_P2G_TILED) { msg = "unknown P2G mode"; return ARIEL_ERR_INVALID; }
+    if (d->p2g == ARIEL_P2G_TILED && d->splat == ARIEL_SPLAT_SHARPEN && !(d->sharpen_re <= 1.5f)) {
+        // the tiled scatter visits the window without its first cell along each
+        // tangential axis, which holds only zero weights while RE <= 1.5
+        msg = "tiled P2G needs a Sharpen radius <= 1.5 cells"; return ARIEL_ERR_UNSUPPORTED;
+    }
     return ARIEL_OK;
 }
 
@@ -299,7 +305,8 @@ int stage_mark(ariel_sim* h) {
 
 int stage_p2g(ariel_sim* h) {
     Prof prof(h, ARIEL_KERNEL_P2G);
-    launch_p2g(h->stream, h->params(), h->pos[h->cur], h->vel[h->cur], h->offsets, h->A, h->u, h->du);
+    launch_p2g(h->stream, h->params(), h->pos[h->cur], h->vel[h->cur], h->offsets, h->A, h->u, h->du,
+               h->p2g_part, h->p2g_tflag);
     return check_launch(h, "p2g");
 }
 
@@ -523,6 +530,11 @@ int ariel_create(const ariel_scene_desc* d, ariel_sim** out) {
         (rc = dalloc(h, &h->sc, 1)))
         return bail(rc);
     if (h->d.precond == ARIEL_PRECOND_MG && (rc = ensure_mg(h)) != ARIEL_OK) return bail(rc);
+    if (h->d.p2g == ARIEL_P2G_TILED) {
+        const int64_t ntiles = p2g_tile_count(h->N);
+        if ((rc = dalloc(h, &h->p2g_part, ntiles * kP2gTilePart)) || (rc = dalloc(h, &h->p2g_tflag, ntiles)))
+            return bail(rc);
+    }
     if (hipHostMalloc((void**)&h->sc_host, sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess)
         return bail(fail(nullptr, ARIEL_ERR_NOMEM, "hipHostMalloc failed"));
     for (int i = 0; i < ARIEL_NUM_STAGES; ++i) {
@@ -570,7 +582,7 @@ int ariel_destroy(ariel_sim* h) {
                     (void*)h->Rt, (void*)h->Zt, (void*)h->St, (void*)h->Yt, (void*)h->Pt, (void*)h->Gt, (void*)h->sb,
                     (void*)h->diag, (void*)h->nup, (void*)h->fl8, (void*)h->P64, (void*)h->partials, (void*)h->counter, (void*)h->sc,
                     (void*)h->seg_flag, (void*)h->seg_scan, (void*)h->seg_list, (void*)h->tile_any,
-                    (void*)h->tile_lists, (void*)h->net_mem, (void*)h->R_old, (void*)h->net.wd})
+                    (void*)h->tile_lists, (void*)h->p2g_part, (void*)h->p2g_tflag, (void*)h->net_mem, (void*)h->R_old, (void*)h->net.wd})
         hipFree(p);
     for (int c = 0; c < 3; ++c) {
         hipFree(h->u[c]);
@@ -613,6 +625,7 @@ void take_solver(ariel_scene_desc& d, const ariel_scene_desc* solver) {
     d.resample = solver->resample;
     d.sharpen_re = solver->sharpen_re;
     d.spring = solver->spring;
+    d.p2g = solver->p2g;
 }
 
 int ariel_create_from_json(const char* json, const char* base_dir, const ariel_scene_desc* solver,

@@ -79,6 +79,9 @@ struct ariel_sim {
     float* du[3] = {nullptr, nullptr, nullptr};
     float *P = nullptr, *D = nullptr, *L = nullptr;
     uint8_t* A = nullptr;
+    // ARIEL_P2G_TILED: per-tile face partials and the tiles' particle flags
+    float* p2g_part = nullptr;
+    uint8_t* p2g_tflag = nullptr;
     // pc: 2 N^3 floats, the factor in the natural layout and then the tile-major
     // copy the preconditioner applies read. R, Z, S: natural-layout staging of
     // the accessors and of the learned network's input and output. The solve's
@@ -137,6 +140,7 @@ struct ariel_sim {
         // the learned preconditioner's fallback and set-up are the tiled MIC(0)
         p.precond = d.precond == ARIEL_PRECOND_LEARNED ? ARIEL_PRECOND_MIC0_TILED : d.precond;
         p.p2g_tile = p2g_tile;
+        p.p2g = d.p2g;
         return p;
     }
     Reduce reduce() const { return Reduce{partials, counter, sc, partials + 2 * part_cap}; }

@@ -101,6 +101,7 @@ struct Params {
     float mic_tau, mic_sigma;
     int precond;
     int p2g_tile;      // edge of the face-column tiles that order P2G blocks
+    int p2g;           // ARIEL_P2G_* (ordered gather or tiled scatter)
 };
 
 // Reduction workspace: per-block double partials + an arrival counter.
@@ -238,8 +239,19 @@ void launch_density(hipStream_t s, const Params& p, const float4* pos, float4* v
                     const int32_t* offsets, int apply_forces);
 void launch_mark(hipStream_t s, const Params& p, const float4* vel, const int32_t* offsets,
                  uint8_t* A, float* L);
+// ARIEL_P2G_TILED keeps, per 8^3 cell tile, the (weight, weighted velocity) sums of
+// every face its cells reach: 3 components x 900 faces x 2 floats, and one flag
+// byte per tile (0: the tile holds no particle and wrote no sums).
+constexpr int kP2gTileEdge = 8;
+constexpr int64_t kP2gTileFaces = 900;
+constexpr int64_t kP2gTilePart = 3 * 2 * kP2gTileFaces;
+inline int64_t p2g_tile_count(int N) {
+    const int64_t t = (N + kP2gTileEdge - 1) / kP2gTileEdge;
+    return t * t * t;
+}
 void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
-                const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3]);
+                const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3],
+                float* part, uint8_t* tflag);
 void launch_g2p_advect(hipStream_t s, const Params& p, float4* pos, float4* vel, int64_t np,
                        float* const u[3], float* const du[3]);
 void launch_resample(hipStream_t s, const Params& p, const float4* pos, const float4* vel,

@@ -292,7 +292,8 @@ int autotune(ariel_batch* B) {
             const Params p = h->params();
             for (int rep = 0; rep < 4; ++rep) {
                 if (rep == 1) BHIP(B, hipEventRecord(e0, h->stream));
-                launch_p2g(h->stream, p, h->pos[h->cur], h->vel[h->cur], h->offsets, h->A, h->u, h->du);
+                launch_p2g(h->stream, p, h->pos[h->cur], h->vel[h->cur], h->offsets, h->A, h->u, h->du,
+                           h->p2g_part, h->p2g_tflag);
             }
             BHIP(B, hipEventRecord(e1, h->stream));
             BHIP(B, hipEventSynchronize(e1));
@@ -325,10 +326,10 @@ int ariel_batch_create(const ariel_scene_desc* descs, int32_t ntanks, ariel_batc
     for (int32_t t = 1; t < ntanks; ++t) {
         const ariel_scene_desc& a = descs[0];
         const ariel_scene_desc& b = descs[t];
-        if (a.precond != b.precond || a.splat != b.splat || a.refine_every != b.refine_every ||
+        if (a.precond != b.precond || a.splat != b.splat || a.p2g != b.p2g || a.refine_every != b.refine_every ||
             a.resample != b.resample || a.max_iters != b.max_iters)
             return bfail(nullptr, ARIEL_ERR_INVALID,
-                         "batch_create: tanks must share preconditioner, splat, refinement, "
+                         "batch_create: tanks must share preconditioner, splat, P2G mode, refinement, "
                          "resampling and iteration limit");
     }
     ariel_batch* B = new ariel_batch();

@@ -713,6 +713,41 @@ __device__ __forceinline__ void p2g_span(const float (*__restrict__ sp)[kP2GCap]
     }
 }
 
+// The weighted means of the three faces at (i, j, k) (0 where no weight reached a
+// face), stored after StorePreviousGrid's copy (flip.cpp:324-363) into du and,
+// with EnforceBoundaryVelocity (particlegridoperations.inl:45-104) applied, into u.
+__device__ __forceinline__ void p2g_store(int N, const uint8_t* __restrict__ A, int i, int j, int k,
+                                          bool vx, bool vy, bool vz, float sux, float swx, float suy,
+                                          float swy, float suz, float swz, float* __restrict__ ux,
+                                          float* __restrict__ uy, float* __restrict__ uz,
+                                          float* __restrict__ dux, float* __restrict__ duy,
+                                          float* __restrict__ duz) {
+    if (vx) {
+        const int64_t f = fidx(N, 0, i, j, k);
+        const float val = swx > 0 ? sux / swx : 0.0f;
+        dux[f] = val;
+        bool zero = (i == 0 || i == N);
+        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i - 1, j, k)] == kSolid);
+        ux[f] = zero ? 0.0f : val;
+    }
+    if (vy) {
+        const int64_t f = fidx(N, 1, i, j, k);
+        const float val = swy > 0 ? suy / swy : 0.0f;
+        duy[f] = val;
+        bool zero = (j == 0 || j == N);
+        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j - 1, k)] == kSolid);
+        uy[f] = zero ? 0.0f : val;
+    }
+    if (vz) {
+        const int64_t f = fidx(N, 2, i, j, k);
+        const float val = swz > 0 ? suz / swz : 0.0f;
+        duz[f] = val;
+        bool zero = (k == 0 || k == N);
+        if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j, k - 1)] == kSolid);
+        uz[f] = zero ? 0.0f : val;
+    }
+}
+
 template <bool TRI>
 struct KP2gCols {
     static constexpr int kThreads = 64;
@@ -844,31 +879,276 @@ struct KP2gCols {
                 }
             }
         }
-        // StorePreviousGrid copy, then EnforceBoundaryVelocity
-        if (vx) {
-            const int64_t f = fidx(N, 0, i, j, k);
-            const float val = swx > 0 ? sux / swx : 0.0f;
-            dux[f] = val;
-            bool zero = (i == 0 || i == N);
-            if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i - 1, j, k)] == kSolid);
-            ux[f] = zero ? 0.0f : val;
+        p2g_store(N, A, i, j, k, vx, vy, vz, sux, swx, suy, swy, suz, swz, ux, uy, uz, dux, duy, duz);
+    }
+};
+
+// ------------------------------------------------ P2G, tiled scatter (ARIEL_P2G_TILED)
+// The same sums as KP2gCols (particlegridoperations.inl:146-249 over the window of
+// GetWallNeighbors, particlegrid.cpp:52-73), gathered the other way round. A
+// particle in cell (cx, cy, cz) lies in the window of the x faces i in {cx, cx+1},
+// j in {cy-1, cy, cy+1}, k in {cz-1, cz, cz+1} (the window's first cell along a
+// tangential axis, 1.5 cells or more from the face, carries only zero weights
+// while RE <= 1.5), and likewise for y and z: 54 candidate faces.
+//
+// Pass 1 (KP2gTiles): a 512-lane block owns an 8^3 tile of cells and takes the
+// three components in turn. Four lanes share a cell: they walk its particles
+// (contiguous in sorted order) four apart, so a wave's loads cover a few
+// contiguous runs of particles, and each lane keeps the (weight, weighted
+// velocity) sums of the cell's 18 candidate faces of the component in
+// registers. The four lanes' sums are added within the quad (DPP), stored in
+// LDS per cell, and each face the tile's cells reach (900 per component) adds
+// the sums of the tile's cells in its window in a fixed order and goes to HBM as
+// the tile's partial. No atomics: the sums are the same on every run. Pass 2
+// (KP2gCombine): each face adds the partials of the <= 8 tiles that hold cells
+// of its window, in tile order, and stores the mean (p2g_store).
+//
+// Sharpen's quotient uses the hardware reciprocal: the sums differ from the
+// ordered gather's by rounding only (per-cell partial sums, a reciprocal within
+// 1 ulp), far inside the 1e-5 relative bound the face velocities are held to.
+struct KP2gTilesBase {
+    // local index of a face in its component's region of the tile: x faces
+    // (i', j', k') in 9 x 10 x 10 (i' = i - 8 tx, j' = j - 8 ty + 1, ...), y in
+    // 10 x 9 x 10, z in 10 x 10 x 9
+    static __device__ __forceinline__ int xi(int i, int j, int k) { return (i * 10 + j) * 10 + k; }
+    static __device__ __forceinline__ int yi(int i, int j, int k) { return (i * 9 + j) * 10 + k; }
+    static __device__ __forceinline__ int zi(int i, int j, int k) { return (i * 10 + j) * 9 + k; }
+};
+
+template <bool TRI>
+struct KP2gTiles : KP2gTilesBase {
+    static constexpr int kThreads = 512;
+    static constexpr int kMinWavesPerEU = 4;   // two blocks per CU
+    struct Args {
+        Params p;
+        const float4* pos;
+        const float4* vel;
+        const int32_t* offsets;
+        float* part;
+        uint8_t* tflag;
+    };
+    // Factors of one axis for the faces normal to it (E: distances to the integer
+    // faces c, c+1) and along it (H: to the half-integer centres c-0.5, c+0.5,
+    // c+1.5), from f = p - c; squares for Sharpen, hat values for trilinear.
+    static __device__ __forceinline__ float fac(float d) {
+        return TRI ? gmax_num(1.0f - fabsf(d), 0.0f) : d * d;
+    }
+    // Sum of a value over the four lanes of a quad (DPP quad permutations).
+    static __device__ __forceinline__ float quad_sum(float v) {
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+        return v;
+    }
+    // One component's 18 candidate sums over the particles q0 + slot, q0 + slot +
+    // 4, ... < q1 of the cell (cx, cy, cz). NA, NB, NC: extents of the candidate
+    // box (2 along the component's own axis, else 3).
+    template <int C>
+    static __device__ __forceinline__ void sums(const Args& a, int32_t q0, int32_t q1, float maxd,
+                                                float fcx, float fcy, float fcz, float hh,
+                                                float (&w)[18], float (&u)[18]) {
+#pragma unroll
+        for (int n = 0; n < 18; ++n) w[n] = u[n] = 0.0f;
+        float4 on = make_float4(0, 0, 0, 0);
+        float vn = 0.0f;
+        if (q0 < q1) {
+            on = a.pos[q0];
+            vn = (&a.vel[q0].x)[C];
         }
-        if (vy) {
-            const int64_t f = fidx(N, 1, i, j, k);
-            const float val = swy > 0 ? suy / swy : 0.0f;
-            duy[f] = val;
-            bool zero = (j == 0 || j == N);
-            if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j - 1, k)] == kSolid);
-            uy[f] = zero ? 0.0f : val;
+        for (int32_t q = q0; q < q1; q += 4) {
+            const float4 o = on;
+            const float v = vn;
+            if (q + 4 < q1) {   // next particle's loads in flight during this one's sums
+                on = a.pos[q + 4];
+                vn = (&a.vel[q + 4].x)[C];
+            }
+            const float fx = gmax(0.0f, gmin(maxd, maxd * o.x)) - fcx;
+            const float fy = gmax(0.0f, gmin(maxd, maxd * o.y)) - fcy;
+            const float fz = gmax(0.0f, gmin(maxd, maxd * o.z)) - fcz;
+            float X[3], Y[3], Z[3];
+            X[0] = fac(C == 0 ? fx : fx + 0.5f);
+            X[1] = fac(C == 0 ? fx - 1.0f : fx - 0.5f);
+            X[2] = fac(fx - 1.5f);
+            Y[0] = fac(C == 1 ? fy : fy + 0.5f);
+            Y[1] = fac(C == 1 ? fy - 1.0f : fy - 0.5f);
+            Y[2] = fac(fy - 1.5f);
+            Z[0] = fac(C == 2 ? fz : fz + 0.5f);
+            Z[1] = fac(C == 2 ? fz - 1.0f : fz - 0.5f);
+            Z[2] = fac(fz - 1.5f);
+            constexpr int NA = C == 0 ? 2 : 3, NB = C == 1 ? 2 : 3, NC = C == 2 ? 2 : 3;
+#pragma unroll
+            for (int aa = 0; aa < NA; ++aa)
+#pragma unroll
+                for (int bb = 0; bb < NB; ++bb) {
+                    const float xy = TRI ? X[aa] * Y[bb] : X[aa] + Y[bb];
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc) {
+                        float wt;
+                        if constexpr (TRI) {
+                            wt = xy * Z[cc];
+                        } else {
+                            const float r2 = gmax_num(xy + Z[cc], 1.0e-5f);
+                            wt = gmax_num(fmaf(hh, __builtin_amdgcn_rcpf(r2), -1.0f), 0.0f);
+                        }
+                        const int n = (aa * NB + bb) * NC + cc;
+                        w[n] += wt;
+                        u[n] = fmaf(wt, v, u[n]);
+                    }
+                }
         }
-        if (vz) {
-            const int64_t f = fidx(N, 2, i, j, k);
-            const float val = swz > 0 ? suz / swz : 0.0f;
-            duz[f] = val;
-            bool zero = (k == 0 || k == N);
-            if (!zero) zero = (A[cidx(N, i, j, k)] == kSolid) != (A[cidx(N, i, j, k - 1)] == kSolid);
-            uz[f] = zero ? 0.0f : val;
+    }
+    static constexpr int kRounds = 4;   // 512 cells, 128 per round (16 per wave)
+    static __device__ void run(const Args& a) {
+        const Params& p = a.p;
+        const int N = p.N;
+        const int nt = (N + kP2gTileEdge - 1) / kP2gTileEdge;
+        const int64_t nbl = (int64_t)nt * nt * nt;
+        const int64_t b = xcd_block(nbl);
+        if (b >= nbl) return;
+        const int tz = (int)(b % nt), ty = (int)((b / nt) % nt), tx = (int)(b / ((int64_t)nt * nt));
+        const int t = threadIdx.x;
+        const int wave = t >> 6, lane = t & 63, c16 = lane >> 2, slot = lane & 3;
+        // round r: the wave's cells are the 8 cells of columns 2 (wave + 8 r) and
+        // 2 (wave + 8 r) + 1 of the tile (column = 8 lx + ly)
+        int32_t qa[kRounds], qb[kRounds];
+        int cell[kRounds];
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const int col = 2 * (wave + 8 * r) + (c16 >> 3);
+            const int lx = col >> 3, ly = col & 7, lz = c16 & 7;
+            cell[r] = (lx * 8 + ly) * 8 + lz;
+            const int cx = tx * 8 + lx, cy = ty * 8 + ly, cz = tz * 8 + lz;
+            qa[r] = qb[r] = 0;
+            if (cx < N && cy < N && cz < N) {
+                const int64_t c = cidx(N, cx, cy, cz);
+                qa[r] = a.offsets[c];
+                qb[r] = a.offsets[c + 1];
+            }
+            any = any || qb[r] > qa[r];
         }
+        if (!__syncthreads_or(any)) {
+            if (t == 0) a.tflag[b] = 0;
+            return;
+        }
+        // the cells' candidate sums of one component: [candidate][cell], weights
+        // then weighted velocities
+        __shared__ float cs[2][18][512];
+        const float maxd = (float)N;
+        const float hh = p.sharpen_re * p.sharpen_re;
+        float* __restrict__ out = a.part + b * kP2gTilePart;
+#pragma unroll 1
+        for (int comp = 0; comp < 3; ++comp) {
+            if (comp > 0) __syncthreads();   // the previous component's reads are done
+#pragma unroll 1
+            for (int r = 0; r < kRounds; ++r) {
+                const int lc = cell[r];
+                const float fcx = (float)(tx * 8 + (lc >> 6)), fcy = (float)(ty * 8 + ((lc >> 3) & 7)),
+                            fcz = (float)(tz * 8 + (lc & 7));
+                float w[18], u[18];
+                const int32_t q0 = qa[r] + slot, q1 = qb[r];
+                if (comp == 0) sums<0>(a, q0, q1, maxd, fcx, fcy, fcz, hh, w, u);
+                else if (comp == 1) sums<1>(a, q0, q1, maxd, fcx, fcy, fcz, hh, w, u);
+                else sums<2>(a, q0, q1, maxd, fcx, fcy, fcz, hh, w, u);
+#pragma unroll
+                for (int n = 0; n < 18; ++n) {
+                    w[n] = quad_sum(w[n]);
+                    u[n] = quad_sum(u[n]);
+                }
+                if (slot == 0) {
+#pragma unroll
+                    for (int n = 0; n < 18; ++n) {
+                        cs[0][n][lc] = w[n];
+                        cs[1][n][lc] = u[n];
+                    }
+                }
+            }
+            __syncthreads();
+            // the candidate box's extents along x, y, z and the region's
+            const int NA = comp == 0 ? 2 : 3, NB = comp == 1 ? 2 : 3, NC = comp == 2 ? 2 : 3;
+            const int RB = 7 + NB, RC = 7 + NC;
+            for (int f = t; f < kP2gTileFaces; f += kThreads) {
+                const int fk = f % RC, fj = (f / RC) % RB, fi = f / (RC * RB);
+                float sw = 0.0f, su = 0.0f;
+                for (int aa = 0; aa < NA; ++aa) {
+                    const int x = fi - aa;
+                    if (x < 0 || x > 7) continue;
+                    for (int bb = 0; bb < NB; ++bb) {
+                        const int y = fj - bb;
+                        if (y < 0 || y > 7) continue;
+                        for (int cc = 0; cc < NC; ++cc) {
+                            const int z = fk - cc;
+                            if (z < 0 || z > 7) continue;
+                            const int n = (aa * NB + bb) * NC + cc, c = (x * 8 + y) * 8 + z;
+                            sw += cs[0][n][c];
+                            su += cs[1][n][c];
+                        }
+                    }
+                }
+                out[comp * 2 * kP2gTileFaces + f] = sw;
+                out[comp * 2 * kP2gTileFaces + kP2gTileFaces + f] = su;
+            }
+        }
+        if (t == 0) a.tflag[b] = 1;
+    }
+};
+
+// Pass 2: the faces at (i, j, k) for 0 <= i, j, k <= N (as KP2gCols), each the sum
+// of its window's tiles' partials taken in tile order.
+struct KP2gCombine : KP2gTilesBase {
+    static constexpr int kThreads = 256;
+    struct Args {
+        Params p;
+        const float* part;
+        const uint8_t* tflag;
+        const uint8_t* A;
+        float* ux;
+        float* uy;
+        float* uz;
+        float* dux;
+        float* duy;
+        float* duz;
+    };
+    // sum over the tiles holding window cells [lo, hi] per axis (cells clamped to
+    // the grid); (oi, oj, ok) = 1 on tangential axes (the region starts one cell
+    // before the tile there)
+    static __device__ __forceinline__ void gather(const Args& a, int nt, int comp, int i, int j, int k,
+                                                  int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
+                                                  int oi, int oj, int ok, float& sw, float& su) {
+        sw = 0.0f;
+        su = 0.0f;
+        for (int tx = xlo >> 3; tx <= xhi >> 3; ++tx)
+            for (int ty = ylo >> 3; ty <= yhi >> 3; ++ty)
+                for (int tz = zlo >> 3; tz <= zhi >> 3; ++tz) {
+                    const int64_t tb = ((int64_t)tx * nt + ty) * nt + tz;
+                    if (!a.tflag[tb]) continue;
+                    const int li = i - 8 * tx + oi, lj = j - 8 * ty + oj, lk = k - 8 * tz + ok;
+                    const int f = comp == 0 ? xi(li, lj, lk) : comp == 1 ? yi(li, lj, lk) : zi(li, lj, lk);
+                    const float* t = a.part + tb * kP2gTilePart + comp * 2 * kP2gTileFaces;
+                    sw += t[f];
+                    su += t[kP2gTileFaces + f];
+                }
+    }
+    static __device__ void run(const Args& a) {
+        const int N = a.p.N;
+        const int nt = (N + kP2gTileEdge - 1) / kP2gTileEdge;
+        const int64_t n1 = N + 1;
+        const int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+        if (idx >= n1 * n1 * n1) return;
+        const int k = (int)(idx % n1), j = (int)((idx / n1) % n1), i = (int)(idx / (n1 * n1));
+        const bool vx = j < N && k < N, vy = i < N && k < N, vz = i < N && j < N;
+        float swx = 0, sux = 0, swy = 0, suy = 0, swz = 0, suz = 0;
+        const int m = N - 1;
+        if (vx)
+            gather(a, nt, 0, i, j, k, max(i - 1, 0), min(i, m), max(j - 1, 0), min(j + 1, m),
+                   max(k - 1, 0), min(k + 1, m), 0, 1, 1, swx, sux);
+        if (vy)
+            gather(a, nt, 1, i, j, k, max(i - 1, 0), min(i + 1, m), max(j - 1, 0), min(j, m),
+                   max(k - 1, 0), min(k + 1, m), 1, 0, 1, swy, suy);
+        if (vz)
+            gather(a, nt, 2, i, j, k, max(i - 1, 0), min(i + 1, m), max(j - 1, 0), min(j + 1, m),
+                   max(k - 1, 0), min(k, m), 1, 1, 0, swz, suz);
+        p2g_store(N, a.A, i, j, k, vx, vy, vz, sux, swx, suy, swy, suz, swz, a.ux, a.uy, a.uz,
+                  a.dux, a.duy, a.duz);
     }
 };
 
@@ -1343,7 +1623,20 @@ void launch_mark(hipStream_t s, const Params& p, const float4* vel, const int32_
 }
 
 void launch_p2g(hipStream_t s, const Params& p, const float4* pos, const float4* vel,
-                const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3]) {
+                const int32_t* offsets, const uint8_t* A, float* const u[3], float* const du[3],
+                float* part, uint8_t* tflag) {
+    if (p.p2g == ARIEL_P2G_TILED) {
+        const int64_t nb = p2g_tile_count(p.N);
+        const dim3 g((unsigned)xcd_grid(nb));
+        if (p.splat == 1)
+            launch<KP2gTiles<true>>(s, g, dim3(KP2gTiles<true>::kThreads), {p, pos, vel, offsets, part, tflag});
+        else
+            launch<KP2gTiles<false>>(s, g, dim3(KP2gTiles<false>::kThreads), {p, pos, vel, offsets, part, tflag});
+        const int64_t n1 = p.N + 1;
+        launch<KP2gCombine>(s, dim3(grid_for(n1 * n1 * n1, KP2gCombine::kThreads)), dim3(KP2gCombine::kThreads),
+                            {p, part, tflag, A, u[0], u[1], u[2], du[0], du[1], du[2]});
+        return;
+    }
     const int64_t nkc = (p.N + 1 + 63) / 64;
     const int64_t nti = (p.N + 1 + p.p2g_tile - 1) / p.p2g_tile;
     const int64_t nb = nti * nti * p.p2g_tile * p.p2g_tile * nkc;

@@ -24,8 +24,10 @@ oracle's restatement of the GPU algorithm with its PCG capped and scaled.
 The 1e-6 of the metric is the PCG stop rule on the recursive residual; the true
 residuals of the FP64 solution and of the FP32 pressure are reported beside it.
 `impact_phase` times steps 56-100, where the cube meets the pool.
-`multigrid` repeats the workload with the multigrid preconditioner (not the
-workload's MIC(0); an extension) on a fresh handle after the main run.
+`multigrid`, `tiled_p2g` and `multigrid_tiled_p2g` repeat the workload on fresh
+handles after the main run with the multigrid preconditioner, the tiled P2G
+scatter (face velocities within 1e-5 of the reference's sums) or both; they are
+extensions beside the workload's MIC(0) with the reference's P2G sums.
 """
 import argparse
 import json
@@ -54,10 +56,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precond", type=int, default=None)
+    ap.add_argument("--p2g", type=int, default=None, help="P2G mode (0 ordered, 1 tiled)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-multigrid", action="store_true",
-                    help="skip the multigrid-preconditioner leg of the default run")
+                    help="skip the variant legs (multigrid, tiled P2G) of the default run")
     ap.add_argument("--no-impact", dest="impact", action="store_false",
                     help="skip the impact-phase timing (steps 56-100)")
     ap.add_argument("--batched", action="store_true",
@@ -469,6 +472,8 @@ def main():
     desc = scenes.pool_and_cube(args.n, seed=1)
     if args.precond is not None:
         desc.precond = args.precond
+    if args.p2g is not None:
+        desc.p2g = args.p2g
     sim = FlipSim(desc)
     sim.Init()
     np_ = sim.num_particles
@@ -560,6 +565,7 @@ def main():
             "grid": args.n, "particles": int(np_), "dt": float(desc.dt),
             "preconditioner": ["mic0_tiled_rb", "mic0", "jacobi", "reference", "mic0_block",
                                "learned", "multigrid"][desc.precond],
+            "p2g": ["ordered", "tiled"][desc.p2g],
             "tol": float(desc.tol), "refine_every": int(desc.refine_every),
             "parallelism": "single GPU",
         },
@@ -583,32 +589,54 @@ def main():
         out["cpu_port_extrapolated"] = cpu_baseline(sim, desc, float(np.mean(iters)), args.cpu_iters)
         log("cpu baseline done")
     sim.close()
-    if args.precond is None and args.impact and args.n == 256 and not args.no_multigrid:
-        out["multigrid"] = multigrid_leg(args, ms_per_step, impact)
+    if args.precond is None and args.p2g is None and args.impact and args.n == 256 and not args.no_multigrid:
+        for name in LEGS:
+            out[name] = variant_leg(args, name, ms_per_step, impact)
     print(json.dumps(out), flush=True)
 
 
-def multigrid_leg(args, mic_ms, mic_impact):
-    """The same workload with the multigrid preconditioner (ARIEL_PRECOND_MG, not
-    the workload's MIC(0)): the same warm-up and timed steps, then steps 56-100,
-    on a fresh handle after the main run; the main line keeps MIC(0)."""
+LEGS = {
+    # name: (preconditioner, P2G mode, description)
+    "multigrid": ("PRECOND_MG", "P2G_ORDERED",
+                  "multigrid V(1,1), Jacobi smoothing, piecewise-constant transfers; ordered P2G"),
+    "tiled_p2g": ("PRECOND_MIC0_TILED", "P2G_TILED",
+                  "tiled red-black MIC(0); P2G scattered from 8^3 cell tiles (within 1e-5 of "
+                  "the reference's sums, not bit-exact)"),
+    "multigrid_tiled_p2g": ("PRECOND_MG", "P2G_TILED",
+                            "multigrid preconditioner and tiled P2G together"),
+}
+
+
+def variant_leg(args, name, mic_ms, mic_impact):
+    """The same workload with another preconditioner and/or P2G mode (LEGS; not
+    the workload's MIC(0) with the reference's P2G sums): the same warm-up and
+    timed steps, then steps 56-100, on a fresh handle after the main run; the
+    main line keeps MIC(0) and the ordered P2G."""
     from ariel_amd import FlipSim, _abi, scenes
+    pc, p2g, what = LEGS[name]
     desc = scenes.pool_and_cube(args.n, seed=1)
-    desc.precond = _abi.PRECOND_MG
+    desc.precond = getattr(_abi, pc)
+    desc.p2g = getattr(_abi, p2g)
     sim = FlipSim(desc)
     sim.Init()
     for _ in range(args.warmup):
         sim.Step(1)
     sim.sync()
-    iters, solve_ms = [], []
+    sim.set_profiling(True)
+    sim.reset_kernel_stats()
+    iters, solve_ms, status = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sim.Step(1)
         st = sim.stats()
         iters.append(st.iterations)
         solve_ms.append(st.solve_ms)
+        status.append(st.solve_status)
     sim.sync()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ks = sim.kernel_stats(_abi.KERNEL_NAMES.index("p2g"))
+    p2g_ms = ks.total_ms / ks.launches if ks.launches else None
+    sim.set_profiling(False)
     done = args.warmup + args.steps
     while done < 55:
         sim.Step(1)
@@ -625,11 +653,11 @@ def multigrid_leg(args, mic_ms, mic_impact):
         imp = {"steps": f"{done + 1}-100", "ms_per_step": (time.perf_counter() - t1) * 1e3 / len(it_imp),
                "pcg_iterations": float(np.mean(it_imp))}
     sim.close()
-    log(f"multigrid: {ms:.2f} ms/step, {np.mean(iters):.1f} PCG iterations")
-    return {"preconditioner": "multigrid V(1,1), Jacobi smoothing, piecewise-constant transfers",
-            "steps_per_s": 1e3 / ms, "ms_per_step": ms, "pcg_solve_ms": float(np.mean(solve_ms)),
-            "pcg_iterations": float(np.mean(iters)), "impact_phase": imp,
-            "step_speedup_vs_mic0": mic_ms / ms,
+    log(f"{name}: {ms:.2f} ms/step, P2G {p2g_ms} ms, {np.mean(iters):.1f} PCG iterations")
+    return {"variant": what, "steps_per_s": 1e3 / ms, "ms_per_step": ms,
+            "pcg_solve_ms": float(np.mean(solve_ms)), "pcg_iterations": float(np.mean(iters)),
+            "pcg_all_converged": all(s == 0 for s in status), "p2g_ms": p2g_ms,
+            "impact_phase": imp, "step_speedup_vs_mic0": mic_ms / ms,
             "impact_speedup_vs_mic0": (mic_impact["ms_per_step"] / imp["ms_per_step"]
                                        if (mic_impact and imp) else None)}
 

@@ -68,6 +68,16 @@ extern "C" {
 #define ARIEL_SPLAT_SHARPEN   0      /* reference (particlegridoperations.inl:146-249) */
 #define ARIEL_SPLAT_TRILINEAR 1      /* hat weights over the same window       */
 
+/* How P2G sums each face (particlegridoperations.inl:146-249 defines the sums). */
+#define ARIEL_P2G_ORDERED 0   /* per-face gather in the reference's order: bit-exact      */
+#define ARIEL_P2G_TILED   1   /* scatter from 8^3 cell tiles: each lane sums its cell's
+                                 particles into its 54 candidate faces in registers, the
+                                 waves add those sums into an LDS face tile in a fixed
+                                 order, and a second pass combines the <= 8 tiles that
+                                 cover a face in a fixed order. Deterministic, within
+                                 1e-6 relative of the ordered sums (not bit-exact).
+                                 Sharpen radius <= 1.5 cells or the hat kernel.         */
+
 /* ---- grid fields for ariel_get_macgrid / ariel_set_macgrid -------------- */
 #define ARIEL_FIELD_UX   0   /* m_u_x                                   */
 #define ARIEL_FIELD_UY   1   /* m_u_y                                   */
@@ -150,7 +160,8 @@ typedef struct ariel_scene_desc {
     int32_t  resample;      /* 1: run ResampleParticles each step           */
     float    sharpen_re;    /* P2G Sharpen radius in cells (1.4)            */
     float    spring;        /* resampler spring force (50)                  */
-    int32_t  reserved[8];
+    int32_t  p2g;           /* ARIEL_P2G_* (0: ordered, the reference's sums) */
+    int32_t  reserved[7];
 } ariel_scene_desc;
 
 typedef struct ariel_step_stats {
