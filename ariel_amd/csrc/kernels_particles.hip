@@ -935,8 +935,8 @@ struct KP2gTiles : KP2gTilesBase {
     }
     // Sum of a value over the four lanes of a quad (DPP quad permutations).
     static __device__ __forceinline__ float quad_sum(float v) {
-        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
-        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+        v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
+        v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true));
         return v;
     }
     // One component's 18 candidate sums over the particles q0 + slot, q0 + slot +
@@ -948,19 +948,18 @@ struct KP2gTiles : KP2gTilesBase {
                                                 float (&w)[18], float (&u)[18]) {
 #pragma unroll
         for (int n = 0; n < 18; ++n) w[n] = u[n] = 0.0f;
-        float4 on = make_float4(0, 0, 0, 0);
-        float vn = 0.0f;
-        if (q0 < q1) {
-            on = a.pos[q0];
-            vn = (&a.vel[q0].x)[C];
-        }
+        if (q0 >= q1) return;
+        const float wcap = fmaf(hh, 1.0e5f, -1.0f);
+        float4 on = a.pos[q0];
+        float vn = (&a.vel[q0].x)[C];
         for (int32_t q = q0; q < q1; q += 4) {
             const float4 o = on;
             const float v = vn;
-            if (q + 4 < q1) {   // next particle's loads in flight during this one's sums
-                on = a.pos[q + 4];
-                vn = (&a.vel[q + 4].x)[C];
-            }
+            // the next particle's loads in flight during this one's sums (the last
+            // iteration reloads its own particle: no branch in the loop)
+            const int32_t qn = min(q + 4, q1 - 1);
+            on = a.pos[qn];
+            vn = (&a.vel[qn].x)[C];
             const float fx = gmax(0.0f, gmin(maxd, maxd * o.x)) - fcx;
             const float fy = gmax(0.0f, gmin(maxd, maxd * o.y)) - fcy;
             const float fz = gmax(0.0f, gmin(maxd, maxd * o.z)) - fcz;
@@ -986,8 +985,10 @@ struct KP2gTiles : KP2gTilesBase {
                         if constexpr (TRI) {
                             wt = xy * Z[cc];
                         } else {
-                            const float r2 = gmax_num(xy + Z[cc], 1.0e-5f);
-                            wt = gmax_num(fmaf(hh, __builtin_amdgcn_rcpf(r2), -1.0f), 0.0f);
+                            // max(hh / max(r2, 1e-5) - 1, 0) as one clamp to
+                            // [0, hh / 1e-5 - 1] (r2 = 0 gives +inf, clamped)
+                            wt = __builtin_amdgcn_fmed3f(fmaf(hh, __builtin_amdgcn_rcpf(xy + Z[cc]), -1.0f),
+                                                         0.0f, wcap);
                         }
                         const int n = (aa * NB + bb) * NC + cc;
                         w[n] += wt;
