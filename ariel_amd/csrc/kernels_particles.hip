@@ -943,15 +943,17 @@ struct KP2gTiles : KP2gTilesBase {
     // 4, ... < q1 of the cell (cx, cy, cz). NA, NB, NC: extents of the candidate
     // box (2 along the component's own axis, else 3).
     template <int C>
+    // (po, pv): the particle q0, loaded by the caller ahead of time.
     static __device__ __forceinline__ void sums(const Args& a, int32_t q0, int32_t q1, float maxd,
                                                 float fcx, float fcy, float fcz, float hh,
+                                                const float4& po, const float4& pv,
                                                 float (&w)[18], float (&u)[18]) {
 #pragma unroll
         for (int n = 0; n < 18; ++n) w[n] = u[n] = 0.0f;
         if (q0 >= q1) return;
         const float wcap = fmaf(hh, 1.0e5f, -1.0f);
-        float4 on = a.pos[q0];
-        float vn = (&a.vel[q0].x)[C];
+        float4 on = po;
+        float vn = (&pv.x)[C];
         for (int32_t q = q0; q < q1; q += 4) {
             const float4 o = on;
             const float v = vn;
@@ -1037,6 +1039,17 @@ struct KP2gTiles : KP2gTilesBase {
         const float maxd = (float)N;
         const float hh = p.sharpen_re * p.sharpen_re;
         float* __restrict__ out = a.part + b * kP2gTilePart;
+        // the first particle of the next round, loaded while the current round's
+        // sums are reduced and stored
+        float4 po = make_float4(0, 0, 0, 0), pv = make_float4(0, 0, 0, 0);
+        auto first = [&](int r) {
+            const int32_t q = qa[r] + slot;
+            if (q < qb[r]) {
+                po = a.pos[q];
+                pv = a.vel[q];
+            }
+        };
+        first(0);
 #pragma unroll 1
         for (int comp = 0; comp < 3; ++comp) {
             if (comp > 0) __syncthreads();   // the previous component's reads are done
@@ -1047,9 +1060,10 @@ struct KP2gTiles : KP2gTilesBase {
                             fcz = (float)(tz * 8 + (lc & 7));
                 float w[18], u[18];
                 const int32_t q0 = qa[r] + slot, q1 = qb[r];
-                if (comp == 0) sums<0>(a, q0, q1, maxd, fcx, fcy, fcz, hh, w, u);
-                else if (comp == 1) sums<1>(a, q0, q1, maxd, fcx, fcy, fcz, hh, w, u);
-                else sums<2>(a, q0, q1, maxd, fcx, fcy, fcz, hh, w, u);
+                if (comp == 0) sums<0>(a, q0, q1, maxd, fcx, fcy, fcz, hh, po, pv, w, u);
+                else if (comp == 1) sums<1>(a, q0, q1, maxd, fcx, fcy, fcz, hh, po, pv, w, u);
+                else sums<2>(a, q0, q1, maxd, fcx, fcy, fcz, hh, po, pv, w, u);
+                first(r + 1 < kRounds ? r + 1 : 0);   // next round (the next component's first)
 #pragma unroll
                 for (int n = 0; n < 18; ++n) {
                     w[n] = quad_sum(w[n]);
